@@ -1,0 +1,29 @@
+import os
+import subprocess
+import sys
+
+import pytest
+
+ROOT = os.path.dirname(os.path.dirname(os.path.abspath(__file__)))
+if ROOT not in sys.path:
+    sys.path.insert(0, ROOT)
+
+
+def pytest_configure(config):
+    config.addinivalue_line("markers", "gpu: needs an MI355X (gfx950) GPU and the built HIP library")
+
+
+@pytest.fixture(scope="session")
+def oracle():
+    """The CPU FP64 restatement of the reference (the checker), built on demand."""
+    from oracle import pyoracle
+    if not os.path.exists(pyoracle.LIB_PATH):
+        subprocess.check_call(["make", "-C", os.path.join(ROOT, "oracle")])
+    return pyoracle.get()
+
+
+@pytest.fixture(scope="session")
+def fixtures():
+    import json
+    with open(os.path.join(ROOT, "tests", "golden", "reference_fixtures.json")) as f:
+        return json.load(f)

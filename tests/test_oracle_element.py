@@ -1,0 +1,233 @@
+"""Pins the oracle's NS element kernels (specialised P1 tri/tet, generic quadrature, tau, combine).
+
+The reference's only comparison test of these kernels is commented out
+(plugins/UFEM/test/utest-proto-ns-assembly.cpp:162-202), so the pins are mathematical:
+  * closed forms of the P1 integrals (mass V/20(1+d), grad-grad V gradNi.gradNj) with gradients
+    computed here by numpy from the nodes, independent of the oracle's normals;
+  * METRIC tau on the unit reference tetrahedron, where gij is the identity;
+  * specialised vs generic agreement for uniform fields, and the exact documented differences
+    (stress-form diffusion, the vk/wk substitution of NavierStokesSpecializations.hpp:224-226,236-238);
+  * generic hexahedron terms against an independent 4x4x4 tensor Gauss integration on affine hexes;
+  * the combine step (p rows / theta, rhs formed before the division, node-major reorder).
+"""
+import numpy as np
+import pytest
+
+from oracle.pyoracle import HEXA3D, SUPG_METRIC, SUPG_TEZDUYAR, TETRA3D, TRIAG2D, ns_params
+
+
+def p1_grads(nodes):
+    d = nodes.shape[1]
+    J = (nodes[1:] - nodes[0]).T            # x = x0 + J xi
+    dN = np.hstack([-np.ones((d, 1)), np.eye(d)])   # d x k reference gradients
+    G = np.linalg.inv(J).T @ dN                     # physical gradients, d x k
+    vol = abs(np.linalg.det(J)) / (2 if d == 2 else 6)
+    return G, vol
+
+
+def rand_tet(rng, scale=1.0):
+    base = np.array([[0, 0, 0], [1, 0, 0], [0, 1, 0], [0, 0, 1.0]])
+    while True:
+        X = (base + rng.uniform(-0.2, 0.2, (4, 3))) * scale
+        if np.linalg.det((X[1:] - X[0]).T) > 0.05 * scale ** 3:
+            return X
+
+
+def var_major_block(M, k, R, C):
+    return M[R * k:(R + 1) * k, C * k:(C + 1) * k]
+
+
+def test_tau_metric_unit_tet(oracle):
+    X = np.array([[0, 0, 0], [1, 0, 0], [0, 1, 0], [0, 0, 1.0]])
+    u = np.tile([0.3, -0.2, 0.5], (4, 1))
+    nu, dt = 0.02, 0.01
+    prm = ns_params(dt=dt)
+    _, _, _, tau = oracle.ns_element(TETRA3D, prm, X, u, u, np.full(4, nu))
+    adv = (u[0] ** 2).sum()
+    tau_su = 1 / np.sqrt(4 / dt ** 2 + adv + 16 * 3 * nu ** 2)
+    tau_bulk = np.sqrt(4 / dt ** 2 + adv + 16 * 3 * nu ** 2) / 3
+    assert tau[1] == pytest.approx(tau_su, rel=1e-14)
+    assert tau[0] == pytest.approx(tau_su, rel=1e-14)
+    assert tau[2] == pytest.approx(tau_bulk, rel=1e-14)
+
+
+def test_tet_specialised_closed_forms(oracle):
+    rng = np.random.default_rng(11)
+    X = rand_tet(rng)
+    G, V = p1_grads(X)
+    nu, dt, rho = 0.05, 0.02, 1.3
+    prm = ns_params(rho=rho, dt=dt)
+    zero = np.zeros((4, 3))
+    A, T, a, tau = oracle.ns_element(TETRA3D, prm, X, zero, zero, np.full(4, nu))
+    k = 4
+    mass = V / 20 * (np.ones((4, 4)) + np.eye(4))
+    for R in range(3):
+        assert np.allclose(var_major_block(T, k, R, R), mass, rtol=1e-13, atol=0)
+    # diffusion in stress form + bulk viscosity (u = 0, so no convection terms)
+    tb = tau[2]
+    for R in range(3):
+        for C in range(3):
+            exp = nu * V * (np.outer(G[R], G[C]) / 3 + (R == C) * G.T @ G) + tb * V * np.outer(G[R], G[C])
+            assert np.allclose(var_major_block(A, k, R, C), exp, rtol=1e-12, atol=1e-14 * np.abs(exp).max())
+        # pressure gradient: A(u_R, p) = int N_i dN_j/dx_R / rho = V/4 G[R, j] / rho
+        exp = np.tile(V / 4 * G[R] / rho, (4, 1))
+        assert np.allclose(var_major_block(A, k, R, 3), exp, rtol=1e-12)
+        # continuity: A(p, u_C) = V/4 G[C, j]
+        assert np.allclose(var_major_block(A, k, 3, R), np.tile(V / 4 * G[R], (4, 1)), rtol=1e-12)
+    # PSPG Laplacian
+    assert np.allclose(var_major_block(A, k, 3, 3), tau[0] / rho * V * G.T @ G, rtol=1e-12)
+    assert np.all(a == 0)
+
+
+@pytest.mark.parametrize("nu", [0.0, 0.03])
+@pytest.mark.parametrize("vw_equal", [True, False])
+def test_tet_specialised_vs_generic(oracle, nu, vw_equal):
+    rng = np.random.default_rng(5)
+    X = rand_tet(rng, 0.3)
+    uu = np.array([0.7, -0.4, -0.4 if vw_equal else 0.9])
+    u = np.tile(uu, (4, 1))
+    nus = np.full(4, nu)
+    A_s, T_s, _, tau_s = oracle.ns_element(TETRA3D, ns_params(dt=0.01), X, u, u, nus)
+    A_g, T_g, _, tau_g = oracle.ns_element(TETRA3D, ns_params(dt=0.01, use_specializations=False), X, u, u, nus)
+    assert np.allclose(tau_s, tau_g, rtol=1e-14)
+    G, V = p1_grads(X)
+    k = 4
+    scale = np.abs(A_g).max()
+    D = A_s - A_g
+    expected = np.zeros_like(D)
+    n = 3 * V * G.T                       # area vectors n_i = 3 V grad N_i
+    un = n @ uu
+    for R in range(3):
+        for C in range(3):
+            expected[R * k:(R + 1) * k, C * k:(C + 1) * k] += nu * V / 3 * np.outer(G[R], G[C])
+    dv = uu[1] - uu[2]
+    for C in range(3):
+        # W rows use vk for the skew-symmetric convection (NavierStokesSpecializations.hpp:224-226)
+        expected[2 * k:3 * k, C * k:(C + 1) * k] += np.outer(0.5 * (1 / 12 + tau_s[1] * un / (9 * V)), dv * n[:, C])
+        # ZZ PSPG skew term uses vk (NavierStokesSpecializations.hpp:236-238)
+        expected[3 * k:4 * k, C * k:(C + 1) * k] += 0.5 * tau_s[0] / (9 * V) * np.outer(n[:, 2], dv * n[:, C])
+    assert np.abs(D - expected).max() <= 1e-12 * scale
+    assert np.abs(T_s - T_g).max() <= 1e-12 * np.abs(T_g).max()
+
+
+def test_tri_specialised_vs_generic(oracle):
+    X = np.array([[0.1, 0.05], [0.9, 0.2], [0.3, 0.8]])
+    uu = np.array([0.6, -0.3])
+    u = np.tile(uu, (3, 1))
+    nu = 0.02
+    A_s, T_s, _, _ = oracle.ns_element(TRIAG2D, ns_params(), X, u, u, np.full(3, nu))
+    A_g, T_g, _, _ = oracle.ns_element(TRIAG2D, ns_params(use_specializations=False), X, u, u, np.full(3, nu))
+    G, V = p1_grads(X)
+    k = 3
+    expected = np.zeros_like(A_s)
+    for R in range(2):
+        for C in range(2):
+            expected[R * k:(R + 1) * k, C * k:(C + 1) * k] = nu * V / 3 * np.outer(G[R], G[C])
+    assert np.abs(A_s - A_g - expected).max() <= 1e-12 * np.abs(A_g).max()
+    assert np.abs(T_s - T_g).max() <= 1e-12 * np.abs(T_g).max()
+    # tri mass matrix V/12 (1 + delta)
+    A0, T0, _, _ = oracle.ns_element(TRIAG2D, ns_params(), X, 0 * u, 0 * u, np.full(3, nu))
+    assert np.allclose(T0[:3, :3], V / 12 * (np.ones((3, 3)) + np.eye(3)), rtol=1e-13)
+
+
+# ---- generic hexahedron vs an independent 4x4x4 tensor Gauss integration on affine hexes ----
+_LOC = np.array([[-1, -1, -1], [1, -1, -1], [1, 1, -1], [-1, 1, -1], [-1, -1, 1], [1, -1, 1], [1, 1, 1], [-1, 1, 1.]])
+
+
+def q1(xi):
+    N = np.prod(1 + _LOC * xi, axis=1) / 8
+    dN = np.zeros((3, 8))
+    for d in range(3):
+        o = [e for e in range(3) if e != d]
+        dN[d] = _LOC[:, d] * (1 + _LOC[:, o[0]] * xi[o[0]]) * (1 + _LOC[:, o[1]] * xi[o[1]]) / 8
+    return N, dN
+
+
+def hex_reference_terms(X, ua, nu, tau_ps, tau_su, tau_bulk, rho):
+    x, w = np.polynomial.legendre.leggauss(4)
+    k, n = 8, 32
+    A = np.zeros((n, n)); T = np.zeros((n, n))
+    for i, a in enumerate(x):
+        for j, b in enumerate(x):
+            for l, c in enumerate(x):
+                xi = np.array([a, b, c])
+                N, dN = q1(xi)
+                J = dN @ X                       # J(ksi, x)
+                G = np.linalg.solve(J, dN)       # physical gradients, 3 x 8
+                wq = w[i] * w[j] * w[l] * np.linalg.det(J)
+                L = ua @ G
+                SU = N + tau_su * L
+                P = 3 * k
+                for R in range(3):
+                    A[P:, R * k:(R + 1) * k] += wq * (np.outer(N + 0.5 * tau_ps * L, G[R]) + tau_ps * np.outer(G[R], L))
+                    A[R * k:(R + 1) * k, R * k:(R + 1) * k] += wq * (nu * G.T @ G + np.outer(SU, L))
+                    A[R * k:(R + 1) * k, P:] += wq * np.outer(SU, G[R]) / rho
+                    for C in range(3):
+                        A[R * k:(R + 1) * k, C * k:(C + 1) * k] += wq * np.outer(tau_bulk * G[R] + 0.5 * ua[R] * SU, G[C])
+                    T[P:, R * k:(R + 1) * k] += wq * tau_ps * np.outer(G[R], N)
+                    T[R * k:(R + 1) * k, R * k:(R + 1) * k] += wq * np.outer(SU, N)
+                A[P:, P:] += wq * tau_ps * G.T @ G / rho
+    return A, T
+
+
+@pytest.mark.parametrize("case", [0, 1])
+def test_generic_hex_vs_tensor_gauss(oracle, case):
+    rng = np.random.default_rng(40 + case)
+    B = np.eye(3) * 0.2 + rng.uniform(-0.03, 0.03, (3, 3))   # affine map: Gauss 2x2x2 is exact
+    X = (_LOC + 1) / 2 @ B.T + rng.uniform(0, 1, 3)
+    ua = np.array([0.4, -0.7, 0.2]) * case
+    nu, rho = 0.01, 1.1
+    u = np.tile(ua, (8, 1))
+    A, T, a, tau = oracle.ns_element(HEXA3D, ns_params(rho=rho, dt=0.02), X, u, u, np.full(8, nu))
+    Ar, Tr = hex_reference_terms(X, ua, nu, tau[0], tau[1], tau[2], rho)
+    assert np.abs(A - Ar).max() <= 1e-12 * np.abs(Ar).max()
+    assert np.abs(T - Tr).max() <= 1e-12 * np.abs(Tr).max()
+    # the mass matrix of every velocity block sums to the volume (partition of unity)
+    if case == 0:
+        assert T[:8, :8].sum() == pytest.approx(abs(np.linalg.det(B)), rel=1e-13)
+
+
+def test_hex_body_force_and_tezduyar(oracle):
+    X = (_LOC + 1) / 2 * 0.1
+    g = np.tile([0.0, 0.0, -9.81], (8, 1))
+    zero = np.zeros((8, 3))
+    _, _, a, _ = oracle.ns_element(HEXA3D, ns_params(), X, zero, zero, np.full(8, 1e-3), g)
+    # int N_r g dV = g V / 8 per node for a cube when u_adv = 0
+    assert np.allclose(a[16:24], -9.81 * 1e-3 / 8, rtol=1e-13)
+    assert np.all(a[:16] == 0)
+    # TEZDUYAR tau (SUPG.hpp:149-175): h = min edge, tau_su = 1/(2|u|/h + 2/dt + 4 nu/h^2)
+    u = np.tile([1.0, 0.0, 0.0], (8, 1))
+    prm = ns_params(dt=0.01, use_specializations=False, supg_type=SUPG_TEZDUYAR)
+    _, _, _, tau = oracle.ns_element(HEXA3D, prm, X, u, u, np.full(8, 1e-3))
+    h = 0.1
+    ts = 1 / (2 / h + 2 / 0.01 + 4e-3 / h ** 2)
+    assert tau[1] == pytest.approx(ts, rel=1e-14) and tau[2] == pytest.approx(ts, rel=1e-14)
+
+
+@pytest.mark.parametrize("etype", [TRIAG2D, TETRA3D, HEXA3D])
+def test_combine_theta_and_rhs(oracle, etype):
+    """rhs = -A x + a with the undivided A; matrix = invdt T + theta A' (A' = A with p rows / theta)."""
+    rng = np.random.default_rng(etype + 100)
+    k, dim = {TRIAG2D: (3, 2), TETRA3D: (4, 3), HEXA3D: (8, 3)}[etype]
+    if etype == TRIAG2D:
+        X = np.array([[0, 0], [1, 0.1], [0.2, 0.9]]) * 0.1
+    elif etype == TETRA3D:
+        X = rand_tet(rng, 0.1)
+    else:
+        X = (_LOC + 1) / 2 * 0.1 + rng.uniform(-0.005, 0.005, (8, 3))
+    u = rng.uniform(-1, 1, (k, dim)); ua = rng.uniform(-1, 1, (k, dim))
+    p = rng.uniform(-1, 1, k); nu = np.full(k, 0.01)
+    theta, dt = 0.6, 0.01
+    prm = ns_params(dt=dt, theta=theta)
+    A, T, a, _ = oracle.ns_element(etype, prm, X, u, ua, nu)
+    M, rhs = oracle.ns_element_system(etype, prm, X, u, p, ua, nu)
+    neq = dim + 1
+    x = np.concatenate([u.T.ravel(), p])
+    perm = np.array([(r % k) * neq + r // k for r in range(k * neq)])   # var-major -> node-major
+    rhs_vm = -A @ x + a
+    Ap = A.copy(); Ap[dim * k:] /= theta
+    M_vm = T / dt + theta * Ap
+    M_ref = np.zeros_like(M_vm); M_ref[np.ix_(perm, perm)] = M_vm
+    r_ref = np.zeros_like(rhs_vm); r_ref[perm] = rhs_vm
+    assert np.abs(M - M_ref).max() <= 1e-13 * np.abs(M_ref).max()
+    assert np.abs(rhs - r_ref).max() <= 1e-12 * (np.abs(A) @ np.abs(x)).max()
