@@ -1,0 +1,170 @@
+"""The UFEM Navier-Stokes workloads: meshes, seeded initial fields, boundary-condition lists and
+physics of the benchmark configurations, plus the implicit time step driven through the C ABI.
+
+Boundary conditions follow plugins/UFEM/src/UFEM/BoundaryConditions.cpp (constant vector/scalar BCs,
+applied per region in creation order over the region's nodes in ascending order, in increment form
+new - old: solver/actions/Proto/DirichletBC.hpp:34-49).  The time step is the action sequence of
+ns_implementation/NavierStokes.cpp:103-121: ZeroLSS, LinearizeU, Assembly, BoundaryConditions,
+SolveLSS, UpdateActions.
+"""
+from dataclasses import dataclass, field
+
+import numpy as np
+
+from . import cabi, meshgen
+
+DEFAULT_SEED = 20261202
+
+
+@dataclass
+class Case:
+    name: str
+    etype: int
+    coords: np.ndarray
+    conn: np.ndarray
+    dims: tuple
+    nu: float
+    dt: float
+    rho: float = 1.0
+    theta: float = 1.0
+    u0: np.ndarray = None
+    p0: np.ndarray = None
+    bc_regions: list = field(default_factory=list)   # [(name, nodes, {eq: value or callable(coords)})]
+    solver: str = "bicgstab"
+    tol: float = 1e-10
+
+    @property
+    def dim(self):
+        return self.coords.shape[1]
+
+    @property
+    def n_nodes(self):
+        return self.coords.shape[0]
+
+    @property
+    def n_elems(self):
+        return self.conn.shape[0]
+
+
+def _face_nodes(dims, axis, side):
+    shape = [d + 1 for d in dims]
+    idx = np.unravel_index(np.arange(int(np.prod(shape))), shape[::-1])[::-1]
+    target = 0 if side == 0 else dims[axis]
+    return np.nonzero(idx[axis] == target)[0].astype(np.uint32)
+
+
+def lid_cavity_2d(n=32, seed=DEFAULT_SEED, nu=0.01, dt=0.01):
+    """Unit square, n x n cells split in 2 P1 triangles; lid u=(1,0) on top applied last, no-slip on
+    the other edges, p = 0 at node 0 (corner (0,0)); velocity U[-1,1] per node, p = 0."""
+    coords, conn = meshgen.rect_tris(n, n)
+    N = coords.shape[0]
+    u0 = meshgen.uniform(seed, N * 2, -1.0, 1.0).reshape(N, 2)
+    dims = (n, n)
+    zero = {0: 0.0, 1: 0.0}
+    regions = [("left", _face_nodes(dims, 0, 0), zero), ("right", _face_nodes(dims, 0, 1), zero),
+               ("bottom", _face_nodes(dims, 1, 0), zero), ("top", _face_nodes(dims, 1, 1), {0: 1.0, 1: 0.0}),
+               ("pressure", np.array([0], np.uint32), {2: 0.0})]
+    return Case("lid_cavity_2d_%d" % n, cabi.TRIAG2D, coords, conn, dims, nu, dt, u0=u0, p0=np.zeros(N),
+                bc_regions=regions, solver="gmres")
+
+
+def inflow_cube_tets(n, seed=DEFAULT_SEED, nu=1e-3, dt=5e-3, jitter=0.1, amp=0.0):
+    """Unit cube, n^3 hexahedra in 6 Kuhn tetrahedra, interior jitter +-10% h (tangential on the
+    boundary); no-slip on all faces except x = 1 with u = (-16 y(1-y) z(1-z)(1 + A sin(pi y) sin(pi z)), 0, 0)
+    applied last; p = 0 at the node (0, n/2, n/2); velocity U[-1,1] per node, p = 0."""
+    coords, conn = meshgen.box_tetras(n, n, n)
+    dims = (n, n, n)
+    coords, used = meshgen.jitter(coords, dims, jitter, seed)
+    N = coords.shape[0]
+    u0 = meshgen.uniform(seed, N * 3, -1.0, 1.0, offset=used).reshape(N, 3)
+    zero = {0: 0.0, 1: 0.0, 2: 0.0}
+
+    def inflow(c):
+        y, z = c[:, 1], c[:, 2]
+        return -16.0 * y * (1 - y) * z * (1 - z) * (1 + amp * np.sin(np.pi * y) * np.sin(np.pi * z))
+    regions = [("xmin", _face_nodes(dims, 0, 0), zero), ("ymin", _face_nodes(dims, 1, 0), zero),
+               ("ymax", _face_nodes(dims, 1, 1), zero), ("zmin", _face_nodes(dims, 2, 0), zero),
+               ("zmax", _face_nodes(dims, 2, 1), zero), ("inflow", _face_nodes(dims, 0, 1), {0: inflow, 1: 0.0, 2: 0.0}),
+               ("pressure", np.array([(n // 2 * (n + 1) + n // 2) * (n + 1)], np.uint32), {3: 0.0})]
+    return Case("inflow_cube_tets_%d" % n, cabi.TETRA3D, coords, conn, dims, nu, dt, u0=u0, p0=np.zeros(N),
+                bc_regions=regions)
+
+
+def bc_lists(case, u, p):
+    """Ordered (nodes, eqs, values) of the Dirichlet increments new - old for the current fields."""
+    nodes, eqs, vals = [], [], []
+    for _, rn, spec in case.bc_regions:
+        c = case.coords[rn]
+        for eq, v in sorted(spec.items()):
+            new = v(c) if callable(v) else np.full(len(rn), float(v))
+            old = p[rn] if eq == case.dim else u[rn, eq]
+            nodes.append(rn)
+            eqs.append(np.full(len(rn), eq, np.uint32))
+            vals.append(new - old)
+    # region by region, node by node: within a region the equations of one node are consecutive
+    out_n, out_e, out_v = [], [], []
+    for rn, spec_eqs, spec_vals in _group(case, nodes, eqs, vals):
+        out_n.append(rn); out_e.append(spec_eqs); out_v.append(spec_vals)
+    return (np.concatenate(out_n).astype(np.uint32), np.concatenate(out_e).astype(np.uint32),
+            np.concatenate(out_v))
+
+
+def _group(case, nodes, eqs, vals):
+    i = 0
+    for _, rn, spec in case.bc_regions:
+        ne = len(spec)
+        n_stack = np.stack(nodes[i:i + ne], axis=1).ravel()
+        e_stack = np.stack(eqs[i:i + ne], axis=1).ravel()
+        v_stack = np.stack(vals[i:i + ne], axis=1).ravel()
+        i += ne
+        yield n_stack, e_stack, v_stack
+
+
+class GpuCase:
+    """A case resident on the device: mesh, fields, LSS system; one implicit step per call."""
+
+    def __init__(self, ctx, case, precision=cabi.FP64, use_specializations=True, supg_type=cabi.SUPG_METRIC):
+        self.ctx = ctx
+        self.case = case
+        self.mesh = cabi.Mesh(ctx, case.etype, case.coords, case.conn)
+        self.fields = cabi.Fields(self.mesh)
+        self.system = cabi.System(ctx, self.mesh)
+        self.precision = precision
+        if precision == cabi.FP32:
+            self.system.enable_fp32()
+        self.fields.set(cabi.FIELD_U, case.u0)
+        self.fields.set(cabi.FIELD_P, case.p0)
+        self.fields.initialize(case.nu)
+        self.prm = cabi.ns_params(rho=case.rho, dt=case.dt, theta=case.theta,
+                                  use_specializations=use_specializations, supg_type=supg_type)
+        self.u = case.u0.copy()
+        self.p = case.p0.copy()
+
+    def assemble(self, precision=cabi.FP64):
+        prm = cabi.NSParams.from_buffer_copy(self.prm)
+        prm.precision = precision
+        self.system.assemble(self.fields, prm)
+
+    def step(self, solver_params=None):
+        s = self.system
+        self.fields.linearize()                                   # LinearizeU
+        self.assemble(cabi.FP64)                                  # ZeroLSS + Assembly
+        if self.precision == cabi.FP32:
+            self.assemble(cabi.FP32)
+        n, e, v = bc_lists(self.case, self.u, self.p)            # BoundaryConditions
+        s.dirichlet(n, e, v)
+        if solver_params is None:
+            if self.case.solver == "gmres":
+                solver_params = cabi.solver_params(cabi.GMRES_ILU0, tol=self.case.tol)
+            else:
+                solver_params = cabi.solver_params(tol=self.case.tol, precision=self.precision)
+        st = s.solve(solver_params)                               # SolveLSS
+        s.update(self.fields)                                     # UpdateActions
+        self.u = self.fields.get(cabi.FIELD_U)
+        self.p = self.fields.get(cabi.FIELD_P)
+        return st
+
+    def close(self):
+        self.system.close()
+        self.fields.close()
+        self.mesh.close()

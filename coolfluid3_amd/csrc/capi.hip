@@ -1,0 +1,722 @@
+// C ABI of cf3hip (declared in include/cf3hip.h): argument checks, object lifetimes, host<->device
+// copies; every entry point turns exceptions into status codes.
+#include <algorithm>
+#include <cstring>
+#include <vector>
+
+#include "internal.hpp"
+
+struct cf3hip_context_s { cf3hip::Context c; };
+struct cf3hip_mesh_s { cf3hip::Mesh m; };
+struct cf3hip_lss_s { cf3hip::Lss l; };
+struct cf3hip_fields_s { cf3hip::Fields f; };
+
+namespace cf3hip {
+
+thread_local std::string g_last_error;
+
+void set_error(const std::string& msg) { g_last_error = msg; }
+
+void DevBuf::alloc(Context* c, size_t n)
+{
+  release();
+  ctx = c;
+  if(n == 0) n = 16;
+  HIP_OK(hipSetDevice(c->device));
+  hipError_t e = hipMalloc(&ptr, n);
+  if(e != hipSuccess) {
+    ptr = nullptr;
+    (void)hipGetLastError();
+    CF3_THROW(CF3HIP_ERR_OUT_OF_MEMORY, "device allocation of " + std::to_string(n) + " bytes failed");
+  }
+  bytes = n;
+  c->bytes_in_use += n;
+}
+
+void DevBuf::release()
+{
+  if(ptr) {
+    (void)hipFree(ptr);
+    if(ctx) ctx->bytes_in_use -= bytes;
+  }
+  ptr = nullptr;
+  bytes = 0;
+}
+
+}  // namespace cf3hip
+
+using namespace cf3hip;
+
+#define API_BEGIN try {
+#define API_END                                                                                  \
+  }                                                                                              \
+  catch(const cf3hip::Error& e) {                                                                \
+    set_error(e.msg);                                                                            \
+    return e.code;                                                                               \
+  }                                                                                              \
+  catch(const std::exception& e) {                                                               \
+    set_error(e.what());                                                                         \
+    return CF3HIP_ERR_INVALID_ARG;                                                               \
+  }                                                                                              \
+  return CF3HIP_OK;
+
+#define REQUIRE(cond, code, msg) do { if(!(cond)) CF3_THROW(code, msg); } while(0)
+
+namespace {
+
+int etype_nodes(int et) { return et == CF3HIP_TRIAG2D ? 3 : (et == CF3HIP_TETRA3D ? 4 : 8); }
+int etype_dim(int et) { return et == CF3HIP_TRIAG2D ? 2 : 3; }
+
+void h2d(Context& c, void* dst, const void* src, size_t bytes)
+{
+  HIP_OK(hipMemcpyAsync(dst, src, bytes, hipMemcpyHostToDevice, c.stream));
+}
+
+void d2h(Context& c, void* dst, const void* src, size_t bytes)
+{
+  HIP_OK(hipMemcpyAsync(dst, src, bytes, hipMemcpyDeviceToHost, c.stream));
+  HIP_OK(hipStreamSynchronize(c.stream));
+}
+
+int64_t find_block_host(Lss& L, uint32_t row, uint32_t col)
+{
+  // one node row of the graph (small) is copied to the host to locate a column
+  Context& c = *L.ctx;
+  uint32_t rp[2];
+  d2h(c, rp, L.row_ptr.as<uint32_t>() + row, sizeof(rp));
+  std::vector<uint32_t> cols(rp[1] - rp[0]);
+  if(!cols.empty()) d2h(c, cols.data(), L.cols.as<uint32_t>() + rp[0], sizeof(uint32_t) * cols.size());
+  auto it = std::lower_bound(cols.begin(), cols.end(), col);
+  if(it == cols.end() || *it != col) return -1;
+  return rp[0] + (it - cols.begin());
+}
+
+double* vec_ptr(Lss& L, int which)
+{
+  if(which == CF3HIP_RHS) return L.rhs.as<double>();
+  if(which == CF3HIP_SOLUTION) return L.sol.as<double>();
+  CF3_THROW(CF3HIP_ERR_INVALID_ARG, "vector selector must be CF3HIP_RHS or CF3HIP_SOLUTION");
+}
+
+void alloc_system_storage(Lss& L)
+{
+  Context& c = *L.ctx;
+  const size_t nv = L.n_blocks * L.neq * L.neq;
+  L.values.alloc(&c, sizeof(double) * nv);
+  L.rhs.alloc(&c, sizeof(double) * L.dofs);
+  L.sol.alloc(&c, sizeof(double) * L.dofs);
+  lss_fill(c, L.values.as<double>(), nv, 0.);
+  lss_fill(c, L.rhs.as<double>(), L.dofs, 0.);
+  lss_fill(c, L.sol.as<double>(), L.dofs, 0.);
+  HIP_OK(hipStreamSynchronize(c.stream));
+}
+
+DevBuf* field_buf(Fields& F, int which, int& ncomp)
+{
+  const int dim = F.mesh->dim;
+  switch(which) {
+    case CF3HIP_FIELD_U: ncomp = dim; return F.u;
+    case CF3HIP_FIELD_P: ncomp = 1; return &F.p;
+    case CF3HIP_FIELD_UADV: ncomp = dim; return F.uadv;
+    case CF3HIP_FIELD_U1: ncomp = dim; return F.u1;
+    case CF3HIP_FIELD_U2: ncomp = dim; return F.u2;
+    case CF3HIP_FIELD_U3: ncomp = dim; return F.u3;
+    case CF3HIP_FIELD_NU: ncomp = 1; return &F.nu;
+    case CF3HIP_FIELD_G: ncomp = dim; return F.g;
+    default: CF3_THROW(CF3HIP_ERR_INVALID_ARG, "unknown field selector");
+  }
+}
+
+}  // namespace
+
+extern "C" {
+
+const char* cf3hip_last_error(void) { return g_last_error.c_str(); }
+const char* cf3hip_version(void) { return "cf3hip 0.1 gfx950"; }
+
+int cf3hip_context_create(int device, cf3hip_context* out)
+{
+  API_BEGIN
+  REQUIRE(out, CF3HIP_ERR_INVALID_ARG, "null output");
+  int n = 0;
+  HIP_OK(hipGetDeviceCount(&n));
+  REQUIRE(device >= 0 && device < n, CF3HIP_ERR_INVALID_ARG, "no such device");
+  HIP_OK(hipSetDevice(device));
+  auto* h = new cf3hip_context_s();
+  h->c.device = device;
+  h->c.stream = nullptr;
+  try {
+    h->c.red_partials.alloc(&h->c, sizeof(double) * kMaxRedBlocks * 8);
+    h->c.scalars.alloc(&h->c, sizeof(double) * 64);
+    HIP_OK(hipHostMalloc((void**)&h->c.scalars_host, sizeof(double) * 64, hipHostMallocDefault));
+  } catch(...) {
+    delete h;
+    throw;
+  }
+  *out = h;
+  API_END
+}
+
+int cf3hip_context_destroy(cf3hip_context ctx)
+{
+  API_BEGIN
+  if(!ctx) return CF3HIP_OK;
+  (void)hipStreamSynchronize(ctx->c.stream);
+  if(ctx->c.scalars_host) (void)hipHostFree(ctx->c.scalars_host);
+  delete ctx;
+  API_END
+}
+
+int cf3hip_context_set_stream(cf3hip_context ctx, void* stream)
+{
+  API_BEGIN
+  REQUIRE(ctx, CF3HIP_ERR_INVALID_ARG, "null context");
+  ctx->c.stream = static_cast<hipStream_t>(stream);
+  API_END
+}
+
+int cf3hip_context_synchronize(cf3hip_context ctx)
+{
+  API_BEGIN
+  REQUIRE(ctx, CF3HIP_ERR_INVALID_ARG, "null context");
+  HIP_OK(hipStreamSynchronize(ctx->c.stream));
+  API_END
+}
+
+int cf3hip_context_memory(cf3hip_context ctx, uint64_t* bytes)
+{
+  API_BEGIN
+  REQUIRE(ctx && bytes, CF3HIP_ERR_INVALID_ARG, "null argument");
+  *bytes = ctx->c.bytes_in_use;
+  API_END
+}
+
+// ---------------------------------------------------------------------------------------------
+int cf3hip_mesh_create(cf3hip_context ctx, int etype, uint32_t n_nodes, uint32_t n_elems, const double* coords,
+                       const uint32_t* conn, cf3hip_mesh* out)
+{
+  API_BEGIN
+  REQUIRE(ctx && coords && conn && out, CF3HIP_ERR_INVALID_ARG, "null argument");
+  REQUIRE(etype == CF3HIP_TRIAG2D || etype == CF3HIP_TETRA3D || etype == CF3HIP_HEXA3D, CF3HIP_ERR_INVALID_ARG,
+          "unknown element type");
+  REQUIRE(n_nodes > 0 && n_elems > 0, CF3HIP_ERR_INVALID_ARG, "empty mesh");
+  const int k = etype_nodes(etype), dim = etype_dim(etype);
+  for(uint64_t i = 0; i < (uint64_t)n_elems * k; ++i)
+    REQUIRE(conn[i] < n_nodes, CF3HIP_ERR_INVALID_ARG, "connectivity refers to a node outside the mesh");
+  auto* h = new cf3hip_mesh_s();
+  Mesh& M = h->m;
+  try {
+    Context& c = ctx->c;
+    M.ctx = &c;
+    M.etype = etype; M.dim = dim; M.k = k; M.n_nodes = n_nodes; M.n_elems = n_elems;
+    std::vector<double> comp(n_nodes);
+    for(int d = 0; d < dim; ++d) {
+      for(uint32_t i = 0; i < n_nodes; ++i) comp[i] = coords[(uint64_t)i * dim + d];
+      M.x[d].alloc(&c, sizeof(double) * n_nodes);
+      h2d(c, M.x[d].ptr, comp.data(), sizeof(double) * n_nodes);
+      HIP_OK(hipStreamSynchronize(c.stream));
+    }
+    M.conn.alloc(&c, sizeof(uint32_t) * (uint64_t)n_elems * k);
+    h2d(c, M.conn.ptr, conn, sizeof(uint32_t) * (uint64_t)n_elems * k);
+    HIP_OK(hipStreamSynchronize(c.stream));
+  } catch(...) {
+    delete h;
+    throw;
+  }
+  *out = h;
+  API_END
+}
+
+int cf3hip_mesh_destroy(cf3hip_mesh mesh)
+{
+  API_BEGIN
+  delete mesh;
+  API_END
+}
+
+int cf3hip_mesh_set_coordinates(cf3hip_mesh mesh, const double* coords)
+{
+  API_BEGIN
+  REQUIRE(mesh && coords, CF3HIP_ERR_INVALID_ARG, "null argument");
+  Mesh& M = mesh->m;
+  std::vector<double> comp(M.n_nodes);
+  for(int d = 0; d < M.dim; ++d) {
+    for(uint32_t i = 0; i < M.n_nodes; ++i) comp[i] = coords[(uint64_t)i * M.dim + d];
+    h2d(*M.ctx, M.x[d].ptr, comp.data(), sizeof(double) * M.n_nodes);
+    HIP_OK(hipStreamSynchronize(M.ctx->stream));
+  }
+  API_END
+}
+
+// ---------------------------------------------------------------------------------------------
+int cf3hip_lss_create(cf3hip_context ctx, cf3hip_mesh mesh, int neq, cf3hip_lss* out)
+{
+  API_BEGIN
+  REQUIRE(ctx && mesh && out, CF3HIP_ERR_INVALID_ARG, "null argument");
+  REQUIRE(neq >= 1 && neq <= 4, CF3HIP_ERR_INVALID_ARG, "neq must be in [1, 4]");
+  auto* h = new cf3hip_lss_s();
+  Lss& L = h->l;
+  try {
+    L.ctx = &ctx->c;
+    L.mesh = &mesh->m;
+    L.neq = neq;
+    L.n_nodes = mesh->m.n_nodes;
+    L.dofs = (uint64_t)L.n_nodes * neq;
+    build_element_structures(L);
+    alloc_system_storage(L);
+  } catch(...) {
+    delete h;
+    throw;
+  }
+  *out = h;
+  API_END
+}
+
+int cf3hip_lss_create_from_graph(cf3hip_context ctx, uint32_t n_nodes, int neq, const uint32_t* start,
+                                 const uint32_t* conn, cf3hip_lss* out)
+{
+  API_BEGIN
+  REQUIRE(ctx && start && conn && out, CF3HIP_ERR_INVALID_ARG, "null argument");
+  REQUIRE(neq >= 1 && neq <= 4, CF3HIP_ERR_INVALID_ARG, "neq must be in [1, 4]");
+  REQUIRE(start[0] == 0, CF3HIP_ERR_INVALID_ARG, "starting_indices must start at 0");
+  for(uint32_t i = 0; i < n_nodes; ++i) {
+    REQUIRE(start[i + 1] >= start[i], CF3HIP_ERR_INVALID_ARG, "starting_indices must be non-decreasing");
+    for(uint32_t q = start[i]; q < start[i + 1]; ++q) {
+      REQUIRE(conn[q] < n_nodes, CF3HIP_ERR_INVALID_ARG, "node_connectivity refers to a node outside the graph");
+      REQUIRE(q == start[i] || conn[q] > conn[q - 1], CF3HIP_ERR_INVALID_ARG, "rows must be sorted and unique");
+    }
+  }
+  auto* h = new cf3hip_lss_s();
+  Lss& L = h->l;
+  try {
+    Context& c = ctx->c;
+    L.ctx = &c;
+    L.neq = neq;
+    L.n_nodes = n_nodes;
+    L.dofs = (uint64_t)n_nodes * neq;
+    L.n_blocks = start[n_nodes];
+    L.row_ptr.alloc(&c, sizeof(uint32_t) * (n_nodes + 1));
+    L.cols.alloc(&c, sizeof(uint32_t) * std::max<uint64_t>(L.n_blocks, 1));
+    h2d(c, L.row_ptr.ptr, start, sizeof(uint32_t) * (n_nodes + 1));
+    if(L.n_blocks) h2d(c, L.cols.ptr, conn, sizeof(uint32_t) * L.n_blocks);
+    HIP_OK(hipStreamSynchronize(c.stream));
+    build_graph_structures(L);
+    alloc_system_storage(L);
+  } catch(...) {
+    delete h;
+    throw;
+  }
+  *out = h;
+  API_END
+}
+
+int cf3hip_lss_destroy(cf3hip_lss lss)
+{
+  API_BEGIN
+  delete lss;
+  API_END
+}
+
+int cf3hip_lss_info(cf3hip_lss lss, uint32_t* n_nodes, int* neq, uint64_t* n_blocks)
+{
+  API_BEGIN
+  REQUIRE(lss, CF3HIP_ERR_NOT_CREATED, "system not created");
+  if(n_nodes) *n_nodes = lss->l.n_nodes;
+  if(neq) *neq = lss->l.neq;
+  if(n_blocks) *n_blocks = lss->l.n_blocks;
+  API_END
+}
+
+int cf3hip_lss_get_graph(cf3hip_lss lss, uint32_t* start, uint32_t* conn)
+{
+  API_BEGIN
+  REQUIRE(lss && start && conn, CF3HIP_ERR_INVALID_ARG, "null argument");
+  Lss& L = lss->l;
+  d2h(*L.ctx, start, L.row_ptr.ptr, sizeof(uint32_t) * (L.n_nodes + 1));
+  d2h(*L.ctx, conn, L.cols.ptr, sizeof(uint32_t) * L.n_blocks);
+  API_END
+}
+
+int cf3hip_lss_get_slots(cf3hip_lss lss, uint32_t* slots)
+{
+  API_BEGIN
+  REQUIRE(lss && slots, CF3HIP_ERR_INVALID_ARG, "null argument");
+  Lss& L = lss->l;
+  REQUIRE(L.mesh, CF3HIP_ERR_NOT_CREATED, "system was created from a graph: no element-to-slot map");
+  d2h(*L.ctx, slots, L.slots.ptr, sizeof(uint32_t) * (uint64_t)L.mesh->n_elems * L.mesh->k * L.mesh->k);
+  API_END
+}
+
+int cf3hip_lss_enable_fp32(cf3hip_lss lss)
+{
+  API_BEGIN
+  REQUIRE(lss, CF3HIP_ERR_NOT_CREATED, "system not created");
+  Lss& L = lss->l;
+  if(!L.has_fp32) {
+    const size_t nv = L.n_blocks * L.neq * L.neq;
+    L.values32.alloc(L.ctx, sizeof(float) * nv);
+    lss_fill32(*L.ctx, L.values32.as<float>(), nv, 0.f);
+    HIP_OK(hipStreamSynchronize(L.ctx->stream));
+    L.has_fp32 = true;
+  }
+  API_END
+}
+
+// ---------------------------------------------------------------------------------------------
+int cf3hip_matrix_reset(cf3hip_lss lss, double value)
+{
+  API_BEGIN
+  REQUIRE(lss, CF3HIP_ERR_NOT_CREATED, "system not created");
+  Lss& L = lss->l;
+  lss_fill(*L.ctx, L.values.as<double>(), L.n_blocks * L.neq * L.neq, value);
+  if(L.has_fp32) lss_fill32(*L.ctx, L.values32.as<float>(), L.n_blocks * L.neq * L.neq, (float)value);
+  HIP_OK(hipStreamSynchronize(L.ctx->stream));
+  API_END
+}
+
+static int entry_access(cf3hip_lss lss, uint32_t icol, uint32_t irow, double* v, int mode)
+{
+  API_BEGIN
+  REQUIRE(lss, CF3HIP_ERR_NOT_CREATED, "system not created");
+  Lss& L = lss->l;
+  REQUIRE(irow < L.dofs && icol < L.dofs, CF3HIP_ERR_ILLEGAL_ENTRY, "Trying to access an illegal entry.");
+  const int64_t q = find_block_host(L, irow / L.neq, icol / L.neq);
+  REQUIRE(q >= 0, CF3HIP_ERR_ILLEGAL_ENTRY, "Trying to access an illegal entry.");
+  double* p = L.values.as<double>() + (uint64_t)q * L.neq * L.neq + (irow % L.neq) * L.neq + (icol % L.neq);
+  double cur = 0.;
+  d2h(*L.ctx, &cur, p, sizeof(double));
+  if(mode == 0) { *v = cur; return CF3HIP_OK; }
+  const double nv = mode == 1 ? *v : cur + *v;
+  h2d(*L.ctx, p, &nv, sizeof(double));
+  HIP_OK(hipStreamSynchronize(L.ctx->stream));
+  API_END
+}
+
+int cf3hip_matrix_set_value(cf3hip_lss lss, uint32_t icol, uint32_t irow, double v) { return entry_access(lss, icol, irow, &v, 1); }
+int cf3hip_matrix_add_value(cf3hip_lss lss, uint32_t icol, uint32_t irow, double v) { return entry_access(lss, icol, irow, &v, 2); }
+int cf3hip_matrix_get_value(cf3hip_lss lss, uint32_t icol, uint32_t irow, double* v)
+{
+  if(!v) { set_error("null argument"); return CF3HIP_ERR_INVALID_ARG; }
+  return entry_access(lss, icol, irow, v, 0);
+}
+
+int cf3hip_matrix_set_row(cf3hip_lss lss, uint32_t blockrow, uint32_t ieq, double diag, double offdiag)
+{
+  API_BEGIN
+  REQUIRE(lss, CF3HIP_ERR_NOT_CREATED, "system not created");
+  Lss& L = lss->l;
+  REQUIRE(blockrow < L.n_nodes && ieq < (uint32_t)L.neq, CF3HIP_ERR_ILLEGAL_ENTRY, "Trying to access an illegal entry.");
+  uint32_t rp[2];
+  d2h(*L.ctx, rp, L.row_ptr.as<uint32_t>() + blockrow, sizeof(rp));
+  const uint32_t nb = rp[1] - rp[0];
+  const int nn = L.neq * L.neq;
+  std::vector<uint32_t> cols(nb);
+  std::vector<double> blk((size_t)nb * nn);
+  d2h(*L.ctx, cols.data(), L.cols.as<uint32_t>() + rp[0], sizeof(uint32_t) * nb);
+  d2h(*L.ctx, blk.data(), L.values.as<double>() + (uint64_t)rp[0] * nn, sizeof(double) * blk.size());
+  for(uint32_t s = 0; s < nb; ++s) {
+    for(int j = 0; j < L.neq; ++j) blk[(size_t)s * nn + ieq * L.neq + j] = offdiag;
+    if(cols[s] == blockrow) blk[(size_t)s * nn + ieq * L.neq + ieq] = diag;
+  }
+  h2d(*L.ctx, L.values.as<double>() + (uint64_t)rp[0] * nn, blk.data(), sizeof(double) * blk.size());
+  HIP_OK(hipStreamSynchronize(L.ctx->stream));
+  API_END
+}
+
+int cf3hip_matrix_add_values(cf3hip_lss lss, uint32_t nb, const uint32_t* indices, const double* mat)
+{
+  API_BEGIN
+  REQUIRE(lss && indices && mat, CF3HIP_ERR_INVALID_ARG, "null argument");
+  Lss& L = lss->l;
+  const int neq = L.neq, nn = neq * neq;
+  const uint32_t n = nb * neq;
+  for(uint32_t a = 0; a < nb; ++a) {
+    REQUIRE(indices[a] < L.n_nodes, CF3HIP_ERR_ILLEGAL_ENTRY, "Trying to access an illegal entry.");
+    for(uint32_t b = 0; b < nb; ++b) {
+      const int64_t q = find_block_host(L, indices[a], indices[b]);
+      REQUIRE(q >= 0, CF3HIP_ERR_ILLEGAL_ENTRY, "Trying to access an illegal entry.");
+      double blk[16];
+      double* p = L.values.as<double>() + (uint64_t)q * nn;
+      d2h(*L.ctx, blk, p, sizeof(double) * nn);
+      for(int al = 0; al < neq; ++al)
+        for(int be = 0; be < neq; ++be) blk[al * neq + be] += mat[(uint64_t)(a * neq + al) * n + b * neq + be];
+      h2d(*L.ctx, p, blk, sizeof(double) * nn);
+      HIP_OK(hipStreamSynchronize(L.ctx->stream));
+    }
+  }
+  API_END
+}
+
+int cf3hip_matrix_get_diagonal(cf3hip_lss lss, double* diag)
+{
+  API_BEGIN
+  REQUIRE(lss && diag, CF3HIP_ERR_INVALID_ARG, "null argument");
+  Lss& L = lss->l;
+  std::vector<uint32_t> db(L.n_nodes);
+  d2h(*L.ctx, db.data(), L.diag_blk.ptr, sizeof(uint32_t) * L.n_nodes);
+  const int nn = L.neq * L.neq;
+  std::vector<double> vals(L.n_blocks * nn);
+  d2h(*L.ctx, vals.data(), L.values.ptr, sizeof(double) * vals.size());
+  for(uint32_t i = 0; i < L.n_nodes; ++i)
+    for(int a = 0; a < L.neq; ++a) diag[(uint64_t)i * L.neq + a] = vals[(uint64_t)db[i] * nn + a * L.neq + a];
+  API_END
+}
+
+int cf3hip_matrix_apply(cf3hip_lss lss, double alpha, const double* x, double beta, double* y)
+{
+  API_BEGIN
+  REQUIRE(lss && x && y, CF3HIP_ERR_INVALID_ARG, "null argument");
+  lss_spmv(lss->l, alpha, x, beta, y);
+  HIP_OK(hipStreamSynchronize(lss->l.ctx->stream));
+  API_END
+}
+
+int cf3hip_matrix_apply_fp32(cf3hip_lss lss, float alpha, const float* x, float beta, float* y)
+{
+  API_BEGIN
+  REQUIRE(lss && x && y, CF3HIP_ERR_INVALID_ARG, "null argument");
+  lss_spmv32(lss->l, alpha, x, beta, y);
+  HIP_OK(hipStreamSynchronize(lss->l.ctx->stream));
+  API_END
+}
+
+int cf3hip_matrix_get_values(cf3hip_lss lss, double* values)
+{
+  API_BEGIN
+  REQUIRE(lss && values, CF3HIP_ERR_INVALID_ARG, "null argument");
+  Lss& L = lss->l;
+  d2h(*L.ctx, values, L.values.ptr, sizeof(double) * L.n_blocks * L.neq * L.neq);
+  API_END
+}
+
+int cf3hip_matrix_get_values_fp32(cf3hip_lss lss, float* values)
+{
+  API_BEGIN
+  REQUIRE(lss && values, CF3HIP_ERR_INVALID_ARG, "null argument");
+  Lss& L = lss->l;
+  REQUIRE(L.has_fp32, CF3HIP_ERR_NOT_CREATED, "FP32 values not enabled");
+  d2h(*L.ctx, values, L.values32.ptr, sizeof(float) * L.n_blocks * L.neq * L.neq);
+  API_END
+}
+
+int cf3hip_matrix_debug_data(cf3hip_lss lss, uint32_t* row_ptr, uint32_t* col_idx, double* vals)
+{
+  API_BEGIN
+  REQUIRE(lss && row_ptr && col_idx && vals, CF3HIP_ERR_INVALID_ARG, "null argument");
+  Lss& L = lss->l;
+  const uint64_t nnz = L.n_blocks * L.neq * L.neq;
+  DevBuf pr, pc, pv;
+  pr.alloc(L.ctx, sizeof(uint32_t) * (L.dofs + 1));
+  pc.alloc(L.ctx, sizeof(uint32_t) * nnz);
+  pv.alloc(L.ctx, sizeof(double) * nnz);
+  lss_point_csr(L, pr.as<uint32_t>(), pc.as<uint32_t>(), pv.as<double>());
+  d2h(*L.ctx, row_ptr, pr.ptr, sizeof(uint32_t) * (L.dofs + 1));
+  d2h(*L.ctx, col_idx, pc.ptr, sizeof(uint32_t) * nnz);
+  d2h(*L.ctx, vals, pv.ptr, sizeof(double) * nnz);
+  API_END
+}
+
+// ---------------------------------------------------------------------------------------------
+int cf3hip_vector_reset(cf3hip_lss lss, int which, double value)
+{
+  API_BEGIN
+  REQUIRE(lss, CF3HIP_ERR_NOT_CREATED, "system not created");
+  lss_fill(*lss->l.ctx, vec_ptr(lss->l, which), lss->l.dofs, value);
+  HIP_OK(hipStreamSynchronize(lss->l.ctx->stream));
+  API_END
+}
+
+int cf3hip_vector_get(cf3hip_lss lss, int which, double* host)
+{
+  API_BEGIN
+  REQUIRE(lss && host, CF3HIP_ERR_INVALID_ARG, "null argument");
+  d2h(*lss->l.ctx, host, vec_ptr(lss->l, which), sizeof(double) * lss->l.dofs);
+  API_END
+}
+
+int cf3hip_vector_set(cf3hip_lss lss, int which, const double* host)
+{
+  API_BEGIN
+  REQUIRE(lss && host, CF3HIP_ERR_INVALID_ARG, "null argument");
+  h2d(*lss->l.ctx, vec_ptr(lss->l, which), host, sizeof(double) * lss->l.dofs);
+  HIP_OK(hipStreamSynchronize(lss->l.ctx->stream));
+  API_END
+}
+
+int cf3hip_vector_set_value(cf3hip_lss lss, int which, uint32_t blockrow, uint32_t ieq, double v)
+{
+  API_BEGIN
+  REQUIRE(lss, CF3HIP_ERR_NOT_CREATED, "system not created");
+  REQUIRE(blockrow < lss->l.n_nodes && ieq < (uint32_t)lss->l.neq, CF3HIP_ERR_ILLEGAL_ENTRY, "Trying to access an illegal entry.");
+  h2d(*lss->l.ctx, vec_ptr(lss->l, which) + (uint64_t)blockrow * lss->l.neq + ieq, &v, sizeof(double));
+  HIP_OK(hipStreamSynchronize(lss->l.ctx->stream));
+  API_END
+}
+
+int cf3hip_vector_get_value(cf3hip_lss lss, int which, uint32_t blockrow, uint32_t ieq, double* v)
+{
+  API_BEGIN
+  REQUIRE(lss && v, CF3HIP_ERR_INVALID_ARG, "null argument");
+  REQUIRE(blockrow < lss->l.n_nodes && ieq < (uint32_t)lss->l.neq, CF3HIP_ERR_ILLEGAL_ENTRY, "Trying to access an illegal entry.");
+  d2h(*lss->l.ctx, v, vec_ptr(lss->l, which) + (uint64_t)blockrow * lss->l.neq + ieq, sizeof(double));
+  API_END
+}
+
+int cf3hip_vector_device_ptr(cf3hip_lss lss, int which, double** dev)
+{
+  API_BEGIN
+  REQUIRE(lss && dev, CF3HIP_ERR_INVALID_ARG, "null argument");
+  *dev = vec_ptr(lss->l, which);
+  API_END
+}
+
+int cf3hip_lss_dirichlet(cf3hip_lss lss, uint32_t nbc, const uint32_t* nodes, const uint32_t* eqs, const double* values)
+{
+  API_BEGIN
+  REQUIRE(lss, CF3HIP_ERR_NOT_CREATED, "system not created");
+  REQUIRE(nbc == 0 || (nodes && eqs && values), CF3HIP_ERR_INVALID_ARG, "null argument");
+  lss_dirichlet(lss->l, nbc, nodes, eqs, values);
+  API_END
+}
+
+int cf3hip_lss_solve(cf3hip_lss lss, const cf3hip_solver_params* prm, cf3hip_solve_stats* stats)
+{
+  API_BEGIN
+  REQUIRE(lss && prm, CF3HIP_ERR_INVALID_ARG, "null argument");
+  REQUIRE(prm->tol > 0. && prm->max_iter > 0, CF3HIP_ERR_INVALID_ARG, "tolerance and iteration limit must be positive");
+  cf3hip_solve_stats st;
+  lss_solve(lss->l, *prm, st);
+  if(stats) *stats = st;
+  if(!st.converged) {
+    set_error("Krylov solve did not reach the tolerance");
+    return CF3HIP_ERR_NOT_CONVERGED;
+  }
+  API_END
+}
+
+int cf3hip_lss_residual(cf3hip_lss lss, double* rel)
+{
+  API_BEGIN
+  REQUIRE(lss && rel, CF3HIP_ERR_INVALID_ARG, "null argument");
+  *rel = lss_residual(lss->l);
+  API_END
+}
+
+// ---------------------------------------------------------------------------------------------
+int cf3hip_fields_create(cf3hip_mesh mesh, cf3hip_fields* out)
+{
+  API_BEGIN
+  REQUIRE(mesh && out, CF3HIP_ERR_INVALID_ARG, "null argument");
+  auto* h = new cf3hip_fields_s();
+  Fields& F = h->f;
+  try {
+    Mesh& M = mesh->m;
+    Context& c = *M.ctx;
+    F.mesh = &M;
+    const size_t bytes = sizeof(double) * M.n_nodes;
+    for(int d = 0; d < M.dim; ++d) {
+      DevBuf* all[6] = {&F.u[d], &F.uadv[d], &F.u1[d], &F.u2[d], &F.u3[d], &F.g[d]};
+      for(DevBuf* b : all) {
+        b->alloc(&c, bytes);
+        lss_fill(c, b->as<double>(), M.n_nodes, 0.);
+      }
+    }
+    F.p.alloc(&c, bytes);
+    F.nu.alloc(&c, bytes);
+    lss_fill(c, F.p.as<double>(), M.n_nodes, 0.);
+    lss_fill(c, F.nu.as<double>(), M.n_nodes, 0.);
+    HIP_OK(hipStreamSynchronize(c.stream));
+  } catch(...) {
+    delete h;
+    throw;
+  }
+  *out = h;
+  API_END
+}
+
+int cf3hip_fields_destroy(cf3hip_fields fields)
+{
+  API_BEGIN
+  delete fields;
+  API_END
+}
+
+int cf3hip_fields_set(cf3hip_fields fields, int which, const double* host)
+{
+  API_BEGIN
+  REQUIRE(fields && host, CF3HIP_ERR_INVALID_ARG, "null argument");
+  Fields& F = fields->f;
+  int nc = 0;
+  DevBuf* b = field_buf(F, which, nc);
+  const uint32_t n = F.mesh->n_nodes;
+  std::vector<double> comp(n);
+  for(int d = 0; d < nc; ++d) {
+    for(uint32_t i = 0; i < n; ++i) comp[i] = host[(uint64_t)i * nc + d];
+    h2d(*F.mesh->ctx, b[d].ptr, comp.data(), sizeof(double) * n);
+    HIP_OK(hipStreamSynchronize(F.mesh->ctx->stream));
+  }
+  API_END
+}
+
+int cf3hip_fields_get(cf3hip_fields fields, int which, double* host)
+{
+  API_BEGIN
+  REQUIRE(fields && host, CF3HIP_ERR_INVALID_ARG, "null argument");
+  Fields& F = fields->f;
+  int nc = 0;
+  DevBuf* b = field_buf(F, which, nc);
+  const uint32_t n = F.mesh->n_nodes;
+  std::vector<double> comp(n);
+  for(int d = 0; d < nc; ++d) {
+    d2h(*F.mesh->ctx, comp.data(), b[d].ptr, sizeof(double) * n);
+    for(uint32_t i = 0; i < n; ++i) host[(uint64_t)i * nc + d] = comp[i];
+  }
+  API_END
+}
+
+int cf3hip_fields_initialize(cf3hip_fields fields, double nu)
+{
+  API_BEGIN
+  REQUIRE(fields, CF3HIP_ERR_INVALID_ARG, "null argument");
+  Fields& F = fields->f;
+  Mesh& M = *F.mesh;
+  Context& c = *M.ctx;
+  const size_t bytes = sizeof(double) * M.n_nodes;
+  for(int d = 0; d < M.dim; ++d) {
+    HIP_OK(hipMemcpyAsync(F.uadv[d].ptr, F.u[d].ptr, bytes, hipMemcpyDeviceToDevice, c.stream));
+    HIP_OK(hipMemcpyAsync(F.u1[d].ptr, F.u[d].ptr, bytes, hipMemcpyDeviceToDevice, c.stream));
+    HIP_OK(hipMemcpyAsync(F.u2[d].ptr, F.u[d].ptr, bytes, hipMemcpyDeviceToDevice, c.stream));
+    HIP_OK(hipMemcpyAsync(F.u3[d].ptr, F.u[d].ptr, bytes, hipMemcpyDeviceToDevice, c.stream));
+  }
+  lss_fill(c, F.nu.as<double>(), M.n_nodes, nu);
+  HIP_OK(hipStreamSynchronize(c.stream));
+  API_END
+}
+
+int cf3hip_ns_linearize(cf3hip_fields fields)
+{
+  API_BEGIN
+  REQUIRE(fields, CF3HIP_ERR_INVALID_ARG, "null argument");
+  ns_linearize(fields->f);
+  HIP_OK(hipStreamSynchronize(fields->f.mesh->ctx->stream));
+  API_END
+}
+
+int cf3hip_ns_assemble(cf3hip_lss lss, cf3hip_fields fields, const cf3hip_ns_params* prm)
+{
+  API_BEGIN
+  REQUIRE(lss && fields && prm, CF3HIP_ERR_INVALID_ARG, "null argument");
+  assemble(lss->l, fields->f, *prm);
+  API_END
+}
+
+int cf3hip_ns_update(cf3hip_lss lss, cf3hip_fields fields)
+{
+  API_BEGIN
+  REQUIRE(lss && fields, CF3HIP_ERR_INVALID_ARG, "null argument");
+  ns_update(lss->l, fields->f);
+  HIP_OK(hipStreamSynchronize(fields->f.mesh->ctx->stream));
+  API_END
+}
+
+}  // extern "C"

@@ -1,0 +1,140 @@
+// Internal declarations shared by the cf3hip translation units (gfx950 only).
+#pragma once
+
+#include <hip/hip_runtime.h>
+
+#include <cstdint>
+#include <string>
+#include <vector>
+
+#include "../../include/cf3hip.h"
+
+namespace cf3hip {
+
+// ---------------------------------------------------------------------------------------------
+// errors
+// ---------------------------------------------------------------------------------------------
+void set_error(const std::string& msg);
+
+struct Error {
+  int code;
+  std::string msg;
+};
+
+#define CF3_THROW(code, msg) throw ::cf3hip::Error{code, msg}
+#define HIP_OK(expr)                                                                             \
+  do {                                                                                           \
+    hipError_t e_ = (expr);                                                                      \
+    if(e_ != hipSuccess)                                                                         \
+      CF3_THROW(e_ == hipErrorOutOfMemory ? CF3HIP_ERR_OUT_OF_MEMORY : CF3HIP_ERR_HIP,           \
+                std::string(#expr) + ": " + hipGetErrorString(e_));                             \
+  } while(0)
+
+// ---------------------------------------------------------------------------------------------
+// device buffers
+// ---------------------------------------------------------------------------------------------
+struct Context;
+
+struct DevBuf {
+  void* ptr = nullptr;
+  size_t bytes = 0;
+  Context* ctx = nullptr;
+  DevBuf() = default;
+  DevBuf(const DevBuf&) = delete;
+  DevBuf& operator=(const DevBuf&) = delete;
+  ~DevBuf() { release(); }
+  void alloc(Context* c, size_t n);
+  void release();
+  template<typename T> T* as() const { return static_cast<T*>(ptr); }
+};
+
+struct Context {
+  int device = 0;
+  hipStream_t stream = nullptr;
+  uint64_t bytes_in_use = 0;
+  // scratch for reductions: partial sums and device scalars
+  DevBuf red_partials;   // double[kMaxRedBlocks * 8]
+  DevBuf scalars;        // double[64]
+  DevBuf host_flag;      // unused placeholder for alignment
+  double* scalars_host = nullptr;   // pinned mirror
+};
+
+constexpr int kMaxRedBlocks = 1024;
+
+// ---------------------------------------------------------------------------------------------
+// mesh / system / fields
+// ---------------------------------------------------------------------------------------------
+struct Mesh {
+  Context* ctx = nullptr;
+  int etype = 0, dim = 0, k = 0;
+  uint32_t n_nodes = 0, n_elems = 0;
+  DevBuf x[3];        // SoA coordinates (double)
+  DevBuf conn;        // uint32 n_elems*k, element-major
+};
+
+struct Lss {
+  Context* ctx = nullptr;
+  Mesh* mesh = nullptr;   // null when created from a caller graph
+  int neq = 0;
+  uint32_t n_nodes = 0;
+  uint64_t n_blocks = 0;
+  uint64_t dofs = 0;
+  DevBuf row_ptr;       // uint32 n_nodes+1 (starting_indices)
+  DevBuf cols;          // uint32 n_blocks   (node_connectivity)
+  DevBuf diag_blk;      // uint32 n_nodes: block index of the diagonal
+  DevBuf values;        // double n_blocks*neq*neq
+  DevBuf values32;      // float mirror (precision form)
+  bool has_fp32 = false;
+  DevBuf rhs, sol;      // double dofs
+  // element scatter structures (lss_create from a mesh)
+  DevBuf slots;         // uint32 n_elems*k*k   element-to-slot map (block slots)
+  DevBuf inc_ptr;       // uint32 n_nodes+1
+  DevBuf inc;           // uint32 n_elems*k : e*k + a, sorted by (node, e)
+  DevBuf contrib_ptr;   // uint32 n_blocks+1
+  DevBuf contrib;       // uint16 n_elems*k*k : (q << 4) | b, sorted by (slot, b, q)
+  DevBuf wg_rows;       // uint32 n_wg+1 : row ranges of the assembly workgroups
+  uint32_t n_wg = 0;
+  // Krylov workspace (allocated lazily)
+  DevBuf work;          // generic vector workspace
+  size_t work_vectors = 0;
+  int work_real_bytes = 0;
+  DevBuf ilu;           // ILU0 data (point CSR) for GMRES
+  bool ilu_ready = false;
+};
+
+struct Fields {
+  Mesh* mesh = nullptr;
+  DevBuf u[3], p, uadv[3], u1[3], u2[3], u3[3], nu, g[3];
+};
+
+// ---------------------------------------------------------------------------------------------
+// entry points between translation units
+// ---------------------------------------------------------------------------------------------
+void build_element_structures(Lss& L);                   // sparsity.hip
+void build_graph_structures(Lss& L);                     // sparsity.hip (diag blocks)
+void assemble(Lss& L, Fields& F, const cf3hip_ns_params& prm);   // assembly.hip
+
+// lss.hip
+void lss_fill(Context& c, double* p, size_t n, double v);
+void lss_fill32(Context& c, float* p, size_t n, float v);
+void lss_spmv(Lss& L, double alpha, const double* x, double beta, double* y);
+void lss_spmv32(Lss& L, float alpha, const float* x, float beta, float* y);
+void lss_dirichlet(Lss& L, uint32_t nbc, const uint32_t* nodes, const uint32_t* eqs, const double* values);
+void lss_point_csr(Lss& L, uint32_t* row_ptr_dev, uint32_t* col_dev, double* val_dev);
+
+// krylov.hip
+void lss_solve(Lss& L, const cf3hip_solver_params& prm, cf3hip_solve_stats& st);
+double lss_residual(Lss& L);
+
+// nodes.hip
+void ns_linearize(Fields& F);
+void ns_update(Lss& L, Fields& F);
+
+inline unsigned grid_for(size_t n, unsigned block)
+{
+  size_t g = (n + block - 1) / block;
+  if(g == 0) g = 1;
+  return (unsigned)g;
+}
+
+}  // namespace cf3hip

@@ -1,0 +1,374 @@
+// Krylov solve on the device (gfx950): BiCGStab with Jacobi right preconditioning in FP64 or FP32,
+// FP64 iterative refinement around the FP32 solve, residual.
+//
+// Reference: math::LSS::SolutionStrategy::solve / compute_residual
+// (cf3/math/LSS/Trilinos/TrilinosStratimikosStrategy.cpp:135-207).  The Belos/Ifpack arithmetic is
+// third-party; the method here is the textbook BiCGStab of van der Vorst (1992) with the stopping rule
+// ||r|| <= tol ||b||, restated identically in the oracle.
+//
+// All scalars live in device memory; reductions are two-pass with a fixed grid, so every dot product
+// has a fixed summation order (deterministic results, no floating-point atomics).  The host reads one
+// small block of scalars per iteration for the convergence test.
+#include <algorithm>
+#include <cmath>
+
+#include "internal.hpp"
+
+namespace cf3hip {
+
+namespace {
+
+constexpr int kRT = 256;   // reduction block size
+
+inline unsigned red_grid(uint64_t n) { return std::min<unsigned>(kMaxRedBlocks, grid_for(n, kRT)); }
+
+template<int NDOT>
+__device__ inline void block_reduce_store(double (&v)[NDOT], double* partials, unsigned G)
+{
+  __shared__ double sh[NDOT][kRT / 64];
+  for(int d = 0; d < NDOT; ++d) {
+    double x = v[d];
+    for(int off = 32; off > 0; off >>= 1) x += __shfl_down(x, off, 64);
+    if((threadIdx.x & 63) == 0) sh[d][threadIdx.x >> 6] = x;
+  }
+  __syncthreads();
+  if(threadIdx.x == 0)
+    for(int d = 0; d < NDOT; ++d) {
+      double s = 0.;
+      for(int w = 0; w < kRT / 64; ++w) s += sh[d][w];
+      partials[d * G + blockIdx.x] = s;
+    }
+}
+
+// S layout (double scalars)
+enum { S_RHO = 0, S_RHO_PREV, S_ALPHA, S_OMEGA, S_RHV, S_SN2, S_TS, S_TT, S_RN2, S_BN2, S_SCONV, S_THR2,
+       S_TMP0, S_TMP1, S_COUNT };
+
+// finalize: sum partials of NDOT dots into S[dst + d], then apply a mode-specific scalar update
+enum { FIN_STORE = 0, FIN_ALPHA, FIN_SNORM, FIN_OMEGA, FIN_RNORM };
+
+template<int NDOT>
+__global__ void k_finalize(const double* __restrict__ partials, unsigned G, double* __restrict__ S, int dst, int mode)
+{
+  __shared__ double sh[NDOT][kRT / 64];
+  double v[NDOT];
+  for(int d = 0; d < NDOT; ++d) {
+    double s = 0.;
+    for(unsigned i = threadIdx.x; i < G; i += kRT) s += partials[d * G + i];
+    for(int off = 32; off > 0; off >>= 1) s += __shfl_down(s, off, 64);
+    if((threadIdx.x & 63) == 0) sh[d][threadIdx.x >> 6] = s;
+  }
+  __syncthreads();
+  if(threadIdx.x != 0) return;
+  for(int d = 0; d < NDOT; ++d) {
+    double s = 0.;
+    for(int w = 0; w < kRT / 64; ++w) s += sh[d][w];
+    v[d] = s;
+  }
+  if(mode == FIN_STORE) {
+    for(int d = 0; d < NDOT; ++d) S[dst + d] = v[d];
+  } else if(mode == FIN_ALPHA) {          // v0 = rh . v
+    S[S_RHV] = v[0];
+    S[S_ALPHA] = S[S_RHO] / v[0];
+  } else if(mode == FIN_SNORM) {          // v0 = s . s
+    S[S_SN2] = v[0];
+    S[S_SCONV] = v[0] <= S[S_THR2] ? 1. : 0.;
+  } else if(mode == FIN_OMEGA) {          // v0 = t . s, v1 = t . t
+    S[S_TS] = v[0];
+    S[S_TT] = v[NDOT > 1 ? 1 : 0];
+    S[S_OMEGA] = S[S_SCONV] != 0. ? 0. : v[0] / v[NDOT > 1 ? 1 : 0];
+  } else if(mode == FIN_RNORM) {          // v0 = r . r, v1 = rh . r
+    S[S_RN2] = v[0];
+    S[S_RHO_PREV] = S[S_RHO];
+    S[S_RHO] = v[NDOT > 1 ? 1 : 0];
+  }
+}
+
+template<typename R>
+__global__ void k_dot2(uint64_t n, const R* __restrict__ a0, const R* __restrict__ b0, const R* __restrict__ a1,
+                       const R* __restrict__ b1, double* partials)
+{
+  double v[2] = {0., 0.};
+  const uint64_t stride = (uint64_t)gridDim.x * kRT;
+  for(uint64_t i = (uint64_t)blockIdx.x * kRT + threadIdx.x; i < n; i += stride) {
+    v[0] += (double)a0[i] * (double)b0[i];
+    if(a1) v[1] += (double)a1[i] * (double)b1[i];
+  }
+  block_reduce_store<2>(v, partials, gridDim.x);
+}
+
+template<typename R>
+__global__ void k_jacobi(uint32_t N, int neq, const uint32_t* __restrict__ diag_blk, const R* __restrict__ vals,
+                         R* __restrict__ dinv)
+{
+  const uint64_t r = (uint64_t)blockIdx.x * blockDim.x + threadIdx.x;
+  if(r >= (uint64_t)N * neq) return;
+  const uint32_t i = (uint32_t)(r / neq);
+  const int a = (int)(r % neq);
+  dinv[r] = R(1) / vals[(uint64_t)diag_blk[i] * neq * neq + a * neq + a];
+}
+
+// r = b - r (r holds A x on entry), rh = r; partial ||r||^2 and ||b||^2
+template<typename R>
+__global__ void k_bicg_init(uint64_t n, const R* __restrict__ b, R* __restrict__ r, R* __restrict__ rh,
+                            R* __restrict__ p, R* __restrict__ v, double* partials)
+{
+  double acc[2] = {0., 0.};
+  const uint64_t stride = (uint64_t)gridDim.x * kRT;
+  for(uint64_t i = (uint64_t)blockIdx.x * kRT + threadIdx.x; i < n; i += stride) {
+    const R ri = b[i] - r[i];
+    r[i] = ri; rh[i] = ri; p[i] = R(0); v[i] = R(0);
+    acc[0] += (double)ri * (double)ri;
+    acc[1] += (double)b[i] * (double)b[i];
+  }
+  block_reduce_store<2>(acc, partials, gridDim.x);
+}
+
+template<typename R>
+__global__ void k_bicg_p(uint64_t n, const double* __restrict__ S, const R* __restrict__ r, R* __restrict__ p,
+                         const R* __restrict__ v, R* __restrict__ ph, const R* __restrict__ dinv)
+{
+  const double beta_d = (S[S_RHO] / S[S_RHO_PREV]) * (S[S_ALPHA] / S[S_OMEGA]);
+  const R beta = (R)beta_d, omega = (R)S[S_OMEGA];
+  const uint64_t stride = (uint64_t)gridDim.x * blockDim.x;
+  for(uint64_t i = (uint64_t)blockIdx.x * blockDim.x + threadIdx.x; i < n; i += stride) {
+    const R pi = r[i] + beta * (p[i] - omega * v[i]);
+    p[i] = pi;
+    ph[i] = dinv[i] * pi;
+  }
+}
+
+template<typename R>
+__global__ void k_bicg_s(uint64_t n, const double* __restrict__ S, const R* __restrict__ r, const R* __restrict__ v,
+                         R* __restrict__ s, R* __restrict__ sh, const R* __restrict__ dinv, double* partials)
+{
+  const R alpha = (R)S[S_ALPHA];
+  double acc[1] = {0.};
+  const uint64_t stride = (uint64_t)gridDim.x * kRT;
+  for(uint64_t i = (uint64_t)blockIdx.x * kRT + threadIdx.x; i < n; i += stride) {
+    const R si = r[i] - alpha * v[i];
+    s[i] = si;
+    sh[i] = dinv[i] * si;
+    acc[0] += (double)si * (double)si;
+  }
+  block_reduce_store<1>(acc, partials, gridDim.x);
+}
+
+template<typename R>
+__global__ void k_bicg_x(uint64_t n, const double* __restrict__ S, R* __restrict__ x, const R* __restrict__ ph,
+                         const R* __restrict__ sh, R* __restrict__ r, const R* __restrict__ s, const R* __restrict__ t,
+                         const R* __restrict__ rh, double* partials)
+{
+  const R alpha = (R)S[S_ALPHA], omega = (R)S[S_OMEGA];
+  double acc[2] = {0., 0.};
+  const uint64_t stride = (uint64_t)gridDim.x * kRT;
+  for(uint64_t i = (uint64_t)blockIdx.x * kRT + threadIdx.x; i < n; i += stride) {
+    x[i] += alpha * ph[i] + omega * sh[i];
+    const R ri = s[i] - omega * t[i];
+    r[i] = ri;
+    acc[0] += (double)ri * (double)ri;
+    acc[1] += (double)rh[i] * (double)ri;
+  }
+  block_reduce_store<2>(acc, partials, gridDim.x);
+}
+
+__global__ void k_set_scalars(double* S, double thr2)
+{
+  S[S_RHO_PREV] = 1.; S[S_ALPHA] = 1.; S[S_OMEGA] = 1.; S[S_THR2] = thr2; S[S_SCONV] = 0.;
+}
+
+template<typename Src, typename Dst>
+__global__ void k_convert_scaled(uint64_t n, const Src* __restrict__ a, Dst* __restrict__ b, double scale)
+{
+  const uint64_t stride = (uint64_t)gridDim.x * blockDim.x;
+  for(uint64_t i = (uint64_t)blockIdx.x * blockDim.x + threadIdx.x; i < n; i += stride) b[i] = (Dst)((double)a[i] * scale);
+}
+
+__global__ void k_axpy_f32(uint64_t n, double a, const float* __restrict__ d, double* __restrict__ x)
+{
+  const uint64_t stride = (uint64_t)gridDim.x * blockDim.x;
+  for(uint64_t i = (uint64_t)blockIdx.x * blockDim.x + threadIdx.x; i < n; i += stride) x[i] += a * (double)d[i];
+}
+
+template<typename R>
+void spmv_t(Lss& L, R alpha, const R* x, R beta, R* y);
+template<>
+void spmv_t<double>(Lss& L, double alpha, const double* x, double beta, double* y) { lss_spmv(L, alpha, x, beta, y); }
+template<>
+void spmv_t<float>(Lss& L, float alpha, const float* x, float beta, float* y) { lss_spmv32(L, alpha, x, beta, y); }
+
+struct Scalars {
+  double v[S_COUNT];
+};
+
+void read_scalars(Context& c, Scalars& s)
+{
+  HIP_OK(hipMemcpyAsync(c.scalars_host, c.scalars.ptr, sizeof(double) * S_COUNT, hipMemcpyDeviceToHost, c.stream));
+  HIP_OK(hipStreamSynchronize(c.stream));
+  std::copy(c.scalars_host, c.scalars_host + S_COUNT, s.v);
+}
+
+// BiCGStab on A (values of type R) for A x = b; x is the initial guess and the result.
+// Returns the number of iterations; conv = recursive residual below tol*||b||.
+template<typename R>
+int bicgstab(Lss& L, const R* b, R* x, R* work, double tol, int maxit, bool& conv, int& spmvs)
+{
+  Context& c = *L.ctx;
+  const uint64_t n = L.dofs;
+  R* r = work;
+  R* rh = r + n;
+  R* p = rh + n;
+  R* v = p + n;
+  R* ph = v + n;
+  R* s = ph + n;
+  R* sh = s + n;
+  R* t = sh + n;
+  R* dinv = t + n;
+  double* S = c.scalars.as<double>();
+  double* part = c.red_partials.as<double>();
+  const unsigned G = red_grid(n);
+  const R* vals = sizeof(R) == sizeof(double) ? (const R*)L.values.ptr : (const R*)L.values32.ptr;
+  hipLaunchKernelGGL(k_jacobi<R>, dim3(grid_for(n, 256)), dim3(256), 0, c.stream, L.n_nodes, L.neq,
+                     L.diag_blk.as<uint32_t>(), vals, dinv);
+  spmv_t<R>(L, R(1), x, R(0), r);
+  hipLaunchKernelGGL(k_bicg_init<R>, dim3(G), dim3(kRT), 0, c.stream, n, b, r, rh, p, v, part);
+  hipLaunchKernelGGL(k_finalize<2>, dim3(1), dim3(kRT), 0, c.stream, part, G, S, (int)S_RN2, (int)FIN_STORE);
+  spmvs = 1;
+  Scalars sc;
+  read_scalars(c, sc);
+  double bn2 = sc.v[S_RN2 + 1];
+  if(bn2 == 0.) bn2 = 1.;
+  const double thr2 = tol * tol * bn2;
+  // S_RHO <- rh.r = ||r||^2 initially
+  hipLaunchKernelGGL(k_set_scalars, dim3(1), dim3(1), 0, c.stream, S, thr2);
+  hipLaunchKernelGGL(k_finalize<1>, dim3(1), dim3(kRT), 0, c.stream, part, G, S, (int)S_RHO, (int)FIN_STORE);
+  HIP_OK(hipGetLastError());
+  conv = sc.v[S_RN2] <= thr2;
+  int it = 0;
+  while(!conv && it < maxit) {
+    ++it;
+    hipLaunchKernelGGL(k_bicg_p<R>, dim3(G), dim3(kRT), 0, c.stream, n, S, r, p, v, ph, dinv);
+    spmv_t<R>(L, R(1), ph, R(0), v);
+    hipLaunchKernelGGL(k_dot2<R>, dim3(G), dim3(kRT), 0, c.stream, n, rh, v, (const R*)nullptr, (const R*)nullptr, part);
+    hipLaunchKernelGGL(k_finalize<1>, dim3(1), dim3(kRT), 0, c.stream, part, G, S, 0, (int)FIN_ALPHA);
+    hipLaunchKernelGGL(k_bicg_s<R>, dim3(G), dim3(kRT), 0, c.stream, n, S, r, v, s, sh, dinv, part);
+    hipLaunchKernelGGL(k_finalize<1>, dim3(1), dim3(kRT), 0, c.stream, part, G, S, 0, (int)FIN_SNORM);
+    spmv_t<R>(L, R(1), sh, R(0), t);
+    hipLaunchKernelGGL(k_dot2<R>, dim3(G), dim3(kRT), 0, c.stream, n, t, s, t, t, part);
+    hipLaunchKernelGGL(k_finalize<2>, dim3(1), dim3(kRT), 0, c.stream, part, G, S, 0, (int)FIN_OMEGA);
+    hipLaunchKernelGGL(k_bicg_x<R>, dim3(G), dim3(kRT), 0, c.stream, n, S, x, ph, sh, r, s, t, rh, part);
+    hipLaunchKernelGGL(k_finalize<2>, dim3(1), dim3(kRT), 0, c.stream, part, G, S, 0, (int)FIN_RNORM);
+    HIP_OK(hipGetLastError());
+    spmvs += 2;
+    read_scalars(c, sc);
+    if(sc.v[S_SCONV] != 0. || sc.v[S_RN2] <= thr2) conv = true;
+    if(!std::isfinite(sc.v[S_RN2]) || sc.v[S_RHO] == 0. || sc.v[S_OMEGA] == 0.) {
+      if(!conv) break;
+    }
+  }
+  return it;
+}
+
+double norm2(Context& c, const double* a, uint64_t n)
+{
+  double* part = c.red_partials.as<double>();
+  const unsigned G = red_grid(n);
+  hipLaunchKernelGGL(k_dot2<double>, dim3(G), dim3(kRT), 0, c.stream, n, a, a, (const double*)nullptr,
+                     (const double*)nullptr, part);
+  hipLaunchKernelGGL(k_finalize<1>, dim3(1), dim3(kRT), 0, c.stream, part, G, c.scalars.as<double>(), (int)S_TMP0,
+                     (int)FIN_STORE);
+  HIP_OK(hipGetLastError());
+  Scalars sc;
+  read_scalars(c, sc);
+  return std::sqrt(sc.v[S_TMP0]);
+}
+
+void ensure_work(Lss& L, size_t nvec, int real_bytes)
+{
+  const size_t need = nvec * L.dofs * real_bytes;
+  if(L.work.bytes < need) L.work.alloc(L.ctx, need);
+}
+
+}  // namespace
+
+void convert_f64_to_f32(Context& c, const double* a, float* b, size_t n)
+{
+  hipLaunchKernelGGL((k_convert_scaled<double, float>), dim3(std::min<unsigned>(grid_for(n, 256), 8192)), dim3(256), 0,
+                     c.stream, (uint64_t)n, a, b, 1.0);
+  HIP_OK(hipGetLastError());
+}
+
+double lss_residual(Lss& L)
+{
+  Context& c = *L.ctx;
+  ensure_work(L, 1, sizeof(double));
+  double* r = L.work.as<double>();
+  HIP_OK(hipMemcpyAsync(r, L.rhs.ptr, sizeof(double) * L.dofs, hipMemcpyDeviceToDevice, c.stream));
+  lss_spmv(L, -1., L.sol.as<double>(), 1., r);
+  const double rn = norm2(c, r, L.dofs);
+  double bn = norm2(c, L.rhs.as<double>(), L.dofs);
+  if(bn == 0.) bn = 1.;
+  return rn / bn;
+}
+
+void lss_solve(Lss& L, const cf3hip_solver_params& prm, cf3hip_solve_stats& st)
+{
+  Context& c = *L.ctx;
+  st = cf3hip_solve_stats{0, 0, 0, 0, 0.};
+  if(prm.method == CF3HIP_GMRES_ILU0) {
+    extern void gmres_ilu0(Lss&, const cf3hip_solver_params&, cf3hip_solve_stats&);
+    gmres_ilu0(L, prm, st);
+    return;
+  }
+  if(prm.method != CF3HIP_BICGSTAB_JACOBI) CF3_THROW(CF3HIP_ERR_INVALID_ARG, "unknown Krylov method");
+  if(prm.precision == CF3HIP_FP64) {
+    ensure_work(L, 9, sizeof(double));
+    bool conv = false;
+    int spmvs = 0;
+    st.iterations = bicgstab<double>(L, L.rhs.as<double>(), L.sol.as<double>(), L.work.as<double>(), prm.tol,
+                                     prm.max_iter, conv, spmvs);
+    st.outer = 1;
+    st.spmvs = spmvs;
+    st.rel_residual = lss_residual(L);
+    st.converged = conv ? 1 : 0;
+    return;
+  }
+  // FP32 inner BiCGStab + FP64 iterative refinement: r = b - A x (FP64), A d = r/||r|| (FP32), x += ||r|| d
+  if(!L.has_fp32) CF3_THROW(CF3HIP_ERR_NOT_CREATED, "FP32 values not enabled");
+  const uint64_t n = L.dofs;
+  ensure_work(L, 12, sizeof(double));
+  double* r64 = L.work.as<double>();
+  float* r32 = reinterpret_cast<float*>(r64 + n);
+  float* d32 = r32 + n;
+  float* w32 = d32 + n;
+  double bn = norm2(c, L.rhs.as<double>(), n);
+  if(bn == 0.) bn = 1.;
+  int outer = 0, iters = 0, spmvs = 0;
+  double rel = 0.;
+  for(;;) {
+    HIP_OK(hipMemcpyAsync(r64, L.rhs.ptr, sizeof(double) * n, hipMemcpyDeviceToDevice, c.stream));
+    lss_spmv(L, -1., L.sol.as<double>(), 1., r64);
+    ++spmvs;
+    const double rn = norm2(c, r64, n);
+    rel = rn / bn;
+    if(rel <= prm.tol || outer >= prm.max_outer || iters >= prm.max_iter) break;
+    ++outer;
+    hipLaunchKernelGGL((k_convert_scaled<double, float>), dim3(std::min<unsigned>(grid_for(n, 256), 8192)), dim3(256), 0,
+                       c.stream, n, r64, r32, 1.0 / rn);
+    lss_fill32(c, d32, n, 0.f);
+    bool conv = false;
+    int sp = 0;
+    iters += bicgstab<float>(L, r32, d32, w32, prm.inner_tol, prm.max_iter - iters, conv, sp);
+    spmvs += sp;
+    hipLaunchKernelGGL(k_axpy_f32, dim3(std::min<unsigned>(grid_for(n, 256), 8192)), dim3(256), 0, c.stream, n, rn,
+                       d32, L.sol.as<double>());
+    HIP_OK(hipGetLastError());
+  }
+  st.iterations = iters;
+  st.outer = outer;
+  st.spmvs = spmvs;
+  st.rel_residual = rel;
+  st.converged = rel <= prm.tol ? 1 : 0;
+}
+
+}  // namespace cf3hip

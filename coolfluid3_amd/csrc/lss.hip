@@ -1,0 +1,316 @@
+// LSS matrix/vector kernels on the block-CSR system (gfx950): fill, SpMV (FP64 / FP32),
+// symmetric Dirichlet, point-CSR export.
+//
+// Matrix layout: node row i holds blocks row_ptr[i] .. row_ptr[i+1]-1 (columns sorted), each block
+// neq x neq row-major; all values of a node row are contiguous, so a node row streams as one
+// coalesced segment.  Semantics: cf3/math/LSS/Trilinos/TrilinosFEVbrMatrix.cpp (reset :661-665,
+// symmetric_dirichlet :482-553) and TrilinosDetail.cpp:380-411 (apply y = alpha A x + beta y).
+#include <algorithm>
+#include <unordered_map>
+
+#include "internal.hpp"
+
+namespace cf3hip {
+
+namespace {
+
+template<typename T>
+__global__ void k_fill(T* __restrict__ p, uint64_t n, T v)
+{
+  const uint64_t i = (uint64_t)blockIdx.x * blockDim.x + threadIdx.x;
+  const uint64_t stride = (uint64_t)gridDim.x * blockDim.x;
+  for(uint64_t j = i; j < n; j += stride) p[j] = v;
+}
+
+// FP64, neq = 4: 16 lanes per node row, each lane streams 2 consecutive doubles (16 B) of a block
+// row: lanes 0-7 take block 2k, lanes 8-15 block 2k+1.  Lane l: alpha = (l & 7) >> 1, beta pair
+// 2*(l & 1).  Partial sums are reduced across the beta pair and the block parity.
+__global__ __launch_bounds__(256) void k_spmv4_f64(uint32_t N, const uint32_t* __restrict__ row_ptr,
+                                                   const uint32_t* __restrict__ cols, const double* __restrict__ vals,
+                                                   double alpha, const double* __restrict__ x, double beta,
+                                                   double* __restrict__ y)
+{
+  const uint32_t row = (blockIdx.x * blockDim.x + threadIdx.x) >> 4;
+  const int l = threadIdx.x & 15;
+  if(row >= N) return;
+  const uint32_t q0 = row_ptr[row], q1 = row_ptr[row + 1];
+  const int sub = l >> 3, li = l & 7;
+  double acc = 0.;
+  for(uint32_t q = q0 + sub; q < q1; q += 2) {
+    const double2 v = *reinterpret_cast<const double2*>(vals + (uint64_t)q * 16 + 2 * li);
+    const double2 xv = *reinterpret_cast<const double2*>(x + (uint64_t)cols[q] * 4 + 2 * (li & 1));
+    acc += v.x * xv.x + v.y * xv.y;
+  }
+  acc += __shfl_xor(acc, 1, 16);
+  acc += __shfl_xor(acc, 8, 16);
+  if(l < 8 && (l & 1) == 0) {
+    const uint64_t o = (uint64_t)row * 4 + (li >> 1);
+    double r = alpha * acc;
+    if(beta != 0.) r += beta * y[o];
+    y[o] = r;
+  }
+}
+
+// FP32, neq = 4: 16 lanes per node row, a lane streams one block row (4 floats, 16 B) of blocks
+// 4k + (l >> 2); alpha = l & 3.  Partial sums reduced across the 4 blocks of an iteration.
+__global__ __launch_bounds__(256) void k_spmv4_f32(uint32_t N, const uint32_t* __restrict__ row_ptr,
+                                                   const uint32_t* __restrict__ cols, const float* __restrict__ vals,
+                                                   float alpha, const float* __restrict__ x, float beta,
+                                                   float* __restrict__ y)
+{
+  const uint32_t row = (blockIdx.x * blockDim.x + threadIdx.x) >> 4;
+  const int l = threadIdx.x & 15;
+  if(row >= N) return;
+  const uint32_t q0 = row_ptr[row], q1 = row_ptr[row + 1];
+  const int sub = l >> 2, a = l & 3;
+  float acc = 0.f;
+  for(uint32_t q = q0 + sub; q < q1; q += 4) {
+    const float4 v = *reinterpret_cast<const float4*>(vals + (uint64_t)q * 16 + 4 * a);
+    const float4 xv = *reinterpret_cast<const float4*>(x + (uint64_t)cols[q] * 4);
+    acc += v.x * xv.x + v.y * xv.y + v.z * xv.z + v.w * xv.w;
+  }
+  acc += __shfl_xor(acc, 4, 16);
+  acc += __shfl_xor(acc, 8, 16);
+  if(l < 4) {
+    const uint64_t o = (uint64_t)row * 4 + a;
+    float r = alpha * acc;
+    if(beta != 0.f) r += beta * y[o];
+    y[o] = r;
+  }
+}
+
+// any neq: one thread per scalar row
+template<typename T>
+__global__ void k_spmv_generic(uint32_t N, int neq, const uint32_t* __restrict__ row_ptr,
+                               const uint32_t* __restrict__ cols, const T* __restrict__ vals, T alpha,
+                               const T* __restrict__ x, T beta, T* __restrict__ y)
+{
+  const uint64_t r = (uint64_t)blockIdx.x * blockDim.x + threadIdx.x;
+  if(r >= (uint64_t)N * neq) return;
+  const uint32_t i = (uint32_t)(r / neq);
+  const int a = (int)(r % neq);
+  const int nn = neq * neq;
+  T acc = T(0);
+  for(uint32_t q = row_ptr[i]; q < row_ptr[i + 1]; ++q) {
+    const T* blk = vals + (uint64_t)q * nn + a * neq;
+    const T* xb = x + (uint64_t)cols[q] * neq;
+    for(int b = 0; b < neq; ++b) acc += blk[b] * xb[b];
+  }
+  T res = alpha * acc;
+  if(beta != T(0)) res += beta * y[r];
+  y[r] = res;
+}
+
+// symmetric Dirichlet, column part: every node row m eliminates the columns of BC dofs
+// (rhs[m, j] -= A[(m, j), (n, e)] * v_first(n, e); A[(m, j), (n, e)] = 0)
+template<typename T>
+__global__ void k_dirichlet_columns(uint32_t N, int neq, const uint32_t* __restrict__ row_ptr,
+                                    const uint32_t* __restrict__ cols, T* __restrict__ vals,
+                                    const uint8_t* __restrict__ mask, const double* __restrict__ vfirst,
+                                    double* __restrict__ rhs)
+{
+  const uint32_t m = blockIdx.x * blockDim.x + threadIdx.x;
+  if(m >= N) return;
+  const int nn = neq * neq;
+  double r[4];
+  bool touched = false;
+  for(int j = 0; j < neq; ++j) r[j] = 0.;
+  for(uint32_t q = row_ptr[m]; q < row_ptr[m + 1]; ++q) {
+    const uint32_t n = cols[q];
+    const uint8_t mk = mask[n];
+    if(!mk) continue;
+    if(!touched) { for(int j = 0; j < neq; ++j) r[j] = rhs ? rhs[(uint64_t)m * neq + j] : 0.; touched = true; }
+    T* blk = vals + (uint64_t)q * nn;
+    for(int e = 0; e < neq; ++e) {
+      if(!((mk >> e) & 1)) continue;
+      const double v = vfirst[(uint64_t)n * neq + e];
+      for(int j = 0; j < neq; ++j) {
+        r[j] += -(double)blk[j * neq + e] * v;
+        blk[j * neq + e] = T(0);
+      }
+    }
+  }
+  if(touched && rhs)
+    for(int j = 0; j < neq; ++j) rhs[(uint64_t)m * neq + j] = r[j];
+}
+
+// symmetric Dirichlet, row part: BC row -> identity; rhs = sol = last value
+template<typename T>
+__global__ void k_dirichlet_rows(uint32_t nbc, int neq, const uint32_t* __restrict__ dof, const double* __restrict__ vlast,
+                                 const uint32_t* __restrict__ row_ptr, const uint32_t* __restrict__ cols,
+                                 T* __restrict__ vals, double* __restrict__ rhs, double* __restrict__ sol)
+{
+  const uint32_t b = blockIdx.x * blockDim.x + threadIdx.x;
+  if(b >= nbc) return;
+  const uint32_t d = dof[b];
+  const uint32_t n = d / neq;
+  const int e = d % neq;
+  const int nn = neq * neq;
+  for(uint32_t q = row_ptr[n]; q < row_ptr[n + 1]; ++q) {
+    T* blk = vals + (uint64_t)q * nn;
+    for(int j = 0; j < neq; ++j) blk[e * neq + j] = T(0);
+    if(cols[q] == n) blk[e * neq + e] = T(1);
+  }
+  if(rhs) rhs[d] = vlast[b];
+  if(sol) sol[d] = vlast[b];
+}
+
+__global__ void k_scatter_mask(uint32_t nnodes_bc, const uint32_t* __restrict__ node, const uint8_t* __restrict__ m,
+                               uint8_t* __restrict__ mask)
+{
+  const uint32_t i = blockIdx.x * blockDim.x + threadIdx.x;
+  if(i < nnodes_bc) mask[node[i]] = m[i];
+}
+
+__global__ void k_scatter_values(uint32_t n, const uint32_t* __restrict__ dof, const double* __restrict__ v,
+                                 double* __restrict__ dst)
+{
+  const uint32_t i = blockIdx.x * blockDim.x + threadIdx.x;
+  if(i < n) dst[dof[i]] = v[i];
+}
+
+__global__ void k_point_csr(uint32_t N, int neq, const uint32_t* __restrict__ row_ptr, const uint32_t* __restrict__ cols,
+                            const double* __restrict__ vals, uint32_t* __restrict__ prow, uint32_t* __restrict__ pcol,
+                            double* __restrict__ pval)
+{
+  const uint32_t i = blockIdx.x * blockDim.x + threadIdx.x;
+  if(i >= N) return;
+  const uint32_t q0 = row_ptr[i], deg = row_ptr[i + 1] - q0;
+  const int nn = neq * neq;
+  for(int a = 0; a < neq; ++a) {
+    const uint64_t base = (uint64_t)nn * q0 + (uint64_t)a * neq * deg;
+    prow[(uint64_t)i * neq + a] = (uint32_t)base;
+    for(uint32_t s = 0; s < deg; ++s)
+      for(int b = 0; b < neq; ++b) {
+        pcol[base + s * neq + b] = cols[q0 + s] * neq + b;
+        pval[base + s * neq + b] = vals[(uint64_t)(q0 + s) * nn + a * neq + b];
+      }
+  }
+  if(i == N - 1) prow[(uint64_t)N * neq] = (uint32_t)((uint64_t)nn * row_ptr[N]);
+}
+
+}  // namespace
+
+void lss_fill(Context& c, double* p, size_t n, double v)
+{
+  if(!n) return;
+  hipLaunchKernelGGL(k_fill<double>, dim3(std::min<unsigned>(grid_for(n, 256), 4096)), dim3(256), 0, c.stream, p,
+                     (uint64_t)n, v);
+  HIP_OK(hipGetLastError());
+}
+
+void lss_fill32(Context& c, float* p, size_t n, float v)
+{
+  if(!n) return;
+  hipLaunchKernelGGL(k_fill<float>, dim3(std::min<unsigned>(grid_for(n, 256), 4096)), dim3(256), 0, c.stream, p,
+                     (uint64_t)n, v);
+  HIP_OK(hipGetLastError());
+}
+
+void lss_spmv(Lss& L, double alpha, const double* x, double beta, double* y)
+{
+  Context& c = *L.ctx;
+  if(L.neq == 4) {
+    hipLaunchKernelGGL(k_spmv4_f64, dim3(grid_for((uint64_t)L.n_nodes * 16, 256)), dim3(256), 0, c.stream, L.n_nodes,
+                       L.row_ptr.as<uint32_t>(), L.cols.as<uint32_t>(), L.values.as<double>(), alpha, x, beta, y);
+  } else {
+    hipLaunchKernelGGL(k_spmv_generic<double>, dim3(grid_for(L.dofs, 256)), dim3(256), 0, c.stream, L.n_nodes, L.neq,
+                       L.row_ptr.as<uint32_t>(), L.cols.as<uint32_t>(), L.values.as<double>(), alpha, x, beta, y);
+  }
+  HIP_OK(hipGetLastError());
+}
+
+void lss_spmv32(Lss& L, float alpha, const float* x, float beta, float* y)
+{
+  Context& c = *L.ctx;
+  if(!L.has_fp32) CF3_THROW(CF3HIP_ERR_NOT_CREATED, "FP32 values not enabled");
+  if(L.neq == 4) {
+    hipLaunchKernelGGL(k_spmv4_f32, dim3(grid_for((uint64_t)L.n_nodes * 16, 256)), dim3(256), 0, c.stream, L.n_nodes,
+                       L.row_ptr.as<uint32_t>(), L.cols.as<uint32_t>(), L.values32.as<float>(), alpha, x, beta, y);
+  } else {
+    hipLaunchKernelGGL(k_spmv_generic<float>, dim3(grid_for(L.dofs, 256)), dim3(256), 0, c.stream, L.n_nodes, L.neq,
+                       L.row_ptr.as<uint32_t>(), L.cols.as<uint32_t>(), L.values32.as<float>(), alpha, x, beta, y);
+  }
+  HIP_OK(hipGetLastError());
+}
+
+void lss_dirichlet(Lss& L, uint32_t nbc, const uint32_t* nodes, const uint32_t* eqs, const double* values)
+{
+  if(!nbc) return;
+  Context& c = *L.ctx;
+  const int neq = L.neq;
+  // host bookkeeping of the ordered list: first value (column elimination), last value (row)
+  std::unordered_map<uint64_t, std::pair<double, double> > dofv;
+  std::vector<uint64_t> order;
+  order.reserve(nbc);
+  for(uint32_t b = 0; b < nbc; ++b) {
+    if(nodes[b] >= L.n_nodes || eqs[b] >= (uint32_t)neq)
+      CF3_THROW(CF3HIP_ERR_INVALID_ARG, "Dirichlet condition on a dof outside the system");
+    const uint64_t d = (uint64_t)nodes[b] * neq + eqs[b];
+    auto it = dofv.find(d);
+    if(it == dofv.end()) { dofv.emplace(d, std::make_pair(values[b], values[b])); order.push_back(d); }
+    else it->second.second = values[b];
+  }
+  const uint32_t nd = (uint32_t)order.size();
+  std::vector<uint32_t> dof(nd);
+  std::vector<double> vfirst(nd), vlast(nd);
+  std::unordered_map<uint32_t, uint8_t> nmask;
+  for(uint32_t i = 0; i < nd; ++i) {
+    dof[i] = (uint32_t)order[i];
+    vfirst[i] = dofv[order[i]].first;
+    vlast[i] = dofv[order[i]].second;
+    nmask[dof[i] / neq] |= (uint8_t)(1u << (dof[i] % neq));
+  }
+  std::vector<uint32_t> mnode;
+  std::vector<uint8_t> mval;
+  mnode.reserve(nmask.size());
+  for(auto& kv : nmask) { mnode.push_back(kv.first); mval.push_back(kv.second); }
+
+  DevBuf d_dof, d_vf, d_vl, d_mnode, d_mval, d_mask, d_vfirst;
+  d_dof.alloc(&c, sizeof(uint32_t) * nd);
+  d_vf.alloc(&c, sizeof(double) * nd);
+  d_vl.alloc(&c, sizeof(double) * nd);
+  d_mnode.alloc(&c, sizeof(uint32_t) * mnode.size());
+  d_mval.alloc(&c, mval.size());
+  d_mask.alloc(&c, L.n_nodes);
+  d_vfirst.alloc(&c, sizeof(double) * L.dofs);
+  HIP_OK(hipMemcpyAsync(d_dof.ptr, dof.data(), sizeof(uint32_t) * nd, hipMemcpyHostToDevice, c.stream));
+  HIP_OK(hipMemcpyAsync(d_vf.ptr, vfirst.data(), sizeof(double) * nd, hipMemcpyHostToDevice, c.stream));
+  HIP_OK(hipMemcpyAsync(d_vl.ptr, vlast.data(), sizeof(double) * nd, hipMemcpyHostToDevice, c.stream));
+  HIP_OK(hipMemcpyAsync(d_mnode.ptr, mnode.data(), sizeof(uint32_t) * mnode.size(), hipMemcpyHostToDevice, c.stream));
+  HIP_OK(hipMemcpyAsync(d_mval.ptr, mval.data(), mval.size(), hipMemcpyHostToDevice, c.stream));
+  hipLaunchKernelGGL(k_fill<uint8_t>, dim3(std::min<unsigned>(grid_for(L.n_nodes, 256), 4096)), dim3(256), 0, c.stream,
+                     d_mask.as<uint8_t>(), (uint64_t)L.n_nodes, (uint8_t)0);
+  hipLaunchKernelGGL(k_scatter_mask, dim3(grid_for(mnode.size(), 256)), dim3(256), 0, c.stream, (uint32_t)mnode.size(),
+                     d_mnode.as<uint32_t>(), d_mval.as<uint8_t>(), d_mask.as<uint8_t>());
+  hipLaunchKernelGGL(k_scatter_values, dim3(grid_for(nd, 256)), dim3(256), 0, c.stream, nd, d_dof.as<uint32_t>(),
+                     d_vf.as<double>(), d_vfirst.as<double>());
+  hipLaunchKernelGGL(k_dirichlet_columns<double>, dim3(grid_for(L.n_nodes, 256)), dim3(256), 0, c.stream, L.n_nodes, neq,
+                     L.row_ptr.as<uint32_t>(), L.cols.as<uint32_t>(), L.values.as<double>(), d_mask.as<uint8_t>(),
+                     d_vfirst.as<double>(), L.rhs.as<double>());
+  hipLaunchKernelGGL(k_dirichlet_rows<double>, dim3(grid_for(nd, 256)), dim3(256), 0, c.stream, nd, neq, d_dof.as<uint32_t>(),
+                     d_vl.as<double>(), L.row_ptr.as<uint32_t>(), L.cols.as<uint32_t>(), L.values.as<double>(),
+                     L.rhs.as<double>(), L.sol.as<double>());
+  HIP_OK(hipGetLastError());
+  if(L.has_fp32) {
+    // the FP32 mirror receives the same elimination (its right-hand side is the FP64 one)
+    hipLaunchKernelGGL(k_dirichlet_columns<float>, dim3(grid_for(L.n_nodes, 256)), dim3(256), 0, c.stream, L.n_nodes,
+                       neq, L.row_ptr.as<uint32_t>(), L.cols.as<uint32_t>(), L.values32.as<float>(),
+                       d_mask.as<uint8_t>(), d_vfirst.as<double>(), (double*)nullptr);
+    hipLaunchKernelGGL(k_dirichlet_rows<float>, dim3(grid_for(nd, 256)), dim3(256), 0, c.stream, nd, neq,
+                       d_dof.as<uint32_t>(), d_vl.as<double>(), L.row_ptr.as<uint32_t>(), L.cols.as<uint32_t>(),
+                       L.values32.as<float>(), (double*)nullptr, (double*)nullptr);
+    HIP_OK(hipGetLastError());
+  }
+  HIP_OK(hipStreamSynchronize(c.stream));
+}
+
+void lss_point_csr(Lss& L, uint32_t* prow, uint32_t* pcol, double* pval)
+{
+  Context& c = *L.ctx;
+  hipLaunchKernelGGL(k_point_csr, dim3(grid_for(L.n_nodes, 128)), dim3(128), 0, c.stream, L.n_nodes, L.neq,
+                     L.row_ptr.as<uint32_t>(), L.cols.as<uint32_t>(), L.values.as<double>(), prow, pcol, pval);
+  HIP_OK(hipGetLastError());
+}
+
+}  // namespace cf3hip

@@ -1,0 +1,200 @@
+/*
+ * cf3hip: C ABI of the MI355X (gfx950) implementation of COOLFluiD 3's UFEM incompressible
+ * Navier-Stokes hot path: PSPG/SUPG/bulk-viscosity element assembly, its deterministic scatter into
+ * the LSS block-CSR matrix, symmetric Dirichlet conditions, SpMV and the Krylov solve.
+ *
+ * Plain pointers and sizes only.  Host arrays are marked "host"; arrays marked "device" are device
+ * pointers (for zero-copy interop with callers that keep vectors on the GPU).  Every function
+ * returns CF3HIP_OK (0) or an error code; cf3hip_last_error() gives the message of the last failure
+ * in the calling thread.  All calls are synchronous with respect to the host unless stated.
+ *
+ * Reference interfaces replaced (paths relative to the coolfluid3 source tree) are cited per entry.
+ */
+#ifndef CF3HIP_H
+#define CF3HIP_H
+
+#include <stddef.h>
+#include <stdint.h>
+
+#ifdef __cplusplus
+extern "C" {
+#endif
+
+/* ---- status codes: mapped to the reference exceptions by the C++ host layer ---- */
+#define CF3HIP_OK                 0
+#define CF3HIP_ERR_INVALID_ARG    1  /* common::BadValue                                   */
+#define CF3HIP_ERR_NOT_CREATED    2  /* common::SetupError (LSS not created, bad theta)     */
+#define CF3HIP_ERR_HIP            3  /* HIP runtime failure                                 */
+#define CF3HIP_ERR_OUT_OF_MEMORY  4  /* device allocation failed                            */
+#define CF3HIP_ERR_NOT_CONVERGED  5  /* Krylov solve hit max iterations                     */
+#define CF3HIP_ERR_ILLEGAL_ENTRY  6  /* BadValue "Trying to access an illegal entry"        */
+#define CF3HIP_ERR_NOT_IMPLEMENTED 7 /* common::NotImplemented                              */
+#define CF3HIP_ERR_LIMIT          8  /* mesh exceeds a documented kernel limit              */
+#define CF3HIP_ERR_SINGULAR       9  /* non-invertible element Jacobian (ElementData.hpp:312-313) */
+
+/* element types (cf3/mesh/LagrangeP1: Triag2D, Tetra3D, Hexa3D) */
+#define CF3HIP_TRIAG2D 0
+#define CF3HIP_TETRA3D 1
+#define CF3HIP_HEXA3D  2
+
+/* SUPG tau variants (plugins/UFEM/src/UFEM/SUPG.hpp:62-74) */
+#define CF3HIP_SUPG_TEZDUYAR 0
+#define CF3HIP_SUPG_METRIC   1
+#define CF3HIP_SUPG_CF2      2
+
+/* value precision of an assembled matrix / Krylov solve */
+#define CF3HIP_FP64 0
+#define CF3HIP_FP32 1
+
+typedef struct cf3hip_context_s* cf3hip_context;
+typedef struct cf3hip_mesh_s*    cf3hip_mesh;
+typedef struct cf3hip_lss_s*     cf3hip_lss;
+typedef struct cf3hip_fields_s*  cf3hip_fields;
+
+const char* cf3hip_last_error(void);
+const char* cf3hip_version(void);
+
+/* ---- context: one device + one HIP stream (no reference counterpart: process/device setup,
+ *      the role of common::PE::Comm::init in cf3/common/PE/Comm.cpp:87) ---- */
+int cf3hip_context_create(int device, cf3hip_context* ctx);
+int cf3hip_context_destroy(cf3hip_context ctx);
+/* use a caller-owned hipStream_t (NULL = default stream) for every later launch */
+int cf3hip_context_set_stream(cf3hip_context ctx, void* hip_stream);
+int cf3hip_context_synchronize(cf3hip_context ctx);
+/* bytes currently held by device allocations of this context */
+int cf3hip_context_memory(cf3hip_context ctx, uint64_t* bytes);
+
+/* ---- mesh: one Elements component of one LagrangeP1 type, device SoA coordinates
+ *      (cf3/mesh Dictionary coordinates + Connectivity, cf3/mesh/Field.hpp:34, Connectivity.hpp:23) ---- */
+/* coords: host, node-major n_nodes x dim (cf3 Field table layout); conn: host, n_elems x k */
+int cf3hip_mesh_create(cf3hip_context ctx, int etype, uint32_t n_nodes, uint32_t n_elems,
+                       const double* coords, const uint32_t* conn, cf3hip_mesh* mesh);
+int cf3hip_mesh_destroy(cf3hip_mesh mesh);
+/* replace coordinates (same topology), host node-major */
+int cf3hip_mesh_set_coordinates(cf3hip_mesh mesh, const double* coords);
+
+/* ---- LSS system: matrix + rhs + solution.
+ *      Replaces math::LSS::System::create (cf3/math/LSS/System.cpp:82-110) with the sparsity of
+ *      plugins/UFEM/src/UFEM/SparsityBuilder.cpp:31-192 built on the device.  The matrix is block
+ *      CSR with neq x neq row-major blocks, node rows contiguous (semantics of
+ *      TrilinosFEVbrMatrix, cf3/math/LSS/Trilinos/TrilinosFEVbrMatrix.cpp:71-134). ---- */
+int cf3hip_lss_create(cf3hip_context ctx, cf3hip_mesh mesh, int neq, cf3hip_lss* lss);
+/* System::create(cp, neq, node_connectivity, starting_indices) for a caller-given node graph
+ * (rows must be sorted and contain the diagonal).  No element scatter map: assembly needs lss_create. */
+int cf3hip_lss_create_from_graph(cf3hip_context ctx, uint32_t n_nodes, int neq,
+                                 const uint32_t* starting_indices, const uint32_t* node_connectivity,
+                                 cf3hip_lss* lss);
+int cf3hip_lss_destroy(cf3hip_lss lss);                  /* System::destroy (System.cpp:192-199) */
+int cf3hip_lss_info(cf3hip_lss lss, uint32_t* n_nodes, int* neq, uint64_t* n_blocks);
+/* node graph (starting_indices n_nodes+1, node_connectivity n_blocks), host */
+int cf3hip_lss_get_graph(cf3hip_lss lss, uint32_t* starting_indices, uint32_t* node_connectivity);
+/* element-to-slot map: block slot of element pair (a, b), host, n_elems*k*k */
+int cf3hip_lss_get_slots(cf3hip_lss lss, uint32_t* slots);
+/* enable an FP32 mirror of the values (precision form); the FP32 assembly writes it */
+int cf3hip_lss_enable_fp32(cf3hip_lss lss);
+
+/* ---- Matrix interface (cf3/math/LSS/Matrix.hpp:35-180) ---- */
+int cf3hip_matrix_reset(cf3hip_lss lss, double value);                          /* Matrix::reset */
+int cf3hip_matrix_set_value(cf3hip_lss lss, uint32_t icol, uint32_t irow, double v);  /* column first */
+int cf3hip_matrix_add_value(cf3hip_lss lss, uint32_t icol, uint32_t irow, double v);
+int cf3hip_matrix_get_value(cf3hip_lss lss, uint32_t icol, uint32_t irow, double* v);
+int cf3hip_matrix_set_row(cf3hip_lss lss, uint32_t blockrow, uint32_t ieq, double diag, double offdiag);
+/* Matrix::add_values(BlockAccumulator): indices (host, nb node ids), mat (host, row-major
+ * (nb*neq)^2, node-major rows) -- BlockAccumulator.hpp:28-74, TrilinosFEVbrMatrix.cpp:315-355 */
+int cf3hip_matrix_add_values(cf3hip_lss lss, uint32_t nb, const uint32_t* indices, const double* mat);
+int cf3hip_matrix_get_diagonal(cf3hip_lss lss, double* diag);                  /* host, dofs */
+/* y = alpha*A*x + beta*y (Matrix::apply, TrilinosDetail.cpp:380-411); device pointers, dofs each */
+int cf3hip_matrix_apply(cf3hip_lss lss, double alpha, const double* x_dev, double beta, double* y_dev);
+/* FP32 mirror apply, float device vectors */
+int cf3hip_matrix_apply_fp32(cf3hip_lss lss, float alpha, const float* x_dev, float beta, float* y_dev);
+/* raw block values (host, n_blocks*neq*neq) and point-CSR view (Matrix::debug_data) */
+int cf3hip_matrix_get_values(cf3hip_lss lss, double* values);
+int cf3hip_matrix_get_values_fp32(cf3hip_lss lss, float* values);
+int cf3hip_matrix_debug_data(cf3hip_lss lss, uint32_t* row_ptr, uint32_t* col_idx, double* vals);
+
+/* ---- Vector interface (cf3/math/LSS/Vector.hpp:36-173) for the system rhs / solution ---- */
+#define CF3HIP_RHS      0
+#define CF3HIP_SOLUTION 1
+int cf3hip_vector_reset(cf3hip_lss lss, int which, double value);
+int cf3hip_vector_get(cf3hip_lss lss, int which, double* host);
+int cf3hip_vector_set(cf3hip_lss lss, int which, const double* host);
+int cf3hip_vector_set_value(cf3hip_lss lss, int which, uint32_t blockrow, uint32_t ieq, double v);
+int cf3hip_vector_get_value(cf3hip_lss lss, int which, uint32_t blockrow, uint32_t ieq, double* v);
+/* device pointer of the rhs / solution storage (FP64, dofs entries), owned by the system */
+int cf3hip_vector_device_ptr(cf3hip_lss lss, int which, double** dev);
+
+/* ---- System::dirichlet with preserve_symmetry (System.cpp:240-255 ->
+ *      TrilinosFEVbrMatrix::symmetric_dirichlet, TrilinosFEVbrMatrix.cpp:482-553), applied to a list
+ *      in order: later conditions on a dof set its rhs/solution value, the column elimination
+ *      uses the first value (the matrix column is already zero afterwards).  Host arrays. ---- */
+int cf3hip_lss_dirichlet(cf3hip_lss lss, uint32_t nbc, const uint32_t* nodes, const uint32_t* eqs,
+                         const double* values);
+
+/* ---- SolutionStrategy (cf3/math/LSS/SolutionStrategy.hpp:35-59; Trilinos strategy
+ *      TrilinosStratimikosStrategy.cpp:135-207) ---- */
+#define CF3HIP_BICGSTAB_JACOBI 0
+#define CF3HIP_GMRES_ILU0      1
+typedef struct cf3hip_solver_params {
+  int method;        /* CF3HIP_BICGSTAB_JACOBI or CF3HIP_GMRES_ILU0 */
+  int restart;       /* GMRES restart length */
+  int max_iter;
+  double tol;        /* stop on ||b - A x||_2 <= tol ||b||_2 */
+  int precision;     /* CF3HIP_FP64, or CF3HIP_FP32 = FP32 inner Krylov + FP64 iterative refinement */
+  double inner_tol;  /* FP32 inner relative tolerance per refinement sweep */
+  int max_outer;     /* refinement sweeps */
+} cf3hip_solver_params;
+typedef struct cf3hip_solve_stats {
+  int iterations;    /* Krylov iterations (all sweeps) */
+  int outer;         /* refinement sweeps (1 for FP64) */
+  int spmvs;
+  int converged;
+  double rel_residual;  /* ||b - A x|| / ||b|| in FP64 at exit */
+} cf3hip_solve_stats;
+/* solve A x = rhs into the solution vector, using it as the initial guess */
+int cf3hip_lss_solve(cf3hip_lss lss, const cf3hip_solver_params* prm, cf3hip_solve_stats* stats);
+/* ||rhs - A sol||_2 / ||rhs||_2 (TrilinosStratimikosStrategy::compute_residual) */
+int cf3hip_lss_residual(cf3hip_lss lss, double* rel);
+
+/* ---- UFEM NavierStokes fields (tags navier_stokes_solution, linearized_velocity,
+ *      navier_stokes_viscosity, body_force; ns_implementation/NavierStokes.cpp:49-68) ---- */
+#define CF3HIP_FIELD_U      0   /* Velocity (dim)            */
+#define CF3HIP_FIELD_P      1   /* Pressure (1)              */
+#define CF3HIP_FIELD_UADV   2   /* AdvectionVelocity (dim)   */
+#define CF3HIP_FIELD_U1     3   /* AdvectionVelocity1 (dim)  */
+#define CF3HIP_FIELD_U2     4
+#define CF3HIP_FIELD_U3     5
+#define CF3HIP_FIELD_NU     6   /* EffectiveViscosity (1)    */
+#define CF3HIP_FIELD_G      7   /* Force (dim), body force   */
+int cf3hip_fields_create(cf3hip_mesh mesh, cf3hip_fields* fields);
+int cf3hip_fields_destroy(cf3hip_fields fields);
+/* host, node-major n_nodes x ncomp */
+int cf3hip_fields_set(cf3hip_fields fields, int which, const double* host);
+int cf3hip_fields_get(cf3hip_fields fields, int which, double* host);
+/* initial conditions of NavierStokes::on_initial_conditions_set (NavierStokes.cpp:194-205):
+ * u_adv = u1 = u2 = u3 = u, nu_eff = nu */
+int cf3hip_fields_initialize(cf3hip_fields fields, double nu);
+
+/* ---- NavierStokes LSS action (plugins/UFEM/src/UFEM/NavierStokes.hpp:34-118) ---- */
+typedef struct cf3hip_ns_params {
+  double rho;               /* NavierStokesPhysics "density" */
+  double dt;                /* LSSActionUnsteady dt (invdt = 1/dt) */
+  double theta;             /* "theta" option, 0 <= theta <= 1 */
+  int use_specializations;  /* "use_specializations" option (tri/tet) */
+  int supg_type;            /* "supg_type" option for the generic expression */
+  double c1, c2;            /* METRIC constants (1, 16) */
+  double alpha_ps, alpha_su, alpha_bu;
+  int precision;            /* CF3HIP_FP64 (drop-in) or CF3HIP_FP32 (element arithmetic and values) */
+} cf3hip_ns_params;
+/* LinearizeU node loop (NavierStokes.cpp:105) */
+int cf3hip_ns_linearize(cf3hip_fields fields);
+/* ZeroLSS (matrix and rhs reset, solution kept; ZeroLSS.cpp:56-85) + Assembly
+ * (NavierStokesAssembly.hpp:28-75 / NavierStokesSpecializations.hpp) + scatter into the system.
+ * precision FP32 writes the FP32 mirror values and the rhs in FP64. */
+int cf3hip_ns_assemble(cf3hip_lss lss, cf3hip_fields fields, const cf3hip_ns_params* prm);
+/* UpdateActions (NavierStokes.cpp:172-180): u3=u2, u2=u1, u1=u, u += du, p += dp from the solution */
+int cf3hip_ns_update(cf3hip_lss lss, cf3hip_fields fields);
+
+#ifdef __cplusplus
+}
+#endif
+#endif

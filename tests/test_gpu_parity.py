@@ -1,0 +1,273 @@
+"""Parity of the HIP path (through the C ABI) with the oracle on the same seeded inputs.
+
+Bars (stated per check):
+  * sparsity (starting_indices, node_connectivity) and element-to-slot map: bit-exact;
+  * FP64 matrix entries: |gpu - oracle| <= 1e-12 * max|oracle row|, row by row;
+  * FP64 rhs: |gpu - oracle| <= 1e-12 * max|oracle rhs|;
+  * FP32 assembly: |gpu - oracle| <= 2e-6 * max|oracle row| (single precision, stated);
+  * solutions: ||b - A x||_2 <= 1e-10 ||b||_2 evaluated with the oracle's FP64 matrix.
+"""
+import numpy as np
+import pytest
+
+from coolfluid3_amd import cabi, cases
+from oracle import pyoracle
+
+pytestmark = pytest.mark.gpu
+
+
+@pytest.fixture(scope="module")
+def ctx():
+    c = cabi.Context(0)
+    yield c
+    c.close()
+
+
+def row_rel_err(gpu, ref, neq, start):
+    nn = neq * neq
+    g = gpu.reshape(-1, nn)
+    r = ref.reshape(-1, nn)
+    rows = np.repeat(np.arange(len(start) - 1), np.diff(start).astype(np.int64))
+    worst = 0.0
+    d = np.abs(g - r).reshape(-1, neq, neq)
+    a = np.abs(r).reshape(-1, neq, neq)
+    for al in range(neq):
+        rowmax = np.zeros(len(start) - 1)
+        np.maximum.at(rowmax, rows, a[:, al, :].max(axis=1))
+        err = np.zeros(len(start) - 1)
+        np.maximum.at(err, rows, d[:, al, :].max(axis=1))
+        worst = max(worst, float((err / np.maximum(rowmax, 1e-300)).max()))
+    return worst
+
+
+def oracle_system(o, case, spec=True, supg=pyoracle.SUPG_METRIC, theta=1.0):
+    start, cols, slots = o.build_sparsity(case.conn, case.n_nodes)
+    prm = pyoracle.ns_params(rho=case.rho, dt=case.dt, theta=theta, use_specializations=spec, supg_type=supg)
+    ua = o.linearize_u(case.u0, case.u0, case.u0, case.u0)
+    et = {cabi.TRIAG2D: pyoracle.TRIAG2D, cabi.TETRA3D: pyoracle.TETRA3D}[case.etype]
+    vals, rhs = o.ns_assemble(et, prm, case.coords, case.conn, case.u0, case.p0, ua,
+                              np.full(case.n_nodes, case.nu), start, cols, slots)
+    return start, cols, slots, vals, rhs
+
+
+def gpu_case(ctx, case, spec=True, supg=cabi.SUPG_METRIC, theta=1.0, fp32=False):
+    g = cases.GpuCase(ctx, case, precision=cabi.FP32 if fp32 else cabi.FP64, use_specializations=spec,
+                      supg_type=supg)
+    g.prm.theta = theta
+    g.fields.linearize()
+    return g
+
+
+CASES = {
+    "tri8": lambda: cases.lid_cavity_2d(8),
+    "tet5": lambda: cases.inflow_cube_tets(5),
+    "tet12": lambda: cases.inflow_cube_tets(12),
+}
+
+
+@pytest.mark.parametrize("name", ["tri8", "tet5", "tet12"])
+def test_sparsity_and_slot_map_bit_exact(ctx, oracle, name):
+    case = CASES[name]()
+    print("case", name, flush=True)
+    g = gpu_case(ctx, case)
+    start, cols, slots = oracle.build_sparsity(case.conn, case.n_nodes)
+    s, c = g.system.graph()
+    assert np.array_equal(s, start) and np.array_equal(c, cols)
+    assert np.array_equal(g.system.slots(), slots)
+    g.close()
+
+
+@pytest.mark.parametrize("name,spec,supg,theta", [
+    ("tri8", True, cabi.SUPG_METRIC, 1.0), ("tri8", False, cabi.SUPG_METRIC, 0.5),
+    ("tet5", True, cabi.SUPG_METRIC, 1.0), ("tet5", False, cabi.SUPG_METRIC, 0.7),
+    ("tet5", False, cabi.SUPG_TEZDUYAR, 1.0), ("tet12", True, cabi.SUPG_METRIC, 1.0)])
+def test_assembly_fp64(ctx, oracle, name, spec, supg, theta):
+    case = CASES[name]()
+    print("case", name, spec, supg, theta, flush=True)
+    start, cols, slots, vref, rref = oracle_system(oracle, case, spec, supg, theta)
+    g = gpu_case(ctx, case, spec, supg, theta)
+    g.assemble(cabi.FP64)
+    v = g.system.values()
+    r = g.system.vector(cabi.RHS)
+    neq = case.dim + 1
+    assert row_rel_err(v, vref, neq, start) <= 1e-12
+    assert np.abs(r - rref).max() <= 1e-12 * np.abs(rref).max()
+    # deterministic: a second assembly reproduces every bit
+    g.assemble(cabi.FP64)
+    assert np.array_equal(g.system.values(), v) and np.array_equal(g.system.vector(cabi.RHS), r)
+    g.close()
+
+
+@pytest.mark.parametrize("name", ["tri8", "tet12"])
+def test_assembly_fp32_error_stated(ctx, oracle, name):
+    case = CASES[name]()
+    start, cols, slots, vref, rref = oracle_system(oracle, case)
+    g = gpu_case(ctx, case, fp32=True)
+    g.assemble(cabi.FP32)
+    v32 = g.system.values_fp32().astype(np.float64)
+    err = row_rel_err(v32, vref, case.dim + 1, start)
+    print(f"FP32 assembly row-relative error vs FP64 oracle: {err:.3e}")
+    assert err <= 2e-6
+    g.close()
+
+
+def test_dirichlet_and_spmv(ctx, oracle):
+    case = CASES["tet5"]()
+    start, cols, slots, vref, rref = oracle_system(oracle, case)
+    g = gpu_case(ctx, case)
+    g.assemble(cabi.FP64)
+    n, e, vals = cases.bc_lists(case, case.u0, case.p0)
+    g.system.dirichlet(n, e, vals)
+    sol_ref = np.zeros_like(rref)
+    oracle.dirichlet(4, start, cols, vref, rref, sol_ref, n, e, vals)
+    assert row_rel_err(g.system.values(), vref, 4, start) <= 1e-12
+    r = g.system.vector(cabi.RHS)
+    assert np.abs(r - rref).max() <= 1e-12 * np.abs(rref).max()
+    assert np.array_equal(g.system.vector(cabi.SOLUTION), sol_ref)
+    # SpMV y = 2 A x + 0.5 y against the oracle, using the system's own vectors
+    x = np.random.default_rng(1).uniform(-1, 1, g.system.dofs)
+    y0 = np.random.default_rng(2).uniform(-1, 1, g.system.dofs)
+    g.system.set_vector(cabi.SOLUTION, x)
+    g.system.set_vector(cabi.RHS, y0)
+    g.system.apply_device(2.0, g.system.vector_ptr(cabi.SOLUTION), 0.5, g.system.vector_ptr(cabi.RHS))
+    y = g.system.vector(cabi.RHS)
+    yref = oracle.spmv(4, start, cols, vref, x, alpha=2.0, beta=0.5, y=y0)
+    assert np.abs(y - yref).max() <= 1e-13 * np.abs(yref).max()
+    g.close()
+
+
+def test_point_csr_export(ctx, oracle):
+    case = CASES["tri8"]()
+    start, cols, slots, vref, rref = oracle_system(oracle, case)
+    g = gpu_case(ctx, case)
+    g.assemble(cabi.FP64)
+    rp, ci, v = g.system.debug_data()
+    rp2, ci2, v2 = oracle.point_csr(3, start, cols, g.system.values())
+    assert np.array_equal(rp, rp2) and np.array_equal(ci, ci2) and np.array_equal(v, v2)
+    g.close()
+
+
+def oracle_residual(o, neq, start, cols, vals, b, x):
+    r = b - o.spmv(neq, start, cols, vals, x)
+    return np.linalg.norm(r) / np.linalg.norm(b)
+
+
+def test_bicgstab_solution_parity(ctx, oracle):
+    case = CASES["tet12"]()
+    start, cols, slots, vref, rref = oracle_system(oracle, case)
+    g = gpu_case(ctx, case)
+    g.assemble(cabi.FP64)
+    n, e, vals = cases.bc_lists(case, case.u0, case.p0)
+    g.system.dirichlet(n, e, vals)
+    sol = np.zeros_like(rref)
+    oracle.dirichlet(4, start, cols, vref, rref, sol, n, e, vals)
+    st = g.system.solve(cabi.solver_params(tol=1e-10))
+    x = g.system.vector(cabi.SOLUTION)
+    assert st.converged
+    assert oracle_residual(oracle, 4, start, cols, vref, rref, x) <= 1e-10
+    xo, sto = oracle.bicgstab(4, start, cols, vref, rref, sol, tol=1e-10)
+    print(f"bicgstab gpu {st.iterations} it, oracle {sto.iterations} it")
+    assert abs(st.iterations - sto.iterations) <= max(3, sto.iterations // 10)
+    g.close()
+
+
+def test_gmres_ilu0_lid_cavity(ctx, oracle):
+    case = cases.lid_cavity_2d(32)
+    start, cols, slots, vref, rref = oracle_system(oracle, case)
+    g = gpu_case(ctx, case)
+    g.assemble(cabi.FP64)
+    n, e, vals = cases.bc_lists(case, case.u0, case.p0)
+    g.system.dirichlet(n, e, vals)
+    sol = np.zeros_like(rref)
+    oracle.dirichlet(3, start, cols, vref, rref, sol, n, e, vals)
+    st = g.system.solve(cabi.solver_params(cabi.GMRES_ILU0, tol=1e-10, restart=30))
+    x = g.system.vector(cabi.SOLUTION)
+    assert oracle_residual(oracle, 3, start, cols, vref, rref, x) <= 1e-10
+    xo, sto = oracle.gmres_ilu0(3, start, cols, vref, rref, sol, restart=30, tol=1e-10)
+    print(f"gmres gpu {st.iterations} it, oracle {sto.iterations} it")
+    assert abs(st.iterations - sto.iterations) <= max(2, sto.iterations // 10)
+    g.close()
+
+
+def test_fp32_refinement_reaches_fp64_residual(ctx, oracle):
+    case = CASES["tet12"]()
+    start, cols, slots, vref, rref = oracle_system(oracle, case)
+    g = gpu_case(ctx, case, fp32=True)
+    g.assemble(cabi.FP64)
+    g.assemble(cabi.FP32)
+    n, e, vals = cases.bc_lists(case, case.u0, case.p0)
+    g.system.dirichlet(n, e, vals)
+    sol = np.zeros_like(rref)
+    oracle.dirichlet(4, start, cols, vref, rref, sol, n, e, vals)
+    st = g.system.solve(cabi.solver_params(tol=1e-8, precision=cabi.FP32, inner_tol=1e-4))
+    x = g.system.vector(cabi.SOLUTION)
+    res = oracle_residual(oracle, 4, start, cols, vref, rref, x)
+    xo, _ = oracle.bicgstab(4, start, cols, vref, rref, sol, tol=1e-12)
+    err = np.abs(x - xo).max() / np.abs(xo).max()
+    print(f"FP32+IR: {st.outer} sweeps, {st.iterations} inner it, residual {res:.2e}, error vs FP64 {err:.2e}")
+    assert st.converged and res <= 1e-8
+    g.close()
+
+
+def test_time_step_vs_oracle(ctx, oracle):
+    case = CASES["tet5"]()
+    g = gpu_case(ctx, case)
+    st = g.step(cabi.solver_params(tol=1e-12))
+    # oracle step
+    start, cols, slots, vref, rref = oracle_system(oracle, case)
+    n, e, vals = cases.bc_lists(case, case.u0, case.p0)
+    sol = np.zeros_like(rref)
+    oracle.dirichlet(4, start, cols, vref, rref, sol, n, e, vals)
+    x, _ = oracle.bicgstab(4, start, cols, vref, rref, sol, tol=1e-12)
+    u, p = case.u0.copy(), case.p0.copy()
+    u1, u2, u3 = u.copy(), u.copy(), u.copy()
+    oracle.update(u, p, u1, u2, u3, x)
+    assert np.abs(g.u - u).max() <= 1e-9 * np.abs(u).max()
+    assert np.abs(g.p - p).max() <= 1e-9 * max(np.abs(p).max(), 1.0)
+    g.close()
+
+
+def test_reference_lss_fixtures(ctx, fixtures):
+    f = fixtures["symmetric_dirichlet"]
+    s = cabi.System(ctx, neq=1, graph=(f["starting_indices"], f["node_connectivity"], 4))
+    for i in range(4):
+        s.set_row(i, 0, 2.0, 1.0)
+    s.dirichlet([1], [0], [10.0])
+    dense = np.array([[s.get_value(c, r) if abs(r - c) <= 1 else 0.0 for c in range(4)] for r in range(4)])
+    assert np.array_equal(dense, np.array(f["expected_matrix"], float))
+    assert np.array_equal(s.vector(cabi.RHS), np.array(f["expected_rhs"]))
+    with pytest.raises(cabi.Cf3HipError) as ei:
+        s.get_value(3, 0)                    # not in the graph: illegal entry
+    assert ei.value.code == 6
+    s.close()
+    # apply on an all-ones matrix: y = 2 A x + y (utest-lss-vector.cpp:144-187)
+    case = cases.lid_cavity_2d(3)
+    m = cabi.Mesh(ctx, case.etype, case.coords, case.conn)
+    s = cabi.System(ctx, m)
+    s.reset(1.0)
+    x = np.random.default_rng(0).uniform(-1, 1, s.dofs)
+    s.set_vector(cabi.SOLUTION, x)
+    s.reset_vector(cabi.RHS, 1.0)
+    s.apply_device(2.0, s.vector_ptr(cabi.SOLUTION), 1.0, s.vector_ptr(cabi.RHS))
+    start, cols = s.graph()
+    y = s.vector(cabi.RHS)
+    for i in range(s.n_nodes):
+        nb = cols[start[i]:start[i + 1]]
+        expect = 1 + 2 * x.reshape(-1, 3)[nb].sum()
+        assert np.allclose(y[3 * i:3 * i + 3], expect, rtol=1e-14)
+    s.close()
+    m.close()
+
+
+def test_error_paths(ctx):
+    case = CASES["tet5"]()
+    g = gpu_case(ctx, case)
+    prm = cabi.ns_params(theta=1.5)
+    with pytest.raises(cabi.Cf3HipError) as ei:
+        g.system.assemble(g.fields, prm)
+    assert ei.value.code == 2                     # SetupError: theta outside [0, 1]
+    prm = cabi.ns_params(precision=cabi.FP32)
+    with pytest.raises(cabi.Cf3HipError) as ei:
+        g.system.assemble(g.fields, prm)         # FP32 mirror not enabled
+    assert ei.value.code == 2
+    g.close()
