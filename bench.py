@@ -1,0 +1,184 @@
+"""Benchmark of the UFEM Navier-Stokes hot path on one MI355X.
+
+A step is one Assembly action of the reference (ZeroLSS + element assembly + scatter into the
+block-CSR system, the quantity timed by plugins/UFEM/test/ptest-navier-stokes-assembly.py) over the
+synthetic 64^3-hexahedra cube split in 6 P1 tetrahedra (1,572,864 elements, FP64 drop-in), with all
+inputs resident in HBM.  value = elements assembled per second (whole job).
+
+Also reported (outside the timed steps, same mesh): the dominant kernel's roofline (HIP-event
+duration of the assembly kernel against its algorithmic bytes), the FP64 SpMV bandwidth, one implicit
+time step to a 1e-8 relative residual in FP64 and in the FP32 + FP64-refinement form, and the CPU
+baseline (the oracle restatement, OpenMP over the host cores, bounded sample).
+"""
+import argparse
+import json
+import os
+import sys
+import time
+
+import numpy as np
+
+ROOT = os.path.dirname(os.path.abspath(__file__))
+sys.path.insert(0, ROOT)
+
+METRIC = "Melem/s assembly; time-to-1e-8 residual (FP32+refine vs FP64); p99 latency"
+HBM_PEAK_GBS = 8000.0
+
+
+def algorithmic_assembly_bytes(n_nodes, n_elems, k, dim, n_blocks, value_bytes):
+    neq = dim + 1
+    nnz = n_blocks * neq * neq
+    return n_nodes * (3 * dim * 8 + 16) + n_elems * k * 4 + nnz * value_bytes + n_nodes * neq * 8
+
+
+def algorithmic_spmv_bytes(n_nodes, dim, n_blocks, value_bytes):
+    neq = dim + 1
+    return n_blocks * neq * neq * value_bytes + n_blocks * 4 + (n_nodes + 1) * 4 + 2 * n_nodes * neq * value_bytes
+
+
+def cpu_baseline(case, budget_s=20.0):
+    """Oracle restatement on the host cores (the checker library, bounded sample)."""
+    from oracle import pyoracle
+    o = pyoracle.get()
+    start, cols, slots = o.build_sparsity(case.conn, case.n_nodes)
+    prm = pyoracle.ns_params(dt=case.dt)
+    nu = np.full(case.n_nodes, case.nu)
+    cores = min(os.cpu_count() or 1, 16)
+    reps, t_used = 0, 0.0
+    while t_used < budget_s and reps < 5:
+        t0 = time.perf_counter()
+        o.ns_assemble(pyoracle.TETRA3D, prm, case.coords, case.conn, case.u0, case.p0, case.u0, nu, start, cols,
+                      slots, nthreads=cores)
+        t_used += time.perf_counter() - t0
+        reps += 1
+        if t_used > budget_s / 2:
+            break
+    t1 = time.perf_counter()
+    o.ns_assemble(pyoracle.TETRA3D, prm, case.coords, case.conn, case.u0, case.p0, case.u0, nu, start, cols, slots,
+                  nthreads=1)
+    single = case.n_elems / (time.perf_counter() - t1) / 1e6
+    return {"value": case.n_elems * reps / t_used / 1e6, "unit": "Melem/s", "cores": cores, "kind": "port",
+            "sample": "%d full assemblies of the same mesh (oracle restatement, element colouring, OpenMP)" % reps,
+            "single_thread_value": single}
+
+
+def main():
+    ap = argparse.ArgumentParser()
+    ap.add_argument("--gpus", type=int, default=1)
+    ap.add_argument("--steps", type=int, default=20)
+    ap.add_argument("--warmup", type=int, default=3)
+    ap.add_argument("--cells", type=int, default=64, help="cube cells per side (64 = the drop-in-at-scale case)")
+    ap.add_argument("--no-cpu-baseline", action="store_true")
+    ap.add_argument("--no-solve", action="store_true")
+    args = ap.parse_args()
+    if args.gpus != 1:
+        print("bench.py: this build measures one GPU only; multi-GPU runs are not part of it (use --gpus 1)",
+              file=sys.stderr)
+        sys.exit(2)
+
+    from coolfluid3_amd import cabi, cases
+
+    case = cases.inflow_cube_tets(args.cells)
+    ctx = cabi.Context(0)
+    g = cases.GpuCase(ctx, case, precision=cabi.FP32)
+    g.fields.linearize()
+    s = g.system
+    E, N, nb = case.n_elems, case.n_nodes, s.n_blocks
+    ctx.set_async(True)
+    for _ in range(args.warmup):
+        g.assemble(cabi.FP64)
+    ctx.synchronize()
+    ctx.enable_timing(True)
+    t0 = time.perf_counter()
+    for _ in range(args.steps):
+        g.assemble(cabi.FP64)
+    ctx.synchronize()
+    t1 = time.perf_counter()
+    asm_ms, asm_n = ctx.kernel_time(cabi.KERNEL_ASSEMBLY)
+    ms_per_step = (t1 - t0) * 1e3 / args.steps
+    kern_ms = asm_ms / max(asm_n, 1)
+    alg = algorithmic_assembly_bytes(N, E, 4, 3, nb, 8)
+    achieved = alg / (kern_ms * 1e-3) / 1e9
+
+    # FP32 assembly of the precision form
+    ctx.enable_timing(True)
+    for _ in range(args.steps):
+        g.assemble(cabi.FP32)
+    ctx.synchronize()
+    a32_ms, a32_n = ctx.kernel_time(cabi.KERNEL_ASSEMBLY)
+    a32_kern = a32_ms / max(a32_n, 1)
+    alg32 = algorithmic_assembly_bytes(N, E, 4, 3, nb, 4)
+
+    # SpMV bandwidth (FP64 and FP32 values)
+    g.assemble(cabi.FP64)
+    g.assemble(cabi.FP32)
+    ctx.synchronize()
+    ctx.enable_timing(True)
+    x = s.vector_ptr(cabi.SOLUTION)
+    y = s.vector_ptr(cabi.RHS)
+    for _ in range(args.steps):
+        s.apply_device(1.0, x, 0.0, y)
+    ctx.synchronize()
+    sp_ms, sp_n = ctx.kernel_time(cabi.KERNEL_SPMV)
+    sp_kern = sp_ms / max(sp_n, 1)
+    sp_bytes = algorithmic_spmv_bytes(N, 3, nb, 8)
+    # plain copies of a known size (dofs doubles read + written): the k_copy kernel calibrates the
+    # FETCH_SIZE / WRITE_SIZE counters of the profiled runs
+    for _ in range(3):
+        s.assign(cabi.RHS, cabi.SOLUTION)
+    ctx.synchronize()
+    ctx.set_async(False)
+    ctx.enable_timing(False)
+
+    extra = {
+        "elements": E, "nodes": N, "dofs": s.dofs, "node_blocks": nb,
+        "assembly_kernel_ms": kern_ms, "assembly_fp32_kernel_ms": a32_kern,
+        "assembly_fp32_GBps": alg32 / (a32_kern * 1e-3) / 1e9,
+        "assembly_fp32_Melem_s": E / a32_kern / 1e3,
+        "spmv_fp64_ms": sp_kern, "spmv_fp64_GBps": sp_bytes / (sp_kern * 1e-3) / 1e9,
+        "dofs_per_s_assembly": s.dofs / (ms_per_step * 1e-3),
+    }
+
+    if not args.no_solve:
+        # one implicit step (assembly + BC + Krylov) to 1e-8 in FP64 and in FP32 + FP64 refinement
+        for label, prec in (("fp64", cabi.FP64), ("fp32_ir", cabi.FP32)):
+            gc = cases.GpuCase(ctx, case, precision=prec)
+            sp = cabi.solver_params(tol=1e-8, precision=prec, inner_tol=1e-4)
+            ctx.synchronize()
+            t0 = time.perf_counter()
+            st = gc.step(sp)
+            ctx.synchronize()
+            dt = time.perf_counter() - t0
+            extra["step_" + label] = {"ms": dt * 1e3, "iterations": st.iterations, "sweeps": st.outer,
+                                      "spmvs": st.spmvs, "rel_residual": st.rel_residual,
+                                      "converged": bool(st.converged)}
+            if label == "fp64":
+                u64 = gc.u.copy()
+            else:
+                extra["step_fp32_ir"]["velocity_rel_error_vs_fp64"] = float(np.abs(gc.u - u64).max() /
+                                                                           np.abs(u64).max())
+            gc.close()
+        extra["fp32_ir_speedup_per_step"] = extra["step_fp64"]["ms"] / extra["step_fp32_ir"]["ms"]
+
+    out = {
+        "metric": METRIC, "value": E / (ms_per_step * 1e-3) / 1e6, "unit": "Melem/s", "n_gpus": 1,
+        "steps": args.steps, "warmup": args.warmup, "ms_per_step": ms_per_step, "higher_is_better": True,
+        "scaling": "weak", "vs_baseline": None, "dtype": "f64", "data": "synthetic",
+        "config": {"workload": "UFEM NS assembly+scatter, unit cube %d^3 hexes x 6 P1 tets (%d elements, %d dofs), "
+                               "FP64 drop-in, nu=1e-3 dt=5e-3, seed 20261202" % (args.cells, E, s.dofs),
+                   "elements": E, "element_type": "P1 tetrahedron (specialised kernel)"},
+        "roofline": {"bound": "hbm", "achieved": achieved, "peak": HBM_PEAK_GBS, "unit": "GB/s",
+                     "frac": achieved / HBM_PEAK_GBS, "traffic": None,
+                     "kernel": "k_assemble<SpecP1<3,double>>", "algorithmic_bytes": alg,
+                     "kernel_ms": kern_ms},
+        "extra": extra,
+    }
+    g.close()
+    ctx.close()
+    if not args.no_cpu_baseline:
+        out["cpu_baseline"] = cpu_baseline(case)
+    print(json.dumps(out))
+
+
+if __name__ == "__main__":
+    main()

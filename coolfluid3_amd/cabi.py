@@ -20,6 +20,7 @@ FP64, FP32 = 0, 1
 RHS, SOLUTION = 0, 1
 BICGSTAB_JACOBI, GMRES_ILU0 = 0, 1
 FIELD_U, FIELD_P, FIELD_UADV, FIELD_U1, FIELD_U2, FIELD_U3, FIELD_NU, FIELD_G = range(8)
+KERNEL_ASSEMBLY, KERNEL_SPMV, KERNEL_SPMV_FP32, KERNEL_RESIDUAL = range(4)
 
 
 class Cf3HipError(RuntimeError):
@@ -76,6 +77,9 @@ def lib():
         "cf3hip_context_set_stream": [_vp, _vp],
         "cf3hip_context_synchronize": [_vp],
         "cf3hip_context_memory": [_vp, C.POINTER(C.c_uint64)],
+        "cf3hip_context_set_async": [_vp, C.c_int],
+        "cf3hip_context_enable_timing": [_vp, C.c_int],
+        "cf3hip_context_kernel_time": [_vp, C.c_int, _d, C.POINTER(C.c_uint64)],
         "cf3hip_mesh_create": [_vp, C.c_int, C.c_uint32, C.c_uint32, _d, _u32, C.POINTER(_vp)],
         "cf3hip_mesh_destroy": [_vp],
         "cf3hip_mesh_set_coordinates": [_vp, _d],
@@ -104,6 +108,7 @@ def lib():
         "cf3hip_vector_set_value": [_vp, C.c_int, C.c_uint32, C.c_uint32, C.c_double],
         "cf3hip_vector_get_value": [_vp, C.c_int, C.c_uint32, C.c_uint32, _d],
         "cf3hip_vector_device_ptr": [_vp, C.c_int, C.POINTER(_vp)],
+        "cf3hip_vector_assign": [_vp, C.c_int, C.c_int],
         "cf3hip_lss_dirichlet": [_vp, C.c_uint32, _u32, _u32, _d],
         "cf3hip_lss_solve": [_vp, C.POINTER(SolverParams), C.POINTER(SolveStats)],
         "cf3hip_lss_residual": [_vp, _d],
@@ -156,6 +161,19 @@ class Context:
         b = C.c_uint64()
         check(lib().cf3hip_context_memory(self.h, C.byref(b)))
         return b.value
+
+    def set_async(self, enable):
+        check(lib().cf3hip_context_set_async(self.h, int(enable)))
+
+    def enable_timing(self, enable=True):
+        check(lib().cf3hip_context_enable_timing(self.h, int(enable)))
+
+    def kernel_time(self, kernel):
+        """(total ms, launches) of a hot kernel since timing was enabled (HIP events on the stream)."""
+        ms = C.c_double()
+        n = C.c_uint64()
+        check(lib().cf3hip_context_kernel_time(self.h, kernel, C.byref(ms), C.byref(n)))
+        return ms.value, n.value
 
     def close(self):
         if self.h:
@@ -317,6 +335,9 @@ class System:
         p = _vp()
         check(lib().cf3hip_vector_device_ptr(self.h, which, C.byref(p)))
         return p.value
+
+    def assign(self, to, frm):
+        check(lib().cf3hip_vector_assign(self.h, to, frm))
 
     def dirichlet(self, nodes, eqs, values):
         n, e, v = _u32a(nodes), _u32a(eqs), _f64(values)

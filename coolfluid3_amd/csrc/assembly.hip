@@ -605,6 +605,7 @@ __global__ __launch_bounds__(kThreads) void k_assemble(AsmArgs g)
     for(int r = 0; r < NEQ; ++r) s_rhs[t * NEQ + r] = rhs[r];
   }
   __syncthreads();
+  if(!g.rhs) return;   // the FP32 matrix pass leaves the FP64 right-hand side alone
   for(uint32_t o = t; o < nrows * NEQ; o += kThreads) {
     const uint32_t row = o / NEQ, r = o % NEQ;
     R s = R(0);
@@ -616,9 +617,14 @@ __global__ __launch_bounds__(kThreads) void k_assemble(AsmArgs g)
 template<class Op, typename R, typename VR>
 void launch(const AsmArgs& g, const Lss& L)
 {
-  hipLaunchKernelGGL((k_assemble<Op, R, VR>), dim3(L.n_wg), dim3(kThreads), 0, L.ctx->stream, g);
+  Context& c = *L.ctx;
+  timer_begin(c, CF3HIP_KERNEL_ASSEMBLY);
+  hipLaunchKernelGGL((k_assemble<Op, R, VR>), dim3(L.n_wg), dim3(kThreads), 0, c.stream, g);
   HIP_OK(hipGetLastError());
+  timer_end(c, CF3HIP_KERNEL_ASSEMBLY);
 }
+
+__global__ void k_clear_flag(int* f) { f[0] = 0; }
 
 }  // namespace
 
@@ -657,14 +663,17 @@ void assemble(Lss& L, Fields& F, const cf3hip_ns_params& prm)
   g.inc = L.inc.as<uint32_t>();
   g.contrib_ptr = L.contrib_ptr.as<uint32_t>();
   g.contrib = L.contrib.as<uint16_t>();
-  g.rhs = L.rhs.as<double>();
+  const bool fp32 = prm.precision == CF3HIP_FP32;
+  // the right-hand side is formed by the FP64 pass only (FP64 outer residual of the precision form)
+  g.rhs = fp32 ? nullptr : L.rhs.as<double>();
   g.rho = prm.rho; g.dt = prm.dt; g.theta = prm.theta; g.invdt = 1. / prm.dt;
   g.supg_type = prm.supg_type;
   g.c1 = prm.c1; g.c2 = prm.c2; g.a_ps = prm.alpha_ps; g.a_su = prm.alpha_su; g.a_bu = prm.alpha_bu;
-  int* status = reinterpret_cast<int*>(c.scalars.as<double>() + 63);
-  HIP_OK(hipMemsetAsync(status, 0, sizeof(int), c.stream));
+  // element-validity flag: cleared per call in synchronous mode; in asynchronous mode it accumulates
+  // and is reported by cf3hip_context_synchronize
+  int* status = c.status_flag();
+  if(!c.async_mode) hipLaunchKernelGGL(k_clear_flag, dim3(1), dim3(1), 0, c.stream, status);
   g.status = status;
-  const bool fp32 = prm.precision == CF3HIP_FP32;
   g.values = fp32 ? L.values32.ptr : L.values.ptr;
 
   if(spec && M.etype == CF3HIP_TETRA3D) {
@@ -680,6 +689,7 @@ void assemble(Lss& L, Fields& F, const cf3hip_ns_params& prm)
     if(fp32) launch<GenericP1<2>, double, float>(g, L);
     else launch<GenericP1<2>, double, double>(g, L);
   }
+  if(c.async_mode) return;
   int st = 0;
   HIP_OK(hipMemcpyAsync(&st, status, sizeof(int), hipMemcpyDeviceToHost, c.stream));
   HIP_OK(hipStreamSynchronize(c.stream));

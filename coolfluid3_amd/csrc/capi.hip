@@ -33,6 +33,28 @@ void DevBuf::alloc(Context* c, size_t n)
   c->bytes_in_use += n;
 }
 
+void timer_begin(Context& c, int k)
+{
+  if(!c.timing) return;
+  KernelTimer& t = c.timers[k];
+  if(t.used + 2 > t.ev.size()) {
+    for(int i = 0; i < 64; ++i) {
+      hipEvent_t e;
+      HIP_OK(hipEventCreate(&e));
+      t.ev.push_back(e);
+    }
+  }
+  HIP_OK(hipEventRecord(t.ev[t.used], c.stream));
+}
+
+void timer_end(Context& c, int k)
+{
+  if(!c.timing) return;
+  KernelTimer& t = c.timers[k];
+  HIP_OK(hipEventRecord(t.ev[t.used + 1], c.stream));
+  t.used += 2;
+}
+
 void DevBuf::release()
 {
   if(ptr) {
@@ -162,6 +184,8 @@ int cf3hip_context_destroy(cf3hip_context ctx)
   API_BEGIN
   if(!ctx) return CF3HIP_OK;
   (void)hipStreamSynchronize(ctx->c.stream);
+  for(auto& t : ctx->c.timers)
+    for(hipEvent_t e : t.ev) (void)hipEventDestroy(e);
   if(ctx->c.scalars_host) (void)hipHostFree(ctx->c.scalars_host);
   delete ctx;
   API_END
@@ -179,7 +203,61 @@ int cf3hip_context_synchronize(cf3hip_context ctx)
 {
   API_BEGIN
   REQUIRE(ctx, CF3HIP_ERR_INVALID_ARG, "null context");
-  HIP_OK(hipStreamSynchronize(ctx->c.stream));
+  Context& c = ctx->c;
+  HIP_OK(hipStreamSynchronize(c.stream));
+  if(c.async_mode) {
+    int st = 0;
+    d2h(c, &st, c.status_flag(), sizeof(int));
+    if(st) {
+      const int zero = 0;
+      h2d(c, c.status_flag(), &zero, sizeof(int));
+      HIP_OK(hipStreamSynchronize(c.stream));
+      CF3_THROW(CF3HIP_ERR_SINGULAR, "non-positive element Jacobian determinant");
+    }
+  }
+  API_END
+}
+
+int cf3hip_context_set_async(cf3hip_context ctx, int enable)
+{
+  API_BEGIN
+  REQUIRE(ctx, CF3HIP_ERR_INVALID_ARG, "null context");
+  Context& c = ctx->c;
+  HIP_OK(hipStreamSynchronize(c.stream));
+  const int zero = 0;
+  h2d(c, c.status_flag(), &zero, sizeof(int));
+  HIP_OK(hipStreamSynchronize(c.stream));
+  c.async_mode = enable != 0;
+  API_END
+}
+
+int cf3hip_context_enable_timing(cf3hip_context ctx, int enable)
+{
+  API_BEGIN
+  REQUIRE(ctx, CF3HIP_ERR_INVALID_ARG, "null context");
+  Context& c = ctx->c;
+  HIP_OK(hipStreamSynchronize(c.stream));
+  for(auto& t : c.timers) t.used = 0;
+  c.timing = enable != 0;
+  API_END
+}
+
+int cf3hip_context_kernel_time(cf3hip_context ctx, int kernel, double* total_ms, uint64_t* launches)
+{
+  API_BEGIN
+  REQUIRE(ctx && total_ms, CF3HIP_ERR_INVALID_ARG, "null argument");
+  REQUIRE(kernel >= 0 && kernel < CF3HIP_KERNEL_COUNT, CF3HIP_ERR_INVALID_ARG, "unknown kernel id");
+  Context& c = ctx->c;
+  HIP_OK(hipStreamSynchronize(c.stream));
+  KernelTimer& t = c.timers[kernel];
+  double sum = 0.;
+  for(size_t i = 0; i + 1 < t.used; i += 2) {
+    float ms = 0.f;
+    HIP_OK(hipEventElapsedTime(&ms, t.ev[i], t.ev[i + 1]));
+    sum += ms;
+  }
+  *total_ms = sum;
+  if(launches) *launches = t.used / 2;
   API_END
 }
 
@@ -467,7 +545,7 @@ int cf3hip_matrix_apply(cf3hip_lss lss, double alpha, const double* x, double be
   API_BEGIN
   REQUIRE(lss && x && y, CF3HIP_ERR_INVALID_ARG, "null argument");
   lss_spmv(lss->l, alpha, x, beta, y);
-  HIP_OK(hipStreamSynchronize(lss->l.ctx->stream));
+  if(!lss->l.ctx->async_mode) HIP_OK(hipStreamSynchronize(lss->l.ctx->stream));
   API_END
 }
 
@@ -476,7 +554,7 @@ int cf3hip_matrix_apply_fp32(cf3hip_lss lss, float alpha, const float* x, float 
   API_BEGIN
   REQUIRE(lss && x && y, CF3HIP_ERR_INVALID_ARG, "null argument");
   lss_spmv32(lss->l, alpha, x, beta, y);
-  HIP_OK(hipStreamSynchronize(lss->l.ctx->stream));
+  if(!lss->l.ctx->async_mode) HIP_OK(hipStreamSynchronize(lss->l.ctx->stream));
   API_END
 }
 
@@ -567,6 +645,16 @@ int cf3hip_vector_device_ptr(cf3hip_lss lss, int which, double** dev)
   API_BEGIN
   REQUIRE(lss && dev, CF3HIP_ERR_INVALID_ARG, "null argument");
   *dev = vec_ptr(lss->l, which);
+  API_END
+}
+
+int cf3hip_vector_assign(cf3hip_lss lss, int to, int from)
+{
+  API_BEGIN
+  REQUIRE(lss, CF3HIP_ERR_NOT_CREATED, "system not created");
+  Lss& L = lss->l;
+  lss_copy(*L.ctx, vec_ptr(L, to), vec_ptr(L, from), L.dofs);
+  if(!L.ctx->async_mode) HIP_OK(hipStreamSynchronize(L.ctx->stream));
   API_END
 }
 

@@ -48,16 +48,28 @@ struct DevBuf {
   template<typename T> T* as() const { return static_cast<T*>(ptr); }
 };
 
+// HIP-event timing of the hot kernels (per launch, summed at query time)
+struct KernelTimer {
+  std::vector<hipEvent_t> ev;   // start/stop pairs
+  size_t used = 0;              // events recorded since the last reset
+};
+
 struct Context {
   int device = 0;
   hipStream_t stream = nullptr;
   uint64_t bytes_in_use = 0;
+  bool async_mode = false;      // no host synchronisation inside calls; errors at synchronize
+  bool timing = false;
+  KernelTimer timers[CF3HIP_KERNEL_COUNT];
   // scratch for reductions: partial sums and device scalars
   DevBuf red_partials;   // double[kMaxRedBlocks * 8]
-  DevBuf scalars;        // double[64]
-  DevBuf host_flag;      // unused placeholder for alignment
+  DevBuf scalars;        // double[64]; the last slot holds the element-validity flag
   double* scalars_host = nullptr;   // pinned mirror
+  int* status_flag() { return reinterpret_cast<int*>(scalars.as<double>() + 63); }
 };
+
+void timer_begin(Context& c, int kernel);
+void timer_end(Context& c, int kernel);
 
 constexpr int kMaxRedBlocks = 1024;
 
@@ -117,6 +129,7 @@ void assemble(Lss& L, Fields& F, const cf3hip_ns_params& prm);   // assembly.hip
 // lss.hip
 void lss_fill(Context& c, double* p, size_t n, double v);
 void lss_fill32(Context& c, float* p, size_t n, float v);
+void lss_copy(Context& c, double* dst, const double* src, size_t n);
 void lss_spmv(Lss& L, double alpha, const double* x, double beta, double* y);
 void lss_spmv32(Lss& L, float alpha, const float* x, float beta, float* y);
 void lss_dirichlet(Lss& L, uint32_t nbc, const uint32_t* nodes, const uint32_t* eqs, const double* values);

@@ -199,6 +199,20 @@ void lss_fill(Context& c, double* p, size_t n, double v)
   HIP_OK(hipGetLastError());
 }
 
+__global__ void k_copy(double* __restrict__ dst, const double* __restrict__ src, uint64_t n)
+{
+  const uint64_t stride = (uint64_t)gridDim.x * blockDim.x;
+  for(uint64_t j = (uint64_t)blockIdx.x * blockDim.x + threadIdx.x; j < n; j += stride) dst[j] = src[j];
+}
+
+void lss_copy(Context& c, double* dst, const double* src, size_t n)
+{
+  if(!n || dst == src) return;
+  hipLaunchKernelGGL(k_copy, dim3(std::min<unsigned>(grid_for(n, 256), 4096)), dim3(256), 0, c.stream, dst, src,
+                     (uint64_t)n);
+  HIP_OK(hipGetLastError());
+}
+
 void lss_fill32(Context& c, float* p, size_t n, float v)
 {
   if(!n) return;
@@ -210,6 +224,7 @@ void lss_fill32(Context& c, float* p, size_t n, float v)
 void lss_spmv(Lss& L, double alpha, const double* x, double beta, double* y)
 {
   Context& c = *L.ctx;
+  timer_begin(c, CF3HIP_KERNEL_SPMV);
   if(L.neq == 4) {
     hipLaunchKernelGGL(k_spmv4_f64, dim3(grid_for((uint64_t)L.n_nodes * 16, 256)), dim3(256), 0, c.stream, L.n_nodes,
                        L.row_ptr.as<uint32_t>(), L.cols.as<uint32_t>(), L.values.as<double>(), alpha, x, beta, y);
@@ -218,12 +233,14 @@ void lss_spmv(Lss& L, double alpha, const double* x, double beta, double* y)
                        L.row_ptr.as<uint32_t>(), L.cols.as<uint32_t>(), L.values.as<double>(), alpha, x, beta, y);
   }
   HIP_OK(hipGetLastError());
+  timer_end(c, CF3HIP_KERNEL_SPMV);
 }
 
 void lss_spmv32(Lss& L, float alpha, const float* x, float beta, float* y)
 {
   Context& c = *L.ctx;
   if(!L.has_fp32) CF3_THROW(CF3HIP_ERR_NOT_CREATED, "FP32 values not enabled");
+  timer_begin(c, CF3HIP_KERNEL_SPMV_FP32);
   if(L.neq == 4) {
     hipLaunchKernelGGL(k_spmv4_f32, dim3(grid_for((uint64_t)L.n_nodes * 16, 256)), dim3(256), 0, c.stream, L.n_nodes,
                        L.row_ptr.as<uint32_t>(), L.cols.as<uint32_t>(), L.values32.as<float>(), alpha, x, beta, y);
@@ -232,6 +249,7 @@ void lss_spmv32(Lss& L, float alpha, const float* x, float beta, float* y)
                        L.row_ptr.as<uint32_t>(), L.cols.as<uint32_t>(), L.values32.as<float>(), alpha, x, beta, y);
   }
   HIP_OK(hipGetLastError());
+  timer_end(c, CF3HIP_KERNEL_SPMV_FP32);
 }
 
 void lss_dirichlet(Lss& L, uint32_t nbc, const uint32_t* nodes, const uint32_t* eqs, const double* values)

@@ -63,6 +63,19 @@ int cf3hip_context_set_stream(cf3hip_context ctx, void* hip_stream);
 int cf3hip_context_synchronize(cf3hip_context ctx);
 /* bytes currently held by device allocations of this context */
 int cf3hip_context_memory(cf3hip_context ctx, uint64_t* bytes);
+/* asynchronous mode: assembly, Dirichlet and apply calls return without waiting for the device;
+ * element-validity errors are reported by the next cf3hip_context_synchronize */
+int cf3hip_context_set_async(cf3hip_context ctx, int enable);
+/* per-launch HIP-event timing of the hot kernels (the role of the per-action timers of
+ * cf3/common/TimedComponent.cpp:24-107); enabling resets the accumulated times */
+#define CF3HIP_KERNEL_ASSEMBLY   0
+#define CF3HIP_KERNEL_SPMV       1
+#define CF3HIP_KERNEL_SPMV_FP32  2
+#define CF3HIP_KERNEL_RESIDUAL   3
+#define CF3HIP_KERNEL_COUNT      4
+int cf3hip_context_enable_timing(cf3hip_context ctx, int enable);
+/* total milliseconds and number of timed launches of one kernel since timing was enabled */
+int cf3hip_context_kernel_time(cf3hip_context ctx, int kernel, double* total_ms, uint64_t* launches);
 
 /* ---- mesh: one Elements component of one LagrangeP1 type, device SoA coordinates
  *      (cf3/mesh Dictionary coordinates + Connectivity, cf3/mesh/Field.hpp:34, Connectivity.hpp:23) ---- */
@@ -122,6 +135,8 @@ int cf3hip_vector_set_value(cf3hip_lss lss, int which, uint32_t blockrow, uint32
 int cf3hip_vector_get_value(cf3hip_lss lss, int which, uint32_t blockrow, uint32_t ieq, double* v);
 /* device pointer of the rhs / solution storage (FP64, dofs entries), owned by the system */
 int cf3hip_vector_device_ptr(cf3hip_lss lss, int which, double** dev);
+/* Vector::assign (Vector.hpp): copy one system vector into the other */
+int cf3hip_vector_assign(cf3hip_lss lss, int to, int from);
 
 /* ---- System::dirichlet with preserve_symmetry (System.cpp:240-255 ->
  *      TrilinosFEVbrMatrix::symmetric_dirichlet, TrilinosFEVbrMatrix.cpp:482-553), applied to a list
