@@ -145,7 +145,7 @@ class GpuCase:
         prm.precision = precision
         self.system.assemble(self.fields, prm)
 
-    def step(self, solver_params=None):
+    def step(self, solver_params=None, raise_on_failure=True):
         s = self.system
         self.fields.linearize()                                   # LinearizeU
         self.assemble(cabi.FP64)                                  # ZeroLSS + Assembly
@@ -158,7 +158,7 @@ class GpuCase:
                 solver_params = cabi.solver_params(cabi.GMRES_ILU0, tol=self.case.tol)
             else:
                 solver_params = cabi.solver_params(tol=self.case.tol, precision=self.precision)
-        st = s.solve(solver_params)                               # SolveLSS
+        st = s.solve(solver_params, raise_on_failure)             # SolveLSS
         s.update(self.fields)                                     # UpdateActions
         self.u = self.fields.get(cabi.FIELD_U)
         self.p = self.fields.get(cabi.FIELD_P)
