@@ -120,6 +120,8 @@ def lib():
         "cf3hip_ns_linearize": [_vp],
         "cf3hip_ns_assemble": [_vp, _vp, C.POINTER(NSParams)],
         "cf3hip_ns_update": [_vp, _vp],
+        "cf3hip_ns_solve": [_vp, _vp, C.POINTER(NSParams), C.POINTER(SolverParams), C.POINTER(SolveStats)],
+        "cf3hip_ns_residual": [_vp, _vp, C.POINTER(NSParams), _vp, _vp],
     }
     for name, args in sig.items():
         fn = getattr(L, name)
@@ -360,6 +362,16 @@ class System:
 
     def update(self, fields):
         check(lib().cf3hip_ns_update(self.h, fields.h))
+
+    def ns_solve(self, fields, ns_prm, params, raise_on_failure=True):
+        st = SolveStats()
+        code = lib().cf3hip_ns_solve(self.h, fields.h, C.byref(ns_prm), C.byref(params), C.byref(st))
+        if code != OK and (raise_on_failure or code != 5):
+            check(code)
+        return st
+
+    def ns_residual_device(self, fields, ns_prm, x_ptr, r_ptr):
+        check(lib().cf3hip_ns_residual(self.h, fields.h, C.byref(ns_prm), _vp(x_ptr), _vp(r_ptr)))
 
     def close(self):
         if self.h:

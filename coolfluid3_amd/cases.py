@@ -148,9 +148,7 @@ class GpuCase:
     def step(self, solver_params=None, raise_on_failure=True):
         s = self.system
         self.fields.linearize()                                   # LinearizeU
-        self.assemble(cabi.FP64)                                  # ZeroLSS + Assembly
-        if self.precision == cabi.FP32:
-            self.assemble(cabi.FP32)
+        self.assemble(self.precision)                             # ZeroLSS + Assembly (FP64 or FP32 values)
         n, e, v = bc_lists(self.case, self.u, self.p)            # BoundaryConditions
         s.dirichlet(n, e, v)
         if solver_params is None:
@@ -158,7 +156,9 @@ class GpuCase:
                 solver_params = cabi.solver_params(cabi.GMRES_ILU0, tol=self.case.tol)
             else:
                 solver_params = cabi.solver_params(tol=self.case.tol, precision=self.precision)
-        st = s.solve(solver_params, raise_on_failure)             # SolveLSS
+        prm = cabi.NSParams.from_buffer_copy(self.prm)
+        prm.precision = self.precision
+        st = s.ns_solve(self.fields, prm, solver_params, raise_on_failure)   # SolveLSS
         s.update(self.fields)                                     # UpdateActions
         self.u = self.fields.get(cabi.FIELD_U)
         self.p = self.fields.get(cabi.FIELD_P)

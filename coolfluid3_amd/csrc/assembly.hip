@@ -47,6 +47,8 @@ struct AsmArgs {
   const uint16_t* contrib;
   void* values;
   double* rhs;
+  const double* xt;    // matrix-free residual: x with constrained columns set to their first value
+  double* res;         // matrix-free residual output
   double rho, dt, theta, invdt;
   int supg_type;
   double c1, c2, a_ps, a_su, a_bu;
@@ -614,6 +616,79 @@ __global__ __launch_bounds__(kThreads) void k_assemble(AsmArgs g)
   }
 }
 
+// Matrix-free FP64 residual of the assembled, Dirichlet-constrained system: per node row
+// r_i = sum over incident elements of (b_e(a) - M_e(a, :) x~_e), with x~ = x except on constrained dofs
+// where it holds the eliminated value (the symmetric elimination moves A[:, c] v_c to the rhs).
+// Constrained rows are fixed afterwards (r = v - x).  No matrix is read or written.
+template<class Op>
+__global__ __launch_bounds__(kThreads) void k_residual(AsmArgs g)
+{
+  constexpr int K = Op::K, NEQ = Op::NEQ, NN = NEQ * NEQ;
+  __shared__ double s_res[kThreads * NEQ];
+  __shared__ uint32_t s_rowinc[kThreads + 1];
+  const uint32_t r0 = g.wg_rows[blockIdx.x], r1 = g.wg_rows[blockIdx.x + 1];
+  const uint32_t nrows = r1 - r0;
+  const uint32_t i0 = g.inc_ptr[r0];
+  const uint32_t ninc = g.inc_ptr[r1] - i0;
+  const int t = threadIdx.x;
+  for(uint32_t j = t; j <= nrows; j += kThreads) s_rowinc[j] = g.inc_ptr[r0 + j] - i0;
+  const bool active = (uint32_t)t < ninc;
+  Op op;
+  double rhs[NEQ], y[NEQ], xt[K][NEQ];
+#pragma unroll
+  for(int r = 0; r < NEQ; ++r) { rhs[r] = 0.; y[r] = 0.; }
+  if(active) {
+    const uint32_t ea = g.inc[i0 + t];
+    const uint32_t e = ea / K;
+    op.prepare(g, e, (int)(ea % K));
+#pragma unroll
+    for(int m = 0; m < K; ++m) {
+      const uint32_t nd = g.conn[(uint64_t)e * K + m];
+#pragma unroll
+      for(int c = 0; c < NEQ; ++c) xt[m][c] = g.xt[(uint64_t)nd * NEQ + c];
+    }
+  }
+  static_for<0, K>([&](auto bc) {
+    constexpr int B = decltype(bc)::value;
+    if(active) {
+      double mb[NN];
+      op.template block<B, double>(g, mb, rhs);
+#pragma unroll
+      for(int r = 0; r < NEQ; ++r)
+#pragma unroll
+        for(int c = 0; c < NEQ; ++c) y[r] += mb[r * NEQ + c] * xt[B][c];
+    }
+  });
+  if(active) {
+    op.rhs_source(g, rhs);
+#pragma unroll
+    for(int r = 0; r < NEQ; ++r) s_res[t * NEQ + r] = rhs[r] - y[r];
+  }
+  __syncthreads();
+  for(uint32_t o = t; o < nrows * NEQ; o += kThreads) {
+    const uint32_t row = o / NEQ, r = o % NEQ;
+    double s = 0.;
+    for(uint32_t q = s_rowinc[row]; q < s_rowinc[row + 1]; ++q) s += s_res[q * NEQ + r];
+    g.res[(uint64_t)(r0 + row) * NEQ + r] = s;
+  }
+}
+
+__global__ void k_xtilde(uint64_t dofs, int neq, const uint8_t* __restrict__ mask, const double* __restrict__ vfirst,
+                         const double* __restrict__ x, double* __restrict__ xt)
+{
+  const uint64_t i = (uint64_t)blockIdx.x * blockDim.x + threadIdx.x;
+  if(i >= dofs) return;
+  const uint8_t m = mask ? mask[i / neq] : 0;
+  xt[i] = ((m >> (i % neq)) & 1) ? vfirst[i] : x[i];
+}
+
+__global__ void k_bc_residual(uint32_t n, const uint32_t* __restrict__ dof, const double* __restrict__ vlast,
+                              const double* __restrict__ x, double* __restrict__ r)
+{
+  const uint32_t i = blockIdx.x * blockDim.x + threadIdx.x;
+  if(i < n) r[dof[i]] = vlast[i] - x[dof[i]];
+}
+
 template<class Op, typename R, typename VR>
 void launch(const AsmArgs& g, const Lss& L)
 {
@@ -628,7 +703,9 @@ __global__ void k_clear_flag(int* f) { f[0] = 0; }
 
 }  // namespace
 
-void assemble(Lss& L, Fields& F, const cf3hip_ns_params& prm)
+namespace {
+
+AsmArgs make_args(Lss& L, Fields& F, const cf3hip_ns_params& prm)
 {
   if(!L.mesh) CF3_THROW(CF3HIP_ERR_NOT_CREATED, "system was created without a mesh: no element scatter map");
   if(prm.theta < 0. || prm.theta > 1.)
@@ -637,17 +714,13 @@ void assemble(Lss& L, Fields& F, const cf3hip_ns_params& prm)
   const Mesh& M = *L.mesh;
   if(F.mesh != L.mesh) CF3_THROW(CF3HIP_ERR_INVALID_ARG, "fields belong to another mesh");
   if(L.neq != M.dim + 1) CF3_THROW(CF3HIP_ERR_INVALID_ARG, "the NS system needs neq = dim + 1");
-  if(prm.precision == CF3HIP_FP32 && !L.has_fp32)
-    CF3_THROW(CF3HIP_ERR_NOT_CREATED, "FP32 assembly needs cf3hip_lss_enable_fp32");
   const bool spec = prm.use_specializations && M.etype != CF3HIP_HEXA3D;
   if(!spec && prm.supg_type == CF3HIP_SUPG_CF2)
     CF3_THROW(CF3HIP_ERR_NOT_IMPLEMENTED,
               "supg_type cf2 divides by sum_n u.grad(N_n), identically zero by the partition of unity");
   if(M.etype == CF3HIP_HEXA3D)
     CF3_THROW(CF3HIP_ERR_NOT_IMPLEMENTED, "hexahedra assembly is provided by the Q1 kernel");
-  Context& c = *L.ctx;
-
-  AsmArgs g;
+  AsmArgs g = {};
   for(int d = 0; d < 3; ++d) {
     g.nd.x[d] = d < M.dim ? M.x[d].as<double>() : nullptr;
     g.nd.u[d] = d < M.dim ? F.u[d].as<double>() : nullptr;
@@ -663,17 +736,51 @@ void assemble(Lss& L, Fields& F, const cf3hip_ns_params& prm)
   g.inc = L.inc.as<uint32_t>();
   g.contrib_ptr = L.contrib_ptr.as<uint32_t>();
   g.contrib = L.contrib.as<uint16_t>();
-  const bool fp32 = prm.precision == CF3HIP_FP32;
-  // the right-hand side is formed by the FP64 pass only (FP64 outer residual of the precision form)
-  g.rhs = fp32 ? nullptr : L.rhs.as<double>();
   g.rho = prm.rho; g.dt = prm.dt; g.theta = prm.theta; g.invdt = 1. / prm.dt;
   g.supg_type = prm.supg_type;
   g.c1 = prm.c1; g.c2 = prm.c2; g.a_ps = prm.alpha_ps; g.a_su = prm.alpha_su; g.a_bu = prm.alpha_bu;
+  g.status = L.ctx->status_flag();
+  return g;
+}
+
+void check_status(Context& c)
+{
+  if(c.async_mode) return;
+  int st = 0;
+  HIP_OK(hipMemcpyAsync(&st, c.status_flag(), sizeof(int), hipMemcpyDeviceToHost, c.stream));
+  HIP_OK(hipStreamSynchronize(c.stream));
+  if(st) CF3_THROW(CF3HIP_ERR_SINGULAR, "non-positive element Jacobian determinant");
+}
+
+template<class Op>
+void launch_residual(const AsmArgs& g, const Lss& L)
+{
+  Context& c = *L.ctx;
+  timer_begin(c, CF3HIP_KERNEL_RESIDUAL);
+  hipLaunchKernelGGL((k_residual<Op>), dim3(L.n_wg), dim3(kThreads), 0, c.stream, g);
+  HIP_OK(hipGetLastError());
+  timer_end(c, CF3HIP_KERNEL_RESIDUAL);
+}
+
+}  // namespace
+
+void assemble(Lss& L, Fields& F, const cf3hip_ns_params& prm)
+{
+  AsmArgs g = make_args(L, F, prm);
+  const Mesh& M = *L.mesh;
+  if(prm.precision == CF3HIP_FP32 && !L.has_fp32)
+    CF3_THROW(CF3HIP_ERR_NOT_CREATED, "FP32 assembly needs cf3hip_lss_enable_fp32");
+  const bool spec = prm.use_specializations && M.etype != CF3HIP_HEXA3D;
+  Context& c = *L.ctx;
+  const bool fp32 = prm.precision == CF3HIP_FP32;
+  // ZeroLSS: every value and rhs entry is overwritten by the kernel; the conditions of the previous
+  // step are forgotten
+  lss_clear_conditions(L);
+  // the right-hand side is formed by the FP64 pass only (FP64 outer residual of the precision form)
+  g.rhs = fp32 ? nullptr : L.rhs.as<double>();
   // element-validity flag: cleared per call in synchronous mode; in asynchronous mode it accumulates
   // and is reported by cf3hip_context_synchronize
-  int* status = c.status_flag();
-  if(!c.async_mode) hipLaunchKernelGGL(k_clear_flag, dim3(1), dim3(1), 0, c.stream, status);
-  g.status = status;
+  if(!c.async_mode) hipLaunchKernelGGL(k_clear_flag, dim3(1), dim3(1), 0, c.stream, g.status);
   g.values = fp32 ? L.values32.ptr : L.values.ptr;
 
   if(spec && M.etype == CF3HIP_TETRA3D) {
@@ -689,11 +796,29 @@ void assemble(Lss& L, Fields& F, const cf3hip_ns_params& prm)
     if(fp32) launch<GenericP1<2>, double, float>(g, L);
     else launch<GenericP1<2>, double, double>(g, L);
   }
-  if(c.async_mode) return;
-  int st = 0;
-  HIP_OK(hipMemcpyAsync(&st, status, sizeof(int), hipMemcpyDeviceToHost, c.stream));
-  HIP_OK(hipStreamSynchronize(c.stream));
-  if(st) CF3_THROW(CF3HIP_ERR_SINGULAR, "non-positive element Jacobian determinant");
+  check_status(c);
+}
+
+void ns_residual(Lss& L, Fields& F, const cf3hip_ns_params& prm, const double* x, double* r)
+{
+  AsmArgs g = make_args(L, F, prm);
+  const Mesh& M = *L.mesh;
+  Context& c = *L.ctx;
+  if(L.xtilde.bytes < sizeof(double) * L.dofs) L.xtilde.alloc(&c, sizeof(double) * L.dofs);
+  hipLaunchKernelGGL(k_xtilde, dim3(grid_for(L.dofs, 256)), dim3(256), 0, c.stream, L.dofs, L.neq,
+                     L.bc_n ? L.bc_mask.as<uint8_t>() : (const uint8_t*)nullptr,
+                     L.bc_n ? L.bc_vfirst.as<double>() : (const double*)nullptr, x, L.xtilde.as<double>());
+  g.xt = L.xtilde.as<double>();
+  g.res = r;
+  const bool spec = prm.use_specializations && M.etype != CF3HIP_HEXA3D;
+  if(spec && M.etype == CF3HIP_TETRA3D) launch_residual<SpecP1<3, double> >(g, L);
+  else if(spec && M.etype == CF3HIP_TRIAG2D) launch_residual<SpecP1<2, double> >(g, L);
+  else if(M.etype == CF3HIP_TETRA3D) launch_residual<GenericP1<3> >(g, L);
+  else launch_residual<GenericP1<2> >(g, L);
+  if(L.bc_n)
+    hipLaunchKernelGGL(k_bc_residual, dim3(grid_for(L.bc_n, 256)), dim3(256), 0, c.stream, L.bc_n,
+                       L.bc_dof.as<uint32_t>(), L.bc_vlast.as<double>(), x, r);
+  HIP_OK(hipGetLastError());
 }
 
 }  // namespace cf3hip

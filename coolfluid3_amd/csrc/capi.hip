@@ -447,6 +447,7 @@ int cf3hip_matrix_reset(cf3hip_lss lss, double value)
   REQUIRE(lss, CF3HIP_ERR_NOT_CREATED, "system not created");
   Lss& L = lss->l;
   lss_fill(*L.ctx, L.values.as<double>(), L.n_blocks * L.neq * L.neq, value);
+  lss_clear_conditions(L);
   if(L.has_fp32) lss_fill32(*L.ctx, L.values32.as<float>(), L.n_blocks * L.neq * L.neq, (float)value);
   HIP_OK(hipStreamSynchronize(L.ctx->stream));
   API_END
@@ -795,6 +796,31 @@ int cf3hip_ns_assemble(cf3hip_lss lss, cf3hip_fields fields, const cf3hip_ns_par
   API_BEGIN
   REQUIRE(lss && fields && prm, CF3HIP_ERR_INVALID_ARG, "null argument");
   assemble(lss->l, fields->f, *prm);
+  API_END
+}
+
+int cf3hip_ns_solve(cf3hip_lss lss, cf3hip_fields fields, const cf3hip_ns_params* nsp, const cf3hip_solver_params* prm,
+                    cf3hip_solve_stats* stats)
+{
+  API_BEGIN
+  REQUIRE(lss && fields && nsp && prm, CF3HIP_ERR_INVALID_ARG, "null argument");
+  REQUIRE(prm->tol > 0. && prm->max_iter > 0, CF3HIP_ERR_INVALID_ARG, "tolerance and iteration limit must be positive");
+  cf3hip_solve_stats st;
+  ns_solve(lss->l, fields->f, *nsp, *prm, st);
+  if(stats) *stats = st;
+  if(!st.converged) {
+    set_error("Krylov solve did not reach the tolerance");
+    return CF3HIP_ERR_NOT_CONVERGED;
+  }
+  API_END
+}
+
+int cf3hip_ns_residual(cf3hip_lss lss, cf3hip_fields fields, const cf3hip_ns_params* nsp, const double* x, double* r)
+{
+  API_BEGIN
+  REQUIRE(lss && fields && nsp && x && r, CF3HIP_ERR_INVALID_ARG, "null argument");
+  ns_residual(lss->l, fields->f, *nsp, x, r);
+  if(!lss->l.ctx->async_mode) HIP_OK(hipStreamSynchronize(lss->l.ctx->stream));
   API_END
 }
 

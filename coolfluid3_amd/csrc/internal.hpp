@@ -5,6 +5,7 @@
 
 #include <cstdint>
 #include <string>
+#include <unordered_map>
 #include <vector>
 
 #include "../../include/cf3hip.h"
@@ -106,6 +107,17 @@ struct Lss {
   DevBuf contrib;       // uint16 n_elems*k*k : (q << 4) | b, sorted by (slot, b, q)
   DevBuf wg_rows;       // uint32 n_wg+1 : row ranges of the assembly workgroups
   uint32_t n_wg = 0;
+  // Dirichlet conditions applied since the last reset (needed by the matrix-free FP64 residual):
+  // per dof the first value (column elimination) and the last value (row / solution)
+  std::unordered_map<uint64_t, std::pair<double, double> > bc_host;
+  std::vector<uint64_t> bc_order;
+  uint32_t bc_n = 0;
+  DevBuf bc_dof;        // uint32 bc_n
+  DevBuf bc_vlast;      // double bc_n
+  DevBuf bc_vfirst;     // double dofs (valid at BC dofs)
+  DevBuf bc_mask;       // uint8 n_nodes: bit e set when dof (node, e) carries a condition
+  DevBuf bc_newmask;    // uint8 n_nodes: conditions added by the current call
+  DevBuf xtilde;        // matrix-free residual input (x with eliminated columns at their values)
   // Krylov workspace (allocated lazily)
   DevBuf work;          // generic vector workspace
   size_t work_vectors = 0;
@@ -130,6 +142,14 @@ void assemble(Lss& L, Fields& F, const cf3hip_ns_params& prm);   // assembly.hip
 void lss_fill(Context& c, double* p, size_t n, double v);
 void lss_fill32(Context& c, float* p, size_t n, float v);
 void lss_copy(Context& c, double* dst, const double* src, size_t n);
+void lss_fill8(Context& c, uint8_t* p, size_t n, uint8_t v);
+void lss_clear_conditions(Lss& L);
+// FP32 values, FP64 vectors and accumulation: y = A32 x
+void lss_spmv_mixed(Lss& L, const double* x, double* y);
+// matrix-free FP64 residual r = b' - A' x of the NS system (assembly.hip)
+void ns_residual(Lss& L, Fields& F, const cf3hip_ns_params& prm, const double* x, double* r);
+void ns_solve(Lss& L, Fields& F, const cf3hip_ns_params& nsp, const cf3hip_solver_params& prm,
+              cf3hip_solve_stats& st);
 void lss_spmv(Lss& L, double alpha, const double* x, double beta, double* y);
 void lss_spmv32(Lss& L, float alpha, const float* x, float beta, float* y);
 void lss_dirichlet(Lss& L, uint32_t nbc, const uint32_t* nodes, const uint32_t* eqs, const double* values);
@@ -137,6 +157,7 @@ void lss_point_csr(Lss& L, uint32_t* row_ptr_dev, uint32_t* col_dev, double* val
 
 // krylov.hip
 void lss_solve(Lss& L, const cf3hip_solver_params& prm, cf3hip_solve_stats& st);
+void lss_solve_reliable(Lss& L, const cf3hip_solver_params& prm, cf3hip_solve_stats& st);
 double lss_residual(Lss& L);
 
 // nodes.hip

@@ -333,42 +333,166 @@ void lss_solve(Lss& L, const cf3hip_solver_params& prm, cf3hip_solve_stats& st)
     st.converged = conv ? 1 : 0;
     return;
   }
-  // FP32 inner BiCGStab + FP64 iterative refinement: r = b - A x (FP64), A d = r/||r|| (FP32), x += ||r|| d
-  if(!L.has_fp32) CF3_THROW(CF3HIP_ERR_NOT_CREATED, "FP32 values not enabled");
-  const uint64_t n = L.dofs;
-  ensure_work(L, 12, sizeof(double));
-  double* r64 = L.work.as<double>();
-  float* r32 = reinterpret_cast<float*>(r64 + n);
-  float* d32 = r32 + n;
-  float* w32 = d32 + n;
-  double bn = norm2(c, L.rhs.as<double>(), n);
-  if(bn == 0.) bn = 1.;
-  int outer = 0, iters = 0, spmvs = 0;
-  double rel = 0.;
-  for(;;) {
-    HIP_OK(hipMemcpyAsync(r64, L.rhs.ptr, sizeof(double) * n, hipMemcpyDeviceToDevice, c.stream));
-    lss_spmv(L, -1., L.sol.as<double>(), 1., r64);
-    ++spmvs;
-    const double rn = norm2(c, r64, n);
-    rel = rn / bn;
-    if(rel <= prm.tol || outer >= prm.max_outer || iters >= prm.max_iter) break;
-    ++outer;
-    hipLaunchKernelGGL((k_convert_scaled<double, float>), dim3(std::min<unsigned>(grid_for(n, 256), 8192)), dim3(256), 0,
-                       c.stream, n, r64, r32, 1.0 / rn);
-    lss_fill32(c, d32, n, 0.f);
-    bool conv = false;
-    int sp = 0;
-    iters += bicgstab<float>(L, r32, d32, w32, prm.inner_tol, prm.max_iter - iters, conv, sp);
-    spmvs += sp;
-    hipLaunchKernelGGL(k_axpy_f32, dim3(std::min<unsigned>(grid_for(n, 256), 8192)), dim3(256), 0, c.stream, n, rn,
-                       d32, L.sol.as<double>());
-    HIP_OK(hipGetLastError());
+  // FP32 values with FP64 refinement of the stored FP64 system (reliable updates)
+  (void)c;
+  lss_solve_reliable(L, prm, st);
+}
+
+namespace {
+
+__global__ void k_flush_increment(uint64_t n, double* __restrict__ xhi, double* __restrict__ xlo)
+{
+  const uint64_t stride = (uint64_t)gridDim.x * blockDim.x;
+  for(uint64_t i = (uint64_t)blockIdx.x * blockDim.x + threadIdx.x; i < n; i += stride) {
+    xhi[i] += xlo[i];
+    xlo[i] = 0.;
   }
-  st.iterations = iters;
-  st.outer = outer;
+}
+
+__global__ void k_jacobi_f32(uint32_t N, int neq, const uint32_t* __restrict__ diag_blk, const float* __restrict__ vals,
+                             double* __restrict__ dinv)
+{
+  const uint64_t r = (uint64_t)blockIdx.x * blockDim.x + threadIdx.x;
+  if(r >= (uint64_t)N * neq) return;
+  const uint32_t i = (uint32_t)(r / neq);
+  const int a = (int)(r % neq);
+  dinv[r] = 1. / (double)vals[(uint64_t)diag_blk[i] * neq * neq + a * neq + a];
+}
+
+__global__ void k_set_scalar(double* S, int idx, double v) { S[idx] = v; }
+
+}  // namespace
+
+// Precision form: BiCGStab with Jacobi preconditioning on the FP32 values (FP32 stored, FP64 vectors and
+// FP64 accumulation in the SpMV), with reliable updates: the FP64 solution accumulates the increments
+// and, whenever the recursive residual has dropped by the factor delta (or meets the tolerance), the
+// residual is replaced by the true FP64 residual b - A x given by `resid`.  The search directions are
+// kept across replacements (no restart), so the iteration count stays close to that of the FP64
+// method.  Convergence is declared on the FP64 true residual only.
+template<class Resid>
+void reliable_bicgstab(Lss& L, Resid resid, const cf3hip_solver_params& prm, cf3hip_solve_stats& st)
+{
+  if(prm.method != CF3HIP_BICGSTAB_JACOBI) CF3_THROW(CF3HIP_ERR_NOT_IMPLEMENTED, "precision form uses BiCGStab");
+  if(!L.has_fp32) CF3_THROW(CF3HIP_ERR_NOT_CREATED, "FP32 values not enabled");
+  Context& c = *L.ctx;
+  const uint64_t n = L.dofs;
+  ensure_work(L, 10, sizeof(double));
+  double* r = L.work.as<double>();
+  double* rh = r + n;
+  double* p = rh + n;
+  double* v = p + n;
+  double* ph = v + n;
+  double* s = ph + n;
+  double* sh = s + n;
+  double* t = sh + n;
+  double* dinv = t + n;
+  double* xlo = dinv + n;
+  double* xhi = L.sol.as<double>();
+  double* S = c.scalars.as<double>();
+  double* part = c.red_partials.as<double>();
+  const unsigned G = red_grid(n);
+  const double delta = 0.01;
+  st = cf3hip_solve_stats{0, 0, 0, 0, 0.};
+
+  // ||b'|| = ||r(0)||
+  lss_fill(c, xlo, n, 0.);
+  resid(xlo, r);
+  double bn = norm2(c, r, n);
+  if(bn == 0.) bn = 1.;
+  const double thr2 = prm.tol * prm.tol * bn * bn;
+  resid(xhi, r);
+  int updates = 1;
+  double rtrue = norm2(c, r, n);
+  hipLaunchKernelGGL(k_jacobi_f32, dim3(grid_for(n, 256)), dim3(256), 0, c.stream, L.n_nodes, L.neq,
+                     L.diag_blk.as<uint32_t>(), L.values32.as<float>(), dinv);
+  lss_copy(c, rh, r, n);
+  lss_fill(c, p, n, 0.);
+  lss_fill(c, v, n, 0.);
+  hipLaunchKernelGGL(k_set_scalars, dim3(1), dim3(1), 0, c.stream, S, thr2);
+  hipLaunchKernelGGL(k_dot2<double>, dim3(G), dim3(kRT), 0, c.stream, n, rh, r, (const double*)nullptr,
+                     (const double*)nullptr, part);
+  hipLaunchKernelGGL(k_finalize<1>, dim3(1), dim3(kRT), 0, c.stream, part, G, S, (int)S_RHO, (int)FIN_STORE);
+  HIP_OK(hipGetLastError());
+  bool conv = rtrue * rtrue <= thr2;
+  double maxr = rtrue;
+  int it = 0, spmvs = 0;
+  Scalars sc;
+  while(!conv && it < prm.max_iter) {
+    ++it;
+    hipLaunchKernelGGL(k_bicg_p<double>, dim3(G), dim3(kRT), 0, c.stream, n, S, r, p, v, ph, dinv);
+    lss_spmv_mixed(L, ph, v);
+    hipLaunchKernelGGL(k_dot2<double>, dim3(G), dim3(kRT), 0, c.stream, n, rh, v, (const double*)nullptr,
+                       (const double*)nullptr, part);
+    hipLaunchKernelGGL(k_finalize<1>, dim3(1), dim3(kRT), 0, c.stream, part, G, S, 0, (int)FIN_ALPHA);
+    hipLaunchKernelGGL(k_bicg_s<double>, dim3(G), dim3(kRT), 0, c.stream, n, S, r, v, s, sh, dinv, part);
+    hipLaunchKernelGGL(k_finalize<1>, dim3(1), dim3(kRT), 0, c.stream, part, G, S, 0, (int)FIN_SNORM);
+    lss_spmv_mixed(L, sh, t);
+    hipLaunchKernelGGL(k_dot2<double>, dim3(G), dim3(kRT), 0, c.stream, n, t, s, t, t, part);
+    hipLaunchKernelGGL(k_finalize<2>, dim3(1), dim3(kRT), 0, c.stream, part, G, S, 0, (int)FIN_OMEGA);
+    hipLaunchKernelGGL(k_bicg_x<double>, dim3(G), dim3(kRT), 0, c.stream, n, S, xlo, ph, sh, r, s, t, rh, part);
+    hipLaunchKernelGGL(k_finalize<2>, dim3(1), dim3(kRT), 0, c.stream, part, G, S, 0, (int)FIN_RNORM);
+    HIP_OK(hipGetLastError());
+    spmvs += 2;
+    read_scalars(c, sc);
+    const double rn = std::sqrt(sc.v[S_RN2]);
+    const bool breakdown = !std::isfinite(sc.v[S_RN2]) || sc.v[S_RHO] == 0.;
+    if(rn < delta * maxr || sc.v[S_RN2] <= thr2 || sc.v[S_SCONV] != 0. || breakdown) {
+      // reliable update: move the increment into the FP64 solution, replace the residual
+      hipLaunchKernelGGL(k_flush_increment, dim3(std::min<unsigned>(grid_for(n, 256), 8192)), dim3(256), 0, c.stream,
+                         n, xhi, xlo);
+      resid(xhi, r);
+      ++updates;
+      rtrue = norm2(c, r, n);
+      if(rtrue * rtrue <= thr2) { conv = true; break; }
+      if(breakdown) {   // restart the recurrences from the true residual
+        lss_copy(c, rh, r, n);
+        lss_fill(c, p, n, 0.);
+        lss_fill(c, v, n, 0.);
+        hipLaunchKernelGGL(k_set_scalars, dim3(1), dim3(1), 0, c.stream, S, thr2);
+      }
+      hipLaunchKernelGGL(k_dot2<double>, dim3(G), dim3(kRT), 0, c.stream, n, rh, r, (const double*)nullptr,
+                         (const double*)nullptr, part);
+      hipLaunchKernelGGL(k_finalize<1>, dim3(1), dim3(kRT), 0, c.stream, part, G, S, (int)S_RHO, (int)FIN_STORE);
+      hipLaunchKernelGGL(k_set_scalar, dim3(1), dim3(1), 0, c.stream, S, (int)S_SCONV, 0.);
+      HIP_OK(hipGetLastError());
+      maxr = rtrue;
+    } else {
+      maxr = std::max(maxr, rn);
+    }
+  }
+  if(!conv) {
+    hipLaunchKernelGGL(k_flush_increment, dim3(std::min<unsigned>(grid_for(n, 256), 8192)), dim3(256), 0, c.stream, n,
+                       xhi, xlo);
+    resid(xhi, r);
+    rtrue = norm2(c, r, n);
+    conv = rtrue * rtrue <= thr2;
+  }
+  st.iterations = it;
+  st.outer = updates;
   st.spmvs = spmvs;
-  st.rel_residual = rel;
-  st.converged = rel <= prm.tol ? 1 : 0;
+  st.rel_residual = rtrue / bn;
+  st.converged = conv ? 1 : 0;
+}
+
+void ns_solve(Lss& L, Fields& F, const cf3hip_ns_params& nsp, const cf3hip_solver_params& prm,
+              cf3hip_solve_stats& st)
+{
+  if(prm.precision == CF3HIP_FP64) {
+    lss_solve(L, prm, st);
+    return;
+  }
+  // the true residual is evaluated matrix-free from the FP64 element matrices of the fields
+  reliable_bicgstab(L, [&](const double* x, double* r) { ns_residual(L, F, nsp, x, r); }, prm, st);
+}
+
+// precision form on a stored system: the true residual uses the stored FP64 values and rhs
+void lss_solve_reliable(Lss& L, const cf3hip_solver_params& prm, cf3hip_solve_stats& st)
+{
+  Context& c = *L.ctx;
+  reliable_bicgstab(L, [&](const double* x, double* r) {
+    lss_copy(c, r, L.rhs.as<double>(), L.dofs);
+    lss_spmv(L, -1., x, 1., r);
+  }, prm, st);
 }
 
 }  // namespace cf3hip

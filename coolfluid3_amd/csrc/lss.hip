@@ -79,6 +79,46 @@ __global__ __launch_bounds__(256) void k_spmv4_f32(uint32_t N, const uint32_t* _
   }
 }
 
+// FP32 values with FP64 vectors and FP64 accumulation, neq = 4: lane layout of k_spmv4_f32
+__global__ __launch_bounds__(256) void k_spmv4_mixed(uint32_t N, const uint32_t* __restrict__ row_ptr,
+                                                     const uint32_t* __restrict__ cols, const float* __restrict__ vals,
+                                                     const double* __restrict__ x, double* __restrict__ y)
+{
+  const uint32_t row = (blockIdx.x * blockDim.x + threadIdx.x) >> 4;
+  const int l = threadIdx.x & 15;
+  if(row >= N) return;
+  const uint32_t q0 = row_ptr[row], q1 = row_ptr[row + 1];
+  const int sub = l >> 2, a = l & 3;
+  double acc = 0.;
+  for(uint32_t q = q0 + sub; q < q1; q += 4) {
+    const float4 v = *reinterpret_cast<const float4*>(vals + (uint64_t)q * 16 + 4 * a);
+    const double2 x01 = *reinterpret_cast<const double2*>(x + (uint64_t)cols[q] * 4);
+    const double2 x23 = *reinterpret_cast<const double2*>(x + (uint64_t)cols[q] * 4 + 2);
+    acc += (double)v.x * x01.x + (double)v.y * x01.y + (double)v.z * x23.x + (double)v.w * x23.y;
+  }
+  acc += __shfl_xor(acc, 4, 16);
+  acc += __shfl_xor(acc, 8, 16);
+  if(l < 4) y[(uint64_t)row * 4 + a] = acc;
+}
+
+__global__ void k_spmv_mixed_generic(uint32_t N, int neq, const uint32_t* __restrict__ row_ptr,
+                                     const uint32_t* __restrict__ cols, const float* __restrict__ vals,
+                                     const double* __restrict__ x, double* __restrict__ y)
+{
+  const uint64_t r = (uint64_t)blockIdx.x * blockDim.x + threadIdx.x;
+  if(r >= (uint64_t)N * neq) return;
+  const uint32_t i = (uint32_t)(r / neq);
+  const int a = (int)(r % neq);
+  const int nn = neq * neq;
+  double acc = 0.;
+  for(uint32_t q = row_ptr[i]; q < row_ptr[i + 1]; ++q) {
+    const float* blk = vals + (uint64_t)q * nn + a * neq;
+    const double* xb = x + (uint64_t)cols[q] * neq;
+    for(int b = 0; b < neq; ++b) acc += (double)blk[b] * xb[b];
+  }
+  y[r] = acc;
+}
+
 // any neq: one thread per scalar row
 template<typename T>
 __global__ void k_spmv_generic(uint32_t N, int neq, const uint32_t* __restrict__ row_ptr,
@@ -252,75 +292,159 @@ void lss_spmv32(Lss& L, float alpha, const float* x, float beta, float* y)
   timer_end(c, CF3HIP_KERNEL_SPMV_FP32);
 }
 
+void lss_spmv_mixed(Lss& L, const double* x, double* y)
+{
+  Context& c = *L.ctx;
+  if(!L.has_fp32) CF3_THROW(CF3HIP_ERR_NOT_CREATED, "FP32 values not enabled");
+  timer_begin(c, CF3HIP_KERNEL_SPMV_FP32);
+  if(L.neq == 4) {
+    hipLaunchKernelGGL(k_spmv4_mixed, dim3(grid_for((uint64_t)L.n_nodes * 16, 256)), dim3(256), 0, c.stream, L.n_nodes,
+                       L.row_ptr.as<uint32_t>(), L.cols.as<uint32_t>(), L.values32.as<float>(), x, y);
+  } else {
+    hipLaunchKernelGGL(k_spmv_mixed_generic, dim3(grid_for(L.dofs, 256)), dim3(256), 0, c.stream, L.n_nodes, L.neq,
+                       L.row_ptr.as<uint32_t>(), L.cols.as<uint32_t>(), L.values32.as<float>(), x, y);
+  }
+  HIP_OK(hipGetLastError());
+  timer_end(c, CF3HIP_KERNEL_SPMV_FP32);
+}
+
 void lss_dirichlet(Lss& L, uint32_t nbc, const uint32_t* nodes, const uint32_t* eqs, const double* values)
 {
   if(!nbc) return;
   Context& c = *L.ctx;
   const int neq = L.neq;
-  // host bookkeeping of the ordered list: first value (column elimination), last value (row)
-  std::unordered_map<uint64_t, std::pair<double, double> > dofv;
-  std::vector<uint64_t> order;
-  order.reserve(nbc);
+  // Host bookkeeping of the ordered list.  A dof constrained for the first time (in this call or an
+  // earlier one since the last reset) gets its column eliminated with that first value; every
+  // condition sets the row to the identity and the rhs / solution to its value, so the last value
+  // wins there (TrilinosFEVbrMatrix::symmetric_dirichlet leaves an already zero column untouched).
+  std::vector<uint64_t> fresh;              // dofs constrained for the first time by this call
+  std::unordered_map<uint64_t, double> last_here;
+  std::vector<uint64_t> here;               // dofs of this call in first-appearance order
   for(uint32_t b = 0; b < nbc; ++b) {
     if(nodes[b] >= L.n_nodes || eqs[b] >= (uint32_t)neq)
       CF3_THROW(CF3HIP_ERR_INVALID_ARG, "Dirichlet condition on a dof outside the system");
     const uint64_t d = (uint64_t)nodes[b] * neq + eqs[b];
-    auto it = dofv.find(d);
-    if(it == dofv.end()) { dofv.emplace(d, std::make_pair(values[b], values[b])); order.push_back(d); }
-    else it->second.second = values[b];
+    auto it = L.bc_host.find(d);
+    if(it == L.bc_host.end()) {
+      L.bc_host.emplace(d, std::make_pair(values[b], values[b]));
+      L.bc_order.push_back(d);
+      fresh.push_back(d);
+    } else {
+      it->second.second = values[b];
+    }
+    if(last_here.find(d) == last_here.end()) here.push_back(d);
+    last_here[d] = values[b];
   }
-  const uint32_t nd = (uint32_t)order.size();
-  std::vector<uint32_t> dof(nd);
-  std::vector<double> vfirst(nd), vlast(nd);
-  std::unordered_map<uint32_t, uint8_t> nmask;
-  for(uint32_t i = 0; i < nd; ++i) {
-    dof[i] = (uint32_t)order[i];
-    vfirst[i] = dofv[order[i]].first;
-    vlast[i] = dofv[order[i]].second;
-    nmask[dof[i] / neq] |= (uint8_t)(1u << (dof[i] % neq));
+  // this call: rows of its dofs, columns of its fresh dofs
+  const uint32_t nh = (uint32_t)here.size(), nf = (uint32_t)fresh.size();
+  std::vector<uint32_t> hdof(nh), fdof(nf);
+  std::vector<double> hval(nh), fval(nf);
+  for(uint32_t i = 0; i < nh; ++i) { hdof[i] = (uint32_t)here[i]; hval[i] = last_here[here[i]]; }
+  std::unordered_map<uint32_t, uint8_t> fmask;
+  for(uint32_t i = 0; i < nf; ++i) {
+    fdof[i] = (uint32_t)fresh[i];
+    fval[i] = L.bc_host[fresh[i]].first;
+    fmask[fdof[i] / neq] |= (uint8_t)(1u << (fdof[i] % neq));
   }
-  std::vector<uint32_t> mnode;
-  std::vector<uint8_t> mval;
-  mnode.reserve(nmask.size());
-  for(auto& kv : nmask) { mnode.push_back(kv.first); mval.push_back(kv.second); }
+  // persistent state of all conditions since the reset (used by the matrix-free residual)
+  const uint32_t nall = (uint32_t)L.bc_order.size();
+  std::vector<uint32_t> adof(nall);
+  std::vector<double> alast(nall);
+  std::unordered_map<uint32_t, uint8_t> amask;
+  for(uint32_t i = 0; i < nall; ++i) {
+    adof[i] = (uint32_t)L.bc_order[i];
+    alast[i] = L.bc_host[L.bc_order[i]].second;
+    amask[adof[i] / neq] |= (uint8_t)(1u << (adof[i] % neq));
+  }
+  if(!L.bc_mask.ptr) {
+    L.bc_mask.alloc(&c, L.n_nodes);
+    L.bc_newmask.alloc(&c, L.n_nodes);
+    L.bc_vfirst.alloc(&c, sizeof(double) * L.dofs);
+    hipLaunchKernelGGL(k_fill<uint8_t>, dim3(std::min<unsigned>(grid_for(L.n_nodes, 256), 4096)), dim3(256), 0,
+                       c.stream, L.bc_mask.as<uint8_t>(), (uint64_t)L.n_nodes, (uint8_t)0);
+  }
+  if(L.bc_dof.bytes < sizeof(uint32_t) * nall) {
+    L.bc_dof.alloc(&c, sizeof(uint32_t) * nall * 2);
+    L.bc_vlast.alloc(&c, sizeof(double) * nall * 2);
+  }
+  L.bc_n = nall;
+  std::vector<uint32_t> fnode, anode;
+  std::vector<uint8_t> fmv, amv;
+  for(auto& kv : fmask) { fnode.push_back(kv.first); fmv.push_back(kv.second); }
+  for(auto& kv : amask) { anode.push_back(kv.first); amv.push_back(kv.second); }
 
-  DevBuf d_dof, d_vf, d_vl, d_mnode, d_mval, d_mask, d_vfirst;
-  d_dof.alloc(&c, sizeof(uint32_t) * nd);
-  d_vf.alloc(&c, sizeof(double) * nd);
-  d_vl.alloc(&c, sizeof(double) * nd);
-  d_mnode.alloc(&c, sizeof(uint32_t) * mnode.size());
-  d_mval.alloc(&c, mval.size());
-  d_mask.alloc(&c, L.n_nodes);
-  d_vfirst.alloc(&c, sizeof(double) * L.dofs);
-  HIP_OK(hipMemcpyAsync(d_dof.ptr, dof.data(), sizeof(uint32_t) * nd, hipMemcpyHostToDevice, c.stream));
-  HIP_OK(hipMemcpyAsync(d_vf.ptr, vfirst.data(), sizeof(double) * nd, hipMemcpyHostToDevice, c.stream));
-  HIP_OK(hipMemcpyAsync(d_vl.ptr, vlast.data(), sizeof(double) * nd, hipMemcpyHostToDevice, c.stream));
-  HIP_OK(hipMemcpyAsync(d_mnode.ptr, mnode.data(), sizeof(uint32_t) * mnode.size(), hipMemcpyHostToDevice, c.stream));
-  HIP_OK(hipMemcpyAsync(d_mval.ptr, mval.data(), mval.size(), hipMemcpyHostToDevice, c.stream));
-  hipLaunchKernelGGL(k_fill<uint8_t>, dim3(std::min<unsigned>(grid_for(L.n_nodes, 256), 4096)), dim3(256), 0, c.stream,
-                     d_mask.as<uint8_t>(), (uint64_t)L.n_nodes, (uint8_t)0);
-  hipLaunchKernelGGL(k_scatter_mask, dim3(grid_for(mnode.size(), 256)), dim3(256), 0, c.stream, (uint32_t)mnode.size(),
-                     d_mnode.as<uint32_t>(), d_mval.as<uint8_t>(), d_mask.as<uint8_t>());
-  hipLaunchKernelGGL(k_scatter_values, dim3(grid_for(nd, 256)), dim3(256), 0, c.stream, nd, d_dof.as<uint32_t>(),
-                     d_vf.as<double>(), d_vfirst.as<double>());
-  hipLaunchKernelGGL(k_dirichlet_columns<double>, dim3(grid_for(L.n_nodes, 256)), dim3(256), 0, c.stream, L.n_nodes, neq,
-                     L.row_ptr.as<uint32_t>(), L.cols.as<uint32_t>(), L.values.as<double>(), d_mask.as<uint8_t>(),
-                     d_vfirst.as<double>(), L.rhs.as<double>());
-  hipLaunchKernelGGL(k_dirichlet_rows<double>, dim3(grid_for(nd, 256)), dim3(256), 0, c.stream, nd, neq, d_dof.as<uint32_t>(),
-                     d_vl.as<double>(), L.row_ptr.as<uint32_t>(), L.cols.as<uint32_t>(), L.values.as<double>(),
-                     L.rhs.as<double>(), L.sol.as<double>());
+  DevBuf d_hdof, d_hval, d_fdof, d_fval, d_fnode, d_fmv, d_anode, d_amv;
+  d_hdof.alloc(&c, sizeof(uint32_t) * nh);
+  d_hval.alloc(&c, sizeof(double) * nh);
+  d_fdof.alloc(&c, sizeof(uint32_t) * nf);
+  d_fval.alloc(&c, sizeof(double) * nf);
+  d_fnode.alloc(&c, sizeof(uint32_t) * fnode.size());
+  d_fmv.alloc(&c, fmv.size());
+  d_anode.alloc(&c, sizeof(uint32_t) * anode.size());
+  d_amv.alloc(&c, amv.size());
+  auto up = [&](DevBuf& b, const void* src, size_t bytes) {
+    if(bytes) HIP_OK(hipMemcpyAsync(b.ptr, src, bytes, hipMemcpyHostToDevice, c.stream));
+  };
+  up(d_hdof, hdof.data(), sizeof(uint32_t) * nh);
+  up(d_hval, hval.data(), sizeof(double) * nh);
+  up(d_fdof, fdof.data(), sizeof(uint32_t) * nf);
+  up(d_fval, fval.data(), sizeof(double) * nf);
+  up(d_fnode, fnode.data(), sizeof(uint32_t) * fnode.size());
+  up(d_fmv, fmv.data(), fmv.size());
+  up(d_anode, anode.data(), sizeof(uint32_t) * anode.size());
+  up(d_amv, amv.data(), amv.size());
+  up(L.bc_dof, adof.data(), sizeof(uint32_t) * nall);
+  up(L.bc_vlast, alast.data(), sizeof(double) * nall);
+  const unsigned gN = std::min<unsigned>(grid_for(L.n_nodes, 256), 4096);
+  hipLaunchKernelGGL(k_fill<uint8_t>, dim3(gN), dim3(256), 0, c.stream, L.bc_newmask.as<uint8_t>(),
+                     (uint64_t)L.n_nodes, (uint8_t)0);
+  if(!fnode.empty())
+    hipLaunchKernelGGL(k_scatter_mask, dim3(grid_for(fnode.size(), 256)), dim3(256), 0, c.stream,
+                       (uint32_t)fnode.size(), d_fnode.as<uint32_t>(), d_fmv.as<uint8_t>(), L.bc_newmask.as<uint8_t>());
+  if(!anode.empty())
+    hipLaunchKernelGGL(k_scatter_mask, dim3(grid_for(anode.size(), 256)), dim3(256), 0, c.stream,
+                       (uint32_t)anode.size(), d_anode.as<uint32_t>(), d_amv.as<uint8_t>(), L.bc_mask.as<uint8_t>());
+  if(nf)
+    hipLaunchKernelGGL(k_scatter_values, dim3(grid_for(nf, 256)), dim3(256), 0, c.stream, nf, d_fdof.as<uint32_t>(),
+                       d_fval.as<double>(), L.bc_vfirst.as<double>());
+  if(nf)
+    hipLaunchKernelGGL(k_dirichlet_columns<double>, dim3(grid_for(L.n_nodes, 256)), dim3(256), 0, c.stream, L.n_nodes,
+                       neq, L.row_ptr.as<uint32_t>(), L.cols.as<uint32_t>(), L.values.as<double>(),
+                       L.bc_newmask.as<uint8_t>(), L.bc_vfirst.as<double>(), L.rhs.as<double>());
+  if(nh)
+    hipLaunchKernelGGL(k_dirichlet_rows<double>, dim3(grid_for(nh, 256)), dim3(256), 0, c.stream, nh, neq,
+                       d_hdof.as<uint32_t>(), d_hval.as<double>(), L.row_ptr.as<uint32_t>(), L.cols.as<uint32_t>(),
+                       L.values.as<double>(), L.rhs.as<double>(), L.sol.as<double>());
   HIP_OK(hipGetLastError());
   if(L.has_fp32) {
-    // the FP32 mirror receives the same elimination (its right-hand side is the FP64 one)
-    hipLaunchKernelGGL(k_dirichlet_columns<float>, dim3(grid_for(L.n_nodes, 256)), dim3(256), 0, c.stream, L.n_nodes,
-                       neq, L.row_ptr.as<uint32_t>(), L.cols.as<uint32_t>(), L.values32.as<float>(),
-                       d_mask.as<uint8_t>(), d_vfirst.as<double>(), (double*)nullptr);
-    hipLaunchKernelGGL(k_dirichlet_rows<float>, dim3(grid_for(nd, 256)), dim3(256), 0, c.stream, nd, neq,
-                       d_dof.as<uint32_t>(), d_vl.as<double>(), L.row_ptr.as<uint32_t>(), L.cols.as<uint32_t>(),
-                       L.values32.as<float>(), (double*)nullptr, (double*)nullptr);
+    // the FP32 values receive the same elimination (the right-hand side is the FP64 one)
+    if(nf)
+      hipLaunchKernelGGL(k_dirichlet_columns<float>, dim3(grid_for(L.n_nodes, 256)), dim3(256), 0, c.stream,
+                         L.n_nodes, neq, L.row_ptr.as<uint32_t>(), L.cols.as<uint32_t>(), L.values32.as<float>(),
+                         L.bc_newmask.as<uint8_t>(), L.bc_vfirst.as<double>(), (double*)nullptr);
+    if(nh)
+      hipLaunchKernelGGL(k_dirichlet_rows<float>, dim3(grid_for(nh, 256)), dim3(256), 0, c.stream, nh, neq,
+                         d_hdof.as<uint32_t>(), d_hval.as<double>(), L.row_ptr.as<uint32_t>(), L.cols.as<uint32_t>(),
+                         L.values32.as<float>(), (double*)nullptr, (double*)nullptr);
     HIP_OK(hipGetLastError());
   }
   HIP_OK(hipStreamSynchronize(c.stream));
+}
+
+void lss_clear_conditions(Lss& L)
+{
+  L.bc_host.clear();
+  L.bc_order.clear();
+  L.bc_n = 0;
+  if(L.bc_mask.ptr) lss_fill8(*L.ctx, L.bc_mask.as<uint8_t>(), L.n_nodes, 0);
+}
+
+void lss_fill8(Context& c, uint8_t* p, size_t n, uint8_t v)
+{
+  if(!n) return;
+  hipLaunchKernelGGL(k_fill<uint8_t>, dim3(std::min<unsigned>(grid_for(n, 256), 4096)), dim3(256), 0, c.stream, p,
+                     (uint64_t)n, v);
+  HIP_OK(hipGetLastError());
 }
 
 void lss_point_csr(Lss& L, uint32_t* prow, uint32_t* pcol, double* pval)

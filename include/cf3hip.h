@@ -208,6 +208,17 @@ int cf3hip_ns_linearize(cf3hip_fields fields);
 int cf3hip_ns_assemble(cf3hip_lss lss, cf3hip_fields fields, const cf3hip_ns_params* prm);
 /* UpdateActions (NavierStokes.cpp:172-180): u3=u2, u2=u1, u1=u, u += du, p += dp from the solution */
 int cf3hip_ns_update(cf3hip_lss lss, cf3hip_fields fields);
+/* SolveLSS of the NS system (SolveLSS.cpp:44-55).  precision FP64: the stored FP64 system (as
+ * cf3hip_lss_solve).  precision FP32: BiCGStab on the FP32 element matrices (FP32 values, FP64
+ * vectors and accumulation) with reliable updates against the FP64 residual b' - A' x evaluated
+ * matrix-free from the FP64 element matrices of the same fields, after Dirichlet conditions applied
+ * with cf3hip_lss_dirichlet; needs only the FP32 assembly of the step. */
+int cf3hip_ns_solve(cf3hip_lss lss, cf3hip_fields fields, const cf3hip_ns_params* ns_prm,
+                    const cf3hip_solver_params* prm, cf3hip_solve_stats* stats);
+/* matrix-free FP64 residual r = b' - A' x of the NS system with its Dirichlet conditions
+ * (device pointers, dofs entries) -- no counterpart in the reference */
+int cf3hip_ns_residual(cf3hip_lss lss, cf3hip_fields fields, const cf3hip_ns_params* ns_prm, const double* x_dev,
+                       double* r_dev);
 
 #ifdef __cplusplus
 }
