@@ -1,14 +1,15 @@
 // UFEM Navier-Stokes element assembly fused with a deterministic, atomic-free scatter into the LSS
 // block-CSR matrix (gfx950).
 //
-// Work decomposition (row owner): a 256-thread workgroup owns a contiguous range of node rows and
-// all their incidences (element e, local node a).  Each thread takes one incidence, computes the
-// element quantities it needs from coalesced SoA node data, then produces the element row block
+// Work decomposition (row owner): a kAsmThreads-thread workgroup owns a contiguous range of node rows
+// and all their incidences (element e, local node a).  Each thread takes one incidence, gathers the
+// element's nodes from packed node records (double4: coordinates, state (u.., p), advection velocity
+// and viscosity), computes the element quantities it needs, then produces the element row block
 // M(a, b) = invdt*T(a,b) + theta*A'(a,b) one column node b at a time.  Per pass b the blocks are
-// staged in LDS and the owner threads of every output value sum the contributions of their slot,
-// walking the precomputed per-slot contribution list (sorted by (b, q)): a sorted segmented
+// staged in LDS and the owner thread of every CSR block sums the contributions of its slot, walking
+// the precomputed per-slot contribution list (sorted by (b, q), staged in LDS): a sorted segmented
 // reduction with a fixed order, so results are deterministic and no floating-point atomics are used.
-// Every CSR value and rhs entry is written exactly once, coalesced.
+// Every CSR value and rhs entry is written exactly once.
 //
 // Element mathematics (reference files relative to the coolfluid3 tree):
 //   specialised P1 triangle / tetrahedron: plugins/UFEM/src/UFEM/NavierStokesSpecializations.hpp:38-303
@@ -16,7 +17,8 @@
 //   generic expression (use_specializations = false): NavierStokesAssembly.hpp:44-66, order-2 Gauss
 //   tau METRIC / TEZDUYAR: plugins/UFEM/src/UFEM/SUPG.hpp:141-219 (gij = J^-T J^-1 = Gram matrix of
 //     the shape-function gradients of nodes 1..d for affine elements)
-//   combine: NavierStokesAssembly.hpp:69-71 (rhs = -A x + a before the p rows are divided by theta)
+//   combine: NavierStokesAssembly.hpp:69-71 (rhs = -A x + a before the p rows are divided by theta;
+//     theta * (A_p / theta) is formed as (theta * (1/theta)) * A_p, equal to within one rounding)
 #include <type_traits>
 
 #include "internal.hpp"
@@ -25,19 +27,13 @@ namespace cf3hip {
 
 namespace {
 
-constexpr int kThreads = 256;   // one incidence and at most one owned CSR block per thread
-
-struct NodeData {
-  const double* x[3];
-  const double* u[3];
-  const double* ua[3];
-  const double* p;
-  const double* nu;
-  const double* g[3];
-};
+constexpr int kThreads = kAsmThreads;   // one incidence and at most one owned CSR block per thread
 
 struct AsmArgs {
-  NodeData nd;
+  const double4* xp;   // packed coordinates (x, y, z | 0, 0)
+  const double4* st;   // packed state (u, v, w, p) in 3D, (u, v, p, 0) in 2D
+  const double4* ad;   // packed (u_adv x, y, z | 0, nu_eff)
+  const double* g[3];  // body force (SoA), generic expression only
   const uint32_t* conn;
   const uint32_t* wg_rows;
   const uint32_t* row_ptr;
@@ -49,19 +45,52 @@ struct AsmArgs {
   double* rhs;
   const double* xt;    // matrix-free residual: x with constrained columns set to their first value
   double* res;         // matrix-free residual output
-  double rho, dt, theta, invdt;
+  double rho, inv_rho, dt, theta, thq, invdt, f4dt2;   // thq = theta * (1 / theta); f4dt2 = 4 / dt^2
   int supg_type;
   double c1, c2, a_ps, a_su, a_bu;
   int* status;
 };
 
 template<int B, int K, typename F>
-__device__ inline void static_for(F&& f)
+__device__ __forceinline__ void static_for(F&& f)
 {
   if constexpr(B < K) {
     f(std::integral_constant<int, B>{});
     static_for<B + 1, K>(f);
   }
+}
+
+// one LDS row of a staged element block: 16-byte aligned, odd number of 16-byte chunks (bank spread)
+template<typename VR, int NN>
+constexpr int lds_row()
+{
+  constexpr int bytes = (NN * (int)sizeof(VR) + 15) / 16 * 16;
+  constexpr int padded = (bytes / 16) % 2 == 0 ? bytes + 16 : bytes;
+  return padded / (int)sizeof(VR);
+}
+
+__device__ __forceinline__ void unpack(const double4& v, double o[4]) { o[0] = v.x; o[1] = v.y; o[2] = v.z; o[3] = v.w; }
+
+template<int K>
+__device__ __forceinline__ void load_conn(const uint32_t* conn, uint32_t e, uint32_t cn[K])
+{
+  if constexpr(K == 4) {
+    const uint4 c = reinterpret_cast<const uint4*>(conn)[e];
+    cn[0] = c.x; cn[1] = c.y; cn[2] = c.z; cn[3] = c.w;
+  } else {
+#pragma unroll
+    for(int m = 0; m < K; ++m) cn[m] = conn[(uint64_t)e * K + m];
+  }
+}
+
+// nodal unknowns (u.., p) of node nd in arithmetic type R
+template<int NEQ, typename R>
+__device__ __forceinline__ void load_state(const AsmArgs& g, uint32_t nd, R xb[NEQ])
+{
+  double s[4];
+  unpack(g.st[nd], s);
+#pragma unroll
+  for(int c = 0; c < NEQ; ++c) xb[c] = (R)s[c];
 }
 
 // ---------------------------------------------------------------------------------------------
@@ -70,18 +99,17 @@ __device__ inline void static_for(F&& f)
 template<int D, typename R>
 struct SpecP1 {
   static constexpr int K = D + 1, NEQ = D + 1, NN = NEQ * NEQ;
+  uint32_t cn[K];
   R n[K][D];        // reference face normals (area vectors for D = 3, rotated edges for D = 2)
-  R na[D];          // normal of the row node a
-  R V;
-  R tps;
   R ua[D];          // mean advection velocity u_avg
-  R xb[K][NEQ];     // element nodal (u.., p) for -A x
+  R na[D];          // normal of the row node a
+  R Q[D];           // velocity-row coefficient: skew-symmetric advection + stress diffusion + bulk
+  R cc, nudV, cpres, tpdV, pn, pp, tdo, tdd, tpc;
   int a;
-  R u_ni, cskew, cpres, time_ii, tp_dV, ts_dV, nu_dV, tb_dV, ni_ut;
 
   // geometry in type T: normals and volume
   template<typename T>
-  __device__ static void geometry(const T X[K][D], T nrm[K][D], T& vol)
+  __device__ __forceinline__ static void geometry(const T X[K][D], T nrm[K][D], T& vol)
   {
     if constexpr(D == 2) {
       nrm[0][0] = X[1][1] - X[2][1]; nrm[0][1] = X[2][0] - X[1][0];
@@ -90,60 +118,55 @@ struct SpecP1 {
       vol = T(0.5) * ((X[1][0] - X[0][0]) * (X[2][1] - X[0][1]) - (X[2][0] - X[0][0]) * (X[1][1] - X[0][1]));
     } else {
       // NavierStokesSpecializations.hpp:162-173
-      nrm[0][0] = ((X[3][1]-X[1][1])*(X[2][2]-X[1][2]) - (X[2][1]-X[1][1])*(X[3][2]-X[1][2])) / T(2);
-      nrm[0][1] = ((X[3][2]-X[1][2])*(X[2][0]-X[1][0]) - (X[2][2]-X[1][2])*(X[3][0]-X[1][0])) / T(2);
-      nrm[0][2] = ((X[3][0]-X[1][0])*(X[2][1]-X[1][1]) - (X[2][0]-X[1][0])*(X[3][1]-X[1][1])) / T(2);
-      nrm[1][0] = ((X[2][1]-X[0][1])*(X[3][2]-X[0][2]) - (X[3][1]-X[0][1])*(X[2][2]-X[0][2])) / T(2);
-      nrm[1][1] = ((X[2][2]-X[0][2])*(X[3][0]-X[0][0]) - (X[3][2]-X[0][2])*(X[2][0]-X[0][0])) / T(2);
-      nrm[1][2] = ((X[2][0]-X[0][0])*(X[3][1]-X[0][1]) - (X[3][0]-X[0][0])*(X[2][1]-X[0][1])) / T(2);
-      nrm[2][0] = ((X[3][1]-X[0][1])*(X[1][2]-X[0][2]) - (X[1][1]-X[0][1])*(X[3][2]-X[0][2])) / T(2);
-      nrm[2][1] = ((X[3][2]-X[0][2])*(X[1][0]-X[0][0]) - (X[1][2]-X[0][2])*(X[3][0]-X[0][0])) / T(2);
-      nrm[2][2] = ((X[3][0]-X[0][0])*(X[1][1]-X[0][1]) - (X[1][0]-X[0][0])*(X[3][1]-X[0][1])) / T(2);
-      nrm[3][0] = ((X[1][1]-X[0][1])*(X[2][2]-X[0][2]) - (X[2][1]-X[0][1])*(X[1][2]-X[0][2])) / T(2);
-      nrm[3][1] = ((X[1][2]-X[0][2])*(X[2][0]-X[0][0]) - (X[2][2]-X[0][2])*(X[1][0]-X[0][0])) / T(2);
-      nrm[3][2] = ((X[1][0]-X[0][0])*(X[2][1]-X[0][1]) - (X[2][0]-X[0][0])*(X[1][1]-X[0][1])) / T(2);
+      nrm[0][0] = T(0.5) * ((X[3][1]-X[1][1])*(X[2][2]-X[1][2]) - (X[2][1]-X[1][1])*(X[3][2]-X[1][2]));
+      nrm[0][1] = T(0.5) * ((X[3][2]-X[1][2])*(X[2][0]-X[1][0]) - (X[2][2]-X[1][2])*(X[3][0]-X[1][0]));
+      nrm[0][2] = T(0.5) * ((X[3][0]-X[1][0])*(X[2][1]-X[1][1]) - (X[2][0]-X[1][0])*(X[3][1]-X[1][1]));
+      nrm[1][0] = T(0.5) * ((X[2][1]-X[0][1])*(X[3][2]-X[0][2]) - (X[3][1]-X[0][1])*(X[2][2]-X[0][2]));
+      nrm[1][1] = T(0.5) * ((X[2][2]-X[0][2])*(X[3][0]-X[0][0]) - (X[3][2]-X[0][2])*(X[2][0]-X[0][0]));
+      nrm[1][2] = T(0.5) * ((X[2][0]-X[0][0])*(X[3][1]-X[0][1]) - (X[3][0]-X[0][0])*(X[2][1]-X[0][1]));
+      nrm[2][0] = T(0.5) * ((X[3][1]-X[0][1])*(X[1][2]-X[0][2]) - (X[1][1]-X[0][1])*(X[3][2]-X[0][2]));
+      nrm[2][1] = T(0.5) * ((X[3][2]-X[0][2])*(X[1][0]-X[0][0]) - (X[1][2]-X[0][2])*(X[3][0]-X[0][0]));
+      nrm[2][2] = T(0.5) * ((X[3][0]-X[0][0])*(X[1][1]-X[0][1]) - (X[1][0]-X[0][0])*(X[3][1]-X[0][1]));
+      nrm[3][0] = T(0.5) * ((X[1][1]-X[0][1])*(X[2][2]-X[0][2]) - (X[2][1]-X[0][1])*(X[1][2]-X[0][2]));
+      nrm[3][1] = T(0.5) * ((X[1][2]-X[0][2])*(X[2][0]-X[0][0]) - (X[2][2]-X[0][2])*(X[1][0]-X[0][0]));
+      nrm[3][2] = T(0.5) * ((X[1][0]-X[0][0])*(X[2][1]-X[0][1]) - (X[2][0]-X[0][0])*(X[1][1]-X[0][1]));
       // det J = 2 (x3 - x0) . n_3 ; V = det / 6
-      vol = (T(2) * ((X[3][0]-X[0][0])*nrm[3][0] + (X[3][1]-X[0][1])*nrm[3][1] + (X[3][2]-X[0][2])*nrm[3][2])) / T(6);
+      vol = (T(2) * ((X[3][0]-X[0][0])*nrm[3][0] + (X[3][1]-X[0][1])*nrm[3][1] + (X[3][2]-X[0][2])*nrm[3][2]))
+            * T(1.0 / 6.0);
     }
   }
 
-  // METRIC tau from the velocity at the centroid (SUPG.hpp:176-190); gij = n_i.n_l / (D^2 V^2)
+  // METRIC tau from the velocity at the centroid (SUPG.hpp:176-190); gij = n_i.n_l / (D^2 V^2).
+  // s = 1 / (D V).  c1 = 1, c2 = 16, alpha = 1 on the specialised path.
   template<typename T>
-  __device__ static void tau_metric(const T nrm[K][D], T vol, const T uc[D], T nu, T dt, T& t_ps, T& t_su, T& t_bu)
+  __device__ __forceinline__ static void tau_metric(const T nrm[K][D], T s, const T uc[D], T nu, T f4dt2, T& t_su, T& t_bu)
   {
-    const T s = T(1) / (T(D) * vol);
     T gn[D][D];   // grad N of nodes 1..D
     for(int i = 0; i < D; ++i)
       for(int d = 0; d < D; ++d) gn[i][d] = nrm[i + 1][d] * s;
-    T w[D];
+    T adv = T(0);
     for(int d = 0; d < D; ++d) {
       T acc = T(0);
       for(int i = 0; i < D; ++i) acc += uc[i] * gn[i][d];
-      w[d] = acc;
+      adv += acc * acc;
     }
-    T adv = T(0);
-    for(int d = 0; d < D; ++d) adv += w[d] * w[d];
     T gsq = T(0), tr = T(0);
     for(int i = 0; i < D; ++i)
-      for(int l = 0; l < D; ++l) {
+      for(int l = i; l < D; ++l) {
         T gil = T(0);
         for(int d = 0; d < D; ++d) gil += gn[i][d] * gn[l][d];
-        gsq += gil * gil;
+        gsq += (i == l ? T(1) : T(2)) * gil * gil;
         if(i == l) tr += gil;
       }
-    const T diff = nu * nu * gsq;
-    const T sq = sqrt(T(4) / (dt * dt) + adv + T(16) * diff);   // c1 = 1, c2 = 16 on the specialised path
+    const T sq = sqrt(f4dt2 + adv + T(16) * (nu * nu * gsq));
     t_su = T(1) / sq;
-    t_ps = t_su;
     t_bu = sq / tr;
   }
 
-  __device__ void prepare(const AsmArgs& g, uint32_t e, int a_)
+  __device__ __forceinline__ void prepare(const AsmArgs& g, uint32_t e, int a_)
   {
     a = a_;
-    uint32_t cn[K];
-#pragma unroll
-    for(int m = 0; m < K; ++m) cn[m] = g.conn[(uint64_t)e * K + m];
+    load_conn<K>(g.conn, e, cn);
     double X[K][D];
     double uc[D];
     double nus = 0.;
@@ -151,25 +174,25 @@ struct SpecP1 {
     for(int d = 0; d < D; ++d) { uc[d] = 0.; uas[d] = R(0); }
 #pragma unroll
     for(int m = 0; m < K; ++m) {
-      const uint32_t nd = cn[m];
+      double xp[4], st[4], ad[4];
+      unpack(g.xp[cn[m]], xp);
+      unpack(g.st[cn[m]], st);
+      unpack(g.ad[cn[m]], ad);
 #pragma unroll
       for(int d = 0; d < D; ++d) {
-        X[m][d] = g.nd.x[d][nd];
-        const double um = g.nd.u[d][nd];
-        uc[d] += um;
-        uas[d] += (R)g.nd.ua[d][nd];
-        xb[m][d] = (R)um;
+        X[m][d] = xp[d];
+        uc[d] += st[d];
+        uas[d] += (R)ad[d];
       }
-      xb[m][D] = (R)g.nd.p[nd];
-      nus += g.nd.nu[nd];
+      nus += ad[3];
     }
     for(int d = 0; d < D; ++d) {
-      uc[d] = uc[d] / K;
-      ua[d] = uas[d] / R(K);
+      uc[d] = uc[d] * (1.0 / K);
+      ua[d] = uas[d] * R(1.0 / K);
     }
-    const double nu_mean = fabs(nus / K);
+    const double nu_mean = fabs(nus * (1.0 / K));
     const R nu = (R)nu_mean;
-    R tsu, tbu;
+    R V, invV, tsu, tbu;
     bool fp64_geom = sizeof(R) == sizeof(double);
     if constexpr(sizeof(R) != sizeof(double)) {
       R Xr[K][D];
@@ -181,23 +204,25 @@ struct SpecP1 {
       for(int i = 0; i < K; ++i)
         for(int j = i + 1; j < K; ++j)
           for(int d = 0; d < D; ++d) { const R df = Xr[i][d] - Xr[j][d]; l2 += df * df; }
-      const R lm = sqrt(l2 / R(K * (K - 1) / 2));
+      const R lm = sqrt(l2 * R(2.0 / (K * (K - 1))));
       R lp = lm;
       for(int d = 1; d < D; ++d) lp *= lm;
       if(fabs(V) < R(0.05) * lp) fp64_geom = true;
     }
     if(fp64_geom) {
-      double n64[K][D], V64, tp, ts, tb;
+      double n64[K][D], V64, ts, tb;
       geometry<double>(X, n64, V64);
-      tau_metric<double>(n64, V64, uc, nu_mean, g.dt, tp, ts, tb);
+      const double iv = 1. / V64;
+      tau_metric<double>(n64, iv * (1.0 / D), uc, nu_mean, g.f4dt2, ts, tb);
       for(int m = 0; m < K; ++m)
         for(int d = 0; d < D; ++d) n[m][d] = (R)n64[m][d];
-      V = (R)V64; tps = (R)tp; tsu = (R)ts; tbu = (R)tb;
+      V = (R)V64; invV = (R)iv; tsu = (R)ts; tbu = (R)tb;
       if(!(V64 > 0.)) g.status[0] = 1;   // every writer stores the same flag value
     } else {
       R ucr[D];
       for(int d = 0; d < D; ++d) ucr[d] = (R)uc[d];
-      tau_metric<R>(n, V, ucr, nu, (R)g.dt, tps, tsu, tbu);
+      invV = R(1) / V;
+      tau_metric<R>(n, invV * R(1.0 / D), ucr, nu, (R)g.f4dt2, tsu, tbu);
       if(!(V > R(0))) g.status[0] = 1;
     }
     // normal of the row node, selected without dynamic register indexing
@@ -208,62 +233,83 @@ struct SpecP1 {
       if(m == a)
 #pragma unroll
         for(int d = 0; d < D; ++d) na[d] = n[m][d];
-    // per-row (node a) coefficients
-    const R c0 = D == 2 ? R(1) / R(6) : R(1) / R(12);
-    const R rdV = R(1) / (R(D * D) * V);
-    ts_dV = tsu * rdV;
-    tp_dV = tps * rdV;
-    nu_dV = nu * rdV;
-    tb_dV = tbu * rdV;
-    u_ni = R(0);
+    // per-row (node a) coefficients (NavierStokesSpecializations.hpp:66-145 / 196-301)
+    const R c0 = R(D == 2 ? 1.0 / 6.0 : 1.0 / 12.0);
+    const R rdV = invV * R(1.0 / (D * D));
+    const R tsdV = tsu * rdV;
+    nudV = nu * rdV;
+    const R tbdV = tbu * rdV;
+    R u_ni = R(0);
     for(int d = 0; d < D; ++d) u_ni += ua[d] * na[d];
-    cskew = R(0.5) * c0 + R(0.5) * ts_dV * u_ni;
-    cpres = c0 / (R)g.rho + ts_dV / (R)g.rho * u_ni;
-    time_ii = tsu * c0 * u_ni;
-    // n_i . u~ with the tetrahedron substitution of vk for wk (NavierStokesSpecializations.hpp:236-238)
-    ni_ut = na[0] * ua[0] + na[1] * ua[1];
+    const R cskew = R(0.5) * c0 + R(0.5) * tsdV * u_ni;
+    cc = c0 + tsdV * u_ni;
+    const R inv_rho = (R)g.inv_rho;
+    cpres = cc * inv_rho;
+    const R time_ii = tsu * c0 * u_ni;
+    // n_i . u~ with the tetrahedron substitution of vk for wk (lines 236-238)
+    R ni_ut = na[0] * ua[0] + na[1] * ua[1];
     if constexpr(D == 3) ni_ut += na[2] * ua[1];
+    const R visc = nudV * R(1.0 / 3.0) + tbdV;
+#pragma unroll
+    for(int r = 0; r < D; ++r) {
+      const R uk = ua[(D == 3 && r == 2) ? 1 : r];   // W rows use vk (lines 224-226)
+      Q[r] = cskew * uk + visc * na[r];
+    }
+    tpdV = tsdV;   // tau_ps = tau_su
+    pn = R(0.5) * tpdV * ni_ut + c0;
+    pp = tpdV * inv_rho;
+    const R m0 = V * R(D == 2 ? 1.0 / 12.0 : 1.0 / 20.0);
+    const R invdt = (R)g.invdt;
+    tdo = invdt * (m0 + time_ii);
+    tdd = invdt * (R(2) * m0 + time_ii);
+    tpc = invdt * tsu * c0;
   }
 
-  // block (a, B): M = invdt*T + theta*A' into m[NN] (row-major); rhs -= A x_B
-  template<int B, typename VR>
-  __device__ void block(const AsmArgs& g, VR* m, R* rhs) const
+  // block (a, B): M = invdt*T + theta*A' into m[NN] (row-major); with RHS, rhs -= A x_B
+  template<int B, bool RHS, typename VR>
+  __device__ __forceinline__ void block(const AsmArgs& g, VR* m, R* rhs, const R* xb) const
   {
-    const R c0 = D == 2 ? R(1) / R(6) : R(1) / R(12);
     const R* nj = n[B];
     R uknj = R(0), ninj = R(0);
+#pragma unroll
     for(int d = 0; d < D; ++d) { uknj += ua[d] * nj[d]; ninj += na[d] * nj[d]; }
-    const R conv = c0 * uknj + ts_dV * uknj * u_ni;
-    const R uk_row[3] = {ua[0], ua[1], ua[D == 3 ? 1 : 0]};   // W rows use vk (lines 224-226)
-    R A[NEQ][NEQ];
+    const R dsum = uknj * cc + nudV * ninj;   // convection (std + SUPG) + diffusion n_a.n_b
+    const R td = a == B ? tdd : tdo;
+    const R th = (R)g.theta, thq = (R)g.thq;
+#pragma unroll
     for(int r = 0; r < D; ++r) {
-      for(int c = 0; c < D; ++c) {
-        R v = cskew * uk_row[r] * nj[c] + (nu_dV / R(3) + tb_dV) * na[r] * nj[c];
-        if(r == c) v += conv + nu_dV * ninj;
-        A[r][c] = v;
+      R Ar[NEQ];
+#pragma unroll
+      for(int c = 0; c < D; ++c) Ar[c] = Q[r] * nj[c] + (r == c ? dsum : R(0));
+      Ar[D] = cpres * nj[r];
+      if constexpr(RHS) {
+        R s = R(0);
+#pragma unroll
+        for(int c = 0; c < NEQ; ++c) s += Ar[c] * xb[c];
+        rhs[r] -= s;
       }
-      A[r][D] = nj[r] * cpres;
+#pragma unroll
+      for(int c = 0; c < D; ++c) m[r * NEQ + c] = (VR)(th * Ar[c] + (r == c ? td : R(0)));
+      m[r * NEQ + D] = (VR)(th * Ar[D]);
     }
-    for(int c = 0; c < D; ++c)
-      A[D][c] = tp_dV * uknj * na[c] + R(0.5) * tp_dV * ni_ut * nj[c] + c0 * nj[c];
-    A[D][D] = tp_dV / (R)g.rho * ninj;
-    const R mass = (D == 2 ? V / R(12) : V / R(20)) * (a == B ? R(2) : R(1));
-    const R tdiag = mass + time_ii;
-    const R invdt = (R)g.invdt, theta = (R)g.theta;
-    for(int r = 0; r < NEQ; ++r) {
-      R s = R(0);
-      for(int c = 0; c < NEQ; ++c) s += A[r][c] * xb[B][c];
-      rhs[r] -= s;
+    {
+      R Ar[NEQ];
+#pragma unroll
+      for(int c = 0; c < D; ++c) Ar[c] = uknj * (tpdV * na[c]) + pn * nj[c];
+      Ar[D] = pp * ninj;
+      if constexpr(RHS) {
+        R s = R(0);
+#pragma unroll
+        for(int c = 0; c < NEQ; ++c) s += Ar[c] * xb[c];
+        rhs[D] -= s;
+      }
+#pragma unroll
+      for(int c = 0; c < D; ++c) m[D * NEQ + c] = (VR)(tpc * na[c] + thq * Ar[c]);
+      m[D * NEQ + D] = (VR)(thq * Ar[D]);
     }
-    for(int r = 0; r < D; ++r)
-      for(int c = 0; c < NEQ; ++c)
-        m[r * NEQ + c] = (VR)((r == c ? invdt * tdiag : R(0)) + theta * A[r][c]);
-    for(int c = 0; c < D; ++c)
-      m[D * NEQ + c] = (VR)(invdt * (tps * c0 * na[c]) + theta * (A[D][c] / theta));
-    m[D * NEQ + D] = (VR)(theta * (A[D][D] / theta));
   }
 
-  __device__ void rhs_source(const AsmArgs&, R*) const {}
+  __device__ __forceinline__ void rhs_source(const AsmArgs&, R*) const {}
 };
 
 // ---------------------------------------------------------------------------------------------
@@ -274,17 +320,17 @@ template<int D>
 struct GenericP1 {
   typedef double R;
   static constexpr int K = D + 1, NEQ = D + 1, NN = NEQ * NEQ, NQ = D == 2 ? 3 : 4;
+  uint32_t cn[K];
   R G[D][K];         // physical gradients
   R Ga[D];           // gradient of the row node a
   R detw;            // |J| (weights applied per point)
   R ua_n[K][D];      // nodal advection velocity
   R nu_n[K];
-  R g_n[K][D];       // nodal body force
-  R xb[K][NEQ];
   R tps, tsu, tbu;
+  R oh[K];           // one-hot of the row node a
   int a;
 
-  __device__ static void gauss(int q, R xi[D], R& w)
+  __device__ __forceinline__ static void gauss(int q, R xi[D], R& w)
   {
     if constexpr(D == 2) {
       xi[0] = q == 2 ? 0.0 : 0.5;
@@ -299,49 +345,50 @@ struct GenericP1 {
     }
   }
 
-  __device__ static void shape(const R xi[D], R N[K])
+  __device__ __forceinline__ static void shape(const R xi[D], R N[K])
   {
     R s = 1.;
-    #pragma unroll
+#pragma unroll
     for(int d = 0; d < D; ++d) { N[d + 1] = xi[d]; s -= xi[d]; }
     N[0] = s;
   }
 
-  __device__ R select_a(const R v[K]) const
+  // v[a] as a one-hot weighted sum: a select chain over a local array is turned into a dynamically
+  // indexed (scratch) load by the optimiser
+  __device__ __forceinline__ R select_a(const R v[K]) const
   {
-    R r = v[0];
+    R r = 0.;
 #pragma unroll
-    for(int m = 1; m < K; ++m) if(m == a) r = v[m];
+    for(int m = 0; m < K; ++m) r += oh[m] * v[m];
     return r;
   }
 
-  __device__ void prepare(const AsmArgs& g, uint32_t e, int a_)
+  __device__ __forceinline__ void prepare(const AsmArgs& g, uint32_t e, int a_)
   {
     a = a_;
-    uint32_t cn[K];
 #pragma unroll
-    for(int m = 0; m < K; ++m) cn[m] = g.conn[(uint64_t)e * K + m];
+    for(int m = 0; m < K; ++m) oh[m] = m == a ? 1. : 0.;
+    load_conn<K>(g.conn, e, cn);
     R X[K][D];
     R nus = 0.;
 #pragma unroll
     for(int m = 0; m < K; ++m) {
-      const uint32_t nd = cn[m];
+      double xp[4], ad[4];
+      unpack(g.xp[cn[m]], xp);
+      unpack(g.ad[cn[m]], ad);
 #pragma unroll
       for(int d = 0; d < D; ++d) {
-        X[m][d] = g.nd.x[d][nd];
-        ua_n[m][d] = g.nd.ua[d][nd];
-        g_n[m][d] = g.nd.g[d][nd];
-        xb[m][d] = g.nd.u[d][nd];
+        X[m][d] = xp[d];
+        ua_n[m][d] = ad[d];
       }
-      xb[m][D] = g.nd.p[nd];
-      nu_n[m] = g.nd.nu[nd];
+      nu_n[m] = ad[3];
       nus += nu_n[m];
     }
     // J(ksi, x) = x_{ksi+1} - x_0, inverse by the adjugate
     R J[D][D];
-    #pragma unroll
+#pragma unroll
     for(int i = 0; i < D; ++i)
-      #pragma unroll
+#pragma unroll
       for(int d = 0; d < D; ++d) J[i][d] = X[i + 1][d] - X[0][d];
     R Ji[D][D], det;
     if constexpr(D == 2) {
@@ -366,66 +413,65 @@ struct GenericP1 {
     if(!(det > 0.)) g.status[0] = 1;
     detw = det;
     // grad N = Jinv * dN/dksi ; dN/dksi of node 0 = -1, node m+1 = e_m
-    #pragma unroll
+#pragma unroll
     for(int d = 0; d < D; ++d) {
       R s = 0.;
-      #pragma unroll
+#pragma unroll
       for(int l = 0; l < D; ++l) { G[d][l + 1] = Ji[d][l]; s -= Ji[d][l]; }
       G[d][0] = s;
     }
-    #pragma unroll
+#pragma unroll
     for(int d = 0; d < D; ++d) Ga[d] = select_a(G[d]);
     // tau from u_adv at the centroid (NavierStokesAssembly.hpp:54, SUPG.hpp:137-219)
-    const R nu_e = fabs(nus / K);
+    const R nu_e = fabs(nus * (1.0 / K));
     R uc[D];
-    #pragma unroll
+#pragma unroll
     for(int d = 0; d < D; ++d) {
       R s = 0.;
-      #pragma unroll
+#pragma unroll
       for(int m = 0; m < K; ++m) s += ua_n[m][d];
-      uc[d] = s / K;
+      uc[d] = s * (1.0 / K);
     }
     if(g.supg_type == CF3HIP_SUPG_METRIC) {
       R gij[D][D];
-      #pragma unroll
+#pragma unroll
       for(int i = 0; i < D; ++i)
-        #pragma unroll
+#pragma unroll
         for(int j = 0; j < D; ++j) {
           R s = 0.;
-          #pragma unroll
+#pragma unroll
           for(int l = 0; l < D; ++l) s += Ji[l][i] * Ji[l][j];
           gij[i][j] = s;
         }
       R ugu = 0., gsq = 0., tr = 0.;
-      #pragma unroll
+#pragma unroll
       for(int j = 0; j < D; ++j) {
         R s = 0.;
-        #pragma unroll
+#pragma unroll
         for(int i = 0; i < D; ++i) s += uc[i] * gij[i][j];
         ugu += s * uc[j];
         tr += gij[j][j];
-        #pragma unroll
+#pragma unroll
         for(int i = 0; i < D; ++i) gsq += gij[i][j] * gij[i][j];
       }
       const R diff = nu_e * nu_e * gsq;
-      tsu = 1. / sqrt(4. * g.c1 * g.c1 / (g.dt * g.dt) + fabs(ugu) + g.c2 * diff);
+      tsu = 1. / sqrt(g.c1 * g.c1 * g.f4dt2 + fabs(ugu) + g.c2 * diff);
       tps = tsu;
-      tbu = sqrt(4. / (g.dt * g.dt) + fabs(ugu) + 16. * diff) / tr;
+      tbu = sqrt(g.f4dt2 + fabs(ugu) + 16. * diff) / tr;
     } else {
       R h = 1e10;
-      #pragma unroll
+#pragma unroll
       for(int i = 0; i < K; ++i)
-        #pragma unroll
-        for(int j = 0; j < K; ++j)
-          if(i != j) {
-            R s = 0.;
-            #pragma unroll
-            for(int d = 0; d < D; ++d) { const R df = X[i][d] - X[j][d]; s += df * df; }
-            h = fmin(h, s);
-          }
+#pragma unroll
+        for(int j = i + 1; j < K; ++j) {
+          R s = 0.;
+#pragma unroll
+          for(int d = 0; d < D; ++d) { const R df = X[i][d] - X[j][d]; s += df * df; }
+          h = fmin(h, s);
+        }
       h = sqrt(h);
       R um = 0.;
-      #pragma unroll
+#pragma unroll
       for(int d = 0; d < D; ++d) um += uc[d] * uc[d];
       um = sqrt(um);
       tsu = 1. / ((2. * um) / h + 2. / g.dt + (4. * nu_e) / (h * h));
@@ -435,105 +481,123 @@ struct GenericP1 {
     tps *= g.a_ps; tsu *= g.a_su; tbu *= g.a_bu;
   }
 
-  template<int B, typename VR>
-  __device__ void block(const AsmArgs& g, VR* m, R* rhs) const
+  template<int B, bool RHS, typename VR>
+  __device__ __forceinline__ void block(const AsmArgs& g, VR* m, R* rhs, const R* xb) const
   {
-    R A[NEQ][NEQ], T[NEQ][NEQ];
-    #pragma unroll
+    R A[NEQ][NEQ];
+#pragma unroll
     for(int r = 0; r < NEQ; ++r)
-      #pragma unroll
-      for(int c = 0; c < NEQ; ++c) { A[r][c] = 0.; T[r][c] = 0.; }
+#pragma unroll
+      for(int c = 0; c < NEQ; ++c) A[r][c] = 0.;
+    R Tuu = 0., Tp[D];   // T is diagonal in the velocity rows; T[p][u_i] in the pressure row
+#pragma unroll
+    for(int d = 0; d < D; ++d) Tp[d] = 0.;
     R gtg = 0.;
-    #pragma unroll
+#pragma unroll
     for(int d = 0; d < D; ++d) gtg += Ga[d] * G[d][B];
-    #pragma unroll
+    const R inv_rho = g.inv_rho;
+#pragma unroll
     for(int q = 0; q < NQ; ++q) {
       R xi[D], wq, N[K];
       gauss(q, xi, wq);
       shape(xi, N);
       const R Na = select_a(N);
       const R w = wq * detw;
-      R ua[D], nuq = 0.;
-      #pragma unroll
+      R uq[D], nuq = 0.;
+#pragma unroll
       for(int d = 0; d < D; ++d) {
         R s = 0.;
-        #pragma unroll
+#pragma unroll
         for(int mm = 0; mm < K; ++mm) s += N[mm] * ua_n[mm][d];
-        ua[d] = s;
+        uq[d] = s;
       }
-      #pragma unroll
+#pragma unroll
       for(int mm = 0; mm < K; ++mm) nuq += N[mm] * nu_n[mm];
       R La = 0., Lb = 0.;
-      #pragma unroll
-      for(int d = 0; d < D; ++d) { La += ua[d] * Ga[d]; Lb += ua[d] * G[d][B]; }
+#pragma unroll
+      for(int d = 0; d < D; ++d) { La += uq[d] * Ga[d]; Lb += uq[d] * G[d][B]; }
       const R SUa = Na + tsu * La;
       const R Lps = 0.5 * tps * La;
-      #pragma unroll
+#pragma unroll
       for(int i = 0; i < D; ++i) {
         A[D][i] += w * ((Na + Lps) * G[i][B] + (tps * Ga[i]) * Lb);
         A[i][i] += w * (nuq * gtg + SUa * Lb);
-        A[i][D] += w * (SUa * G[i][B] / g.rho);
-        #pragma unroll
-        for(int j = 0; j < D; ++j) A[i][j] += w * ((tbu * Ga[i] + 0.5 * ua[i] * SUa) * G[j][B]);
-        T[D][i] += w * (tps * Ga[i] * N[B]);
-        T[i][i] += w * (SUa * N[B]);
+        A[i][D] += w * (SUa * G[i][B] * inv_rho);
+#pragma unroll
+        for(int j = 0; j < D; ++j) A[i][j] += w * ((tbu * Ga[i] + 0.5 * uq[i] * SUa) * G[j][B]);
+        Tp[i] += w * (tps * Ga[i] * N[B]);
       }
-      A[D][D] += w * ((tps * gtg) / g.rho);
+      Tuu += w * (SUa * N[B]);
+      A[D][D] += w * ((tps * gtg) * inv_rho);
     }
-    #pragma unroll
-    for(int r = 0; r < NEQ; ++r) {
-      R s = 0.;
-      #pragma unroll
-      for(int c = 0; c < NEQ; ++c) s += A[r][c] * xb[B][c];
-      rhs[r] -= s;
-    }
-    #pragma unroll
-    for(int r = 0; r < NEQ; ++r)
-      #pragma unroll
-      for(int c = 0; c < NEQ; ++c) {
-        const R ar = r == D ? A[r][c] / g.theta : A[r][c];
-        m[r * NEQ + c] = (VR)(g.invdt * T[r][c] + g.theta * ar);
+    if constexpr(RHS) {
+#pragma unroll
+      for(int r = 0; r < NEQ; ++r) {
+        R s = 0.;
+#pragma unroll
+        for(int c = 0; c < NEQ; ++c) s += A[r][c] * xb[c];
+        rhs[r] -= s;
       }
+    }
+    const R invdt = g.invdt, th = g.theta, thq = g.thq;
+#pragma unroll
+    for(int r = 0; r < D; ++r)
+#pragma unroll
+      for(int c = 0; c < NEQ; ++c)
+        m[r * NEQ + c] = (VR)((r == c ? invdt * Tuu : 0.) + th * A[r][c]);
+#pragma unroll
+    for(int c = 0; c < D; ++c) m[D * NEQ + c] = (VR)(invdt * Tp[c] + thq * A[D][c]);
+    m[D * NEQ + D] = (VR)(thq * A[D][D]);
   }
 
   // _a[u[_i]] += (N + tau_su u_adv grad N)^T g  (NavierStokesAssembly.hpp:65)
-  __device__ void rhs_source(const AsmArgs& g, R* rhs) const
+  __device__ __forceinline__ void rhs_source(const AsmArgs& g, R* rhs) const
   {
-    #pragma unroll
+    R g_n[K][D];
+#pragma unroll
+    for(int m = 0; m < K; ++m)
+#pragma unroll
+      for(int d = 0; d < D; ++d) g_n[m][d] = g.g[d][cn[m]];
+#pragma unroll
     for(int q = 0; q < NQ; ++q) {
       R xi[D], wq, N[K];
       gauss(q, xi, wq);
       shape(xi, N);
       const R Na = select_a(N);
       const R w = wq * detw;
-      R ua[D], gq[D];
-      #pragma unroll
+      R uq[D], gq[D];
+#pragma unroll
       for(int d = 0; d < D; ++d) {
         R s = 0., sg = 0.;
-        #pragma unroll
+#pragma unroll
         for(int mm = 0; mm < K; ++mm) { s += N[mm] * ua_n[mm][d]; sg += N[mm] * g_n[mm][d]; }
-        ua[d] = s; gq[d] = sg;
+        uq[d] = s; gq[d] = sg;
       }
       R La = 0.;
-      #pragma unroll
-      for(int d = 0; d < D; ++d) La += ua[d] * Ga[d];
+#pragma unroll
+      for(int d = 0; d < D; ++d) La += uq[d] * Ga[d];
       const R SUa = Na + tsu * La;
-      #pragma unroll
+#pragma unroll
       for(int i = 0; i < D; ++i) rhs[i] += w * (SUa * gq[i]);
     }
   }
 };
 
 // ---------------------------------------------------------------------------------------------
-// Row-owner assembly kernel
+// Row-owner assembly kernel.  RHS = false (FP32 matrix pass of the precision form) skips the
+// right-hand side, which is formed by the FP64 pass.
 // ---------------------------------------------------------------------------------------------
-template<class Op, typename R, typename VR>
+template<class Op, typename R, typename VR, bool RHS>
 __global__ __launch_bounds__(kThreads) void k_assemble(AsmArgs g)
 {
   constexpr int K = Op::K, NEQ = Op::NEQ, NN = NEQ * NEQ;
-  constexpr int LDW = NN + 1;   // padded row of the staging tile
-  __shared__ VR s_blk[kThreads * LDW];
-  __shared__ R s_rhs[kThreads * NEQ];
+  constexpr int LDW = lds_row<VR, NN>();
+  constexpr size_t kBlkBytes = sizeof(VR) * kThreads * LDW;
+  constexpr size_t kRhsBytes = sizeof(R) * kThreads * NEQ;
+  __shared__ __align__(16) unsigned char s_raw[kBlkBytes > kRhsBytes ? kBlkBytes : kRhsBytes];
+  VR* s_blk = reinterpret_cast<VR*>(s_raw);
+  R* s_rhs = reinterpret_cast<R*>(s_raw);   // reused after the last pass
+  __shared__ uint16_t s_con[kThreads * K];
   __shared__ uint32_t s_rowblk[kThreads + 1];
   __shared__ uint32_t s_rowinc[kThreads + 1];
 
@@ -543,30 +607,22 @@ __global__ __launch_bounds__(kThreads) void k_assemble(AsmArgs g)
   const uint32_t ninc = g.inc_ptr[r1] - i0;
   const uint32_t blk0 = g.row_ptr[r0];
   const uint32_t nblk = g.row_ptr[r1] - blk0;
+  const uint32_t cb0 = g.contrib_ptr[blk0];
+  const uint32_t ncon = g.contrib_ptr[blk0 + nblk] - cb0;   // = ninc * K
   const int t = threadIdx.x;
   for(uint32_t j = t; j <= nrows; j += kThreads) {
     s_rowblk[j] = g.row_ptr[r0 + j] - blk0;
     s_rowinc[j] = g.inc_ptr[r0 + j] - i0;
   }
-  __syncthreads();
+  for(uint32_t j = t; j < ncon; j += kThreads) s_con[j] = g.contrib[cb0 + j];
 
   // owner of block t of this workgroup's contiguous block range
   const bool owner = (uint32_t)t < nblk;
-  R acc[NN];
-#pragma unroll
-  for(int v = 0; v < NN; ++v) acc[v] = R(0);
   uint32_t cur = 0, cend = 0, qb = 0;
   if(owner) {
-    uint32_t lo = 0, hi = nrows;        // row j with s_rowblk[j] <= t < s_rowblk[j+1]
-    while(hi - lo > 1) {
-      const uint32_t mid = (lo + hi) >> 1;
-      if(s_rowblk[mid] <= (uint32_t)t) lo = mid; else hi = mid;
-    }
-    qb = s_rowinc[lo];
-    cur = g.contrib_ptr[blk0 + t];
-    cend = g.contrib_ptr[blk0 + t + 1];
+    cur = g.contrib_ptr[blk0 + t] - cb0;
+    cend = g.contrib_ptr[blk0 + t + 1] - cb0;
   }
-
   const bool active = (uint32_t)t < ninc;
   Op op;
   R rhs[NEQ];
@@ -576,18 +632,30 @@ __global__ __launch_bounds__(kThreads) void k_assemble(AsmArgs g)
     const uint32_t ea = g.inc[i0 + t];
     op.prepare(g, ea / K, (int)(ea % K));
   }
+  __syncthreads();
+  if(owner) {
+    uint32_t lo = 0, hi = nrows;        // row j with s_rowblk[j] <= t < s_rowblk[j+1]
+    while(hi - lo > 1) {
+      const uint32_t mid = (lo + hi) >> 1;
+      if(s_rowblk[mid] <= (uint32_t)t) lo = mid; else hi = mid;
+    }
+    qb = s_rowinc[lo];
+  }
+  R acc[NN];
+#pragma unroll
+  for(int v = 0; v < NN; ++v) acc[v] = R(0);
+
   static_for<0, K>([&](auto bc) {
     constexpr int B = decltype(bc)::value;
-    VR mb[NN];
-    if(active) op.template block<B, VR>(g, mb, rhs);
-    __syncthreads();
-    if(active)
-#pragma unroll
-      for(int v = 0; v < NN; ++v) s_blk[t * LDW + v] = mb[v];
+    if(active) {
+      R xb[NEQ];
+      if constexpr(RHS) load_state<NEQ, R>(g, op.cn[B], xb);
+      op.template block<B, RHS, VR>(g, s_blk + t * LDW, rhs, xb);
+    }
     __syncthreads();
     if(owner) {
       while(cur < cend) {
-        const uint32_t c = g.contrib[cur];
+        const uint32_t c = s_con[cur];
         if((int)(c & 15u) != B) break;
         const VR* src = s_blk + (qb + (c >> 4)) * LDW;
 #pragma unroll
@@ -595,24 +663,26 @@ __global__ __launch_bounds__(kThreads) void k_assemble(AsmArgs g)
         ++cur;
       }
     }
+    __syncthreads();
   });
   if(owner) {
     VR* dst = static_cast<VR*>(g.values) + (uint64_t)(blk0 + t) * NN;
 #pragma unroll
     for(int v = 0; v < NN; ++v) dst[v] = (VR)acc[v];
   }
-  if(active) {
-    op.rhs_source(g, rhs);
+  if constexpr(RHS) {
+    if(active) {
+      op.rhs_source(g, rhs);
 #pragma unroll
-    for(int r = 0; r < NEQ; ++r) s_rhs[t * NEQ + r] = rhs[r];
-  }
-  __syncthreads();
-  if(!g.rhs) return;   // the FP32 matrix pass leaves the FP64 right-hand side alone
-  for(uint32_t o = t; o < nrows * NEQ; o += kThreads) {
-    const uint32_t row = o / NEQ, r = o % NEQ;
-    R s = R(0);
-    for(uint32_t q = s_rowinc[row]; q < s_rowinc[row + 1]; ++q) s += s_rhs[q * NEQ + r];
-    g.rhs[(uint64_t)(r0 + row) * NEQ + r] = (double)s;
+      for(int r = 0; r < NEQ; ++r) s_rhs[t * NEQ + r] = rhs[r];
+    }
+    __syncthreads();
+    for(uint32_t o = t; o < nrows * NEQ; o += kThreads) {
+      const uint32_t row = o / NEQ, r = o % NEQ;
+      R s = R(0);
+      for(uint32_t q = s_rowinc[row]; q < s_rowinc[row + 1]; ++q) s += s_rhs[q * NEQ + r];
+      g.rhs[(uint64_t)(r0 + row) * NEQ + r] = (double)s;
+    }
   }
 }
 
@@ -634,32 +704,24 @@ __global__ __launch_bounds__(kThreads) void k_residual(AsmArgs g)
   for(uint32_t j = t; j <= nrows; j += kThreads) s_rowinc[j] = g.inc_ptr[r0 + j] - i0;
   const bool active = (uint32_t)t < ninc;
   Op op;
-  double rhs[NEQ], y[NEQ], xt[K][NEQ];
+  double rhs[NEQ], y[NEQ];
 #pragma unroll
   for(int r = 0; r < NEQ; ++r) { rhs[r] = 0.; y[r] = 0.; }
   if(active) {
     const uint32_t ea = g.inc[i0 + t];
-    const uint32_t e = ea / K;
-    op.prepare(g, e, (int)(ea % K));
+    op.prepare(g, ea / K, (int)(ea % K));
+    static_for<0, K>([&](auto bc) {
+      constexpr int B = decltype(bc)::value;
+      double xb[NEQ], xt[NEQ], mb[NN];
+      load_state<NEQ, double>(g, op.cn[B], xb);
 #pragma unroll
-    for(int m = 0; m < K; ++m) {
-      const uint32_t nd = g.conn[(uint64_t)e * K + m];
-#pragma unroll
-      for(int c = 0; c < NEQ; ++c) xt[m][c] = g.xt[(uint64_t)nd * NEQ + c];
-    }
-  }
-  static_for<0, K>([&](auto bc) {
-    constexpr int B = decltype(bc)::value;
-    if(active) {
-      double mb[NN];
-      op.template block<B, double>(g, mb, rhs);
+      for(int c = 0; c < NEQ; ++c) xt[c] = g.xt[(uint64_t)op.cn[B] * NEQ + c];
+      op.template block<B, true, double>(g, mb, rhs, xb);
 #pragma unroll
       for(int r = 0; r < NEQ; ++r)
 #pragma unroll
-        for(int c = 0; c < NEQ; ++c) y[r] += mb[r * NEQ + c] * xt[B][c];
-    }
-  });
-  if(active) {
+        for(int c = 0; c < NEQ; ++c) y[r] += mb[r * NEQ + c] * xt[c];
+    });
     op.rhs_source(g, rhs);
 #pragma unroll
     for(int r = 0; r < NEQ; ++r) s_res[t * NEQ + r] = rhs[r] - y[r];
@@ -689,21 +751,67 @@ __global__ void k_bc_residual(uint32_t n, const uint32_t* __restrict__ dof, cons
   if(i < n) r[dof[i]] = vlast[i] - x[dof[i]];
 }
 
-template<class Op, typename R, typename VR>
+// packed node records
+__global__ void k_pack_coords(uint32_t n, int dim, const double* __restrict__ x, const double* __restrict__ y,
+                              const double* __restrict__ z, double4* __restrict__ xp)
+{
+  const uint32_t i = blockIdx.x * blockDim.x + threadIdx.x;
+  if(i >= n) return;
+  xp[i] = make_double4(x[i], y[i], dim == 3 ? z[i] : 0., 0.);
+}
+
+struct PackArgs {
+  const double* u[3];
+  const double* ua[3];
+  const double* p;
+  const double* nu;
+  double4* st;
+  double4* ad;
+};
+
+template<int D>
+__global__ void k_pack_fields(uint32_t n, PackArgs a)
+{
+  const uint32_t i = blockIdx.x * blockDim.x + threadIdx.x;
+  if(i >= n) return;
+  if constexpr(D == 3) {
+    a.st[i] = make_double4(a.u[0][i], a.u[1][i], a.u[2][i], a.p[i]);
+    a.ad[i] = make_double4(a.ua[0][i], a.ua[1][i], a.ua[2][i], a.nu[i]);
+  } else {
+    a.st[i] = make_double4(a.u[0][i], a.u[1][i], a.p[i], 0.);
+    a.ad[i] = make_double4(a.ua[0][i], a.ua[1][i], 0., a.nu[i]);
+  }
+}
+
+void pack_fields(Fields& F)
+{
+  const Mesh& M = *F.mesh;
+  Context& c = *M.ctx;
+  PackArgs a = {};
+  for(int d = 0; d < M.dim; ++d) {
+    a.u[d] = F.u[d].as<double>();
+    a.ua[d] = F.uadv[d].as<double>();
+  }
+  a.p = F.p.as<double>();
+  a.nu = F.nu.as<double>();
+  a.st = F.st.as<double4>();
+  a.ad = F.ad.as<double4>();
+  if(M.dim == 3) hipLaunchKernelGGL(k_pack_fields<3>, dim3(grid_for(M.n_nodes, 256)), dim3(256), 0, c.stream, M.n_nodes, a);
+  else hipLaunchKernelGGL(k_pack_fields<2>, dim3(grid_for(M.n_nodes, 256)), dim3(256), 0, c.stream, M.n_nodes, a);
+  HIP_OK(hipGetLastError());
+}
+
+template<class Op, typename R, typename VR, bool RHS>
 void launch(const AsmArgs& g, const Lss& L)
 {
   Context& c = *L.ctx;
   timer_begin(c, CF3HIP_KERNEL_ASSEMBLY);
-  hipLaunchKernelGGL((k_assemble<Op, R, VR>), dim3(L.n_wg), dim3(kThreads), 0, c.stream, g);
+  hipLaunchKernelGGL((k_assemble<Op, R, VR, RHS>), dim3(L.n_wg), dim3(kThreads), 0, c.stream, g);
   HIP_OK(hipGetLastError());
   timer_end(c, CF3HIP_KERNEL_ASSEMBLY);
 }
 
 __global__ void k_clear_flag(int* f) { f[0] = 0; }
-
-}  // namespace
-
-namespace {
 
 AsmArgs make_args(Lss& L, Fields& F, const cf3hip_ns_params& prm)
 {
@@ -721,14 +829,10 @@ AsmArgs make_args(Lss& L, Fields& F, const cf3hip_ns_params& prm)
   if(M.etype == CF3HIP_HEXA3D)
     CF3_THROW(CF3HIP_ERR_NOT_IMPLEMENTED, "hexahedra assembly is provided by the Q1 kernel");
   AsmArgs g = {};
-  for(int d = 0; d < 3; ++d) {
-    g.nd.x[d] = d < M.dim ? M.x[d].as<double>() : nullptr;
-    g.nd.u[d] = d < M.dim ? F.u[d].as<double>() : nullptr;
-    g.nd.ua[d] = d < M.dim ? F.uadv[d].as<double>() : nullptr;
-    g.nd.g[d] = d < M.dim ? F.g[d].as<double>() : nullptr;
-  }
-  g.nd.p = F.p.as<double>();
-  g.nd.nu = F.nu.as<double>();
+  g.xp = M.xp.as<double4>();
+  g.st = F.st.as<double4>();
+  g.ad = F.ad.as<double4>();
+  for(int d = 0; d < 3; ++d) g.g[d] = d < M.dim ? F.g[d].as<double>() : nullptr;
   g.conn = M.conn.as<uint32_t>();
   g.wg_rows = L.wg_rows.as<uint32_t>();
   g.row_ptr = L.row_ptr.as<uint32_t>();
@@ -736,10 +840,13 @@ AsmArgs make_args(Lss& L, Fields& F, const cf3hip_ns_params& prm)
   g.inc = L.inc.as<uint32_t>();
   g.contrib_ptr = L.contrib_ptr.as<uint32_t>();
   g.contrib = L.contrib.as<uint16_t>();
-  g.rho = prm.rho; g.dt = prm.dt; g.theta = prm.theta; g.invdt = 1. / prm.dt;
+  g.rho = prm.rho; g.inv_rho = 1. / prm.rho;
+  g.dt = prm.dt; g.theta = prm.theta; g.thq = prm.theta * (1. / prm.theta);
+  g.invdt = 1. / prm.dt; g.f4dt2 = 4. / (prm.dt * prm.dt);
   g.supg_type = prm.supg_type;
   g.c1 = prm.c1; g.c2 = prm.c2; g.a_ps = prm.alpha_ps; g.a_su = prm.alpha_su; g.a_bu = prm.alpha_bu;
   g.status = L.ctx->status_flag();
+  pack_fields(F);
   return g;
 }
 
@@ -764,6 +871,16 @@ void launch_residual(const AsmArgs& g, const Lss& L)
 
 }  // namespace
 
+void mesh_pack_coords(Mesh& M)
+{
+  Context& c = *M.ctx;
+  if(!M.xp.ptr) M.xp.alloc(&c, sizeof(double4) * M.n_nodes);
+  hipLaunchKernelGGL(k_pack_coords, dim3(grid_for(M.n_nodes, 256)), dim3(256), 0, c.stream, M.n_nodes, M.dim,
+                     M.x[0].as<double>(), M.x[1].as<double>(), M.dim == 3 ? M.x[2].as<double>() : nullptr,
+                     M.xp.as<double4>());
+  HIP_OK(hipGetLastError());
+}
+
 void assemble(Lss& L, Fields& F, const cf3hip_ns_params& prm)
 {
   AsmArgs g = make_args(L, F, prm);
@@ -784,17 +901,17 @@ void assemble(Lss& L, Fields& F, const cf3hip_ns_params& prm)
   g.values = fp32 ? L.values32.ptr : L.values.ptr;
 
   if(spec && M.etype == CF3HIP_TETRA3D) {
-    if(fp32) launch<SpecP1<3, float>, float, float>(g, L);
-    else launch<SpecP1<3, double>, double, double>(g, L);
+    if(fp32) launch<SpecP1<3, float>, float, float, false>(g, L);
+    else launch<SpecP1<3, double>, double, double, true>(g, L);
   } else if(spec && M.etype == CF3HIP_TRIAG2D) {
-    if(fp32) launch<SpecP1<2, float>, float, float>(g, L);
-    else launch<SpecP1<2, double>, double, double>(g, L);
+    if(fp32) launch<SpecP1<2, float>, float, float, false>(g, L);
+    else launch<SpecP1<2, double>, double, double, true>(g, L);
   } else if(M.etype == CF3HIP_TETRA3D) {
-    if(fp32) launch<GenericP1<3>, double, float>(g, L);
-    else launch<GenericP1<3>, double, double>(g, L);
+    if(fp32) launch<GenericP1<3>, double, float, false>(g, L);
+    else launch<GenericP1<3>, double, double, true>(g, L);
   } else {
-    if(fp32) launch<GenericP1<2>, double, float>(g, L);
-    else launch<GenericP1<2>, double, double>(g, L);
+    if(fp32) launch<GenericP1<2>, double, float, false>(g, L);
+    else launch<GenericP1<2>, double, double, true>(g, L);
   }
   check_status(c);
 }

@@ -296,6 +296,7 @@ int cf3hip_mesh_create(cf3hip_context ctx, int etype, uint32_t n_nodes, uint32_t
     }
     M.conn.alloc(&c, sizeof(uint32_t) * (uint64_t)n_elems * k);
     h2d(c, M.conn.ptr, conn, sizeof(uint32_t) * (uint64_t)n_elems * k);
+    mesh_pack_coords(M);
     HIP_OK(hipStreamSynchronize(c.stream));
   } catch(...) {
     delete h;
@@ -323,6 +324,8 @@ int cf3hip_mesh_set_coordinates(cf3hip_mesh mesh, const double* coords)
     h2d(*M.ctx, M.x[d].ptr, comp.data(), sizeof(double) * M.n_nodes);
     HIP_OK(hipStreamSynchronize(M.ctx->stream));
   }
+  mesh_pack_coords(M);
+  HIP_OK(hipStreamSynchronize(M.ctx->stream));
   API_END
 }
 
@@ -712,6 +715,8 @@ int cf3hip_fields_create(cf3hip_mesh mesh, cf3hip_fields* out)
     }
     F.p.alloc(&c, bytes);
     F.nu.alloc(&c, bytes);
+    F.st.alloc(&c, 4 * bytes);
+    F.ad.alloc(&c, 4 * bytes);
     lss_fill(c, F.p.as<double>(), M.n_nodes, 0.);
     lss_fill(c, F.nu.as<double>(), M.n_nodes, 0.);
     HIP_OK(hipStreamSynchronize(c.stream));

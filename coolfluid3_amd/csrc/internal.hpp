@@ -73,6 +73,9 @@ void timer_begin(Context& c, int kernel);
 void timer_end(Context& c, int kernel);
 
 constexpr int kMaxRedBlocks = 1024;
+// assembly workgroup: at most kAsmThreads incidences (one per thread) and kAsmThreads CSR blocks
+// (one owner thread per block); the row schedule of the system respects both
+constexpr int kAsmThreads = 128;
 
 // ---------------------------------------------------------------------------------------------
 // mesh / system / fields
@@ -82,6 +85,7 @@ struct Mesh {
   int etype = 0, dim = 0, k = 0;
   uint32_t n_nodes = 0, n_elems = 0;
   DevBuf x[3];        // SoA coordinates (double)
+  DevBuf xp;          // packed coordinates double4 (x, y, z | 0, 0) for the element gathers
   DevBuf conn;        // uint32 n_elems*k, element-major
 };
 
@@ -129,6 +133,9 @@ struct Lss {
 struct Fields {
   Mesh* mesh = nullptr;
   DevBuf u[3], p, uadv[3], u1[3], u2[3], u3[3], nu, g[3];
+  // packed double4 node records refreshed by every assembly / residual call:
+  // st = (u, v, w, p) in 3D, (u, v, p, 0) in 2D; ad = (u_adv x, y, z | 0, nu_eff)
+  DevBuf st, ad;
 };
 
 // ---------------------------------------------------------------------------------------------
@@ -137,6 +144,7 @@ struct Fields {
 void build_element_structures(Lss& L);                   // sparsity.hip
 void build_graph_structures(Lss& L);                     // sparsity.hip (diag blocks)
 void assemble(Lss& L, Fields& F, const cf3hip_ns_params& prm);   // assembly.hip
+void mesh_pack_coords(Mesh& M);                                  // assembly.hip
 
 // lss.hip
 void lss_fill(Context& c, double* p, size_t n, double v);

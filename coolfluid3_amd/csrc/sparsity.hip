@@ -23,7 +23,6 @@ namespace cf3hip {
 
 namespace {
 
-constexpr int kAsmThreads = 256;      // assembly workgroup size (must match assembly.hip)
 
 int bits_for(uint64_t v)
 {
