@@ -6,10 +6,11 @@
 // element's nodes from packed node records (double4: coordinates, state (u.., p), advection velocity
 // and viscosity), computes the element quantities it needs, then produces the element row block
 // M(a, b) = invdt*T(a,b) + theta*A'(a,b) one column node b at a time.  Per pass b the blocks are
-// staged in LDS and the owner thread of every CSR block sums the contributions of its slot, walking
-// the precomputed per-slot contribution list (sorted by (b, q), staged in LDS): a sorted segmented
-// reduction with a fixed order, so results are deterministic and no floating-point atomics are used.
-// Every CSR value and rhs entry is written exactly once.
+// staged in LDS at rows sorted by target CSR block (precomputed staging map); the accumulator threads
+// of every CSR block (one, or a few for blocks with many contributions) sum its run of rows, and the
+// partial sums are added in thread order when the block is written: a sorted segmented reduction
+// with a fixed order, so results are deterministic and no floating-point atomics are used.  Every
+// CSR value and rhs entry is written exactly once, the values with coalesced stores.
 //
 // Element mathematics (reference files relative to the coolfluid3 tree):
 //   specialised P1 triangle / tetrahedron: plugins/UFEM/src/UFEM/NavierStokesSpecializations.hpp:38-303
@@ -39,8 +40,9 @@ struct AsmArgs {
   const uint32_t* row_ptr;
   const uint32_t* inc_ptr;
   const uint32_t* inc;
-  const uint32_t* contrib_ptr;
-  const uint16_t* contrib;
+  const uint8_t* inc_pos;   // per incidence: staging row of its block in pass b (byte b)
+  const uint8_t* blk_beg;   // per block: first staging row of pass p (byte p)
+  const uint16_t* blk_acc;  // per block: first accumulator thread (low byte), accumulator count (high byte)
   void* values;
   double* rhs;
   const double* xt;    // matrix-free residual: x with constrained columns set to their first value
@@ -587,19 +589,38 @@ struct GenericP1 {
 // Row-owner assembly kernel.  RHS = false (FP32 matrix pass of the precision form) skips the
 // right-hand side, which is formed by the FP64 pass.
 // ---------------------------------------------------------------------------------------------
+// LDS row access in 16-byte pieces
+template<typename VR, int NN>
+__device__ __forceinline__ void lds_load_row(const VR* src, VR v[NN])
+{
+  if constexpr((NN * sizeof(VR)) % 16 == 0) {
+    typedef typename std::conditional<sizeof(VR) == 8, double2, float4>::type V16;
+    constexpr int W = 16 / sizeof(VR);
+#pragma unroll
+    for(int c = 0; c < NN / W; ++c) {
+      const V16 x = reinterpret_cast<const V16*>(src)[c];
+      if constexpr(W == 2) { v[2 * c] = x.x; v[2 * c + 1] = x.y; }
+      else { v[4 * c] = x.x; v[4 * c + 1] = x.y; v[4 * c + 2] = x.z; v[4 * c + 3] = x.w; }
+    }
+  } else {
+#pragma unroll
+    for(int c = 0; c < NN; ++c) v[c] = src[c];
+  }
+}
+
 template<class Op, typename R, typename VR, bool RHS>
 __global__ __launch_bounds__(kThreads) void k_assemble(AsmArgs g)
 {
   constexpr int K = Op::K, NEQ = Op::NEQ, NN = NEQ * NEQ;
+  static_assert(K <= 4, "one staging byte per column local node in a 32-bit word");
   constexpr int LDW = lds_row<VR, NN>();
   constexpr size_t kBlkBytes = sizeof(VR) * kThreads * LDW;
   constexpr size_t kRhsBytes = sizeof(R) * kThreads * NEQ;
   __shared__ __align__(16) unsigned char s_raw[kBlkBytes > kRhsBytes ? kBlkBytes : kRhsBytes];
   VR* s_blk = reinterpret_cast<VR*>(s_raw);
-  R* s_rhs = reinterpret_cast<R*>(s_raw);   // reused after the last pass
-  __shared__ uint16_t s_con[kThreads * K];
-  __shared__ uint32_t s_rowblk[kThreads + 1];
+  R* s_rhs = reinterpret_cast<R*>(s_raw);   // reused after the values are written
   __shared__ uint32_t s_rowinc[kThreads + 1];
+  __shared__ uint16_t s_acc[kThreads];
 
   const uint32_t r0 = g.wg_rows[blockIdx.x], r1 = g.wg_rows[blockIdx.x + 1];
   const uint32_t nrows = r1 - r0;
@@ -607,39 +628,39 @@ __global__ __launch_bounds__(kThreads) void k_assemble(AsmArgs g)
   const uint32_t ninc = g.inc_ptr[r1] - i0;
   const uint32_t blk0 = g.row_ptr[r0];
   const uint32_t nblk = g.row_ptr[r1] - blk0;
-  const uint32_t cb0 = g.contrib_ptr[blk0];
-  const uint32_t ncon = g.contrib_ptr[blk0 + nblk] - cb0;   // = ninc * K
   const int t = threadIdx.x;
-  for(uint32_t j = t; j <= nrows; j += kThreads) {
-    s_rowblk[j] = g.row_ptr[r0 + j] - blk0;
-    s_rowinc[j] = g.inc_ptr[r0 + j] - i0;
-  }
-  for(uint32_t j = t; j < ncon; j += kThreads) s_con[j] = g.contrib[cb0 + j];
+  if constexpr(RHS)
+    for(uint32_t j = t; j <= nrows; j += kThreads) s_rowinc[j] = g.inc_ptr[r0 + j] - i0;
+  if((uint32_t)t < nblk) s_acc[t] = g.blk_acc[blk0 + t];
 
-  // owner of block t of this workgroup's contiguous block range
-  const bool owner = (uint32_t)t < nblk;
-  uint32_t cur = 0, cend = 0, qb = 0;
-  if(owner) {
-    cur = g.contrib_ptr[blk0 + t] - cb0;
-    cend = g.contrib_ptr[blk0 + t + 1] - cb0;
-  }
   const bool active = (uint32_t)t < ninc;
+  uint32_t posw = 0;
   Op op;
   R rhs[NEQ];
 #pragma unroll
   for(int r = 0; r < NEQ; ++r) rhs[r] = R(0);
   if(active) {
+    posw = reinterpret_cast<const uint32_t*>(g.inc_pos)[i0 + t];
     const uint32_t ea = g.inc[i0 + t];
     op.prepare(g, ea / K, (int)(ea % K));
   }
   __syncthreads();
+  // accumulator thread t: block bs of this workgroup, rank h of c; rows [beg_p + h, end_p) step c
+  const uint32_t alast = s_acc[nblk - 1];
+  const uint32_t n_acc = (alast & 255u) + (alast >> 8);
+  const bool owner = (uint32_t)t < n_acc;
+  uint32_t begw = 0, endw = 0, h = 0, c = 1;
   if(owner) {
-    uint32_t lo = 0, hi = nrows;        // row j with s_rowblk[j] <= t < s_rowblk[j+1]
+    uint32_t lo = 0, hi = nblk;
     while(hi - lo > 1) {
       const uint32_t mid = (lo + hi) >> 1;
-      if(s_rowblk[mid] <= (uint32_t)t) lo = mid; else hi = mid;
+      if((s_acc[mid] & 255u) <= (uint32_t)t) lo = mid; else hi = mid;
     }
-    qb = s_rowinc[lo];
+    h = (uint32_t)t - (s_acc[lo] & 255u);
+    c = s_acc[lo] >> 8;
+    const uint32_t* bb = reinterpret_cast<const uint32_t*>(g.blk_beg);
+    begw = bb[blk0 + lo];
+    endw = lo + 1 < nblk ? bb[blk0 + lo + 1] : ninc * 0x01010101u;
   }
   R acc[NN];
 #pragma unroll
@@ -650,27 +671,40 @@ __global__ __launch_bounds__(kThreads) void k_assemble(AsmArgs g)
     if(active) {
       R xb[NEQ];
       if constexpr(RHS) load_state<NEQ, R>(g, op.cn[B], xb);
-      op.template block<B, RHS, VR>(g, s_blk + t * LDW, rhs, xb);
+      op.template block<B, RHS, VR>(g, s_blk + ((posw >> (8 * B)) & 255u) * LDW, rhs, xb);
     }
     __syncthreads();
     if(owner) {
-      while(cur < cend) {
-        const uint32_t c = s_con[cur];
-        if((int)(c & 15u) != B) break;
-        const VR* src = s_blk + (qb + (c >> 4)) * LDW;
+      const uint32_t je = (endw >> (8 * B)) & 255u;
+      for(uint32_t j = ((begw >> (8 * B)) & 255u) + h; j < je; j += c) {
+        VR v[NN];
+        lds_load_row<VR, NN>(s_blk + j * LDW, v);
 #pragma unroll
-        for(int v = 0; v < NN; ++v) acc[v] += (R)src[v];
-        ++cur;
+        for(int q = 0; q < NN; ++q) acc[q] += (R)v[q];
       }
     }
     __syncthreads();
   });
+  // partial sums to LDS row t, then each block = sum of its accumulator rows in thread order,
+  // stored with consecutive lanes on consecutive values
   if(owner) {
-    VR* dst = static_cast<VR*>(g.values) + (uint64_t)(blk0 + t) * NN;
+    VR* d = s_blk + t * LDW;
 #pragma unroll
-    for(int v = 0; v < NN; ++v) dst[v] = (VR)acc[v];
+    for(int q = 0; q < NN; ++q) d[q] = (VR)acc[q];
+  }
+  __syncthreads();
+  {
+    VR* dst = static_cast<VR*>(g.values) + (uint64_t)blk0 * NN;
+    for(uint32_t o = t; o < nblk * NN; o += kThreads) {
+      const uint32_t sb = o / NN, v = o % NN;
+      const uint32_t a = s_acc[sb], f = a & 255u, na = a >> 8;
+      R sum = (R)s_blk[f * LDW + v];
+      for(uint32_t r = 1; r < na; ++r) sum += (R)s_blk[(f + r) * LDW + v];
+      dst[o] = (VR)sum;
+    }
   }
   if constexpr(RHS) {
+    __syncthreads();
     if(active) {
       op.rhs_source(g, rhs);
 #pragma unroll
@@ -679,9 +713,18 @@ __global__ __launch_bounds__(kThreads) void k_assemble(AsmArgs g)
     __syncthreads();
     for(uint32_t o = t; o < nrows * NEQ; o += kThreads) {
       const uint32_t row = o / NEQ, r = o % NEQ;
-      R s = R(0);
-      for(uint32_t q = s_rowinc[row]; q < s_rowinc[row + 1]; ++q) s += s_rhs[q * NEQ + r];
-      g.rhs[(uint64_t)(r0 + row) * NEQ + r] = (double)s;
+      const uint32_t qe = s_rowinc[row + 1];
+      uint32_t q = s_rowinc[row];
+      // four interleaved partial sums (fixed order) shorten the LDS latency chain
+      R s0 = R(0), s1 = R(0), s2 = R(0), s3 = R(0);
+      for(; q + 4 <= qe; q += 4) {
+        s0 += s_rhs[q * NEQ + r];
+        s1 += s_rhs[(q + 1) * NEQ + r];
+        s2 += s_rhs[(q + 2) * NEQ + r];
+        s3 += s_rhs[(q + 3) * NEQ + r];
+      }
+      for(; q < qe; ++q) s0 += s_rhs[q * NEQ + r];
+      g.rhs[(uint64_t)(r0 + row) * NEQ + r] = (double)((s0 + s1) + (s2 + s3));
     }
   }
 }
@@ -838,8 +881,9 @@ AsmArgs make_args(Lss& L, Fields& F, const cf3hip_ns_params& prm)
   g.row_ptr = L.row_ptr.as<uint32_t>();
   g.inc_ptr = L.inc_ptr.as<uint32_t>();
   g.inc = L.inc.as<uint32_t>();
-  g.contrib_ptr = L.contrib_ptr.as<uint32_t>();
-  g.contrib = L.contrib.as<uint16_t>();
+  g.inc_pos = L.inc_pos.as<uint8_t>();
+  g.blk_beg = L.blk_beg.as<uint8_t>();
+  g.blk_acc = L.blk_acc.as<uint16_t>();
   g.rho = prm.rho; g.inv_rho = 1. / prm.rho;
   g.dt = prm.dt; g.theta = prm.theta; g.thq = prm.theta * (1. / prm.theta);
   g.invdt = 1. / prm.dt; g.f4dt2 = 4. / (prm.dt * prm.dt);

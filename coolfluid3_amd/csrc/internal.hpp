@@ -107,10 +107,15 @@ struct Lss {
   DevBuf slots;         // uint32 n_elems*k*k   element-to-slot map (block slots)
   DevBuf inc_ptr;       // uint32 n_nodes+1
   DevBuf inc;           // uint32 n_elems*k : e*k + a, sorted by (node, e)
-  DevBuf contrib_ptr;   // uint32 n_blocks+1
-  DevBuf contrib;       // uint16 n_elems*k*k : (q << 4) | b, sorted by (slot, b, q)
   DevBuf wg_rows;       // uint32 n_wg+1 : row ranges of the assembly workgroups
   uint32_t n_wg = 0;
+  // staging maps of the row-owner reduction (see sparsity.hip): per incidence and column local node
+  // the LDS row its element block is staged to, per block and pass the first staged row it sums
+  int pos_stride = 4;   // bytes per incidence in inc_pos (4 for simplices, 8 for hexahedra)
+  int passes = 0;       // staging passes per workgroup (k for simplices, 1 for hexahedra)
+  DevBuf inc_pos;       // uint8 n_elems*k*pos_stride
+  DevBuf blk_beg;       // uint8 n_blocks*4
+  DevBuf blk_acc;       // uint16 n_blocks: first accumulator thread (low byte) and count (high byte)
   // Dirichlet conditions applied since the last reset (needed by the matrix-free FP64 residual):
   // per dof the first value (column elimination) and the last value (row / solution)
   std::unordered_map<uint64_t, std::pair<double, double> > bc_host;

@@ -10,9 +10,18 @@
 //
 // Row-owner scatter structures used by the assembly kernel:
 //   inc_ptr/inc     node -> incident (element, local node) pairs, sorted by element
-//   contrib_ptr/... per block slot, the contributing (local incidence q, column local node b)
-//                   pairs sorted by (b, q): the sorted segmented reduction order
 //   wg_rows         node-row ranges handled by one assembly workgroup
+//   inc_pos/blk_beg staging maps.  Within a workgroup, the element blocks of one staging pass are
+//                   written to LDS rows sorted by (CSR block slot, incidence): the block of incidence
+//                   q and column local node b goes to row inc_pos[q][b]; the rows of block s in pass
+//                   p are [blk_beg[s][p], blk_beg[s+1][p]).  Simplices stage one pass per column
+//                   local node b (p = b); hexahedra stage all (q, b) at once.
+//   blk_acc         accumulator threads of each block (first index within the workgroup, count):
+//                   c threads share the rows of a block (thread h takes rows h, h+c, ...) so that no
+//                   thread sums more than kAccRows rows per pass; their partial sums are added in
+//                   thread order when the block is written.
+//   The maps come from the per-slot contribution lists sorted by (slot, b, q): the summation order is
+//   fixed, so results are deterministic and no floating-point atomics are used.
 #include <hipcub/hipcub.hpp>
 
 #include <algorithm>
@@ -23,6 +32,7 @@ namespace cf3hip {
 
 namespace {
 
+constexpr uint32_t kAccRows = 2;   // staged rows one accumulator thread sums per pass (at most)
 
 int bits_for(uint64_t v)
 {
@@ -140,6 +150,82 @@ __global__ void k_contrib_from_keys(const uint64_t* __restrict__ keys, uint64_t 
   contrib[j] = (uint16_t)((q << 4) | b);
 }
 
+// Accumulator threads of a block: enough that none sums more than kAccRows staged rows per pass.
+__global__ void k_block_acc(uint64_t n_blocks, int passes, const uint32_t* __restrict__ contrib_ptr,
+                            const uint16_t* __restrict__ contrib, uint8_t* __restrict__ acc_cnt)
+{
+  const uint64_t s = (uint64_t)blockIdx.x * blockDim.x + threadIdx.x;
+  if(s >= n_blocks) return;
+  uint32_t cnt[4] = {0, 0, 0, 0};
+  for(uint32_t j = contrib_ptr[s]; j < contrib_ptr[s + 1]; ++j) ++cnt[passes == 1 ? 0 : (contrib[j] & 15u)];
+  const uint32_t mx = max(max(cnt[0], cnt[1]), max(cnt[2], cnt[3]));
+  acc_cnt[s] = (uint8_t)max(1u, (mx + kAccRows - 1) / kAccRows);
+}
+
+__global__ void k_row_acc(uint32_t n, const uint32_t* __restrict__ row_ptr, const uint8_t* __restrict__ acc_cnt,
+                          uint32_t* __restrict__ row_acc)
+{
+  const uint32_t i = blockIdx.x * blockDim.x + threadIdx.x;
+  if(i >= n) return;
+  uint32_t a = 0;
+  for(uint32_t s = row_ptr[i]; s < row_ptr[i + 1]; ++s) a += acc_cnt[s];
+  row_acc[i] = a;
+}
+
+// One workgroup per assembly workgroup: derive the staging maps from the contribution lists of its
+// blocks (contrib_ptr/contrib, sorted by (slot, b, q)) and number its accumulator threads.
+__global__ __launch_bounds__(kAsmThreads) void k_stage_maps(int passes, int pos_stride,
+                                                            const uint32_t* __restrict__ wg_rows,
+                                                            const uint32_t* __restrict__ row_ptr,
+                                                            const uint32_t* __restrict__ inc_ptr,
+                                                            const uint32_t* __restrict__ contrib_ptr,
+                                                            const uint16_t* __restrict__ contrib,
+                                                            const uint8_t* __restrict__ acc_cnt,
+                                                            uint8_t* __restrict__ inc_pos, uint8_t* __restrict__ blk_beg,
+                                                            uint16_t* __restrict__ blk_acc)
+{
+  __shared__ uint32_t s_cnt[5][kAsmThreads];
+  const uint32_t r0 = wg_rows[blockIdx.x], r1 = wg_rows[blockIdx.x + 1];
+  const uint32_t blk0 = row_ptr[r0], nblk = row_ptr[r1] - blk0;
+  const uint32_t t = threadIdx.x;
+  uint32_t cnt[4] = {0, 0, 0, 0};
+  if(t < nblk)
+    for(uint32_t j = contrib_ptr[blk0 + t]; j < contrib_ptr[blk0 + t + 1]; ++j)
+      ++cnt[passes == 1 ? 0 : (contrib[j] & 15u)];
+  for(int p = 0; p < 4; ++p) s_cnt[p][t] = cnt[p];
+  s_cnt[4][t] = t < nblk ? acc_cnt[blk0 + t] : 0;
+  __syncthreads();
+  if(t < 5) {   // exclusive scans over the blocks: one per pass, and the accumulator numbering
+    uint32_t run = 0;
+    for(uint32_t j = 0; j < nblk; ++j) {
+      const uint32_t c = s_cnt[t][j];
+      s_cnt[t][j] = run;
+      run += c;
+    }
+  }
+  __syncthreads();
+  if(t < nblk) {
+    // row of block t: row_ptr[row] <= blk0 + t < row_ptr[row + 1]; q counts the incidences of that row
+    uint32_t lo = r0, hi = r1;
+    while(hi - lo > 1) {
+      const uint32_t mid = (lo + hi) >> 1;
+      if(row_ptr[mid] <= blk0 + t) lo = mid; else hi = mid;
+    }
+    const uint32_t ib = inc_ptr[lo];
+    uint32_t beg[4];
+    for(int p = 0; p < 4; ++p) {
+      beg[p] = s_cnt[p][t];
+      blk_beg[(uint64_t)(blk0 + t) * 4 + p] = (uint8_t)beg[p];
+    }
+    blk_acc[blk0 + t] = (uint16_t)(s_cnt[4][t] | ((uint32_t)acc_cnt[blk0 + t] << 8));
+    for(uint32_t j = contrib_ptr[blk0 + t]; j < contrib_ptr[blk0 + t + 1]; ++j) {
+      const uint32_t c = contrib[j], b = c & 15u, q = c >> 4;
+      const uint32_t p = passes == 1 ? 0 : b;
+      inc_pos[(uint64_t)(ib + q) * pos_stride + b] = (uint8_t)(beg[p]++);
+    }
+  }
+}
+
 template<typename KeyT>
 void radix_sort(Context& c, DevBuf& in, DevBuf& out, uint64_t n, int end_bit)
 {
@@ -235,43 +321,14 @@ void build_element_structures(Lss& L)
     HIP_OK(hipGetLastError());
   }
 
-  // ---- host-side schedule of rows into assembly workgroups (structure only) ----
-  std::vector<uint32_t> inc_ptr(N + 1), row_ptr(N + 1);
-  HIP_OK(hipMemcpyAsync(inc_ptr.data(), L.inc_ptr.ptr, sizeof(uint32_t) * (N + 1), hipMemcpyDeviceToHost, c.stream));
-  HIP_OK(hipMemcpyAsync(row_ptr.data(), L.row_ptr.ptr, sizeof(uint32_t) * (N + 1), hipMemcpyDeviceToHost, c.stream));
-  HIP_OK(hipStreamSynchronize(c.stream));
-  // a workgroup holds at most kAsmThreads incidences (one per thread) and kAsmThreads CSR blocks
-  // (one owner thread per block)
-  std::vector<uint32_t> wg;
-  wg.push_back(0);
-  uint32_t r = 0;
-  while(r < N) {
-    uint32_t r1 = r;
-    uint64_t ninc = 0, nblk = 0;
-    while(r1 < N) {
-      const uint64_t di = inc_ptr[r1 + 1] - inc_ptr[r1];
-      const uint64_t db = row_ptr[r1 + 1] - row_ptr[r1];
-      if(di > (uint64_t)kAsmThreads || db > (uint64_t)kAsmThreads)
-        CF3_THROW(CF3HIP_ERR_LIMIT, "node " + std::to_string(r1) + " has " + std::to_string(di) +
-                                        " incident elements and " + std::to_string(db) +
-                                        " neighbours: above the assembly workgroup limit of " +
-                                        std::to_string(kAsmThreads));
-      if(ninc + di > (uint64_t)kAsmThreads || nblk + db > (uint64_t)kAsmThreads || r1 - r >= (uint32_t)kAsmThreads)
-        break;
-      ninc += di;
-      nblk += db;
-      ++r1;
-    }
-    wg.push_back(r1);
-    r = r1;
-  }
-  L.n_wg = (uint32_t)(wg.size() - 1);
-  L.wg_rows.alloc(&c, sizeof(uint32_t) * wg.size());
-  HIP_OK(hipMemcpyAsync(L.wg_rows.ptr, wg.data(), sizeof(uint32_t) * wg.size(), hipMemcpyHostToDevice, c.stream));
+  const bool hexa = M.etype == CF3HIP_HEXA3D;
+  L.passes = hexa ? 1 : (int)k;
+  L.pos_stride = hexa ? 8 : 4;
 
   // ---- per-slot contribution lists sorted by (slot, b, q) ----
-  L.contrib_ptr.alloc(&c, sizeof(uint32_t) * (L.n_blocks + 1));
-  L.contrib.alloc(&c, sizeof(uint16_t) * npairs);
+  DevBuf contrib_ptr, contrib, acc_cnt, row_acc;
+  contrib_ptr.alloc(&c, sizeof(uint32_t) * (L.n_blocks + 1));
+  contrib.alloc(&c, sizeof(uint16_t) * npairs);
   {
     DevBuf keys, sorted;
     keys.alloc(&c, sizeof(uint64_t) * npairs);
@@ -281,11 +338,65 @@ void build_element_structures(Lss& L)
     HIP_OK(hipGetLastError());
     radix_sort<uint64_t>(c, keys, sorted, npairs, bits_for(((uint64_t)L.n_blocks << 16) - 1));
     hipLaunchKernelGGL(k_row_ptr_from_keys, dim3(grid_for(L.n_blocks + 1, 256)), dim3(256), 0, c.stream,
-                       sorted.as<uint64_t>(), npairs, L.n_blocks, (uint64_t)65536, L.contrib_ptr.as<uint32_t>());
+                       sorted.as<uint64_t>(), npairs, L.n_blocks, (uint64_t)65536, contrib_ptr.as<uint32_t>());
     hipLaunchKernelGGL(k_contrib_from_keys, dim3(grid_for(npairs, 256)), dim3(256), 0, c.stream,
-                       sorted.as<uint64_t>(), npairs, L.contrib.as<uint16_t>());
+                       sorted.as<uint64_t>(), npairs, contrib.as<uint16_t>());
     HIP_OK(hipGetLastError());
   }
+  // accumulator threads per block and per row
+  acc_cnt.alloc(&c, L.n_blocks);
+  row_acc.alloc(&c, sizeof(uint32_t) * N);
+  hipLaunchKernelGGL(k_block_acc, dim3(grid_for(L.n_blocks, 256)), dim3(256), 0, c.stream, L.n_blocks, L.passes,
+                     contrib_ptr.as<uint32_t>(), contrib.as<uint16_t>(), acc_cnt.as<uint8_t>());
+  hipLaunchKernelGGL(k_row_acc, dim3(grid_for(N, 256)), dim3(256), 0, c.stream, (uint32_t)N, L.row_ptr.as<uint32_t>(),
+                     acc_cnt.as<uint8_t>(), row_acc.as<uint32_t>());
+  HIP_OK(hipGetLastError());
+
+  // ---- host-side schedule of rows into assembly workgroups (structure only) ----
+  std::vector<uint32_t> inc_ptr(N + 1), racc(N);
+  HIP_OK(hipMemcpyAsync(inc_ptr.data(), L.inc_ptr.ptr, sizeof(uint32_t) * (N + 1), hipMemcpyDeviceToHost, c.stream));
+  HIP_OK(hipMemcpyAsync(racc.data(), row_acc.ptr, sizeof(uint32_t) * N, hipMemcpyDeviceToHost, c.stream));
+  HIP_OK(hipStreamSynchronize(c.stream));
+  // a workgroup holds at most inc_limit incidences (simplices: one incidence per thread; hexahedra:
+  // one (incidence, column node) pair per thread) and kAsmThreads accumulator threads
+  const uint64_t inc_limit = hexa ? kAsmThreads / k : kAsmThreads;
+  std::vector<uint32_t> wg;
+  wg.push_back(0);
+  uint32_t r = 0;
+  while(r < N) {
+    uint32_t r1 = r;
+    uint64_t ninc = 0, nacc = 0;
+    while(r1 < N) {
+      const uint64_t di = inc_ptr[r1 + 1] - inc_ptr[r1];
+      const uint64_t da = racc[r1];
+      if(di > inc_limit || da > (uint64_t)kAsmThreads)
+        CF3_THROW(CF3HIP_ERR_LIMIT, "node " + std::to_string(r1) + " has " + std::to_string(di) +
+                                        " incident elements and needs " + std::to_string(da) +
+                                        " accumulators: above the assembly workgroup limits of " +
+                                        std::to_string(inc_limit) + " and " + std::to_string(kAsmThreads));
+      if(ninc + di > inc_limit || nacc + da > (uint64_t)kAsmThreads || r1 - r >= (uint32_t)kAsmThreads)
+        break;
+      ninc += di;
+      nacc += da;
+      ++r1;
+    }
+    wg.push_back(r1);
+    r = r1;
+  }
+  L.n_wg = (uint32_t)(wg.size() - 1);
+  L.wg_rows.alloc(&c, sizeof(uint32_t) * wg.size());
+  HIP_OK(hipMemcpyAsync(L.wg_rows.ptr, wg.data(), sizeof(uint32_t) * wg.size(), hipMemcpyHostToDevice, c.stream));
+
+  // ---- staging maps ----
+  L.inc_pos.alloc(&c, (size_t)nek * L.pos_stride);
+  L.blk_beg.alloc(&c, (size_t)L.n_blocks * 4);
+  L.blk_acc.alloc(&c, sizeof(uint16_t) * L.n_blocks);
+  lss_fill8(c, L.inc_pos.as<uint8_t>(), (size_t)nek * L.pos_stride, 0);
+  hipLaunchKernelGGL(k_stage_maps, dim3(L.n_wg), dim3(kAsmThreads), 0, c.stream, L.passes, L.pos_stride,
+                     L.wg_rows.as<uint32_t>(), L.row_ptr.as<uint32_t>(), L.inc_ptr.as<uint32_t>(),
+                     contrib_ptr.as<uint32_t>(), contrib.as<uint16_t>(), acc_cnt.as<uint8_t>(),
+                     L.inc_pos.as<uint8_t>(), L.blk_beg.as<uint8_t>(), L.blk_acc.as<uint16_t>());
+  HIP_OK(hipGetLastError());
   HIP_OK(hipStreamSynchronize(c.stream));
 }
 
