@@ -117,6 +117,7 @@ def lib():
         "cf3hip_fields_set": [_vp, C.c_int, _d],
         "cf3hip_fields_get": [_vp, C.c_int, _d],
         "cf3hip_fields_initialize": [_vp, C.c_double],
+        "cf3hip_fields_copy": [_vp, C.c_int, C.c_int],
         "cf3hip_ns_linearize": [_vp],
         "cf3hip_ns_assemble": [_vp, _vp, C.POINTER(NSParams)],
         "cf3hip_ns_update": [_vp, _vp],
@@ -223,6 +224,9 @@ class Fields:
 
     def initialize(self, nu):
         check(lib().cf3hip_fields_initialize(self.h, nu))
+
+    def copy(self, dst, src):
+        check(lib().cf3hip_fields_copy(self.h, dst, src))
 
     def linearize(self):
         check(lib().cf3hip_ns_linearize(self.h))

@@ -90,6 +90,28 @@ def inflow_cube_tets(n, seed=DEFAULT_SEED, nu=1e-3, dt=5e-3, jitter=0.1, amp=0.0
                 bc_regions=regions)
 
 
+def lid_cavity_cube_hexas(n, seed=DEFAULT_SEED, nu=2e-3, dt=5e-2, jitter=0.05, random_velocity=False):
+    """Unit cube, n^3 trilinear Q1 hexahedra (create_box node order), node jitter +-5% h (tangential on
+    the boundary); lid u = (1, 0, 0) on z = 1 applied last, no-slip on the other faces, p = 0 at node 0.
+    Velocity starts at rest (or U[-1,1] per node with random_velocity, for parity checks), p = 0.  The
+    steady solve is a sequence of Picard iterations, each an implicit step of size dt with u_adv = u."""
+    coords, conn = meshgen.box_hexas(n, n, n)
+    dims = (n, n, n)
+    coords, used = meshgen.jitter(coords, dims, jitter, seed)
+    N = coords.shape[0]
+    if random_velocity:
+        u0 = meshgen.uniform(seed, N * 3, -1.0, 1.0, offset=used).reshape(N, 3)
+    else:
+        u0 = np.zeros((N, 3))
+    zero = {0: 0.0, 1: 0.0, 2: 0.0}
+    regions = [("xmin", _face_nodes(dims, 0, 0), zero), ("xmax", _face_nodes(dims, 0, 1), zero),
+               ("ymin", _face_nodes(dims, 1, 0), zero), ("ymax", _face_nodes(dims, 1, 1), zero),
+               ("zmin", _face_nodes(dims, 2, 0), zero), ("lid", _face_nodes(dims, 2, 1), {0: 1.0, 1: 0.0, 2: 0.0}),
+               ("pressure", np.array([0], np.uint32), {3: 0.0})]
+    return Case("lid_cavity_cube_hexas_%d" % n, cabi.HEXA3D, coords, conn, dims, nu, dt, u0=u0, p0=np.zeros(N),
+                bc_regions=regions)
+
+
 def bc_lists(case, u, p):
     """Ordered (nodes, eqs, values) of the Dirichlet increments new - old for the current fields."""
     nodes, eqs, vals = [], [], []
@@ -160,6 +182,23 @@ class GpuCase:
         prm.precision = self.precision
         st = s.ns_solve(self.fields, prm, solver_params, raise_on_failure)   # SolveLSS
         s.update(self.fields)                                     # UpdateActions
+        self.u = self.fields.get(cabi.FIELD_U)
+        self.p = self.fields.get(cabi.FIELD_P)
+        return st
+
+    def picard_step(self, solver_params=None, raise_on_failure=True):
+        """One Picard iteration of the steady solve: u_adv := u, then the implicit step."""
+        self.fields.copy(cabi.FIELD_UADV, cabi.FIELD_U)
+        s = self.system
+        self.assemble(self.precision)
+        n, e, v = bc_lists(self.case, self.u, self.p)
+        s.dirichlet(n, e, v)
+        if solver_params is None:
+            solver_params = cabi.solver_params(tol=self.case.tol, precision=self.precision)
+        prm = cabi.NSParams.from_buffer_copy(self.prm)
+        prm.precision = self.precision
+        st = s.ns_solve(self.fields, prm, solver_params, raise_on_failure)
+        s.update(self.fields)
         self.u = self.fields.get(cabi.FIELD_U)
         self.p = self.fields.get(cabi.FIELD_P)
         return st

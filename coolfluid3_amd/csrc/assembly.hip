@@ -20,80 +20,13 @@
 //     the shape-function gradients of nodes 1..d for affine elements)
 //   combine: NavierStokesAssembly.hpp:69-71 (rhs = -A x + a before the p rows are divided by theta;
 //     theta * (A_p / theta) is formed as (theta * (1/theta)) * A_p, equal to within one rounding)
-#include <type_traits>
-
-#include "internal.hpp"
+#include "assembly_common.hpp"
 
 namespace cf3hip {
 
+using namespace asmk;
+
 namespace {
-
-constexpr int kThreads = kAsmThreads;   // one incidence and at most one owned CSR block per thread
-
-struct AsmArgs {
-  const double4* xp;   // packed coordinates (x, y, z | 0, 0)
-  const double4* st;   // packed state (u, v, w, p) in 3D, (u, v, p, 0) in 2D
-  const double4* ad;   // packed (u_adv x, y, z | 0, nu_eff)
-  const double* g[3];  // body force (SoA), generic expression only
-  const uint32_t* conn;
-  const uint32_t* wg_rows;
-  const uint32_t* row_ptr;
-  const uint32_t* inc_ptr;
-  const uint32_t* inc;
-  const uint8_t* inc_pos;   // per incidence: staging row of its block in pass b (byte b)
-  const uint8_t* blk_beg;   // per block: first staging row of pass p (byte p)
-  const uint16_t* blk_acc;  // per block: first accumulator thread (low byte), accumulator count (high byte)
-  void* values;
-  double* rhs;
-  const double* xt;    // matrix-free residual: x with constrained columns set to their first value
-  double* res;         // matrix-free residual output
-  double rho, inv_rho, dt, theta, thq, invdt, f4dt2;   // thq = theta * (1 / theta); f4dt2 = 4 / dt^2
-  int supg_type;
-  double c1, c2, a_ps, a_su, a_bu;
-  int* status;
-};
-
-template<int B, int K, typename F>
-__device__ __forceinline__ void static_for(F&& f)
-{
-  if constexpr(B < K) {
-    f(std::integral_constant<int, B>{});
-    static_for<B + 1, K>(f);
-  }
-}
-
-// one LDS row of a staged element block: 16-byte aligned, odd number of 16-byte chunks (bank spread)
-template<typename VR, int NN>
-constexpr int lds_row()
-{
-  constexpr int bytes = (NN * (int)sizeof(VR) + 15) / 16 * 16;
-  constexpr int padded = (bytes / 16) % 2 == 0 ? bytes + 16 : bytes;
-  return padded / (int)sizeof(VR);
-}
-
-__device__ __forceinline__ void unpack(const double4& v, double o[4]) { o[0] = v.x; o[1] = v.y; o[2] = v.z; o[3] = v.w; }
-
-template<int K>
-__device__ __forceinline__ void load_conn(const uint32_t* conn, uint32_t e, uint32_t cn[K])
-{
-  if constexpr(K == 4) {
-    const uint4 c = reinterpret_cast<const uint4*>(conn)[e];
-    cn[0] = c.x; cn[1] = c.y; cn[2] = c.z; cn[3] = c.w;
-  } else {
-#pragma unroll
-    for(int m = 0; m < K; ++m) cn[m] = conn[(uint64_t)e * K + m];
-  }
-}
-
-// nodal unknowns (u.., p) of node nd in arithmetic type R
-template<int NEQ, typename R>
-__device__ __forceinline__ void load_state(const AsmArgs& g, uint32_t nd, R xb[NEQ])
-{
-  double s[4];
-  unpack(g.st[nd], s);
-#pragma unroll
-  for(int c = 0; c < NEQ; ++c) xb[c] = (R)s[c];
-}
 
 // ---------------------------------------------------------------------------------------------
 // Specialised P1 simplex (D = 2 triangle, D = 3 tetrahedron) in arithmetic type R
@@ -194,7 +127,7 @@ struct SpecP1 {
     }
     const double nu_mean = fabs(nus * (1.0 / K));
     const R nu = (R)nu_mean;
-    R V, invV, tsu, tbu;
+    R V = R(0), invV, tsu, tbu;
     bool fp64_geom = sizeof(R) == sizeof(double);
     if constexpr(sizeof(R) != sizeof(double)) {
       R Xr[K][D];
@@ -589,25 +522,6 @@ struct GenericP1 {
 // Row-owner assembly kernel.  RHS = false (FP32 matrix pass of the precision form) skips the
 // right-hand side, which is formed by the FP64 pass.
 // ---------------------------------------------------------------------------------------------
-// LDS row access in 16-byte pieces
-template<typename VR, int NN>
-__device__ __forceinline__ void lds_load_row(const VR* src, VR v[NN])
-{
-  if constexpr((NN * sizeof(VR)) % 16 == 0) {
-    typedef typename std::conditional<sizeof(VR) == 8, double2, float4>::type V16;
-    constexpr int W = 16 / sizeof(VR);
-#pragma unroll
-    for(int c = 0; c < NN / W; ++c) {
-      const V16 x = reinterpret_cast<const V16*>(src)[c];
-      if constexpr(W == 2) { v[2 * c] = x.x; v[2 * c + 1] = x.y; }
-      else { v[4 * c] = x.x; v[4 * c + 1] = x.y; v[4 * c + 2] = x.z; v[4 * c + 3] = x.w; }
-    }
-  } else {
-#pragma unroll
-    for(int c = 0; c < NN; ++c) v[c] = src[c];
-  }
-}
-
 template<class Op, typename R, typename VR, bool RHS>
 __global__ __launch_bounds__(kThreads) void k_assemble(AsmArgs g)
 {
@@ -706,7 +620,7 @@ __global__ __launch_bounds__(kThreads) void k_assemble(AsmArgs g)
   if constexpr(RHS) {
     __syncthreads();
     if(active) {
-      op.rhs_source(g, rhs);
+      if(g.body_force) op.rhs_source(g, rhs);
 #pragma unroll
       for(int r = 0; r < NEQ; ++r) s_rhs[t * NEQ + r] = rhs[r];
     }
@@ -765,7 +679,7 @@ __global__ __launch_bounds__(kThreads) void k_residual(AsmArgs g)
 #pragma unroll
         for(int c = 0; c < NEQ; ++c) y[r] += mb[r * NEQ + c] * xt[c];
     });
-    op.rhs_source(g, rhs);
+    if(g.body_force) op.rhs_source(g, rhs);
 #pragma unroll
     for(int r = 0; r < NEQ; ++r) s_res[t * NEQ + r] = rhs[r] - y[r];
   }
@@ -869,9 +783,8 @@ AsmArgs make_args(Lss& L, Fields& F, const cf3hip_ns_params& prm)
   if(!spec && prm.supg_type == CF3HIP_SUPG_CF2)
     CF3_THROW(CF3HIP_ERR_NOT_IMPLEMENTED,
               "supg_type cf2 divides by sum_n u.grad(N_n), identically zero by the partition of unity");
-  if(M.etype == CF3HIP_HEXA3D)
-    CF3_THROW(CF3HIP_ERR_NOT_IMPLEMENTED, "hexahedra assembly is provided by the Q1 kernel");
   AsmArgs g = {};
+  g.body_force = F.has_g ? 1 : 0;
   g.xp = M.xp.as<double4>();
   g.st = F.st.as<double4>();
   g.ad = F.ad.as<double4>();
@@ -944,7 +857,9 @@ void assemble(Lss& L, Fields& F, const cf3hip_ns_params& prm)
   if(!c.async_mode) hipLaunchKernelGGL(k_clear_flag, dim3(1), dim3(1), 0, c.stream, g.status);
   g.values = fp32 ? L.values32.ptr : L.values.ptr;
 
-  if(spec && M.etype == CF3HIP_TETRA3D) {
+  if(M.etype == CF3HIP_HEXA3D) {
+    launch_hex(g, L, fp32 ? 1 : 0);
+  } else if(spec && M.etype == CF3HIP_TETRA3D) {
     if(fp32) launch<SpecP1<3, float>, float, float, false>(g, L);
     else launch<SpecP1<3, double>, double, double, true>(g, L);
   } else if(spec && M.etype == CF3HIP_TRIAG2D) {
@@ -972,7 +887,8 @@ void ns_residual(Lss& L, Fields& F, const cf3hip_ns_params& prm, const double* x
   g.xt = L.xtilde.as<double>();
   g.res = r;
   const bool spec = prm.use_specializations && M.etype != CF3HIP_HEXA3D;
-  if(spec && M.etype == CF3HIP_TETRA3D) launch_residual<SpecP1<3, double> >(g, L);
+  if(M.etype == CF3HIP_HEXA3D) launch_hex(g, L, 2);
+  else if(spec && M.etype == CF3HIP_TETRA3D) launch_residual<SpecP1<3, double> >(g, L);
   else if(spec && M.etype == CF3HIP_TRIAG2D) launch_residual<SpecP1<2, double> >(g, L);
   else if(M.etype == CF3HIP_TETRA3D) launch_residual<GenericP1<3> >(g, L);
   else launch_residual<GenericP1<2> >(g, L);

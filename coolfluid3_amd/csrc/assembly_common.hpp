@@ -1,0 +1,102 @@
+// Definitions shared by the element assembly translation units (assembly.hip, assembly_hex.hip).
+#pragma once
+
+#include <type_traits>
+
+#include "internal.hpp"
+
+namespace cf3hip {
+namespace asmk {
+
+constexpr int kThreads = kAsmThreads;   // one incidence and at most one owned CSR block per thread
+
+struct AsmArgs {
+  const double4* xp;   // packed coordinates (x, y, z | 0, 0)
+  const double4* st;   // packed state (u, v, w, p) in 3D, (u, v, p, 0) in 2D
+  const double4* ad;   // packed (u_adv x, y, z | 0, nu_eff)
+  const double* g[3];  // body force (SoA), generic expression only
+  const uint32_t* conn;
+  const uint32_t* wg_rows;
+  const uint32_t* row_ptr;
+  const uint32_t* inc_ptr;
+  const uint32_t* inc;
+  const uint8_t* inc_pos;   // per incidence: staging row of its block in pass b (byte b)
+  const uint8_t* blk_beg;   // per block: first staging row of pass p (byte p)
+  const uint16_t* blk_acc;  // per block: first accumulator thread (low byte), accumulator count (high byte)
+  void* values;
+  double* rhs;
+  const double* xt;    // matrix-free residual: x with constrained columns set to their first value
+  double* res;         // matrix-free residual output
+  double rho, inv_rho, dt, theta, thq, invdt, f4dt2;   // thq = theta * (1 / theta); f4dt2 = 4 / dt^2
+  int supg_type;
+  int body_force;      // nonzero when the body force field holds a nonzero value
+  double c1, c2, a_ps, a_su, a_bu;
+  int* status;
+};
+
+template<int B, int K, typename F>
+__device__ __forceinline__ void static_for(F&& f)
+{
+  if constexpr(B < K) {
+    f(std::integral_constant<int, B>{});
+    static_for<B + 1, K>(f);
+  }
+}
+
+// one LDS row of a staged element block: 16-byte aligned, odd number of 16-byte chunks (bank spread)
+template<typename VR, int NN>
+constexpr int lds_row()
+{
+  constexpr int bytes = (NN * (int)sizeof(VR) + 15) / 16 * 16;
+  constexpr int padded = (bytes / 16) % 2 == 0 ? bytes + 16 : bytes;
+  return padded / (int)sizeof(VR);
+}
+
+__device__ __forceinline__ void unpack(const double4& v, double o[4]) { o[0] = v.x; o[1] = v.y; o[2] = v.z; o[3] = v.w; }
+
+template<int K>
+__device__ __forceinline__ void load_conn(const uint32_t* conn, uint32_t e, uint32_t cn[K])
+{
+  if constexpr(K == 4) {
+    const uint4 c = reinterpret_cast<const uint4*>(conn)[e];
+    cn[0] = c.x; cn[1] = c.y; cn[2] = c.z; cn[3] = c.w;
+  } else {
+#pragma unroll
+    for(int m = 0; m < K; ++m) cn[m] = conn[(uint64_t)e * K + m];
+  }
+}
+
+// nodal unknowns (u.., p) of node nd in arithmetic type R
+template<int NEQ, typename R>
+__device__ __forceinline__ void load_state(const AsmArgs& g, uint32_t nd, R xb[NEQ])
+{
+  double s[4];
+  unpack(g.st[nd], s);
+#pragma unroll
+  for(int c = 0; c < NEQ; ++c) xb[c] = (R)s[c];
+}
+
+// LDS row access in 16-byte pieces
+template<typename VR, int NN>
+__device__ __forceinline__ void lds_load_row(const VR* src, VR v[NN])
+{
+  if constexpr((NN * sizeof(VR)) % 16 == 0) {
+    typedef typename std::conditional<sizeof(VR) == 8, double2, float4>::type V16;
+    constexpr int W = 16 / sizeof(VR);
+#pragma unroll
+    for(int c = 0; c < NN / W; ++c) {
+      const V16 x = reinterpret_cast<const V16*>(src)[c];
+      if constexpr(W == 2) { v[2 * c] = x.x; v[2 * c + 1] = x.y; }
+      else { v[4 * c] = x.x; v[4 * c + 1] = x.y; v[4 * c + 2] = x.z; v[4 * c + 3] = x.w; }
+    }
+  } else {
+#pragma unroll
+    for(int c = 0; c < NN; ++c) v[c] = src[c];
+  }
+}
+
+// hexahedra (assembly_hex.hip): mode 0 = FP64 matrix + rhs, 1 = FP32 matrix, 2 = matrix-free residual
+void launch_hex(const AsmArgs& g, const Lss& L, int mode);
+
+}  // namespace asmk
+}  // namespace cf3hip

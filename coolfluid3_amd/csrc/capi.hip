@@ -744,11 +744,16 @@ int cf3hip_fields_set(cf3hip_fields fields, int which, const double* host)
   DevBuf* b = field_buf(F, which, nc);
   const uint32_t n = F.mesh->n_nodes;
   std::vector<double> comp(n);
+  bool nonzero = false;
   for(int d = 0; d < nc; ++d) {
-    for(uint32_t i = 0; i < n; ++i) comp[i] = host[(uint64_t)i * nc + d];
+    for(uint32_t i = 0; i < n; ++i) {
+      comp[i] = host[(uint64_t)i * nc + d];
+      nonzero = nonzero || comp[i] != 0.;
+    }
     h2d(*F.mesh->ctx, b[d].ptr, comp.data(), sizeof(double) * n);
     HIP_OK(hipStreamSynchronize(F.mesh->ctx->stream));
   }
+  if(which == CF3HIP_FIELD_G) F.has_g = nonzero;
   API_END
 }
 
@@ -784,6 +789,23 @@ int cf3hip_fields_initialize(cf3hip_fields fields, double nu)
   }
   lss_fill(c, F.nu.as<double>(), M.n_nodes, nu);
   HIP_OK(hipStreamSynchronize(c.stream));
+  API_END
+}
+
+int cf3hip_fields_copy(cf3hip_fields fields, int dst, int src)
+{
+  API_BEGIN
+  REQUIRE(fields, CF3HIP_ERR_INVALID_ARG, "null argument");
+  Fields& F = fields->f;
+  Context& c = *F.mesh->ctx;
+  int nd = 0, ns = 0;
+  DevBuf* bd = field_buf(F, dst, nd);
+  DevBuf* bs = field_buf(F, src, ns);
+  REQUIRE(nd == ns, CF3HIP_ERR_INVALID_ARG, "fields of different shapes");
+  for(int d = 0; d < nd; ++d)
+    if(bd[d].ptr != bs[d].ptr) lss_copy(c, bd[d].as<double>(), bs[d].as<double>(), F.mesh->n_nodes);
+  if(dst == CF3HIP_FIELD_G) F.has_g = F.has_g || src != CF3HIP_FIELD_G;
+  if(!c.async_mode) HIP_OK(hipStreamSynchronize(c.stream));
   API_END
 }
 

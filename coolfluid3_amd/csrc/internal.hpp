@@ -141,6 +141,7 @@ struct Fields {
   // packed double4 node records refreshed by every assembly / residual call:
   // st = (u, v, w, p) in 3D, (u, v, p, 0) in 2D; ad = (u_adv x, y, z | 0, nu_eff)
   DevBuf st, ad;
+  bool has_g = false;   // the body force holds a nonzero value (its source term is skipped otherwise)
 };
 
 // ---------------------------------------------------------------------------------------------

@@ -188,6 +188,9 @@ int cf3hip_fields_get(cf3hip_fields fields, int which, double* host);
 /* initial conditions of NavierStokes::on_initial_conditions_set (NavierStokes.cpp:194-205):
  * u_adv = u1 = u2 = u3 = u, nu_eff = nu */
 int cf3hip_fields_initialize(cf3hip_fields fields, double nu);
+/* device copy of one field into another of the same shape (e.g. u_adv := u for a Picard iteration
+ * of a steady solve, the linearisation of NavierStokes' steady variants); stream-ordered */
+int cf3hip_fields_copy(cf3hip_fields fields, int dst, int src);
 
 /* ---- NavierStokes LSS action (plugins/UFEM/src/UFEM/NavierStokes.hpp:34-118) ---- */
 typedef struct cf3hip_ns_params {

@@ -40,13 +40,13 @@ def row_rel_err(gpu, ref, neq, start):
     return worst
 
 
-def oracle_system(o, case, spec=True, supg=pyoracle.SUPG_METRIC, theta=1.0):
+def oracle_system(o, case, spec=True, supg=pyoracle.SUPG_METRIC, theta=1.0, g=None):
     start, cols, slots = o.build_sparsity(case.conn, case.n_nodes)
     prm = pyoracle.ns_params(rho=case.rho, dt=case.dt, theta=theta, use_specializations=spec, supg_type=supg)
     ua = o.linearize_u(case.u0, case.u0, case.u0, case.u0)
-    et = {cabi.TRIAG2D: pyoracle.TRIAG2D, cabi.TETRA3D: pyoracle.TETRA3D}[case.etype]
+    et = {cabi.TRIAG2D: pyoracle.TRIAG2D, cabi.TETRA3D: pyoracle.TETRA3D, cabi.HEXA3D: pyoracle.HEXA3D}[case.etype]
     vals, rhs = o.ns_assemble(et, prm, case.coords, case.conn, case.u0, case.p0, ua,
-                              np.full(case.n_nodes, case.nu), start, cols, slots)
+                              np.full(case.n_nodes, case.nu), start, cols, slots, g=g)
     return start, cols, slots, vals, rhs
 
 
@@ -62,10 +62,12 @@ CASES = {
     "tri8": lambda: cases.lid_cavity_2d(8),
     "tet5": lambda: cases.inflow_cube_tets(5),
     "tet12": lambda: cases.inflow_cube_tets(12),
+    "hex3": lambda: cases.lid_cavity_cube_hexas(3, random_velocity=True),
+    "hex5": lambda: cases.lid_cavity_cube_hexas(5, random_velocity=True),
 }
 
 
-@pytest.mark.parametrize("name", ["tri8", "tet5", "tet12"])
+@pytest.mark.parametrize("name", ["tri8", "tet5", "tet12", "hex5"])
 def test_sparsity_and_slot_map_bit_exact(ctx, oracle, name):
     case = CASES[name]()
     print("case", name, flush=True)
@@ -80,7 +82,9 @@ def test_sparsity_and_slot_map_bit_exact(ctx, oracle, name):
 @pytest.mark.parametrize("name,spec,supg,theta", [
     ("tri8", True, cabi.SUPG_METRIC, 1.0), ("tri8", False, cabi.SUPG_METRIC, 0.5),
     ("tet5", True, cabi.SUPG_METRIC, 1.0), ("tet5", False, cabi.SUPG_METRIC, 0.7),
-    ("tet5", False, cabi.SUPG_TEZDUYAR, 1.0), ("tet12", True, cabi.SUPG_METRIC, 1.0)])
+    ("tet5", False, cabi.SUPG_TEZDUYAR, 1.0), ("tet12", True, cabi.SUPG_METRIC, 1.0),
+    ("hex3", False, cabi.SUPG_METRIC, 1.0), ("hex5", False, cabi.SUPG_METRIC, 0.5),
+    ("hex3", False, cabi.SUPG_TEZDUYAR, 0.8)])
 def test_assembly_fp64(ctx, oracle, name, spec, supg, theta):
     case = CASES[name]()
     print("case", name, spec, supg, theta, flush=True)
@@ -98,7 +102,7 @@ def test_assembly_fp64(ctx, oracle, name, spec, supg, theta):
     g.close()
 
 
-@pytest.mark.parametrize("name", ["tri8", "tet12"])
+@pytest.mark.parametrize("name", ["tri8", "tet12", "hex5"])
 def test_assembly_fp32_error_stated(ctx, oracle, name):
     case = CASES[name]()
     start, cols, slots, vref, rref = oracle_system(oracle, case)
@@ -108,6 +112,21 @@ def test_assembly_fp32_error_stated(ctx, oracle, name):
     err = row_rel_err(v32, vref, case.dim + 1, start)
     print(f"FP32 assembly row-relative error vs FP64 oracle: {err:.3e}")
     assert err <= 2e-6
+    g.close()
+
+
+@pytest.mark.parametrize("name,spec", [("tet5", False), ("hex3", False)])
+def test_assembly_body_force(ctx, oracle, name, spec):
+    """The body-force source term a[u_i] += (N + tau_su u_adv.grad N)^T g of the generic expression."""
+    case = CASES[name]()
+    gforce = np.random.default_rng(7).uniform(-2, 2, (case.n_nodes, case.dim))
+    start, cols, slots, vref, rref = oracle_system(oracle, case, spec, g=gforce)
+    g = gpu_case(ctx, case, spec)
+    g.fields.set(cabi.FIELD_G, gforce)
+    g.assemble(cabi.FP64)
+    assert row_rel_err(g.system.values(), vref, case.dim + 1, start) <= 1e-12
+    r = g.system.vector(cabi.RHS)
+    assert np.abs(r - rref).max() <= 1e-12 * np.abs(rref).max()
     g.close()
 
 
@@ -208,9 +227,11 @@ def test_fp32_refinement_reaches_fp64_residual(ctx, oracle):
     g.close()
 
 
-def test_matrix_free_residual_matches_stored_system(ctx, oracle):
+@pytest.mark.parametrize("name", ["tet5", "hex3"])
+def test_matrix_free_residual_matches_stored_system(ctx, oracle, name):
     """r = b' - A' x from the FP64 element operator equals rhs - A x of the assembled, constrained system."""
-    case = CASES["tet5"]()
+    case = CASES[name]()
+    print("case", name, flush=True)
     g = gpu_case(ctx, case)
     g.assemble(cabi.FP64)
     n, e, vals = cases.bc_lists(case, case.u0, case.p0)
@@ -221,6 +242,8 @@ def test_matrix_free_residual_matches_stored_system(ctx, oracle):
     A = s.values()
     b = s.vector(cabi.RHS)
     r_ref = b - oracle.spmv(4, start, cols, A, x)
+    if case.etype == cabi.HEXA3D:
+        g.fields.copy(cabi.FIELD_UADV, cabi.FIELD_U)   # same linearisation as the assembly
     s.set_vector(cabi.SOLUTION, x)
     s.ns_residual_device(g.fields, g.prm, s.vector_ptr(cabi.SOLUTION), s.vector_ptr(cabi.RHS))
     r = s.vector(cabi.RHS)
@@ -250,6 +273,26 @@ def test_precision_form_matrix_free(ctx, oracle):
     assert st.converged and res <= 1e-8 * 1.001
     assert st.iterations <= 1.5 * st64.iterations + 5
     g.close()
+    g64.close()
+
+
+def test_hexa_picard_precision_form(ctx, oracle):
+    """Q1 lid-driven cavity, two Picard iterations: FP32 element matrices with the FP64 matrix-free
+    residual reach 1e-8 of the FP64 system; the iterates agree with the FP64 path (stated)."""
+    case = cases.lid_cavity_cube_hexas(6)
+    g32 = cases.GpuCase(ctx, case, precision=cabi.FP32, use_specializations=False)
+    g64 = cases.GpuCase(ctx, case, use_specializations=False)
+    for it in range(2):
+        sp = cabi.solver_params(tol=1e-8)
+        st64 = g64.picard_step(sp)
+        st32 = g32.picard_step(cabi.solver_params(tol=1e-8, precision=cabi.FP32))
+        assert st64.converged and st32.converged
+        assert st32.rel_residual <= 1e-8 * 1.001
+        err = np.abs(g32.u - g64.u).max() / np.abs(g64.u).max()
+        print(f"picard {it}: FP64 {st64.iterations} it, FP32 form {st32.iterations} it ({st32.outer} FP64 "
+              f"residuals); velocity difference {err:.2e}")
+        assert err <= 1e-3
+    g32.close()
     g64.close()
 
 
