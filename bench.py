@@ -5,10 +5,12 @@ block-CSR system, the quantity timed by plugins/UFEM/test/ptest-navier-stokes-as
 synthetic 64^3-hexahedra cube split in 6 P1 tetrahedra (1,572,864 elements, FP64 drop-in), with all
 inputs resident in HBM.  value = elements assembled per second (whole job).
 
-Also reported (outside the timed steps, same mesh): the dominant kernel's roofline (HIP-event
-duration of the assembly kernel against its algorithmic bytes), the FP64 SpMV bandwidth, one implicit
-time step to a 1e-8 relative residual in FP64 and in the FP32 + FP64-refinement form, and the CPU
-baseline (the oracle restatement, OpenMP over the host cores, bounded sample).
+Also reported (outside the timed steps): the dominant kernel's roofline (HIP-event duration of the
+assembly kernel against its algorithmic bytes), the FP64 SpMV bandwidth, one implicit time step to a
+1e-8 relative residual in FP64 and in the FP32 + FP64-refinement form on the same mesh, the CPU
+baseline (the oracle restatement, OpenMP over the host cores, bounded sample), and the other
+workloads: the precision form on the 128^3 tetrahedral cube with 1% skewed elements, the 160^3 Q1
+hexahedral lid-driven cavity (assembly and Picard iterations) and the 32x32 triangle cavity step.
 """
 import argparse
 import json
@@ -62,6 +64,104 @@ def cpu_baseline(case, budget_s=20.0):
             "single_thread_value": single}
 
 
+def timed_step(ctx, gc, sp, picard=False):
+    ctx.synchronize()
+    t0 = time.perf_counter()
+    st = gc.picard_step(sp, raise_on_failure=False) if picard else gc.step(sp, raise_on_failure=False)
+    ctx.synchronize()
+    ms = (time.perf_counter() - t0) * 1e3
+    return {"ms": ms, "iterations": st.iterations, "fp64_residuals": st.outer, "spmvs": st.spmvs,
+            "rel_residual": st.rel_residual, "converged": bool(st.converged)}
+
+
+def assembly_rates(ctx, gc, reps, k, dim):
+    """HIP-event kernel times of the FP64 and FP32 assembly on a resident case."""
+    from coolfluid3_amd import cabi
+    s = gc.system
+    out = {}
+    for label, prec, vb in (("fp64", cabi.FP64, 8), ("fp32", cabi.FP32, 4)):
+        gc.assemble(prec)
+        ctx.synchronize()
+        ctx.enable_timing(True)
+        for _ in range(reps):
+            gc.assemble(prec)
+        ctx.synchronize()
+        ms, n = ctx.kernel_time(cabi.KERNEL_ASSEMBLY)
+        kms = ms / max(n, 1)
+        alg = algorithmic_assembly_bytes(gc.case.n_nodes, gc.case.n_elems, k, dim, s.n_blocks, vb)
+        out[label] = {"kernel_ms": kms, "Melem_s": gc.case.n_elems / kms / 1e3,
+                      "GBps": alg / (kms * 1e-3) / 1e9, "hbm_frac": alg / (kms * 1e-3) / 1e9 / HBM_PEAK_GBS}
+    ctx.enable_timing(False)
+    return out
+
+
+def precision_workload(ctx, cells, steps=2):
+    """Precision form on the tetrahedral cube with ~1% elements of aspect ratio ~20 (nu = 5e-4):
+    FP32 element matrices + reliable-update BiCGStab with the FP64 matrix-free residual, against the
+    FP64 assembly and BiCGStab, both to a 1e-8 relative residual of the FP64 system.  Step 1 includes
+    one-time allocations; the per-step comparison uses the last step."""
+    from coolfluid3_amd import cabi, cases
+    case = cases.inflow_cube_tets(cells, nu=5e-4, dt=5e-3, skew=True)
+    res = {"workload": "unit cube %d^3 hexes x 6 P1 tets (%d elements, %d dofs), 1%% skewed (aspect ratio ~20), "
+                       "nu=5e-4 dt=5e-3" % (cells, case.n_elems, 4 * case.n_nodes)}
+    runs = {}
+    for label, prec in (("fp64", cabi.FP64), ("fp32_refined", cabi.FP32)):
+        gc = cases.GpuCase(ctx, case, precision=prec)
+        sp = cabi.solver_params(tol=1e-8, precision=prec, inner_tol=1e-4, max_iter=5000)
+        runs[label] = [timed_step(ctx, gc, sp) for _ in range(steps)]
+        runs[label + "_u"] = gc.u.copy()
+        if label == "fp32_refined":
+            res["assembly"] = assembly_rates(ctx, gc, 5, 4, 3)
+        gc.close()
+    res["fp64_steps"] = runs["fp64"]
+    res["fp32_refined_steps"] = runs["fp32_refined"]
+    res["speedup_per_step"] = runs["fp64"][-1]["ms"] / runs["fp32_refined"][-1]["ms"]
+    u64 = runs["fp64_u"]
+    res["velocity_rel_error_vs_fp64"] = float(np.abs(runs["fp32_refined_u"] - u64).max() / np.abs(u64).max())
+    return res
+
+
+HEX_KFLOP = 34.0   # algorithmic FP64 kflop per Q1 element (8 Gauss points, element-level counting)
+
+
+def hexa_workload(ctx, cells, picard):
+    """Q1 hexahedral lid-driven cavity (nu = 2e-3, 5% jitter): assembly rates, then Picard iterations
+    (u_adv := u, reassembly, solve to 1e-8) in FP64 and in the precision form."""
+    from coolfluid3_amd import cabi, cases
+    case = cases.lid_cavity_cube_hexas(cells)
+    res = {"workload": "unit cube %d^3 Q1 hexahedra (%d elements, %d dofs), lid-driven cavity, nu=2e-3, "
+                       "Picard step dt=5e-2" % (cells, case.n_elems, 4 * case.n_nodes)}
+    runs = {}
+    for label, prec in (("fp64", cabi.FP64), ("fp32_refined", cabi.FP32)):
+        gc = cases.GpuCase(ctx, case, precision=prec, use_specializations=False)
+        if label == "fp32_refined":
+            res["assembly"] = assembly_rates(ctx, gc, 3, 8, 3)
+            k64 = res["assembly"]["fp64"]["kernel_ms"]
+            res["assembly"]["fp64"]["fp64_alu_frac"] = case.n_elems * HEX_KFLOP * 1e3 / (k64 * 1e-3) / 78.6e12
+        sp = cabi.solver_params(tol=1e-8, precision=prec, inner_tol=1e-4, max_iter=5000)
+        runs[label] = [timed_step(ctx, gc, sp, picard=True) for _ in range(picard)]
+        runs[label + "_u"] = gc.u.copy()
+        gc.close()
+    res["fp64_picard"] = runs["fp64"]
+    res["fp32_refined_picard"] = runs["fp32_refined"]
+    res["speedup_per_iteration"] = runs["fp64"][-1]["ms"] / runs["fp32_refined"][-1]["ms"]
+    u64 = runs["fp64_u"]
+    res["velocity_rel_error_vs_fp64"] = float(np.abs(runs["fp32_refined_u"] - u64).max() /
+                                              max(np.abs(u64).max(), 1e-300))
+    return res
+
+
+def cavity2d_workload(ctx):
+    """32x32 triangle cavity: one implicit FP64 step, GMRES(30) + ILU0 to 1e-10."""
+    from coolfluid3_amd import cabi, cases
+    case = cases.lid_cavity_2d(32)
+    gc = cases.GpuCase(ctx, case)
+    steps = [timed_step(ctx, gc, None) for _ in range(2)]
+    gc.close()
+    return {"workload": "unit square 32x32 x 2 P1 triangles (2048 elements, 3267 dofs), nu=0.01 dt=0.01",
+            "steps": steps}
+
+
 def main():
     ap = argparse.ArgumentParser()
     ap.add_argument("--gpus", type=int, default=1)
@@ -70,6 +170,10 @@ def main():
     ap.add_argument("--cells", type=int, default=64, help="cube cells per side (64 = the drop-in-at-scale case)")
     ap.add_argument("--no-cpu-baseline", action="store_true")
     ap.add_argument("--no-solve", action="store_true")
+    ap.add_argument("--no-workloads", action="store_true", help="skip the 128^3 / 160^3 / 2D workloads")
+    ap.add_argument("--precision-cells", type=int, default=128)
+    ap.add_argument("--hexa-cells", type=int, default=160)
+    ap.add_argument("--picard", type=int, default=2)
     args = ap.parse_args()
     if args.gpus != 1:
         print("bench.py: this build measures one GPU only; multi-GPU runs are not part of it (use --gpus 1)",
@@ -174,6 +278,14 @@ def main():
         "extra": extra,
     }
     g.close()
+    if not args.no_workloads:
+        for key, fn in (("cavity_2d", lambda: cavity2d_workload(ctx)),
+                        ("precision_tets", lambda: precision_workload(ctx, args.precision_cells)),
+                        ("hexa_cavity", lambda: hexa_workload(ctx, args.hexa_cells, args.picard))):
+            t0 = time.perf_counter()
+            extra[key] = fn()
+            extra[key]["wall_s"] = time.perf_counter() - t0
+            print("bench: %s done in %.1f s" % (key, extra[key]["wall_s"]), file=sys.stderr, flush=True)
     ctx.close()
     if not args.no_cpu_baseline:
         out["cpu_baseline"] = cpu_baseline(case)

@@ -68,15 +68,22 @@ def lid_cavity_2d(n=32, seed=DEFAULT_SEED, nu=0.01, dt=0.01):
                 bc_regions=regions, solver="gmres")
 
 
-def inflow_cube_tets(n, seed=DEFAULT_SEED, nu=1e-3, dt=5e-3, jitter=0.1, amp=0.0):
+SKEW_SITE_PROB = 0.0125   # ~1% of the tetrahedra flattened to aspect ratio ~20
+
+
+def inflow_cube_tets(n, seed=DEFAULT_SEED, nu=1e-3, dt=5e-3, jitter=0.1, amp=0.0, skew=False):
     """Unit cube, n^3 hexahedra in 6 Kuhn tetrahedra, interior jitter +-10% h (tangential on the
     boundary); no-slip on all faces except x = 1 with u = (-16 y(1-y) z(1-z)(1 + A sin(pi y) sin(pi z)), 0, 0)
-    applied last; p = 0 at the node (0, n/2, n/2); velocity U[-1,1] per node, p = 0."""
+    applied last; p = 0 at the node (0, n/2, n/2); velocity U[-1,1] per node, p = 0.  With skew, about
+    1% of the elements are then flattened to aspect ratio ~20 (meshgen.skew_kuhn, draws after the
+    velocity draws)."""
     coords, conn = meshgen.box_tetras(n, n, n)
     dims = (n, n, n)
     coords, used = meshgen.jitter(coords, dims, jitter, seed)
     N = coords.shape[0]
     u0 = meshgen.uniform(seed, N * 3, -1.0, 1.0, offset=used).reshape(N, 3)
+    if skew:
+        coords, _, _ = meshgen.skew_kuhn(coords, dims, SKEW_SITE_PROB, seed, used + 3 * N)
     zero = {0: 0.0, 1: 0.0, 2: 0.0}
 
     def inflow(c):
@@ -86,7 +93,7 @@ def inflow_cube_tets(n, seed=DEFAULT_SEED, nu=1e-3, dt=5e-3, jitter=0.1, amp=0.0
                ("ymax", _face_nodes(dims, 1, 1), zero), ("zmin", _face_nodes(dims, 2, 0), zero),
                ("zmax", _face_nodes(dims, 2, 1), zero), ("inflow", _face_nodes(dims, 0, 1), {0: inflow, 1: 0.0, 2: 0.0}),
                ("pressure", np.array([(n // 2 * (n + 1) + n // 2) * (n + 1)], np.uint32), {3: 0.0})]
-    return Case("inflow_cube_tets_%d" % n, cabi.TETRA3D, coords, conn, dims, nu, dt, u0=u0, p0=np.zeros(N),
+    return Case("inflow_cube_tets_%d%s" % (n, "_skew" if skew else ""), cabi.TETRA3D, coords, conn, dims, nu, dt, u0=u0, p0=np.zeros(N),
                 bc_regions=regions)
 
 

@@ -134,7 +134,9 @@ struct SpecP1 {
       for(int m = 0; m < K; ++m)
         for(int d = 0; d < D; ++d) Xr[m][d] = (R)X[m][d];
       geometry<R>(Xr, n, V);
-      // strongly skewed elements (volume small against the edge lengths): geometry and tau in FP64
+      // strongly skewed elements: geometry and tau in FP64.  Criterion |V| < 0.0125 l_rms^3 (l_rms: rms
+      // edge length), i.e. aspect ratio (longest edge / smallest height) above ~14; the regular Kuhn
+      // tetrahedron has V = 0.0775 l_rms^3.
       R l2 = R(0);
       for(int i = 0; i < K; ++i)
         for(int j = i + 1; j < K; ++j)
@@ -142,7 +144,7 @@ struct SpecP1 {
       const R lm = sqrt(l2 * R(2.0 / (K * (K - 1))));
       R lp = lm;
       for(int d = 1; d < D; ++d) lp *= lm;
-      if(fabs(V) < R(0.05) * lp) fp64_geom = true;
+      if(fabs(V) < R(0.0125) * lp) fp64_geom = true;
     }
     if(fp64_geom) {
       double n64[K][D], V64, ts, tb;

@@ -93,3 +93,46 @@ def jitter(coords, dims, frac, seed, lengths=(1.0, 1.0, 1.0)):
 def tet_volumes(coords, conn):
     x = coords[conn]
     return np.einsum("ij,ij->i", np.cross(x[:, 1] - x[:, 0], x[:, 2] - x[:, 0]), x[:, 3] - x[:, 0]) / 6.0
+
+
+def skew_kuhn(coords, dims, site_prob, seed, offset, frac=0.93):
+    """Deterministic skewed elements in a Kuhn-split box: one draw per node in node order (stream
+    positions offset+1 ..); an interior node whose draw is below site_prob becomes a site unless a
+    site was already taken in its 3x3x3 neighbourhood (node order).  A site moves the fraction `frac`
+    of the way to its Kuhn-diagonal neighbour (i+1, j+1, k+1), which flattens the 6 tetrahedra of the
+    hexahedron between them to an aspect ratio (longest edge / smallest height) of about 20 for
+    frac = 0.93.  Returns (coords, sites, draws used)."""
+    nx, ny, nz = dims
+    n = coords.shape[0]
+    r = uniform(seed, n, 0.0, 1.0, offset=offset)
+    idx = np.arange(n)
+    i = idx % (nx + 1)
+    j = (idx // (nx + 1)) % (ny + 1)
+    k = idx // ((nx + 1) * (ny + 1))
+    interior = (i > 0) & (i < nx) & (j > 0) & (j < ny) & (k > 0) & (k < nz)
+    cand = np.nonzero(interior & (r < site_prob))[0]
+    taken = np.zeros(n, bool)
+    sites = []
+    ring = np.array([(dk * (ny + 1) + dj) * (nx + 1) + di
+                     for dk in (-1, 0, 1) for dj in (-1, 0, 1) for di in (-1, 0, 1)])
+    for s in cand:
+        if taken[s + ring].any():
+            continue
+        taken[s] = True
+        sites.append(s)
+    sites = np.array(sites, dtype=np.int64)
+    out = coords.copy()
+    diag = sites + 1 + (nx + 1) + (nx + 1) * (ny + 1)
+    out[sites] += frac * (coords[diag] - coords[sites])
+    return out, sites, n
+
+
+def tet_quality(coords, conn):
+    """Aspect ratio (longest edge / smallest height) of each tetrahedron."""
+    x = coords[conn]
+    v = np.abs(tet_volumes(coords, conn))
+    lmax = np.max([np.linalg.norm(x[:, a] - x[:, b], axis=1) for a in range(4) for b in range(a + 1, 4)], axis=0)
+    faces = ((1, 2, 3), (0, 2, 3), (0, 1, 3), (0, 1, 2))
+    amax = np.max([0.5 * np.linalg.norm(np.cross(x[:, f[1]] - x[:, f[0]], x[:, f[2]] - x[:, f[0]]), axis=1)
+                   for f in faces], axis=0)
+    return lmax * amax / (3.0 * v)

@@ -62,3 +62,39 @@ def test_case_definitions():
     t = cases.inflow_cube_tets(6)
     assert (t.n_nodes, t.n_elems) == (343, 1296)
     assert meshgen.tet_volumes(t.coords, t.conn).min() > 0
+
+
+def test_skewed_tetra_mesh():
+    """About 1% of the elements flattened to aspect ratio ~20, all valid, deterministic; the skew
+    draws come after the jitter and velocity draws, so the velocity field is unchanged."""
+    t = cases.inflow_cube_tets(24, skew=True)
+    t0 = cases.inflow_cube_tets(24)
+    ar = meshgen.tet_quality(t.coords, t.conn)
+    assert 0.008 <= (ar > 15).mean() <= 0.012
+    assert 18 <= ar.max() <= 40
+    assert meshgen.tet_quality(t0.coords, t0.conn).max() < 6
+    assert meshgen.tet_volumes(t.coords, t.conn).min() > 0
+    assert np.array_equal(t.u0, t0.u0)
+    t2 = cases.inflow_cube_tets(24, skew=True)
+    assert np.array_equal(t.coords, t2.coords)
+    # a moved node sits 93% of the way to its Kuhn-diagonal neighbour
+    moved = np.nonzero((t.coords != t0.coords).any(1))[0]
+    s = moved[0]
+    d = s + 1 + 25 + 25 * 25
+    assert np.allclose(t.coords[s], t0.coords[s] + 0.93 * (t0.coords[d] - t0.coords[s]), rtol=0, atol=1e-15)
+
+
+def test_hexa_cavity_case():
+    c = cases.lid_cavity_cube_hexas(4)
+    assert (c.n_nodes, c.n_elems) == (125, 64)
+    x = c.coords[c.conn]
+    # trilinear hexahedra stay valid under 5% jitter: Jacobian determinant at the centre > 0
+    j = np.stack([(x[:, [1, 2, 5, 6]].sum(1) - x[:, [0, 3, 4, 7]].sum(1)),
+                  (x[:, [2, 3, 6, 7]].sum(1) - x[:, [0, 1, 4, 5]].sum(1)),
+                  (x[:, [4, 5, 6, 7]].sum(1) - x[:, [0, 1, 2, 3]].sum(1))], axis=1) / 8
+    assert np.linalg.det(j).min() > 0
+    n, e, v = cases.bc_lists(c, c.u0, c.p0)
+    lid = {int(a) for a, q in zip(n, e) if q == 0}
+    assert (n[-1], e[-1]) == (0, 3)
+    top = set(range(4 * 25, 125))
+    assert top <= lid
