@@ -505,11 +505,10 @@ int cf3hip_matrix_set_row(cf3hip_lss lss, uint32_t blockrow, uint32_t ieq, doubl
   API_END
 }
 
-int cf3hip_matrix_add_values(cf3hip_lss lss, uint32_t nb, const uint32_t* indices, const double* mat)
+namespace {
+// BlockAccumulator access to the matrix: mode 0 set, 1 add, 2 get; mat row-major (nb*neq)^2
+void matrix_block_values(Lss& L, uint32_t nb, const uint32_t* indices, double* mat, int mode)
 {
-  API_BEGIN
-  REQUIRE(lss && indices && mat, CF3HIP_ERR_INVALID_ARG, "null argument");
-  Lss& L = lss->l;
   const int neq = L.neq, nn = neq * neq;
   const uint32_t n = nb * neq;
   for(uint32_t a = 0; a < nb; ++a) {
@@ -521,11 +520,91 @@ int cf3hip_matrix_add_values(cf3hip_lss lss, uint32_t nb, const uint32_t* indice
       double* p = L.values.as<double>() + (uint64_t)q * nn;
       d2h(*L.ctx, blk, p, sizeof(double) * nn);
       for(int al = 0; al < neq; ++al)
-        for(int be = 0; be < neq; ++be) blk[al * neq + be] += mat[(uint64_t)(a * neq + al) * n + b * neq + be];
-      h2d(*L.ctx, p, blk, sizeof(double) * nn);
+        for(int be = 0; be < neq; ++be) {
+          double& m = mat[(uint64_t)(a * neq + al) * n + b * neq + be];
+          double& v = blk[al * neq + be];
+          if(mode == 0) v = m; else if(mode == 1) v += m; else m = v;
+        }
+      if(mode != 2) {
+        h2d(*L.ctx, p, blk, sizeof(double) * nn);
+        HIP_OK(hipStreamSynchronize(L.ctx->stream));
+      }
+    }
+  }
+}
+
+// BlockAccumulator access to a system vector: mode 0 set, 1 add, 2 get; vals node-major nb*neq
+void vector_block_values(Lss& L, int which, uint32_t nb, const uint32_t* indices, double* vals, int mode)
+{
+  double* v = vec_ptr(L, which);
+  const int neq = L.neq;
+  for(uint32_t a = 0; a < nb; ++a) {
+    REQUIRE(indices[a] < L.n_nodes, CF3HIP_ERR_ILLEGAL_ENTRY, "Trying to access an illegal entry.");
+    double node[4];
+    double* p = v + (uint64_t)indices[a] * neq;
+    d2h(*L.ctx, node, p, sizeof(double) * neq);
+    for(int e = 0; e < neq; ++e) {
+      double& x = vals[(uint64_t)a * neq + e];
+      if(mode == 0) node[e] = x; else if(mode == 1) node[e] += x; else x = node[e];
+    }
+    if(mode != 2) {
+      h2d(*L.ctx, p, node, sizeof(double) * neq);
       HIP_OK(hipStreamSynchronize(L.ctx->stream));
     }
   }
+}
+}  // namespace
+
+int cf3hip_matrix_add_values(cf3hip_lss lss, uint32_t nb, const uint32_t* indices, const double* mat)
+{
+  API_BEGIN
+  REQUIRE(lss && indices && mat, CF3HIP_ERR_INVALID_ARG, "null argument");
+  matrix_block_values(lss->l, nb, indices, const_cast<double*>(mat), 1);
+  API_END
+}
+
+int cf3hip_matrix_set_values(cf3hip_lss lss, uint32_t nb, const uint32_t* indices, const double* mat)
+{
+  API_BEGIN
+  REQUIRE(lss && indices && mat, CF3HIP_ERR_INVALID_ARG, "null argument");
+  matrix_block_values(lss->l, nb, indices, const_cast<double*>(mat), 0);
+  API_END
+}
+
+int cf3hip_matrix_get_block_values(cf3hip_lss lss, uint32_t nb, const uint32_t* indices, double* mat)
+{
+  API_BEGIN
+  REQUIRE(lss && indices && mat, CF3HIP_ERR_INVALID_ARG, "null argument");
+  matrix_block_values(lss->l, nb, indices, mat, 2);
+  API_END
+}
+
+static int matrix_diagonal(cf3hip_lss lss, const double* diag, int mode)
+{
+  API_BEGIN
+  REQUIRE(lss && diag, CF3HIP_ERR_INVALID_ARG, "null argument");
+  Lss& L = lss->l;
+  DevBuf d;
+  d.alloc(L.ctx, sizeof(double) * L.dofs);
+  h2d(*L.ctx, d.ptr, diag, sizeof(double) * L.dofs);
+  lss_diagonal(L, d.as<double>(), mode);
+  HIP_OK(hipStreamSynchronize(L.ctx->stream));
+  API_END
+}
+
+int cf3hip_matrix_set_diagonal(cf3hip_lss lss, const double* diag) { return matrix_diagonal(lss, diag, 0); }
+int cf3hip_matrix_add_diagonal(cf3hip_lss lss, const double* diag) { return matrix_diagonal(lss, diag, 1); }
+
+int cf3hip_matrix_get_column_and_replace_to_zero(cf3hip_lss lss, uint32_t iblockcol, uint32_t ieq, double* values)
+{
+  API_BEGIN
+  REQUIRE(lss && values, CF3HIP_ERR_INVALID_ARG, "null argument");
+  Lss& L = lss->l;
+  REQUIRE(iblockcol < L.n_nodes && ieq < (uint32_t)L.neq, CF3HIP_ERR_ILLEGAL_ENTRY, "Trying to access an illegal entry.");
+  DevBuf d;
+  d.alloc(L.ctx, sizeof(double) * L.dofs);
+  lss_column_zero(L, iblockcol, (int)ieq, d.as<double>());
+  d2h(*L.ctx, values, d.ptr, sizeof(double) * L.dofs);
   API_END
 }
 
@@ -658,6 +737,65 @@ int cf3hip_vector_assign(cf3hip_lss lss, int to, int from)
   REQUIRE(lss, CF3HIP_ERR_NOT_CREATED, "system not created");
   Lss& L = lss->l;
   lss_copy(*L.ctx, vec_ptr(L, to), vec_ptr(L, from), L.dofs);
+  if(!L.ctx->async_mode) HIP_OK(hipStreamSynchronize(L.ctx->stream));
+  API_END
+}
+
+int cf3hip_vector_add_value(cf3hip_lss lss, int which, uint32_t blockrow, uint32_t ieq, double v)
+{
+  API_BEGIN
+  REQUIRE(lss, CF3HIP_ERR_NOT_CREATED, "system not created");
+  REQUIRE(blockrow < lss->l.n_nodes && ieq < (uint32_t)lss->l.neq, CF3HIP_ERR_ILLEGAL_ENTRY, "Trying to access an illegal entry.");
+  double* p = vec_ptr(lss->l, which) + (uint64_t)blockrow * lss->l.neq + ieq;
+  double x = 0.;
+  d2h(*lss->l.ctx, &x, p, sizeof(double));
+  x += v;
+  h2d(*lss->l.ctx, p, &x, sizeof(double));
+  HIP_OK(hipStreamSynchronize(lss->l.ctx->stream));
+  API_END
+}
+
+int cf3hip_vector_set_values(cf3hip_lss lss, int which, uint32_t nb, const uint32_t* indices, const double* vals)
+{
+  API_BEGIN
+  REQUIRE(lss && indices && vals, CF3HIP_ERR_INVALID_ARG, "null argument");
+  vector_block_values(lss->l, which, nb, indices, const_cast<double*>(vals), 0);
+  API_END
+}
+
+int cf3hip_vector_add_values(cf3hip_lss lss, int which, uint32_t nb, const uint32_t* indices, const double* vals)
+{
+  API_BEGIN
+  REQUIRE(lss && indices && vals, CF3HIP_ERR_INVALID_ARG, "null argument");
+  vector_block_values(lss->l, which, nb, indices, const_cast<double*>(vals), 1);
+  API_END
+}
+
+int cf3hip_vector_get_values(cf3hip_lss lss, int which, uint32_t nb, const uint32_t* indices, double* vals)
+{
+  API_BEGIN
+  REQUIRE(lss && indices && vals, CF3HIP_ERR_INVALID_ARG, "null argument");
+  vector_block_values(lss->l, which, nb, indices, vals, 2);
+  API_END
+}
+
+int cf3hip_vector_update(cf3hip_lss lss, int to, int from, double alpha)
+{
+  API_BEGIN
+  REQUIRE(lss, CF3HIP_ERR_NOT_CREATED, "system not created");
+  Lss& L = lss->l;
+  lss_axpby(*L.ctx, L.dofs, alpha, vec_ptr(L, from), 1., vec_ptr(L, to));
+  if(!L.ctx->async_mode) HIP_OK(hipStreamSynchronize(L.ctx->stream));
+  API_END
+}
+
+int cf3hip_vector_scale(cf3hip_lss lss, int which, double alpha)
+{
+  API_BEGIN
+  REQUIRE(lss, CF3HIP_ERR_NOT_CREATED, "system not created");
+  Lss& L = lss->l;
+  double* v = vec_ptr(L, which);
+  if(alpha != 1.) lss_axpby(*L.ctx, L.dofs, 0., v, alpha, v);
   if(!L.ctx->async_mode) HIP_OK(hipStreamSynchronize(L.ctx->stream));
   API_END
 }

@@ -168,6 +168,9 @@ void lss_spmv(Lss& L, double alpha, const double* x, double beta, double* y);
 void lss_spmv32(Lss& L, float alpha, const float* x, float beta, float* y);
 void lss_dirichlet(Lss& L, uint32_t nbc, const uint32_t* nodes, const uint32_t* eqs, const double* values);
 void lss_point_csr(Lss& L, uint32_t* row_ptr_dev, uint32_t* col_dev, double* val_dev);
+void lss_diagonal(Lss& L, const double* d_dev, int mode);            // mode 0 set, 1 add
+void lss_column_zero(Lss& L, uint32_t col, int ieq, double* out_dev);
+void lss_axpby(Context& c, size_t n, double alpha, const double* x, double beta, double* y);
 
 // krylov.hip
 void lss_solve(Lss& L, const cf3hip_solver_params& prm, cf3hip_solve_stats& st);

@@ -456,3 +456,81 @@ void lss_point_csr(Lss& L, uint32_t* prow, uint32_t* pcol, double* pval)
 }
 
 }  // namespace cf3hip
+
+// ---------------------------------------------------------------------------------------------
+// Matrix / Vector interface helpers (cf3/math/LSS/Matrix.hpp:98-120, Vector.hpp:146-154)
+// ---------------------------------------------------------------------------------------------
+namespace cf3hip {
+
+namespace {
+
+// diagonal of every node block: mode 0 set, 1 add (Matrix::set_diagonal / add_diagonal)
+__global__ void k_diagonal(uint32_t N, int neq, const uint32_t* __restrict__ diag_blk, double* __restrict__ values,
+                           const double* __restrict__ d, int mode)
+{
+  const uint64_t t = (uint64_t)blockIdx.x * blockDim.x + threadIdx.x;
+  if(t >= (uint64_t)N * neq) return;
+  const uint32_t i = (uint32_t)(t / neq);
+  const int a = (int)(t % neq);
+  double* v = values + ((uint64_t)diag_blk[i] * neq + a) * neq + a;
+  *v = mode == 0 ? d[t] : *v + d[t];
+}
+
+// column (iblockcol, ieq): out[k*neq + j] = A(k*neq + j, iblockcol*neq + ieq), then zeroed
+// (TrilinosFEVbrMatrix::get_column_and_replace_to_zero, TrilinosFEVbrMatrix.cpp:452-478)
+__global__ void k_column_zero(uint32_t N, int neq, const uint32_t* __restrict__ row_ptr, const uint32_t* __restrict__ cols,
+                              double* __restrict__ values, uint32_t col, int ieq, double* __restrict__ out)
+{
+  const uint32_t k = blockIdx.x * blockDim.x + threadIdx.x;
+  if(k >= N) return;
+  uint32_t lo = row_ptr[k], hi = row_ptr[k + 1];
+  while(lo < hi) {
+    const uint32_t mid = (lo + hi) >> 1;
+    if(cols[mid] < col) lo = mid + 1; else hi = mid;
+  }
+  const bool found = lo < row_ptr[k + 1] && cols[lo] == col;
+  for(int j = 0; j < neq; ++j) {
+    double v = 0.;
+    if(found) {
+      double* p = values + ((uint64_t)lo * neq + j) * neq + ieq;
+      v = *p;
+      *p = 0.;
+    }
+    out[(uint64_t)k * neq + j] = v;
+  }
+}
+
+// y = alpha x + beta y (x may alias y)
+__global__ void k_axpby(uint64_t n, double alpha, const double* x, double beta, double* y)
+{
+  const uint64_t stride = (uint64_t)gridDim.x * blockDim.x;
+  for(uint64_t j = (uint64_t)blockIdx.x * blockDim.x + threadIdx.x; j < n; j += stride) y[j] = alpha * x[j] + beta * y[j];
+}
+
+}  // namespace
+
+void lss_diagonal(Lss& L, const double* d_dev, int mode)
+{
+  Context& c = *L.ctx;
+  hipLaunchKernelGGL(k_diagonal, dim3(grid_for(L.dofs, 256)), dim3(256), 0, c.stream, L.n_nodes, L.neq,
+                     L.diag_blk.as<uint32_t>(), L.values.as<double>(), d_dev, mode);
+  HIP_OK(hipGetLastError());
+}
+
+void lss_column_zero(Lss& L, uint32_t col, int ieq, double* out_dev)
+{
+  Context& c = *L.ctx;
+  hipLaunchKernelGGL(k_column_zero, dim3(grid_for(L.n_nodes, 256)), dim3(256), 0, c.stream, L.n_nodes, L.neq,
+                     L.row_ptr.as<uint32_t>(), L.cols.as<uint32_t>(), L.values.as<double>(), col, ieq, out_dev);
+  HIP_OK(hipGetLastError());
+}
+
+void lss_axpby(Context& c, size_t n, double alpha, const double* x, double beta, double* y)
+{
+  if(!n) return;
+  hipLaunchKernelGGL(k_axpby, dim3(std::min<unsigned>(grid_for(n, 256), 4096)), dim3(256), 0, c.stream, (uint64_t)n,
+                     alpha, x, beta, y);
+  HIP_OK(hipGetLastError());
+}
+
+}  // namespace cf3hip

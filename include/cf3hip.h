@@ -115,6 +115,15 @@ int cf3hip_matrix_set_row(cf3hip_lss lss, uint32_t blockrow, uint32_t ieq, doubl
 /* Matrix::add_values(BlockAccumulator): indices (host, nb node ids), mat (host, row-major
  * (nb*neq)^2, node-major rows) -- BlockAccumulator.hpp:28-74, TrilinosFEVbrMatrix.cpp:315-355 */
 int cf3hip_matrix_add_values(cf3hip_lss lss, uint32_t nb, const uint32_t* indices, const double* mat);
+/* Matrix::set_values / get_values(BlockAccumulator) (Matrix.hpp:87, 95), same layout as add_values */
+int cf3hip_matrix_set_values(cf3hip_lss lss, uint32_t nb, const uint32_t* indices, const double* mat);
+int cf3hip_matrix_get_block_values(cf3hip_lss lss, uint32_t nb, const uint32_t* indices, double* mat);
+/* Matrix::set_diagonal / add_diagonal (Matrix.hpp:114-117): host, dofs */
+int cf3hip_matrix_set_diagonal(cf3hip_lss lss, const double* diag);
+int cf3hip_matrix_add_diagonal(cf3hip_lss lss, const double* diag);
+/* Matrix::get_column_and_replace_to_zero (Matrix.hpp:103, TrilinosFEVbrMatrix.cpp:452-478):
+ * values (host, dofs) receive column (iblockcol, ieq), which is then zeroed */
+int cf3hip_matrix_get_column_and_replace_to_zero(cf3hip_lss lss, uint32_t iblockcol, uint32_t ieq, double* values);
 int cf3hip_matrix_get_diagonal(cf3hip_lss lss, double* diag);                  /* host, dofs */
 /* y = alpha*A*x + beta*y (Matrix::apply, TrilinosDetail.cpp:380-411); device pointers, dofs each */
 int cf3hip_matrix_apply(cf3hip_lss lss, double alpha, const double* x_dev, double beta, double* y_dev);
@@ -137,6 +146,16 @@ int cf3hip_vector_get_value(cf3hip_lss lss, int which, uint32_t blockrow, uint32
 int cf3hip_vector_device_ptr(cf3hip_lss lss, int which, double** dev);
 /* Vector::assign (Vector.hpp): copy one system vector into the other */
 int cf3hip_vector_assign(cf3hip_lss lss, int to, int from);
+/* Vector::add_value(iblockrow, ieq) (Vector.hpp:79) */
+int cf3hip_vector_add_value(cf3hip_lss lss, int which, uint32_t blockrow, uint32_t ieq, double v);
+/* Vector::set/add/get_rhs_values and _sol_values(BlockAccumulator) (Vector.hpp:90-105): indices
+ * (host, nb node ids), vals (host, nb*neq, node-major) */
+int cf3hip_vector_set_values(cf3hip_lss lss, int which, uint32_t nb, const uint32_t* indices, const double* vals);
+int cf3hip_vector_add_values(cf3hip_lss lss, int which, uint32_t nb, const uint32_t* indices, const double* vals);
+int cf3hip_vector_get_values(cf3hip_lss lss, int which, uint32_t nb, const uint32_t* indices, double* vals);
+/* Vector::update (this += alpha * source, Vector.hpp:150) and Vector::scale (Vector.hpp:154) */
+int cf3hip_vector_update(cf3hip_lss lss, int to, int from, double alpha);
+int cf3hip_vector_scale(cf3hip_lss lss, int which, double alpha);
 
 /* ---- System::dirichlet with preserve_symmetry (System.cpp:240-255 ->
  *      TrilinosFEVbrMatrix::symmetric_dirichlet, TrilinosFEVbrMatrix.cpp:482-553), applied to a list

@@ -1,0 +1,285 @@
+// C++ checks of the host layer (coolfluid3_amd/host/cf3hip_lss.hpp), written like the reference's
+// LSS unit tests: test/math/utest-lss-symmetric-dirichlet.cpp (exact values), utest-lss-atomic.cpp
+// (set/add/get, BlockAccumulator, set_row, diagonal, column replace, vector operations, solve) and
+// utest-lss-vector.cpp (apply), plus the UFEM assembly through the C++ action compared with the
+// oracle.  Usage: test_lss_api --no-device (exception mapping only) | test_lss_api (needs a GPU).
+#include <cmath>
+#include <cstdio>
+#include <cstdlib>
+#include <cstring>
+#include <vector>
+
+#include "cf3hip_lss.hpp"
+#include "oracle.h"
+
+using namespace cf3hip;
+
+static int g_fail = 0;
+#define CHECK(cond)                                                                                \
+  do {                                                                                             \
+    if(!(cond)) {                                                                                  \
+      std::printf("CHECK failed %s:%d: %s\n", __FILE__, __LINE__, #cond);                         \
+      ++g_fail;                                                                                    \
+    }                                                                                              \
+  } while(0)
+
+template<typename Ex, typename F>
+static bool throws(F&& f)
+{
+  try {
+    f();
+  } catch(const Ex&) {
+    return true;
+  } catch(...) {
+    return false;
+  }
+  return false;
+}
+
+// 4-node chain 0-1-2-3, one equation per node (utest-lss-symmetric-dirichlet.cpp:146-210)
+static void symmetric_dirichlet(Context& ctx)
+{
+  const std::vector<uint32_t> start = {0, 2, 5, 8, 10};
+  const std::vector<uint32_t> conn = {0, 1, 0, 1, 2, 1, 2, 3, 2, 3};
+  lss::System s(ctx, 1, conn, start);
+  for(uint32_t i = 0; i < 4; ++i) s.matrix().set_row(i, 0, 2., 1.);
+  s.dirichlet(1, 0, 10.);
+  const double expect[4][4] = {{2, 0, 0, 0}, {0, 1, 0, 0}, {0, 0, 2, 1}, {0, 0, 1, 2}};
+  for(uint32_t r = 0; r < 4; ++r)
+    for(uint32_t c = 0; c < 4; ++c) {
+      if((r > c ? r - c : c - r) > 1) continue;
+      double v = -1.;
+      s.matrix().get_value(c, r, v);
+      CHECK(v == expect[r][c]);
+    }
+  std::vector<double> rhs;
+  s.rhs().get(rhs);
+  CHECK(rhs[0] == -10. && rhs[1] == 10. && rhs[2] == -10. && rhs[3] == 0.);
+  double sv = 0.;
+  s.solution().get_value(1, 0, sv);
+  CHECK(sv == 10.);
+  // an entry outside the graph is illegal (TrilinosFEVbrMatrix.cpp:175)
+  CHECK(throws<BadValue>([&] { double v; s.matrix().get_value(3, 0, v); }));
+  CHECK(throws<NotImplemented>([&] { s.matrix().tie_blockrow_pairs(0, 3); }));
+}
+
+// 1D chain of n nodes with neq = 2 (utest-lss-atomic.cpp style)
+static void atomic_operations(Context& ctx)
+{
+  const uint32_t n = 6, neq = 2;
+  std::vector<uint32_t> start(1, 0), conn;
+  for(uint32_t i = 0; i < n; ++i) {
+    for(int d = -1; d <= 1; ++d)
+      if((int)i + d >= 0 && (int)i + d < (int)n) conn.push_back(i + d);
+    start.push_back((uint32_t)conn.size());
+  }
+  lss::System s(ctx, neq, conn, start);
+  lss::Matrix& A = s.matrix();
+  CHECK(A.is_created() && A.neq() == neq && A.blockrow_size() == n && A.blockcol_size() == n);
+  A.reset(0.);
+  // scalar access: column first
+  A.set_value(3, 2, 1.5);
+  A.add_value(3, 2, 0.25);
+  double v = 0.;
+  A.get_value(3, 2, v);
+  CHECK(v == 1.75);
+  // BlockAccumulator on nodes (1, 2)
+  lss::BlockAccumulator ba;
+  ba.resize(2, neq);
+  ba.indices[0] = 1;
+  ba.indices[1] = 2;
+  for(size_t i = 0; i < ba.mat.size(); ++i) ba.mat[i] = (double)i;
+  A.set_values(ba);
+  A.add_values(ba);
+  lss::BlockAccumulator back;
+  back.resize(2, neq);
+  back.indices = ba.indices;
+  A.get_values(back);
+  for(size_t i = 0; i < ba.mat.size(); ++i) CHECK(back.mat[i] == 2. * ba.mat[i]);
+  // set_row: row (node 4, eq 1) = offdiag everywhere in the graph, diag on the diagonal
+  A.set_row(4, 1, 7., -1.);
+  A.get_value(9, 9, v);
+  CHECK(v == 7.);
+  A.get_value(8, 9, v);
+  CHECK(v == -1.);
+  A.get_value(6, 9, v);
+  CHECK(v == -1.);
+  // diagonal
+  std::vector<double> d(n * neq, 1.), dg;
+  A.set_diagonal(d);
+  A.add_diagonal(d);
+  A.get_diagonal(dg);
+  for(double x : dg) CHECK(x == 2.);
+  // column (node 2, eq 0) out and zeroed
+  A.set_value(4, 3, 3.);   // row 3 (node 1, eq 1), column 4 (node 2, eq 0)
+  std::vector<double> col;
+  A.get_column_and_replace_to_zero(2, 0, col);
+  CHECK(col.size() == n * neq);
+  CHECK(col[3] == 3.);
+  A.get_value(4, 3, v);
+  CHECK(v == 0.);
+  CHECK(col[0] == 0. && col[10] == 0.);   // nodes 0 and 5 are not neighbours of node 2
+  // vectors
+  lss::Vector& b = s.rhs();
+  lss::Vector& x = s.solution();
+  b.reset(1.);
+  b.set_value(5, 4.);                     // dof 5 = (node 2, eq 1)
+  b.add_value(2, 1, 1.);
+  b.get_value(2, 1, v);
+  CHECK(v == 5.);
+  lss::BlockAccumulator bv;
+  bv.resize(1, neq);
+  bv.indices[0] = 3;
+  bv.rhs[0] = 2.;
+  bv.rhs[1] = 3.;
+  b.add_rhs_values(bv);
+  b.get_rhs_values(bv);
+  CHECK(bv.rhs[0] == 3. && bv.rhs[1] == 4.);
+  x.assign(b);
+  x.update(b, -0.5);
+  x.scale(4.);
+  std::vector<double> xv, bvv;
+  x.get(xv);
+  b.get(bvv);
+  for(size_t i = 0; i < xv.size(); ++i) CHECK(xv[i] == 2. * bvv[i]);
+  // apply: y = 2 A x + 1 y against a host product over the point-CSR debug data
+  std::vector<uint32_t> rows, cols;
+  std::vector<double> vals;
+  A.debug_data(rows, cols, vals);
+  std::vector<double> y0 = bvv;
+  A.apply(b, x, 2., 1.);
+  std::vector<double> y;
+  b.get(y);
+  for(uint32_t r = 0; r < n * neq; ++r) {
+    double acc = 0.;
+    for(uint32_t q = rows[r]; q < rows[r + 1]; ++q) acc += vals[q] * xv[cols[q]];
+    CHECK(std::fabs(y[r] - (2. * acc + y0[r])) <= 1e-13 * (std::fabs(y[r]) + 1.));
+  }
+}
+
+// symmetric positive definite chain: -x_{i-1} + 4 x_i - x_{i+1} = f, solved to 1e-10
+static void solve_chain(Context& ctx)
+{
+  const uint32_t n = 200;
+  std::vector<uint32_t> start(1, 0), conn;
+  for(uint32_t i = 0; i < n; ++i) {
+    for(int d = -1; d <= 1; ++d)
+      if((int)i + d >= 0 && (int)i + d < (int)n) conn.push_back(i + d);
+    start.push_back((uint32_t)conn.size());
+  }
+  lss::System s(ctx, 1, conn, start);
+  s.reset(0.);
+  std::vector<double> ref(n), f(n, 0.);
+  for(uint32_t i = 0; i < n; ++i) ref[i] = std::sin(0.05 * i) + 0.1 * i;
+  for(uint32_t i = 0; i < n; ++i) {
+    s.matrix().set_value(i, i, 4.);
+    f[i] += 4. * ref[i];
+    if(i > 0) { s.matrix().set_value(i - 1, i, -1.); f[i] -= ref[i - 1]; }
+    if(i + 1 < n) { s.matrix().set_value(i + 1, i, -1.); f[i] -= ref[i + 1]; }
+  }
+  s.rhs().set(f);
+  s.solution().reset(0.);
+  s.solution_strategy().params.tol = 1e-12;
+  CHECK(s.solve());
+  CHECK(s.solution_strategy().compute_residual() <= 1e-12);
+  std::vector<double> xs;
+  s.solution().get(xs);
+  double err = 0.;
+  for(uint32_t i = 0; i < n; ++i) err = std::fmax(err, std::fabs(xs[i] - ref[i]));
+  CHECK(err <= 1e-9);
+}
+
+// UFEM assembly through the C++ action on a 3^3 Kuhn cube, against the oracle's FP64 assembly
+static void ns_assembly(Context& ctx)
+{
+  const uint32_t c = 3, N = (c + 1) * (c + 1) * (c + 1), E = 6 * c * c * c;
+  std::vector<double> X((size_t)N * 3);
+  std::vector<uint32_t> conn((size_t)E * 4);
+  orc_mesh_box_tetras(c, c, c, 1., 1., 1., X.data(), conn.data());
+  uint64_t st = 12345;
+  std::vector<double> u((size_t)N * 3), p(N, 0.), nu(N, 1e-2);
+  for(double& x : u) x = orc_uniform(&st, -1., 1.);
+  Mesh mesh(ctx, CF3HIP_TETRA3D, X, conn);
+  ufem::Fields f(mesh);
+  f.set(CF3HIP_FIELD_U, u);
+  f.set(CF3HIP_FIELD_P, p);
+  f.initialize(1e-2);
+  lss::System s(ctx, mesh, 4);
+  ufem::NavierStokesAssembly ns;
+  ns.density = 1.;
+  ns.dt = 0.01;
+  ns.linearize(f);
+  ns.execute(s, f);
+  // oracle
+  std::vector<uint32_t> start(N + 1), cols(64 * N), slots((size_t)E * 16);
+  const int64_t nb = orc_build_sparsity(4, N, E, conn.data(), start.data(), cols.data(), (int64_t)cols.size(), slots.data());
+  CHECK(nb > 0);
+  orc_ns_params prm = {1., 0.01, 1., 1, ORC_SUPG_METRIC, 1., 16., 1., 1., 1.};
+  std::vector<double> vref((size_t)nb * 16), rref((size_t)N * 4);
+  CHECK(orc_ns_assemble(ORC_TETRA3D, &prm, N, E, X.data(), conn.data(), u.data(), p.data(), u.data(), nu.data(),
+                        nullptr, start.data(), cols.data(), slots.data(), vref.data(), rref.data(), 1) == 0);
+  std::vector<uint32_t> rows, ci;
+  std::vector<double> vals;
+  s.matrix().debug_data(rows, ci, vals);
+  // compare through scalar access on the diagonal blocks and the full rhs
+  double vmax = 0., dmax = 0.;
+  for(uint32_t i = 0; i < N; ++i)
+    for(uint32_t q = start[i]; q < start[i + 1]; ++q)
+      for(int a = 0; a < 4; ++a)
+        for(int b2 = 0; b2 < 4; ++b2) {
+          double g = 0.;
+          s.matrix().get_value(cols[q] * 4 + b2, i * 4 + a, g);
+          const double r = vref[(size_t)q * 16 + a * 4 + b2];
+          vmax = std::fmax(vmax, std::fabs(r));
+          dmax = std::fmax(dmax, std::fabs(g - r));
+        }
+  CHECK(dmax <= 1e-12 * vmax);
+  std::vector<double> rhs;
+  s.rhs().get(rhs);
+  double rmax = 0., rd = 0.;
+  for(size_t i = 0; i < rhs.size(); ++i) {
+    rmax = std::fmax(rmax, std::fabs(rref[i]));
+    rd = std::fmax(rd, std::fabs(rhs[i] - rref[i]));
+  }
+  CHECK(rd <= 1e-12 * rmax);
+  // option errors as in the reference
+  ns.theta = 1.5;
+  CHECK(throws<SetupError>([&] { ns.execute(s, f); }));
+  ns.theta = 1.;
+  ns.supg_type = "unknown";
+  CHECK(throws<BadValue>([&] { ns.execute(s, f); }));
+}
+
+int main(int argc, char** argv)
+{
+  if(argc > 1 && std::strcmp(argv[1], "--no-device") == 0) {
+    // without a GPU: creating a context fails with a mapped exception, never a crash
+    bool mapped = false;
+    try {
+      Context ctx(0);
+    } catch(const Error& e) {
+      mapped = e.code != 0;
+    }
+    CHECK(mapped);
+    CHECK(throws<SetupError>([] { check(CF3HIP_ERR_NOT_CREATED); }));
+    CHECK(throws<BadValue>([] { check(CF3HIP_ERR_ILLEGAL_ENTRY); }));
+    CHECK(throws<NotImplemented>([] { check(CF3HIP_ERR_NOT_IMPLEMENTED); }));
+    CHECK(throws<FailedAssertion>([] { check(CF3HIP_ERR_HIP); }));
+  } else {
+    Context ctx(0);
+    std::printf("symmetric_dirichlet\n");
+    std::fflush(stdout);
+    symmetric_dirichlet(ctx);
+    std::printf("atomic_operations\n");
+    std::fflush(stdout);
+    atomic_operations(ctx);
+    std::printf("solve_chain\n");
+    std::fflush(stdout);
+    solve_chain(ctx);
+    std::printf("ns_assembly\n");
+    std::fflush(stdout);
+    ns_assembly(ctx);
+  }
+  std::printf("%s: %d failed checks\n", g_fail ? "FAIL" : "OK", g_fail);
+  return g_fail ? 1 : 0;
+}
