@@ -159,13 +159,14 @@ void lss_copy(Context& c, double* dst, const double* src, size_t n);
 void lss_fill8(Context& c, uint8_t* p, size_t n, uint8_t v);
 void lss_clear_conditions(Lss& L);
 // FP32 values, FP64 vectors and accumulation: y = A32 x
-void lss_spmv_mixed(Lss& L, const double* x, double* y);
+void lss_spmv_mixed(Lss& L, const double* x, double* y, const double* stop = nullptr);
 // matrix-free FP64 residual r = b' - A' x of the NS system (assembly.hip)
 void ns_residual(Lss& L, Fields& F, const cf3hip_ns_params& prm, const double* x, double* r);
 void ns_solve(Lss& L, Fields& F, const cf3hip_ns_params& nsp, const cf3hip_solver_params& prm,
               cf3hip_solve_stats& st);
-void lss_spmv(Lss& L, double alpha, const double* x, double beta, double* y);
-void lss_spmv32(Lss& L, float alpha, const float* x, float beta, float* y);
+// stop (optional, device): a Krylov loop's stop flag; the product is skipped once it is set
+void lss_spmv(Lss& L, double alpha, const double* x, double beta, double* y, const double* stop = nullptr);
+void lss_spmv32(Lss& L, float alpha, const float* x, float beta, float* y, const double* stop = nullptr);
 void lss_dirichlet(Lss& L, uint32_t nbc, const uint32_t* nodes, const uint32_t* eqs, const double* values);
 void lss_point_csr(Lss& L, uint32_t* row_ptr_dev, uint32_t* col_dev, double* val_dev);
 void lss_diagonal(Lss& L, const double* d_dev, int mode);            // mode 0 set, 1 add

@@ -7,8 +7,10 @@
 // ||r|| <= tol ||b||, restated identically in the oracle.
 //
 // All scalars live in device memory; reductions are two-pass with a fixed grid, so every dot product
-// has a fixed summation order (deterministic results, no floating-point atomics).  The host reads one
-// small block of scalars per iteration for the convergence test.
+// has a fixed summation order (deterministic results, no floating-point atomics).  The convergence
+// and breakdown tests run on the device: they raise a stop flag that turns every later kernel of the
+// loop into a no-op, so the host enqueues batches of iterations and reads the scalars once per batch
+// (the iterates are the same as with a test after every iteration).
 #include <algorithm>
 #include <cmath>
 
@@ -42,10 +44,13 @@ __device__ inline void block_reduce_store(double (&v)[NDOT], double* partials, u
 
 // S layout (double scalars)
 enum { S_RHO = 0, S_RHO_PREV, S_ALPHA, S_OMEGA, S_RHV, S_SN2, S_TS, S_TT, S_RN2, S_BN2, S_SCONV, S_THR2,
-       S_TMP0, S_TMP1, S_COUNT };
+       S_TMP0, S_TMP1, S_STOP, S_ITER, S_MAXR, S_DELTA, S_COUNT };
 
-// finalize: sum partials of NDOT dots into S[dst + d], then apply a mode-specific scalar update
-enum { FIN_STORE = 0, FIN_ALPHA, FIN_SNORM, FIN_OMEGA, FIN_RNORM };
+// finalize: sum partials of NDOT dots into S[dst + d], then apply a mode-specific scalar update.
+// FIN_RNORM ends an iteration: it counts it and raises S_STOP on convergence or breakdown;
+// FIN_RNORM_REL also raises it when the residual has dropped by S_DELTA since the last reliable
+// update (running maximum S_MAXR).
+enum { FIN_STORE = 0, FIN_ALPHA, FIN_SNORM, FIN_OMEGA, FIN_RNORM, FIN_RNORM_REL };
 
 template<int NDOT>
 __global__ void k_finalize(const double* __restrict__ partials, unsigned G, double* __restrict__ S, int dst, int mode)
@@ -60,6 +65,7 @@ __global__ void k_finalize(const double* __restrict__ partials, unsigned G, doub
   }
   __syncthreads();
   if(threadIdx.x != 0) return;
+  if(mode != FIN_STORE && S[S_STOP] != 0.) return;
   for(int d = 0; d < NDOT; ++d) {
     double s = 0.;
     for(int w = 0; w < kRT / 64; ++w) s += sh[d][w];
@@ -77,17 +83,28 @@ __global__ void k_finalize(const double* __restrict__ partials, unsigned G, doub
     S[S_TS] = v[0];
     S[S_TT] = v[NDOT > 1 ? 1 : 0];
     S[S_OMEGA] = S[S_SCONV] != 0. ? 0. : v[0] / v[NDOT > 1 ? 1 : 0];
-  } else if(mode == FIN_RNORM) {          // v0 = r . r, v1 = rh . r
-    S[S_RN2] = v[0];
+  } else if(mode == FIN_RNORM || mode == FIN_RNORM_REL) {   // v0 = r . r, v1 = rh . r
+    const double rn2 = v[0], rho = v[NDOT > 1 ? 1 : 0];
+    S[S_RN2] = rn2;
     S[S_RHO_PREV] = S[S_RHO];
-    S[S_RHO] = v[NDOT > 1 ? 1 : 0];
+    S[S_RHO] = rho;
+    S[S_ITER] += 1.;
+    const bool conv = S[S_SCONV] != 0. || rn2 <= S[S_THR2];
+    if(mode == FIN_RNORM) {
+      if(conv || !isfinite(rn2) || rho == 0. || S[S_OMEGA] == 0.) S[S_STOP] = 1.;
+    } else {
+      const double rn = sqrt(rn2);
+      if(conv || !isfinite(rn2) || rho == 0. || rn < S[S_DELTA] * S[S_MAXR]) S[S_STOP] = 1.;
+      else S[S_MAXR] = fmax(S[S_MAXR], rn);
+    }
   }
 }
 
 template<typename R>
 __global__ void k_dot2(uint64_t n, const R* __restrict__ a0, const R* __restrict__ b0, const R* __restrict__ a1,
-                       const R* __restrict__ b1, double* partials)
+                       const R* __restrict__ b1, double* partials, const double* __restrict__ stop)
 {
+  if(stop && stop[0] != 0.) return;
   double v[2] = {0., 0.};
   const uint64_t stride = (uint64_t)gridDim.x * kRT;
   for(uint64_t i = (uint64_t)blockIdx.x * kRT + threadIdx.x; i < n; i += stride) {
@@ -128,6 +145,7 @@ template<typename R>
 __global__ void k_bicg_p(uint64_t n, const double* __restrict__ S, const R* __restrict__ r, R* __restrict__ p,
                          const R* __restrict__ v, R* __restrict__ ph, const R* __restrict__ dinv)
 {
+  if(S[S_STOP] != 0.) return;
   const double beta_d = (S[S_RHO] / S[S_RHO_PREV]) * (S[S_ALPHA] / S[S_OMEGA]);
   const R beta = (R)beta_d, omega = (R)S[S_OMEGA];
   const uint64_t stride = (uint64_t)gridDim.x * blockDim.x;
@@ -142,6 +160,7 @@ template<typename R>
 __global__ void k_bicg_s(uint64_t n, const double* __restrict__ S, const R* __restrict__ r, const R* __restrict__ v,
                          R* __restrict__ s, R* __restrict__ sh, const R* __restrict__ dinv, double* partials)
 {
+  if(S[S_STOP] != 0.) return;
   const R alpha = (R)S[S_ALPHA];
   double acc[1] = {0.};
   const uint64_t stride = (uint64_t)gridDim.x * kRT;
@@ -159,6 +178,7 @@ __global__ void k_bicg_x(uint64_t n, const double* __restrict__ S, R* __restrict
                          const R* __restrict__ sh, R* __restrict__ r, const R* __restrict__ s, const R* __restrict__ t,
                          const R* __restrict__ rh, double* partials)
 {
+  if(S[S_STOP] != 0.) return;
   const R alpha = (R)S[S_ALPHA], omega = (R)S[S_OMEGA];
   double acc[2] = {0., 0.};
   const uint64_t stride = (uint64_t)gridDim.x * kRT;
@@ -175,7 +195,10 @@ __global__ void k_bicg_x(uint64_t n, const double* __restrict__ S, R* __restrict
 __global__ void k_set_scalars(double* S, double thr2)
 {
   S[S_RHO_PREV] = 1.; S[S_ALPHA] = 1.; S[S_OMEGA] = 1.; S[S_THR2] = thr2; S[S_SCONV] = 0.;
+  S[S_STOP] = 0.; S[S_ITER] = 0.;
 }
+
+constexpr int kMaxBatch = 16;   // iterations enqueued between two host reads of the scalars
 
 template<typename Src, typename Dst>
 __global__ void k_convert_scaled(uint64_t n, const Src* __restrict__ a, Dst* __restrict__ b, double scale)
@@ -191,11 +214,17 @@ __global__ void k_axpy_f32(uint64_t n, double a, const float* __restrict__ d, do
 }
 
 template<typename R>
-void spmv_t(Lss& L, R alpha, const R* x, R beta, R* y);
+void spmv_t(Lss& L, R alpha, const R* x, R beta, R* y, const double* stop);
 template<>
-void spmv_t<double>(Lss& L, double alpha, const double* x, double beta, double* y) { lss_spmv(L, alpha, x, beta, y); }
+void spmv_t<double>(Lss& L, double alpha, const double* x, double beta, double* y, const double* stop)
+{
+  lss_spmv(L, alpha, x, beta, y, stop);
+}
 template<>
-void spmv_t<float>(Lss& L, float alpha, const float* x, float beta, float* y) { lss_spmv32(L, alpha, x, beta, y); }
+void spmv_t<float>(Lss& L, float alpha, const float* x, float beta, float* y, const double* stop)
+{
+  lss_spmv32(L, alpha, x, beta, y, stop);
+}
 
 struct Scalars {
   double v[S_COUNT];
@@ -230,7 +259,7 @@ int bicgstab(Lss& L, const R* b, R* x, R* work, double tol, int maxit, bool& con
   const R* vals = sizeof(R) == sizeof(double) ? (const R*)L.values.ptr : (const R*)L.values32.ptr;
   hipLaunchKernelGGL(k_jacobi<R>, dim3(grid_for(n, 256)), dim3(256), 0, c.stream, L.n_nodes, L.neq,
                      L.diag_blk.as<uint32_t>(), vals, dinv);
-  spmv_t<R>(L, R(1), x, R(0), r);
+  spmv_t<R>(L, R(1), x, R(0), r, nullptr);
   hipLaunchKernelGGL(k_bicg_init<R>, dim3(G), dim3(kRT), 0, c.stream, n, b, r, rh, p, v, part);
   hipLaunchKernelGGL(k_finalize<2>, dim3(1), dim3(kRT), 0, c.stream, part, G, S, (int)S_RN2, (int)FIN_STORE);
   spmvs = 1;
@@ -244,28 +273,33 @@ int bicgstab(Lss& L, const R* b, R* x, R* work, double tol, int maxit, bool& con
   hipLaunchKernelGGL(k_finalize<1>, dim3(1), dim3(kRT), 0, c.stream, part, G, S, (int)S_RHO, (int)FIN_STORE);
   HIP_OK(hipGetLastError());
   conv = sc.v[S_RN2] <= thr2;
-  int it = 0;
+  int it = 0, batch = 1;
   while(!conv && it < maxit) {
-    ++it;
-    hipLaunchKernelGGL(k_bicg_p<R>, dim3(G), dim3(kRT), 0, c.stream, n, S, r, p, v, ph, dinv);
-    spmv_t<R>(L, R(1), ph, R(0), v);
-    hipLaunchKernelGGL(k_dot2<R>, dim3(G), dim3(kRT), 0, c.stream, n, rh, v, (const R*)nullptr, (const R*)nullptr, part);
-    hipLaunchKernelGGL(k_finalize<1>, dim3(1), dim3(kRT), 0, c.stream, part, G, S, 0, (int)FIN_ALPHA);
-    hipLaunchKernelGGL(k_bicg_s<R>, dim3(G), dim3(kRT), 0, c.stream, n, S, r, v, s, sh, dinv, part);
-    hipLaunchKernelGGL(k_finalize<1>, dim3(1), dim3(kRT), 0, c.stream, part, G, S, 0, (int)FIN_SNORM);
-    spmv_t<R>(L, R(1), sh, R(0), t);
-    hipLaunchKernelGGL(k_dot2<R>, dim3(G), dim3(kRT), 0, c.stream, n, t, s, t, t, part);
-    hipLaunchKernelGGL(k_finalize<2>, dim3(1), dim3(kRT), 0, c.stream, part, G, S, 0, (int)FIN_OMEGA);
-    hipLaunchKernelGGL(k_bicg_x<R>, dim3(G), dim3(kRT), 0, c.stream, n, S, x, ph, sh, r, s, t, rh, part);
-    hipLaunchKernelGGL(k_finalize<2>, dim3(1), dim3(kRT), 0, c.stream, part, G, S, 0, (int)FIN_RNORM);
-    HIP_OK(hipGetLastError());
-    spmvs += 2;
-    read_scalars(c, sc);
-    if(sc.v[S_SCONV] != 0. || sc.v[S_RN2] <= thr2) conv = true;
-    if(!std::isfinite(sc.v[S_RN2]) || sc.v[S_RHO] == 0. || sc.v[S_OMEGA] == 0.) {
-      if(!conv) break;
+    const int nb = std::min(batch, maxit - it);
+    for(int k = 0; k < nb; ++k) {
+      hipLaunchKernelGGL(k_bicg_p<R>, dim3(G), dim3(kRT), 0, c.stream, n, S, r, p, v, ph, dinv);
+      spmv_t<R>(L, R(1), ph, R(0), v, S + S_STOP);
+      hipLaunchKernelGGL(k_dot2<R>, dim3(G), dim3(kRT), 0, c.stream, n, rh, v, (const R*)nullptr, (const R*)nullptr, part,
+                         S + S_STOP);
+      hipLaunchKernelGGL(k_finalize<1>, dim3(1), dim3(kRT), 0, c.stream, part, G, S, 0, (int)FIN_ALPHA);
+      hipLaunchKernelGGL(k_bicg_s<R>, dim3(G), dim3(kRT), 0, c.stream, n, S, r, v, s, sh, dinv, part);
+      hipLaunchKernelGGL(k_finalize<1>, dim3(1), dim3(kRT), 0, c.stream, part, G, S, 0, (int)FIN_SNORM);
+      spmv_t<R>(L, R(1), sh, R(0), t, S + S_STOP);
+      hipLaunchKernelGGL(k_dot2<R>, dim3(G), dim3(kRT), 0, c.stream, n, t, s, t, t, part, S + S_STOP);
+      hipLaunchKernelGGL(k_finalize<2>, dim3(1), dim3(kRT), 0, c.stream, part, G, S, 0, (int)FIN_OMEGA);
+      hipLaunchKernelGGL(k_bicg_x<R>, dim3(G), dim3(kRT), 0, c.stream, n, S, x, ph, sh, r, s, t, rh, part);
+      hipLaunchKernelGGL(k_finalize<2>, dim3(1), dim3(kRT), 0, c.stream, part, G, S, 0, (int)FIN_RNORM);
     }
+    HIP_OK(hipGetLastError());
+    read_scalars(c, sc);
+    it = (int)sc.v[S_ITER];
+    if(sc.v[S_STOP] != 0.) {   // converged, or broke down
+      conv = sc.v[S_SCONV] != 0. || sc.v[S_RN2] <= thr2;
+      break;
+    }
+    batch = std::min(2 * batch, kMaxBatch);
   }
+  spmvs = 1 + 2 * it;
   return it;
 }
 
@@ -274,7 +308,7 @@ double norm2(Context& c, const double* a, uint64_t n)
   double* part = c.red_partials.as<double>();
   const unsigned G = red_grid(n);
   hipLaunchKernelGGL(k_dot2<double>, dim3(G), dim3(kRT), 0, c.stream, n, a, a, (const double*)nullptr,
-                     (const double*)nullptr, part);
+                     (const double*)nullptr, part, (const double*)nullptr);
   hipLaunchKernelGGL(k_finalize<1>, dim3(1), dim3(kRT), 0, c.stream, part, G, c.scalars.as<double>(), (int)S_TMP0,
                      (int)FIN_STORE);
   HIP_OK(hipGetLastError());
@@ -410,56 +444,63 @@ void reliable_bicgstab(Lss& L, Resid resid, const cf3hip_solver_params& prm, cf3
   lss_fill(c, v, n, 0.);
   hipLaunchKernelGGL(k_set_scalars, dim3(1), dim3(1), 0, c.stream, S, thr2);
   hipLaunchKernelGGL(k_dot2<double>, dim3(G), dim3(kRT), 0, c.stream, n, rh, r, (const double*)nullptr,
-                     (const double*)nullptr, part);
+                     (const double*)nullptr, part, (const double*)nullptr);
   hipLaunchKernelGGL(k_finalize<1>, dim3(1), dim3(kRT), 0, c.stream, part, G, S, (int)S_RHO, (int)FIN_STORE);
   HIP_OK(hipGetLastError());
   bool conv = rtrue * rtrue <= thr2;
-  double maxr = rtrue;
-  int it = 0, spmvs = 0;
+  hipLaunchKernelGGL(k_set_scalar, dim3(1), dim3(1), 0, c.stream, S, (int)S_MAXR, rtrue);
+  hipLaunchKernelGGL(k_set_scalar, dim3(1), dim3(1), 0, c.stream, S, (int)S_DELTA, delta);
+  int it = 0, it_base = 0, batch = 1;
   Scalars sc;
   while(!conv && it < prm.max_iter) {
-    ++it;
-    hipLaunchKernelGGL(k_bicg_p<double>, dim3(G), dim3(kRT), 0, c.stream, n, S, r, p, v, ph, dinv);
-    lss_spmv_mixed(L, ph, v);
-    hipLaunchKernelGGL(k_dot2<double>, dim3(G), dim3(kRT), 0, c.stream, n, rh, v, (const double*)nullptr,
-                       (const double*)nullptr, part);
-    hipLaunchKernelGGL(k_finalize<1>, dim3(1), dim3(kRT), 0, c.stream, part, G, S, 0, (int)FIN_ALPHA);
-    hipLaunchKernelGGL(k_bicg_s<double>, dim3(G), dim3(kRT), 0, c.stream, n, S, r, v, s, sh, dinv, part);
-    hipLaunchKernelGGL(k_finalize<1>, dim3(1), dim3(kRT), 0, c.stream, part, G, S, 0, (int)FIN_SNORM);
-    lss_spmv_mixed(L, sh, t);
-    hipLaunchKernelGGL(k_dot2<double>, dim3(G), dim3(kRT), 0, c.stream, n, t, s, t, t, part);
-    hipLaunchKernelGGL(k_finalize<2>, dim3(1), dim3(kRT), 0, c.stream, part, G, S, 0, (int)FIN_OMEGA);
-    hipLaunchKernelGGL(k_bicg_x<double>, dim3(G), dim3(kRT), 0, c.stream, n, S, xlo, ph, sh, r, s, t, rh, part);
-    hipLaunchKernelGGL(k_finalize<2>, dim3(1), dim3(kRT), 0, c.stream, part, G, S, 0, (int)FIN_RNORM);
-    HIP_OK(hipGetLastError());
-    spmvs += 2;
-    read_scalars(c, sc);
-    const double rn = std::sqrt(sc.v[S_RN2]);
-    const bool breakdown = !std::isfinite(sc.v[S_RN2]) || sc.v[S_RHO] == 0.;
-    if(rn < delta * maxr || sc.v[S_RN2] <= thr2 || sc.v[S_SCONV] != 0. || breakdown) {
-      // reliable update: move the increment into the FP64 solution, replace the residual
-      hipLaunchKernelGGL(k_flush_increment, dim3(std::min<unsigned>(grid_for(n, 256), 8192)), dim3(256), 0, c.stream,
-                         n, xhi, xlo);
-      resid(xhi, r);
-      ++updates;
-      rtrue = norm2(c, r, n);
-      if(rtrue * rtrue <= thr2) { conv = true; break; }
-      if(breakdown) {   // restart the recurrences from the true residual
-        lss_copy(c, rh, r, n);
-        lss_fill(c, p, n, 0.);
-        lss_fill(c, v, n, 0.);
-        hipLaunchKernelGGL(k_set_scalars, dim3(1), dim3(1), 0, c.stream, S, thr2);
-      }
-      hipLaunchKernelGGL(k_dot2<double>, dim3(G), dim3(kRT), 0, c.stream, n, rh, r, (const double*)nullptr,
-                         (const double*)nullptr, part);
-      hipLaunchKernelGGL(k_finalize<1>, dim3(1), dim3(kRT), 0, c.stream, part, G, S, (int)S_RHO, (int)FIN_STORE);
-      hipLaunchKernelGGL(k_set_scalar, dim3(1), dim3(1), 0, c.stream, S, (int)S_SCONV, 0.);
-      HIP_OK(hipGetLastError());
-      maxr = rtrue;
-    } else {
-      maxr = std::max(maxr, rn);
+    const int nb = std::min(batch, prm.max_iter - it);
+    for(int k = 0; k < nb; ++k) {
+      hipLaunchKernelGGL(k_bicg_p<double>, dim3(G), dim3(kRT), 0, c.stream, n, S, r, p, v, ph, dinv);
+      lss_spmv_mixed(L, ph, v, S + S_STOP);
+      hipLaunchKernelGGL(k_dot2<double>, dim3(G), dim3(kRT), 0, c.stream, n, rh, v, (const double*)nullptr,
+                         (const double*)nullptr, part, S + S_STOP);
+      hipLaunchKernelGGL(k_finalize<1>, dim3(1), dim3(kRT), 0, c.stream, part, G, S, 0, (int)FIN_ALPHA);
+      hipLaunchKernelGGL(k_bicg_s<double>, dim3(G), dim3(kRT), 0, c.stream, n, S, r, v, s, sh, dinv, part);
+      hipLaunchKernelGGL(k_finalize<1>, dim3(1), dim3(kRT), 0, c.stream, part, G, S, 0, (int)FIN_SNORM);
+      lss_spmv_mixed(L, sh, t, S + S_STOP);
+      hipLaunchKernelGGL(k_dot2<double>, dim3(G), dim3(kRT), 0, c.stream, n, t, s, t, t, part, S + S_STOP);
+      hipLaunchKernelGGL(k_finalize<2>, dim3(1), dim3(kRT), 0, c.stream, part, G, S, 0, (int)FIN_OMEGA);
+      hipLaunchKernelGGL(k_bicg_x<double>, dim3(G), dim3(kRT), 0, c.stream, n, S, xlo, ph, sh, r, s, t, rh, part);
+      hipLaunchKernelGGL(k_finalize<2>, dim3(1), dim3(kRT), 0, c.stream, part, G, S, 0, (int)FIN_RNORM_REL);
     }
+    HIP_OK(hipGetLastError());
+    read_scalars(c, sc);
+    it = it_base + (int)sc.v[S_ITER];
+    if(sc.v[S_STOP] == 0.) {
+      batch = std::min(2 * batch, kMaxBatch);
+      continue;
+    }
+    // reliable update: move the increment into the FP64 solution, replace the residual
+    const bool breakdown = !std::isfinite(sc.v[S_RN2]) || sc.v[S_RHO] == 0.;
+    hipLaunchKernelGGL(k_flush_increment, dim3(std::min<unsigned>(grid_for(n, 256), 8192)), dim3(256), 0, c.stream,
+                       n, xhi, xlo);
+    resid(xhi, r);
+    ++updates;
+    rtrue = norm2(c, r, n);
+    if(rtrue * rtrue <= thr2) { conv = true; break; }
+    if(breakdown) {   // restart the recurrences from the true residual
+      lss_copy(c, rh, r, n);
+      lss_fill(c, p, n, 0.);
+      lss_fill(c, v, n, 0.);
+      hipLaunchKernelGGL(k_set_scalars, dim3(1), dim3(1), 0, c.stream, S, thr2);
+    }
+    hipLaunchKernelGGL(k_dot2<double>, dim3(G), dim3(kRT), 0, c.stream, n, rh, r, (const double*)nullptr,
+                       (const double*)nullptr, part, (const double*)nullptr);
+    hipLaunchKernelGGL(k_finalize<1>, dim3(1), dim3(kRT), 0, c.stream, part, G, S, (int)S_RHO, (int)FIN_STORE);
+    hipLaunchKernelGGL(k_set_scalar, dim3(1), dim3(1), 0, c.stream, S, (int)S_SCONV, 0.);
+    hipLaunchKernelGGL(k_set_scalar, dim3(1), dim3(1), 0, c.stream, S, (int)S_MAXR, rtrue);
+    hipLaunchKernelGGL(k_set_scalar, dim3(1), dim3(1), 0, c.stream, S, (int)S_ITER, 0.);
+    hipLaunchKernelGGL(k_set_scalar, dim3(1), dim3(1), 0, c.stream, S, (int)S_STOP, 0.);
+    HIP_OK(hipGetLastError());
+    it_base = it;
+    batch = 1;
   }
+  const int spmvs = 2 * it;
   if(!conv) {
     hipLaunchKernelGGL(k_flush_increment, dim3(std::min<unsigned>(grid_for(n, 256), 8192)), dim3(256), 0, c.stream, n,
                        xhi, xlo);

@@ -28,8 +28,9 @@ __global__ void k_fill(T* __restrict__ p, uint64_t n, T v)
 __global__ __launch_bounds__(256) void k_spmv4_f64(uint32_t N, const uint32_t* __restrict__ row_ptr,
                                                    const uint32_t* __restrict__ cols, const double* __restrict__ vals,
                                                    double alpha, const double* __restrict__ x, double beta,
-                                                   double* __restrict__ y)
+                                                   double* __restrict__ y, const double* __restrict__ stop)
 {
+  if(stop && stop[0] != 0.) return;   // Krylov loop already stopped on the device
   const uint32_t row = (blockIdx.x * blockDim.x + threadIdx.x) >> 4;
   const int l = threadIdx.x & 15;
   if(row >= N) return;
@@ -56,8 +57,9 @@ __global__ __launch_bounds__(256) void k_spmv4_f64(uint32_t N, const uint32_t* _
 __global__ __launch_bounds__(256) void k_spmv4_f32(uint32_t N, const uint32_t* __restrict__ row_ptr,
                                                    const uint32_t* __restrict__ cols, const float* __restrict__ vals,
                                                    float alpha, const float* __restrict__ x, float beta,
-                                                   float* __restrict__ y)
+                                                   float* __restrict__ y, const double* __restrict__ stop)
 {
+  if(stop && stop[0] != 0.) return;
   const uint32_t row = (blockIdx.x * blockDim.x + threadIdx.x) >> 4;
   const int l = threadIdx.x & 15;
   if(row >= N) return;
@@ -82,8 +84,10 @@ __global__ __launch_bounds__(256) void k_spmv4_f32(uint32_t N, const uint32_t* _
 // FP32 values with FP64 vectors and FP64 accumulation, neq = 4: lane layout of k_spmv4_f32
 __global__ __launch_bounds__(256) void k_spmv4_mixed(uint32_t N, const uint32_t* __restrict__ row_ptr,
                                                      const uint32_t* __restrict__ cols, const float* __restrict__ vals,
-                                                     const double* __restrict__ x, double* __restrict__ y)
+                                                     const double* __restrict__ x, double* __restrict__ y,
+                                                     const double* __restrict__ stop)
 {
+  if(stop && stop[0] != 0.) return;
   const uint32_t row = (blockIdx.x * blockDim.x + threadIdx.x) >> 4;
   const int l = threadIdx.x & 15;
   if(row >= N) return;
@@ -103,8 +107,10 @@ __global__ __launch_bounds__(256) void k_spmv4_mixed(uint32_t N, const uint32_t*
 
 __global__ void k_spmv_mixed_generic(uint32_t N, int neq, const uint32_t* __restrict__ row_ptr,
                                      const uint32_t* __restrict__ cols, const float* __restrict__ vals,
-                                     const double* __restrict__ x, double* __restrict__ y)
+                                     const double* __restrict__ x, double* __restrict__ y,
+                                     const double* __restrict__ stop)
 {
+  if(stop && stop[0] != 0.) return;
   const uint64_t r = (uint64_t)blockIdx.x * blockDim.x + threadIdx.x;
   if(r >= (uint64_t)N * neq) return;
   const uint32_t i = (uint32_t)(r / neq);
@@ -123,8 +129,9 @@ __global__ void k_spmv_mixed_generic(uint32_t N, int neq, const uint32_t* __rest
 template<typename T>
 __global__ void k_spmv_generic(uint32_t N, int neq, const uint32_t* __restrict__ row_ptr,
                                const uint32_t* __restrict__ cols, const T* __restrict__ vals, T alpha,
-                               const T* __restrict__ x, T beta, T* __restrict__ y)
+                               const T* __restrict__ x, T beta, T* __restrict__ y, const double* __restrict__ stop)
 {
+  if(stop && stop[0] != 0.) return;
   const uint64_t r = (uint64_t)blockIdx.x * blockDim.x + threadIdx.x;
   if(r >= (uint64_t)N * neq) return;
   const uint32_t i = (uint32_t)(r / neq);
@@ -261,48 +268,50 @@ void lss_fill32(Context& c, float* p, size_t n, float v)
   HIP_OK(hipGetLastError());
 }
 
-void lss_spmv(Lss& L, double alpha, const double* x, double beta, double* y)
+void lss_spmv(Lss& L, double alpha, const double* x, double beta, double* y, const double* stop)
 {
   Context& c = *L.ctx;
   timer_begin(c, CF3HIP_KERNEL_SPMV);
   if(L.neq == 4) {
     hipLaunchKernelGGL(k_spmv4_f64, dim3(grid_for((uint64_t)L.n_nodes * 16, 256)), dim3(256), 0, c.stream, L.n_nodes,
-                       L.row_ptr.as<uint32_t>(), L.cols.as<uint32_t>(), L.values.as<double>(), alpha, x, beta, y);
+                       L.row_ptr.as<uint32_t>(), L.cols.as<uint32_t>(), L.values.as<double>(), alpha, x, beta, y, stop);
   } else {
     hipLaunchKernelGGL(k_spmv_generic<double>, dim3(grid_for(L.dofs, 256)), dim3(256), 0, c.stream, L.n_nodes, L.neq,
-                       L.row_ptr.as<uint32_t>(), L.cols.as<uint32_t>(), L.values.as<double>(), alpha, x, beta, y);
+                       L.row_ptr.as<uint32_t>(), L.cols.as<uint32_t>(), L.values.as<double>(), alpha, x, beta, y,
+                       stop);
   }
   HIP_OK(hipGetLastError());
   timer_end(c, CF3HIP_KERNEL_SPMV);
 }
 
-void lss_spmv32(Lss& L, float alpha, const float* x, float beta, float* y)
+void lss_spmv32(Lss& L, float alpha, const float* x, float beta, float* y, const double* stop)
 {
   Context& c = *L.ctx;
   if(!L.has_fp32) CF3_THROW(CF3HIP_ERR_NOT_CREATED, "FP32 values not enabled");
   timer_begin(c, CF3HIP_KERNEL_SPMV_FP32);
   if(L.neq == 4) {
     hipLaunchKernelGGL(k_spmv4_f32, dim3(grid_for((uint64_t)L.n_nodes * 16, 256)), dim3(256), 0, c.stream, L.n_nodes,
-                       L.row_ptr.as<uint32_t>(), L.cols.as<uint32_t>(), L.values32.as<float>(), alpha, x, beta, y);
+                       L.row_ptr.as<uint32_t>(), L.cols.as<uint32_t>(), L.values32.as<float>(), alpha, x, beta, y, stop);
   } else {
     hipLaunchKernelGGL(k_spmv_generic<float>, dim3(grid_for(L.dofs, 256)), dim3(256), 0, c.stream, L.n_nodes, L.neq,
-                       L.row_ptr.as<uint32_t>(), L.cols.as<uint32_t>(), L.values32.as<float>(), alpha, x, beta, y);
+                       L.row_ptr.as<uint32_t>(), L.cols.as<uint32_t>(), L.values32.as<float>(), alpha, x, beta, y,
+                       stop);
   }
   HIP_OK(hipGetLastError());
   timer_end(c, CF3HIP_KERNEL_SPMV_FP32);
 }
 
-void lss_spmv_mixed(Lss& L, const double* x, double* y)
+void lss_spmv_mixed(Lss& L, const double* x, double* y, const double* stop)
 {
   Context& c = *L.ctx;
   if(!L.has_fp32) CF3_THROW(CF3HIP_ERR_NOT_CREATED, "FP32 values not enabled");
   timer_begin(c, CF3HIP_KERNEL_SPMV_FP32);
   if(L.neq == 4) {
     hipLaunchKernelGGL(k_spmv4_mixed, dim3(grid_for((uint64_t)L.n_nodes * 16, 256)), dim3(256), 0, c.stream, L.n_nodes,
-                       L.row_ptr.as<uint32_t>(), L.cols.as<uint32_t>(), L.values32.as<float>(), x, y);
+                       L.row_ptr.as<uint32_t>(), L.cols.as<uint32_t>(), L.values32.as<float>(), x, y, stop);
   } else {
     hipLaunchKernelGGL(k_spmv_mixed_generic, dim3(grid_for(L.dofs, 256)), dim3(256), 0, c.stream, L.n_nodes, L.neq,
-                       L.row_ptr.as<uint32_t>(), L.cols.as<uint32_t>(), L.values32.as<float>(), x, y);
+                       L.row_ptr.as<uint32_t>(), L.cols.as<uint32_t>(), L.values32.as<float>(), x, y, stop);
   }
   HIP_OK(hipGetLastError());
   timer_end(c, CF3HIP_KERNEL_SPMV_FP32);
