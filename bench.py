@@ -10,7 +10,8 @@ assembly kernel against its algorithmic bytes), the FP64 SpMV bandwidth, one imp
 1e-8 relative residual in FP64 and in the FP32 + FP64-refinement form on the same mesh, the CPU
 baseline (the oracle restatement, OpenMP over the host cores, bounded sample), and the other
 workloads: the precision form on the 128^3 tetrahedral cube with 1% skewed elements, the 160^3 Q1
-hexahedral lid-driven cavity (assembly and Picard iterations) and the 32x32 triangle cavity step.
+hexahedral lid-driven cavity (assembly and Picard iterations), the 32x32 triangle cavity step, and
+the serving loop (continuous batching against one-at-a-time requests, p50/p99 latency).
 """
 import argparse
 import json
@@ -162,6 +163,20 @@ def cavity2d_workload(ctx):
             "steps": steps}
 
 
+def serving_workload(count, max_active):
+    """Continuous batching against one-request-at-a-time serving on the same request stream."""
+    from coolfluid3_amd import serving
+    reqs = serving.make_requests(count)
+    srv = serving.Server(max_active=max_active)
+    _, seq = srv.run(reqs, batched=False)
+    _, bat = srv.run(reqs, batched=True)
+    return {"workload": "%d requests, Poisson arrivals 200 req/s, cube tets with cells/side log-uniform in [12,64], "
+                        "Re U[10,1000], inflow amplitude U[0,0.1], 10 steps dt=1e-2 each, FP32 + FP64-refined "
+                        "to 1e-8, HBM budget 200 GB" % count,
+            "sequential": seq, "continuous_batching": bat,
+            "throughput_ratio": bat["throughput_req_s"] / seq["throughput_req_s"]}
+
+
 def main():
     ap = argparse.ArgumentParser()
     ap.add_argument("--gpus", type=int, default=1)
@@ -174,6 +189,8 @@ def main():
     ap.add_argument("--precision-cells", type=int, default=128)
     ap.add_argument("--hexa-cells", type=int, default=160)
     ap.add_argument("--picard", type=int, default=2)
+    ap.add_argument("--serving-requests", type=int, default=128)
+    ap.add_argument("--serving-slots", type=int, default=16)
     args = ap.parse_args()
     if args.gpus != 1:
         print("bench.py: this build measures one GPU only; multi-GPU runs are not part of it (use --gpus 1)",
@@ -281,7 +298,8 @@ def main():
     if not args.no_workloads:
         for key, fn in (("cavity_2d", lambda: cavity2d_workload(ctx)),
                         ("precision_tets", lambda: precision_workload(ctx, args.precision_cells)),
-                        ("hexa_cavity", lambda: hexa_workload(ctx, args.hexa_cells, args.picard))):
+                        ("hexa_cavity", lambda: hexa_workload(ctx, args.hexa_cells, args.picard)),
+                        ("serving", lambda: serving_workload(args.serving_requests, args.serving_slots))):
             t0 = time.perf_counter()
             extra[key] = fn()
             extra[key]["wall_s"] = time.perf_counter() - t0

@@ -123,6 +123,10 @@ def lib():
         "cf3hip_ns_update": [_vp, _vp],
         "cf3hip_ns_solve": [_vp, _vp, C.POINTER(NSParams), C.POINTER(SolverParams), C.POINTER(SolveStats)],
         "cf3hip_ns_residual": [_vp, _vp, C.POINTER(NSParams), _vp, _vp],
+        "cf3hip_batch_create": [_vp, C.c_uint32, C.POINTER(_vp)],
+        "cf3hip_context_create_stream": [_vp],
+        "cf3hip_batch_destroy": [_vp],
+        "cf3hip_batch_assemble": [_vp, C.c_uint32, C.POINTER(_vp), C.POINTER(_vp), C.POINTER(NSParams)],
     }
     for name, args in sig.items():
         fn = getattr(L, name)
@@ -160,6 +164,10 @@ class Context:
     def set_stream(self, stream_ptr):
         check(lib().cf3hip_context_set_stream(self.h, _vp(stream_ptr)))
 
+    def create_stream(self):
+        """Use a private non-blocking stream (work of different contexts then overlaps)."""
+        check(lib().cf3hip_context_create_stream(self.h))
+
     def memory(self):
         b = C.c_uint64()
         check(lib().cf3hip_context_memory(self.h, C.byref(b)))
@@ -181,6 +189,27 @@ class Context:
     def close(self):
         if self.h:
             check(lib().cf3hip_context_destroy(self.h))
+            self.h = _vp()
+
+
+class Batch:
+    """Batched FP32 assembly of many tetrahedral systems in one persistent launch (hipGraph)."""
+
+    def __init__(self, ctx, max_cases):
+        self.h = _vp()
+        self.max_cases = max_cases
+        check(lib().cf3hip_batch_create(ctx.h, max_cases, C.byref(self.h)))
+
+    def assemble(self, systems, fields, params):
+        n = len(systems)
+        ls = (_vp * max(n, 1))(*[s.h for s in systems])
+        fs = (_vp * max(n, 1))(*[f.h for f in fields])
+        ps = (NSParams * max(n, 1))(*params)
+        check(lib().cf3hip_batch_assemble(self.h, n, ls, fs, ps))
+
+    def close(self):
+        if self.h:
+            check(lib().cf3hip_batch_destroy(self.h))
             self.h = _vp()
 
 

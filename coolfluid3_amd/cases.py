@@ -193,6 +193,22 @@ class GpuCase:
         self.p = self.fields.get(cabi.FIELD_P)
         return st
 
+    def post_assembly_step(self, solver_params=None, raise_on_failure=True):
+        """BoundaryConditions, SolveLSS and UpdateActions of a step whose LinearizeU and Assembly ran
+        elsewhere (the batched assembly of the serving loop)."""
+        s = self.system
+        n, e, v = bc_lists(self.case, self.u, self.p)
+        s.dirichlet(n, e, v)
+        if solver_params is None:
+            solver_params = cabi.solver_params(tol=self.case.tol, precision=self.precision)
+        prm = cabi.NSParams.from_buffer_copy(self.prm)
+        prm.precision = self.precision
+        st = s.ns_solve(self.fields, prm, solver_params, raise_on_failure)
+        s.update(self.fields)
+        self.u = self.fields.get(cabi.FIELD_U)
+        self.p = self.fields.get(cabi.FIELD_P)
+        return st
+
     def picard_step(self, solver_params=None, raise_on_failure=True):
         """One Picard iteration of the steady solve: u_adv := u, then the implicit step."""
         self.fields.copy(cabi.FIELD_UADV, cabi.FIELD_U)

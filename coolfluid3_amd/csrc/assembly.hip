@@ -524,8 +524,9 @@ struct GenericP1 {
 // Row-owner assembly kernel.  RHS = false (FP32 matrix pass of the precision form) skips the
 // right-hand side, which is formed by the FP64 pass.
 // ---------------------------------------------------------------------------------------------
+// the work of one assembly workgroup (row range wg of the system described by g)
 template<class Op, typename R, typename VR, bool RHS>
-__global__ __launch_bounds__(kThreads) void k_assemble(AsmArgs g)
+__device__ __forceinline__ void assemble_wg(const AsmArgs& g, uint32_t wg)
 {
   constexpr int K = Op::K, NEQ = Op::NEQ, NN = NEQ * NEQ;
   static_assert(K <= 4, "one staging byte per column local node in a 32-bit word");
@@ -538,7 +539,7 @@ __global__ __launch_bounds__(kThreads) void k_assemble(AsmArgs g)
   __shared__ uint32_t s_rowinc[kThreads + 1];
   __shared__ uint16_t s_acc[kThreads];
 
-  const uint32_t r0 = g.wg_rows[blockIdx.x], r1 = g.wg_rows[blockIdx.x + 1];
+  const uint32_t r0 = g.wg_rows[wg], r1 = g.wg_rows[wg + 1];
   const uint32_t nrows = r1 - r0;
   const uint32_t i0 = g.inc_ptr[r0];
   const uint32_t ninc = g.inc_ptr[r1] - i0;
@@ -645,6 +646,68 @@ __global__ __launch_bounds__(kThreads) void k_assemble(AsmArgs g)
   }
 }
 
+template<class Op, typename R, typename VR, bool RHS>
+__global__ __launch_bounds__(kThreads) void k_assemble(AsmArgs g)
+{
+  assemble_wg<Op, R, VR, RHS>(g, blockIdx.x);
+}
+
+// ---------------------------------------------------------------------------------------------
+// Batched FP32 assembly of many tetrahedral systems (serving): one persistent launch whose
+// workgroups walk the concatenated workgroup lists of the systems in a device table.
+// ---------------------------------------------------------------------------------------------
+struct PackArgs {
+  const double* u[3];
+  const double* ua[3];
+  const double* p;
+  const double* nu;
+  double4* st;
+  double4* ad;
+};
+
+struct BatchEntry {
+  AsmArgs g;
+  PackArgs pk;
+  uint32_t wg_begin, node_begin, n_nodes, pad;
+};
+
+struct BatchHeader {
+  uint32_t n_cases, total_wg, total_nodes, pad;
+};
+
+__device__ __forceinline__ uint32_t batch_find(const BatchEntry* e, uint32_t n, uint32_t key, bool by_wg)
+{
+  uint32_t lo = 0, hi = n;   // last entry with begin <= key
+  while(hi - lo > 1) {
+    const uint32_t mid = (lo + hi) >> 1;
+    if((by_wg ? e[mid].wg_begin : e[mid].node_begin) <= key) lo = mid; else hi = mid;
+  }
+  return lo;
+}
+
+__global__ void k_batch_pack(const BatchHeader* __restrict__ hd, const BatchEntry* __restrict__ e)
+{
+  const uint32_t total = hd->total_nodes, n = hd->n_cases;
+  for(uint32_t j = blockIdx.x * blockDim.x + threadIdx.x; j < total; j += gridDim.x * blockDim.x) {
+    const uint32_t c = batch_find(e, n, j, false);
+    const uint32_t i = j - e[c].node_begin;
+    const PackArgs& a = e[c].pk;
+    a.st[i] = make_double4(a.u[0][i], a.u[1][i], a.u[2][i], a.p[i]);
+    a.ad[i] = make_double4(a.ua[0][i], a.ua[1][i], a.ua[2][i], a.nu[i]);
+  }
+}
+
+__global__ __launch_bounds__(kThreads) void k_batch_assemble(const BatchHeader* __restrict__ hd,
+                                                              const BatchEntry* __restrict__ e)
+{
+  const uint32_t total = hd->total_wg, n = hd->n_cases;
+  for(uint32_t task = blockIdx.x; task < total; task += gridDim.x) {
+    __syncthreads();   // the previous task's LDS reads are done
+    const uint32_t c = batch_find(e, n, task, true);
+    assemble_wg<SpecP1<3, float>, float, float, false>(e[c].g, task - e[c].wg_begin);
+  }
+}
+
 // Matrix-free FP64 residual of the assembled, Dirichlet-constrained system: per node row
 // r_i = sum over incident elements of (b_e(a) - M_e(a, :) x~_e), with x~ = x except on constrained dofs
 // where it holds the eliminated value (the symmetric elimination moves A[:, c] v_c to the rhs).
@@ -719,15 +782,6 @@ __global__ void k_pack_coords(uint32_t n, int dim, const double* __restrict__ x,
   xp[i] = make_double4(x[i], y[i], dim == 3 ? z[i] : 0., 0.);
 }
 
-struct PackArgs {
-  const double* u[3];
-  const double* ua[3];
-  const double* p;
-  const double* nu;
-  double4* st;
-  double4* ad;
-};
-
 template<int D>
 __global__ void k_pack_fields(uint32_t n, PackArgs a)
 {
@@ -772,7 +826,7 @@ void launch(const AsmArgs& g, const Lss& L)
 
 __global__ void k_clear_flag(int* f) { f[0] = 0; }
 
-AsmArgs make_args(Lss& L, Fields& F, const cf3hip_ns_params& prm)
+AsmArgs make_args(Lss& L, Fields& F, const cf3hip_ns_params& prm, bool pack = true)
 {
   if(!L.mesh) CF3_THROW(CF3HIP_ERR_NOT_CREATED, "system was created without a mesh: no element scatter map");
   if(prm.theta < 0. || prm.theta > 1.)
@@ -805,7 +859,7 @@ AsmArgs make_args(Lss& L, Fields& F, const cf3hip_ns_params& prm)
   g.supg_type = prm.supg_type;
   g.c1 = prm.c1; g.c2 = prm.c2; g.a_ps = prm.alpha_ps; g.a_su = prm.alpha_su; g.a_bu = prm.alpha_bu;
   g.status = L.ctx->status_flag();
-  pack_fields(F);
+  if(pack) pack_fields(F);
   return g;
 }
 
@@ -898,6 +952,79 @@ void ns_residual(Lss& L, Fields& F, const cf3hip_ns_params& prm, const double* x
     hipLaunchKernelGGL(k_bc_residual, dim3(grid_for(L.bc_n, 256)), dim3(256), 0, c.stream, L.bc_n,
                        L.bc_dof.as<uint32_t>(), L.bc_vlast.as<double>(), x, r);
   HIP_OK(hipGetLastError());
+}
+
+// ---------------------------------------------------------------------------------------------
+// batched FP32 assembly (serving): device table + persistent launch captured once in a hipGraph
+// ---------------------------------------------------------------------------------------------
+namespace {
+constexpr unsigned kBatchPackGrid = 1024;
+constexpr unsigned kBatchAsmGrid = 2048;   // persistent workgroups (8 per CU)
+constexpr size_t kBatchHeaderBytes = 256;
+}  // namespace
+
+size_t batch_table_bytes(uint32_t max_cases) { return kBatchHeaderBytes + sizeof(BatchEntry) * max_cases; }
+
+void batch_assemble(Batch& B, uint32_t n, Lss* const* L, Fields* const* F, const cf3hip_ns_params* prm)
+{
+  if(n > B.max_cases) CF3_THROW(CF3HIP_ERR_LIMIT, "more systems than the batch was created for");
+  BatchHeader* hd = reinterpret_cast<BatchHeader*>(B.host_table);
+  BatchEntry* ent = reinterpret_cast<BatchEntry*>(static_cast<char*>(B.host_table) + kBatchHeaderBytes);
+  uint32_t wg = 0, nodes = 0;
+  for(uint32_t i = 0; i < n; ++i) {
+    Lss& Li = *L[i];
+    Fields& Fi = *F[i];
+    if(!Li.mesh || Li.mesh->etype != CF3HIP_TETRA3D || !prm[i].use_specializations)
+      CF3_THROW(CF3HIP_ERR_NOT_IMPLEMENTED, "batched assembly takes tetrahedral systems with the specialised kernel");
+    if(!Li.has_fp32) CF3_THROW(CF3HIP_ERR_NOT_CREATED, "batched FP32 assembly needs cf3hip_lss_enable_fp32");
+    if((uint64_t)wg + Li.n_wg > 0xffffffffull || (uint64_t)nodes + Li.n_nodes > 0xffffffffull)
+      CF3_THROW(CF3HIP_ERR_LIMIT, "batch too large");
+    // ZeroLSS: forget the conditions of the previous step (mask cleared on the batch stream)
+    Li.bc_host.clear();
+    Li.bc_order.clear();
+    Li.bc_n = 0;
+    HIP_OK(hipStreamSynchronize(Li.ctx->stream));
+    if(Li.bc_mask.ptr) HIP_OK(hipMemsetAsync(Li.bc_mask.ptr, 0, Li.n_nodes, B.stream));
+    HIP_OK(hipMemsetAsync(Li.ctx->status_flag(), 0, sizeof(int), B.stream));   // element-validity flag
+    AsmArgs g = make_args(Li, Fi, prm[i], false);
+    g.values = Li.values32.ptr;
+    g.rhs = nullptr;
+    ent[i].g = g;
+    PackArgs& pk = ent[i].pk;
+    for(int d = 0; d < 3; ++d) {
+      pk.u[d] = Fi.u[d].as<double>();
+      pk.ua[d] = Fi.uadv[d].as<double>();
+    }
+    pk.p = Fi.p.as<double>();
+    pk.nu = Fi.nu.as<double>();
+    pk.st = Fi.st.as<double4>();
+    pk.ad = Fi.ad.as<double4>();
+    ent[i].wg_begin = wg;
+    ent[i].node_begin = nodes;
+    ent[i].n_nodes = Li.n_nodes;
+    wg += Li.n_wg;
+    nodes += Li.n_nodes;
+  }
+  hd->n_cases = n;
+  hd->total_wg = n ? wg : 0;
+  hd->total_nodes = n ? nodes : 0;
+  HIP_OK(hipMemcpyAsync(B.table.ptr, B.host_table, kBatchHeaderBytes + sizeof(BatchEntry) * (n ? n : 1),
+                        hipMemcpyHostToDevice, B.stream));
+  if(!B.exec) {
+    // the two launches read the table from device memory, so one capture serves every later batch
+    const BatchHeader* dh = B.table.as<BatchHeader>();
+    const BatchEntry* de = reinterpret_cast<const BatchEntry*>(B.table.as<char>() + kBatchHeaderBytes);
+    hipGraph_t graph = nullptr;
+    HIP_OK(hipStreamBeginCapture(B.stream, hipStreamCaptureModeThreadLocal));
+    hipLaunchKernelGGL(k_batch_pack, dim3(kBatchPackGrid), dim3(256), 0, B.stream, dh, de);
+    hipLaunchKernelGGL(k_batch_assemble, dim3(kBatchAsmGrid), dim3(kThreads), 0, B.stream, dh, de);
+    HIP_OK(hipStreamEndCapture(B.stream, &graph));
+    HIP_OK(hipGraphInstantiate(&B.exec, graph, nullptr, nullptr, 0));
+    B.graph = graph;
+  }
+  HIP_OK(hipGraphLaunch(B.exec, B.stream));
+  HIP_OK(hipStreamSynchronize(B.stream));
+  for(uint32_t i = 0; i < n; ++i) check_status(*L[i]->ctx);
 }
 
 }  // namespace cf3hip

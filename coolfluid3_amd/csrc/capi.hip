@@ -10,6 +10,7 @@ struct cf3hip_context_s { cf3hip::Context c; };
 struct cf3hip_mesh_s { cf3hip::Mesh m; };
 struct cf3hip_lss_s { cf3hip::Lss l; };
 struct cf3hip_fields_s { cf3hip::Fields f; };
+struct cf3hip_batch_s { cf3hip::Batch b; };
 
 namespace cf3hip {
 
@@ -187,7 +188,9 @@ int cf3hip_context_destroy(cf3hip_context ctx)
   for(auto& t : ctx->c.timers)
     for(hipEvent_t e : t.ev) (void)hipEventDestroy(e);
   if(ctx->c.scalars_host) (void)hipHostFree(ctx->c.scalars_host);
-  delete ctx;
+  hipStream_t own = ctx->c.own_stream;
+  delete ctx;   // device buffers are released before the stream goes away
+  if(own) (void)hipStreamDestroy(own);
   API_END
 }
 
@@ -196,6 +199,19 @@ int cf3hip_context_set_stream(cf3hip_context ctx, void* stream)
   API_BEGIN
   REQUIRE(ctx, CF3HIP_ERR_INVALID_ARG, "null context");
   ctx->c.stream = static_cast<hipStream_t>(stream);
+  API_END
+}
+
+int cf3hip_context_create_stream(cf3hip_context ctx)
+{
+  API_BEGIN
+  REQUIRE(ctx, CF3HIP_ERR_INVALID_ARG, "null context");
+  Context& c = ctx->c;
+  if(!c.own_stream) {
+    HIP_OK(hipSetDevice(c.device));
+    HIP_OK(hipStreamCreateWithFlags(&c.own_stream, hipStreamNonBlocking));
+  }
+  c.stream = c.own_stream;
   API_END
 }
 
@@ -927,6 +943,63 @@ int cf3hip_fields_initialize(cf3hip_fields fields, double nu)
   }
   lss_fill(c, F.nu.as<double>(), M.n_nodes, nu);
   HIP_OK(hipStreamSynchronize(c.stream));
+  API_END
+}
+
+int cf3hip_batch_create(cf3hip_context ctx, uint32_t max_cases, cf3hip_batch* out)
+{
+  API_BEGIN
+  REQUIRE(ctx && out, CF3HIP_ERR_INVALID_ARG, "null argument");
+  REQUIRE(max_cases > 0 && max_cases <= 4096, CF3HIP_ERR_INVALID_ARG, "max_cases must be in [1, 4096]");
+  auto* h = new cf3hip_batch_s();
+  Batch& B = h->b;
+  try {
+    B.ctx = &ctx->c;
+    B.max_cases = max_cases;
+    HIP_OK(hipSetDevice(ctx->c.device));
+    HIP_OK(hipStreamCreateWithFlags(&B.stream, hipStreamNonBlocking));
+    const size_t bytes = batch_table_bytes(max_cases);
+    B.table.alloc(B.ctx, bytes);
+    HIP_OK(hipHostMalloc(&B.host_table, bytes, hipHostMallocDefault));
+    std::memset(B.host_table, 0, bytes);
+  } catch(...) {
+    if(B.host_table) (void)hipHostFree(B.host_table);
+    if(B.stream) (void)hipStreamDestroy(B.stream);
+    delete h;
+    throw;
+  }
+  *out = h;
+  API_END
+}
+
+int cf3hip_batch_destroy(cf3hip_batch batch)
+{
+  API_BEGIN
+  if(!batch) return CF3HIP_OK;
+  Batch& B = batch->b;
+  if(B.stream) (void)hipStreamSynchronize(B.stream);
+  if(B.exec) (void)hipGraphExecDestroy(B.exec);
+  if(B.graph) (void)hipGraphDestroy(B.graph);
+  if(B.host_table) (void)hipHostFree(B.host_table);
+  B.table.release();
+  if(B.stream) (void)hipStreamDestroy(B.stream);
+  delete batch;
+  API_END
+}
+
+int cf3hip_batch_assemble(cf3hip_batch batch, uint32_t n, const cf3hip_lss* lss, const cf3hip_fields* fields,
+                          const cf3hip_ns_params* params)
+{
+  API_BEGIN
+  REQUIRE(batch && (n == 0 || (lss && fields && params)), CF3HIP_ERR_INVALID_ARG, "null argument");
+  std::vector<Lss*> L(n);
+  std::vector<Fields*> F(n);
+  for(uint32_t i = 0; i < n; ++i) {
+    REQUIRE(lss[i] && fields[i], CF3HIP_ERR_INVALID_ARG, "null system or fields");
+    L[i] = &lss[i]->l;
+    F[i] = &fields[i]->f;
+  }
+  batch_assemble(batch->b, n, L.data(), F.data(), params);
   API_END
 }
 

@@ -66,6 +66,7 @@ struct Context {
   DevBuf red_partials;   // double[kMaxRedBlocks * 8]
   DevBuf scalars;        // double[64]; the last slot holds the element-validity flag
   double* scalars_host = nullptr;   // pinned mirror
+  hipStream_t own_stream = nullptr;  // created by cf3hip_context_create_stream, destroyed with the context
   int* status_flag() { return reinterpret_cast<int*>(scalars.as<double>() + 63); }
 };
 
@@ -144,6 +145,18 @@ struct Fields {
   bool has_g = false;   // the body force holds a nonzero value (its source term is skipped otherwise)
 };
 
+// batched FP32 assembly of many systems (serving): a device table of per-system arguments and one
+// persistent launch captured in a hipGraph on a private stream
+struct Batch {
+  Context* ctx = nullptr;
+  uint32_t max_cases = 0;
+  hipStream_t stream = nullptr;
+  DevBuf table;
+  void* host_table = nullptr;   // pinned mirror
+  hipGraph_t graph = nullptr;
+  hipGraphExec_t exec = nullptr;
+};
+
 // ---------------------------------------------------------------------------------------------
 // entry points between translation units
 // ---------------------------------------------------------------------------------------------
@@ -151,6 +164,8 @@ void build_element_structures(Lss& L);                   // sparsity.hip
 void build_graph_structures(Lss& L);                     // sparsity.hip (diag blocks)
 void assemble(Lss& L, Fields& F, const cf3hip_ns_params& prm);   // assembly.hip
 void mesh_pack_coords(Mesh& M);                                  // assembly.hip
+size_t batch_table_bytes(uint32_t max_cases);                    // assembly.hip
+void batch_assemble(Batch& B, uint32_t n, Lss* const* L, Fields* const* F, const cf3hip_ns_params* prm);
 
 // lss.hip
 void lss_fill(Context& c, double* p, size_t n, double v);

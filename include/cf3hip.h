@@ -60,6 +60,9 @@ int cf3hip_context_create(int device, cf3hip_context* ctx);
 int cf3hip_context_destroy(cf3hip_context ctx);
 /* use a caller-owned hipStream_t (NULL = default stream) for every later launch */
 int cf3hip_context_set_stream(cf3hip_context ctx, void* hip_stream);
+/* give the context a private non-blocking stream of its own (default priority), used from now on
+ * (contexts on separate streams overlap on the device: the serving loop) */
+int cf3hip_context_create_stream(cf3hip_context ctx);
 int cf3hip_context_synchronize(cf3hip_context ctx);
 /* bytes currently held by device allocations of this context */
 int cf3hip_context_memory(cf3hip_context ctx, uint64_t* bytes);
@@ -237,6 +240,18 @@ int cf3hip_ns_update(cf3hip_lss lss, cf3hip_fields fields);
  * with cf3hip_lss_dirichlet; needs only the FP32 assembly of the step. */
 int cf3hip_ns_solve(cf3hip_lss lss, cf3hip_fields fields, const cf3hip_ns_params* ns_prm,
                     const cf3hip_solver_params* prm, cf3hip_solve_stats* stats);
+/* ---- serving: batched assembly (no reference counterpart; the serving form of this build) ----
+ * A batch holds a device table of per-system assembly arguments and a hipGraph with one persistent
+ * launch that assembles the FP32 element matrices of every system in the table (tetrahedra,
+ * specialised kernel; each system needs cf3hip_lss_enable_fp32).  The table is refreshed on every
+ * call, the graph is captured once.  Synchronous: returns when all systems are assembled (ZeroLSS +
+ * Assembly of each system; its Dirichlet conditions of the previous step are forgotten). */
+typedef struct cf3hip_batch_s* cf3hip_batch;
+int cf3hip_batch_create(cf3hip_context ctx, uint32_t max_cases, cf3hip_batch* batch);
+int cf3hip_batch_destroy(cf3hip_batch batch);
+int cf3hip_batch_assemble(cf3hip_batch batch, uint32_t n, const cf3hip_lss* lss, const cf3hip_fields* fields,
+                          const cf3hip_ns_params* params /* n entries */);
+
 /* matrix-free FP64 residual r = b' - A' x of the NS system with its Dirichlet conditions
  * (device pointers, dofs entries) -- no counterpart in the reference */
 int cf3hip_ns_residual(cf3hip_lss lss, cf3hip_fields fields, const cf3hip_ns_params* ns_prm, const double* x_dev,

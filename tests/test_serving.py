@@ -1,0 +1,70 @@
+"""Serving form: request generation (CPU), batched assembly and continuous batching (GPU)."""
+import numpy as np
+import pytest
+
+from coolfluid3_amd import cabi, cases, serving
+
+
+def test_request_stream():
+    reqs = serving.make_requests(512)
+    arr = np.array([r.arrival for r in reqs])
+    cells = np.array([r.cells for r in reqs])
+    assert np.all(np.diff(arr) > 0)
+    assert 150 < 512 / arr[-1] < 260                       # Poisson arrivals at 200 req/s
+    assert cells.min() >= 12 and cells.max() <= 64
+    assert 20 < np.exp(np.log(cells).mean()) < 36          # log-uniform: geometric mean ~ sqrt(12 * 64)
+    re = np.array([r.re for r in reqs])
+    assert re.min() >= 10 and re.max() <= 1000
+    again = serving.make_requests(512)
+    assert [(r.arrival, r.cells, r.re, r.amp) for r in again] == [(r.arrival, r.cells, r.re, r.amp) for r in reqs]
+
+
+@pytest.mark.gpu
+def test_batched_assembly_equals_single_assembly():
+    """One persistent batched launch gives every system the bits of its own single-system assembly."""
+    ctx = cabi.Context(0)
+    mk = lambda: [cases.GpuCase(ctx, cases.inflow_cube_tets(n, seed=100 + n, nu=1e-3 * n), precision=cabi.FP32)
+                  for n in (4, 6, 9)]
+    ref, gs = mk(), mk()
+    single = []
+    for g in ref:
+        g.fields.linearize()
+        g.assemble(cabi.FP32)
+        single.append(g.system.values_fp32().copy())
+    batch = cabi.Batch(ctx, 8)
+    prms = []
+    for g in gs:
+        g.fields.linearize()
+        p = cabi.NSParams.from_buffer_copy(g.prm)
+        p.precision = cabi.FP32
+        prms.append(p)
+    batch.assemble([g.system for g in gs], [g.fields for g in gs], prms)
+    for g, v in zip(gs, single):
+        assert np.abs(v).max() > 0 and np.array_equal(g.system.values_fp32(), v)
+    # a second, smaller batch replays the captured graph with a new table
+    batch.assemble([ref[1].system], [ref[1].fields], prms[1:2])
+    assert np.array_equal(ref[1].system.values_fp32(), single[1])
+    batch.close()
+    for g in ref + gs:
+        g.close()
+    ctx.close()
+
+
+@pytest.mark.gpu
+def test_result_alone_equals_result_in_mix():
+    """Each request gives the same fields served alone and in a mix (continuous batching)."""
+    reqs = [serving.Request(i, 0.0, n, re, amp, steps=3) for i, (n, re, amp) in
+            enumerate([(6, 100.0, 0.05), (5, 500.0, 0.0), (8, 20.0, 0.1), (7, 900.0, 0.02)])]
+    srv = serving.Server(max_active=4, keep_fields=True)
+    alone, _ = srv.run(reqs[:1])
+    mix, summary = srv.run(reqs)
+    a = alone[0]
+    m = [r for r in mix if r.rid == 0][0]
+    du = np.abs(a.u - m.u).max() / np.abs(a.u).max()
+    print(f"alone vs mix: max relative velocity difference {du:.2e}; {summary}")
+    assert du <= 1e-12                                    # stated tolerance (the paths are deterministic)
+    assert np.abs(a.p - m.p).max() <= 1e-12 * max(np.abs(a.p).max(), 1.0)
+    assert summary["all_converged"]
+    # the sequential path serves the same request to the same fields
+    seq, _ = srv.run(reqs[:1], batched=False)
+    assert np.abs(seq[0].u - a.u).max() <= 1e-12 * np.abs(a.u).max()
