@@ -143,6 +143,8 @@ def hexa_workload(ctx, cells, picard):
         runs[label] = [timed_step(ctx, gc, sp, picard=True) for _ in range(picard)]
         runs[label + "_u"] = gc.u.copy()
         gc.close()
+    if picard == 0:
+        return res
     res["fp64_picard"] = runs["fp64"]
     res["fp32_refined_picard"] = runs["fp32_refined"]
     res["speedup_per_iteration"] = runs["fp64"][-1]["ms"] / runs["fp32_refined"][-1]["ms"]
@@ -185,7 +187,9 @@ def main():
     ap.add_argument("--cells", type=int, default=64, help="cube cells per side (64 = the drop-in-at-scale case)")
     ap.add_argument("--no-cpu-baseline", action="store_true")
     ap.add_argument("--no-solve", action="store_true")
-    ap.add_argument("--no-workloads", action="store_true", help="skip the 128^3 / 160^3 / 2D workloads")
+    ap.add_argument("--no-workloads", action="store_true", help="skip the 128^3 / 160^3 / 2D / serving workloads")
+    ap.add_argument("--workloads", default="cavity_2d,precision_tets,hexa_cavity,serving",
+                    help="comma-separated subset of the extra workloads")
     ap.add_argument("--precision-cells", type=int, default=128)
     ap.add_argument("--hexa-cells", type=int, default=160)
     ap.add_argument("--picard", type=int, default=2)
@@ -300,6 +304,8 @@ def main():
                         ("precision_tets", lambda: precision_workload(ctx, args.precision_cells)),
                         ("hexa_cavity", lambda: hexa_workload(ctx, args.hexa_cells, args.picard)),
                         ("serving", lambda: serving_workload(args.serving_requests, args.serving_slots))):
+            if key not in args.workloads.split(","):
+                continue
             t0 = time.perf_counter()
             extra[key] = fn()
             extra[key]["wall_s"] = time.perf_counter() - t0
