@@ -40,7 +40,7 @@ def algorithmic_spmv_bytes(n_nodes, dim, n_blocks, value_bytes):
 
 
 def cpu_baseline(case, budget_s=20.0):
-    """Oracle restatement on the host cores (the checker library, bounded sample)."""
+    """Oracle restatement on the host cores (the test oracle library, bounded sample)."""
     from oracle import pyoracle
     o = pyoracle.get()
     start, cols, slots = o.build_sparsity(case.conn, case.n_nodes)
@@ -197,7 +197,7 @@ def main():
     ap.add_argument("--serving-slots", type=int, default=16)
     args = ap.parse_args()
     if args.gpus != 1:
-        print("bench.py: this build measures one GPU only; multi-GPU runs are not part of it (use --gpus 1)",
+        print("bench.py: this benchmark measures one GPU only; multi-GPU runs are not part of it (use --gpus 1)",
               file=sys.stderr)
         sys.exit(2)
 

@@ -162,7 +162,7 @@ public:
   void symmetric_dirichlet(uint32_t blockrow, uint32_t ieq, double value, Vector& rhs);
   void tie_blockrow_pairs(uint32_t, uint32_t)
   {
-    throw NotImplemented(CF3HIP_ERR_NOT_IMPLEMENTED, "periodic block-row ties are not part of this build");
+    throw NotImplemented(CF3HIP_ERR_NOT_IMPLEMENTED, "periodic block-row ties are not implemented by cf3hip");
   }
   void set_diagonal(const std::vector<double>& diag);
   void add_diagonal(const std::vector<double>& diag);

@@ -240,7 +240,7 @@ int cf3hip_ns_update(cf3hip_lss lss, cf3hip_fields fields);
  * with cf3hip_lss_dirichlet; needs only the FP32 assembly of the step. */
 int cf3hip_ns_solve(cf3hip_lss lss, cf3hip_fields fields, const cf3hip_ns_params* ns_prm,
                     const cf3hip_solver_params* prm, cf3hip_solve_stats* stats);
-/* ---- serving: batched assembly (no reference counterpart; the serving form of this build) ----
+/* ---- serving: batched assembly (no reference counterpart; the serving form of cf3hip) ----
  * A batch holds a device table of per-system assembly arguments and a hipGraph with one persistent
  * launch that assembles the FP32 element matrices of every system in the table (tetrahedra,
  * specialised kernel; each system needs cf3hip_lss_enable_fp32).  The table is refreshed on every

@@ -1,5 +1,5 @@
 // Oracle: CPU FP64 restatement of the COOLFluiD 3 UFEM Navier-Stokes hot path.
-// TEST INFRASTRUCTURE ONLY (the checker).  It is loaded by tests/, __graft_entry__.smoke() and
+// TEST INFRASTRUCTURE ONLY (the test oracle).  It is loaded by tests/, __graft_entry__.smoke() and
 // bench.py's cpu_baseline leg; the product library never links or calls it.
 //
 // Reference citations are relative to the coolfluid3 source tree.

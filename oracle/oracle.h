@@ -2,7 +2,7 @@
  * Oracle: CPU FP64 restatement of COOLFluiD 3's UFEM incompressible Navier-Stokes hot path
  * (element assembly -> LSS scatter -> symmetric Dirichlet -> SpMV/Krylov).
  *
- * TEST INFRASTRUCTURE ONLY.  This library is the checker: it is loaded by tests/, by
+ * TEST INFRASTRUCTURE ONLY.  This library is the test oracle: it is loaded by tests/, by
  * __graft_entry__.smoke() and by bench.py's cpu_baseline leg, never by the product path.
  * Every function cites the reference file:line it restates (paths relative to the
  * coolfluid3 source tree).

@@ -1,7 +1,7 @@
 """ctypes front-end of the oracle library (oracle/build/liboracle.so).
 
-TEST INFRASTRUCTURE ONLY: the oracle is the CPU FP64 restatement of the reference hot path used as
-the checker by tests/, __graft_entry__.smoke() and bench.py's cpu_baseline leg.  The product
+TEST INFRASTRUCTURE ONLY: the oracle is the CPU FP64 restatement of the reference hot path used to
+check results by tests/, __graft_entry__.smoke() and bench.py's cpu_baseline leg.  The product
 package (coolfluid3_amd) never imports this module.
 """
 import ctypes as C

@@ -15,7 +15,7 @@ def pytest_configure(config):
 
 @pytest.fixture(scope="session")
 def oracle():
-    """The CPU FP64 restatement of the reference (the checker), built on demand."""
+    """The CPU FP64 restatement of the reference (the test oracle), built on demand."""
     from oracle import pyoracle
     if not os.path.exists(pyoracle.LIB_PATH):
         subprocess.check_call(["make", "-C", os.path.join(ROOT, "oracle")])
