@@ -134,17 +134,18 @@ struct SpecP1 {
       for(int m = 0; m < K; ++m)
         for(int d = 0; d < D; ++d) Xr[m][d] = (R)X[m][d];
       geometry<R>(Xr, n, V);
-      // strongly skewed elements: geometry and tau in FP64.  Criterion |V| < 0.0125 l_rms^3 (l_rms: rms
-      // edge length), i.e. aspect ratio (longest edge / smallest height) above ~14; the regular Kuhn
-      // tetrahedron has V = 0.0775 l_rms^3.
-      R l2 = R(0);
-      for(int i = 0; i < K; ++i)
-        for(int j = i + 1; j < K; ++j)
-          for(int d = 0; d < D; ++d) { const R df = Xr[i][d] - Xr[j][d]; l2 += df * df; }
-      const R lm = sqrt(l2 * R(2.0 / (K * (K - 1))));
-      R lp = lm;
-      for(int d = 1; d < D; ++d) lp *= lm;
-      if(fabs(V) < R(0.0125) * lp) fp64_geom = true;
+      // strongly skewed elements: geometry and tau in FP64.  Criterion from the face normals already
+      // formed, S = sum |n_i|^2: tetrahedra V^2 < 1.4e-3 S^(3/2) (regular Kuhn tetrahedron 1.5e-2;
+      // aspect ratio (longest edge / smallest height) 15 and more falls below ~7e-4, aspect ratio 8
+      // and less stays above ~2.7e-3); triangles V < S / 240 (rotated edges: |V| < 0.0125 l_rms^2).
+      R S = R(0);
+      for(int m = 0; m < K; ++m)
+        for(int d = 0; d < D; ++d) S += n[m][d] * n[m][d];
+      if constexpr(D == 3) {
+        if(V * V < R(1.4e-3) * S * sqrt(S)) fp64_geom = true;
+      } else {
+        if(fabs(V) < S * R(1.0 / 240.0)) fp64_geom = true;
+      }
     }
     if(fp64_geom) {
       double n64[K][D], V64, ts, tb;
