@@ -73,7 +73,7 @@ using namespace cf3hip;
 #define API_BEGIN try {
 #define API_END                                                                                  \
   }                                                                                              \
-  catch(const cf3hip::Error& e) {                                                                \
+  catch(const cf3hip::Failure& e) {                                                              \
     set_error(e.msg);                                                                            \
     return e.code;                                                                               \
   }                                                                                              \

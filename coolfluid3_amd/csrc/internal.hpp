@@ -17,12 +17,13 @@ namespace cf3hip {
 // ---------------------------------------------------------------------------------------------
 void set_error(const std::string& msg);
 
-struct Error {
+// internal failure carried to the C ABI boundary and turned into a status code there
+struct Failure {
   int code;
   std::string msg;
 };
 
-#define CF3_THROW(code, msg) throw ::cf3hip::Error{code, msg}
+#define CF3_THROW(code, msg) throw ::cf3hip::Failure{code, msg}
 #define HIP_OK(expr)                                                                             \
   do {                                                                                           \
     hipError_t e_ = (expr);                                                                      \

@@ -19,6 +19,10 @@
 #ifdef __cplusplus
 extern "C" {
 #endif
+/* the library is built with hidden visibility: only these C entry points are exported */
+#if defined(__GNUC__)
+#pragma GCC visibility push(default)
+#endif
 
 /* ---- status codes: mapped to the reference exceptions by the C++ host layer ---- */
 #define CF3HIP_OK                 0
@@ -257,6 +261,9 @@ int cf3hip_batch_assemble(cf3hip_batch batch, uint32_t n, const cf3hip_lss* lss,
 int cf3hip_ns_residual(cf3hip_lss lss, cf3hip_fields fields, const cf3hip_ns_params* ns_prm, const double* x_dev,
                        double* r_dev);
 
+#if defined(__GNUC__)
+#pragma GCC visibility pop
+#endif
 #ifdef __cplusplus
 }
 #endif
