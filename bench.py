@@ -169,7 +169,7 @@ def serving_workload(count, max_active):
     """Continuous batching against one-request-at-a-time serving on the same request stream."""
     from coolfluid3_amd import serving
     reqs = serving.make_requests(count)
-    srv = serving.Server(max_active=max_active)
+    srv = serving.Server(max_active=max_active, log=lambda m: print("serving: " + m, file=sys.stderr, flush=True))
     _, seq = srv.run(reqs, batched=False)
     _, bat = srv.run(reqs, batched=True)
     return {"workload": "%d requests, Poisson arrivals 200 req/s, cube tets with cells/side log-uniform in [12,64], "
@@ -193,7 +193,7 @@ def main():
     ap.add_argument("--precision-cells", type=int, default=128)
     ap.add_argument("--hexa-cells", type=int, default=160)
     ap.add_argument("--picard", type=int, default=2)
-    ap.add_argument("--serving-requests", type=int, default=128)
+    ap.add_argument("--serving-requests", type=int, default=48)
     ap.add_argument("--serving-slots", type=int, default=16)
     args = ap.parse_args()
     if args.gpus != 1:

@@ -112,12 +112,14 @@ class Record:
 
 
 class Server:
-    def __init__(self, max_active=16, hbm_budget=200e9, tol=1e-8, seed=cases.DEFAULT_SEED, keep_fields=False):
+    def __init__(self, max_active=16, hbm_budget=200e9, tol=1e-8, seed=cases.DEFAULT_SEED, keep_fields=False,
+                 log=None):
         self.max_active = max_active
         self.hbm_budget = hbm_budget
         self.tol = tol
         self.seed = seed
         self.keep_fields = keep_fields
+        self.log = log   # optional callable(str): one line per finished request
 
     def run(self, requests, batched=True, time_scale=1.0):
         """Serve `requests` (arrival times scaled by time_scale); returns (records, summary)."""
@@ -134,8 +136,13 @@ class Server:
     def _finish(self, c, records, now):
         u = c.g.u.copy() if self.keep_fields else None
         p = c.g.p.copy() if self.keep_fields else None
-        records.append(Record(c.req.rid, c.req.cells, c.req.arrival * self._ts, c.admitted, now(), c.iterations,
-                              c.converged, u, p))
+        rec = Record(c.req.rid, c.req.cells, c.req.arrival * self._ts, c.admitted, now(), c.iterations,
+                     c.converged, u, p)
+        records.append(rec)
+        if self.log:
+            self.log("request %d (%d^3 cells, Re %.0f): %d Krylov iterations in %d steps, latency %.3f s%s"
+                     % (rec.rid, rec.cells, c.req.re, rec.iterations, c.steps_done, rec.latency,
+                        "" if rec.converged else ", NOT converged"))
         c.close()
 
     def _run_sequential(self, pending, records, now, ts):
