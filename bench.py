@@ -309,7 +309,8 @@ def main():
             t0 = time.perf_counter()
             extra[key] = fn()
             extra[key]["wall_s"] = time.perf_counter() - t0
-            print("bench: %s done in %.1f s" % (key, extra[key]["wall_s"]), file=sys.stderr, flush=True)
+            print("bench: %s done in %.1f s: %s" % (key, extra[key]["wall_s"], json.dumps(extra[key])),
+                  file=sys.stderr, flush=True)
     ctx.close()
     if not args.no_cpu_baseline:
         out["cpu_baseline"] = cpu_baseline(case)
