@@ -448,6 +448,14 @@ void reliable_bicgstab(Lss& L, Resid resid, const cf3hip_solver_params& prm, cf3
   hipLaunchKernelGGL(k_finalize<1>, dim3(1), dim3(kRT), 0, c.stream, part, G, S, (int)S_RHO, (int)FIN_STORE);
   HIP_OK(hipGetLastError());
   bool conv = rtrue * rtrue <= thr2;
+  // stagnation guard: the FP32 operator may be too far from the FP64 one for the recurrences to
+  // reduce the true residual (a system whose solution is very sensitive to its matrix).  The solve
+  // stops, unconverged, when kMaxStall replacements in a row fail to halve the true residual of the
+  // last replacement that did, or when it grows beyond kDiverge times that value.
+  constexpr int kMaxStall = 8;
+  constexpr double kDiverge = 1e4;
+  double rprog = rtrue;
+  int stall = 0;
   hipLaunchKernelGGL(k_set_scalar, dim3(1), dim3(1), 0, c.stream, S, (int)S_MAXR, rtrue);
   hipLaunchKernelGGL(k_set_scalar, dim3(1), dim3(1), 0, c.stream, S, (int)S_DELTA, delta);
   int it = 0, it_base = 0, batch = 1;
@@ -483,6 +491,13 @@ void reliable_bicgstab(Lss& L, Resid resid, const cf3hip_solver_params& prm, cf3
     ++updates;
     rtrue = norm2(c, r, n);
     if(rtrue * rtrue <= thr2) { conv = true; break; }
+    if(!std::isfinite(rtrue) || rtrue > kDiverge * rprog) break;
+    if(rtrue < 0.5 * rprog) {
+      rprog = rtrue;
+      stall = 0;
+    } else if(++stall >= kMaxStall) {
+      break;
+    }
     if(breakdown) {   // restart the recurrences from the true residual
       lss_copy(c, rh, r, n);
       lss_fill(c, p, n, 0.);

@@ -1,9 +1,10 @@
 """Serving form: a request loop that runs implicit Navier-Stokes time steps for many meshes at once.
 
-Each request is a unit-cube tetrahedral case (cells per side log-uniform in [12, 64], Re uniform in
-[10, 1000], inflow perturbation amplitude uniform in [0, 0.1], all drawn from one SplitMix64 stream)
-that runs `steps` implicit steps (dt = 1e-2) in the precision form (FP32 element matrices, FP64
-matrix-free residual, reliable-update BiCGStab to 1e-8).
+Each request is a unit-cube tetrahedral channel case (cells per side log-uniform in [12, 64], Re
+uniform in [10, 1000], inflow perturbation amplitude uniform in [0, 0.1], all drawn from one SplitMix64
+stream; inflow on x = 1, outlet p = 0 on x = 0, no-slip on the side faces) that runs `steps` implicit
+steps (dt = 1e-2) in the precision form (FP32 element matrices, FP64 matrix-free residual,
+reliable-update BiCGStab to 1e-8; a step whose solve stagnates is redone in FP64 and counted).
 
 Continuous batching (`Server.run(..., batched=True)`):
   * a request is admitted when it has arrived, a slot is free and its device memory fits the HBM
@@ -66,7 +67,8 @@ class _Case:
         self.req = req
         self.ctx = cabi.Context(0)
         self.ctx.create_stream()
-        case = cases.inflow_cube_tets(req.cells, seed=seed + req.rid, nu=1.0 / req.re, dt=req.dt, amp=req.amp)
+        case = cases.inflow_cube_tets(req.cells, seed=seed + req.rid, nu=1.0 / req.re, dt=req.dt, amp=req.amp,
+                                      outlet=True)
         case.tol = tol
         self.g = cases.GpuCase(self.ctx, case, precision=cabi.FP32)
         self.bytes = self.ctx.memory()
@@ -103,6 +105,7 @@ class Record:
     done: float
     iterations: int
     converged: bool
+    fallbacks: int = 0     # steps whose precision-form solve stagnated and were redone in FP64
     u: np.ndarray = field(default=None, repr=False)
     p: np.ndarray = field(default=None, repr=False)
 
@@ -137,11 +140,12 @@ class Server:
         u = c.g.u.copy() if self.keep_fields else None
         p = c.g.p.copy() if self.keep_fields else None
         rec = Record(c.req.rid, c.req.cells, c.req.arrival * self._ts, c.admitted, now(), c.iterations,
-                     c.converged, u, p)
+                     c.converged, c.g.fallbacks, u, p)
         records.append(rec)
         if self.log:
-            self.log("request %d (%d^3 cells, Re %.0f): %d Krylov iterations in %d steps, latency %.3f s%s"
+            self.log("request %d (%d^3 cells, Re %.0f): %d Krylov iterations in %d steps, latency %.3f s%s%s"
                      % (rec.rid, rec.cells, c.req.re, rec.iterations, c.steps_done, rec.latency,
+                        ", %d FP64 fallback steps" % rec.fallbacks if rec.fallbacks else "",
                         "" if rec.converged else ", NOT converged"))
         c.close()
 
@@ -212,4 +216,5 @@ def summarize(records, batched, max_active):
             "p50_latency_s": float(np.percentile(lat, 50)), "p99_latency_s": float(np.percentile(lat, 99)),
             "mean_latency_s": float(lat.mean()), "throughput_req_s": len(records) / (last - first),
             "makespan_s": last - first, "all_converged": bool(all(r.converged for r in records)),
-            "krylov_iterations": int(sum(r.iterations for r in records))}
+            "krylov_iterations": int(sum(r.iterations for r in records)),
+            "fp64_fallback_steps": int(sum(r.fallbacks for r in records))}

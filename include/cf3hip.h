@@ -30,7 +30,7 @@ extern "C" {
 #define CF3HIP_ERR_NOT_CREATED    2  /* common::SetupError (LSS not created, bad theta)     */
 #define CF3HIP_ERR_HIP            3  /* HIP runtime failure                                 */
 #define CF3HIP_ERR_OUT_OF_MEMORY  4  /* device allocation failed                            */
-#define CF3HIP_ERR_NOT_CONVERGED  5  /* Krylov solve hit max iterations                     */
+#define CF3HIP_ERR_NOT_CONVERGED  5  /* Krylov solve hit max iterations or stagnated        */
 #define CF3HIP_ERR_ILLEGAL_ENTRY  6  /* BadValue "Trying to access an illegal entry"        */
 #define CF3HIP_ERR_NOT_IMPLEMENTED 7 /* common::NotImplemented                              */
 #define CF3HIP_ERR_LIMIT          8  /* mesh exceeds a documented kernel limit              */
@@ -241,7 +241,10 @@ int cf3hip_ns_update(cf3hip_lss lss, cf3hip_fields fields);
  * cf3hip_lss_solve).  precision FP32: BiCGStab on the FP32 element matrices (FP32 values, FP64
  * vectors and accumulation) with reliable updates against the FP64 residual b' - A' x evaluated
  * matrix-free from the FP64 element matrices of the same fields, after Dirichlet conditions applied
- * with cf3hip_lss_dirichlet; needs only the FP32 assembly of the step. */
+ * with cf3hip_lss_dirichlet; needs only the FP32 assembly of the step.  The FP32 form stops early,
+ * unconverged (CF3HIP_ERR_NOT_CONVERGED, stats filled), when 8 residual replacements in a row fail
+ * to halve the FP64 residual, or it grows 1e4-fold: its FP32 operator is then too far from the FP64
+ * one for that system, and the caller re-assembles in FP64 and solves with precision FP64. */
 int cf3hip_ns_solve(cf3hip_lss lss, cf3hip_fields fields, const cf3hip_ns_params* ns_prm,
                     const cf3hip_solver_params* prm, cf3hip_solve_stats* stats);
 /* ---- serving: batched assembly (no reference counterpart; the serving form of cf3hip) ----

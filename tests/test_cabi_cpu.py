@@ -98,3 +98,34 @@ def test_hexa_cavity_case():
     assert (n[-1], e[-1]) == (0, 3)
     top = set(range(4 * 25, 125))
     assert top <= lid
+
+
+def test_outlet_channel_boundary_conditions():
+    """Outlet variant: p = 0 on every node of the x = 0 face and no velocity condition there except on
+    its edges with the no-slip side faces; the inflow face is applied last; the closed box instead
+    walls x = 0 and pins p at one node."""
+    n = 6
+    t = cases.inflow_cube_tets(n, outlet=True, amp=0.05)
+    nodes, eqs, vals = cases.bc_lists(t, t.u0, t.p0)
+    x = t.coords[:, 0]
+    on_outlet = np.isclose(x, 0.0)
+    side = np.zeros(t.n_nodes, bool)
+    for ax in (1, 2):
+        side |= np.isclose(t.coords[:, ax], 0.0) | np.isclose(t.coords[:, ax], 1.0)
+    p_nodes = set(nodes[eqs == 3].tolist())
+    assert p_nodes == set(np.nonzero(on_outlet)[0].tolist())
+    vel_nodes = set(nodes[eqs == 0].tolist())
+    assert not (vel_nodes & set(np.nonzero(on_outlet & ~side)[0].tolist()))
+    inflow = np.nonzero(np.isclose(x, 1.0))[0]
+    # the inflow region comes last: its nodes' last x-velocity condition is the inflow profile
+    last = {}
+    for a, q, v in zip(nodes, eqs, vals):
+        if q == 0:
+            last[int(a)] = v
+    y, z = t.coords[inflow, 1], t.coords[inflow, 2]
+    prof = -16 * y * (1 - y) * z * (1 - z) * (1 + 0.05 * np.sin(np.pi * y) * np.sin(np.pi * z))
+    assert np.allclose([last[int(i)] for i in inflow], prof - t.u0[inflow, 0], rtol=0, atol=1e-15)
+    c = cases.inflow_cube_tets(n)
+    cn, ce, _ = cases.bc_lists(c, c.u0, c.p0)
+    assert (ce == 3).sum() == 1 and np.isclose(c.coords[cn[ce == 3][0], 0], 0.0)
+    assert set(cn[ce == 0].tolist()) >= set(np.nonzero(np.isclose(c.coords[:, 0], 0.0))[0].tolist())

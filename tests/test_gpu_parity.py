@@ -281,6 +281,93 @@ def test_precision_form_matrix_free(ctx, oracle, name):
     g64.close()
 
 
+def test_matrix_free_residual_after_time_steps(ctx, oracle):
+    """After two implicit steps (u, u1, u2, u3 all different, u_adv extrapolated by LinearizeU) the FP64
+    system assembled from the device fields equals the oracle's from the same fields, and the
+    matrix-free residual equals rhs - A x of that system."""
+    case = cases.inflow_cube_tets(5, outlet=True)
+    g = gpu_case(ctx, case)
+    for _ in range(2):
+        assert g.step(cabi.solver_params(tol=1e-10)).converged
+    g.fields.linearize()
+    g.assemble(cabi.FP64)
+    f = {k: g.fields.get(k) for k in (cabi.FIELD_U, cabi.FIELD_U1, cabi.FIELD_U2, cabi.FIELD_U3, cabi.FIELD_UADV)}
+    assert np.abs(f[cabi.FIELD_U] - f[cabi.FIELD_U1]).max() > 1e-3
+    assert np.abs(f[cabi.FIELD_U1] - f[cabi.FIELD_U2]).max() > 1e-3
+    s = g.system
+    start, cols = s.graph()
+    ua = oracle.linearize_u(f[cabi.FIELD_U], f[cabi.FIELD_U1], f[cabi.FIELD_U2], f[cabi.FIELD_U3])
+    assert np.abs(ua - f[cabi.FIELD_UADV]).max() <= 1e-14 * np.abs(ua).max()
+    _, _, slots = oracle.build_sparsity(case.conn, case.n_nodes)
+    prm = pyoracle.ns_params(rho=case.rho, dt=case.dt, theta=case.theta)
+    vref, rref = oracle.ns_assemble(pyoracle.TETRA3D, prm, case.coords, case.conn, g.u, g.p, ua,
+                                    np.full(case.n_nodes, case.nu), start, cols, slots)
+    assert row_rel_err(s.values(), vref, 4, start) <= 1e-12
+    assert np.abs(s.vector(cabi.RHS) - rref).max() <= 1e-12 * np.abs(rref).max()
+    n, e, vals = cases.bc_lists(case, g.u, g.p)
+    s.dirichlet(n, e, vals)
+    A = s.values()
+    b = s.vector(cabi.RHS)
+    x = np.random.default_rng(5).uniform(-1, 1, s.dofs)
+    r_ref = b - oracle.spmv(4, start, cols, A, x)
+    s.set_vector(cabi.SOLUTION, x)
+    s.ns_residual_device(g.fields, g.prm, s.vector_ptr(cabi.SOLUTION), s.vector_ptr(cabi.RHS))
+    r = s.vector(cabi.RHS)
+    scale = np.abs(b).max() + np.abs(A).max() * np.abs(x).max()
+    assert np.abs(r - r_ref).max() <= 1e-12 * scale
+    g.close()
+
+
+def test_precision_form_time_steps(ctx):
+    """Three implicit steps of the outlet channel: every precision-form solve reaches 1e-8 of the FP64
+    system without the FP64 fallback, and the fields follow the FP64 path (difference stated)."""
+    case = cases.inflow_cube_tets(12, outlet=True, nu=1e-2, dt=1e-2, amp=0.05)
+    g32 = cases.GpuCase(ctx, case, precision=cabi.FP32)
+    g64 = cases.GpuCase(ctx, case)
+    for step in range(3):
+        st64 = g64.step(cabi.solver_params(tol=1e-8))
+        st32 = g32.step(cabi.solver_params(tol=1e-8, precision=cabi.FP32))
+        du = np.abs(g32.u - g64.u).max() / np.abs(g64.u).max()
+        print(f"step {step}: FP64 {st64.iterations} it, FP32 form {st32.iterations} it ({st32.outer} FP64 "
+              f"residuals), rel residual {st32.rel_residual:.2e}, velocity difference {du:.2e}")
+        assert st64.converged and st32.converged and st32.rel_residual <= 1e-8 * 1.001
+        assert g32.fallbacks == 0
+        assert st32.iterations <= 1.5 * st64.iterations + 5
+        assert du <= 1e-5
+    g32.close()
+    g64.close()
+
+
+def test_precision_form_fallback_is_the_fp64_step(ctx):
+    """A precision-form solve that does not reach the tolerance (here: 2 iterations allowed) is redone
+    with FP64 values assembled from the same state; the step then gives the FP64 step's fields."""
+    case = cases.inflow_cube_tets(8, outlet=True, nu=1e-2, dt=1e-2)
+    g32 = cases.GpuCase(ctx, case, precision=cabi.FP32)
+    g32.fallback_params = cabi.solver_params(tol=1e-8)
+    g64 = cases.GpuCase(ctx, case)
+    st32 = g32.step(cabi.solver_params(tol=1e-8, precision=cabi.FP32, max_iter=2))
+    st64 = g64.step(cabi.solver_params(tol=1e-8))
+    assert g32.fallbacks == 1 and st32.converged and st32.iterations == st64.iterations + 2
+    assert np.abs(g32.u - g64.u).max() <= 1e-12 * np.abs(g64.u).max()
+    assert np.abs(g32.p - g64.p).max() <= 1e-12 * max(np.abs(g64.p).max(), 1.0)
+    g32.close()
+    g64.close()
+
+
+def test_precision_form_closed_box_stops_early(ctx):
+    """The closed box with net inflow (36^3, Re 82, dt 1e-2): from the second step on its solution is so
+    sensitive to the matrix that the FP32 operator may fail to reduce the FP64 residual; the solver then
+    stops well before max_iter and the step is redone in FP64.  Every step reaches the tolerance."""
+    case = cases.inflow_cube_tets(36, nu=1.0 / 82.0, dt=1e-2, amp=0.05)
+    g = cases.GpuCase(ctx, case, precision=cabi.FP32)
+    sp = cabi.solver_params(tol=1e-8, precision=cabi.FP32, max_iter=5000)
+    sts = [g.step(sp) for _ in range(2)]
+    print("closed box: iterations", [st.iterations for st in sts], "FP64 fallbacks", g.fallbacks, flush=True)
+    assert all(st.converged and st.rel_residual <= 1e-8 * 1.001 for st in sts)
+    assert sts[1].iterations < 5000
+    g.close()
+
+
 def test_hexa_picard_precision_form(ctx, oracle):
     """Q1 lid-driven cavity, two Picard iterations: FP32 element matrices with the FP64 matrix-free
     residual reach 1e-8 of the FP64 system; the iterates agree with the FP64 path (stated)."""
