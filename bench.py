@@ -71,7 +71,7 @@ def timed_step(ctx, gc, sp, picard=False):
     st = gc.picard_step(sp, raise_on_failure=False) if picard else gc.step(sp, raise_on_failure=False)
     ctx.synchronize()
     ms = (time.perf_counter() - t0) * 1e3
-    return {"ms": ms, "iterations": st.iterations, "fp64_residuals": st.outer, "spmvs": st.spmvs,
+    return {"ms": ms, "iterations": st.iterations, "outer_steps": st.outer, "spmvs": st.spmvs,
             "rel_residual": st.rel_residual, "converged": bool(st.converged)}
 
 
@@ -98,10 +98,10 @@ def assembly_rates(ctx, gc, reps, k, dim):
 
 def precision_workload(ctx, cells, steps=2):
     """Precision form on the tetrahedral cube channel (inflow x = 1, outlet p = 0 on x = 0) with ~1%
-    elements of aspect ratio ~20 (nu = 5e-4): FP32 element matrices + reliable-update BiCGStab with the
-    FP64 matrix-free residual, against the FP64 assembly and BiCGStab, both to a 1e-8 relative residual
-    of the FP64 system.  Step 1 includes one-time allocations; the per-step comparison uses the last
-    step."""
+    elements of aspect ratio ~20 (nu = 5e-4): FP32 element matrices, FP32 BiCGStab inner solves under
+    FP64 flexible GMRES on the matrix-free FP64 operator, against the FP64 assembly and BiCGStab, both
+    to a 1e-8 relative residual of the FP64 system.  Step 1 includes one-time allocations; the per-step
+    comparison uses the last step."""
     from coolfluid3_amd import cabi, cases
     case = cases.inflow_cube_tets(cells, nu=5e-4, dt=5e-3, skew=True, outlet=True)
     res = {"workload": "unit cube %d^3 hexes x 6 P1 tets (%d elements, %d dofs), 1%% skewed (aspect ratio ~20), "
@@ -109,7 +109,7 @@ def precision_workload(ctx, cells, steps=2):
     runs = {}
     for label, prec in (("fp64", cabi.FP64), ("fp32_refined", cabi.FP32)):
         gc = cases.GpuCase(ctx, case, precision=prec)
-        sp = cabi.solver_params(tol=1e-8, precision=prec, inner_tol=1e-4, max_iter=5000)
+        sp = cabi.solver_params(tol=1e-8, precision=prec, max_iter=5000)
         runs[label] = [timed_step(ctx, gc, sp) for _ in range(steps)]
         runs[label + "_u"] = gc.u.copy()
         if label == "fp32_refined":
@@ -141,7 +141,7 @@ def hexa_workload(ctx, cells, picard):
             res["assembly"] = assembly_rates(ctx, gc, 3, 8, 3)
             k64 = res["assembly"]["fp64"]["kernel_ms"]
             res["assembly"]["fp64"]["fp64_alu_frac"] = case.n_elems * HEX_KFLOP * 1e3 / (k64 * 1e-3) / 78.6e12
-        sp = cabi.solver_params(tol=1e-8, precision=prec, inner_tol=1e-4, max_iter=5000)
+        sp = cabi.solver_params(tol=1e-8, precision=prec, max_iter=5000)
         runs[label] = [timed_step(ctx, gc, sp, picard=True) for _ in range(picard)]
         runs[label + "_u"] = gc.u.copy()
         gc.close()
@@ -195,7 +195,7 @@ def main():
     ap.add_argument("--precision-cells", type=int, default=128)
     ap.add_argument("--hexa-cells", type=int, default=160)
     ap.add_argument("--picard", type=int, default=2)
-    ap.add_argument("--serving-requests", type=int, default=48)
+    ap.add_argument("--serving-requests", type=int, default=128)
     ap.add_argument("--serving-slots", type=int, default=16)
     args = ap.parse_args()
     if args.gpus != 1:
@@ -270,7 +270,7 @@ def main():
         # one implicit step (assembly + BC + Krylov) to 1e-8 in FP64 and in FP32 + FP64 refinement
         for label, prec in (("fp64", cabi.FP64), ("fp32_ir", cabi.FP32)):
             gc = cases.GpuCase(ctx, case, precision=prec)
-            sp = cabi.solver_params(tol=1e-8, precision=prec, inner_tol=1e-4)
+            sp = cabi.solver_params(tol=1e-8, precision=prec)
             ctx.synchronize()
             t0 = time.perf_counter()
             st = gc.step(sp, raise_on_failure=False)

@@ -49,8 +49,8 @@ def ns_params(rho=1.0, dt=0.01, theta=1.0, use_specializations=True, supg_type=S
     return NSParams(rho, dt, theta, int(use_specializations), supg_type, 1.0, 16.0, 1.0, 1.0, 1.0, precision)
 
 
-def solver_params(method=BICGSTAB_JACOBI, tol=1e-10, max_iter=5000, restart=30, precision=FP64, inner_tol=1e-4,
-                  max_outer=30):
+def solver_params(method=BICGSTAB_JACOBI, tol=1e-10, max_iter=5000, restart=30, precision=FP64, inner_tol=1e-2,
+                  max_outer=0):
     return SolverParams(method, restart, max_iter, tol, precision, inner_tol, max_outer)
 
 

@@ -12,6 +12,7 @@
 #include <algorithm>
 #include <cmath>
 
+#include "arnoldi.hpp"
 #include "internal.hpp"
 
 namespace cf3hip {
@@ -116,140 +117,15 @@ __global__ __launch_bounds__(kWG) void k_ilu0_solve(const uint32_t* __restrict__
   }
 }
 
-// h[l] = V_l . w for l < nv (one workgroup per basis vector, fixed-order reduction)
-__global__ __launch_bounds__(256) void k_multidot(uint64_t n, const double* __restrict__ V, const double* __restrict__ w,
-                                                 double* __restrict__ h)
-{
-  const double* v = V + (uint64_t)blockIdx.x * n;
-  double s = 0.;
-  for(uint64_t i = threadIdx.x; i < n; i += 256) s += v[i] * w[i];
-  __shared__ double sh[4];
-  for(int off = 32; off > 0; off >>= 1) s += __shfl_down(s, off, 64);
-  if((threadIdx.x & 63) == 0) sh[threadIdx.x >> 6] = s;
-  __syncthreads();
-  if(threadIdx.x == 0) h[blockIdx.x] = sh[0] + sh[1] + sh[2] + sh[3];
-}
-
-// w -= sum_l c[l] V_l ; hacc[l] += c[l]
-__global__ void k_orth_update(uint64_t n, int nv, const double* __restrict__ V, const double* __restrict__ c,
-                              double* __restrict__ w)
-{
-  const uint64_t i = (uint64_t)blockIdx.x * blockDim.x + threadIdx.x;
-  if(i >= n) return;
-  double s = w[i];
-  for(int l = 0; l < nv; ++l) s -= c[l] * V[(uint64_t)l * n + i];
-  w[i] = s;
-}
-
-__global__ void k_scale(uint64_t n, const double* __restrict__ a, const double* __restrict__ s, double* __restrict__ b)
-{
-  const uint64_t i = (uint64_t)blockIdx.x * blockDim.x + threadIdx.x;
-  if(i < n) b[i] = a[i] / s[0];
-}
-
-__global__ void k_sub(uint64_t n, const double* __restrict__ b, double* __restrict__ w)
-{
-  const uint64_t i = (uint64_t)blockIdx.x * blockDim.x + threadIdx.x;
-  if(i < n) w[i] = b[i] - w[i];
-}
-
-__global__ void k_lincomb(uint64_t n, int nv, const double* __restrict__ V, const double* __restrict__ y,
-                          double* __restrict__ u)
-{
-  const uint64_t i = (uint64_t)blockIdx.x * blockDim.x + threadIdx.x;
-  if(i >= n) return;
-  double s = 0.;
-  for(int l = 0; l < nv; ++l) s += y[l] * V[(uint64_t)l * n + i];
-  u[i] = s;
-}
-
-__global__ void k_add(uint64_t n, const double* __restrict__ z, double* __restrict__ x)
-{
-  const uint64_t i = (uint64_t)blockIdx.x * blockDim.x + threadIdx.x;
-  if(i < n) x[i] += z[i];
-}
-
-// small dense bookkeeping of one Arnoldi step (single thread): H column j from the two
-// Gram-Schmidt passes, norm, Givens rotations, residual estimate
-// layout: [0] beta or |g_{j+1}|, [1] ||w||, cs[m], sn[m], g[m+1], H[(m+1)*m], h1[m+2], h2[m+2], y[m+2]
-struct GmresDev {
-  int m;
-  double* base;
-  __host__ __device__ double* cs() const { return base + 2; }
-  __host__ __device__ double* sn() const { return base + 2 + m; }
-  __host__ __device__ double* g() const { return base + 2 + 2 * m; }
-  __host__ __device__ double* H() const { return base + 3 + 3 * m; }
-  __host__ __device__ double* h1() const { return H() + (m + 1) * m; }
-  __host__ __device__ double* h2() const { return h1() + (m + 2); }
-  __host__ __device__ double* y() const { return h2() + (m + 2); }
-  static int size(int m) { return 3 + 3 * m + (m + 1) * m + 3 * (m + 2); }
-};
-
-__global__ void k_arnoldi_close(int m, double* base, int j, const double* __restrict__ h1, const double* __restrict__ h2,
-                                const double* __restrict__ wn2)
-{
-  GmresDev G{m, base};
-  double h[64];
-  for(int i = 0; i <= j; ++i) h[i] = h1[i] + h2[i];
-  const double hn = sqrt(wn2[0]);
-  h[j + 1] = hn;
-  base[1] = hn;
-  double* cs = G.cs();
-  double* sn = G.sn();
-  double* g = G.g();
-  for(int i = 0; i < j; ++i) {
-    const double t0 = cs[i] * h[i] + sn[i] * h[i + 1];
-    h[i + 1] = -sn[i] * h[i] + cs[i] * h[i + 1];
-    h[i] = t0;
-  }
-  const double den = sqrt(h[j] * h[j] + h[j + 1] * h[j + 1]);
-  cs[j] = h[j] / den;
-  sn[j] = h[j + 1] / den;
-  h[j] = den;
-  h[j + 1] = 0.;
-  g[j + 1] = -sn[j] * g[j];
-  g[j] = cs[j] * g[j];
-  for(int i = 0; i <= j; ++i) G.H()[i * m + j] = h[i];
-  base[0] = fabs(g[j + 1]);
-}
-
-__global__ void k_back_subst(int m, double* base, int j, double* __restrict__ y)
-{
-  GmresDev G{m, base};
-  for(int i = j - 1; i >= 0; --i) {
-    double s = G.g()[i];
-    for(int l = i + 1; l < j; ++l) s -= G.H()[i * m + l] * y[l];
-    y[i] = s / G.H()[i * m + i];
-  }
-}
-
-__global__ void k_gmres_start(int m, double* base, const double* __restrict__ beta2)
-{
-  GmresDev G{m, base};
-  for(int i = 0; i <= m; ++i) G.g()[i] = 0.;
-  G.g()[0] = sqrt(beta2[0]);
-  base[0] = G.g()[0];
-}
-
-__global__ void k_norm2_single(uint64_t n, const double* __restrict__ a, double* __restrict__ out)
-{
-  double s = 0.;
-  for(uint64_t i = threadIdx.x; i < n; i += 256) s += a[i] * a[i];
-  __shared__ double sh[4];
-  for(int off = 32; off > 0; off >>= 1) s += __shfl_down(s, off, 64);
-  if((threadIdx.x & 63) == 0) sh[threadIdx.x >> 6] = s;
-  __syncthreads();
-  if(threadIdx.x == 0) out[0] = sh[0] + sh[1] + sh[2] + sh[3];
-}
-
 }  // namespace
 
 void gmres_ilu0(Lss& L, const cf3hip_solver_params& prm, cf3hip_solve_stats& st)
 {
+  using namespace arnoldi;
   Context& c = *L.ctx;
   if(prm.precision != CF3HIP_FP64) CF3_THROW(CF3HIP_ERR_NOT_IMPLEMENTED, "GMRES+ILU0 runs in FP64");
   const int m = prm.restart;
-  if(m < 1 || m > 60) CF3_THROW(CF3HIP_ERR_INVALID_ARG, "GMRES restart must be in [1, 60]");
+  if(m < 1 || m > arnoldi::kMaxV - 1) CF3_THROW(CF3HIP_ERR_INVALID_ARG, "GMRES restart must be in [1, 32]");
   const uint64_t n = L.dofs;
   const uint64_t nnz = L.n_blocks * L.neq * L.neq;
   if(n >= 0xffffffffull || nnz >= 0xffffffffull) CF3_THROW(CF3HIP_ERR_LIMIT, "system too large for GMRES+ILU0");
@@ -300,7 +176,7 @@ void gmres_ilu0(Lss& L, const cf3hip_solver_params& prm, cf3hip_solve_stats& st)
   const double* b = L.rhs.as<double>();
   double* Vd = V.as<double>();
 
-  hipLaunchKernelGGL(k_norm2_single, dim3(1), dim3(256), 0, c.stream, n, b, sc + 1);
+  multidot(c, n, 1, b, b, sc + 1);
   double bn2 = 0.;
   HIP_OK(hipMemcpyAsync(&bn2, sc + 1, sizeof(double), hipMemcpyDeviceToHost, c.stream));
   HIP_OK(hipStreamSynchronize(c.stream));
@@ -313,7 +189,7 @@ void gmres_ilu0(Lss& L, const cf3hip_solver_params& prm, cf3hip_solve_stats& st)
     lss_spmv(L, 1., x, 0., w.as<double>());
     ++spmvs;
     hipLaunchKernelGGL(k_sub, dim3(gb), dim3(256), 0, c.stream, n, b, w.as<double>());
-    hipLaunchKernelGGL(k_norm2_single, dim3(1), dim3(256), 0, c.stream, n, w.as<double>(), sc);
+    multidot(c, n, 1, w.as<double>(), w.as<double>(), sc);
     hipLaunchKernelGGL(k_gmres_start, dim3(1), dim3(1), 0, c.stream, m, base, sc);
     double beta = 0.;
     HIP_OK(hipMemcpyAsync(&beta, base, sizeof(double), hipMemcpyDeviceToHost, c.stream));
@@ -326,11 +202,11 @@ void gmres_ilu0(Lss& L, const cf3hip_solver_params& prm, cf3hip_solve_stats& st)
       precond(Vd + (uint64_t)j * n, z.as<double>());
       lss_spmv(L, 1., z.as<double>(), 0., w.as<double>());
       ++spmvs;
-      hipLaunchKernelGGL(k_multidot, dim3(j + 1), dim3(256), 0, c.stream, n, Vd, w.as<double>(), h1);
+      multidot(c, n, j + 1, Vd, w.as<double>(), h1);
       hipLaunchKernelGGL(k_orth_update, dim3(gb), dim3(256), 0, c.stream, n, j + 1, Vd, h1, w.as<double>());
-      hipLaunchKernelGGL(k_multidot, dim3(j + 1), dim3(256), 0, c.stream, n, Vd, w.as<double>(), h2);
+      multidot(c, n, j + 1, Vd, w.as<double>(), h2);
       hipLaunchKernelGGL(k_orth_update, dim3(gb), dim3(256), 0, c.stream, n, j + 1, Vd, h2, w.as<double>());
-      hipLaunchKernelGGL(k_norm2_single, dim3(1), dim3(256), 0, c.stream, n, w.as<double>(), sc);
+      multidot(c, n, 1, w.as<double>(), w.as<double>(), sc);
       hipLaunchKernelGGL(k_arnoldi_close, dim3(1), dim3(1), 0, c.stream, m, base, j, h1, h2, sc);
       // V_{j+1} = w / ||w||  (||w|| kept on the device in base[1])
       hipLaunchKernelGGL(k_scale, dim3(gb), dim3(256), 0, c.stream, n, w.as<double>(), base + 1,

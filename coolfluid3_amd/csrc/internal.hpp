@@ -132,6 +132,7 @@ struct Lss {
   // Krylov workspace (allocated lazily)
   DevBuf work;          // generic vector workspace
   size_t work_vectors = 0;
+  DevBuf ir;            // FP64 FGMRES workspace of the precision form (basis, directions, bookkeeping)
   int work_real_bytes = 0;
   DevBuf ilu;           // ILU0 data (point CSR) for GMRES
   bool ilu_ready = false;
@@ -191,7 +192,7 @@ void lss_axpby(Context& c, size_t n, double alpha, const double* x, double beta,
 
 // krylov.hip
 void lss_solve(Lss& L, const cf3hip_solver_params& prm, cf3hip_solve_stats& st);
-void lss_solve_reliable(Lss& L, const cf3hip_solver_params& prm, cf3hip_solve_stats& st);
+void lss_solve_precision(Lss& L, const cf3hip_solver_params& prm, cf3hip_solve_stats& st);
 double lss_residual(Lss& L);
 
 // nodes.hip

@@ -14,6 +14,7 @@
 #include <algorithm>
 #include <cmath>
 
+#include "arnoldi.hpp"
 #include "internal.hpp"
 
 namespace cf3hip {
@@ -114,15 +115,16 @@ __global__ void k_dot2(uint64_t n, const R* __restrict__ a0, const R* __restrict
   block_reduce_store<2>(v, partials, gridDim.x);
 }
 
-template<typename R>
-__global__ void k_jacobi(uint32_t N, int neq, const uint32_t* __restrict__ diag_blk, const R* __restrict__ vals,
+// dinv = 1 / diag(A) in the vector type R from values of type VR
+template<typename R, typename VR>
+__global__ void k_jacobi(uint32_t N, int neq, const uint32_t* __restrict__ diag_blk, const VR* __restrict__ vals,
                          R* __restrict__ dinv)
 {
   const uint64_t r = (uint64_t)blockIdx.x * blockDim.x + threadIdx.x;
   if(r >= (uint64_t)N * neq) return;
   const uint32_t i = (uint32_t)(r / neq);
   const int a = (int)(r % neq);
-  dinv[r] = R(1) / vals[(uint64_t)diag_blk[i] * neq * neq + a * neq + a];
+  dinv[r] = R(1) / (R)vals[(uint64_t)diag_blk[i] * neq * neq + a * neq + a];
 }
 
 // r = b - r (r holds A x on entry), rh = r; partial ||r||^2 and ||b||^2
@@ -207,23 +209,23 @@ __global__ void k_convert_scaled(uint64_t n, const Src* __restrict__ a, Dst* __r
   for(uint64_t i = (uint64_t)blockIdx.x * blockDim.x + threadIdx.x; i < n; i += stride) b[i] = (Dst)((double)a[i] * scale);
 }
 
-__global__ void k_axpy_f32(uint64_t n, double a, const float* __restrict__ d, double* __restrict__ x)
-{
-  const uint64_t stride = (uint64_t)gridDim.x * blockDim.x;
-  for(uint64_t i = (uint64_t)blockIdx.x * blockDim.x + threadIdx.x; i < n; i += stride) x[i] += a * (double)d[i];
-}
-
-template<typename R>
-void spmv_t(Lss& L, R alpha, const R* x, R beta, R* y, const double* stop);
+// y = A x with vectors of type R and values of type VR (FP64, FP32, or FP32 values with FP64 vectors)
+template<typename R, typename VR>
+void spmv_t(Lss& L, const R* x, R* y, const double* stop);
 template<>
-void spmv_t<double>(Lss& L, double alpha, const double* x, double beta, double* y, const double* stop)
+void spmv_t<double, double>(Lss& L, const double* x, double* y, const double* stop)
 {
-  lss_spmv(L, alpha, x, beta, y, stop);
+  lss_spmv(L, 1., x, 0., y, stop);
 }
 template<>
-void spmv_t<float>(Lss& L, float alpha, const float* x, float beta, float* y, const double* stop)
+void spmv_t<float, float>(Lss& L, const float* x, float* y, const double* stop)
 {
-  lss_spmv32(L, alpha, x, beta, y, stop);
+  lss_spmv32(L, 1.f, x, 0.f, y, stop);
+}
+template<>
+void spmv_t<double, float>(Lss& L, const double* x, double* y, const double* stop)
+{
+  lss_spmv_mixed(L, x, y, stop);
 }
 
 struct Scalars {
@@ -237,9 +239,9 @@ void read_scalars(Context& c, Scalars& s)
   std::copy(c.scalars_host, c.scalars_host + S_COUNT, s.v);
 }
 
-// BiCGStab on A (values of type R) for A x = b; x is the initial guess and the result.
-// Returns the number of iterations; conv = recursive residual below tol*||b||.
-template<typename R>
+// BiCGStab on A (values of type VR, vectors of type R) for A x = b; x is the initial guess and the
+// result.  Returns the number of iterations; conv = recursive residual below tol*||b||.
+template<typename R, typename VR = R>
 int bicgstab(Lss& L, const R* b, R* x, R* work, double tol, int maxit, bool& conv, int& spmvs)
 {
   Context& c = *L.ctx;
@@ -256,10 +258,10 @@ int bicgstab(Lss& L, const R* b, R* x, R* work, double tol, int maxit, bool& con
   double* S = c.scalars.as<double>();
   double* part = c.red_partials.as<double>();
   const unsigned G = red_grid(n);
-  const R* vals = sizeof(R) == sizeof(double) ? (const R*)L.values.ptr : (const R*)L.values32.ptr;
-  hipLaunchKernelGGL(k_jacobi<R>, dim3(grid_for(n, 256)), dim3(256), 0, c.stream, L.n_nodes, L.neq,
+  const VR* vals = sizeof(VR) == sizeof(double) ? (const VR*)L.values.ptr : (const VR*)L.values32.ptr;
+  hipLaunchKernelGGL((k_jacobi<R, VR>), dim3(grid_for(n, 256)), dim3(256), 0, c.stream, L.n_nodes, L.neq,
                      L.diag_blk.as<uint32_t>(), vals, dinv);
-  spmv_t<R>(L, R(1), x, R(0), r, nullptr);
+  spmv_t<R, VR>(L, x, r, nullptr);
   hipLaunchKernelGGL(k_bicg_init<R>, dim3(G), dim3(kRT), 0, c.stream, n, b, r, rh, p, v, part);
   hipLaunchKernelGGL(k_finalize<2>, dim3(1), dim3(kRT), 0, c.stream, part, G, S, (int)S_RN2, (int)FIN_STORE);
   spmvs = 1;
@@ -278,13 +280,13 @@ int bicgstab(Lss& L, const R* b, R* x, R* work, double tol, int maxit, bool& con
     const int nb = std::min(batch, maxit - it);
     for(int k = 0; k < nb; ++k) {
       hipLaunchKernelGGL(k_bicg_p<R>, dim3(G), dim3(kRT), 0, c.stream, n, S, r, p, v, ph, dinv);
-      spmv_t<R>(L, R(1), ph, R(0), v, S + S_STOP);
+      spmv_t<R, VR>(L, ph, v, S + S_STOP);
       hipLaunchKernelGGL(k_dot2<R>, dim3(G), dim3(kRT), 0, c.stream, n, rh, v, (const R*)nullptr, (const R*)nullptr, part,
                          S + S_STOP);
       hipLaunchKernelGGL(k_finalize<1>, dim3(1), dim3(kRT), 0, c.stream, part, G, S, 0, (int)FIN_ALPHA);
       hipLaunchKernelGGL(k_bicg_s<R>, dim3(G), dim3(kRT), 0, c.stream, n, S, r, v, s, sh, dinv, part);
       hipLaunchKernelGGL(k_finalize<1>, dim3(1), dim3(kRT), 0, c.stream, part, G, S, 0, (int)FIN_SNORM);
-      spmv_t<R>(L, R(1), sh, R(0), t, S + S_STOP);
+      spmv_t<R, VR>(L, sh, t, S + S_STOP);
       hipLaunchKernelGGL(k_dot2<R>, dim3(G), dim3(kRT), 0, c.stream, n, t, s, t, t, part, S + S_STOP);
       hipLaunchKernelGGL(k_finalize<2>, dim3(1), dim3(kRT), 0, c.stream, part, G, S, 0, (int)FIN_OMEGA);
       hipLaunchKernelGGL(k_bicg_x<R>, dim3(G), dim3(kRT), 0, c.stream, n, S, x, ph, sh, r, s, t, rh, part);
@@ -367,9 +369,9 @@ void lss_solve(Lss& L, const cf3hip_solver_params& prm, cf3hip_solve_stats& st)
     st.converged = conv ? 1 : 0;
     return;
   }
-  // FP32 values with FP64 refinement of the stored FP64 system (reliable updates)
+  // FP32 values under FP64 refinement of the stored FP64 system
   (void)c;
-  lss_solve_reliable(L, prm, st);
+  lss_solve_precision(L, prm, st);
 }
 
 namespace {
@@ -383,26 +385,117 @@ __global__ void k_flush_increment(uint64_t n, double* __restrict__ xhi, double* 
   }
 }
 
-__global__ void k_jacobi_f32(uint32_t N, int neq, const uint32_t* __restrict__ diag_blk, const float* __restrict__ vals,
-                             double* __restrict__ dinv)
-{
-  const uint64_t r = (uint64_t)blockIdx.x * blockDim.x + threadIdx.x;
-  if(r >= (uint64_t)N * neq) return;
-  const uint32_t i = (uint32_t)(r / neq);
-  const int a = (int)(r % neq);
-  dinv[r] = 1. / (double)vals[(uint64_t)diag_blk[i] * neq * neq + a * neq + a];
-}
-
 __global__ void k_set_scalar(double* S, int idx, double v) { S[idx] = v; }
+
+constexpr int kMaxOuterBasis = 16;   // FGMRES restart length cap of the precision form
+
+// Precision form: flexible GMRES(m) in FP64 around an FP32 inner solve (GMRES-based iterative
+// refinement).  The outer iteration applies the FP64 operator through `resid` (r = b' - A' x, from the
+// stored FP64 values or matrix-free from the FP64 element matrices), so A' z_j is exact in FP64; each
+// preconditioning step z_j = M^-1 v_j is a Jacobi-BiCGStab solve of A32 z = v_j with FP32 values and
+// FP64 vectors from a zero guess to the relative tolerance inner_tol.  The outer Krylov space absorbs
+// the difference between the FP32 and FP64 operators, which plain refinement (or residual replacement
+// inside one BiCGStab) cannot when the solution is sensitive to the matrix.  Convergence is declared
+// on the FP64 true residual ||b' - A' x|| <= tol ||b'||.
+template<class Resid>
+void fgmres_ir(Lss& L, Resid resid, const cf3hip_solver_params& prm, cf3hip_solve_stats& st)
+{
+  using namespace arnoldi;
+  if(prm.method != CF3HIP_BICGSTAB_JACOBI)
+    CF3_THROW(CF3HIP_ERR_NOT_IMPLEMENTED, "the precision form's inner solver is BiCGStab + Jacobi");
+  if(!L.has_fp32) CF3_THROW(CF3HIP_ERR_NOT_CREATED, "FP32 values not enabled");
+  Context& c = *L.ctx;
+  const uint64_t n = L.dofs;
+  const int m = std::min(std::max(prm.restart, 1), kMaxOuterBasis);
+  const double itol = (prm.inner_tol > 0. && prm.inner_tol < 1.) ? prm.inner_tol : 1e-2;
+  // basis V (m+1), preconditioned directions Z (m), w, b', bookkeeping; the inner BiCGStab vectors in
+  // the generic workspace
+  const size_t need = sizeof(double) * ((size_t)(2 * m + 3) * n + GmresDev::size(m));
+  if(L.ir.bytes < need) L.ir.alloc(&c, need);
+  ensure_work(L, 9, sizeof(double));
+  double* V = L.ir.as<double>();
+  double* Z = V + (size_t)(m + 1) * n;
+  double* w = Z + (size_t)m * n;
+  double* b0 = w + n;
+  double* base = b0 + n;
+  const GmresDev gd{m, base};
+  double* sc = c.scalars.as<double>() + 40;   // [0] squared norms of the Arnoldi step
+  const unsigned gb = grid_for(n, 256);
+  double* x = L.sol.as<double>();
+  st = cf3hip_solve_stats{0, 0, 0, 0, 0.};
+  auto read1 = [&](const double* dev) {
+    double v = 0.;
+    HIP_OK(hipMemcpyAsync(&v, dev, sizeof(double), hipMemcpyDeviceToHost, c.stream));
+    HIP_OK(hipStreamSynchronize(c.stream));
+    return v;
+  };
+
+  // b' = r(0)
+  lss_fill(c, w, n, 0.);
+  resid(w, b0);
+  multidot(c, n, 1, b0, b0, sc);
+  double bn = std::sqrt(read1(sc));
+  if(bn == 0.) bn = 1.;
+  const double thr = prm.tol * bn;
+  int its = 0, outer = 0, spmvs = 0;
+  bool conv = false;
+  double rtrue = 0.;
+  for(;;) {
+    resid(x, w);
+    multidot(c, n, 1, w, w, sc);
+    hipLaunchKernelGGL(k_gmres_start, dim3(1), dim3(1), 0, c.stream, m, base, sc);
+    HIP_OK(hipGetLastError());
+    rtrue = read1(base);
+    if(rtrue <= thr) { conv = true; break; }
+    if(its >= prm.max_iter || (prm.max_outer > 0 && outer >= prm.max_outer) || !std::isfinite(rtrue)) break;
+    hipLaunchKernelGGL(k_scale, dim3(gb), dim3(256), 0, c.stream, n, w, base, V);
+    int j = 0;
+    for(; j < m; ++j) {
+      double* vj = V + (size_t)j * n;
+      double* zj = Z + (size_t)j * n;
+      lss_fill(c, zj, n, 0.);
+      bool cv = false;
+      int sp = 0;
+      its += bicgstab<double, float>(L, vj, zj, L.work.as<double>(), itol, std::max(1, prm.max_iter - its), cv, sp);
+      spmvs += sp;
+      ++outer;
+      resid(zj, w);   // A' z_j = b' - r(z_j)
+      hipLaunchKernelGGL(k_sub, dim3(gb), dim3(256), 0, c.stream, n, b0, w);
+      multidot(c, n, j + 1, V, w, gd.h1());
+      hipLaunchKernelGGL(k_orth_update, dim3(gb), dim3(256), 0, c.stream, n, j + 1, V, gd.h1(), w);
+      multidot(c, n, j + 1, V, w, gd.h2());
+      hipLaunchKernelGGL(k_orth_update, dim3(gb), dim3(256), 0, c.stream, n, j + 1, V, gd.h2(), w);
+      multidot(c, n, 1, w, w, sc);
+      hipLaunchKernelGGL(k_arnoldi_close, dim3(1), dim3(1), 0, c.stream, m, base, j, gd.h1(), gd.h2(), sc);
+      hipLaunchKernelGGL(k_scale, dim3(gb), dim3(256), 0, c.stream, n, w, base + 1, V + (size_t)(j + 1) * n);
+      HIP_OK(hipGetLastError());
+      const double est = read1(base);
+      if(!std::isfinite(est)) break;   // breakdown: keep the j columns before this one
+      if(est <= thr || its >= prm.max_iter || (prm.max_outer > 0 && outer >= prm.max_outer)) { ++j; break; }
+    }
+    if(j == 0) break;   // no usable direction
+    hipLaunchKernelGGL(k_back_subst, dim3(1), dim3(1), 0, c.stream, m, base, j, gd.y());
+    hipLaunchKernelGGL(k_add_lincomb, dim3(gb), dim3(256), 0, c.stream, n, j, Z, gd.y(), x);
+    HIP_OK(hipGetLastError());
+  }
+  st.iterations = its;
+  st.outer = outer;
+  st.spmvs = spmvs;
+  st.rel_residual = rtrue / bn;
+  st.converged = conv ? 1 : 0;
+}
 
 }  // namespace
 
-// Precision form: BiCGStab with Jacobi preconditioning on the FP32 values (FP32 stored, FP64 vectors and
-// FP64 accumulation in the SpMV), with reliable updates: the FP64 solution accumulates the increments
-// and, whenever the recursive residual has dropped by the factor delta (or meets the tolerance), the
-// residual is replaced by the true FP64 residual b - A x given by `resid`.  The search directions are
-// kept across replacements (no restart), so the iteration count stays close to that of the FP64
-// method.  Convergence is declared on the FP64 true residual only.
+// Precision form, first phase: BiCGStab with Jacobi preconditioning on the FP32 values (FP32 stored,
+// FP64 vectors and FP64 accumulation in the SpMV), with reliable updates: the FP64 solution accumulates
+// the increments and, whenever the recursive residual has dropped by the factor delta (or meets the
+// tolerance), the residual is replaced by the true FP64 residual b - A x given by `resid`.  The search
+// directions are kept across replacements (no restart), so on most systems the iteration count stays
+// close to that of the FP64 method.  When the FP32 operator is too far from the FP64 one for that
+// (a replacement raises the true residual, or kSwitchStall replacements in a row fail to halve it), the
+// solve continues from the current FP64 iterate with fgmres_ir, which converges in that regime.
+// Convergence is declared on the FP64 true residual only.
 template<class Resid>
 void reliable_bicgstab(Lss& L, Resid resid, const cf3hip_solver_params& prm, cf3hip_solve_stats& st)
 {
@@ -437,7 +530,7 @@ void reliable_bicgstab(Lss& L, Resid resid, const cf3hip_solver_params& prm, cf3
   resid(xhi, r);
   int updates = 1;
   double rtrue = norm2(c, r, n);
-  hipLaunchKernelGGL(k_jacobi_f32, dim3(grid_for(n, 256)), dim3(256), 0, c.stream, L.n_nodes, L.neq,
+  hipLaunchKernelGGL((k_jacobi<double, float>), dim3(grid_for(n, 256)), dim3(256), 0, c.stream, L.n_nodes, L.neq,
                      L.diag_blk.as<uint32_t>(), L.values32.as<float>(), dinv);
   lss_copy(c, rh, r, n);
   lss_fill(c, p, n, 0.);
@@ -448,14 +541,10 @@ void reliable_bicgstab(Lss& L, Resid resid, const cf3hip_solver_params& prm, cf3
   hipLaunchKernelGGL(k_finalize<1>, dim3(1), dim3(kRT), 0, c.stream, part, G, S, (int)S_RHO, (int)FIN_STORE);
   HIP_OK(hipGetLastError());
   bool conv = rtrue * rtrue <= thr2;
-  // stagnation guard: the FP32 operator may be too far from the FP64 one for the recurrences to
-  // reduce the true residual (a system whose solution is very sensitive to its matrix).  The solve
-  // stops, unconverged, when kMaxStall replacements in a row fail to halve the true residual of the
-  // last replacement that did, or when it grows beyond kDiverge times that value.
-  constexpr int kMaxStall = 8;
-  constexpr double kDiverge = 1e4;
-  double rprog = rtrue;
+  constexpr int kSwitchStall = 2;
+  double rprog = rtrue, rprev = rtrue;
   int stall = 0;
+  bool switched = false;
   hipLaunchKernelGGL(k_set_scalar, dim3(1), dim3(1), 0, c.stream, S, (int)S_MAXR, rtrue);
   hipLaunchKernelGGL(k_set_scalar, dim3(1), dim3(1), 0, c.stream, S, (int)S_DELTA, delta);
   int it = 0, it_base = 0, batch = 1;
@@ -491,13 +580,14 @@ void reliable_bicgstab(Lss& L, Resid resid, const cf3hip_solver_params& prm, cf3
     ++updates;
     rtrue = norm2(c, r, n);
     if(rtrue * rtrue <= thr2) { conv = true; break; }
-    if(!std::isfinite(rtrue) || rtrue > kDiverge * rprog) break;
     if(rtrue < 0.5 * rprog) {
       rprog = rtrue;
       stall = 0;
-    } else if(++stall >= kMaxStall) {
-      break;
+    } else {
+      ++stall;
     }
+    if(!std::isfinite(rtrue) || rtrue > rprev || stall >= kSwitchStall) { switched = true; break; }
+    rprev = rtrue;
     if(breakdown) {   // restart the recurrences from the true residual
       lss_copy(c, rh, r, n);
       lss_fill(c, p, n, 0.);
@@ -516,6 +606,19 @@ void reliable_bicgstab(Lss& L, Resid resid, const cf3hip_solver_params& prm, cf3
     batch = 1;
   }
   const int spmvs = 2 * it;
+  if(switched) {
+    if(!std::isfinite(rtrue)) lss_fill(c, xhi, n, 0.);
+    cf3hip_solver_params p2 = prm;
+    p2.max_iter = std::max(1, prm.max_iter - it);
+    cf3hip_solve_stats s2;
+    fgmres_ir(L, resid, p2, s2);
+    st.iterations = it + s2.iterations;
+    st.outer = updates + s2.outer;
+    st.spmvs = spmvs + s2.spmvs;
+    st.rel_residual = s2.rel_residual;
+    st.converged = s2.converged;
+    return;
+  }
   if(!conv) {
     hipLaunchKernelGGL(k_flush_increment, dim3(std::min<unsigned>(grid_for(n, 256), 8192)), dim3(256), 0, c.stream, n,
                        xhi, xlo);
@@ -542,7 +645,7 @@ void ns_solve(Lss& L, Fields& F, const cf3hip_ns_params& nsp, const cf3hip_solve
 }
 
 // precision form on a stored system: the true residual uses the stored FP64 values and rhs
-void lss_solve_reliable(Lss& L, const cf3hip_solver_params& prm, cf3hip_solve_stats& st)
+void lss_solve_precision(Lss& L, const cf3hip_solver_params& prm, cf3hip_solve_stats& st)
 {
   Context& c = *L.ctx;
   reliable_bicgstab(L, [&](const double* x, double* r) {

@@ -190,8 +190,8 @@ public:
     params.max_iter = 5000;
     params.tol = 1e-10;
     params.precision = CF3HIP_FP64;
-    params.inner_tol = 1e-4;
-    params.max_outer = 30;
+    params.inner_tol = 1e-2;
+    params.max_outer = 0;
   }
   cf3hip_solver_params params;
   cf3hip_solve_stats stats = {};

@@ -3,8 +3,9 @@
 Each request is a unit-cube tetrahedral channel case (cells per side log-uniform in [12, 64], Re
 uniform in [10, 1000], inflow perturbation amplitude uniform in [0, 0.1], all drawn from one SplitMix64
 stream; inflow on x = 1, outlet p = 0 on x = 0, no-slip on the side faces) that runs `steps` implicit
-steps (dt = 1e-2) in the precision form (FP32 element matrices, FP64 matrix-free residual,
-reliable-update BiCGStab to 1e-8; a step whose solve stagnates is redone in FP64 and counted).
+steps (dt = 1e-2) in the precision form (FP32 element matrices, FP32 BiCGStab inner solves under FP64
+flexible GMRES on the matrix-free FP64 operator, to 1e-8; a step whose solve fails is redone in FP64
+and counted).
 
 Continuous batching (`Server.run(..., batched=True)`):
   * a request is admitted when it has arrived, a slot is free and its device memory fits the HBM

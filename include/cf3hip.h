@@ -180,13 +180,15 @@ typedef struct cf3hip_solver_params {
   int restart;       /* GMRES restart length */
   int max_iter;
   double tol;        /* stop on ||b - A x||_2 <= tol ||b||_2 */
-  int precision;     /* CF3HIP_FP64, or CF3HIP_FP32 = FP32 inner Krylov + FP64 iterative refinement */
-  double inner_tol;  /* FP32 inner relative tolerance per refinement sweep */
-  int max_outer;     /* refinement sweeps */
+  int precision;     /* CF3HIP_FP64, or CF3HIP_FP32 = FP32-matrix BiCGStab with FP64 residual
+                        replacement, then FP64 flexible GMRES around FP32 inner solves if that stalls
+                        (restart length min(restart, 16)) */
+  double inner_tol;  /* relative tolerance of each FP32 inner solve of the GMRES phase (0: 1e-2) */
+  int max_outer;     /* cap on the FP32 inner solves of the GMRES phase (0: no cap) */
 } cf3hip_solver_params;
 typedef struct cf3hip_solve_stats {
-  int iterations;    /* Krylov iterations (all sweeps) */
-  int outer;         /* refinement sweeps (1 for FP64) */
+  int iterations;    /* BiCGStab iterations (FP32 form: both phases) */
+  int outer;         /* FP32 form: residual replacements + inner solves (1 for FP64) */
   int spmvs;
   int converged;
   double rel_residual;  /* ||b - A x|| / ||b|| in FP64 at exit */
@@ -238,13 +240,15 @@ int cf3hip_ns_assemble(cf3hip_lss lss, cf3hip_fields fields, const cf3hip_ns_par
 /* UpdateActions (NavierStokes.cpp:172-180): u3=u2, u2=u1, u1=u, u += du, p += dp from the solution */
 int cf3hip_ns_update(cf3hip_lss lss, cf3hip_fields fields);
 /* SolveLSS of the NS system (SolveLSS.cpp:44-55).  precision FP64: the stored FP64 system (as
- * cf3hip_lss_solve).  precision FP32: BiCGStab on the FP32 element matrices (FP32 values, FP64
- * vectors and accumulation) with reliable updates against the FP64 residual b' - A' x evaluated
- * matrix-free from the FP64 element matrices of the same fields, after Dirichlet conditions applied
- * with cf3hip_lss_dirichlet; needs only the FP32 assembly of the step.  The FP32 form stops early,
- * unconverged (CF3HIP_ERR_NOT_CONVERGED, stats filled), when 8 residual replacements in a row fail
- * to halve the FP64 residual, or it grows 1e4-fold: its FP32 operator is then too far from the FP64
- * one for that system, and the caller re-assembles in FP64 and solves with precision FP64. */
+ * cf3hip_lss_solve).  precision FP32: the FP64 operator is the residual b' - A' x evaluated
+ * matrix-free from the FP64 element matrices of the same fields (after the Dirichlet conditions
+ * applied with cf3hip_lss_dirichlet); only the FP32 assembly of the step is needed.  BiCGStab +
+ * Jacobi on the FP32 element matrices (FP64 vectors) whose residual is replaced by that FP64
+ * residual after every 100-fold drop; if a replacement raises the FP64 residual or two in a row fail
+ * to halve it, the solve continues with flexible GMRES in FP64 on the FP64 operator, preconditioned
+ * by BiCGStab solves on the FP32 values (FP64 vectors) to inner_tol.  Converged when the FP64 residual
+ * meets tol.  stats: iterations = BiCGStab iterations of both phases, outer = residual replacements
+ * + inner solves. */
 int cf3hip_ns_solve(cf3hip_lss lss, cf3hip_fields fields, const cf3hip_ns_params* ns_prm,
                     const cf3hip_solver_params* prm, cf3hip_solve_stats* stats);
 /* ---- serving: batched assembly (no reference counterpart; the serving form of cf3hip) ----
