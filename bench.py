@@ -167,18 +167,22 @@ def cavity2d_workload(ctx):
             "steps": steps}
 
 
-def serving_workload(count, max_active):
-    """Continuous batching against one-request-at-a-time serving on the same request stream."""
+def serving_workload(count, max_active, mode="both"):
+    """Continuous batching against one-request-at-a-time serving on the same request stream
+    (mode: both, sequential or batched)."""
     from coolfluid3_amd import serving
     reqs = serving.make_requests(count)
     srv = serving.Server(max_active=max_active, log=lambda m: print("serving: " + m, file=sys.stderr, flush=True))
-    _, seq = srv.run(reqs, batched=False)
-    _, bat = srv.run(reqs, batched=True)
-    return {"workload": "%d requests, Poisson arrivals 200 req/s, cube tets with cells/side log-uniform in [12,64], "
-                        "Re U[10,1000], inflow amplitude U[0,0.1], 10 steps dt=1e-2 each, FP32 + FP64-refined "
-                        "to 1e-8, HBM budget 200 GB" % count,
-            "sequential": seq, "continuous_batching": bat,
-            "throughput_ratio": bat["throughput_req_s"] / seq["throughput_req_s"]}
+    out = {"workload": "%d requests, Poisson arrivals 200 req/s, cube tet channels with cells/side log-uniform in "
+                       "[12,64], Re U[10,1000], inflow amplitude U[0,0.1], 10 steps dt=1e-2 each, FP32 element "
+                       "matrices with FP64 residual to 1e-8, HBM budget 200 GB" % count}
+    if mode in ("both", "sequential"):
+        out["sequential"] = srv.run(reqs, batched=False)[1]
+    if mode in ("both", "batched"):
+        out["continuous_batching"] = srv.run(reqs, batched=True)[1]
+    if mode == "both":
+        out["throughput_ratio"] = out["continuous_batching"]["throughput_req_s"] / out["sequential"]["throughput_req_s"]
+    return out
 
 
 def main():
@@ -197,6 +201,7 @@ def main():
     ap.add_argument("--picard", type=int, default=2)
     ap.add_argument("--serving-requests", type=int, default=128)
     ap.add_argument("--serving-slots", type=int, default=16)
+    ap.add_argument("--serving-mode", default="both", choices=("both", "sequential", "batched"))
     args = ap.parse_args()
     if args.gpus != 1:
         print("bench.py: this benchmark measures one GPU only; multi-GPU runs are not part of it (use --gpus 1)",
@@ -305,7 +310,7 @@ def main():
         for key, fn in (("cavity_2d", lambda: cavity2d_workload(ctx)),
                         ("precision_tets", lambda: precision_workload(ctx, args.precision_cells)),
                         ("hexa_cavity", lambda: hexa_workload(ctx, args.hexa_cells, args.picard)),
-                        ("serving", lambda: serving_workload(args.serving_requests, args.serving_slots))):
+                        ("serving", lambda: serving_workload(args.serving_requests, args.serving_slots, args.serving_mode))):
             if key not in args.workloads.split(","):
                 continue
             t0 = time.perf_counter()

@@ -130,9 +130,11 @@ struct SpecP1 {
     R V = R(0), invV, tsu, tbu;
     bool fp64_geom = sizeof(R) == sizeof(double);
     if constexpr(sizeof(R) != sizeof(double)) {
+      // coordinates relative to node 0, formed in FP64: the FP32 differences of absolute coordinates
+      // would lose log2(1/h) bits (the normals and the volume are translation invariant)
       R Xr[K][D];
       for(int m = 0; m < K; ++m)
-        for(int d = 0; d < D; ++d) Xr[m][d] = (R)X[m][d];
+        for(int d = 0; d < D; ++d) Xr[m][d] = (R)(X[m][d] - X[0][d]);
       geometry<R>(Xr, n, V);
       // strongly skewed elements: geometry and tau in FP64.  Criterion from the face normals already
       // formed, S = sum |n_i|^2: tetrahedra V^2 < 1.4e-3 S^(3/2) (regular Kuhn tetrahedron 1.5e-2;

@@ -66,6 +66,7 @@ struct Context {
   // scratch for reductions: partial sums and device scalars
   DevBuf red_partials;   // double[kMaxRedBlocks * 8]
   DevBuf scalars;        // double[64]; the last slot holds the element-validity flag
+  DevBuf fused_partials; // partial sums of the fused Krylov iteration (allocated on first use)
   double* scalars_host = nullptr;   // pinned mirror
   hipStream_t own_stream = nullptr;  // created by cf3hip_context_create_stream, destroyed with the context
   int* status_flag() { return reinterpret_cast<int*>(scalars.as<double>() + 63); }

@@ -16,6 +16,7 @@
 
 #include "arnoldi.hpp"
 #include "internal.hpp"
+#include "spmv4.hpp"
 
 namespace cf3hip {
 
@@ -99,6 +100,159 @@ __global__ void k_finalize(const double* __restrict__ partials, unsigned G, doub
       else S[S_MAXR] = fmax(S[S_MAXR], rn);
     }
   }
+}
+
+// ---- fused iteration (FP64 vectors, neq = 4): five launches, no one-workgroup finalisations ----
+// Each kernel that needs a scalar reduces the partial sums of the kernel that produced them itself, in
+// every workgroup, in the fixed order of k_finalize (so all workgroups obtain the same bits); workgroup
+// 0 records the scalars in S for the host and for later kernels.  A kernel never reads a slot of S
+// that a workgroup of the same launch writes, except the stop flag, whose value every workgroup also
+// derives itself.  rho alternates between two slots by iteration parity.
+enum { S_RHO_EVEN = S_RHO, S_RHO_ODD = S_RHO_PREV };
+
+// sum of G partials per dot product (partials[d * G + i]), result in every thread of the workgroup
+template<int NDOT>
+__device__ inline void workgroup_sum(const double* __restrict__ partials, unsigned G, double (&out)[NDOT])
+{
+  __shared__ double sh[NDOT][kRT / 64];
+  for(int d = 0; d < NDOT; ++d) {
+    double s = 0.;
+    for(unsigned i = threadIdx.x; i < G; i += kRT) s += partials[d * G + i];
+    for(int off = 32; off > 0; off >>= 1) s += __shfl_down(s, off, 64);
+    if((threadIdx.x & 63) == 0) sh[d][threadIdx.x >> 6] = s;
+  }
+  __syncthreads();
+  for(int d = 0; d < NDOT; ++d) {
+    double s = 0.;
+    for(int w = 0; w < kRT / 64; ++w) s += sh[d][w];
+    out[d] = s;
+  }
+}
+
+__device__ inline bool leader() { return blockIdx.x == 0 && threadIdx.x == 0; }
+
+// Start of iteration `it`: finalise iteration it-1 from the partials px = (||r||^2, rh . r) -- count,
+// convergence / breakdown test (rel: also the residual-replacement test of the reliable phase) -- then,
+// unless stopped or fin_only, p = r + beta (p - omega v), ph = D^-1 p.
+template<typename R>
+__global__ void k_fb_p(uint64_t n, double* __restrict__ S, int it, int rel, int fin_only, unsigned Gx,
+                       const double* __restrict__ px, const R* __restrict__ r, R* __restrict__ p,
+                       const R* __restrict__ v, R* __restrict__ ph, const R* __restrict__ dinv)
+{
+  if(S[S_STOP] != 0.) return;
+  double red[2];
+  workgroup_sum<2>(px, Gx, red);
+  const double rn2 = red[0], rho = red[1];
+  const bool conv = S[S_SCONV] != 0. || rn2 <= S[S_THR2];
+  bool stop = conv || !isfinite(rn2) || rho == 0. || S[S_OMEGA] == 0.;
+  if(rel) stop = stop || sqrt(rn2) < S[S_DELTA] * S[S_MAXR];
+  if(leader()) {
+    S[S_RN2] = rn2;
+    S[(it & 1) ? S_RHO_ODD : S_RHO_EVEN] = rho;
+    S[S_ITER] = (double)it;
+    if(stop) S[S_STOP] = 1.;
+    else if(rel) S[S_MAXR] = fmax(S[S_MAXR], sqrt(rn2));
+  }
+  if(stop || fin_only) return;
+  const double rho_old = S[(it & 1) ? S_RHO_EVEN : S_RHO_ODD];
+  const R beta = (R)((rho / rho_old) * (S[S_ALPHA] / S[S_OMEGA]));
+  const R omega = (R)S[S_OMEGA];
+  const uint64_t stride = (uint64_t)gridDim.x * blockDim.x;
+  for(uint64_t i = (uint64_t)blockIdx.x * blockDim.x + threadIdx.x; i < n; i += stride) {
+    const R pi = r[i] + beta * (p[i] - omega * v[i]);
+    p[i] = pi;
+    ph[i] = dinv[i] * pi;
+  }
+}
+
+// y = A x (neq = 4, values VR, FP64 vectors) with the partial dot products of the BiCGStab step:
+// NDOT = 1: d0 . y (d0 = rh); NDOT = 2: d0 . y and y . y (d0 = s).  Fixed grid, rows in a
+// grid-stride loop (16 lanes per node row).
+template<typename VR, int NDOT>
+__global__ __launch_bounds__(kRT) void k_spmv4_dot(uint32_t N, const uint32_t* __restrict__ row_ptr,
+                                                   const uint32_t* __restrict__ cols, const VR* __restrict__ vals,
+                                                   const double* __restrict__ x, double* __restrict__ y,
+                                                   const double* __restrict__ d0, const double* __restrict__ S,
+                                                   double* __restrict__ partials)
+{
+  if(S[S_STOP] != 0.) return;
+  double acc[NDOT];
+  for(int d = 0; d < NDOT; ++d) acc[d] = 0.;
+  const int l = threadIdx.x & 15;
+  const uint32_t step = gridDim.x * (kRT / 16);
+  for(uint32_t row = blockIdx.x * (kRT / 16) + (threadIdx.x >> 4); row < N; row += step) {
+    int a;
+    bool out;
+    const double yv = spmv4::row<VR>(row_ptr, cols, vals, x, row, l, a, out);
+    if(out) {
+      const uint64_t o = (uint64_t)row * 4 + a;
+      y[o] = yv;
+      acc[0] += d0[o] * yv;
+      if constexpr(NDOT > 1) acc[NDOT - 1] += yv * yv;
+    }
+  }
+  block_reduce_store<NDOT>(acc, partials, gridDim.x);
+}
+
+// alpha = rho / (rh . v) from the partials pv; s = r - alpha v, sh = D^-1 s, partials ||s||^2
+template<typename R>
+__global__ void k_fb_s(uint64_t n, double* __restrict__ S, int it, unsigned Gv, const double* __restrict__ pv,
+                       const R* __restrict__ r, const R* __restrict__ v, R* __restrict__ s, R* __restrict__ sh,
+                       const R* __restrict__ dinv, double* __restrict__ ps)
+{
+  if(S[S_STOP] != 0.) return;
+  double red[1];
+  workgroup_sum<1>(pv, Gv, red);
+  const double alpha_d = S[(it & 1) ? S_RHO_ODD : S_RHO_EVEN] / red[0];
+  if(leader()) {
+    S[S_RHV] = red[0];
+    S[S_ALPHA] = alpha_d;
+  }
+  const R alpha = (R)alpha_d;
+  double acc[1] = {0.};
+  const uint64_t stride = (uint64_t)gridDim.x * kRT;
+  for(uint64_t i = (uint64_t)blockIdx.x * kRT + threadIdx.x; i < n; i += stride) {
+    const R si = r[i] - alpha * v[i];
+    s[i] = si;
+    sh[i] = dinv[i] * si;
+    acc[0] += (double)si * (double)si;
+  }
+  block_reduce_store<1>(acc, ps, gridDim.x);
+}
+
+// omega = (t . s) / (t . t) (0 once ||s|| meets the tolerance) from the partials ps, pt;
+// x += alpha ph + omega sh, r = s - omega t, partials (||r||^2, rh . r)
+template<typename R>
+__global__ void k_fb_x(uint64_t n, double* __restrict__ S, unsigned Gs, const double* __restrict__ ps, unsigned Gt,
+                       const double* __restrict__ pt, R* __restrict__ x, const R* __restrict__ ph,
+                       const R* __restrict__ sh, R* __restrict__ r, const R* __restrict__ s, const R* __restrict__ t,
+                       const R* __restrict__ rh, double* __restrict__ px)
+{
+  if(S[S_STOP] != 0.) return;
+  double rs[1], rt[2];
+  workgroup_sum<1>(ps, Gs, rs);
+  __syncthreads();
+  workgroup_sum<2>(pt, Gt, rt);
+  const bool sconv = rs[0] <= S[S_THR2];
+  const double omega_d = sconv ? 0. : rt[0] / rt[1];
+  if(leader()) {
+    S[S_SN2] = rs[0];
+    S[S_SCONV] = sconv ? 1. : 0.;
+    S[S_TS] = rt[0];
+    S[S_TT] = rt[1];
+    S[S_OMEGA] = omega_d;
+  }
+  const R alpha = (R)S[S_ALPHA], omega = (R)omega_d;
+  double acc[2] = {0., 0.};
+  const uint64_t stride = (uint64_t)gridDim.x * kRT;
+  for(uint64_t i = (uint64_t)blockIdx.x * kRT + threadIdx.x; i < n; i += stride) {
+    x[i] += alpha * ph[i] + omega * sh[i];
+    const R ri = s[i] - omega * t[i];
+    r[i] = ri;
+    acc[0] += (double)ri * (double)ri;
+    acc[1] += (double)rh[i] * (double)ri;
+  }
+  block_reduce_store<2>(acc, px, gridDim.x);
 }
 
 template<typename R>
@@ -194,13 +348,19 @@ __global__ void k_bicg_x(uint64_t n, const double* __restrict__ S, R* __restrict
   block_reduce_store<2>(acc, partials, gridDim.x);
 }
 
-__global__ void k_set_scalars(double* S, double thr2)
+__global__ void k_set_scalar(double* S, int idx, double v) { S[idx] = v; }
+
+// fresh recurrence (p = v = 0) at iteration it0
+__global__ void k_set_scalars(double* S, double thr2, int it0)
 {
-  S[S_RHO_PREV] = 1.; S[S_ALPHA] = 1.; S[S_OMEGA] = 1.; S[S_THR2] = thr2; S[S_SCONV] = 0.;
-  S[S_STOP] = 0.; S[S_ITER] = 0.;
+  S[S_RHO] = 1.; S[S_RHO_PREV] = 1.; S[S_ALPHA] = 1.; S[S_OMEGA] = 1.; S[S_THR2] = thr2; S[S_SCONV] = 0.;
+  S[S_STOP] = 0.; S[S_ITER] = (double)it0;
 }
 
 constexpr int kMaxBatch = 16;   // iterations enqueued between two host reads of the scalars
+constexpr unsigned kFusedSpmvBlocks = 2048;   // workgroups of the fused products (8 per CU)
+// partials of the fused iteration: (||r||^2, rh . r), rh . v, ||s||^2, (t . s, t . t)
+constexpr size_t kFusedPartials = 2 * kMaxRedBlocks + kFusedSpmvBlocks + kMaxRedBlocks + 2 * kFusedSpmvBlocks;
 
 template<typename Src, typename Dst>
 __global__ void k_convert_scaled(uint64_t n, const Src* __restrict__ a, Dst* __restrict__ b, double scale)
@@ -239,50 +399,101 @@ void read_scalars(Context& c, Scalars& s)
   std::copy(c.scalars_host, c.scalars_host + S_COUNT, s.v);
 }
 
-// BiCGStab on A (values of type VR, vectors of type R) for A x = b; x is the initial guess and the
-// result.  Returns the number of iterations; conv = recursive residual below tol*||b||.
-template<typename R, typename VR = R>
-int bicgstab(Lss& L, const R* b, R* x, R* work, double tol, int maxit, bool& conv, int& spmvs)
-{
-  Context& c = *L.ctx;
-  const uint64_t n = L.dofs;
-  R* r = work;
-  R* rh = r + n;
-  R* p = rh + n;
-  R* v = p + n;
-  R* ph = v + n;
-  R* s = ph + n;
-  R* sh = s + n;
-  R* t = sh + n;
-  R* dinv = t + n;
-  double* S = c.scalars.as<double>();
-  double* part = c.red_partials.as<double>();
-  const unsigned G = red_grid(n);
-  const VR* vals = sizeof(VR) == sizeof(double) ? (const VR*)L.values.ptr : (const VR*)L.values32.ptr;
-  hipLaunchKernelGGL((k_jacobi<R, VR>), dim3(grid_for(n, 256)), dim3(256), 0, c.stream, L.n_nodes, L.neq,
-                     L.diag_blk.as<uint32_t>(), vals, dinv);
-  spmv_t<R, VR>(L, x, r, nullptr);
-  hipLaunchKernelGGL(k_bicg_init<R>, dim3(G), dim3(kRT), 0, c.stream, n, b, r, rh, p, v, part);
-  hipLaunchKernelGGL(k_finalize<2>, dim3(1), dim3(kRT), 0, c.stream, part, G, S, (int)S_RN2, (int)FIN_STORE);
-  spmvs = 1;
-  Scalars sc;
-  read_scalars(c, sc);
-  double bn2 = sc.v[S_RN2 + 1];
-  if(bn2 == 0.) bn2 = 1.;
-  const double thr2 = tol * tol * bn2;
-  // S_RHO <- rh.r = ||r||^2 initially
-  hipLaunchKernelGGL(k_set_scalars, dim3(1), dim3(1), 0, c.stream, S, thr2);
-  hipLaunchKernelGGL(k_finalize<1>, dim3(1), dim3(kRT), 0, c.stream, part, G, S, (int)S_RHO, (int)FIN_STORE);
-  HIP_OK(hipGetLastError());
-  conv = sc.v[S_RN2] <= thr2;
-  int it = 0, batch = 1;
-  while(!conv && it < maxit) {
-    const int nb = std::min(batch, maxit - it);
-    for(int k = 0; k < nb; ++k) {
+// Device state of one BiCGStab recurrence (Jacobi right preconditioning, dinv = D^-1) and the launches
+// that advance it; the scalars stay on the device and the host reads them once per batch.  With FP64
+// vectors and neq = 4 an iteration is five launches (dot products fused into the products, each
+// reduction completed by the kernel that consumes it); otherwise eleven (separate dot products and
+// one-workgroup finalisations).  Both sum in a fixed order: the results are deterministic.  rel selects
+// the residual-replacement stop test of the reliable phase.
+template<typename R, typename VR>
+struct Recurrence {
+  Lss& L;
+  Context& c;
+  uint64_t n;
+  double* S;
+  R *x, *r, *rh, *p, *v, *ph, *s, *sh, *t;
+  const R* dinv;
+  int rel;
+  bool fused;
+  unsigned Gv, Gs;
+  double *part, *px, *pv, *ps, *pt;
+  int next = 0;   // index of the next iteration to enqueue
+
+  Recurrence(Lss& L_, R* x_, R* work, int rel_)
+      : L(L_), c(*L_.ctx), n(L_.dofs), S(L_.ctx->scalars.as<double>()), x(x_), rel(rel_)
+  {
+    r = work; rh = r + n; p = rh + n; v = p + n; ph = v + n; s = ph + n; sh = s + n; t = sh + n;
+    dinv = t + n;
+    fused = sizeof(R) == sizeof(double) && L.neq == 4;
+    Gv = red_grid(n);
+    Gs = std::min<unsigned>(kFusedSpmvBlocks, grid_for(L.n_nodes, kRT / 16));
+    part = c.red_partials.as<double>();
+    if(fused) {
+      if(!c.fused_partials.ptr) c.fused_partials.alloc(&c, sizeof(double) * kFusedPartials);
+      px = c.fused_partials.as<double>();
+      pv = px + 2 * kMaxRedBlocks;
+      ps = pv + kFusedSpmvBlocks;
+      pt = ps + kMaxRedBlocks;
+    }
+  }
+  const VR* vals() const { return sizeof(VR) == sizeof(double) ? (const VR*)L.values.ptr : (const VR*)L.values32.ptr; }
+  double rho(const Scalars& sc, int it) const
+  {
+    return fused ? sc.v[(it & 1) ? S_RHO_ODD : S_RHO_EVEN] : sc.v[S_RHO];
+  }
+
+  // r holds a new residual.  restart: fresh recurrence from iteration it0 (rh = r, p = v = 0, scalars
+  // reset); otherwise a residual replacement that keeps the search directions and scalars.
+  void seed(double thr2, bool restart, int it0)
+  {
+    if(restart) {
+      if(rh != r) HIP_OK(hipMemcpyAsync(rh, r, sizeof(R) * n, hipMemcpyDeviceToDevice, c.stream));
+      fill(p, 0.);
+      fill(v, 0.);
+      hipLaunchKernelGGL(k_set_scalars, dim3(1), dim3(1), 0, c.stream, S, thr2, it0);
+    } else {
+      hipLaunchKernelGGL(k_set_scalar, dim3(1), dim3(1), 0, c.stream, S, (int)S_SCONV, 0.);
+      hipLaunchKernelGGL(k_set_scalar, dim3(1), dim3(1), 0, c.stream, S, (int)S_STOP, 0.);
+    }
+    if(fused) {
+      hipLaunchKernelGGL(k_dot2<R>, dim3(Gv), dim3(kRT), 0, c.stream, n, r, r, rh, r, px, (const double*)nullptr);
+    } else {
+      hipLaunchKernelGGL(k_dot2<R>, dim3(Gv), dim3(kRT), 0, c.stream, n, rh, r, (const R*)nullptr, (const R*)nullptr,
+                         part, (const double*)nullptr);
+      hipLaunchKernelGGL(k_finalize<1>, dim3(1), dim3(kRT), 0, c.stream, part, Gv, S, (int)S_RHO, (int)FIN_STORE);
+    }
+    HIP_OK(hipGetLastError());
+    next = it0;
+  }
+
+  // nb iterations; afterwards S holds the iteration count, the stop flag and the last residual norm
+  void enqueue(int nb)
+  {
+    if(fused) {
+      const VR* vl = vals();
+      const uint32_t* rp = L.row_ptr.as<uint32_t>();
+      const uint32_t* cl = L.cols.as<uint32_t>();
+      for(int k = 0; k < nb; ++k, ++next) {
+        hipLaunchKernelGGL(k_fb_p<R>, dim3(Gv), dim3(kRT), 0, c.stream, n, S, next, rel, 0, Gv, px, r, p, v, ph, dinv);
+        hipLaunchKernelGGL((k_spmv4_dot<VR, 1>), dim3(Gs), dim3(kRT), 0, c.stream, L.n_nodes, rp, cl, vl, ph, v, rh, S,
+                           pv);
+        hipLaunchKernelGGL(k_fb_s<R>, dim3(Gv), dim3(kRT), 0, c.stream, n, S, next, Gs, pv, r, v, s, sh, dinv, ps);
+        hipLaunchKernelGGL((k_spmv4_dot<VR, 2>), dim3(Gs), dim3(kRT), 0, c.stream, L.n_nodes, rp, cl, vl, sh, t, s, S,
+                           pt);
+        hipLaunchKernelGGL(k_fb_x<R>, dim3(Gv), dim3(kRT), 0, c.stream, n, S, Gv, ps, Gs, pt, x, ph, sh, r, s, t, rh, px);
+      }
+      // close the batch: finalise its last iteration
+      hipLaunchKernelGGL(k_fb_p<R>, dim3(1), dim3(kRT), 0, c.stream, n, S, next, rel, 1, Gv, px, r, p, v, ph, dinv);
+      HIP_OK(hipGetLastError());
+      return;
+    }
+    const int fin_last = rel ? (int)FIN_RNORM_REL : (int)FIN_RNORM;
+    const unsigned G = Gv;
+    for(int k = 0; k < nb; ++k, ++next) {
       hipLaunchKernelGGL(k_bicg_p<R>, dim3(G), dim3(kRT), 0, c.stream, n, S, r, p, v, ph, dinv);
       spmv_t<R, VR>(L, ph, v, S + S_STOP);
-      hipLaunchKernelGGL(k_dot2<R>, dim3(G), dim3(kRT), 0, c.stream, n, rh, v, (const R*)nullptr, (const R*)nullptr, part,
-                         S + S_STOP);
+      hipLaunchKernelGGL(k_dot2<R>, dim3(G), dim3(kRT), 0, c.stream, n, rh, v, (const R*)nullptr, (const R*)nullptr,
+                         part, S + S_STOP);
       hipLaunchKernelGGL(k_finalize<1>, dim3(1), dim3(kRT), 0, c.stream, part, G, S, 0, (int)FIN_ALPHA);
       hipLaunchKernelGGL(k_bicg_s<R>, dim3(G), dim3(kRT), 0, c.stream, n, S, r, v, s, sh, dinv, part);
       hipLaunchKernelGGL(k_finalize<1>, dim3(1), dim3(kRT), 0, c.stream, part, G, S, 0, (int)FIN_SNORM);
@@ -290,9 +501,46 @@ int bicgstab(Lss& L, const R* b, R* x, R* work, double tol, int maxit, bool& con
       hipLaunchKernelGGL(k_dot2<R>, dim3(G), dim3(kRT), 0, c.stream, n, t, s, t, t, part, S + S_STOP);
       hipLaunchKernelGGL(k_finalize<2>, dim3(1), dim3(kRT), 0, c.stream, part, G, S, 0, (int)FIN_OMEGA);
       hipLaunchKernelGGL(k_bicg_x<R>, dim3(G), dim3(kRT), 0, c.stream, n, S, x, ph, sh, r, s, t, rh, part);
-      hipLaunchKernelGGL(k_finalize<2>, dim3(1), dim3(kRT), 0, c.stream, part, G, S, 0, (int)FIN_RNORM);
+      hipLaunchKernelGGL(k_finalize<2>, dim3(1), dim3(kRT), 0, c.stream, part, G, S, 0, fin_last);
     }
     HIP_OK(hipGetLastError());
+  }
+
+ private:
+  void fill(R* a, double val)
+  {
+    if constexpr(sizeof(R) == sizeof(double)) lss_fill(c, a, n, val);
+    else lss_fill32(c, a, n, (float)val);
+  }
+};
+
+// BiCGStab on A (values of type VR, vectors of type R) for A x = b; x is the initial guess and the
+// result.  Returns the number of iterations; conv = recursive residual below tol*||b||.
+template<typename R, typename VR = R>
+int bicgstab(Lss& L, const R* b, R* x, R* work, double tol, int maxit, bool& conv, int& spmvs)
+{
+  Context& c = *L.ctx;
+  const uint64_t n = L.dofs;
+  Recurrence<R, VR> rc(L, x, work, 0);
+  double* S = rc.S;
+  double* part = rc.part;
+  const unsigned G = rc.Gv;
+  hipLaunchKernelGGL((k_jacobi<R, VR>), dim3(grid_for(n, 256)), dim3(256), 0, c.stream, L.n_nodes, L.neq,
+                     L.diag_blk.as<uint32_t>(), rc.vals(), const_cast<R*>(rc.dinv));
+  spmv_t<R, VR>(L, x, rc.r, nullptr);
+  hipLaunchKernelGGL(k_bicg_init<R>, dim3(G), dim3(kRT), 0, c.stream, n, b, rc.r, rc.rh, rc.p, rc.v, part);
+  hipLaunchKernelGGL(k_finalize<2>, dim3(1), dim3(kRT), 0, c.stream, part, G, S, (int)S_RN2, (int)FIN_STORE);
+  spmvs = 1;
+  Scalars sc;
+  read_scalars(c, sc);
+  double bn2 = sc.v[S_RN2 + 1];
+  if(bn2 == 0.) bn2 = 1.;
+  const double thr2 = tol * tol * bn2;
+  rc.seed(thr2, true, 0);
+  conv = sc.v[S_RN2] <= thr2;
+  int it = 0, batch = 1;
+  while(!conv && it < maxit) {
+    rc.enqueue(std::min(batch, maxit - it));
     read_scalars(c, sc);
     it = (int)sc.v[S_ITER];
     if(sc.v[S_STOP] != 0.) {   // converged, or broke down
@@ -384,8 +632,6 @@ __global__ void k_flush_increment(uint64_t n, double* __restrict__ xhi, double* 
     xlo[i] = 0.;
   }
 }
-
-__global__ void k_set_scalar(double* S, int idx, double v) { S[idx] = v; }
 
 constexpr int kMaxOuterBasis = 16;   // FGMRES restart length cap of the precision form
 
@@ -504,20 +750,12 @@ void reliable_bicgstab(Lss& L, Resid resid, const cf3hip_solver_params& prm, cf3
   Context& c = *L.ctx;
   const uint64_t n = L.dofs;
   ensure_work(L, 10, sizeof(double));
-  double* r = L.work.as<double>();
-  double* rh = r + n;
-  double* p = rh + n;
-  double* v = p + n;
-  double* ph = v + n;
-  double* s = ph + n;
-  double* sh = s + n;
-  double* t = sh + n;
-  double* dinv = t + n;
-  double* xlo = dinv + n;
+  double* work = L.work.as<double>();
+  double* xlo = work + 9 * n;
   double* xhi = L.sol.as<double>();
-  double* S = c.scalars.as<double>();
-  double* part = c.red_partials.as<double>();
-  const unsigned G = red_grid(n);
+  Recurrence<double, float> rc(L, xlo, work, 1);
+  double* r = rc.r;
+  double* S = rc.S;
   const double delta = 0.01;
   st = cf3hip_solve_stats{0, 0, 0, 0, 0.};
 
@@ -531,15 +769,8 @@ void reliable_bicgstab(Lss& L, Resid resid, const cf3hip_solver_params& prm, cf3
   int updates = 1;
   double rtrue = norm2(c, r, n);
   hipLaunchKernelGGL((k_jacobi<double, float>), dim3(grid_for(n, 256)), dim3(256), 0, c.stream, L.n_nodes, L.neq,
-                     L.diag_blk.as<uint32_t>(), L.values32.as<float>(), dinv);
-  lss_copy(c, rh, r, n);
-  lss_fill(c, p, n, 0.);
-  lss_fill(c, v, n, 0.);
-  hipLaunchKernelGGL(k_set_scalars, dim3(1), dim3(1), 0, c.stream, S, thr2);
-  hipLaunchKernelGGL(k_dot2<double>, dim3(G), dim3(kRT), 0, c.stream, n, rh, r, (const double*)nullptr,
-                     (const double*)nullptr, part, (const double*)nullptr);
-  hipLaunchKernelGGL(k_finalize<1>, dim3(1), dim3(kRT), 0, c.stream, part, G, S, (int)S_RHO, (int)FIN_STORE);
-  HIP_OK(hipGetLastError());
+                     L.diag_blk.as<uint32_t>(), L.values32.as<float>(), const_cast<double*>(rc.dinv));
+  rc.seed(thr2, true, 0);
   bool conv = rtrue * rtrue <= thr2;
   constexpr int kSwitchStall = 2;
   double rprog = rtrue, rprev = rtrue;
@@ -547,33 +778,18 @@ void reliable_bicgstab(Lss& L, Resid resid, const cf3hip_solver_params& prm, cf3
   bool switched = false;
   hipLaunchKernelGGL(k_set_scalar, dim3(1), dim3(1), 0, c.stream, S, (int)S_MAXR, rtrue);
   hipLaunchKernelGGL(k_set_scalar, dim3(1), dim3(1), 0, c.stream, S, (int)S_DELTA, delta);
-  int it = 0, it_base = 0, batch = 1;
+  int it = 0, batch = 1;
   Scalars sc;
   while(!conv && it < prm.max_iter) {
-    const int nb = std::min(batch, prm.max_iter - it);
-    for(int k = 0; k < nb; ++k) {
-      hipLaunchKernelGGL(k_bicg_p<double>, dim3(G), dim3(kRT), 0, c.stream, n, S, r, p, v, ph, dinv);
-      lss_spmv_mixed(L, ph, v, S + S_STOP);
-      hipLaunchKernelGGL(k_dot2<double>, dim3(G), dim3(kRT), 0, c.stream, n, rh, v, (const double*)nullptr,
-                         (const double*)nullptr, part, S + S_STOP);
-      hipLaunchKernelGGL(k_finalize<1>, dim3(1), dim3(kRT), 0, c.stream, part, G, S, 0, (int)FIN_ALPHA);
-      hipLaunchKernelGGL(k_bicg_s<double>, dim3(G), dim3(kRT), 0, c.stream, n, S, r, v, s, sh, dinv, part);
-      hipLaunchKernelGGL(k_finalize<1>, dim3(1), dim3(kRT), 0, c.stream, part, G, S, 0, (int)FIN_SNORM);
-      lss_spmv_mixed(L, sh, t, S + S_STOP);
-      hipLaunchKernelGGL(k_dot2<double>, dim3(G), dim3(kRT), 0, c.stream, n, t, s, t, t, part, S + S_STOP);
-      hipLaunchKernelGGL(k_finalize<2>, dim3(1), dim3(kRT), 0, c.stream, part, G, S, 0, (int)FIN_OMEGA);
-      hipLaunchKernelGGL(k_bicg_x<double>, dim3(G), dim3(kRT), 0, c.stream, n, S, xlo, ph, sh, r, s, t, rh, part);
-      hipLaunchKernelGGL(k_finalize<2>, dim3(1), dim3(kRT), 0, c.stream, part, G, S, 0, (int)FIN_RNORM_REL);
-    }
-    HIP_OK(hipGetLastError());
+    rc.enqueue(std::min(batch, prm.max_iter - it));
     read_scalars(c, sc);
-    it = it_base + (int)sc.v[S_ITER];
+    it = (int)sc.v[S_ITER];
     if(sc.v[S_STOP] == 0.) {
       batch = std::min(2 * batch, kMaxBatch);
       continue;
     }
     // reliable update: move the increment into the FP64 solution, replace the residual
-    const bool breakdown = !std::isfinite(sc.v[S_RN2]) || sc.v[S_RHO] == 0.;
+    const bool breakdown = !std::isfinite(sc.v[S_RN2]) || rc.rho(sc, it) == 0.;
     hipLaunchKernelGGL(k_flush_increment, dim3(std::min<unsigned>(grid_for(n, 256), 8192)), dim3(256), 0, c.stream,
                        n, xhi, xlo);
     resid(xhi, r);
@@ -588,21 +804,10 @@ void reliable_bicgstab(Lss& L, Resid resid, const cf3hip_solver_params& prm, cf3
     }
     if(!std::isfinite(rtrue) || rtrue > rprev || stall >= kSwitchStall) { switched = true; break; }
     rprev = rtrue;
-    if(breakdown) {   // restart the recurrences from the true residual
-      lss_copy(c, rh, r, n);
-      lss_fill(c, p, n, 0.);
-      lss_fill(c, v, n, 0.);
-      hipLaunchKernelGGL(k_set_scalars, dim3(1), dim3(1), 0, c.stream, S, thr2);
-    }
-    hipLaunchKernelGGL(k_dot2<double>, dim3(G), dim3(kRT), 0, c.stream, n, rh, r, (const double*)nullptr,
-                       (const double*)nullptr, part, (const double*)nullptr);
-    hipLaunchKernelGGL(k_finalize<1>, dim3(1), dim3(kRT), 0, c.stream, part, G, S, (int)S_RHO, (int)FIN_STORE);
-    hipLaunchKernelGGL(k_set_scalar, dim3(1), dim3(1), 0, c.stream, S, (int)S_SCONV, 0.);
+    // breakdown: restart the recurrences from the true residual; otherwise keep the directions
+    rc.seed(thr2, breakdown, it);
     hipLaunchKernelGGL(k_set_scalar, dim3(1), dim3(1), 0, c.stream, S, (int)S_MAXR, rtrue);
-    hipLaunchKernelGGL(k_set_scalar, dim3(1), dim3(1), 0, c.stream, S, (int)S_ITER, 0.);
-    hipLaunchKernelGGL(k_set_scalar, dim3(1), dim3(1), 0, c.stream, S, (int)S_STOP, 0.);
     HIP_OK(hipGetLastError());
-    it_base = it;
     batch = 1;
   }
   const int spmvs = 2 * it;

@@ -9,6 +9,7 @@
 #include <unordered_map>
 
 #include "internal.hpp"
+#include "spmv4.hpp"
 
 namespace cf3hip {
 
@@ -22,9 +23,7 @@ __global__ void k_fill(T* __restrict__ p, uint64_t n, T v)
   for(uint64_t j = i; j < n; j += stride) p[j] = v;
 }
 
-// FP64, neq = 4: 16 lanes per node row, each lane streams 2 consecutive doubles (16 B) of a block
-// row: lanes 0-7 take block 2k, lanes 8-15 block 2k+1.  Lane l: alpha = (l & 7) >> 1, beta pair
-// 2*(l & 1).  Partial sums are reduced across the beta pair and the block parity.
+// FP64, neq = 4: 16 lanes per node row (lane layout of spmv4::row_f64)
 __global__ __launch_bounds__(256) void k_spmv4_f64(uint32_t N, const uint32_t* __restrict__ row_ptr,
                                                    const uint32_t* __restrict__ cols, const double* __restrict__ vals,
                                                    double alpha, const double* __restrict__ x, double beta,
@@ -34,18 +33,11 @@ __global__ __launch_bounds__(256) void k_spmv4_f64(uint32_t N, const uint32_t* _
   const uint32_t row = (blockIdx.x * blockDim.x + threadIdx.x) >> 4;
   const int l = threadIdx.x & 15;
   if(row >= N) return;
-  const uint32_t q0 = row_ptr[row], q1 = row_ptr[row + 1];
-  const int sub = l >> 3, li = l & 7;
-  double acc = 0.;
-  for(uint32_t q = q0 + sub; q < q1; q += 2) {
-    const double2 v = *reinterpret_cast<const double2*>(vals + (uint64_t)q * 16 + 2 * li);
-    const double2 xv = *reinterpret_cast<const double2*>(x + (uint64_t)cols[q] * 4 + 2 * (li & 1));
-    acc += v.x * xv.x + v.y * xv.y;
-  }
-  acc += __shfl_xor(acc, 1, 16);
-  acc += __shfl_xor(acc, 8, 16);
-  if(l < 8 && (l & 1) == 0) {
-    const uint64_t o = (uint64_t)row * 4 + (li >> 1);
+  int a;
+  bool out;
+  const double acc = spmv4::row_f64(row_ptr, cols, vals, x, row, l, a, out);
+  if(out) {
+    const uint64_t o = (uint64_t)row * 4 + a;
     double r = alpha * acc;
     if(beta != 0.) r += beta * y[o];
     y[o] = r;
@@ -81,7 +73,7 @@ __global__ __launch_bounds__(256) void k_spmv4_f32(uint32_t N, const uint32_t* _
   }
 }
 
-// FP32 values with FP64 vectors and FP64 accumulation, neq = 4: lane layout of k_spmv4_f32
+// FP32 values with FP64 vectors and FP64 accumulation, neq = 4 (lane layout of spmv4::row_mixed)
 __global__ __launch_bounds__(256) void k_spmv4_mixed(uint32_t N, const uint32_t* __restrict__ row_ptr,
                                                      const uint32_t* __restrict__ cols, const float* __restrict__ vals,
                                                      const double* __restrict__ x, double* __restrict__ y,
@@ -91,18 +83,10 @@ __global__ __launch_bounds__(256) void k_spmv4_mixed(uint32_t N, const uint32_t*
   const uint32_t row = (blockIdx.x * blockDim.x + threadIdx.x) >> 4;
   const int l = threadIdx.x & 15;
   if(row >= N) return;
-  const uint32_t q0 = row_ptr[row], q1 = row_ptr[row + 1];
-  const int sub = l >> 2, a = l & 3;
-  double acc = 0.;
-  for(uint32_t q = q0 + sub; q < q1; q += 4) {
-    const float4 v = *reinterpret_cast<const float4*>(vals + (uint64_t)q * 16 + 4 * a);
-    const double2 x01 = *reinterpret_cast<const double2*>(x + (uint64_t)cols[q] * 4);
-    const double2 x23 = *reinterpret_cast<const double2*>(x + (uint64_t)cols[q] * 4 + 2);
-    acc += (double)v.x * x01.x + (double)v.y * x01.y + (double)v.z * x23.x + (double)v.w * x23.y;
-  }
-  acc += __shfl_xor(acc, 4, 16);
-  acc += __shfl_xor(acc, 8, 16);
-  if(l < 4) y[(uint64_t)row * 4 + a] = acc;
+  int a;
+  bool out;
+  const double acc = spmv4::row_mixed(row_ptr, cols, vals, x, row, l, a, out);
+  if(out) y[(uint64_t)row * 4 + a] = acc;
 }
 
 __global__ void k_spmv_mixed_generic(uint32_t N, int neq, const uint32_t* __restrict__ row_ptr,

@@ -113,7 +113,7 @@ def test_assembly_fp32_error_stated(ctx, oracle, name):
     v32 = g.system.values_fp32().astype(np.float64)
     err = row_rel_err(v32, vref, case.dim + 1, start)
     print(f"FP32 assembly row-relative error vs FP64 oracle: {err:.3e}")
-    assert err <= 2e-6
+    assert err <= 1e-6   # measured 1.1e-7 (hex) .. 4.0e-7 (tri); a few FP32 ulps per row
     g.close()
 
 
