@@ -244,7 +244,7 @@ int cf3hip_ns_update(cf3hip_lss lss, cf3hip_fields fields);
  * matrix-free from the FP64 element matrices of the same fields (after the Dirichlet conditions
  * applied with cf3hip_lss_dirichlet); only the FP32 assembly of the step is needed.  BiCGStab +
  * Jacobi on the FP32 element matrices (FP64 vectors) whose residual is replaced by that FP64
- * residual after every 100-fold drop; if a replacement raises the FP64 residual or two in a row fail
+ * residual after every 10-fold drop; if a replacement raises the FP64 residual or two in a row fail
  * to halve it, the solve continues with flexible GMRES in FP64 on the FP64 operator, preconditioned
  * by BiCGStab solves on the FP32 values (FP64 vectors) to inner_tol.  Converged when the FP64 residual
  * meets tol.  stats: iterations = BiCGStab iterations of both phases, outer = residual replacements

@@ -68,3 +68,24 @@ def test_result_alone_equals_result_in_mix():
     # the sequential path serves the same request to the same fields
     seq, _ = srv.run(reqs[:1], batched=False)
     assert np.abs(seq[0].u - a.u).max() <= 1e-12 * np.abs(a.u).max()
+
+
+@pytest.mark.gpu
+def test_mix_of_requests_is_deterministic():
+    """Eight mid-sized requests solved concurrently (continuous batching) give every request the
+    iteration counts and the fields of one-at-a-time serving: the concurrent solves share nothing and
+    every reduction has a fixed order (a stop flag raised while other workgroups of the same launch
+    start must not split a workgroup)."""
+    reqs = [serving.Request(i, 0.0, n, re, amp, steps=3) for i, (n, re, amp) in
+            enumerate([(24, 120.0, 0.05), (30, 260.0, 0.0), (27, 80.0, 0.1), (22, 700.0, 0.02),
+                       (28, 300.0, 0.07), (20, 950.0, 0.01), (26, 55.0, 0.04), (25, 410.0, 0.09)])]
+    srv = serving.Server(max_active=8, keep_fields=True)
+    mix, summary = srv.run(reqs)
+    seq, _ = srv.run(reqs, batched=False)
+    by_id = {r.rid: r for r in seq}
+    for m in mix:
+        s = by_id[m.rid]
+        assert m.iterations == s.iterations, (m.rid, m.iterations, s.iterations)
+        assert np.abs(m.u - s.u).max() <= 1e-12 * np.abs(s.u).max()
+        assert np.abs(m.p - s.p).max() <= 1e-12 * max(np.abs(s.p).max(), 1.0)
+    assert summary["all_converged"]
