@@ -652,7 +652,7 @@ __device__ __forceinline__ void assemble_wg(const AsmArgs& g, uint32_t wg)
 template<class Op, typename R, typename VR, bool RHS>
 __global__ __launch_bounds__(kThreads) void k_assemble(AsmArgs g)
 {
-  assemble_wg<Op, R, VR, RHS>(g, blockIdx.x);
+  assemble_wg<Op, R, VR, RHS>(g, xcd_order(blockIdx.x, gridDim.x));
 }
 
 // ---------------------------------------------------------------------------------------------
@@ -721,7 +721,8 @@ __global__ __launch_bounds__(kThreads) void k_residual(AsmArgs g)
   constexpr int K = Op::K, NEQ = Op::NEQ, NN = NEQ * NEQ;
   __shared__ double s_res[kThreads * NEQ];
   __shared__ uint32_t s_rowinc[kThreads + 1];
-  const uint32_t r0 = g.wg_rows[blockIdx.x], r1 = g.wg_rows[blockIdx.x + 1];
+  const uint32_t wg = xcd_order(blockIdx.x, gridDim.x);
+  const uint32_t r0 = g.wg_rows[wg], r1 = g.wg_rows[wg + 1];
   const uint32_t nrows = r1 - r0;
   const uint32_t i0 = g.inc_ptr[r0];
   const uint32_t ninc = g.inc_ptr[r1] - i0;

@@ -54,6 +54,19 @@ constexpr int lds_row()
 
 __device__ __forceinline__ void unpack(const double4& v, double o[4]) { o[0] = v.x; o[1] = v.y; o[2] = v.z; o[3] = v.w; }
 
+// XCD-aware workgroup order: workgroups are dispatched round-robin over the 8 XCDs of the MI355X, each
+// with its own L2.  Workgroup i (the k-th on XCD i mod 8) takes logical workgroup k of that XCD's
+// contiguous share, so each L2 serves a compact part of the mesh (node records and maps shared by
+// neighbouring row ranges).  A bijection of [0, n); the result of a logical workgroup does not depend
+// on where it runs.
+constexpr uint32_t kXcds = 8;
+__device__ __forceinline__ uint32_t xcd_order(uint32_t i, uint32_t n)
+{
+  const uint32_t q = n / kXcds, r = n % kXcds;
+  const uint32_t x = i % kXcds, k = i / kXcds;
+  return x * q + (x < r ? x : r) + k;
+}
+
 template<int K>
 __device__ __forceinline__ void load_conn(const uint32_t* conn, uint32_t e, uint32_t cn[K])
 {

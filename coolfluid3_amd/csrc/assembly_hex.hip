@@ -95,7 +95,8 @@ __global__ __launch_bounds__(kThreads) void k_hexa(AsmArgs g)
   __shared__ uint32_t s_rowinc[kThreads + 1];
   __shared__ uint16_t s_acc[kThreads];
 
-  const uint32_t r0 = g.wg_rows[blockIdx.x], r1 = g.wg_rows[blockIdx.x + 1];
+  const uint32_t wg = xcd_order(blockIdx.x, gridDim.x);
+  const uint32_t r0 = g.wg_rows[wg], r1 = g.wg_rows[wg + 1];
   const uint32_t nrows = r1 - r0;
   const uint32_t i0 = g.inc_ptr[r0];
   const uint32_t ninc = g.inc_ptr[r1] - i0;
