@@ -160,6 +160,31 @@ def _group(case, nodes, eqs, vals):
         yield n_stack, e_stack, v_stack
 
 
+def neu_case(path, etype_name=None, nu=1e-2, dt=1e-2, seed=DEFAULT_SEED, bcs=(), pin=None):
+    """A case on a Gambit neutral mesh (meshio.read_neu) with one element type: velocity U[-1,1] per
+    node from the seed, p = 0.  bcs: [(boundary set, {eq: value or callable(coords)})], applied in that
+    order over each set's nodes in ascending order; pin: (boundary set, eq) pins that variable to 0 at
+    the set's first node."""
+    from . import meshio
+    m = meshio.read_neu(path)
+    if etype_name is None:
+        if len(m.blocks) != 1:
+            raise ValueError("mesh holds several element types; name one of %s" % sorted(m.blocks))
+        etype_name = next(iter(m.blocks))
+    etype = {"Triag2D": cabi.TRIAG2D, "Tetra3D": cabi.TETRA3D, "Hexa3D": cabi.HEXA3D}[etype_name]
+    conn = m.blocks[etype_name]
+    used = np.unique(conn.ravel())
+    if used.size != m.n_nodes:
+        raise ValueError("the %s cells do not use every node of the mesh" % etype_name)
+    N, dim = m.n_nodes, m.dim
+    u0 = meshgen.uniform(seed, N * dim, -1.0, 1.0).reshape(N, dim)
+    regions = [(name, m.boundary_nodes(name), dict(spec)) for name, spec in bcs]
+    if pin is not None:
+        regions.append(("pin", m.boundary_nodes(pin[0])[:1], {pin[1]: 0.0}))
+    return Case("%s_%s" % (m.name, etype_name), etype, m.coords, conn, None, nu, dt, u0=u0, p0=np.zeros(N),
+                bc_regions=regions)
+
+
 class GpuCase:
     """A case resident on the device: mesh, fields, LSS system; one implicit step per call."""
 

@@ -12,11 +12,24 @@ Sources:
   test/math/utest-lss-test-matrix.hpp:32-50,669-709   (per-rank element lists, node-graph CSR, periodic pairs)
   test/math/utest-lss-symmetric-dirichlet.cpp:72-208  (2-rank 4-node system, merged to serial)
   test/math/utest-lss-vector.cpp:144-187               (apply y = 2*A*x + y with A == 1)
+Also copies two Gambit neutral mesh files the reference's tests read (data files, unchanged) into
+meshes/: plugins/UFEM/test/meshes/ring2d-tris.neu (669 nodes, 1204 triangles, boundary sets inner and
+outer) and resources/hextet.neu (hexahedra and tetrahedra; test/mesh/utest-mesh-neu.cpp:123-125).
 """
 import json
 import os
 import re
+import shutil
 import sys
+
+MESHES = ["plugins/UFEM/test/meshes/ring2d-tris.neu", "resources/hextet.neu"]
+
+
+def copy_meshes(ref):
+    dst = os.path.join(os.path.dirname(os.path.abspath(__file__)), "meshes")
+    os.makedirs(dst, exist_ok=True)
+    for rel in MESHES:
+        shutil.copyfile(os.path.join(ref, rel), os.path.join(dst, os.path.basename(rel)))
 
 
 def cpp_array(src, name):
@@ -85,6 +98,7 @@ def main(ref):
                            "expect": "y_i = 1 + 2 * sum_j x_j (rows of an all-ones matrix)"}
     with open(os.path.join(os.path.dirname(os.path.abspath(__file__)), "reference_fixtures.json"), "w") as f:
         json.dump(out, f, indent=1)
+    copy_meshes(ref)
 
 
 if __name__ == "__main__":
