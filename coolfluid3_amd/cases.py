@@ -160,13 +160,13 @@ def _group(case, nodes, eqs, vals):
         yield n_stack, e_stack, v_stack
 
 
-def neu_case(path, etype_name=None, nu=1e-2, dt=1e-2, seed=DEFAULT_SEED, bcs=(), pin=None):
-    """A case on a Gambit neutral mesh (meshio.read_neu) with one element type: velocity U[-1,1] per
+def mesh_case(path, etype_name=None, nu=1e-2, dt=1e-2, seed=DEFAULT_SEED, bcs=(), pin=None):
+    """A case on a mesh file (meshio.read_mesh: .neu or .msh) with one element type: velocity U[-1,1] per
     node from the seed, p = 0.  bcs: [(boundary set, {eq: value or callable(coords)})], applied in that
     order over each set's nodes in ascending order; pin: (boundary set, eq) pins that variable to 0 at
     the set's first node."""
     from . import meshio
-    m = meshio.read_neu(path)
+    m = meshio.read_mesh(path)
     if etype_name is None:
         if len(m.blocks) != 1:
             raise ValueError("mesh holds several element types; name one of %s" % sorted(m.blocks))

@@ -12,9 +12,11 @@ Sources:
   test/math/utest-lss-test-matrix.hpp:32-50,669-709   (per-rank element lists, node-graph CSR, periodic pairs)
   test/math/utest-lss-symmetric-dirichlet.cpp:72-208  (2-rank 4-node system, merged to serial)
   test/math/utest-lss-vector.cpp:144-187               (apply y = 2*A*x + y with A == 1)
-Also copies two Gambit neutral mesh files the reference's tests read (data files, unchanged) into
-meshes/: plugins/UFEM/test/meshes/ring2d-tris.neu (669 nodes, 1204 triangles, boundary sets inner and
-outer) and resources/hextet.neu (hexahedra and tetrahedra; test/mesh/utest-mesh-neu.cpp:123-125).
+Also copies mesh files of the reference (data files, unchanged) into meshes/: the Gambit neutral files
+plugins/UFEM/test/meshes/ring2d-tris.neu (669 nodes, 1204 triangles, boundary sets inner and outer) and
+resources/hextet.neu (hexahedra and tetrahedra; test/mesh/utest-mesh-neu.cpp:123-125), and the Gmsh
+files resources/rectangle-tg-p1.msh (507 nodes, 929 triangles; test/mesh/utest-mesh-gmsh.cpp:96) and
+resources/sqduct_3000e.msh (3751 nodes, 3000 hexahedra of a square duct).
 """
 import json
 import os
@@ -22,7 +24,8 @@ import re
 import shutil
 import sys
 
-MESHES = ["plugins/UFEM/test/meshes/ring2d-tris.neu", "resources/hextet.neu"]
+MESHES = ["plugins/UFEM/test/meshes/ring2d-tris.neu", "resources/hextet.neu", "resources/rectangle-tg-p1.msh",
+          "resources/sqduct_3000e.msh"]
 
 
 def copy_meshes(ref):

@@ -60,18 +60,40 @@ def gpu_case(ctx, case, spec=True, supg=cabi.SUPG_METRIC, theta=1.0, fp32=False)
     return g
 
 
-RING = os.path.join(os.path.dirname(os.path.abspath(__file__)), "golden", "meshes", "ring2d-tris.neu")
+MESHES = os.path.join(os.path.dirname(os.path.abspath(__file__)), "golden", "meshes")
 
 
 def ring_case():
     """The reference's 2D annulus of triangles (Gambit neutral file): no-slip on the inner circle,
     u = (1, 0) on the outer one, p pinned at one outer node."""
-    return cases.neu_case(RING, bcs=[("inner", {0: 0.0, 1: 0.0}), ("outer", {0: 1.0, 1: 0.0})], pin=("outer", 2))
+    return cases.mesh_case(os.path.join(MESHES, "ring2d-tris.neu"),
+                           bcs=[("inner", {0: 0.0, 1: 0.0}), ("outer", {0: 1.0, 1: 0.0})], pin=("outer", 2))
+
+
+def rectangle_case():
+    """The reference's rectangle of triangles (Gmsh file): no-slip on the wall, u = (0, 1) on the inlet,
+    p = 0 on the outlet."""
+    return cases.mesh_case(os.path.join(MESHES, "rectangle-tg-p1.msh"),
+                           bcs=[("Wall", {0: 0.0, 1: 0.0}), ("Inlet", {0: 0.0, 1: 1.0}), ("Outlet", {2: 0.0})])
+
+
+def duct_case():
+    """The reference's square duct of hexahedra (Gmsh file): u = (0, 0, 1) on the inlet (z = 0), no-slip
+    on the side walls, p = 0 on the outlet (z = 3).  The file names no wall region and its outlet region
+    holds no faces: those nodes are selected by their coordinates."""
+    case = cases.mesh_case(os.path.join(MESHES, "sqduct_3000e.msh"), bcs=[("inlet", {0: 0.0, 1: 0.0, 2: 1.0})])
+    c = case.coords
+    wall = np.flatnonzero((np.abs(c[:, 0]) == 0.5) | (np.abs(c[:, 1]) == 0.5)).astype(np.uint32)
+    outlet = np.flatnonzero(c[:, 2] == 3.0).astype(np.uint32)
+    case.bc_regions += [("wall", wall, {0: 0.0, 1: 0.0, 2: 0.0}), ("outlet", outlet, {3: 0.0})]
+    return case
 
 
 CASES = {
     "tri8": lambda: cases.lid_cavity_2d(8),
     "ring": ring_case,
+    "rect": rectangle_case,
+    "duct": duct_case,
     "tet5": lambda: cases.inflow_cube_tets(5),
     "tet12": lambda: cases.inflow_cube_tets(12),
     "tet12s": lambda: cases.inflow_cube_tets(12, skew=True),
@@ -80,7 +102,7 @@ CASES = {
 }
 
 
-@pytest.mark.parametrize("name", ["tri8", "ring", "tet5", "tet12", "hex5"])
+@pytest.mark.parametrize("name", ["tri8", "ring", "rect", "tet5", "tet12", "hex5", "duct"])
 def test_sparsity_and_slot_map_bit_exact(ctx, oracle, name):
     case = CASES[name]()
     print("case", name, flush=True)
@@ -95,6 +117,7 @@ def test_sparsity_and_slot_map_bit_exact(ctx, oracle, name):
 @pytest.mark.parametrize("name,spec,supg,theta", [
     ("tri8", True, cabi.SUPG_METRIC, 1.0), ("tri8", False, cabi.SUPG_METRIC, 0.5),
     ("ring", True, cabi.SUPG_METRIC, 1.0), ("ring", False, cabi.SUPG_TEZDUYAR, 1.0),
+    ("rect", True, cabi.SUPG_METRIC, 1.0), ("duct", False, cabi.SUPG_METRIC, 1.0),
     ("tet5", True, cabi.SUPG_METRIC, 1.0), ("tet5", False, cabi.SUPG_METRIC, 0.7),
     ("tet5", False, cabi.SUPG_TEZDUYAR, 1.0), ("tet12", True, cabi.SUPG_METRIC, 1.0),
     ("tet12s", True, cabi.SUPG_METRIC, 1.0),
@@ -117,7 +140,7 @@ def test_assembly_fp64(ctx, oracle, name, spec, supg, theta):
     g.close()
 
 
-@pytest.mark.parametrize("name", ["tri8", "ring", "tet12", "tet12s", "hex5"])
+@pytest.mark.parametrize("name", ["tri8", "ring", "rect", "tet12", "tet12s", "hex5", "duct"])
 def test_assembly_fp32_error_stated(ctx, oracle, name):
     case = CASES[name]()
     start, cols, slots, vref, rref = oracle_system(oracle, case)
@@ -126,7 +149,7 @@ def test_assembly_fp32_error_stated(ctx, oracle, name):
     v32 = g.system.values_fp32().astype(np.float64)
     err = row_rel_err(v32, vref, case.dim + 1, start)
     print(f"FP32 assembly row-relative error vs FP64 oracle: {err:.3e}")
-    assert err <= 1e-6   # measured 1.1e-7 (hex) .. 4.0e-7 (tri); a few FP32 ulps per row
+    assert err <= 1e-6   # measured 1.1e-7 (hex) .. 6.8e-7 (annulus triangles); a few FP32 ulps per row
     g.close()
 
 
@@ -223,24 +246,27 @@ def test_gmres_ilu0_lid_cavity(ctx, oracle):
     g.close()
 
 
-def test_ring_mesh_step_vs_oracle(ctx, oracle):
-    """A mesh read from a Gambit neutral file: one implicit step (FP64 assembly, BCs, GMRES(30)+ILU0 to
-    1e-10) on the device matches the oracle's system and solution."""
-    case = ring_case()
+@pytest.mark.parametrize("name", ["ring", "rect", "duct"])
+def test_file_mesh_step_vs_oracle(ctx, oracle, name):
+    """Meshes read from the reference's Gambit neutral and Gmsh files: one implicit step (FP64 assembly,
+    BCs, GMRES(30)+ILU0 to 1e-10) on the device matches the oracle's system and solution."""
+    case = CASES[name]()
+    print("case", name, flush=True)
+    neq = case.dim + 1
     start, cols, slots, vref, rref = oracle_system(oracle, case)
     g = gpu_case(ctx, case)
     g.assemble(cabi.FP64)
-    assert row_rel_err(g.system.values(), vref, 3, start) <= 1e-12
+    assert row_rel_err(g.system.values(), vref, neq, start) <= 1e-12
     n, e, vals = cases.bc_lists(case, case.u0, case.p0)
     g.system.dirichlet(n, e, vals)
     sol = np.zeros_like(rref)
-    oracle.dirichlet(3, start, cols, vref, rref, sol, n, e, vals)
+    oracle.dirichlet(neq, start, cols, vref, rref, sol, n, e, vals)
     st = g.system.solve(cabi.solver_params(cabi.GMRES_ILU0, tol=1e-10, restart=30))
     x = g.system.vector(cabi.SOLUTION)
-    assert st.converged and oracle_residual(oracle, 3, start, cols, vref, rref, x) <= 1e-10
-    xo, _ = oracle.gmres_ilu0(3, start, cols, vref, rref, sol, restart=30, tol=1e-12)
-    print(f"ring: {case.n_elems} triangles, GMRES {st.iterations} it, "
-          f"solution difference {np.abs(x - xo).max() / np.abs(xo).max():.2e}")
+    assert st.converged and oracle_residual(oracle, neq, start, cols, vref, rref, x) <= 1e-10
+    xo, _ = oracle.gmres_ilu0(neq, start, cols, vref, rref, sol, restart=30, tol=1e-12)
+    diff = np.abs(x - xo).max() / np.abs(xo).max()
+    print(f"{name}: {case.n_elems} cells, GMRES {st.iterations} it, solution difference {diff:.2e}")
     g.close()
 
 

@@ -1,4 +1,4 @@
-"""Gambit neutral file reading (CPU): the reference's mesh files, read into device-mesh arrays."""
+"""Mesh file reading (CPU): the reference's Gambit neutral and Gmsh files, read into device-mesh arrays."""
 import os
 
 import numpy as np
@@ -48,7 +48,7 @@ def test_hextet_mixed_cells():
     q = m.coords[m.boundaries["inlet"]["Quad3D"]]
     assert all(np.any(np.ptp(face, axis=0) == 0.0) for face in q)
     with pytest.raises(ValueError):
-        cases.neu_case(os.path.join(MESHES, "hextet.neu"))   # two element types: one must be named
+        cases.mesh_case(os.path.join(MESHES, "hextet.neu"))   # two element types: one must be named
 
 
 def test_unsupported_cells_rejected(tmp_path):
@@ -67,7 +67,7 @@ def test_unsupported_cells_rejected(tmp_path):
 
 def test_ring_case_oracle_system():
     """The ring case assembles in the oracle (CPU) with the boundary conditions of its sets."""
-    case = cases.neu_case(os.path.join(MESHES, "ring2d-tris.neu"),
+    case = cases.mesh_case(os.path.join(MESHES, "ring2d-tris.neu"),
                           bcs=[("inner", {0: 0.0, 1: 0.0}), ("outer", {0: 1.0, 1: 0.0})], pin=("outer", 2))
     n, e, v = cases.bc_lists(case, case.u0, case.p0)
     assert len(n) == 2 * 64 + 2 * 70 + 1
@@ -77,3 +77,59 @@ def test_ring_case_oracle_system():
     vals, rhs = o.ns_assemble(pyoracle.TRIAG2D, prm, case.coords, case.conn, case.u0, case.p0, case.u0,
                               np.full(case.n_nodes, case.nu), start, cols, slots)
     assert np.all(np.isfinite(vals)) and np.abs(vals).max() > 0
+
+
+def test_gmsh_rectangle_triangles():
+    """resources/rectangle-tg-p1.msh: the rectangle [-1,1] x [0,1] in 929 triangles, three named line
+    regions (Inlet, Outlet on y = 0, Wall on the other sides) and one cell region."""
+    m = meshio.read_mesh(os.path.join(MESHES, "rectangle-tg-p1.msh"))
+    assert (m.name, m.dim, m.n_nodes) == ("rectangle-tg-p1", 2, 507)
+    t = m.blocks["Triag2D"]
+    assert t.shape == (929, 3) and list(m.blocks) == ["Triag2D"]
+    assert list(m.groups) == ["InnerCells"] and len(m.groups["InnerCells"]) == 929
+    o = pyoracle.get()
+    area = np.array([o.element_volume(pyoracle.TRIAG2D, m.coords[e]) for e in t])
+    assert area.min() > 0 and abs(area.sum() - 2.0) < 1e-12
+    c = m.coords
+    assert {k: {f: a.shape for f, a in v.items()} for k, v in m.boundaries.items()} == {
+        "Inlet": {"Line2D": (15, 2)}, "Outlet": {"Line2D": (15, 2)}, "Wall": {"Line2D": (53, 2)}}
+    inlet, outlet, wall = (m.boundary_nodes(b) for b in ("Inlet", "Outlet", "Wall"))
+    assert np.all(c[inlet, 1] == 0) and np.all(c[inlet, 0] <= 0) and np.all(c[outlet, 0] >= 0)
+    on_wall = (np.abs(c[wall, 0]) == 1) | (c[wall, 1] == 1)
+    assert on_wall.all() and len(wall) == 54
+    # every node of the rectangle's perimeter is on one of the three regions (two corners on two)
+    perim = np.flatnonzero((np.abs(c[:, 0]) == 1) | (c[:, 1] == 0) | (c[:, 1] == 1))
+    assert np.array_equal(np.union1d(np.union1d(inlet, outlet), wall), perim)
+
+
+def test_gmsh_square_duct_hexahedra():
+    """resources/sqduct_3000e.msh: the duct [-0.5,0.5]^2 x [0,3] in 3000 hexahedra (gmsh node order
+    is cf3's), the inlet region's quadrilaterals on z = 0."""
+    m = meshio.read_mesh(os.path.join(MESHES, "sqduct_3000e.msh"))
+    assert (m.dim, m.n_nodes) == (3, 3751) and m.blocks["Hexa3D"].shape == (3000, 8)
+    o = pyoracle.get()
+    vol = np.array([o.element_volume(pyoracle.HEXA3D, m.coords[e]) for e in m.blocks["Hexa3D"]])
+    assert vol.min() > 0 and abs(vol.sum() - 3.0) < 1e-12            # positive: cf3 orientation
+    q = m.boundaries["inlet"]["Quad3D"]
+    assert q.shape == (100, 4) and np.all(m.coords[q][..., 2] == 0.0)
+    assert len(m.boundary_nodes("inlet")) == 121
+
+
+def test_gmsh_files_rejected(tmp_path):
+    head = "$MeshFormat\n2.2 0 8\n$EndMeshFormat\n"
+    nodes = "$Nodes\n6\n1 0 0 0\n2 1 0 0\n3 0 1 0\n4 0 0 1\n5 1 0 1\n6 0 1 1\n$EndNodes\n"
+    prism = "$Elements\n1\n1 6 2 1 1 1 2 3 4 5 6\n$EndElements\n"
+    names = '$PhysicalNames\n1\n3 1 "fluid"\n$EndPhysicalNames\n'
+    for i, text in enumerate([head + nodes + prism,                     # no physical names
+                              head + names + nodes + prism,             # prisms: no kernel
+                              head + names + nodes + prism.replace(" 6 2 1 1", " 4 2 2 1")]):   # unnamed tag
+        p = tmp_path / ("m%d.msh" % i)
+        p.write_text(text)
+        with pytest.raises(ValueError):
+            meshio.read_msh(str(p))
+    p = tmp_path / "ok.msh"
+    p.write_text(head + names + nodes + "$Elements\n1\n7 4 2 1 1 1 2 3 4\n$EndElements\n")
+    m = meshio.read_mesh(str(p))
+    assert m.blocks["Tetra3D"].tolist() == [[0, 1, 2, 3]] and m.groups == {"fluid": [7]}
+    with pytest.raises(ValueError):
+        meshio.read_mesh(str(tmp_path / "ok.vtk"))
