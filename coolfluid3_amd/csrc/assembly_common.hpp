@@ -23,6 +23,9 @@ struct AsmArgs {
   const uint8_t* inc_pos;   // per incidence: staging row of its block in pass b (byte b)
   const uint8_t* blk_beg;   // per block: first staging row of pass p (byte p)
   const uint16_t* blk_acc;  // per block: first accumulator thread (low byte), accumulator count (high byte)
+  const uint32_t* wg_head;  // node staging: per workgroup r0, r1, i0, i1, b0, b1, n0, n1
+  const uint32_t* wg_nodes; // node staging: distinct nodes of each workgroup
+  const uint32_t* inc_loc;  // node staging: per incidence the local indices of its element's nodes
   void* values;
   double* rhs;
   const double* xt;    // matrix-free residual: x with constrained columns set to their first value
@@ -79,14 +82,19 @@ __device__ __forceinline__ void load_conn(const uint32_t* conn, uint32_t e, uint
   }
 }
 
-// nodal unknowns (u.., p) of node nd in arithmetic type R
+// nodal unknowns (u.., p) of record nd of st (the packed state, or its LDS copy) in arithmetic type R
+template<int NEQ, typename R>
+__device__ __forceinline__ void load_state(const double4* st, uint32_t nd, R xb[NEQ])
+{
+  double s[4];
+  unpack(st[nd], s);
+#pragma unroll
+  for(int c = 0; c < NEQ; ++c) xb[c] = (R)s[c];
+}
 template<int NEQ, typename R>
 __device__ __forceinline__ void load_state(const AsmArgs& g, uint32_t nd, R xb[NEQ])
 {
-  double s[4];
-  unpack(g.st[nd], s);
-#pragma unroll
-  for(int c = 0; c < NEQ; ++c) xb[c] = (R)s[c];
+  load_state<NEQ, R>(g.st, nd, xb);
 }
 
 // LDS row access in 16-byte pieces

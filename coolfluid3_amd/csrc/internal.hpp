@@ -79,6 +79,7 @@ constexpr int kMaxRedBlocks = 1024;
 // assembly workgroup: at most kAsmThreads incidences (one per thread) and kAsmThreads CSR blocks
 // (one owner thread per block); the row schedule of the system respects both
 constexpr int kAsmThreads = 128;
+constexpr uint32_t kMaxWgNodes = 255;   // distinct nodes per staged workgroup (one-byte local indices)
 
 // ---------------------------------------------------------------------------------------------
 // mesh / system / fields
@@ -119,6 +120,13 @@ struct Lss {
   DevBuf inc_pos;       // uint8 n_elems*k*pos_stride
   DevBuf blk_beg;       // uint8 n_blocks*4
   DevBuf blk_acc;       // uint16 n_blocks: first accumulator thread (low byte) and count (high byte)
+  // node staging of the simplex kernels: each workgroup copies the records of the distinct nodes its
+  // elements touch into LDS once, and its incidences address them by local index
+  bool staged = false;  // built (simplices whose workgroups touch at most kMaxWgNodes nodes)
+  uint32_t wg_nodes_max = 0;
+  DevBuf wg_head;       // uint32 8 per workgroup: rows [r0, r1), incidences [i0, i1), blocks [b0, b1), nodes [n0, n1)
+  DevBuf wg_nodes;      // uint32: the distinct nodes of each workgroup, ascending
+  DevBuf inc_loc;       // uint32 n_elems*k: local indices (one byte per element node) of the element's nodes
   // Dirichlet conditions applied since the last reset (needed by the matrix-free FP64 residual):
   // per dof the first value (column elimination) and the last value (row / solution)
   std::unordered_map<uint64_t, std::pair<double, double> > bc_host;

@@ -226,6 +226,89 @@ __global__ __launch_bounds__(kAsmThreads) void k_stage_maps(int passes, int pos_
   }
 }
 
+// Node staging (simplices): the distinct nodes of the elements of one assembly workgroup, and per
+// incidence the local indices of its element's nodes in that list.  One workgroup per assembly
+// workgroup: the (at most kAsmThreads * 4) node ids are bitonic-sorted in LDS and de-duplicated.
+// FILL = false writes the count of each workgroup; FILL = true writes the list at nodes_ptr[wg] and
+// the local indices (byte m of inc_loc = local index of element node m).
+template<bool FILL>
+__global__ __launch_bounds__(kAsmThreads) void k_wg_nodes(int k, const uint32_t* __restrict__ wg_rows,
+                                                         const uint32_t* __restrict__ inc_ptr,
+                                                         const uint32_t* __restrict__ inc,
+                                                         const uint32_t* __restrict__ conn,
+                                                         const uint32_t* __restrict__ nodes_ptr,
+                                                         uint32_t* __restrict__ out, uint32_t* __restrict__ inc_loc)
+{
+  constexpr uint32_t S = kAsmThreads * 4;
+  __shared__ uint32_t s[S];
+  __shared__ uint32_t s_u[S];
+  __shared__ uint32_t s_cnt[kAsmThreads + 1];
+  const uint32_t r0 = wg_rows[blockIdx.x], r1 = wg_rows[blockIdx.x + 1];
+  const uint32_t i0 = inc_ptr[r0], ninc = inc_ptr[r1] - i0;
+  const uint32_t t = threadIdx.x;
+  uint32_t mine[4] = {0xffffffffu, 0xffffffffu, 0xffffffffu, 0xffffffffu};
+  if(t < ninc) {
+    const uint64_t e = inc[i0 + t] / (uint32_t)k;
+    for(int m = 0; m < k; ++m) mine[m] = conn[e * k + m];
+  }
+  for(int m = 0; m < 4; ++m) s[m * kAsmThreads + t] = mine[m];
+  __syncthreads();
+  // bitonic sort, ascending (unused slots hold 0xffffffff and sort last)
+  for(uint32_t size = 2; size <= S; size <<= 1)
+    for(uint32_t stride = size >> 1; stride > 0; stride >>= 1) {
+      for(uint32_t p = t; p < S / 2; p += kAsmThreads) {
+        const uint32_t i = 2 * p - (p & (stride - 1)), j = i + stride;   // i has the stride bit clear
+        const uint32_t a = s[i], b = s[j];
+        if((a > b) == ((i & size) == 0)) { s[i] = b; s[j] = a; }
+      }
+      __syncthreads();
+    }
+  // distinct values: thread t takes positions 4t .. 4t+3
+  uint32_t f = 0;
+  for(uint32_t m = 0; m < 4; ++m) {
+    const uint32_t i = 4 * t + m;
+    if(s[i] != 0xffffffffu && (i == 0 || s[i] != s[i - 1])) f |= 1u << m;
+  }
+  s_cnt[t] = __popc(f);
+  __syncthreads();
+  if(t == 0) {
+    uint32_t run = 0;
+    for(uint32_t j = 0; j < kAsmThreads; ++j) {
+      const uint32_t c = s_cnt[j];
+      s_cnt[j] = run;
+      run += c;
+    }
+    s_cnt[kAsmThreads] = run;
+  }
+  __syncthreads();
+  const uint32_t total = s_cnt[kAsmThreads];
+  if constexpr(!FILL) {
+    if(t == 0) out[blockIdx.x] = total;
+  } else {
+    uint32_t o = s_cnt[t];
+    const uint32_t base = nodes_ptr[blockIdx.x];
+    for(uint32_t m = 0; m < 4; ++m)
+      if(f & (1u << m)) {
+        s_u[o] = s[4 * t + m];
+        out[base + o] = s[4 * t + m];
+        ++o;
+      }
+    __syncthreads();
+    if(t < ninc) {
+      uint32_t packed = 0;
+      for(int m = 0; m < k; ++m) {
+        uint32_t lo = 0, hi = total;   // s_u[lo] == mine[m]
+        while(hi - lo > 1) {
+          const uint32_t mid = (lo + hi) >> 1;
+          if(s_u[mid] <= mine[m]) lo = mid; else hi = mid;
+        }
+        packed |= lo << (8 * m);
+      }
+      inc_loc[i0 + t] = packed;
+    }
+  }
+}
+
 template<typename KeyT>
 void radix_sort(Context& c, DevBuf& in, DevBuf& out, uint64_t n, int end_bit)
 {
@@ -397,6 +480,48 @@ void build_element_structures(Lss& L)
                      contrib_ptr.as<uint32_t>(), contrib.as<uint16_t>(), acc_cnt.as<uint8_t>(),
                      L.inc_pos.as<uint8_t>(), L.blk_beg.as<uint8_t>(), L.blk_acc.as<uint16_t>());
   HIP_OK(hipGetLastError());
+
+  // ---- node staging (simplices) ----
+  L.staged = false;
+  if(!hexa && L.n_wg > 0) {
+    DevBuf counts;
+    counts.alloc(&c, sizeof(uint32_t) * L.n_wg);
+    hipLaunchKernelGGL(k_wg_nodes<false>, dim3(L.n_wg), dim3(kAsmThreads), 0, c.stream, (int)k,
+                       L.wg_rows.as<uint32_t>(), L.inc_ptr.as<uint32_t>(), L.inc.as<uint32_t>(), conn,
+                       (const uint32_t*)nullptr, counts.as<uint32_t>(), (uint32_t*)nullptr);
+    HIP_OK(hipGetLastError());
+    std::vector<uint32_t> cnt(L.n_wg), row_ptr(N + 1);
+    HIP_OK(hipMemcpyAsync(cnt.data(), counts.ptr, sizeof(uint32_t) * L.n_wg, hipMemcpyDeviceToHost, c.stream));
+    HIP_OK(hipMemcpyAsync(row_ptr.data(), L.row_ptr.ptr, sizeof(uint32_t) * (N + 1), hipMemcpyDeviceToHost, c.stream));
+    HIP_OK(hipStreamSynchronize(c.stream));
+    const uint32_t mx = *std::max_element(cnt.begin(), cnt.end());
+    if(mx <= kMaxWgNodes) {
+      std::vector<uint32_t> nptr(L.n_wg + 1), head((size_t)L.n_wg * 8);
+      nptr[0] = 0;
+      for(uint32_t w = 0; w < L.n_wg; ++w) nptr[w + 1] = nptr[w] + cnt[w];
+      for(uint32_t w = 0; w < L.n_wg; ++w) {
+        const uint32_t r0 = wg[w], r1 = wg[w + 1];
+        uint32_t* h = &head[(size_t)w * 8];
+        h[0] = r0; h[1] = r1; h[2] = inc_ptr[r0]; h[3] = inc_ptr[r1];
+        h[4] = row_ptr[r0]; h[5] = row_ptr[r1]; h[6] = nptr[w]; h[7] = nptr[w + 1];
+      }
+      DevBuf dptr;
+      dptr.alloc(&c, sizeof(uint32_t) * (L.n_wg + 1));
+      L.wg_head.alloc(&c, sizeof(uint32_t) * head.size());
+      L.wg_nodes.alloc(&c, sizeof(uint32_t) * std::max<uint32_t>(nptr[L.n_wg], 1));
+      L.inc_loc.alloc(&c, sizeof(uint32_t) * nek);
+      HIP_OK(hipMemcpyAsync(dptr.ptr, nptr.data(), sizeof(uint32_t) * nptr.size(), hipMemcpyHostToDevice, c.stream));
+      HIP_OK(hipMemcpyAsync(L.wg_head.ptr, head.data(), sizeof(uint32_t) * head.size(), hipMemcpyHostToDevice,
+                            c.stream));
+      hipLaunchKernelGGL(k_wg_nodes<true>, dim3(L.n_wg), dim3(kAsmThreads), 0, c.stream, (int)k,
+                         L.wg_rows.as<uint32_t>(), L.inc_ptr.as<uint32_t>(), L.inc.as<uint32_t>(), conn,
+                         dptr.as<uint32_t>(), L.wg_nodes.as<uint32_t>(), L.inc_loc.as<uint32_t>());
+      HIP_OK(hipGetLastError());
+      HIP_OK(hipStreamSynchronize(c.stream));   // dptr and the host vectors go out of scope
+      L.wg_nodes_max = mx;
+      L.staged = true;
+    }
+  }
   HIP_OK(hipStreamSynchronize(c.stream));
 }
 

@@ -89,11 +89,25 @@ def duct_case():
     return case
 
 
+def scattered_tets_case(n=512, seed=7):
+    """Disjoint tetrahedra whose nodes are numbered a quarter of the mesh apart (tet k has nodes k,
+    k + n, k + 2n, k + 3n): a workgroup of 128 one-incidence rows touches 512 distinct nodes, above
+    the 255 of node staging, so the FP64 kernel runs unstaged on this mesh."""
+    rng = np.random.default_rng(seed)
+    base = rng.uniform(0.0, 10.0, (n, 3))
+    unit = np.array([[0.0, 0.0, 0.0], [1.0, 0.0, 0.0], [0.0, 1.0, 0.0], [0.0, 0.0, 1.0]])
+    coords = np.concatenate([base + unit[m] * rng.uniform(0.5, 1.0, (n, 1)) for m in range(4)])
+    conn = np.stack([np.arange(n) + m * n for m in range(4)], axis=1).astype(np.uint32)
+    u0 = rng.uniform(-1.0, 1.0, (4 * n, 3))
+    return cases.Case("scattered_tets", cabi.TETRA3D, coords, conn, None, 1e-2, 1e-2, u0=u0, p0=np.zeros(4 * n))
+
+
 CASES = {
     "tri8": lambda: cases.lid_cavity_2d(8),
     "ring": ring_case,
     "rect": rectangle_case,
     "duct": duct_case,
+    "scattered": scattered_tets_case,
     "tet5": lambda: cases.inflow_cube_tets(5),
     "tet12": lambda: cases.inflow_cube_tets(12),
     "tet12s": lambda: cases.inflow_cube_tets(12, skew=True),
@@ -102,7 +116,7 @@ CASES = {
 }
 
 
-@pytest.mark.parametrize("name", ["tri8", "ring", "rect", "tet5", "tet12", "hex5", "duct"])
+@pytest.mark.parametrize("name", ["tri8", "ring", "rect", "tet5", "tet12", "hex5", "duct", "scattered"])
 def test_sparsity_and_slot_map_bit_exact(ctx, oracle, name):
     case = CASES[name]()
     print("case", name, flush=True)
@@ -118,6 +132,7 @@ def test_sparsity_and_slot_map_bit_exact(ctx, oracle, name):
     ("tri8", True, cabi.SUPG_METRIC, 1.0), ("tri8", False, cabi.SUPG_METRIC, 0.5),
     ("ring", True, cabi.SUPG_METRIC, 1.0), ("ring", False, cabi.SUPG_TEZDUYAR, 1.0),
     ("rect", True, cabi.SUPG_METRIC, 1.0), ("duct", False, cabi.SUPG_METRIC, 1.0),
+    ("scattered", True, cabi.SUPG_METRIC, 1.0),
     ("tet5", True, cabi.SUPG_METRIC, 1.0), ("tet5", False, cabi.SUPG_METRIC, 0.7),
     ("tet5", False, cabi.SUPG_TEZDUYAR, 1.0), ("tet12", True, cabi.SUPG_METRIC, 1.0),
     ("tet12s", True, cabi.SUPG_METRIC, 1.0),
