@@ -84,6 +84,8 @@ def lib():
         "cf3hip_mesh_destroy": [_vp],
         "cf3hip_mesh_set_coordinates": [_vp, _d],
         "cf3hip_lss_create": [_vp, _vp, C.c_int, C.POINTER(_vp)],
+        "cf3hip_lss_create_periodic": [_vp, _vp, C.c_int, _u32, C.POINTER(C.c_uint8), C.POINTER(_vp)],
+        "cf3hip_lss_get_p2m": [_vp, _u32],
         "cf3hip_lss_create_from_graph": [_vp, C.c_uint32, C.c_int, _u32, _u32, C.POINTER(_vp)],
         "cf3hip_lss_destroy": [_vp],
         "cf3hip_lss_info": [_vp, _u32, C.POINTER(C.c_int), C.POINTER(C.c_uint64)],
@@ -269,12 +271,20 @@ class Fields:
 class System:
     """The LSS system (matrix + rhs + solution) of one mesh."""
 
-    def __init__(self, ctx, mesh=None, neq=None, graph=None):
+    def __init__(self, ctx, mesh=None, neq=None, graph=None, periodic=None):
+        """periodic: (links, active) per mesh node (cf3hip_lss_create_periodic); the system then has one
+        block row per unlinked node and p2m() maps mesh nodes to block rows."""
         self.ctx = ctx
         self.h = _vp()
         if mesh is not None:
             neq = neq or mesh.dim + 1
-            check(lib().cf3hip_lss_create(ctx.h, mesh.h, neq, C.byref(self.h)))
+            if periodic is None:
+                check(lib().cf3hip_lss_create(ctx.h, mesh.h, neq, C.byref(self.h)))
+            else:
+                links = _u32a(periodic[0])
+                active = np.ascontiguousarray(periodic[1], dtype=np.uint8)
+                check(lib().cf3hip_lss_create_periodic(ctx.h, mesh.h, neq, _p(links, _u32),
+                                                       active.ctypes.data_as(C.POINTER(C.c_uint8)), C.byref(self.h)))
         else:
             start, conn, n_nodes = graph
             s, cn = _u32a(start), _u32a(conn)
@@ -292,6 +302,12 @@ class System:
         cols = np.zeros(self.n_blocks, dtype=np.uint32)
         check(lib().cf3hip_lss_get_graph(self.h, _p(start, _u32), _p(cols, _u32)))
         return start, cols
+
+    def p2m(self):
+        """Block row of every mesh node (the identity without periodic links)."""
+        m = np.zeros(self.mesh.n_nodes, dtype=np.uint32)
+        check(lib().cf3hip_lss_get_p2m(self.h, _p(m, _u32)))
+        return m
 
     def slots(self):
         m = self.mesh

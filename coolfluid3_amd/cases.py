@@ -32,6 +32,8 @@ class Case:
     bc_regions: list = field(default_factory=list)   # [(name, nodes, {eq: value or callable(coords)})]
     solver: str = "bicgstab"
     tol: float = 1e-10
+    periodic: tuple = None   # (links, active) per mesh node: cf3hip_lss_create_periodic
+    t: float = 0.0           # time at the start of the next step (time-dependent conditions)
 
     @property
     def dim(self):
@@ -160,6 +162,82 @@ def _group(case, nodes, eqs, vals):
         yield n_stack, e_stack, v_stack
 
 
+def periodic_links(coords, pairs):
+    """LinkPeriodicNodes::execute (cf3/mesh/actions/LinkPeriodicNodes.cpp:136-180) for each
+    (source nodes, destination nodes, translation) in order: source node s is linked to the destination
+    node d with |x_s + t - x_d|^2 < 1e-8 (the last such d; a later link of the same node replaces an
+    earlier one).  Returns (links, active) per mesh node."""
+    n = coords.shape[0]
+    links = np.zeros(n, dtype=np.uint32)
+    active = np.zeros(n, dtype=np.uint8)
+    for src, dst, t in pairs:
+        dst = np.asarray(dst)
+        for s in src:
+            d2 = ((coords[dst] - (coords[s] + np.asarray(t))) ** 2).sum(axis=1)
+            m = np.nonzero(d2 < 1e-8)[0]
+            if len(m):
+                links[s] = dst[m[-1]]
+                active[s] = 1
+    return links, active
+
+
+def periodic_p2m(links, active):
+    """Block row of every mesh node under periodic links, as create_map_data on one rank
+    (cf3/math/LSS/Trilinos/TrilinosDetail.cpp:147-227): the unlinked nodes in node order, a linked node
+    the row of the end of its chain of links."""
+    n = len(links)
+    p2m = np.zeros(n, dtype=np.uint32)
+    free = np.flatnonzero(active == 0)
+    p2m[free] = np.arange(len(free), dtype=np.uint32)
+    for i in np.flatnonzero(active):
+        f, hops = int(links[i]), 0
+        while active[f]:
+            f, hops = int(links[f]), hops + 1
+            if hops > n:
+                raise ValueError("periodic links form a cycle")
+        p2m[i] = p2m[f]
+    return p2m
+
+
+TG_UA, TG_VA, TG_D, TG_NU = 0.3, 0.2, 0.5, 0.001
+
+
+def taylor_green_exact(coords, t, ua=TG_UA, va=TG_VA, d=TG_D, nu=TG_NU):
+    """The translating Taylor-Green vortex of plugins/UFEM/test/atest-ufem-navier-stokes-taylor-green.py
+    (check_result, :289-291): velocity (N x 2) and pressure at time t."""
+    x, y = coords[:, 0], coords[:, 1]
+    e = np.exp(-2.0 * nu * np.pi ** 2 / d ** 2 * t)
+    u = ua - np.cos(np.pi / d * (x - ua * t)) * np.sin(np.pi / d * (y - va * t)) * e
+    v = va + np.sin(np.pi / d * (x - ua * t)) * np.cos(np.pi / d * (y - va * t)) * e
+    p = -0.25 * (np.cos(2 * np.pi / d * (x - ua * t)) + np.cos(2 * np.pi / d * (y - va * t))) * \
+        np.exp(-4.0 * nu * np.pi ** 2 / d ** 2 * t)
+    return np.stack([u, v], axis=1), p
+
+
+def taylor_green_tris(segments=32, dt=0.004, theta=0.5):
+    """The Taylor-Green acceptance case of plugins/UFEM/test/atest-ufem-navier-stokes-taylor-green.py
+    with triangles (its --elem=triag variant): the square [-0.5, 0.5]^2 in segments^2 cells split in two,
+    periodic right -> left and top -> bottom (:85-97), density 1 and dynamic viscosity 0.001 (:163-164),
+    the generic expression (use_specializations false, :183), the exact velocity and pressure at t = 0
+    (setup_ic, :124-141) and the exact pressure at t + dt imposed at the centre node (add_pressure_bc,
+    :166-169).  The reference bounds the errors after the fifth step by 1.4e-3 (u, v) and 3.3e-3 (p)."""
+    coords, conn = meshgen.rect_tris(segments, segments)
+    coords = coords - 0.5
+    x, y = coords[:, 0], coords[:, 1]
+    tol = 1e-12
+    right, left = np.flatnonzero(np.abs(x - 0.5) < tol), np.flatnonzero(np.abs(x + 0.5) < tol)
+    top, bottom = np.flatnonzero(np.abs(y - 0.5) < tol), np.flatnonzero(np.abs(y + 0.5) < tol)
+    links = periodic_links(coords, [(right, left, (-1.0, 0.0)), (top, bottom, (0.0, -1.0))])
+    center = np.flatnonzero((np.abs(x) < tol) & (np.abs(y) < tol)).astype(np.uint32)
+    if len(center) != 1:
+        raise ValueError("the centre (0, 0) must be a mesh node: use an even number of segments")
+    u0, p0 = taylor_green_exact(coords, 0.0)
+    case = Case("taylor_green_tris_%d" % segments, cabi.TRIAG2D, coords, conn, (segments, segments), TG_NU, dt,
+                theta=theta, u0=np.ascontiguousarray(u0), p0=p0, solver="gmres", tol=1e-12, periodic=links)
+    case.bc_regions = [("center", center, {2: lambda c, case=case: taylor_green_exact(c, case.t + case.dt)[1]})]
+    return case
+
+
 def mesh_case(path, etype_name=None, nu=1e-2, dt=1e-2, seed=DEFAULT_SEED, bcs=(), pin=None):
     """A case on a mesh file (meshio.read_mesh: .neu or .msh) with one element type: velocity U[-1,1] per
     node from the seed, p = 0.  bcs: [(boundary set, {eq: value or callable(coords)})], applied in that
@@ -194,7 +272,8 @@ class GpuCase:
         self.case = case
         self.mesh = cabi.Mesh(ctx, case.etype, case.coords, case.conn)
         self.fields = cabi.Fields(self.mesh)
-        self.system = cabi.System(ctx, self.mesh)
+        self.system = cabi.System(ctx, self.mesh, periodic=case.periodic)
+        self.p2m = self.system.p2m()   # block row of each mesh node (conditions are given by mesh node)
         self.precision = precision
         if precision == cabi.FP32:
             self.system.enable_fp32()
@@ -236,6 +315,7 @@ class GpuCase:
         FP64 values assembled from the same state, from a zero initial guess; `fallbacks` counts them."""
         s = self.system
         n, e, v = bc_lists(self.case, self.u, self.p)             # BoundaryConditions
+        n = self.p2m[n]
         s.dirichlet(n, e, v)
         if solver_params is None:
             if self.case.solver == "gmres" and self.precision == cabi.FP64:
@@ -260,6 +340,7 @@ class GpuCase:
         s.update(self.fields)                                     # UpdateActions
         self.u = self.fields.get(cabi.FIELD_U)
         self.p = self.fields.get(cabi.FIELD_P)
+        self.case.t += self.case.dt
         return st
 
     def close(self):

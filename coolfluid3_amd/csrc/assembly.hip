@@ -1015,6 +1015,8 @@ void assemble(Lss& L, Fields& F, const cf3hip_ns_params& prm)
 
 void ns_residual(Lss& L, Fields& F, const cf3hip_ns_params& prm, const double* x, double* r)
 {
+  if(L.periodic)
+    CF3_THROW(CF3HIP_ERR_NOT_IMPLEMENTED, "the matrix-free residual (precision form) takes systems without periodic links");
   AsmArgs g = make_args(L, F, prm);
   const Mesh& M = *L.mesh;
   Context& c = *L.ctx;
@@ -1059,6 +1061,7 @@ void batch_assemble(Batch& B, uint32_t n, Lss* const* L, Fields* const* F, const
     if(!Li.mesh || Li.mesh->etype != CF3HIP_TETRA3D || !prm[i].use_specializations)
       CF3_THROW(CF3HIP_ERR_NOT_IMPLEMENTED, "batched assembly takes tetrahedral systems with the specialised kernel");
     if(!Li.has_fp32) CF3_THROW(CF3HIP_ERR_NOT_CREATED, "batched FP32 assembly needs cf3hip_lss_enable_fp32");
+    if(Li.periodic) CF3_THROW(CF3HIP_ERR_NOT_IMPLEMENTED, "batched assembly takes systems without periodic links");
     if((uint64_t)wg + Li.n_wg > 0xffffffffull || (uint64_t)nodes + Li.n_nodes > 0xffffffffull)
       CF3_THROW(CF3HIP_ERR_LIMIT, "batch too large");
     // ZeroLSS: forget the conditions of the previous step (mask cleared on the batch stream)

@@ -369,6 +369,68 @@ int cf3hip_lss_create(cf3hip_context ctx, cf3hip_mesh mesh, int neq, cf3hip_lss*
   API_END
 }
 
+int cf3hip_lss_create_periodic(cf3hip_context ctx, cf3hip_mesh mesh, int neq, const uint32_t* links,
+                               const uint8_t* active, cf3hip_lss* out)
+{
+  API_BEGIN
+  REQUIRE(ctx && mesh && links && active && out, CF3HIP_ERR_INVALID_ARG, "null argument");
+  REQUIRE(neq >= 1 && neq <= 4, CF3HIP_ERR_INVALID_ARG, "neq must be in [1, 4]");
+  const uint32_t N = mesh->m.n_nodes;
+  // create_map_data (TrilinosDetail.cpp:147-227) on one rank: the unlinked nodes take the block rows
+  // in node order; a linked node takes the row of the end of its chain of links
+  std::vector<uint32_t> p2m(N);
+  uint32_t rows = 0;
+  for(uint32_t i = 0; i < N; ++i) {
+    REQUIRE(!active[i] || links[i] < N, CF3HIP_ERR_INVALID_ARG, "periodic link to a node outside the mesh");
+    if(!active[i]) p2m[i] = rows++;
+  }
+  REQUIRE(rows > 0, CF3HIP_ERR_INVALID_ARG, "every node is periodically linked");
+  for(uint32_t i = 0; i < N; ++i) {
+    if(!active[i]) continue;
+    uint32_t f = links[i], hops = 0;
+    while(active[f]) {
+      REQUIRE(++hops <= N, CF3HIP_ERR_INVALID_ARG, "periodic links form a cycle");
+      f = links[f];
+    }
+    p2m[i] = p2m[f];
+  }
+  auto* h = new cf3hip_lss_s();
+  Lss& L = h->l;
+  try {
+    L.ctx = &ctx->c;
+    L.mesh = &mesh->m;
+    L.neq = neq;
+    L.n_nodes = rows;
+    L.dofs = (uint64_t)rows * neq;
+    L.periodic = true;
+    L.p2m.alloc(L.ctx, sizeof(uint32_t) * N);
+    h2d(*L.ctx, L.p2m.ptr, p2m.data(), sizeof(uint32_t) * N);
+    build_element_structures(L);
+    alloc_system_storage(L);
+  } catch(...) {
+    delete h;
+    throw;
+  }
+  *out = h;
+  API_END
+}
+
+int cf3hip_lss_get_p2m(cf3hip_lss lss, uint32_t* p2m)
+{
+  API_BEGIN
+  REQUIRE(lss && p2m, CF3HIP_ERR_INVALID_ARG, "null argument");
+  const Lss& L = lss->l;
+  REQUIRE(L.mesh, CF3HIP_ERR_NOT_CREATED, "system was created without a mesh");
+  const uint32_t N = L.mesh->n_nodes;
+  if(L.periodic) {
+    HIP_OK(hipMemcpyAsync(p2m, L.p2m.ptr, sizeof(uint32_t) * N, hipMemcpyDeviceToHost, L.ctx->stream));
+    HIP_OK(hipStreamSynchronize(L.ctx->stream));
+  } else {
+    for(uint32_t i = 0; i < N; ++i) p2m[i] = i;
+  }
+  API_END
+}
+
 int cf3hip_lss_create_from_graph(cf3hip_context ctx, uint32_t n_nodes, int neq, const uint32_t* start,
                                  const uint32_t* conn, cf3hip_lss* out)
 {

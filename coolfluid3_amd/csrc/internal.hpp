@@ -97,7 +97,12 @@ struct Lss {
   Context* ctx = nullptr;
   Mesh* mesh = nullptr;   // null when created from a caller graph
   int neq = 0;
-  uint32_t n_nodes = 0;
+  uint32_t n_nodes = 0;   // block rows (the mesh nodes, less the periodically linked ones)
+  // periodic links (System::create with periodic_links_nodes/active): p2m[mesh node] = block row; a
+  // linked node shares the row of the end of its chain of links.  Elements are still gathered by mesh
+  // node (geometry and fields); only rows, columns and the solution update go through p2m.
+  bool periodic = false;
+  DevBuf p2m;             // uint32 mesh->n_nodes
   uint64_t n_blocks = 0;
   uint64_t dofs = 0;
   DevBuf row_ptr;       // uint32 n_nodes+1 (starting_indices)

@@ -18,22 +18,27 @@ __global__ void k_linearize(uint32_t n, const double* __restrict__ u, const doub
   ua[i] = 2.1875 * u[i] - 2.1875 * u1[i] + 1.3125 * u2[i] - 0.3125 * u3[i];
 }
 
+// p2m (null: identity) gives the block row of mesh node i under periodic links
 __global__ void k_update(uint32_t n, int dim, double* __restrict__ u, double* __restrict__ u1,
-                         double* __restrict__ u2, double* __restrict__ u3, const double* __restrict__ sol, int comp)
+                         double* __restrict__ u2, double* __restrict__ u3, const double* __restrict__ sol,
+                         const uint32_t* __restrict__ p2m, int comp)
 {
   const uint32_t i = blockIdx.x * blockDim.x + threadIdx.x;
   if(i >= n) return;
+  const uint64_t row = p2m ? p2m[i] : i;
   u3[i] = u2[i];
   u2[i] = u1[i];
   u1[i] = u[i];
-  u[i] += sol[(uint64_t)i * (dim + 1) + comp];
+  u[i] += sol[row * (dim + 1) + comp];
 }
 
-__global__ void k_update_p(uint32_t n, int dim, double* __restrict__ p, const double* __restrict__ sol)
+__global__ void k_update_p(uint32_t n, int dim, double* __restrict__ p, const double* __restrict__ sol,
+                           const uint32_t* __restrict__ p2m)
 {
   const uint32_t i = blockIdx.x * blockDim.x + threadIdx.x;
   if(i >= n) return;
-  p[i] += sol[(uint64_t)i * (dim + 1) + dim];
+  const uint64_t row = p2m ? p2m[i] : i;
+  p[i] += sol[row * (dim + 1) + dim];
 }
 
 }  // namespace
@@ -53,13 +58,15 @@ void ns_update(Lss& L, Fields& F)
 {
   const Mesh& M = *F.mesh;
   Context& c = *M.ctx;
-  if(L.n_nodes != M.n_nodes || L.neq != M.dim + 1) CF3_THROW(CF3HIP_ERR_INVALID_ARG, "system does not match the fields");
+  if(L.neq != M.dim + 1 || (L.periodic ? L.mesh != F.mesh : L.n_nodes != M.n_nodes))
+    CF3_THROW(CF3HIP_ERR_INVALID_ARG, "system does not match the fields");
+  const uint32_t* p2m = L.periodic ? L.p2m.as<uint32_t>() : nullptr;
   for(int d = 0; d < M.dim; ++d)
     hipLaunchKernelGGL(k_update, dim3(grid_for(M.n_nodes, 256)), dim3(256), 0, c.stream, M.n_nodes, M.dim,
                        F.u[d].as<double>(), F.u1[d].as<double>(), F.u2[d].as<double>(), F.u3[d].as<double>(),
-                       L.sol.as<double>(), d);
+                       L.sol.as<double>(), p2m, d);
   hipLaunchKernelGGL(k_update_p, dim3(grid_for(M.n_nodes, 256)), dim3(256), 0, c.stream, M.n_nodes, M.dim,
-                     F.p.as<double>(), L.sol.as<double>());
+                     F.p.as<double>(), L.sol.as<double>(), p2m);
   HIP_OK(hipGetLastError());
 }
 

@@ -309,6 +309,14 @@ __global__ __launch_bounds__(kAsmThreads) void k_wg_nodes(int k, const uint32_t*
   }
 }
 
+// out[i] = map[in[i]]
+__global__ void k_map_nodes(uint64_t n, const uint32_t* __restrict__ in, const uint32_t* __restrict__ map,
+                            uint32_t* __restrict__ out)
+{
+  const uint64_t i = (uint64_t)blockIdx.x * blockDim.x + threadIdx.x;
+  if(i < n) out[i] = map[in[i]];
+}
+
 template<typename KeyT>
 void radix_sort(Context& c, DevBuf& in, DevBuf& out, uint64_t n, int end_bit)
 {
@@ -342,9 +350,19 @@ void build_element_structures(Lss& L)
 {
   Context& c = *L.ctx;
   const Mesh& M = *L.mesh;
-  const uint64_t N = M.n_nodes, E = M.n_elems, k = M.k;
+  const uint64_t N = L.n_nodes, E = M.n_elems, k = M.k;   // N block rows
   const uint64_t npairs = E * k * k, nek = E * k;
+  // rows and columns of the element nodes (the mesh nodes, or their block rows under periodic links);
+  // the gathers of the kernels (and the node staging lists) keep the mesh nodes of M.conn
+  DevBuf dconn;
   const uint32_t* conn = M.conn.as<uint32_t>();
+  if(L.periodic) {
+    dconn.alloc(&c, sizeof(uint32_t) * nek);
+    hipLaunchKernelGGL(k_map_nodes, dim3(grid_for(nek, 256)), dim3(256), 0, c.stream, nek, M.conn.as<uint32_t>(),
+                       L.p2m.as<uint32_t>(), dconn.as<uint32_t>());
+    HIP_OK(hipGetLastError());
+    conn = dconn.as<uint32_t>();
+  }
 
   // ---- node graph ----
   {
@@ -487,7 +505,7 @@ void build_element_structures(Lss& L)
     DevBuf counts;
     counts.alloc(&c, sizeof(uint32_t) * L.n_wg);
     hipLaunchKernelGGL(k_wg_nodes<false>, dim3(L.n_wg), dim3(kAsmThreads), 0, c.stream, (int)k,
-                       L.wg_rows.as<uint32_t>(), L.inc_ptr.as<uint32_t>(), L.inc.as<uint32_t>(), conn,
+                       L.wg_rows.as<uint32_t>(), L.inc_ptr.as<uint32_t>(), L.inc.as<uint32_t>(), M.conn.as<uint32_t>(),
                        (const uint32_t*)nullptr, counts.as<uint32_t>(), (uint32_t*)nullptr);
     HIP_OK(hipGetLastError());
     std::vector<uint32_t> cnt(L.n_wg), row_ptr(N + 1);
@@ -514,7 +532,7 @@ void build_element_structures(Lss& L)
       HIP_OK(hipMemcpyAsync(L.wg_head.ptr, head.data(), sizeof(uint32_t) * head.size(), hipMemcpyHostToDevice,
                             c.stream));
       hipLaunchKernelGGL(k_wg_nodes<true>, dim3(L.n_wg), dim3(kAsmThreads), 0, c.stream, (int)k,
-                         L.wg_rows.as<uint32_t>(), L.inc_ptr.as<uint32_t>(), L.inc.as<uint32_t>(), conn,
+                         L.wg_rows.as<uint32_t>(), L.inc_ptr.as<uint32_t>(), L.inc.as<uint32_t>(), M.conn.as<uint32_t>(),
                          dptr.as<uint32_t>(), L.wg_nodes.as<uint32_t>(), L.inc_loc.as<uint32_t>());
       HIP_OK(hipGetLastError());
       HIP_OK(hipStreamSynchronize(c.stream));   // dptr and the host vectors go out of scope

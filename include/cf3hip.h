@@ -99,6 +99,20 @@ int cf3hip_mesh_set_coordinates(cf3hip_mesh mesh, const double* coords);
  *      CSR with neq x neq row-major blocks, node rows contiguous (semantics of
  *      TrilinosFEVbrMatrix, cf3/math/LSS/Trilinos/TrilinosFEVbrMatrix.cpp:71-134). ---- */
 int cf3hip_lss_create(cf3hip_context ctx, cf3hip_mesh mesh, int neq, cf3hip_lss* lss);
+/* System::create with periodic links (System.cpp:82-99 with periodic_links_nodes / periodic_links_active,
+ * the matrix map of cf3/math/LSS/Trilinos/TrilinosDetail.cpp:147-227; the links are those of
+ * cf3/mesh/actions/LinkPeriodicNodes.cpp:136-180).  links, active: host, one entry per mesh node; a node
+ * with active != 0 is linked to links[node].  The unlinked nodes take the block rows in node order, a
+ * linked node shares the row of the end of its chain of links (m_p2m of TrilinosFEVbrMatrix).  Element
+ * geometry and fields stay per mesh node; the sparsity, the scatter and cf3hip_ns_update go through the
+ * map.  Every index argument of the lss, matrix and vector calls on such a system is a block row, not a
+ * mesh node (the reference converts inside, with m_p2m): cf3hip_lss_get_p2m gives the map.  The
+ * matrix-free residual (cf3hip_ns_residual, the FP32 form of cf3hip_ns_solve) and batched assembly
+ * return CF3HIP_ERR_NOT_IMPLEMENTED on such a system. */
+int cf3hip_lss_create_periodic(cf3hip_context ctx, cf3hip_mesh mesh, int neq, const uint32_t* links,
+                               const uint8_t* active, cf3hip_lss* lss);
+/* block row of every mesh node (host, mesh n_nodes entries; the identity without periodic links) */
+int cf3hip_lss_get_p2m(cf3hip_lss lss, uint32_t* p2m);
 /* System::create(cp, neq, node_connectivity, starting_indices) for a caller-given node graph
  * (rows must be sorted and contain the diagonal).  No element scatter map: assembly needs lss_create. */
 int cf3hip_lss_create_from_graph(cf3hip_context ctx, uint32_t n_nodes, int neq,

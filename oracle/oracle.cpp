@@ -662,9 +662,12 @@ void gather_element(int et, const Uint* cn, const Real* coords, const Real* u, c
   }
 }
 
+// p2m (optional): matrix block row of each mesh node (periodic links; TrilinosDetail.cpp:147-227).  The
+// element is gathered by its mesh nodes; its rhs rows are those of p2m (the slots already are).
 int assemble_range(int et, const orc_ns_params* prm, const Real* coords, const Uint* conn,
                    const Real* u, const Real* p, const Real* u_adv, const Real* nu, const Real* g,
-                   const Uint* slots, Real* values, Real* rhs, const Uint* elems, size_t ne)
+                   const Uint* slots, Real* values, Real* rhs, const Uint* elems, size_t ne,
+                   const Uint* p2m = nullptr)
 {
   const int k = enodes(et), dim = edim(et), neq = dim+1, n = k*neq, nn = neq*neq;
   std::vector<Real> A(n*n), T(n*n), a(n), M(n*n), b(n);
@@ -685,7 +688,8 @@ int assemble_range(int et, const orc_ns_params* prm, const Real* coords, const U
             blk[al*neq+be] += M[(ra*neq+al)*n + cb*neq+be];
       }
       // TrilinosVector::add_rhs_values (TrilinosVector.cpp:189-203)
-      for(int al = 0; al < neq; ++al) rhs[(size_t)cn[ra]*neq+al] += b[ra*neq+al];
+      const Uint row = p2m ? p2m[cn[ra]] : cn[ra];
+      for(int al = 0; al < neq; ++al) rhs[(size_t)row*neq+al] += b[ra*neq+al];
     }
   }
   return 0;
@@ -1002,6 +1006,18 @@ int orc_ns_assemble(int et, const orc_ns_params* prm, uint32_t n_nodes, uint32_t
     }
   }
   return err;
+}
+
+int orc_ns_assemble_p2m(int et, const orc_ns_params* prm, uint32_t n_rows, uint32_t n_elems,
+                        const double* coords, const uint32_t* conn, const uint32_t* p2m, const double* u,
+                        const double* p, const double* u_adv, const double* nu, const double* g,
+                        const uint32_t* start, const uint32_t* slots, double* values, double* rhs)
+{
+  if(prm->theta < 0. || prm->theta > 1.) return -4;
+  const int neq = edim(et)+1;
+  std::fill(values, values + (size_t)start[n_rows]*neq*neq, 0.);
+  std::fill(rhs, rhs + (size_t)n_rows*neq, 0.);
+  return assemble_range(et, prm, coords, conn, u, p, u_adv, nu, g, slots, values, rhs, 0, n_elems, p2m);
 }
 
 void orc_set_row(int neq, const uint32_t* start, const uint32_t* cols, double* values,

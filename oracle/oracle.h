@@ -92,6 +92,14 @@ int orc_ns_assemble(int etype, const orc_ns_params* prm, uint32_t n_nodes, uint3
                     const double* u_adv, const double* nu, const double* g,
                     const uint32_t* start, const uint32_t* cols, const uint32_t* slots,
                     double* values, double* rhs, int nthreads);
+/* The same with periodic links: the system has n_rows block rows, p2m gives the block row of each mesh
+ * node (a linked node shares its master's row; System::create with periodic_links_nodes/active,
+ * cf3/math/LSS/Trilinos/TrilinosDetail.cpp:147-227); start/slots come from the sparsity of p2m[conn].
+ * Elements and fields are gathered by mesh node.  Serial. */
+int orc_ns_assemble_p2m(int etype, const orc_ns_params* prm, uint32_t n_rows, uint32_t n_elems,
+                        const double* coords, const uint32_t* conn, const uint32_t* p2m, const double* u,
+                        const double* p, const double* u_adv, const double* nu, const double* g,
+                        const uint32_t* start, const uint32_t* slots, double* values, double* rhs);
 
 /* symmetric Dirichlet, TrilinosFEVbrMatrix::symmetric_dirichlet (TrilinosFEVbrMatrix.cpp:482-553)
  * + System::dirichlet (System.cpp:240-255), applied sequentially in list order. */

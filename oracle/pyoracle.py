@@ -73,6 +73,8 @@ class Oracle:
         L.orc_ns_element_system.argtypes = [C.c_int, C.POINTER(NSParams)] + [_d] * 8
         L.orc_ns_assemble.argtypes = ([C.c_int, C.POINTER(NSParams), C.c_uint32, C.c_uint32, _d, _u32]
                                       + [_d] * 5 + [_u32] * 3 + [_d, _d, C.c_int])
+        L.orc_ns_assemble_p2m.argtypes = ([C.c_int, C.POINTER(NSParams), C.c_uint32, C.c_uint32, _d, _u32, _u32]
+                                          + [_d] * 5 + [_u32] * 2 + [_d, _d])
         L.orc_dirichlet.argtypes = [C.c_int, C.c_uint32, _u32, _u32, _d, _d, _d, C.c_uint32, _u32, _u32, _d]
         L.orc_set_row.argtypes = [C.c_int, _u32, _u32, _d, C.c_uint32, C.c_uint32, C.c_double, C.c_double]
         L.orc_spmv.argtypes = [C.c_int, C.c_uint32, _u32, _u32, _d, C.c_double, _d, C.c_double, _d]
@@ -176,6 +178,23 @@ class Oracle:
                                        _p(_u32a(slots), _u32), _p(values), _p(rhs), nthreads)
         if err:
             raise RuntimeError(f"orc_ns_assemble error {err}")
+        return values, rhs
+
+    def ns_assemble_p2m(self, etype, prm, coords, conn, p2m, u, p, u_adv, nu, start, slots, g=None):
+        """Assembly with periodic links: p2m maps mesh nodes to block rows (len(start) - 1 of them)."""
+        dim = DIM[etype]
+        neq = dim + 1
+        n_rows = len(start) - 1
+        conn = _u32a(conn)
+        values = np.zeros(int(start[-1]) * neq * neq)
+        rhs = np.zeros(n_rows * neq)
+        gg = _f64(g) if g is not None else np.zeros((coords.shape[0], dim))
+        err = self.lib.orc_ns_assemble_p2m(etype, C.byref(prm), n_rows, conn.shape[0], _p(_f64(coords)),
+                                           _p(conn, _u32), _p(_u32a(p2m), _u32), _p(_f64(u)), _p(_f64(p)),
+                                           _p(_f64(u_adv)), _p(_f64(nu)), _p(gg), _p(_u32a(start), _u32),
+                                           _p(_u32a(slots), _u32), _p(values), _p(rhs))
+        if err:
+            raise RuntimeError(f"orc_ns_assemble_p2m error {err}")
         return values, rhs
 
     def dirichlet(self, neq, start, cols, values, rhs, sol, bc_node, bc_eq, bc_val):

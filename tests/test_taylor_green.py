@@ -1,0 +1,116 @@
+"""Periodic links and the Taylor-Green acceptance test of the reference
+(plugins/UFEM/test/atest-ufem-navier-stokes-taylor-green.py, triangle variant).
+
+The oracle run is pinned by the analytic solution: after the fifth step the reference requires the
+velocity error to stay below 1.4e-3 and the pressure error below 3.3e-3 (:315-321).  The device run
+follows the oracle run step by step (sparsity bit-exact, FP64 values within 1e-12 row-relative,
+fields within 1e-8 after each step: both solve to 1e-12) and meets the same bounds.
+"""
+import numpy as np
+import pytest
+
+from coolfluid3_amd import cabi, cases
+from oracle import pyoracle
+
+STEPS = 5
+U_BOUND, P_BOUND = 1.4e-3, 3.3e-3   # the reference's bounds after step 4 (iteration > 3)
+
+
+def oracle_run(case, steps=STEPS, record=None):
+    """The implicit steps of the case in the oracle: LinearizeU, assembly through p2m, the centre
+    pressure condition (increment form), GMRES(30)+ILU0 to 1e-12, UpdateActions through p2m."""
+    o = pyoracle.get()
+    p2m = cases.periodic_p2m(*case.periodic)
+    rows = int(p2m.max()) + 1
+    dconn = p2m[case.conn]
+    start, cols, slots = o.build_sparsity(dconn, rows)
+    U = np.ascontiguousarray(case.u0.copy())
+    P = case.p0.copy()
+    U1, U2, U3 = U.copy(), U.copy(), U.copy()
+    nu = np.full(case.n_nodes, case.nu)
+    prm = pyoracle.ns_params(rho=case.rho, dt=case.dt, theta=case.theta, use_specializations=False)
+    sol = np.zeros(rows * 3)
+    t = 0.0
+    errors = []
+    for step in range(steps):
+        ua = o.linearize_u(U, U1, U2, U3)
+        vals, rhs = o.ns_assemble_p2m(pyoracle.TRIAG2D, prm, case.coords, case.conn, p2m, U, P, ua, nu, start, slots)
+        if record is not None:
+            record.append((vals.copy(), rhs.copy()))
+        case.t = t
+        n, e, v = cases.bc_lists(case, U, P)
+        o.dirichlet(3, start, cols, vals, rhs, sol, p2m[n], e, v)
+        sol, st = o.gmres_ilu0(3, start, cols, vals, rhs, sol, restart=30, tol=1e-12)
+        assert st.converged
+        o.update(U, P, U1, U2, U3, sol.reshape(-1, 3)[p2m].ravel())
+        t += case.dt
+        ue, pe = cases.taylor_green_exact(case.coords, t)
+        errors.append((np.abs(U - ue).max(axis=0), np.abs(P - pe).max(), U.copy(), P.copy()))
+    case.t = 0.0
+    return (start, cols, slots, p2m), errors
+
+
+def test_periodic_links_and_map():
+    """Right -> left then top -> bottom: 1089 nodes, 1024 block rows; the four corners share the row
+    of (-0.5, -0.5) through a chain of two links; every linked node sits a lattice vector from its row's
+    node; a cycle of links is rejected."""
+    case = cases.taylor_green_tris(32)
+    links, active = case.periodic
+    assert active.sum() == 65 and case.n_nodes == 1089
+    p2m = cases.periodic_p2m(links, active)
+    assert p2m.max() + 1 == 1024
+    c = case.coords
+    free = np.flatnonzero(active == 0)
+    master = free[p2m]                                     # the unlinked node of each node's row
+    shift = c - c[master]
+    assert np.allclose(shift, np.round(shift)) and np.all(np.isin(np.round(shift), (0.0, 1.0)))
+    corners = np.flatnonzero((np.abs(c[:, 0]) == 0.5) & (np.abs(c[:, 1]) == 0.5))
+    assert len(set(p2m[corners].tolist())) == 1 and np.allclose(c[master[corners[0]]], (-0.5, -0.5))
+    with pytest.raises(ValueError):
+        cases.periodic_p2m(np.array([1, 0], np.uint32), np.array([1, 1], np.uint8))
+
+
+def test_taylor_green_oracle_meets_reference_bounds():
+    """The oracle (periodic assembly, generic P1 expression, theta = 0.5, dt = 0.004, 32 segments)
+    against the exact vortex: the reference's bounds after the fifth step."""
+    case = cases.taylor_green_tris(32)
+    _, errors = oracle_run(case)
+    for step, (eu, ep, _, _) in enumerate(errors, 1):
+        print(f"step {step}: max error u {eu[0]:.3e} v {eu[1]:.3e} p {ep:.4e}")
+    eu, ep, _, _ = errors[-1]
+    assert eu.max() < U_BOUND and ep < P_BOUND
+    assert errors[-1][0].max() > errors[0][0].max()        # the error grows with time (not a trivial pass)
+
+
+@pytest.mark.gpu
+def test_taylor_green_device_follows_oracle():
+    case = cases.taylor_green_tris(32)
+    rec = []
+    (start, cols, slots, p2m), errors = oracle_run(case, record=rec)
+    ctx = cabi.Context(0)
+    g = cases.GpuCase(ctx, case, use_specializations=False)
+    g.prm.theta = case.theta
+    assert np.array_equal(g.p2m, p2m)
+    s, c = g.system.graph()
+    assert np.array_equal(s, start) and np.array_equal(c, cols)
+    assert np.array_equal(g.system.slots(), slots)
+    sp = cabi.solver_params(cabi.GMRES_ILU0, tol=1e-12, restart=30)
+    for step in range(STEPS):
+        g.fields.linearize()
+        g.assemble(cabi.FP64)
+        vals, rhs = rec[step]
+        v = g.system.values()
+        err = np.abs(v - vals).max() / np.abs(vals).max()
+        assert err <= 1e-12, (step, err)
+        assert np.abs(g.system.vector(cabi.RHS) - rhs).max() <= 1e-12 * np.abs(rhs).max()
+        st = g._solve_update(sp, True)
+        eu_ref, ep_ref, u_ref, p_ref = errors[step]
+        du = np.abs(g.u - u_ref).max() / np.abs(u_ref).max()
+        dp = np.abs(g.p - p_ref).max() / np.abs(p_ref).max()
+        ue, pe = cases.taylor_green_exact(case.coords, case.t)
+        print(f"step {step + 1}: GMRES {st.iterations} it, device vs oracle u {du:.2e} p {dp:.2e}, "
+              f"max error u {np.abs(g.u - ue).max():.3e} p {np.abs(g.p - pe).max():.4e}", flush=True)
+        assert du <= 1e-8 and dp <= 1e-8
+    assert np.abs(g.u - ue).max() < U_BOUND and np.abs(g.p - pe).max() < P_BOUND
+    g.close()
+    ctx.close()
