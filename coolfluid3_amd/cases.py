@@ -214,14 +214,23 @@ def taylor_green_exact(coords, t, ua=TG_UA, va=TG_VA, d=TG_D, nu=TG_NU):
     return np.stack([u, v], axis=1), p
 
 
-def taylor_green_tris(segments=32, dt=0.004, theta=0.5):
+def taylor_green_case(segments=32, dt=0.004, theta=0.5, element="quad"):
     """The Taylor-Green acceptance case of plugins/UFEM/test/atest-ufem-navier-stokes-taylor-green.py
-    with triangles (its --elem=triag variant): the square [-0.5, 0.5]^2 in segments^2 cells split in two,
-    periodic right -> left and top -> bottom (:85-97), density 1 and dynamic viscosity 0.001 (:163-164),
-    the generic expression (use_specializations false, :183), the exact velocity and pressure at t = 0
-    (setup_ic, :124-141) and the exact pressure at t + dt imposed at the centre node (add_pressure_bc,
-    :166-169).  The reference bounds the errors after the fifth step by 1.4e-3 (u, v) and 3.3e-3 (p)."""
-    coords, conn = meshgen.rect_tris(segments, segments)
+    (plugins/UFEM/test/CMakeLists.txt:94-97 runs it with --dt=0.004 --elem=quad --segs=32 --theta=0.5
+    --tsteps=5): the square [-0.5, 0.5]^2 in segments^2 quadrilaterals, or each split in two triangles
+    (--elem=triag), periodic right -> left and top -> bottom (:85-97), density 1 and dynamic viscosity
+    0.001 (:163-164), the generic expression (use_specializations false, :183), the exact velocity and
+    pressure at t = 0 (setup_ic, :124-141) and the exact pressure at t + dt imposed at the centre node
+    (add_pressure_bc, :166-169).  The reference bounds the errors after the fifth step by 1.4e-3 (u, v)
+    and 3.3e-3 (p)."""
+    if element == "quad":
+        coords, conn = meshgen.rect_quads(segments, segments)
+        etype = cabi.QUAD2D
+    elif element == "triag":
+        coords, conn = meshgen.rect_tris(segments, segments)
+        etype = cabi.TRIAG2D
+    else:
+        raise ValueError("element is 'quad' or 'triag'")
     coords = coords - 0.5
     x, y = coords[:, 0], coords[:, 1]
     tol = 1e-12
@@ -232,7 +241,7 @@ def taylor_green_tris(segments=32, dt=0.004, theta=0.5):
     if len(center) != 1:
         raise ValueError("the centre (0, 0) must be a mesh node: use an even number of segments")
     u0, p0 = taylor_green_exact(coords, 0.0)
-    case = Case("taylor_green_tris_%d" % segments, cabi.TRIAG2D, coords, conn, (segments, segments), TG_NU, dt,
+    case = Case("taylor_green_%s_%d" % (element, segments), etype, coords, conn, (segments, segments), TG_NU, dt,
                 theta=theta, u0=np.ascontiguousarray(u0), p0=p0, solver="gmres", tol=1e-12, periodic=links)
     case.bc_regions = [("center", center, {2: lambda c, case=case: taylor_green_exact(c, case.t + case.dt)[1]})]
     return case

@@ -15,7 +15,7 @@
 //                   written to LDS rows sorted by (CSR block slot, incidence): the block of incidence
 //                   q and column local node b goes to row inc_pos[q][b]; the rows of block s in pass
 //                   p are [blk_beg[s][p], blk_beg[s+1][p]).  Simplices stage one pass per column
-//                   local node b (p = b); hexahedra stage all (q, b) at once.
+//                   local node b (p = b); Q1 cells (hexahedra, quadrilaterals) stage all (q, b) at once.
 //   blk_acc         accumulator threads of each block (first index within the workgroup, count):
 //                   c threads share the rows of a block (thread h takes rows h, h+c, ...) so that no
 //                   thread sums more than kAccRows rows per pass; their partial sums are added in
@@ -422,9 +422,9 @@ void build_element_structures(Lss& L)
     HIP_OK(hipGetLastError());
   }
 
-  const bool hexa = M.etype == CF3HIP_HEXA3D;
-  L.passes = hexa ? 1 : (int)k;
-  L.pos_stride = hexa ? 8 : 4;
+  const bool q1 = M.etype == CF3HIP_HEXA3D || M.etype == CF3HIP_QUAD2D;   // one lane per column node
+  L.passes = q1 ? 1 : (int)k;
+  L.pos_stride = q1 ? (int)k : 4;
 
   // ---- per-slot contribution lists sorted by (slot, b, q) ----
   DevBuf contrib_ptr, contrib, acc_cnt, row_acc;
@@ -458,9 +458,9 @@ void build_element_structures(Lss& L)
   HIP_OK(hipMemcpyAsync(inc_ptr.data(), L.inc_ptr.ptr, sizeof(uint32_t) * (N + 1), hipMemcpyDeviceToHost, c.stream));
   HIP_OK(hipMemcpyAsync(racc.data(), row_acc.ptr, sizeof(uint32_t) * N, hipMemcpyDeviceToHost, c.stream));
   HIP_OK(hipStreamSynchronize(c.stream));
-  // a workgroup holds at most inc_limit incidences (simplices: one incidence per thread; hexahedra:
+  // a workgroup holds at most inc_limit incidences (simplices: one incidence per thread; Q1 cells:
   // one (incidence, column node) pair per thread) and kAsmThreads accumulator threads
-  const uint64_t inc_limit = hexa ? kAsmThreads / k : kAsmThreads;
+  const uint64_t inc_limit = q1 ? kAsmThreads / k : kAsmThreads;
   std::vector<uint32_t> wg;
   wg.push_back(0);
   uint32_t r = 0;
@@ -501,7 +501,7 @@ void build_element_structures(Lss& L)
 
   // ---- node staging (simplices) ----
   L.staged = false;
-  if(!hexa && L.n_wg > 0) {
+  if(!q1 && L.n_wg > 0) {
     DevBuf counts;
     counts.alloc(&c, sizeof(uint32_t) * L.n_wg);
     hipLaunchKernelGGL(k_wg_nodes<false>, dim3(L.n_wg), dim3(kAsmThreads), 0, c.stream, (int)k,

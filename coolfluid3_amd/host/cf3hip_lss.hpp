@@ -69,8 +69,8 @@ class Mesh {
 public:
   Mesh(Context& ctx, int etype, const std::vector<double>& coords, const std::vector<uint32_t>& conn)
   {
-    const int k = etype == CF3HIP_TRIAG2D ? 3 : (etype == CF3HIP_TETRA3D ? 4 : 8);
-    const int dim = etype == CF3HIP_TRIAG2D ? 2 : 3;
+    const int k = etype == CF3HIP_TRIAG2D ? 3 : (etype == CF3HIP_HEXA3D ? 8 : 4);
+    const int dim = etype == CF3HIP_TRIAG2D || etype == CF3HIP_QUAD2D ? 2 : 3;
     if(coords.size() % dim || conn.size() % k) throw BadValue(CF3HIP_ERR_INVALID_ARG, "mesh array sizes");
     check(cf3hip_mesh_create(ctx.handle(), etype, (uint32_t)(coords.size() / dim), (uint32_t)(conn.size() / k),
                              coords.data(), conn.data(), &h_));

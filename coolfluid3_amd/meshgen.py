@@ -33,6 +33,18 @@ def uniform(seed, n, a, b, offset=0):
     return a + (b - a) * u
 
 
+def rect_quads(nx, ny, lx=1.0, ly=1.0):
+    """nx x ny quadrilaterals on [0, lx] x [0, ly], nodes numbered x fastest, cells counter-clockwise from
+    their lower-left node (the cf3 Quad2D node order)."""
+    j, i = np.meshgrid(np.arange(ny + 1), np.arange(nx + 1), indexing="ij")
+    coords = np.stack([i.ravel() * (lx / nx), j.ravel() * (ly / ny)], axis=1).astype(np.float64)
+    jj, ii = np.meshgrid(np.arange(ny), np.arange(nx), indexing="ij")
+    n0 = (jj * (nx + 1) + ii).ravel()
+    n3 = ((jj + 1) * (nx + 1) + ii).ravel()
+    conn = np.stack([n0, n0 + 1, n3 + 1, n3], axis=1).astype(np.uint32)
+    return coords, conn
+
+
 def rect_tris(nx, ny, lx=1.0, ly=1.0):
     j, i = np.meshgrid(np.arange(ny + 1), np.arange(nx + 1), indexing="ij")
     coords = np.stack([i.ravel() * (lx / nx), j.ravel() * (ly / ny)], axis=1).astype(np.float64)

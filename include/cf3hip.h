@@ -40,6 +40,7 @@ extern "C" {
 #define CF3HIP_TRIAG2D 0
 #define CF3HIP_TETRA3D 1
 #define CF3HIP_HEXA3D  2
+#define CF3HIP_QUAD2D  3   /* LagrangeP1 Quad2D: generic expression, 2 x 2 Gauss points */
 
 /* SUPG tau variants (plugins/UFEM/src/UFEM/SUPG.hpp:62-74) */
 #define CF3HIP_SUPG_TEZDUYAR 0

@@ -33,15 +33,22 @@ namespace {
 typedef double Real;
 typedef uint32_t Uint;
 
-inline int enodes(int et) { return et == ORC_TRIAG2D ? 3 : (et == ORC_TETRA3D ? 4 : 8); }
-inline int edim(int et) { return et == ORC_TRIAG2D ? 2 : 3; }
+inline int enodes(int et) { return et == ORC_TRIAG2D ? 3 : (et == ORC_HEXA3D ? 8 : 4); }
+inline int edim(int et) { return et == ORC_TRIAG2D || et == ORC_QUAD2D ? 2 : 3; }
 
 // ---------------------------------------------------------------------------------------------
-// Shape functions: Triag.cpp:24-41, Tetra.cpp:24-48, Hexa.cpp:24-89
+// Shape functions: Triag.cpp:24-41, Tetra.cpp:24-48, Hexa.cpp:24-89, Quad.cpp (compute_value,
+// compute_gradient: node order (-1,-1), (1,-1), (1,1), (-1,1))
 // ---------------------------------------------------------------------------------------------
 void shape_values(int et, const Real* xi, Real* N)
 {
-  if(et == ORC_TRIAG2D) {
+  if(et == ORC_QUAD2D) {
+    const Real ksi = xi[0], eta = xi[1];
+    N[0] = 0.25 * (1.0 - ksi) * (1.0 - eta);
+    N[1] = 0.25 * (1.0 + ksi) * (1.0 - eta);
+    N[2] = 0.25 * (1.0 + ksi) * (1.0 + eta);
+    N[3] = 0.25 * (1.0 - ksi) * (1.0 + eta);
+  } else if(et == ORC_TRIAG2D) {
     N[0] = 1. - xi[0] - xi[1];
     N[1] = xi[0];
     N[2] = xi[1];
@@ -63,7 +70,14 @@ void shape_values(int et, const Real* xi, Real* N)
 // dN(xi_dir, node) row-major dim x k
 void shape_gradients(int et, const Real* xi, Real* dN)
 {
-  if(et == ORC_TRIAG2D) {
+  if(et == ORC_QUAD2D) {
+    const Real ksi = xi[0], eta = xi[1];
+    Real* K = dN; Real* E = dN + 4;
+    K[0] = 0.25 * (-1. + eta); E[0] = 0.25 * (-1. + ksi);
+    K[1] = 0.25 * ( 1. - eta); E[1] = 0.25 * (-1. - ksi);
+    K[2] = 0.25 * ( 1. + eta); E[2] = 0.25 * ( 1. + ksi);
+    K[3] = 0.25 * (-1. - eta); E[3] = 0.25 * ( 1. - ksi);
+  } else if(et == ORC_TRIAG2D) {
     const Real g[6] = {-1., 1., 0., -1., 0., 1.};
     std::memcpy(dN, g, sizeof(g));
   } else if(et == ORC_TETRA3D) {
@@ -90,7 +104,15 @@ void shape_gradients(int et, const Real* xi, Real* dN)
 // ---------------------------------------------------------------------------------------------
 void jacobian(int et, const Real* xi, const Real* X, Real* J)
 {
-  if(et == ORC_TRIAG2D) {
+  if(et == ORC_QUAD2D) {
+    // Quad2D::compute_jacobian with the JacobianCoefficients of Quad2D.hpp:71-96
+    const Real bx = 0.25*(-X[0] + X[2] + X[4] - X[6]), cx = 0.25*(-X[0] - X[2] + X[4] + X[6]);
+    const Real dx = 0.25*( X[0] - X[2] + X[4] - X[6]);
+    const Real by = 0.25*(-X[1] + X[3] + X[5] - X[7]), cy = 0.25*(-X[1] - X[3] + X[5] + X[7]);
+    const Real dy = 0.25*( X[1] - X[3] + X[5] - X[7]);
+    J[0] = bx + dx*xi[1]; J[1] = by + dy*xi[1];
+    J[2] = cx + dx*xi[0]; J[3] = cy + dy*xi[0];
+  } else if(et == ORC_TRIAG2D) {
     J[0] = X[2] - X[0]; J[1] = X[3] - X[1];
     J[2] = X[4] - X[0]; J[3] = X[5] - X[1];
   } else if(et == ORC_TETRA3D) {
@@ -236,6 +258,8 @@ Real hexa_det(const Real* m, const Real* X)
 Real volume(int et, const Real* X)
 {
   if(et == ORC_TRIAG2D) return 0.5 * triag_det(X);
+  if(et == ORC_QUAD2D)   // Quad2D::volume: half the cross product of the diagonals
+    return 0.5 * ((X[4] - X[0]) * (X[7] - X[3]) - (X[5] - X[1]) * (X[6] - X[2]));
   if(et == ORC_TETRA3D) return tetra_det(X) / 6.;
   const Real zero[3] = {0., 0., 0.};
   return 8 * hexa_det(zero, X);
@@ -247,6 +271,13 @@ Real volume(int et, const Real* X)
 // ---------------------------------------------------------------------------------------------
 int gauss(int et, int order, Real* xi, Real* w)
 {
+  if(et == ORC_QUAD2D) {   // GaussImplementation.hpp:131-148 (centre, w 4) and :566-600 (2 x 2 order)
+    if(order == 1) { xi[0] = xi[1] = 0.; w[0] = 4.; return 1; }
+    const Real x = 0.5773502691896257645091488;
+    const Real s[4][2] = {{x, x}, {-x, x}, {x, -x}, {-x, -x}};
+    for(int i = 0; i < 4; ++i) { xi[2*i] = s[i][0]; xi[2*i+1] = s[i][1]; w[i] = 1.; }
+    return 4;
+  }
   if(et == ORC_TRIAG2D) {
     if(order == 1) { xi[0] = xi[1] = 0.3333333333333333333333333; w[0] = 0.5; return 1; }
     const Real c[6] = {0.5, 0.0, 0.5, 0.5, 0.0, 0.5};

@@ -7,13 +7,15 @@ The reference's only comparison test of these kernels is commented out
   * METRIC tau on the unit reference tetrahedron, where gij is the identity;
   * specialised vs generic agreement for uniform fields, and the exact documented differences
     (stress-form diffusion, the vk/wk substitution of NavierStokesSpecializations.hpp:224-226,236-238);
-  * generic hexahedron terms against an independent 4x4x4 tensor Gauss integration on affine hexes;
+  * generic hexahedron terms against an independent 4x4x4 tensor Gauss integration on affine hexes,
+    and generic quadrilateral (Quad2D) terms against a 4x4 rule on parallelograms, with its shape
+    functions, Gauss rule, volume and METRIC tau checked against closed forms;
   * the combine step (p rows / theta, rhs formed before the division, node-major reorder).
 """
 import numpy as np
 import pytest
 
-from oracle.pyoracle import HEXA3D, SUPG_METRIC, SUPG_TEZDUYAR, TETRA3D, TRIAG2D, ns_params
+from oracle.pyoracle import HEXA3D, QUAD2D, SUPG_METRIC, SUPG_TEZDUYAR, TETRA3D, TRIAG2D, ns_params
 
 
 def p1_grads(nodes):
@@ -185,6 +187,99 @@ def test_generic_hex_vs_tensor_gauss(oracle, case):
     # the mass matrix of every velocity block sums to the volume (partition of unity)
     if case == 0:
         assert T[:8, :8].sum() == pytest.approx(abs(np.linalg.det(B)), rel=1e-13)
+
+
+# ---- generic quadrilateral (Quad2D) ----
+_QLOC = np.array([[-1, -1], [1, -1], [1, 1], [-1, 1.]])
+
+
+def q1_2d(xi):
+    N = np.prod(1 + _QLOC * xi, axis=1) / 4
+    dN = np.stack([_QLOC[:, 0] * (1 + _QLOC[:, 1] * xi[1]), _QLOC[:, 1] * (1 + _QLOC[:, 0] * xi[0])]) / 4
+    return N, dN
+
+
+def test_quad_shapes_gauss_volume(oracle):
+    """Quad.cpp shape functions: Kronecker at the nodes, partition of unity, gradients equal to finite
+    differences; the 2 x 2 rule (points (+-x, +-x) in the order of GaussImplementation.hpp:566-600,
+    w = 1) integrates the Jacobian determinant of a non-affine quadrilateral to its shoelace area, which
+    Quad2D::volume gives too."""
+    for j, node in enumerate(_QLOC):
+        assert np.allclose(oracle.shape_values(QUAD2D, node), np.eye(4)[j], atol=0)
+    rng = np.random.default_rng(5)
+    for _ in range(5):
+        xi = rng.uniform(-1, 1, 2)
+        N = oracle.shape_values(QUAD2D, xi)
+        assert abs(N.sum() - 1) < 1e-15 and np.allclose(N, q1_2d(xi)[0], rtol=1e-15)
+        h = 1e-6
+        fd = np.stack([(oracle.shape_values(QUAD2D, xi + h * e) - oracle.shape_values(QUAD2D, xi - h * e)) / (2 * h)
+                       for e in np.eye(2)])
+        assert np.allclose(oracle.shape_gradients(QUAD2D, xi), fd, atol=1e-9)
+    c, w = oracle.gauss_points(QUAD2D, 2)
+    x = 1 / np.sqrt(3)
+    assert np.allclose(c, [[x, x], [-x, x], [x, -x], [-x, -x]], rtol=1e-15) and np.all(w == 1)
+    X = np.array([[0.0, 0.0], [1.2, 0.1], [1.0, 0.9], [-0.1, 0.8]])
+    area = 0.5 * abs(sum(X[i, 0] * X[(i + 1) % 4, 1] - X[(i + 1) % 4, 0] * X[i, 1] for i in range(4)))
+    assert oracle.element_volume(QUAD2D, X) == pytest.approx(area, rel=1e-15)
+    integ = sum(wq * np.linalg.det(oracle.element_jacobian(QUAD2D, cq, X)) for cq, wq in zip(c, w))
+    assert integ == pytest.approx(area, rel=1e-14)
+
+
+def test_quad_tau_metric_square(oracle):
+    """METRIC tau on the square of side h: J = h/2 I at the centre, so gij = 4/h^2 I."""
+    h, dt, nu = 0.05, 0.01, 2e-3
+    X = (_QLOC + 1) / 2 * h
+    uc = np.array([0.7, -0.4])
+    u = np.tile(uc, (4, 1))
+    _, _, _, tau = oracle.ns_element(QUAD2D, ns_params(dt=dt, use_specializations=False), X, u, u, np.full(4, nu))
+    g = 4 / h ** 2
+    adv = g * (uc @ uc)
+    diff = nu ** 2 * 2 * g ** 2
+    assert tau[1] == pytest.approx(1 / np.sqrt(4 / dt ** 2 + adv + 16 * diff), rel=1e-14)
+    assert tau[0] == tau[1]
+    assert tau[2] == pytest.approx(np.sqrt(4 / dt ** 2 + adv + 16 * diff) / (2 * g), rel=1e-14)
+
+
+def quad_reference_terms(X, ua, nu, tau_ps, tau_su, tau_bulk, rho):
+    x, w = np.polynomial.legendre.leggauss(4)
+    k, n, P = 4, 12, 8
+    A = np.zeros((n, n)); T = np.zeros((n, n))
+    for i, a in enumerate(x):
+        for j, b in enumerate(x):
+            N, dN = q1_2d(np.array([a, b]))
+            J = dN @ X
+            G = np.linalg.solve(J, dN)
+            wq = w[i] * w[j] * np.linalg.det(J)
+            L = ua @ G
+            SU = N + tau_su * L
+            for R in range(2):
+                A[P:, R * k:(R + 1) * k] += wq * (np.outer(N + 0.5 * tau_ps * L, G[R]) + tau_ps * np.outer(G[R], L))
+                A[R * k:(R + 1) * k, R * k:(R + 1) * k] += wq * (nu * G.T @ G + np.outer(SU, L))
+                A[R * k:(R + 1) * k, P:] += wq * np.outer(SU, G[R]) / rho
+                for C in range(2):
+                    A[R * k:(R + 1) * k, C * k:(C + 1) * k] += wq * np.outer(tau_bulk * G[R] + 0.5 * ua[R] * SU, G[C])
+                T[P:, R * k:(R + 1) * k] += wq * tau_ps * np.outer(G[R], N)
+                T[R * k:(R + 1) * k, R * k:(R + 1) * k] += wq * np.outer(SU, N)
+            A[P:, P:] += wq * tau_ps * G.T @ G / rho
+    return A, T
+
+
+@pytest.mark.parametrize("case", [0, 1])
+def test_generic_quad_vs_tensor_gauss(oracle, case):
+    rng = np.random.default_rng(60 + case)
+    B = np.eye(2) * 0.2 + rng.uniform(-0.04, 0.04, (2, 2))   # parallelogram: the 2 x 2 rule is exact
+    X = (_QLOC + 1) / 2 @ B.T + rng.uniform(0, 1, 2)
+    ua = np.array([0.4, -0.7]) * case
+    nu, rho = 0.01, 1.1
+    u = np.tile(ua, (4, 1))
+    A, T, a, tau = oracle.ns_element(QUAD2D, ns_params(rho=rho, dt=0.02, use_specializations=False), X, u, u,
+                                     np.full(4, nu))
+    Ar, Tr = quad_reference_terms(X, ua, nu, tau[0], tau[1], tau[2], rho)
+    assert np.abs(A - Ar).max() <= 1e-12 * np.abs(Ar).max()
+    assert np.abs(T - Tr).max() <= 1e-12 * np.abs(Tr).max()
+    if case == 0:   # mass matrix of a parallelogram: |det B| / 36 [4 2 1 2; ...]
+        m = np.array([[4, 2, 1, 2], [2, 4, 2, 1], [1, 2, 4, 2], [2, 1, 2, 4]]) / 36 * abs(np.linalg.det(B))
+        assert np.allclose(T[:4, :4], m, rtol=1e-13)
 
 
 def test_hex_body_force_and_tezduyar(oracle):

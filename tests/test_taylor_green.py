@@ -1,5 +1,6 @@
 """Periodic links and the Taylor-Green acceptance test of the reference
-(plugins/UFEM/test/atest-ufem-navier-stokes-taylor-green.py, triangle variant).
+(plugins/UFEM/test/atest-ufem-navier-stokes-taylor-green.py, as plugins/UFEM/test/CMakeLists.txt:94-97
+runs it: quadrilaterals, and its triangle variant).
 
 The oracle run is pinned by the analytic solution: after the fifth step the reference requires the
 velocity error to stay below 1.4e-3 and the pressure error below 3.3e-3 (:315-321).  The device run
@@ -34,7 +35,7 @@ def oracle_run(case, steps=STEPS, record=None):
     errors = []
     for step in range(steps):
         ua = o.linearize_u(U, U1, U2, U3)
-        vals, rhs = o.ns_assemble_p2m(pyoracle.TRIAG2D, prm, case.coords, case.conn, p2m, U, P, ua, nu, start, slots)
+        vals, rhs = o.ns_assemble_p2m(case.etype, prm, case.coords, case.conn, p2m, U, P, ua, nu, start, slots)
         if record is not None:
             record.append((vals.copy(), rhs.copy()))
         case.t = t
@@ -54,7 +55,7 @@ def test_periodic_links_and_map():
     """Right -> left then top -> bottom: 1089 nodes, 1024 block rows; the four corners share the row
     of (-0.5, -0.5) through a chain of two links; every linked node sits a lattice vector from its row's
     node; a cycle of links is rejected."""
-    case = cases.taylor_green_tris(32)
+    case = cases.taylor_green_case(32, element="triag")
     links, active = case.periodic
     assert active.sum() == 65 and case.n_nodes == 1089
     p2m = cases.periodic_p2m(links, active)
@@ -70,10 +71,11 @@ def test_periodic_links_and_map():
         cases.periodic_p2m(np.array([1, 0], np.uint32), np.array([1, 1], np.uint8))
 
 
-def test_taylor_green_oracle_meets_reference_bounds():
-    """The oracle (periodic assembly, generic P1 expression, theta = 0.5, dt = 0.004, 32 segments)
-    against the exact vortex: the reference's bounds after the fifth step."""
-    case = cases.taylor_green_tris(32)
+@pytest.mark.parametrize("element", ["quad", "triag"])
+def test_taylor_green_oracle_meets_reference_bounds(element):
+    """The oracle (periodic assembly, generic expression, theta = 0.5, dt = 0.004, 32 segments) against
+    the exact vortex: the reference's bounds after the fifth step."""
+    case = cases.taylor_green_case(32, element=element)
     _, errors = oracle_run(case)
     for step, (eu, ep, _, _) in enumerate(errors, 1):
         print(f"step {step}: max error u {eu[0]:.3e} v {eu[1]:.3e} p {ep:.4e}")
@@ -83,8 +85,9 @@ def test_taylor_green_oracle_meets_reference_bounds():
 
 
 @pytest.mark.gpu
-def test_taylor_green_device_follows_oracle():
-    case = cases.taylor_green_tris(32)
+@pytest.mark.parametrize("element", ["quad", "triag"])
+def test_taylor_green_device_follows_oracle(element):
+    case = cases.taylor_green_case(32, element=element)
     rec = []
     (start, cols, slots, p2m), errors = oracle_run(case, record=rec)
     ctx = cabi.Context(0)
@@ -101,7 +104,7 @@ def test_taylor_green_device_follows_oracle():
         vals, rhs = rec[step]
         v = g.system.values()
         err = np.abs(v - vals).max() / np.abs(vals).max()
-        assert err <= 1e-12, (step, err)
+        assert err <= 1e-12, (step, err)   # blocks of the coupled system: max-relative over the matrix
         assert np.abs(g.system.vector(cabi.RHS) - rhs).max() <= 1e-12 * np.abs(rhs).max()
         st = g._solve_update(sp, True)
         eu_ref, ep_ref, u_ref, p_ref = errors[step]
