@@ -258,7 +258,7 @@ def mesh_case(path, etype_name=None, nu=1e-2, dt=1e-2, seed=DEFAULT_SEED, bcs=()
         if len(m.blocks) != 1:
             raise ValueError("mesh holds several element types; name one of %s" % sorted(m.blocks))
         etype_name = next(iter(m.blocks))
-    etype = {"Triag2D": cabi.TRIAG2D, "Tetra3D": cabi.TETRA3D, "Hexa3D": cabi.HEXA3D}[etype_name]
+    etype = {"Triag2D": cabi.TRIAG2D, "Tetra3D": cabi.TETRA3D, "Hexa3D": cabi.HEXA3D, "Quad2D": cabi.QUAD2D}[etype_name]
     conn = m.blocks[etype_name]
     used = np.unique(conn.ravel())
     if used.size != m.n_nodes:

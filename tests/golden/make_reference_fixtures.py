@@ -15,8 +15,9 @@ Sources:
 Also copies mesh files of the reference (data files, unchanged) into meshes/: the Gambit neutral files
 plugins/UFEM/test/meshes/ring2d-tris.neu (669 nodes, 1204 triangles, boundary sets inner and outer) and
 resources/hextet.neu (hexahedra and tetrahedra; test/mesh/utest-mesh-neu.cpp:123-125), and the Gmsh
-files resources/rectangle-tg-p1.msh (507 nodes, 929 triangles; test/mesh/utest-mesh-gmsh.cpp:96) and
-resources/sqduct_3000e.msh (3751 nodes, 3000 hexahedra of a square duct).
+files resources/rectangle-tg-p1.msh (507 nodes, 929 triangles; test/mesh/utest-mesh-gmsh.cpp:96),
+resources/rectangle-qd-p1.msh (56 nodes, 42 quadrilaterals) and resources/sqduct_3000e.msh (3751 nodes,
+3000 hexahedra of a square duct).
 """
 import json
 import os
@@ -25,7 +26,7 @@ import shutil
 import sys
 
 MESHES = ["plugins/UFEM/test/meshes/ring2d-tris.neu", "resources/hextet.neu", "resources/rectangle-tg-p1.msh",
-          "resources/sqduct_3000e.msh"]
+          "resources/rectangle-qd-p1.msh", "resources/sqduct_3000e.msh"]
 
 
 def copy_meshes(ref):

@@ -78,6 +78,13 @@ def rectangle_case():
                            bcs=[("Wall", {0: 0.0, 1: 0.0}), ("Inlet", {0: 0.0, 1: 1.0}), ("Outlet", {2: 0.0})])
 
 
+def rectangle_quads_case():
+    """The reference's rectangle of quadrilaterals (Gmsh file): u = (1, 0) on the inlet, no-slip on the
+    wall, p = 0 on the outlet."""
+    return cases.mesh_case(os.path.join(MESHES, "rectangle-qd-p1.msh"),
+                           bcs=[("inlet", {0: 1.0, 1: 0.0}), ("Wall", {0: 0.0, 1: 0.0}), ("Outlet", {2: 0.0})])
+
+
 def duct_case():
     """The reference's square duct of hexahedra (Gmsh file): u = (0, 0, 1) on the inlet (z = 0), no-slip
     on the side walls, p = 0 on the outlet (z = 3).  The file names no wall region and its outlet region
@@ -121,6 +128,7 @@ CASES = {
     "ring": ring_case,
     "rect": rectangle_case,
     "duct": duct_case,
+    "rectq": rectangle_quads_case,
     "scattered": scattered_tets_case,
     "quad9": quads_case,
     "tet5": lambda: cases.inflow_cube_tets(5),
@@ -278,7 +286,7 @@ def test_gmres_ilu0_lid_cavity(ctx, oracle):
     g.close()
 
 
-@pytest.mark.parametrize("name", ["ring", "rect", "duct"])
+@pytest.mark.parametrize("name", ["ring", "rect", "rectq", "duct"])
 def test_file_mesh_step_vs_oracle(ctx, oracle, name):
     """Meshes read from the reference's Gambit neutral and Gmsh files: one implicit step (FP64 assembly,
     BCs, GMRES(30)+ILU0 to 1e-10) on the device matches the oracle's system and solution."""

@@ -102,6 +102,18 @@ def test_gmsh_rectangle_triangles():
     assert np.array_equal(np.union1d(np.union1d(inlet, outlet), wall), perim)
 
 
+def test_gmsh_rectangle_quadrilaterals():
+    """resources/rectangle-qd-p1.msh: the rectangle [-1,1] x [0,1] in 42 quadrilaterals (gmsh node order
+    is cf3's: positive Quad2D volumes)."""
+    m = meshio.read_mesh(os.path.join(MESHES, "rectangle-qd-p1.msh"))
+    assert (m.dim, m.n_nodes) == (2, 56) and m.blocks["Quad2D"].shape == (42, 4)
+    o = pyoracle.get()
+    vol = np.array([o.element_volume(pyoracle.QUAD2D, m.coords[e]) for e in m.blocks["Quad2D"]])
+    assert vol.min() > 0 and abs(vol.sum() - 2.0) < 1e-12
+    assert sorted(m.boundaries) == ["Outlet", "Wall", "inlet"]
+    assert np.all(m.coords[m.boundary_nodes("inlet"), 0] == -1.0)
+
+
 def test_gmsh_square_duct_hexahedra():
     """resources/sqduct_3000e.msh: the duct [-0.5,0.5]^2 x [0,3] in 3000 hexahedra (gmsh node order
     is cf3's), the inlet region's quadrilaterals on z = 0."""
