@@ -72,15 +72,18 @@ public:
     const int k = etype == CF3HIP_TRIAG2D ? 3 : (etype == CF3HIP_HEXA3D ? 8 : 4);
     const int dim = etype == CF3HIP_TRIAG2D || etype == CF3HIP_QUAD2D ? 2 : 3;
     if(coords.size() % dim || conn.size() % k) throw BadValue(CF3HIP_ERR_INVALID_ARG, "mesh array sizes");
-    check(cf3hip_mesh_create(ctx.handle(), etype, (uint32_t)(coords.size() / dim), (uint32_t)(conn.size() / k),
-                             coords.data(), conn.data(), &h_));
+    n_nodes_ = (uint32_t)(coords.size() / dim);
+    check(cf3hip_mesh_create(ctx.handle(), etype, n_nodes_, (uint32_t)(conn.size() / k), coords.data(), conn.data(),
+                             &h_));
   }
   ~Mesh() { cf3hip_mesh_destroy(h_); }
   Mesh(const Mesh&) = delete;
   Mesh& operator=(const Mesh&) = delete;
   cf3hip_mesh handle() const { return h_; }
+  uint32_t n_nodes() const { return n_nodes_; }
 private:
   cf3hip_mesh h_ = nullptr;
+  uint32_t n_nodes_ = 0;
 };
 
 namespace lss {
@@ -221,6 +224,41 @@ public:
   {
     check(cf3hip_lss_create(ctx.handle(), mesh.handle(), (int)neq, &h_));
   }
+  // System::create with periodic links (System.cpp:82-99): node i with periodic_links_active[i] shares
+  // the block row of periodic_links_nodes[i] (the end of its chain of links).  As in the reference
+  // (TrilinosFEVbrMatrix's m_p2m), every index given to the matrix, vector and dirichlet calls of this
+  // system is a mesh node, converted here; whole-vector get/set/diagonal calls are in block-row order.
+  System(Context& ctx, Mesh& mesh, uint32_t neq, const std::vector<uint32_t>& periodic_links_nodes,
+         const std::vector<bool>& periodic_links_active)
+      : ctx_(ctx), matrix_(*this), rhs_(*this, CF3HIP_RHS), sol_(*this, CF3HIP_SOLUTION), strategy_(*this)
+  {
+    if(periodic_links_nodes.size() != mesh.n_nodes() || periodic_links_active.size() != mesh.n_nodes())
+      throw BadValue(CF3HIP_ERR_INVALID_ARG, "periodic link lists need one entry per mesh node");
+    std::vector<uint8_t> active(periodic_links_active.begin(), periodic_links_active.end());
+    check(cf3hip_lss_create_periodic(ctx.handle(), mesh.handle(), (int)neq, periodic_links_nodes.data(),
+                                     active.data(), &h_));
+    p2m_.resize(mesh.n_nodes());
+    check(cf3hip_lss_get_p2m(h_, p2m_.data()));
+  }
+  // block row of mesh node i (the identity without periodic links)
+  uint32_t row(uint32_t i) const
+  {
+    if(p2m_.empty()) return i;
+    if(i >= p2m_.size()) throw BadValue(CF3HIP_ERR_INVALID_ARG, "node index outside the mesh");
+    return p2m_[i];
+  }
+  std::vector<uint32_t> rows(const std::vector<uint32_t>& nodes) const
+  {
+    std::vector<uint32_t> r(nodes.size());
+    for(size_t i = 0; i < nodes.size(); ++i) r[i] = row(nodes[i]);
+    return r;
+  }
+  // scalar index node * neq + eq of a mesh node converted to its block row
+  uint32_t scalar_row(uint32_t i) const
+  {
+    const uint32_t q = neq();
+    return row(i / q) * q + i % q;
+  }
   ~System() { cf3hip_lss_destroy(h_); }
   System(const System&) = delete;
   System& operator=(const System&) = delete;
@@ -244,7 +282,8 @@ public:
   // the ordered conditions of one BoundaryConditions pass in one call (symmetric form)
   void dirichlet(const std::vector<uint32_t>& nodes, const std::vector<uint32_t>& eqs, const std::vector<double>& values)
   {
-    check(cf3hip_lss_dirichlet(h_, (uint32_t)nodes.size(), nodes.data(), eqs.data(), values.data()));
+    const std::vector<uint32_t> r = rows(nodes);
+    check(cf3hip_lss_dirichlet(h_, (uint32_t)r.size(), r.data(), eqs.data(), values.data()));
   }
   bool solve() { return strategy_.solve(); }
   void reset(double v = 0.)
@@ -277,6 +316,7 @@ private:
   Matrix matrix_;
   Vector rhs_, sol_;
   SolutionStrategy strategy_;
+  std::vector<uint32_t> p2m_;   // empty: no periodic links
 };
 
 // ---- Vector ----
@@ -285,18 +325,19 @@ inline void Vector::add_value(uint32_t irow, double v) { add_value(irow / neq(),
 inline void Vector::get_value(uint32_t irow, double& v) { get_value(irow / neq(), irow % neq(), v); }
 inline void Vector::set_value(uint32_t iblockrow, uint32_t ieq, double v)
 {
-  check(cf3hip_vector_set_value(sys_.handle(), which_, iblockrow, ieq, v));
+  check(cf3hip_vector_set_value(sys_.handle(), which_, sys_.row(iblockrow), ieq, v));
 }
 inline void Vector::add_value(uint32_t iblockrow, uint32_t ieq, double v)
 {
-  check(cf3hip_vector_add_value(sys_.handle(), which_, iblockrow, ieq, v));
+  check(cf3hip_vector_add_value(sys_.handle(), which_, sys_.row(iblockrow), ieq, v));
 }
 inline void Vector::get_value(uint32_t iblockrow, uint32_t ieq, double& v)
 {
-  check(cf3hip_vector_get_value(sys_.handle(), which_, iblockrow, ieq, &v));
+  check(cf3hip_vector_get_value(sys_.handle(), which_, sys_.row(iblockrow), ieq, &v));
 }
-inline void Vector::block(const std::vector<uint32_t>& idx, double* vals, int mode)
+inline void Vector::block(const std::vector<uint32_t>& nodes, double* vals, int mode)
 {
+  const std::vector<uint32_t> idx = sys_.rows(nodes);
   const uint32_t nb = (uint32_t)idx.size();
   if(mode == 0) check(cf3hip_vector_set_values(sys_.handle(), which_, nb, idx.data(), vals));
   else if(mode == 1) check(cf3hip_vector_add_values(sys_.handle(), which_, nb, idx.data(), vals));
@@ -333,42 +374,46 @@ inline uint32_t Vector::blockrow_size() const { return sys_.n_nodes(); }
 // ---- Matrix ----
 inline void Matrix::set_value(uint32_t icol, uint32_t irow, double v)
 {
-  check(cf3hip_matrix_set_value(sys_.handle(), icol, irow, v));
+  check(cf3hip_matrix_set_value(sys_.handle(), sys_.scalar_row(icol), sys_.scalar_row(irow), v));
 }
 inline void Matrix::add_value(uint32_t icol, uint32_t irow, double v)
 {
-  check(cf3hip_matrix_add_value(sys_.handle(), icol, irow, v));
+  check(cf3hip_matrix_add_value(sys_.handle(), sys_.scalar_row(icol), sys_.scalar_row(irow), v));
 }
 inline void Matrix::get_value(uint32_t icol, uint32_t irow, double& v)
 {
-  check(cf3hip_matrix_get_value(sys_.handle(), icol, irow, &v));
+  check(cf3hip_matrix_get_value(sys_.handle(), sys_.scalar_row(icol), sys_.scalar_row(irow), &v));
 }
 inline void Matrix::set_values(const BlockAccumulator& b)
 {
-  check(cf3hip_matrix_set_values(sys_.handle(), b.size(), b.indices.data(), b.mat.data()));
+  const std::vector<uint32_t> r = sys_.rows(b.indices);
+  check(cf3hip_matrix_set_values(sys_.handle(), b.size(), r.data(), b.mat.data()));
 }
 inline void Matrix::add_values(const BlockAccumulator& b)
 {
-  check(cf3hip_matrix_add_values(sys_.handle(), b.size(), b.indices.data(), b.mat.data()));
+  const std::vector<uint32_t> r = sys_.rows(b.indices);
+  check(cf3hip_matrix_add_values(sys_.handle(), b.size(), r.data(), b.mat.data()));
 }
 inline void Matrix::get_values(BlockAccumulator& b)
 {
-  check(cf3hip_matrix_get_block_values(sys_.handle(), b.size(), b.indices.data(), b.mat.data()));
+  const std::vector<uint32_t> r = sys_.rows(b.indices);
+  check(cf3hip_matrix_get_block_values(sys_.handle(), b.size(), r.data(), b.mat.data()));
 }
 inline void Matrix::set_row(uint32_t iblockrow, uint32_t ieq, double diagval, double offdiagval)
 {
-  check(cf3hip_matrix_set_row(sys_.handle(), iblockrow, ieq, diagval, offdiagval));
+  check(cf3hip_matrix_set_row(sys_.handle(), sys_.row(iblockrow), ieq, diagval, offdiagval));
 }
 inline void Matrix::get_column_and_replace_to_zero(uint32_t iblockcol, uint32_t ieq, std::vector<double>& values)
 {
   values.resize((size_t)blockcol_size() * neq());
-  check(cf3hip_matrix_get_column_and_replace_to_zero(sys_.handle(), iblockcol, ieq, values.data()));
+  check(cf3hip_matrix_get_column_and_replace_to_zero(sys_.handle(), sys_.row(iblockcol), ieq, values.data()));
 }
 inline void Matrix::symmetric_dirichlet(uint32_t blockrow, uint32_t ieq, double value, Vector& rhs)
 {
   if(&rhs.system() != &sys_ || rhs.which() != CF3HIP_RHS)
     throw SetupError(CF3HIP_ERR_NOT_CREATED, "symmetric_dirichlet acts on the rhs of the same system");
-  check(cf3hip_lss_dirichlet(sys_.handle(), 1, &blockrow, &ieq, &value));
+  const uint32_t r = sys_.row(blockrow);
+  check(cf3hip_lss_dirichlet(sys_.handle(), 1, &r, &ieq, &value));
 }
 inline void Matrix::set_diagonal(const std::vector<double>& diag)
 {

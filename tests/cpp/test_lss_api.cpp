@@ -250,6 +250,97 @@ static void ns_assembly(Context& ctx)
   CHECK(throws<BadValue>([&] { ns.execute(s, f); }));
 }
 
+// periodic links (System::create with periodic_links_nodes / active): a 4 x 4 triangle mesh of the unit
+// square with the right edge linked to the left one; indices are mesh nodes, converted by the host layer
+static void periodic_system(Context& ctx)
+{
+  const uint32_t c = 4, N = (c + 1) * (c + 1), E = 2 * c * c;
+  std::vector<double> X((size_t)N * 2);
+  std::vector<uint32_t> conn((size_t)E * 3);
+  orc_mesh_rect_tris(c, c, 1., 1., X.data(), conn.data());
+  std::vector<uint32_t> links(N, 0);
+  std::vector<bool> active(N, false);
+  for(uint32_t i = 0; i < N; ++i)
+    if(std::fabs(X[2 * i] - 1.) < 1e-12)
+      for(uint32_t j = 0; j < N; ++j)
+        if(std::fabs(X[2 * j]) < 1e-12 && std::fabs(X[2 * j + 1] - X[2 * i + 1]) < 1e-12) {
+          links[i] = j;
+          active[i] = true;
+        }
+  uint64_t st = 777;
+  std::vector<double> u((size_t)N * 2), p(N, 0.), nu(N, 1e-2);
+  for(uint32_t i = 0; i < N; ++i) {   // periodic fields: a linked node carries its master's values
+    const uint32_t m = active[i] ? links[i] : i;
+    if(m == i) { u[2 * i] = orc_uniform(&st, -1., 1.); u[2 * i + 1] = orc_uniform(&st, -1., 1.); }
+  }
+  for(uint32_t i = 0; i < N; ++i)
+    if(active[i]) { u[2 * i] = u[2 * links[i]]; u[2 * i + 1] = u[2 * links[i] + 1]; }
+  Mesh mesh(ctx, CF3HIP_TRIAG2D, X, conn);
+  lss::System s(ctx, mesh, 3, links, active);
+  CHECK(s.n_nodes() == N - (c + 1));
+  uint32_t slave = 0;
+  while(!active[slave]) ++slave;
+  const uint32_t master = links[slave];
+  CHECK(s.row(slave) == s.row(master));
+  // a value set through the linked node is the master's
+  s.rhs().reset(0.);
+  s.rhs().set_value(slave, 1, 2.5);
+  double v = 0.;
+  s.rhs().get_value(master, 1, v);
+  CHECK(v == 2.5);
+  // UFEM assembly through the C++ action against the oracle's assembly through the same map
+  ufem::Fields f(mesh);
+  f.set(CF3HIP_FIELD_U, u);
+  f.set(CF3HIP_FIELD_P, p);
+  f.initialize(1e-2);
+  ufem::NavierStokesAssembly ns;
+  ns.density = 1.;
+  ns.dt = 0.01;
+  ns.use_specializations = false;
+  ns.linearize(f);
+  ns.execute(s, f);
+  std::vector<uint32_t> p2m(N), dconn(conn.size());
+  for(uint32_t i = 0; i < N; ++i) p2m[i] = s.row(i);
+  for(size_t i = 0; i < conn.size(); ++i) dconn[i] = p2m[conn[i]];
+  const uint32_t R = s.n_nodes();
+  std::vector<uint32_t> start(R + 1), cols(64 * R), slots((size_t)E * 9);
+  const int64_t nb = orc_build_sparsity(3, R, E, dconn.data(), start.data(), cols.data(), (int64_t)cols.size(), slots.data());
+  CHECK(nb > 0);
+  orc_ns_params prm = {1., 0.01, 1., 0, ORC_SUPG_METRIC, 1., 16., 1., 1., 1.};
+  std::vector<double> vref((size_t)nb * 9), rref((size_t)R * 3);
+  CHECK(orc_ns_assemble_p2m(ORC_TRIAG2D, &prm, R, E, X.data(), conn.data(), p2m.data(), u.data(), p.data(), u.data(),
+                            nu.data(), nullptr, start.data(), slots.data(), vref.data(), rref.data()) == 0);
+  // compare through mesh-node scalar access: the free nodes are their own rows' representatives
+  std::vector<uint32_t> rep(R);
+  for(uint32_t i = 0; i < N; ++i)
+    if(!active[i]) rep[p2m[i]] = i;
+  double vmax = 0., dmax = 0.;
+  for(uint32_t r = 0; r < R; ++r)
+    for(uint32_t q = start[r]; q < start[r + 1]; ++q)
+      for(int a = 0; a < 3; ++a)
+        for(int b2 = 0; b2 < 3; ++b2) {
+          double g = 0.;
+          s.matrix().get_value(rep[cols[q]] * 3 + b2, rep[r] * 3 + a, g);
+          const double ref = vref[(size_t)q * 9 + a * 3 + b2];
+          vmax = std::fmax(vmax, std::fabs(ref));
+          dmax = std::fmax(dmax, std::fabs(g - ref));
+        }
+  CHECK(dmax <= 1e-12 * vmax);
+  std::vector<double> rhs;
+  s.rhs().get(rhs);
+  double rmax = 0., rd = 0.;
+  for(size_t i = 0; i < rhs.size(); ++i) {
+    rmax = std::fmax(rmax, std::fabs(rref[i]));
+    rd = std::fmax(rd, std::fabs(rhs[i] - rref[i]));
+  }
+  CHECK(rd <= 1e-12 * rmax);
+  // cycles of links are refused
+  std::vector<uint32_t> bad(N, 0);
+  std::vector<bool> all(N, true);
+  bad[0] = 1;
+  CHECK(throws<BadValue>([&] { lss::System t(ctx, mesh, 3, bad, all); }));
+}
+
 int main(int argc, char** argv)
 {
   if(argc > 1 && std::strcmp(argv[1], "--no-device") == 0) {
@@ -279,6 +370,9 @@ int main(int argc, char** argv)
     std::printf("ns_assembly\n");
     std::fflush(stdout);
     ns_assembly(ctx);
+    std::printf("periodic_system\n");
+    std::fflush(stdout);
+    periodic_system(ctx);
   }
   std::printf("%s: %d failed checks\n", g_fail ? "FAIL" : "OK", g_fail);
   return g_fail ? 1 : 0;
