@@ -111,10 +111,8 @@ __device__ __forceinline__ double inverse(const double J[D * D], double Ji[D * D
   }
 }
 
-// FP32 hexahedra bounded to 168 VGPRs (3 waves per SIMD instead of 2): a few spills, 14.4 against
-// 15.9 ms at 160^3; the FP64 modes gain nothing from it (19.96 against 19.65 ms) and stay unbounded
 template<int D, int MODE, int SUPG>
-__global__ __launch_bounds__(kThreads, (D == 3 && MODE == MODE_FP32) ? 3 : 1) void k_q1(AsmArgs g)
+__global__ __launch_bounds__(kThreads) void k_q1(AsmArgs g)
 {
   typedef Q1<D> Q;
   constexpr int K = Q::K, NEQ = Q::NEQ, NN = Q::NN, kLanes = Q::kLanes, kRec = Q::kRec, kRecSlot = Q::kRecSlot;
