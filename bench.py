@@ -11,8 +11,9 @@ this configuration (FP64 assembly, BCs, BiCGStab/Jacobi to a 1e-10 relative resi
 baseline (the oracle restatement, OpenMP over the host cores, bounded sample), and the other
 workloads: the precision form on the 128^3 tetrahedral channel with 1% skewed elements (FP32 element
 matrices against FP64, both to 1e-8), the 160^3 Q1 hexahedral lid-driven cavity (assembly and Picard
-iterations in both forms), the 32x32 triangle cavity step (GMRES + ILU0), and the serving loop
-(continuous batching against one-at-a-time requests, p50/p99 latency).
+iterations in both forms), the 32x32 triangle cavity step (GMRES + ILU0), the reference's
+Taylor-Green acceptance run (periodic, quadrilaterals and triangles, errors against the exact vortex),
+and the serving loop (continuous batching against one-at-a-time requests, p50/p99 latency).
 """
 import argparse
 import json
@@ -184,6 +185,29 @@ def cavity2d_workload(ctx):
             "steps": steps}
 
 
+def taylor_green_workload(ctx, element):
+    """The reference's Taylor-Green acceptance run (plugins/UFEM/test/atest-ufem-navier-stokes-taylor-green.py
+    with --dt=0.004 --segs=32 --theta=0.5 --tsteps=5): periodic square, generic expression, GMRES(30) +
+    ILU0 to 1e-12 per step; the maximum errors against the exact vortex after each step and the
+    reference's bounds after the fifth (u, v < 1.4e-3; p < 3.3e-3)."""
+    from coolfluid3_amd import cabi, cases
+    case = cases.taylor_green_case(32, element=element)
+    gc = cases.GpuCase(ctx, case, use_specializations=False)
+    sp = cabi.solver_params(cabi.GMRES_ILU0, tol=1e-12, restart=30)
+    steps = []
+    for _ in range(5):
+        st = timed_step(ctx, gc, sp)
+        ue, pe = cases.taylor_green_exact(case.coords, case.t)
+        st["max_error"] = {"u": float(np.abs(gc.u[:, 0] - ue[:, 0]).max()),
+                           "v": float(np.abs(gc.u[:, 1] - ue[:, 1]).max()), "p": float(np.abs(gc.p - pe).max())}
+        steps.append(st)
+    gc.close()
+    last = steps[-1]["max_error"]
+    return {"workload": "Taylor-Green vortex, [-0.5,0.5]^2 periodic, 32 segments (%s), nu=1e-3 dt=0.004 theta=0.5, "
+                        "5 steps" % ("1024 Q1 quadrilaterals" if element == "quad" else "2048 P1 triangles"),
+            "steps": steps, "within_reference_bounds": bool(max(last["u"], last["v"]) < 1.4e-3 and last["p"] < 3.3e-3)}
+
+
 def serving_workload(count, max_active, mode="both"):
     """Continuous batching against one-request-at-a-time serving on the same request stream
     (mode: both, sequential or batched)."""
@@ -211,7 +235,7 @@ def main():
     ap.add_argument("--no-cpu-baseline", action="store_true")
     ap.add_argument("--no-solve", action="store_true")
     ap.add_argument("--no-workloads", action="store_true", help="skip the 128^3 / 160^3 / 2D / serving workloads")
-    ap.add_argument("--workloads", default="cavity_2d,precision_tets,hexa_cavity,serving",
+    ap.add_argument("--workloads", default="cavity_2d,taylor_green,precision_tets,hexa_cavity,serving",
                     help="comma-separated subset of the extra workloads")
     ap.add_argument("--precision-cells", type=int, default=128)
     ap.add_argument("--hexa-cells", type=int, default=160)
@@ -316,6 +340,7 @@ def main():
     g.close()
     if not args.no_workloads:
         for key, fn in (("cavity_2d", lambda: cavity2d_workload(ctx)),
+                        ("taylor_green", lambda: {e: taylor_green_workload(ctx, e) for e in ("quad", "triag")}),
                         ("precision_tets", lambda: precision_workload(ctx, args.precision_cells)),
                         ("hexa_cavity", lambda: hexa_workload(ctx, args.hexa_cells, args.picard)),
                         ("serving", lambda: serving_workload(args.serving_requests, args.serving_slots, args.serving_mode))):
