@@ -217,21 +217,38 @@ def taylor_green_exact(coords, t, ua=TG_UA, va=TG_VA, d=TG_D, nu=TG_NU):
 def taylor_green_case(segments=32, dt=0.004, theta=0.5, element="quad"):
     """The Taylor-Green acceptance case of plugins/UFEM/test/atest-ufem-navier-stokes-taylor-green.py
     (plugins/UFEM/test/CMakeLists.txt:94-97 runs it with --dt=0.004 --elem=quad --segs=32 --theta=0.5
-    --tsteps=5): the square [-0.5, 0.5]^2 in segments^2 quadrilaterals, or each split in two triangles
+    --tsteps=5): the square [-0.5, 0.5]^2 in segments^2 quadrilaterals (the script's two BlockArrays
+    blocks), or each cell split in two triangles
     (--elem=triag), periodic right -> left and top -> bottom (:85-97), density 1 and dynamic viscosity
     0.001 (:163-164), the generic expression (use_specializations false, :183), the exact velocity and
     pressure at t = 0 (setup_ic, :124-141) and the exact pressure at t + dt imposed at the centre node
     (add_pressure_bc, :166-169).  The reference bounds the errors after the fifth step by 1.4e-3 (u, v)
     and 3.3e-3 (p)."""
     if element == "quad":
-        coords, conn = meshgen.rect_quads(segments, segments)
+        # BlockArrays of the script (:39-76): two blocks of segments x segments/2 cells, four patches
+        from . import blockmesh
+        blocks = blockmesh.BlockArrays()
+        blocks.create_points(2, 6)[:] = [[-0.5, -0.5], [0.5, -0.5], [-0.5, 0.0], [0.5, 0.0], [-0.5, 0.5], [0.5, 0.5]]
+        bl = blocks.create_blocks(2)
+        bl[0], bl[1] = [0, 1, 3, 2], [2, 3, 5, 4]
+        sd = blocks.create_block_subdivisions()
+        sd[0], sd[1] = [segments, segments // 2], [segments, segments // 2]
+        blocks.create_block_gradings()
+        lp = blocks.create_patch_nb_faces("left", 2)
+        lp[0], lp[1] = [2, 0], [4, 2]
+        blocks.create_patch_nb_faces("bottom", 1)[0] = [0, 1]
+        blocks.create_patch_nb_faces("top", 1)[0] = [5, 4]
+        rp = blocks.create_patch_nb_faces("right", 2)
+        rp[0], rp[1] = [1, 3], [3, 5]
+        m = blocks.create_mesh()
+        coords, conn = m.coords, m.blocks["Quad2D"]
         etype = cabi.QUAD2D
     elif element == "triag":
-        coords, conn = meshgen.rect_tris(segments, segments)
+        coords, conn = meshgen.rect_tris(segments, segments)   # the script's MeshTriangulator split is not restated
+        coords = coords - 0.5
         etype = cabi.TRIAG2D
     else:
         raise ValueError("element is 'quad' or 'triag'")
-    coords = coords - 0.5
     x, y = coords[:, 0], coords[:, 1]
     tol = 1e-12
     right, left = np.flatnonzero(np.abs(x - 0.5) < tol), np.flatnonzero(np.abs(x + 0.5) < tol)
