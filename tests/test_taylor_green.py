@@ -117,3 +117,31 @@ def test_taylor_green_device_follows_oracle(element):
     assert np.abs(g.u - ue).max() < U_BOUND and np.abs(g.p - pe).max() < P_BOUND
     g.close()
     ctx.close()
+
+
+@pytest.mark.gpu
+def test_periodic_system_refuses_the_matrix_free_forms():
+    """A periodic system rejects the FP32 solve of the precision form (its FP64 matrix-free residual
+    gathers by mesh node) and batched assembly with CF3HIP_ERR_NOT_IMPLEMENTED; the FP64 path is
+    untouched by the refusals."""
+    case = cases.taylor_green_case(8, element="triag")
+    ctx = cabi.Context(0)
+    g = cases.GpuCase(ctx, case, use_specializations=False)
+    g.system.enable_fp32()
+    g.fields.linearize()
+    g.assemble(cabi.FP32)
+    g.assemble(cabi.FP64)
+    prm = cabi.NSParams.from_buffer_copy(g.prm)
+    prm.precision = cabi.FP32
+    with pytest.raises(cabi.Cf3HipError) as e:
+        g.system.ns_solve(g.fields, prm, cabi.solver_params(tol=1e-8, precision=cabi.FP32))
+    assert e.value.code == 7
+    batch = cabi.Batch(ctx, 2)
+    with pytest.raises(cabi.Cf3HipError) as e:
+        batch.assemble([g.system], [g.fields], [prm])
+    assert e.value.code == 7
+    batch.close()
+    st = g._solve_update(cabi.solver_params(cabi.GMRES_ILU0, tol=1e-12, restart=30), True)
+    assert st.converged
+    g.close()
+    ctx.close()
