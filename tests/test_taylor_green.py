@@ -145,3 +145,45 @@ def test_periodic_system_refuses_the_matrix_free_forms():
     assert st.converged
     g.close()
     ctx.close()
+
+
+@pytest.mark.gpu
+@pytest.mark.parametrize("name", ["tets", "hexas"])
+def test_periodic_3d_assembly_vs_oracle(name):
+    """3D periodic links (x: right -> left, then y: back -> front) on the tetrahedral cube (specialised,
+    node-staged FP64 kernel) and the hexahedral cube (Q1 kernel): sparsity and slot map bit-exact
+    against the oracle's sparsity of p2m[conn], FP64 values and rhs within 1e-12."""
+    o = pyoracle.get()
+    # without jitter: the tangential jitter of the boundary nodes would break the periodic matching
+    base = (cases.inflow_cube_tets(6, jitter=0.0) if name == "tets"
+            else cases.lid_cavity_cube_hexas(5, jitter=0.0, random_velocity=True))
+    c = base.coords
+    sel = lambda d, v: np.flatnonzero(np.abs(c[:, d] - v) < 1e-12)
+    base.periodic = cases.periodic_links(c, [(sel(0, 1.0), sel(0, 0.0), (-1.0, 0.0, 0.0)),
+                                             (sel(1, 1.0), sel(1, 0.0), (0.0, -1.0, 0.0))])
+    base.bc_regions = []
+    p2m = cases.periodic_p2m(*base.periodic)
+    rows = int(p2m.max()) + 1
+    start, cols, slots = o.build_sparsity(p2m[base.conn], rows)
+    et = {cabi.TETRA3D: pyoracle.TETRA3D, cabi.HEXA3D: pyoracle.HEXA3D}[base.etype]
+    # periodic fields: a linked node carries the values of its row's unlinked node
+    free = np.flatnonzero(base.periodic[1] == 0)
+    base.u0 = np.ascontiguousarray(base.u0[free[p2m]])
+    base.p0 = base.p0[free[p2m]]
+    prm = pyoracle.ns_params(rho=base.rho, dt=base.dt, use_specializations=True)
+    ua = o.linearize_u(base.u0, base.u0, base.u0, base.u0)
+    vref, rref = o.ns_assemble_p2m(et, prm, base.coords, base.conn, p2m, base.u0, base.p0, ua,
+                                   np.full(base.n_nodes, base.nu), start, slots)
+    ctx = cabi.Context(0)
+    g = cases.GpuCase(ctx, base)
+    assert g.system.n_nodes == rows and np.array_equal(g.p2m, p2m)
+    s, cl = g.system.graph()
+    assert np.array_equal(s, start) and np.array_equal(cl, cols) and np.array_equal(g.system.slots(), slots)
+    g.fields.linearize()
+    g.assemble(cabi.FP64)
+    v = g.system.values()
+    assert np.abs(v - vref).max() <= 1e-12 * np.abs(vref).max()
+    r = g.system.vector(cabi.RHS)
+    assert np.abs(r - rref).max() <= 1e-12 * np.abs(rref).max()
+    g.close()
+    ctx.close()
