@@ -129,6 +129,16 @@ def lib():
         "cf3hip_context_create_stream": [_vp],
         "cf3hip_batch_destroy": [_vp],
         "cf3hip_batch_assemble": [_vp, C.c_uint32, C.POINTER(_vp), C.POINTER(_vp), C.POINTER(NSParams)],
+        "cf3hip_si_create": [_vp, _vp, C.POINTER(_vp)],
+        "cf3hip_si_destroy": [_vp],
+        "cf3hip_si_initial": [_vp, _vp, C.POINTER(NSParams)],
+        "cf3hip_si_velocity_assemble": [_vp, _vp, C.POINTER(NSParams)],
+        "cf3hip_si_set_solution": [_vp, _vp],
+        "cf3hip_si_action": [_vp, _vp, C.POINTER(NSParams), C.c_int],
+        "cf3hip_si_get": [_vp, C.c_int, _d],
+        "cf3hip_si_set": [_vp, C.c_int, _d],
+        "cf3hip_si_step": [_vp, _vp, C.POINTER(NSParams), C.c_int, C.POINTER(SolverParams), C.POINTER(SolverParams),
+                           C.c_uint32, _u32, _d, C.c_uint32, _u32, _d, C.POINTER(SolveStats)],
     }
     for name, args in sig.items():
         fn = getattr(L, name)
@@ -425,4 +435,58 @@ class System:
     def close(self):
         if self.h:
             check(lib().cf3hip_lss_destroy(self.h))
+            self.h = _vp()
+
+
+SI_URHS, SI_PRHS, SI_AUP = 0, 1, 2
+SI_VEC_A, SI_VEC_U, SI_VEC_P, SI_VEC_DPS, SI_VEC_MINV = 0, 1, 2, 3, 4
+
+
+class SemiImplicit:
+    """NavierStokesSemiImplicit on a velocity system (neq = dim) and a pressure system (neq = 1) of one
+    mesh (cf3hip_si_*)."""
+
+    def __init__(self, u_system, p_system):
+        self.u_system, self.p_system = u_system, p_system
+        self.h = _vp()
+        check(lib().cf3hip_si_create(u_system.h, p_system.h, C.byref(self.h)))
+
+    def initial(self, fields, prm):
+        check(lib().cf3hip_si_initial(self.h, fields.h, C.byref(prm)))
+
+    def velocity_assemble(self, fields, prm):
+        check(lib().cf3hip_si_velocity_assemble(self.h, fields.h, C.byref(prm)))
+
+    def set_solution(self, fields):
+        check(lib().cf3hip_si_set_solution(self.h, fields.h))
+
+    def action(self, fields, prm, which):
+        check(lib().cf3hip_si_action(self.h, fields.h, C.byref(prm), which))
+
+    def _n(self, which):
+        rows = self.u_system.n_nodes
+        return rows * self.u_system.neq if which in (SI_VEC_A, SI_VEC_U, SI_VEC_MINV) else rows
+
+    def get(self, which):
+        a = np.zeros(self._n(which))
+        check(lib().cf3hip_si_get(self.h, which, _p(a)))
+        return a
+
+    def set(self, which, arr):
+        a = _f64(arr).ravel()
+        assert a.size == self._n(which)
+        check(lib().cf3hip_si_set(self.h, which, _p(a)))
+
+    def step(self, fields, prm, u_solver, p_solver, nb_iterations=2, ubc=None, pbc=None):
+        """ubc / pbc: (mesh nodes, new values) of the velocity (n x dim) and pressure conditions."""
+        un, uv = (np.zeros(0, np.uint32), np.zeros(0)) if ubc is None else (_u32a(ubc[0]), _f64(ubc[1]))
+        pn, pv = (np.zeros(0, np.uint32), np.zeros(0)) if pbc is None else (_u32a(pbc[0]), _f64(pbc[1]))
+        stats = (SolveStats * (2 * nb_iterations))()
+        check(lib().cf3hip_si_step(self.h, fields.h, C.byref(prm), nb_iterations, C.byref(u_solver), C.byref(p_solver),
+                                   len(un), _p(un, _u32), _p(uv), len(pn), _p(pn, _u32), _p(pv), stats))
+        return list(stats)
+
+    def close(self):
+        if self.h:
+            check(lib().cf3hip_si_destroy(self.h))
             self.h = _vp()

@@ -11,6 +11,7 @@ struct cf3hip_mesh_s { cf3hip::Mesh m; };
 struct cf3hip_lss_s { cf3hip::Lss l; };
 struct cf3hip_fields_s { cf3hip::Fields f; };
 struct cf3hip_batch_s { cf3hip::Batch b; };
+struct cf3hip_si_s { cf3hip::SemiImplicit s; };
 
 namespace cf3hip {
 
@@ -1130,6 +1131,207 @@ int cf3hip_ns_update(cf3hip_lss lss, cf3hip_fields fields)
   REQUIRE(lss && fields, CF3HIP_ERR_INVALID_ARG, "null argument");
   ns_update(lss->l, fields->f);
   HIP_OK(hipStreamSynchronize(fields->f.mesh->ctx->stream));
+  API_END
+}
+
+
+// ---------------------------------------------------------------------------------------------
+// NavierStokesSemiImplicit
+// ---------------------------------------------------------------------------------------------
+int cf3hip_si_create(cf3hip_lss u_lss, cf3hip_lss p_lss, cf3hip_si* out)
+{
+  API_BEGIN
+  REQUIRE(u_lss && p_lss && out, CF3HIP_ERR_INVALID_ARG, "null argument");
+  auto* h = new cf3hip_si_s();
+  try {
+    si_create(h->s, u_lss->l, p_lss->l);
+  } catch(...) {
+    delete h;
+    throw;
+  }
+  *out = h;
+  API_END
+}
+
+int cf3hip_si_destroy(cf3hip_si si)
+{
+  API_BEGIN
+  if(si) {
+    HIP_OK(hipStreamSynchronize(si->s.ctx->stream));
+    delete si;
+  }
+  API_END
+}
+
+int cf3hip_si_initial(cf3hip_si si, cf3hip_fields fields, const cf3hip_ns_params* prm)
+{
+  API_BEGIN
+  REQUIRE(si && fields && prm, CF3HIP_ERR_INVALID_ARG, "null argument");
+  si_initial(si->s, fields->f, *prm);
+  API_END
+}
+
+int cf3hip_si_velocity_assemble(cf3hip_si si, cf3hip_fields fields, const cf3hip_ns_params* prm)
+{
+  API_BEGIN
+  REQUIRE(si && fields && prm, CF3HIP_ERR_INVALID_ARG, "null argument");
+  si_velocity_assemble(si->s, fields->f, *prm);
+  HIP_OK(hipStreamSynchronize(si->s.ctx->stream));
+  API_END
+}
+
+int cf3hip_si_set_solution(cf3hip_si si, cf3hip_fields fields)
+{
+  API_BEGIN
+  REQUIRE(si && fields, CF3HIP_ERR_INVALID_ARG, "null argument");
+  si_set_solution(si->s, fields->f);
+  HIP_OK(hipStreamSynchronize(si->s.ctx->stream));
+  API_END
+}
+
+int cf3hip_si_action(cf3hip_si si, cf3hip_fields fields, const cf3hip_ns_params* prm, int which)
+{
+  API_BEGIN
+  REQUIRE(si && fields && prm, CF3HIP_ERR_INVALID_ARG, "null argument");
+  si_action(si->s, fields->f, *prm, which);
+  HIP_OK(hipStreamSynchronize(si->s.ctx->stream));
+  API_END
+}
+
+namespace {
+
+DevBuf& si_vector(SemiImplicit& S, int which, size_t& n)
+{
+  const size_t rows = S.u->n_nodes, dim = S.u->neq;
+  switch(which) {
+    case CF3HIP_SI_VEC_A: n = rows * dim; return S.A;
+    case CF3HIP_SI_VEC_U: n = rows * dim; return S.Uv;
+    case CF3HIP_SI_VEC_P: n = rows; return S.Pv;
+    case CF3HIP_SI_VEC_DPS: n = rows; return S.dps;
+    case CF3HIP_SI_VEC_MINV: n = rows * dim; return S.minv;
+  }
+  CF3_THROW(CF3HIP_ERR_INVALID_ARG, "unknown semi-implicit vector");
+}
+
+// Dirichlet increments new - old against a field, at the rows of the given mesh nodes
+void si_conditions(SemiImplicit& S, const std::vector<const double*>& field, uint32_t n, const uint32_t* nodes,
+                   const double* values, std::vector<uint32_t>& rows, std::vector<uint32_t>& eqs,
+                   std::vector<double>& inc)
+{
+  const int nc = (int)field.size();
+  std::vector<uint32_t> p2m(S.mesh->n_nodes);
+  HIP_OK(hipMemcpy(p2m.data(), S.p2m.ptr, sizeof(uint32_t) * p2m.size(), hipMemcpyDeviceToHost));
+  for(uint32_t b = 0; b < n; ++b) {
+    REQUIRE(nodes[b] < S.mesh->n_nodes, CF3HIP_ERR_INVALID_ARG, "boundary node out of range");
+    for(int c = 0; c < nc; ++c) {
+      double old;
+      HIP_OK(hipMemcpy(&old, field[c] + nodes[b], sizeof(double), hipMemcpyDeviceToHost));
+      rows.push_back(p2m[nodes[b]]);
+      eqs.push_back((uint32_t)c);
+      inc.push_back(values[(size_t)b * nc + c] - old);
+    }
+  }
+}
+
+}  // namespace
+
+int cf3hip_si_get(cf3hip_si si, int which, double* host)
+{
+  API_BEGIN
+  REQUIRE(si && host, CF3HIP_ERR_INVALID_ARG, "null argument");
+  size_t n = 0;
+  DevBuf& b = si_vector(si->s, which, n);
+  HIP_OK(hipStreamSynchronize(si->s.ctx->stream));
+  HIP_OK(hipMemcpy(host, b.ptr, sizeof(double) * n, hipMemcpyDeviceToHost));
+  API_END
+}
+
+int cf3hip_si_set(cf3hip_si si, int which, const double* host)
+{
+  API_BEGIN
+  REQUIRE(si && host, CF3HIP_ERR_INVALID_ARG, "null argument");
+  size_t n = 0;
+  DevBuf& b = si_vector(si->s, which, n);
+  HIP_OK(hipStreamSynchronize(si->s.ctx->stream));
+  HIP_OK(hipMemcpy(b.ptr, host, sizeof(double) * n, hipMemcpyHostToDevice));
+  API_END
+}
+
+int cf3hip_si_step(cf3hip_si si, cf3hip_fields fields, const cf3hip_ns_params* prm, int nb_iterations,
+                   const cf3hip_solver_params* u_solver, const cf3hip_solver_params* p_solver, uint32_t n_ubc,
+                   const uint32_t* ubc_nodes, const double* ubc_values, uint32_t n_pbc, const uint32_t* pbc_nodes,
+                   const double* pbc_values, cf3hip_solve_stats* stats)
+{
+  API_BEGIN
+  REQUIRE(si && fields && prm && u_solver && p_solver, CF3HIP_ERR_INVALID_ARG, "null argument");
+  REQUIRE(nb_iterations > 0, CF3HIP_ERR_INVALID_ARG, "nb_iterations must be positive");
+  REQUIRE((n_ubc == 0 || (ubc_nodes && ubc_values)) && (n_pbc == 0 || (pbc_nodes && pbc_values)),
+          CF3HIP_ERR_INVALID_ARG, "null boundary condition arrays");
+  REQUIRE(u_solver->precision == CF3HIP_FP64 && p_solver->precision == CF3HIP_FP64, CF3HIP_ERR_NOT_IMPLEMENTED,
+          "the semi-implicit systems are solved in FP64");
+  SemiImplicit& S = si->s;
+  REQUIRE(S.initialized, CF3HIP_ERR_NOT_CREATED, "cf3hip_si_initial has not run");
+  Fields& F = fields->f;
+  Lss &U = *S.u, &P = *S.p;
+  Context& c = *S.ctx;
+  const double dt = prm->dt;
+  // conditions against the fields before the step (the fields change only in Update)
+  std::vector<uint32_t> urows, ueqs, prows, peqs;
+  std::vector<double> uinc, pinc;
+  std::vector<const double*> uf = {F.u[0].as<double>(), F.u[1].as<double>()};
+  si_conditions(S, uf, n_ubc, ubc_nodes, ubc_values, urows, ueqs, uinc);
+  si_conditions(S, {F.p.as<double>()}, n_pbc, pbc_nodes, pbc_values, prows, peqs, pinc);
+  std::vector<uint32_t> udofs(urows.size()), pdofs(prows);
+  for(size_t j = 0; j < urows.size(); ++j) udofs[j] = urows[j] * U.neq + ueqs[j];
+  ns_linearize(F);                                     // LinearizeU
+  si_velocity_assemble(S, F, *prm);                    // VelocityLSS: ZeroLSS + assembly
+  si_set_solution(S, F);                               // SetSolution
+  // InnerLoop::execute
+  lss_fill(c, S.A.as<double>(), U.dofs, 0.);
+  lss_fill(c, S.dps.as<double>(), P.dofs, 0.);
+  lss_copy(c, S.Uv.as<double>(), U.sol.as<double>(), U.dofs);
+  lss_copy(c, S.Pv.as<double>(), P.sol.as<double>(), P.dofs);
+  for(int i = 0; i < nb_iterations; ++i) {
+    lss_fill(c, U.rhs.as<double>(), U.dofs, 0.);
+    si_action(S, F, *prm, CF3HIP_SI_URHS);
+    if(i == 0) {
+      if(!urows.empty()) {   // velocity BC, written in terms of the acceleration
+        lss_axpby(c, U.dofs, 0., U.rhs.as<double>(), dt, U.rhs.as<double>());
+        lss_dirichlet(U, (uint32_t)urows.size(), urows.data(), ueqs.data(), uinc.data());
+        lss_axpby(c, U.dofs, 0., U.rhs.as<double>(), 1. / dt, U.rhs.as<double>());
+      }
+    } else {
+      si_zero_dofs(S, udofs, U.rhs.as<double>());
+    }
+    lss_fill(c, U.sol.as<double>(), U.dofs, 0.);
+    cf3hip_solve_stats st{};
+    lss_solve(U, *u_solver, st);
+    if(stats) stats[2 * i] = st;
+    REQUIRE(st.converged, CF3HIP_ERR_NOT_CONVERGED, "velocity solve did not reach the tolerance");
+    lss_fill(c, P.rhs.as<double>(), P.dofs, 0.);
+    si_action(S, F, *prm, CF3HIP_SI_PRHS);
+    lss_fill(c, P.sol.as<double>(), P.dofs, 0.);
+    if(i == 0) {
+      // the matrix as assembled, then its conditions (TrilinosCrsMatrix::symmetric_dirichlet reaches the
+      // same matrix and rhs through its cache of eliminated column values)
+      lss_copy(c, P.values.as<double>(), S.pvals0.as<double>(), P.n_blocks);
+      lss_clear_conditions(P);
+      if(!prows.empty()) lss_dirichlet(P, (uint32_t)prows.size(), prows.data(), peqs.data(), pinc.data());
+    } else {
+      si_zero_dofs(S, pdofs, P.rhs.as<double>());
+    }
+    lss_solve(P, *p_solver, st);
+    if(stats) stats[2 * i + 1] = st;
+    REQUIRE(st.converged, CF3HIP_ERR_NOT_CONVERGED, "pressure solve did not reach the tolerance");
+    lss_fill(c, U.rhs.as<double>(), U.dofs, 0.);
+    si_action(S, F, *prm, CF3HIP_SI_AUP);
+    si_zero_dofs(S, udofs, U.rhs.as<double>());
+    si_correct(S, dt);
+  }
+  lss_copy(c, U.sol.as<double>(), S.Uv.as<double>(), U.dofs);
+  lss_copy(c, P.sol.as<double>(), S.Pv.as<double>(), P.dofs);
+  si_update(S, F);                                     // Update
+  HIP_OK(hipStreamSynchronize(c.stream));
   API_END
 }
 

@@ -161,6 +161,24 @@ struct Fields {
   bool has_g = false;   // the body force holds a nonzero value (its source term is skipped otherwise)
 };
 
+// NavierStokesSemiImplicit workspace: a velocity system (neq = dim) and a pressure system (neq = 1)
+// on one mesh; contribution lists of the deterministic gathers, the inner-loop vectors by LSS row
+struct SemiImplicit {
+  Context* ctx = nullptr;
+  Mesh* mesh = nullptr;
+  Lss* u = nullptr;
+  Lss* p = nullptr;
+  DevBuf mat_ptr, mat_idx;   // per block slot: its (e*k + a)*k + b contributions in element order
+  DevBuf vec_ptr, vec_idx;   // per row: its e*k + a incidences in element order
+  DevBuf src;                // per row: the last mesh node of the row (SetSolution)
+  DevBuf p2m;                // per mesh node: its row (the identity without periodic links)
+  DevBuf stage;              // element outputs, n_elems * k * k * 4 doubles
+  DevBuf A, Uv, minv, aup;   // rows x dim: a, u, 1 / lumped mass, scratch
+  DevBuf Pv, dps;            // rows: p, delta_p_sum
+  DevBuf pvals0;             // the pressure matrix before its Dirichlet conditions
+  bool initialized = false;
+};
+
 // batched FP32 assembly of many systems (serving): a device table of per-system arguments and one
 // persistent launch captured in a hipGraph on a private stream
 struct Batch {
@@ -211,6 +229,14 @@ double lss_residual(Lss& L);
 
 // nodes.hip
 void ns_linearize(Fields& F);
+void si_create(SemiImplicit& S, Lss& U, Lss& P);                                            // semi_implicit.hip
+void si_initial(SemiImplicit& S, Fields& F, const cf3hip_ns_params& prm);
+void si_velocity_assemble(SemiImplicit& S, Fields& F, const cf3hip_ns_params& prm);
+void si_action(SemiImplicit& S, Fields& F, const cf3hip_ns_params& prm, int which);
+void si_set_solution(SemiImplicit& S, Fields& F);
+void si_correct(SemiImplicit& S, double dt);
+void si_update(SemiImplicit& S, Fields& F);
+void si_zero_dofs(SemiImplicit& S, const std::vector<uint32_t>& dofs, double* v);
 void ns_update(Lss& L, Fields& F);
 
 inline unsigned grid_for(size_t n, unsigned block)

@@ -1,0 +1,696 @@
+// UFEM NavierStokesSemiImplicit on the device (gfx950): the element actions of
+// plugins/UFEM/src/UFEM/ns_semi_implicit/MatrixAssembly.hpp and PressureSystem.cpp, and the time step of
+// ns_semi_implicit/implementation/NavierStokesSemiImplicit.cpp (LinearizeU, velocity assembly,
+// SetSolution, InnerLoop, Update) on a velocity system (neq = dim) and a pressure system (neq = 1) that
+// share one mesh and its periodic links.  Triangles and quadrilaterals.
+//
+// Each element action runs in two passes.  k_si_elem computes one element per thread and writes its
+// element vector or matrix to a staging buffer; k_si_gather then gives every matrix block slot (or
+// vector row) one thread that adds its contributions in element order, so the sums are deterministic
+// and need no atomics.  The contribution lists are the element-to-slot map of the velocity system (the
+// two systems have the same node graph) and the rows p2m[conn], sorted stably once at set-up.
+#include "internal.hpp"
+
+#include <algorithm>
+#include <vector>
+
+namespace cf3hip {
+
+namespace {
+
+#pragma clang fp contract(off)
+
+// ------------------------------------------------------------------------------------------------
+// element types: shape functions, Gauss rules (cf3/mesh/LagrangeP1/Triag2D.cpp, Quad2D.cpp;
+// cf3/mesh/Integrators/GaussImplementation.hpp: triangle order 1 centroid w 1/2, order 2 edge
+// midpoints w 1/6; quadrilateral order 1 centre w 4, order 2 the points (x,x),(-x,x),(x,-x),(-x,-x))
+// ------------------------------------------------------------------------------------------------
+struct Tri {
+  static constexpr int K = 3, ORDER = 1;   // IntegralOrder (MatrixAssembly.hpp:28-47)
+  __device__ static int npts(int order) { return order == 1 ? 1 : 3; }
+  __device__ static void point(int order, int q, double xi[2], double& w)
+  {
+    if(order == 1) { xi[0] = xi[1] = 0.3333333333333333333333333; w = 0.5; return; }
+    xi[0] = q == 1 ? 0.5 : (q == 0 ? 0.5 : 0.0);
+    xi[1] = q == 0 ? 0.0 : 0.5;
+    w = 1.0 / 6.0;
+  }
+  __device__ static void shape(const double xi[2], double N[3], double dN[2][3])
+  {
+    N[0] = 1. - xi[0] - xi[1]; N[1] = xi[0]; N[2] = xi[1];
+    dN[0][0] = -1.; dN[0][1] = 1.; dN[0][2] = 0.;
+    dN[1][0] = -1.; dN[1][1] = 0.; dN[1][2] = 1.;
+  }
+};
+
+struct Quad {
+  static constexpr int K = 4, ORDER = 2;
+  __device__ static int npts(int order) { return order == 1 ? 1 : 4; }
+  __device__ static void point(int order, int q, double xi[2], double& w)
+  {
+    if(order == 1) { xi[0] = xi[1] = 0.; w = 4.; return; }
+    const double x = 0.5773502691896257645091488;
+    xi[0] = (q & 1) ? -x : x;
+    xi[1] = (q & 2) ? -x : x;
+    w = 1.;
+  }
+  __device__ static void shape(const double xi[2], double N[4], double dN[2][4])
+  {
+    const double s = xi[0], t = xi[1];
+    N[0] = 0.25 * (1. - s) * (1. - t); N[1] = 0.25 * (1. + s) * (1. - t);
+    N[2] = 0.25 * (1. + s) * (1. + t); N[3] = 0.25 * (1. - s) * (1. + t);
+    dN[0][0] = 0.25 * (-1. + t); dN[0][1] = 0.25 * (1. - t); dN[0][2] = 0.25 * (1. + t); dN[0][3] = 0.25 * (-1. - t);
+    dN[1][0] = 0.25 * (-1. + s); dN[1][1] = 0.25 * (-1. - s); dN[1][2] = 0.25 * (1. + s); dN[1][3] = 0.25 * (1. - s);
+  }
+};
+
+// N, physical gradients G = J^-1 dN (J(l, d) = dx_d / dxi_l) and det J at xi
+template<class E>
+__device__ __forceinline__ void point_data(const double X[E::K][2], const double xi[2], double N[E::K],
+                                           double G[2][E::K], double& det)
+{
+  double dN[2][E::K];
+  E::shape(xi, N, dN);
+  double J[2][2];
+  for(int l = 0; l < 2; ++l)
+    for(int d = 0; d < 2; ++d) {
+      double s = 0.;
+      for(int m = 0; m < E::K; ++m) s += dN[l][m] * X[m][d];
+      J[l][d] = s;
+    }
+  det = J[0][0] * J[1][1] - J[0][1] * J[1][0];
+  const double inv = 1. / det;
+  const double Ji[2][2] = {{J[1][1] * inv, -J[0][1] * inv}, {-J[1][0] * inv, J[0][0] * inv}};
+  for(int d = 0; d < 2; ++d)
+    for(int m = 0; m < E::K; ++m) G[d][m] = Ji[d][0] * dN[0][m] + Ji[d][1] * dN[1][m];
+}
+
+template<class E>
+__device__ __forceinline__ void inverse_jacobian(const double X[E::K][2], const double xi[2], double Ji[2][2])
+{
+  double N[E::K], dN[2][E::K], J[2][2];
+  E::shape(xi, N, dN);
+  for(int l = 0; l < 2; ++l)
+    for(int d = 0; d < 2; ++d) {
+      double s = 0.;
+      for(int m = 0; m < E::K; ++m) s += dN[l][m] * X[m][d];
+      J[l][d] = s;
+    }
+  const double inv = 1. / (J[0][0] * J[1][1] - J[0][1] * J[1][0]);
+  Ji[0][0] = J[1][1] * inv; Ji[0][1] = -J[0][1] * inv;
+  Ji[1][0] = -J[1][0] * inv; Ji[1][1] = J[0][0] * inv;
+}
+
+struct TauP { int type; double dt, c1, c2, a_ps, a_su, a_bu; };
+
+// ComputeTauImpl (SUPG.hpp:107-219), METRIC or TEZDUYAR, at the element centre with nodal velocity u
+// and element viscosity |mean(nu)|
+template<class E>
+__device__ void tau(const TauP& t, const double X[E::K][2], const double u[E::K][2], const double nu[E::K],
+                    double& tau_ps, double& tau_su, double& tau_bulk)
+{
+  double s = 0.;
+  for(int m = 0; m < E::K; ++m) s += nu[m];
+  const double enu = fabs(s / E::K);
+  double xi[2], w, N[E::K], G[2][E::K], det;
+  E::point(1, 0, xi, w);
+  point_data<E>(X, xi, N, G, det);
+  double uc[2];
+  for(int d = 0; d < 2; ++d) {
+    double v = 0.;
+    for(int m = 0; m < E::K; ++m) v += N[m] * u[m][d];
+    uc[d] = v;
+  }
+  if(t.type == CF3HIP_SUPG_TEZDUYAR) {
+    double h = 1e10;
+    for(int i = 0; i < E::K; ++i)
+      for(int j = 0; j < E::K; ++j)
+        if(i != j) {
+          const double dx = X[i][0] - X[j][0], dy = X[i][1] - X[j][1];
+          h = fmin(h, dx * dx + dy * dy);
+        }
+    h = sqrt(h);
+    const double umag = sqrt(uc[0] * uc[0] + uc[1] * uc[1]);
+    tau_su = 1. / ((2. * umag) / h + 2. / t.dt + (4. * enu) / (h * h));
+    tau_ps = tau_su;
+    tau_bulk = tau_su * umag * umag;
+  } else {
+    double Ji[2][2];
+    inverse_jacobian<E>(X, xi, Ji);
+    double g[2][2];
+    for(int i = 0; i < 2; ++i)
+      for(int j = 0; j < 2; ++j) g[i][j] = Ji[0][i] * Ji[0][j] + Ji[1][i] * Ji[1][j];
+    double ugu = 0.;
+    for(int j = 0; j < 2; ++j) ugu += (uc[0] * g[0][j] + uc[1] * g[1][j]) * uc[j];
+    const double adv2 = fabs(ugu);
+    const double gsq = g[0][0] * g[0][0] + g[0][1] * g[0][1] + g[1][0] * g[1][0] + g[1][1] * g[1][1];
+    const double tr = g[0][0] + g[1][1];
+    const double diff = enu * enu * gsq;
+    tau_su = 1. / sqrt((4. * t.c1 * t.c1 / (t.dt * t.dt)) + adv2 + t.c2 * diff);
+    tau_ps = tau_su;
+    const double su_std = 1. / sqrt((4. / (t.dt * t.dt)) + adv2 + 16. * diff);
+    tau_bulk = (1. / su_std) / tr;
+  }
+  tau_ps *= t.a_ps;
+  tau_su *= t.a_su;
+  tau_bulk *= t.a_bu;
+}
+
+enum { OP_MASS = 0, OP_PMAT = 1, OP_VMAT = 2, OP_URHS = 3, OP_PRHS = 4, OP_AUP = 5 };
+
+// element outputs per element: MASS K, PMAT K*K, VMAT K*K*4 (block (a,b) row-major 2x2),
+// URHS/AUP K*2 (node-major), PRHS K
+struct SiArgs {
+  uint32_t n_elems;
+  const uint32_t* conn;
+  const uint32_t* p2m;
+  const double* x[2];
+  const double* u[2];     // velocity field (tau of PRHS and of the pressure matrix)
+  const double* ua[2];    // advection velocity
+  const double* nu;
+  const double* a;        // LSS vectors (rows x 2 / rows)
+  const double* U;
+  const double* da;
+  const double* P;
+  const double* dps;
+  const double* dp;
+  double theta;
+  TauP t;
+  double* stage;
+};
+
+template<class E, int OP>
+__global__ void __launch_bounds__(256) k_si_elem(SiArgs g)
+{
+  constexpr int K = E::K;
+  const uint32_t e = blockIdx.x * blockDim.x + threadIdx.x;
+  if(e >= g.n_elems) return;
+  uint32_t cn[K], rw[K];
+  double X[K][2], ua[K][2], uf[K][2], nu[K];
+  for(int m = 0; m < K; ++m) {
+    cn[m] = g.conn[(uint64_t)e * K + m];
+    rw[m] = g.p2m[cn[m]];
+    X[m][0] = g.x[0][cn[m]]; X[m][1] = g.x[1][cn[m]];
+    ua[m][0] = g.ua[0][cn[m]]; ua[m][1] = g.ua[1][cn[m]];
+    uf[m][0] = g.u[0][cn[m]]; uf[m][1] = g.u[1][cn[m]];
+    nu[m] = g.nu[cn[m]];
+  }
+  const double dt = g.t.dt;
+  double tau_ps = 0., tau_su = 0., tau_bulk = 0.;
+  if(OP == OP_PMAT || OP == OP_PRHS) tau<E>(g.t, X, uf, nu, tau_ps, tau_su, tau_bulk);
+  else if(OP != OP_MASS) tau<E>(g.t, X, ua, nu, tau_ps, tau_su, tau_bulk);
+  double xi[2], w, N[K], G[2][K], det;
+
+  if constexpr(OP == OP_MASS || OP == OP_PMAT) {
+    // MassMatrixAssembly: row sums of int N^T N; assemble_pp: theta (tau_ps + dt) int G^T G (order 2)
+    double A[K][K];
+    for(int a = 0; a < K; ++a)
+      for(int b = 0; b < K; ++b) A[a][b] = 0.;
+    for(int q = 0; q < E::npts(2); ++q) {
+      E::point(2, q, xi, w);
+      point_data<E>(X, xi, N, G, det);
+      const double wq = w * det;
+      for(int a = 0; a < K; ++a)
+        for(int b = 0; b < K; ++b)
+          A[a][b] += OP == OP_MASS ? (wq * N[a]) * N[b] : wq * (G[0][a] * G[0][b] + G[1][a] * G[1][b]);
+    }
+    double* out = g.stage + (uint64_t)e * (OP == OP_MASS ? K : K * K);
+    if(OP == OP_MASS) {
+      for(int a = 0; a < K; ++a) {
+        double s = 0.;
+        for(int b = 0; b < K; ++b) s += A[a][b];
+        out[a] = s;
+      }
+    } else {
+      const double c = g.theta * (tau_ps + dt);
+      for(int a = 0; a < K; ++a)
+        for(int b = 0; b < K; ++b) out[a * K + b] = c * A[a][b];
+    }
+  } else if constexpr(OP == OP_VMAT) {
+    // VelocityAssembly (MatrixAssembly.hpp:49-146): T + theta dt M
+    double T[K][K], L[K][K], B[2][K][K];
+    for(int a = 0; a < K; ++a)
+      for(int b = 0; b < K; ++b) { T[a][b] = 0.; L[a][b] = 0.; B[0][a][b] = 0.; B[1][a][b] = 0.; }
+    for(int q = 0; q < E::npts(2); ++q) {   // mass with the second-order rule
+      E::point(2, q, xi, w);
+      point_data<E>(X, xi, N, G, det);
+      const double wq = w * det;
+      for(int a = 0; a < K; ++a)
+        for(int b = 0; b < K; ++b) T[a][b] += (wq * N[a]) * N[b];
+    }
+    for(int q = 0; q < E::npts(E::ORDER); ++q) {   // viscous and bulk diagonal blocks
+      E::point(E::ORDER, q, xi, w);
+      point_data<E>(X, xi, N, G, det);
+      double nuq = 0.;
+      for(int m = 0; m < K; ++m) nuq += N[m] * nu[m];
+      const double wq = w * det, bulk = wq * tau_bulk, wn = wq * nuq;
+      for(int a = 0; a < K; ++a)
+        for(int b = 0; b < K; ++b) {
+          L[a][b] += wn * (G[0][a] * G[0][b] + G[1][a] * G[1][b]);
+          B[0][a][b] += (bulk * G[0][a]) * G[0][b];
+          B[1][a][b] += (bulk * G[1][a]) * G[1][b];
+        }
+    }
+    E::point(1, 0, xi, w);   // off-diagonal bulk block with the one-point rule
+    point_data<E>(X, xi, N, G, det);
+    const double bulk = w * det * tau_bulk, tdt = g.theta * dt;
+    double* out = g.stage + (uint64_t)e * K * K * 4;
+    for(int a = 0; a < K; ++a)
+      for(int b = 0; b < K; ++b) {
+        double* o = out + (a * K + b) * 4;
+        o[0] = T[a][b] + tdt * (L[a][b] + B[0][a][b]);
+        o[1] = tdt * ((bulk * G[0][a]) * G[1][b]);
+        o[2] = tdt * ((bulk * G[0][b]) * G[1][a]);
+        o[3] = T[a][b] + tdt * (L[a][b] + B[1][a][b]);
+      }
+  } else if constexpr(OP == OP_URHS) {
+    // VelocityRHS (MatrixAssembly.hpp:206-330) with a, dt (1 - theta) a - u_vec, (1 - theta) dps - p_vec
+    double av[K][2], dtamu[K][2], ppdp[K];
+    for(int m = 0; m < K; ++m) {
+      const uint64_t r = rw[m];
+      for(int d = 0; d < 2; ++d) {
+        av[m][d] = g.a[r * 2 + d];
+        dtamu[m][d] = dt * (1. - g.theta) * av[m][d] - g.U[r * 2 + d];
+      }
+      ppdp[m] = (1. - g.theta) * g.dps[r] - g.P[r];
+    }
+    double res[2][K];
+    for(int a = 0; a < K; ++a) res[0][a] = res[1][a] = 0.;
+    for(int pass = 0; pass < (E::ORDER == 2 ? 1 : 2); ++pass) {
+      // simplices: pass 0 = mass-type terms with the second-order rule, pass 1 = the rest (order 1);
+      // quadrilaterals: all terms in one second-order loop
+      const bool mass_terms = E::ORDER == 2 || pass == 0;
+      const bool other_terms = E::ORDER == 2 || pass == 1;
+      const int order = (E::ORDER == 2 || pass == 0) ? 2 : 1;
+      for(int q = 0; q < E::npts(order); ++q) {
+        E::point(order, q, xi, w);
+        point_data<E>(X, xi, N, G, det);
+        const double wq = w * det;
+        double uq[2];
+        for(int d = 0; d < 2; ++d) {
+          double v = 0.;
+          for(int m = 0; m < K; ++m) v += N[m] * ua[m][d];
+          uq[d] = v;
+        }
+        double adv[K], npa[K];
+        for(int m = 0; m < K; ++m) { adv[m] = uq[0] * G[0][m] + uq[1] * G[1][m]; npa[m] = tau_su * adv[m] + N[m]; }
+        double f = 0.;
+        for(int i = 0; i < 2; ++i)
+          for(int m = 0; m < K; ++m) f += G[i][m] * dtamu[m][i];
+        f *= wq;
+        double nuq = 0., np = 0.;
+        for(int m = 0; m < K; ++m) { nuq += N[m] * nu[m]; np += N[m] * ppdp[m]; }
+        const double b = -wq * np, f_bulk = tau_bulk * f, f_skew = 0.5 * f;
+        for(int i = 0; i < 2; ++i) {
+          double na = 0., ad = 0., gp = 0., dv[2] = {0., 0.};
+          for(int m = 0; m < K; ++m) {
+            na += N[m] * av[m][i];
+            ad += adv[m] * dtamu[m][i];
+            gp += G[i][m] * ppdp[m];
+            dv[0] += G[0][m] * dtamu[m][i];
+            dv[1] += G[1][m] * dtamu[m][i];
+          }
+          const double c = -wq * na, ee = wq * ad, a_ = (wq * tau_su) * gp, wv = wq * nuq;
+          for(int a = 0; a < K; ++a) {
+            double v = 0.;
+            if(other_terms) v += adv[a] * a_ + G[i][a] * (b + f_bulk) + (G[0][a] * (wv * dv[0]) + G[1][a] * (wv * dv[1]));
+            if(mass_terms) v += npa[a] * (c + ee + f_skew * uq[i]);
+            res[i][a] += v;
+          }
+        }
+      }
+    }
+    double* out = g.stage + (uint64_t)e * K * 2;
+    for(int a = 0; a < K; ++a) { out[a * 2] = res[0][a]; out[a * 2 + 1] = res[1][a]; }
+  } else if constexpr(OP == OP_PRHS) {
+    // PressureRHS (MatrixAssembly.hpp:148-204)
+    double upda[K][2], apda[K][2], pv[K];
+    for(int m = 0; m < K; ++m) {
+      const uint64_t r = rw[m];
+      for(int d = 0; d < 2; ++d) {
+        upda[m][d] = g.U[r * 2 + d] + dt * g.da[r * 2 + d];
+        apda[m][d] = g.a[r * 2 + d] + g.da[r * 2 + d];
+      }
+      pv[m] = g.P[r];
+    }
+    double res[K], App[K][K];
+    for(int a = 0; a < K; ++a) {
+      res[a] = 0.;
+      for(int b = 0; b < K; ++b) App[a][b] = 0.;
+    }
+    for(int q = 0; q < E::npts(E::ORDER); ++q) {
+      E::point(E::ORDER, q, xi, w);
+      point_data<E>(X, xi, N, G, det);
+      const double wq = w * det, tw = wq * tau_ps;
+      double uq[2];
+      for(int d = 0; d < 2; ++d) {
+        double v = 0.;
+        for(int m = 0; m < K; ++m) v += N[m] * ua[m][d];
+        uq[d] = v;
+      }
+      double adv[K];
+      for(int m = 0; m < K; ++m) adv[m] = tau_ps * (uq[0] * G[0][m] + uq[1] * G[1][m]);
+      for(int a = 0; a < K; ++a)
+        for(int b = 0; b < K; ++b) App[a][b] += tw * (G[0][a] * G[0][b] + G[1][a] * G[1][b]);
+      for(int i = 0; i < 2; ++i) {
+        double ga = 0., av = 0., na = 0.;
+        for(int m = 0; m < K; ++m) { ga += G[i][m] * upda[m][i]; av += adv[m] * upda[m][i]; na += N[m] * apda[m][i]; }
+        const double a_ = wq * ga, b = wq * av, c = tw * na;
+        for(int m = 0; m < K; ++m) res[m] -= a_ * (N[m] + adv[m] * 0.5) + (b + c) * G[i][m];
+      }
+    }
+    double* out = g.stage + (uint64_t)e * K;
+    for(int a = 0; a < K; ++a) {
+      double s = 0.;
+      for(int b = 0; b < K; ++b) s += App[a][b] * pv[b];
+      out[a] = res[a] - s;
+    }
+  } else {
+    // ApplyAup (MatrixAssembly.hpp:332-376)
+    double dp[K];
+    for(int m = 0; m < K; ++m) dp[m] = g.dp[rw[m]];
+    double res[2][K];
+    for(int a = 0; a < K; ++a) res[0][a] = res[1][a] = 0.;
+    for(int q = 0; q < E::npts(E::ORDER); ++q) {
+      E::point(E::ORDER, q, xi, w);
+      point_data<E>(X, xi, N, G, det);
+      const double wq = g.theta * w * det, ws = wq * tau_su;
+      double uq[2];
+      for(int d = 0; d < 2; ++d) {
+        double v = 0.;
+        for(int m = 0; m < K; ++m) v += N[m] * ua[m][d];
+        uq[d] = v;
+      }
+      double nd = 0.;
+      for(int m = 0; m < K; ++m) nd += N[m] * dp[m];
+      const double b = -wq * nd;
+      for(int i = 0; i < 2; ++i) {
+        double a_ = 0.;
+        for(int m = 0; m < K; ++m) a_ += G[i][m] * dp[m];
+        for(int m = 0; m < K; ++m) res[i][m] += a_ * (ws * (uq[0] * G[0][m] + uq[1] * G[1][m])) + b * G[i][m];
+      }
+    }
+    double* out = g.stage + (uint64_t)e * K * 2;
+    for(int a = 0; a < K; ++a) { out[a * 2] = res[0][a]; out[a * 2 + 1] = res[1][a]; }
+  }
+}
+
+// out[r * width + c] = (add ? out : 0) + sum over j in [ptr[r], ptr[r+1]) of stage[idx[j] * width + c],
+// in list (element) order
+__global__ void __launch_bounds__(256) k_si_gather(uint32_t n, int width, const uint32_t* __restrict__ ptr,
+                                                   const uint32_t* __restrict__ idx,
+                                                   const double* __restrict__ stage, double* __restrict__ out,
+                                                   int add)
+{
+  const uint64_t t = (uint64_t)blockIdx.x * blockDim.x + threadIdx.x;
+  if(t >= (uint64_t)n * width) return;
+  const uint32_t r = (uint32_t)(t / width);
+  const int c = (int)(t - (uint64_t)r * width);
+  double s = add ? out[t] : 0.;
+  for(uint32_t j = ptr[r]; j < ptr[r + 1]; ++j) s += stage[(uint64_t)idx[j] * width + c];
+  out[t] = s;
+}
+
+// SetSolution: row r takes the value of the last mesh node of that row (a node loop in node order)
+__global__ void k_si_set_solution(uint32_t rows, int dim, const uint32_t* __restrict__ src, const double* f0,
+                                  const double* f1, double* __restrict__ out)
+{
+  const uint32_t r = blockIdx.x * blockDim.x + threadIdx.x;
+  if(r >= rows) return;
+  out[(uint64_t)r * dim] = f0[src[r]];
+  if(dim == 2) out[(uint64_t)r * dim + 1] = f1[src[r]];
+}
+
+// delta_a = delta_a - Ml^-1 (Aup delta_p); a += delta_a; U += dt delta_a  (InnerLoop::execute :162-175)
+__global__ void k_si_correct_u(uint64_t n, double dt, const double* __restrict__ minv, const double* __restrict__ aup,
+                               double* __restrict__ da, double* __restrict__ a, double* __restrict__ U)
+{
+  const uint64_t i = (uint64_t)blockIdx.x * blockDim.x + threadIdx.x;
+  if(i >= n) return;
+  const double d = -1. * (minv[i] * aup[i]) + 1. * da[i];
+  da[i] = d;
+  a[i] += d;
+  U[i] += dt * d;
+}
+
+// p += dp; delta_p_sum += dp
+__global__ void k_si_correct_p(uint64_t n, const double* __restrict__ dp, double* __restrict__ P,
+                               double* __restrict__ dps)
+{
+  const uint64_t i = (uint64_t)blockIdx.x * blockDim.x + threadIdx.x;
+  if(i >= n) return;
+  P[i] += dp[i];
+  dps[i] += dp[i];
+}
+
+__global__ void k_si_reciprocal(uint64_t n, const double* __restrict__ x, double* __restrict__ y)
+{
+  const uint64_t i = (uint64_t)blockIdx.x * blockDim.x + threadIdx.x;
+  if(i < n) y[i] = 1. / x[i];
+}
+
+// zero the given dofs of v
+__global__ void k_si_zero_dofs(uint32_t n, const uint32_t* __restrict__ dofs, double* __restrict__ v)
+{
+  const uint32_t i = blockIdx.x * blockDim.x + threadIdx.x;
+  if(i < n) v[dofs[i]] = 0.;
+}
+
+// Update: u3 = u2, u2 = u1, u1 = u, u = U(p2m), p = P(p2m)
+__global__ void k_si_update(uint32_t n, const uint32_t* __restrict__ p2m, double* u0, double* u1_0, double* u2_0,
+                            double* u3_0, double* u0b, double* u1_1, double* u2_1, double* u3_1,
+                            double* __restrict__ p, const double* __restrict__ U, const double* __restrict__ P)
+{
+  const uint32_t i = blockIdx.x * blockDim.x + threadIdx.x;
+  if(i >= n) return;
+  const uint64_t r = p2m[i];
+  u3_0[i] = u2_0[i]; u2_0[i] = u1_0[i]; u1_0[i] = u0[i]; u0[i] = U[r * 2];
+  u3_1[i] = u2_1[i]; u2_1[i] = u1_1[i]; u1_1[i] = u0b[i]; u0b[i] = U[r * 2 + 1];
+  p[i] = P[r];
+}
+
+void stable_lists(const std::vector<uint32_t>& keys, uint32_t n_keys, std::vector<uint32_t>& ptr,
+                  std::vector<uint32_t>& idx)
+{
+  ptr.assign(n_keys + 1, 0);
+  for(uint32_t k : keys) ++ptr[k + 1];
+  for(uint32_t i = 0; i < n_keys; ++i) ptr[i + 1] += ptr[i];
+  std::vector<uint32_t> fill(ptr.begin(), ptr.end() - 1);
+  idx.assign(keys.size(), 0);
+  for(size_t j = 0; j < keys.size(); ++j) idx[fill[keys[j]]++] = (uint32_t)j;
+}
+
+}  // namespace
+
+// ------------------------------------------------------------------------------------------------
+void si_create(SemiImplicit& S, Lss& U, Lss& P)
+{
+  if(!U.mesh || U.mesh != P.mesh)
+    CF3_THROW(CF3HIP_ERR_INVALID_ARG, "the velocity and pressure systems must be created from one mesh");
+  Mesh& M = *U.mesh;
+  if(M.etype != CF3HIP_TRIAG2D && M.etype != CF3HIP_QUAD2D)
+    CF3_THROW(CF3HIP_ERR_NOT_IMPLEMENTED, "the semi-implicit element actions run on triangles and quadrilaterals");
+  if(U.neq != M.dim || P.neq != 1)
+    CF3_THROW(CF3HIP_ERR_INVALID_ARG, "velocity system needs neq = dim, pressure system neq = 1");
+  if(U.periodic != P.periodic || U.n_nodes != P.n_nodes || U.n_blocks != P.n_blocks)
+    CF3_THROW(CF3HIP_ERR_INVALID_ARG, "the two systems must share the node graph and periodic links");
+  Context& c = *M.ctx;
+  S.ctx = &c;
+  S.mesh = &M;
+  S.u = &U;
+  S.p = &P;
+  const uint32_t E = M.n_elems, K = M.k, rows = U.n_nodes;
+  std::vector<uint32_t> p2m(M.n_nodes), conn((size_t)E * K), slots((size_t)E * K * K);
+  if(U.periodic) HIP_OK(hipMemcpy(p2m.data(), U.p2m.ptr, sizeof(uint32_t) * M.n_nodes, hipMemcpyDeviceToHost));
+  else for(uint32_t i = 0; i < M.n_nodes; ++i) p2m[i] = i;
+  HIP_OK(hipMemcpy(conn.data(), M.conn.ptr, sizeof(uint32_t) * conn.size(), hipMemcpyDeviceToHost));
+  HIP_OK(hipMemcpy(slots.data(), U.slots.ptr, sizeof(uint32_t) * slots.size(), hipMemcpyDeviceToHost));
+  std::vector<uint32_t> rowkeys(conn.size()), ptr, idx;
+  for(size_t j = 0; j < conn.size(); ++j) rowkeys[j] = p2m[conn[j]];
+  auto up = [&](DevBuf& b, const std::vector<uint32_t>& h) {
+    b.alloc(&c, sizeof(uint32_t) * std::max<size_t>(h.size(), 1));
+    HIP_OK(hipMemcpy(b.ptr, h.data(), sizeof(uint32_t) * h.size(), hipMemcpyHostToDevice));
+  };
+  stable_lists(slots, (uint32_t)U.n_blocks, ptr, idx);
+  up(S.mat_ptr, ptr);
+  up(S.mat_idx, idx);
+  stable_lists(rowkeys, rows, ptr, idx);
+  up(S.vec_ptr, ptr);
+  up(S.vec_idx, idx);
+  std::vector<uint32_t> src(rows, 0);
+  for(uint32_t i = 0; i < M.n_nodes; ++i) src[p2m[i]] = i;   // node order: the last node of a row wins
+  up(S.src, src);
+  up(S.p2m, p2m);
+  S.stage.alloc(&c, sizeof(double) * (size_t)E * K * K * 4);
+  const size_t vd = (size_t)rows * M.dim;
+  for(DevBuf* b : {&S.A, &S.Uv, &S.minv, &S.aup}) { b->alloc(&c, sizeof(double) * vd); lss_fill(c, b->as<double>(), vd, 0.); }
+  for(DevBuf* b : {&S.Pv, &S.dps}) { b->alloc(&c, sizeof(double) * rows); lss_fill(c, b->as<double>(), rows, 0.); }
+  S.pvals0.alloc(&c, sizeof(double) * P.n_blocks);
+  HIP_OK(hipStreamSynchronize(c.stream));
+}
+
+namespace {
+
+SiArgs si_args(SemiImplicit& S, Fields& F, const cf3hip_ns_params& prm)
+{
+  Mesh& M = *S.mesh;
+  SiArgs g{};
+  g.n_elems = M.n_elems;
+  g.conn = M.conn.as<uint32_t>();
+  g.p2m = S.p2m.as<uint32_t>();
+  for(int d = 0; d < 2; ++d) {
+    g.x[d] = M.x[d].as<double>();
+    g.u[d] = F.u[d].as<double>();
+    g.ua[d] = F.uadv[d].as<double>();
+  }
+  g.nu = F.nu.as<double>();
+  g.a = S.A.as<double>();
+  g.U = S.Uv.as<double>();
+  g.da = S.u->sol.as<double>();
+  g.P = S.Pv.as<double>();
+  g.dps = S.dps.as<double>();
+  g.dp = S.p->sol.as<double>();
+  g.theta = prm.theta;
+  g.t = TauP{prm.supg_type, prm.dt, prm.c1, prm.c2, prm.alpha_ps, prm.alpha_su, prm.alpha_bu};
+  g.stage = S.stage.as<double>();
+  return g;
+}
+
+template<int OP>
+void launch_elem(SemiImplicit& S, const SiArgs& g)
+{
+  const dim3 grid(grid_for(g.n_elems, 256));
+  if(S.mesh->etype == CF3HIP_TRIAG2D) hipLaunchKernelGGL((k_si_elem<Tri, OP>), grid, dim3(256), 0, S.ctx->stream, g);
+  else hipLaunchKernelGGL((k_si_elem<Quad, OP>), grid, dim3(256), 0, S.ctx->stream, g);
+  HIP_OK(hipGetLastError());
+}
+
+void gather(SemiImplicit& S, bool matrix, int width, uint32_t n, double* out, bool add)
+{
+  const uint64_t t = (uint64_t)n * width;
+  hipLaunchKernelGGL(k_si_gather, dim3(grid_for(t, 256)), dim3(256), 0, S.ctx->stream, n, width,
+                     (matrix ? S.mat_ptr : S.vec_ptr).as<uint32_t>(), (matrix ? S.mat_idx : S.vec_idx).as<uint32_t>(),
+                     S.stage.as<double>(), out, add ? 1 : 0);
+  HIP_OK(hipGetLastError());
+}
+
+void check_fields(SemiImplicit& S, Fields& F)
+{
+  if(F.mesh != S.mesh) CF3_THROW(CF3HIP_ERR_INVALID_ARG, "fields belong to another mesh");
+}
+
+}  // namespace
+
+void si_initial(SemiImplicit& S, Fields& F, const cf3hip_ns_params& prm)
+{
+  check_fields(S, F);
+  Lss &U = *S.u, &P = *S.p;
+  Context& c = *S.ctx;
+  const SiArgs g = si_args(S, F, prm);
+  // PressureAssembly: p_lss->reset(0) then the simplified pressure matrix (PressureSystem.cpp:120-176)
+  lss_clear_conditions(P);
+  lss_fill(c, P.rhs.as<double>(), P.dofs, 0.);
+  lss_fill(c, P.sol.as<double>(), P.dofs, 0.);
+  launch_elem<OP_PMAT>(S, g);
+  gather(S, true, 1, (uint32_t)P.n_blocks, P.values.as<double>(), false);
+  lss_copy(c, S.pvals0.as<double>(), P.values.as<double>(), P.n_blocks);
+  // ZeroVelocitySystem + MassMatrixAssembly into the velocity rhs + SetupInnerLoopData (reciprocal)
+  lss_clear_conditions(U);
+  lss_fill(c, U.values.as<double>(), U.n_blocks * U.neq * U.neq, 0.);
+  lss_fill(c, U.sol.as<double>(), U.dofs, 0.);
+  launch_elem<OP_MASS>(S, g);
+  gather(S, false, 1, U.n_nodes, S.aup.as<double>(), false);   // per row; the same for every component
+  // rhs[row * dim + i] = lumped(row)
+  std::vector<double> lumped(U.n_nodes), rhs(U.dofs);
+  HIP_OK(hipMemcpyAsync(lumped.data(), S.aup.ptr, sizeof(double) * U.n_nodes, hipMemcpyDeviceToHost, c.stream));
+  HIP_OK(hipStreamSynchronize(c.stream));
+  for(uint32_t r = 0; r < U.n_nodes; ++r)
+    for(int d = 0; d < U.neq; ++d) rhs[(size_t)r * U.neq + d] = lumped[r];
+  HIP_OK(hipMemcpyAsync(U.rhs.ptr, rhs.data(), sizeof(double) * U.dofs, hipMemcpyHostToDevice, c.stream));
+  hipLaunchKernelGGL(k_si_reciprocal, dim3(grid_for(U.dofs, 256)), dim3(256), 0, c.stream, U.dofs,
+                     U.rhs.as<double>(), S.minv.as<double>());
+  HIP_OK(hipGetLastError());
+  HIP_OK(hipStreamSynchronize(c.stream));
+  S.initialized = true;
+}
+
+void si_velocity_assemble(SemiImplicit& S, Fields& F, const cf3hip_ns_params& prm)
+{
+  check_fields(S, F);
+  Lss& U = *S.u;
+  Context& c = *S.ctx;
+  // ZeroLSS (matrix, rhs and solution) + VelocityAssembly
+  lss_clear_conditions(U);
+  lss_fill(c, U.rhs.as<double>(), U.dofs, 0.);
+  lss_fill(c, U.sol.as<double>(), U.dofs, 0.);
+  launch_elem<OP_VMAT>(S, si_args(S, F, prm));
+  gather(S, true, 4, (uint32_t)U.n_blocks, U.values.as<double>(), false);
+}
+
+void si_action(SemiImplicit& S, Fields& F, const cf3hip_ns_params& prm, int which)
+{
+  check_fields(S, F);
+  const SiArgs g = si_args(S, F, prm);
+  if(which == CF3HIP_SI_URHS) {
+    launch_elem<OP_URHS>(S, g);
+    gather(S, false, 2, S.u->n_nodes, S.u->rhs.as<double>(), true);
+  } else if(which == CF3HIP_SI_PRHS) {
+    launch_elem<OP_PRHS>(S, g);
+    gather(S, false, 1, S.p->n_nodes, S.p->rhs.as<double>(), true);
+  } else if(which == CF3HIP_SI_AUP) {
+    launch_elem<OP_AUP>(S, g);
+    gather(S, false, 2, S.u->n_nodes, S.u->rhs.as<double>(), true);
+  } else {
+    CF3_THROW(CF3HIP_ERR_INVALID_ARG, "unknown semi-implicit action");
+  }
+}
+
+void si_set_solution(SemiImplicit& S, Fields& F)
+{
+  check_fields(S, F);
+  Context& c = *S.ctx;
+  const uint32_t rows = S.u->n_nodes;
+  hipLaunchKernelGGL(k_si_set_solution, dim3(grid_for(rows, 256)), dim3(256), 0, c.stream, rows, 2,
+                     S.src.as<uint32_t>(), F.u[0].as<double>(), F.u[1].as<double>(), S.u->sol.as<double>());
+  hipLaunchKernelGGL(k_si_set_solution, dim3(grid_for(rows, 256)), dim3(256), 0, c.stream, rows, 1,
+                     S.src.as<uint32_t>(), F.p.as<double>(), F.p.as<double>(), S.p->sol.as<double>());
+  HIP_OK(hipGetLastError());
+}
+
+void si_correct(SemiImplicit& S, double dt)
+{
+  Context& c = *S.ctx;
+  Lss &U = *S.u, &P = *S.p;
+  hipLaunchKernelGGL(k_si_correct_u, dim3(grid_for(U.dofs, 256)), dim3(256), 0, c.stream, (uint64_t)U.dofs, dt,
+                     S.minv.as<double>(), U.rhs.as<double>(), U.sol.as<double>(), S.A.as<double>(),
+                     S.Uv.as<double>());
+  hipLaunchKernelGGL(k_si_correct_p, dim3(grid_for(P.dofs, 256)), dim3(256), 0, c.stream, (uint64_t)P.dofs,
+                     P.sol.as<double>(), S.Pv.as<double>(), S.dps.as<double>());
+  HIP_OK(hipGetLastError());
+}
+
+void si_update(SemiImplicit& S, Fields& F)
+{
+  check_fields(S, F);
+  Mesh& M = *S.mesh;
+  hipLaunchKernelGGL(k_si_update, dim3(grid_for(M.n_nodes, 256)), dim3(256), 0, S.ctx->stream, M.n_nodes,
+                     S.p2m.as<uint32_t>(), F.u[0].as<double>(), F.u1[0].as<double>(), F.u2[0].as<double>(),
+                     F.u3[0].as<double>(), F.u[1].as<double>(), F.u1[1].as<double>(), F.u2[1].as<double>(),
+                     F.u3[1].as<double>(), F.p.as<double>(), S.Uv.as<double>(), S.Pv.as<double>());
+  HIP_OK(hipGetLastError());
+}
+
+void si_zero_dofs(SemiImplicit& S, const std::vector<uint32_t>& dofs, double* v)
+{
+  if(dofs.empty()) return;
+  Context& c = *S.ctx;
+  DevBuf d;
+  d.alloc(&c, sizeof(uint32_t) * dofs.size());
+  HIP_OK(hipMemcpyAsync(d.ptr, dofs.data(), sizeof(uint32_t) * dofs.size(), hipMemcpyHostToDevice, c.stream));
+  hipLaunchKernelGGL(k_si_zero_dofs, dim3(grid_for(dofs.size(), 256)), dim3(256), 0, c.stream,
+                     (uint32_t)dofs.size(), d.as<uint32_t>(), v);
+  HIP_OK(hipGetLastError());
+  HIP_OK(hipStreamSynchronize(c.stream));
+}
+
+}  // namespace cf3hip

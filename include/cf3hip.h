@@ -283,6 +283,47 @@ int cf3hip_batch_assemble(cf3hip_batch batch, uint32_t n, const cf3hip_lss* lss,
 int cf3hip_ns_residual(cf3hip_lss lss, cf3hip_fields fields, const cf3hip_ns_params* ns_prm, const double* x_dev,
                        double* r_dev);
 
+/* ---- UFEM NavierStokesSemiImplicit (plugins/UFEM/src/UFEM/ns_semi_implicit) on triangles and
+ *      quadrilaterals.  u_lss: the VelocityLSS system (neq = dim), p_lss: the PressureLSS system
+ *      (neq = 1), created from the same mesh with the same periodic links.  The "solution" vectors of
+ *      the two systems hold delta_a and delta_p inside the inner loop and u, p outside it
+ *      (implementation/NavierStokesSemiImplicit.cpp:84-190, :344, :354). ---- */
+typedef struct cf3hip_si_s* cf3hip_si;
+#define CF3HIP_SI_URHS 0    /* URHSAssembly: velocity rhs += velocity_rhs(...) (MatrixAssembly.hpp:206-330) */
+#define CF3HIP_SI_PRHS 1    /* PRHSAssembly: pressure rhs += pressure_rhs(...) (MatrixAssembly.hpp:148-204) */
+#define CF3HIP_SI_AUP  2    /* ApplyAup: velocity rhs += apply_aup(delta_p) (MatrixAssembly.hpp:332-376) */
+#define CF3HIP_SI_VEC_A    0   /* inner-loop vectors (InnerLoop members a, u, p, delta_p_sum, rows x ncomp) */
+#define CF3HIP_SI_VEC_U    1
+#define CF3HIP_SI_VEC_P    2
+#define CF3HIP_SI_VEC_DPS  3
+#define CF3HIP_SI_VEC_MINV 4   /* reciprocal of the lumped velocity mass (SetupInnerLoopData, :236-262) */
+int cf3hip_si_create(cf3hip_lss u_lss, cf3hip_lss p_lss, cf3hip_si* si);
+int cf3hip_si_destroy(cf3hip_si si);
+/* initial conditions: PressureSystemAssembly (PressureSystem.cpp:120-176: p_lss reset, the simplified
+ * pressure matrix theta (tau_ps + dt) int G^T G with tau_ps from u), ZeroVelocitySystem +
+ * MassMatrixAssembly (lumped mass into the velocity rhs) + SetupInnerLoopData (its reciprocal) */
+int cf3hip_si_initial(cf3hip_si si, cf3hip_fields fields, const cf3hip_ns_params* prm);
+/* VelocityLSS Assembly: ZeroLSS + velocity matrix T + theta dt M (MatrixAssembly.hpp:49-146, :447-460) */
+int cf3hip_si_velocity_assemble(cf3hip_si si, cf3hip_fields fields, const cf3hip_ns_params* prm);
+/* SetSolution (:344): solution(u) = u, solution(p) = p (node loop; the last node of a row wins) */
+int cf3hip_si_set_solution(cf3hip_si si, cf3hip_fields fields);
+/* one element action (CF3HIP_SI_*) accumulated into the rhs of its system from the inner-loop vectors,
+ * the solutions (delta_a, delta_p) and the fields */
+int cf3hip_si_action(cf3hip_si si, cf3hip_fields fields, const cf3hip_ns_params* prm, int which);
+/* inner-loop vectors, host arrays of rows x ncomp */
+int cf3hip_si_get(cf3hip_si si, int which, double* host);
+int cf3hip_si_set(cf3hip_si si, int which, const double* host);
+/* one time step: LinearizeU, velocity assembly, SetSolution, InnerLoop::execute with nb_iterations
+ * passes (URHS, velocity BC in acceleration form at the first pass, velocity solve, PRHS, pressure BC
+ * at the first pass, pressure solve, ApplyAup, delta_a -= Ml^-1 Aup delta_p, a/u/p/delta_p_sum
+ * updates) and Update.  Dirichlet conditions: the new values at mesh nodes (velocity: n x dim);
+ * the increments new - old are taken against the fields.  stats: 2 * nb_iterations entries
+ * (velocity and pressure solve of each pass), may be null. */
+int cf3hip_si_step(cf3hip_si si, cf3hip_fields fields, const cf3hip_ns_params* prm, int nb_iterations,
+                   const cf3hip_solver_params* u_solver, const cf3hip_solver_params* p_solver, uint32_t n_ubc,
+                   const uint32_t* ubc_nodes, const double* ubc_values, uint32_t n_pbc, const uint32_t* pbc_nodes,
+                   const double* pbc_values, cf3hip_solve_stats* stats);
+
 #if defined(__GNUC__)
 #pragma GCC visibility pop
 #endif

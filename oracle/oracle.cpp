@@ -1261,3 +1261,466 @@ void orc_update(uint32_t n, int dim, double* u, double* p, double* u1, double* u
 }
 
 } // extern "C"
+
+// ---------------------------------------------------------------------------------------------
+// NavierStokesSemiImplicit element actions (plugins/UFEM/src/UFEM/ns_semi_implicit/MatrixAssembly.hpp,
+// PressureSystem.cpp:166-176 and implementation/NavierStokesSemiImplicit.cpp).  Element vectors and
+// matrices are var-major like the Proto element matrices (component i of node a at i*k + a).
+// The velocity system has neq = dim, the pressure system neq = 1; both are indexed by LSS row (p2m).
+// ---------------------------------------------------------------------------------------------
+namespace {
+
+bool simplex(int et) { return et == ORC_TRIAG2D || et == ORC_TETRA3D; }
+
+// IntegralOrder (MatrixAssembly.hpp:28-47): 1 for triangles and tetrahedra, 2 otherwise
+int si_order(int et) { return simplex(et) ? 1 : 2; }
+
+// shape values N, physical gradients G (dim x k) and detJ at mapped point xi
+bool point_data(int et, const Real* xi, const Real* X, Real* N, Real* G, Real& det)
+{
+  const int k = enodes(et), dim = edim(et);
+  Real dN[24], J[9], Ji[9];
+  shape_values(et, xi, N);
+  shape_gradients(et, xi, dN);
+  jacobian(et, xi, X, J);
+  if(!inverse_det(dim, J, Ji, det)) return false;
+  for(int d = 0; d < dim; ++d)
+    for(int m = 0; m < k; ++m) {
+      Real s = 0.;
+      for(int l = 0; l < dim; ++l) s += Ji[d*dim+l] * dN[l*k+m];
+      G[d*k+m] = s;
+    }
+  return true;
+}
+
+// value at the point of nodal data f (k x dim, node-major)
+void interp(int k, int dim, const Real* N, const Real* f, Real* out)
+{
+  for(int d = 0; d < dim; ++d) {
+    Real s = 0.;
+    for(int m = 0; m < k; ++m) s += N[m]*f[m*dim+d];
+    out[d] = s;
+  }
+}
+
+Real element_nu(int k, const Real* nu)
+{
+  Real s = 0.;
+  for(int i = 0; i < k; ++i) s += nu[i];
+  return std::fabs(s / k);   // ComputeTauImpl::operator() (SUPG.hpp:110-115)
+}
+
+// VelocityAssembly (MatrixAssembly.hpp:49-146): M and T (dim*k)^2, then system += T + theta*dt*M (:458)
+int si_velocity_element(int et, const TauCfg& cfg, Real dt, Real theta, const Real* X, const Real* uadv,
+                        const Real* nu, Real* S)
+{
+  const int k = enodes(et), dim = edim(et), n = dim*k;
+  Real M[576], T[576];
+  std::fill(M, M + n*n, 0.);
+  std::fill(T, T + n*n, 0.);
+  Real tau_ps, tau_su, tau_bulk;
+  if(compute_tau(et, cfg, X, uadv, element_nu(k, nu), dt, tau_ps, tau_su, tau_bulk)) return -2;
+  Real gxi[24], gw[8], N[8], G[24], det;
+  const int order = si_order(et);
+  if(order != 2) {   // mass part with the second-order rule (:66-86)
+    const int ngp = gauss(et, 2, gxi, gw);
+    for(int q = 0; q < ngp; ++q) {
+      if(!point_data(et, gxi + q*dim, X, N, G, det)) return -2;
+      const Real w = gw[q]*det;
+      for(int i = 0; i < dim; ++i)
+        for(int a = 0; a < k; ++a)
+          for(int b = 0; b < k; ++b) T[(i*k+a)*n + i*k+b] += (w*N[a])*N[b];
+    }
+  }
+  const int ngp = gauss(et, order, gxi, gw);
+  for(int q = 0; q < ngp; ++q) {   // (:88-118)
+    if(!point_data(et, gxi + q*dim, X, N, G, det)) return -2;
+    Real nuq = 0.;
+    for(int m = 0; m < k; ++m) nuq += N[m]*nu[m];
+    const Real w = gw[q]*det;
+    const Real bulk = w*tau_bulk;
+    for(int a = 0; a < k; ++a)
+      for(int b = 0; b < k; ++b) {
+        Real gtg = 0.;
+        for(int d = 0; d < dim; ++d) gtg += G[d*k+a]*G[d*k+b];
+        const Real lap = (w*nuq)*gtg;
+        for(int i = 0; i < dim; ++i) {
+          M[(i*k+a)*n + i*k+b] += lap + (bulk*G[i*k+a])*G[i*k+b];
+          if(order == 2) T[(i*k+a)*n + i*k+b] += (w*N[a])*N[b];
+        }
+      }
+  }
+  // off-diagonal bulk blocks with the one-point rule (:120-145)
+  gauss(et, 1, gxi, gw);
+  if(!point_data(et, gxi, X, N, G, det)) return -2;
+  const Real bulk = gw[0]*det*tau_bulk;
+  for(int i = 0; i < dim; ++i)
+    for(int j = i+1; j < dim; ++j)
+      for(int a = 0; a < k; ++a)
+        for(int b = 0; b < k; ++b) {
+          const Real v = (bulk*G[i*k+a])*G[j*k+b];
+          M[(i*k+a)*n + j*k+b] += v;
+          M[(j*k+b)*n + i*k+a] += v;
+        }
+  const Real tdt = theta*dt;
+  for(int r = 0; r < n*n; ++r) S[r] = T[r] + tdt*M[r];
+  return 0;
+}
+
+// VelocityRHS (MatrixAssembly.hpp:206-330).  avec, dtamu: k x dim node-major (dt(1-theta) a - u);
+// ppdp: k ((1-theta) delta_p_sum - p).  res: dim*k var-major.
+int si_velocity_rhs_element(int et, const TauCfg& cfg, Real dt, const Real* X, const Real* uadv, const Real* nu,
+                            const Real* avec, const Real* dtamu, const Real* ppdp, Real* res)
+{
+  const int k = enodes(et), dim = edim(et);
+  std::fill(res, res + dim*k, 0.);
+  Real tau_ps, tau_su, tau_bulk;
+  if(compute_tau(et, cfg, X, uadv, element_nu(k, nu), dt, tau_ps, tau_su, tau_bulk)) return -2;
+  Real gxi[24], gw[8], N[8], G[24], det, uq[3], adv[8], npa[8];
+  const int order = si_order(et);
+  // adv = u.G, N_plus_adv = tau_su adv + N, f = w sum_i G_i . dtamu_i
+  auto common = [&](Real w, Real& f) {
+    for(int m = 0; m < k; ++m) {
+      Real s = 0.;
+      for(int d = 0; d < dim; ++d) s += uq[d]*G[d*k+m];
+      adv[m] = s;
+      npa[m] = tau_su*s + N[m];
+    }
+    f = 0.;
+    for(int i = 0; i < dim; ++i)
+      for(int m = 0; m < k; ++m) f += G[i*k+m]*dtamu[m*dim+i];
+    f *= w;
+  };
+  if(order != 2) {   // mass-type terms with the second-order rule (:236-268)
+    const int ngp = gauss(et, 2, gxi, gw);
+    for(int q = 0; q < ngp; ++q) {
+      if(!point_data(et, gxi + q*dim, X, N, G, det)) return -2;
+      interp(k, dim, N, uadv, uq);
+      const Real w = gw[q]*det;
+      Real f;
+      common(w, f);
+      const Real f_skew = 0.5*f;
+      for(int i = 0; i < dim; ++i) {
+        Real na = 0., ad = 0.;
+        for(int m = 0; m < k; ++m) { na += N[m]*avec[m*dim+i]; ad += adv[m]*dtamu[m*dim+i]; }
+        const Real c = -w*na, e = w*ad;
+        for(int a = 0; a < k; ++a) res[i*k+a] += npa[a]*(c + e + f_skew*uq[i]);
+      }
+    }
+  }
+  const int ngp = gauss(et, order, gxi, gw);
+  for(int q = 0; q < ngp; ++q) {   // (:270-326)
+    if(!point_data(et, gxi + q*dim, X, N, G, det)) return -2;
+    interp(k, dim, N, uadv, uq);
+    Real nuq = 0.;
+    for(int m = 0; m < k; ++m) nuq += N[m]*nu[m];
+    const Real w = gw[q]*det;
+    const Real w_visc = w*nuq, tau_su_w = w*tau_su;
+    Real np = 0.;
+    for(int m = 0; m < k; ++m) np += N[m]*ppdp[m];
+    const Real b = -w*np;
+    Real f;
+    common(w, f);
+    const Real f_bulk = tau_bulk*f, f_skew = 0.5*f;
+    for(int i = 0; i < dim; ++i) {
+      Real gp = 0.;
+      for(int m = 0; m < k; ++m) gp += G[i*k+m]*ppdp[m];
+      const Real a_ = tau_su_w*gp;
+      Real dv[3];
+      for(int l = 0; l < dim; ++l) {
+        Real s = 0.;
+        for(int m = 0; m < k; ++m) s += G[l*k+m]*dtamu[m*dim+i];
+        dv[l] = w_visc*s;
+      }
+      Real c = 0., e = 0.;
+      if(order == 2) {
+        Real na = 0., ad = 0.;
+        for(int m = 0; m < k; ++m) { na += N[m]*avec[m*dim+i]; ad += adv[m]*dtamu[m*dim+i]; }
+        c = -w*na; e = w*ad;
+      }
+      for(int a = 0; a < k; ++a) {
+        Real gd = 0.;
+        for(int l = 0; l < dim; ++l) gd += G[l*k+a]*dv[l];
+        Real v = adv[a]*a_ + G[i*k+a]*(b + f_bulk);
+        if(order == 2) v += npa[a]*(c + e + f_skew*uq[i]);
+        res[i*k+a] += v + gd;
+      }
+    }
+  }
+  return 0;
+}
+
+// PressureRHS (MatrixAssembly.hpp:148-204).  tau_ps from u (NavierStokesSemiImplicit.cpp PRHSAssembly),
+// the operators from u_adv; uvec, avec, davec k x dim; pvec k.  res: k.
+int si_pressure_rhs_element(int et, const TauCfg& cfg, Real dt, const Real* X, const Real* u, const Real* uadv,
+                            const Real* nu, const Real* uvec, const Real* avec, const Real* davec,
+                            const Real* pvec, Real* res)
+{
+  const int k = enodes(et), dim = edim(et);
+  Real tau_ps, tau_su, tau_bulk;
+  if(compute_tau(et, cfg, X, u, element_nu(k, nu), dt, tau_ps, tau_su, tau_bulk)) return -2;
+  Real upda[24], apda[24];
+  for(int r = 0; r < k*dim; ++r) { upda[r] = uvec[r] + dt*davec[r]; apda[r] = avec[r] + davec[r]; }
+  Real App[64];
+  std::fill(App, App + k*k, 0.);
+  std::fill(res, res + k, 0.);
+  Real gxi[24], gw[8], N[8], G[24], det, uq[3], adv[8];
+  const int ngp = gauss(et, si_order(et), gxi, gw);
+  for(int q = 0; q < ngp; ++q) {
+    if(!point_data(et, gxi + q*dim, X, N, G, det)) return -2;
+    interp(k, dim, N, uadv, uq);
+    const Real w = gw[q]*det, tau_w = w*tau_ps;
+    for(int m = 0; m < k; ++m) {
+      Real s = 0.;
+      for(int d = 0; d < dim; ++d) s += uq[d]*G[d*k+m];
+      adv[m] = tau_ps*s;
+    }
+    for(int a = 0; a < k; ++a)
+      for(int b = 0; b < k; ++b) {
+        Real gtg = 0.;
+        for(int d = 0; d < dim; ++d) gtg += G[d*k+a]*G[d*k+b];
+        App[a*k+b] += tau_w*gtg;
+      }
+    for(int i = 0; i < dim; ++i) {
+      Real ga = 0., av = 0., na = 0.;
+      for(int m = 0; m < k; ++m) {
+        ga += G[i*k+m]*upda[m*dim+i];
+        av += adv[m]*upda[m*dim+i];
+        na += N[m]*apda[m*dim+i];
+      }
+      const Real a_ = w*ga, b = w*av, c = tau_w*na;
+      for(int m = 0; m < k; ++m) res[m] -= a_*(N[m] + adv[m]*0.5) + (b + c)*G[i*k+m];
+    }
+  }
+  for(int a = 0; a < k; ++a) {
+    Real s = 0.;
+    for(int b = 0; b < k; ++b) s += App[a*k+b]*pvec[b];
+    res[a] -= s;
+  }
+  return 0;
+}
+
+// ApplyAup (MatrixAssembly.hpp:332-376): tau_su from u_adv.  dp: k.  res: dim*k var-major.
+int si_apply_aup_element(int et, const TauCfg& cfg, Real dt, Real theta, const Real* X, const Real* uadv,
+                         const Real* nu, const Real* dp, Real* res)
+{
+  const int k = enodes(et), dim = edim(et);
+  Real tau_ps, tau_su, tau_bulk;
+  if(compute_tau(et, cfg, X, uadv, element_nu(k, nu), dt, tau_ps, tau_su, tau_bulk)) return -2;
+  std::fill(res, res + dim*k, 0.);
+  Real gxi[24], gw[8], N[8], G[24], det, uq[3], adv[8];
+  const int ngp = gauss(et, si_order(et), gxi, gw);
+  for(int q = 0; q < ngp; ++q) {
+    if(!point_data(et, gxi + q*dim, X, N, G, det)) return -2;
+    interp(k, dim, N, uadv, uq);
+    const Real w = theta*gw[q]*det;
+    const Real ws = w*tau_su;
+    for(int m = 0; m < k; ++m) {
+      Real s = 0.;
+      for(int d = 0; d < dim; ++d) s += uq[d]*G[d*k+m];
+      adv[m] = ws*s;
+    }
+    Real nd = 0.;
+    for(int m = 0; m < k; ++m) nd += N[m]*dp[m];
+    const Real b = -w*nd;
+    for(int i = 0; i < dim; ++i) {
+      Real a_ = 0.;
+      for(int m = 0; m < k; ++m) a_ += G[i*k+m]*dp[m];
+      for(int m = 0; m < k; ++m) res[i*k+m] += a_*adv[m] + b*G[i*k+m];
+    }
+  }
+  return 0;
+}
+
+// PressureSystemAssembly::assemble_pp (PressureSystem.cpp:166-176): theta (tau_ps + dt) int G^T G,
+// tau_ps from u.  S: k x k.
+int si_pressure_matrix_element(int et, const TauCfg& cfg, Real dt, Real theta, const Real* X, const Real* u,
+                               const Real* nu, Real* S)
+{
+  const int k = enodes(et), dim = edim(et);
+  Real tau_ps, tau_su, tau_bulk;
+  if(compute_tau(et, cfg, X, u, element_nu(k, nu), dt, tau_ps, tau_su, tau_bulk)) return -2;
+  Real A[64];
+  std::fill(A, A + k*k, 0.);
+  Real gxi[24], gw[8], N[8], G[24], det;
+  const int ngp = gauss(et, 2, gxi, gw);
+  for(int q = 0; q < ngp; ++q) {
+    if(!point_data(et, gxi + q*dim, X, N, G, det)) return -2;
+    const Real w = gw[q]*det;
+    for(int a = 0; a < k; ++a)
+      for(int b = 0; b < k; ++b) {
+        Real gtg = 0.;
+        for(int d = 0; d < dim; ++d) gtg += G[d*k+a]*G[d*k+b];
+        A[a*k+b] += w*gtg;
+      }
+  }
+  const Real c = theta*(tau_ps + dt);
+  for(int r = 0; r < k*k; ++r) S[r] = c*A[r];
+  return 0;
+}
+
+// lumped mass matrix (NavierStokesSemiImplicit.cpp MassMatrixAssembly): row sums of int N^T N
+int si_lumped_mass_element(int et, const Real* X, Real* m)
+{
+  const int k = enodes(et), dim = edim(et);
+  Real A[64];
+  std::fill(A, A + k*k, 0.);
+  Real gxi[24], gw[8], N[8], G[24], det;
+  const int ngp = gauss(et, 2, gxi, gw);
+  for(int q = 0; q < ngp; ++q) {
+    if(!point_data(et, gxi + q*dim, X, N, G, det)) return -2;
+    const Real w = gw[q]*det;
+    for(int a = 0; a < k; ++a)
+      for(int b = 0; b < k; ++b) A[a*k+b] += (w*N[a])*N[b];
+  }
+  for(int a = 0; a < k; ++a) {   // lump(_A): row sum onto the diagonal (EigenTransforms.hpp:536-553)
+    Real s = 0.;
+    for(int b = 0; b < k; ++b) s += A[a*k+b];
+    m[a] = s;
+  }
+  return 0;
+}
+
+// gather k x dim node-major values of mesh-node field f (n x dim) or of LSS vector v (rows x dim) via p2m
+void gather(int k, int dim, const uint32_t* c, const uint32_t* p2m, const Real* f, Real* out)
+{
+  for(int a = 0; a < k; ++a) {
+    const size_t r = p2m ? p2m[c[a]] : c[a];
+    for(int d = 0; d < dim; ++d) out[a*dim+d] = f[r*dim+d];
+  }
+}
+
+}  // namespace
+
+extern "C" {
+
+int orc_si_lumped_mass(int et, uint32_t n_elems, const double* coords, const uint32_t* conn,
+                       const uint32_t* p2m, double* diag)
+{
+  const int k = enodes(et), dim = edim(et);
+  Real X[24], m[8];
+  for(uint32_t e = 0; e < n_elems; ++e) {
+    const uint32_t* c = conn + (size_t)e*k;
+    gather(k, dim, c, nullptr, coords, X);
+    if(si_lumped_mass_element(et, X, m)) return -2;
+    for(int a = 0; a < k; ++a)
+      for(int i = 0; i < dim; ++i) diag[(size_t)p2m[c[a]]*dim + i] += m[a];
+  }
+  return 0;
+}
+
+int orc_si_velocity_matrix(int et, const orc_ns_params* prm, uint32_t n_elems, const double* coords,
+                           const uint32_t* conn, const double* uadv, const double* nu, const uint32_t* slots,
+                           double* values)
+{
+  const int k = enodes(et), dim = edim(et), n = dim*k;
+  const TauCfg cfg = tau_cfg(prm);
+  Real X[24], ua[24], nn[8], S[576];
+  for(uint32_t e = 0; e < n_elems; ++e) {
+    const uint32_t* c = conn + (size_t)e*k;
+    gather(k, dim, c, nullptr, coords, X);
+    gather(k, dim, c, nullptr, uadv, ua);
+    gather(k, 1, c, nullptr, nu, nn);
+    if(si_velocity_element(et, cfg, prm->dt, prm->theta, X, ua, nn, S)) return -2;
+    for(int a = 0; a < k; ++a)
+      for(int b = 0; b < k; ++b) {
+        double* blk = values + (size_t)slots[((size_t)e*k + a)*k + b]*dim*dim;
+        for(int i = 0; i < dim; ++i)
+          for(int j = 0; j < dim; ++j) blk[i*dim+j] += S[(i*k+a)*n + j*k+b];
+      }
+  }
+  return 0;
+}
+
+int orc_si_pressure_matrix(int et, const orc_ns_params* prm, uint32_t n_elems, const double* coords,
+                           const uint32_t* conn, const double* u, const double* nu, const uint32_t* slots,
+                           double* values)
+{
+  const int k = enodes(et), dim = edim(et);
+  const TauCfg cfg = tau_cfg(prm);
+  Real X[24], uu[24], nn[8], S[64];
+  for(uint32_t e = 0; e < n_elems; ++e) {
+    const uint32_t* c = conn + (size_t)e*k;
+    gather(k, dim, c, nullptr, coords, X);
+    gather(k, dim, c, nullptr, u, uu);
+    gather(k, 1, c, nullptr, nu, nn);
+    if(si_pressure_matrix_element(et, cfg, prm->dt, prm->theta, X, uu, nn, S)) return -2;
+    for(int a = 0; a < k; ++a)
+      for(int b = 0; b < k; ++b) values[slots[((size_t)e*k + a)*k + b]] += S[a*k+b];
+  }
+  return 0;
+}
+
+int orc_si_velocity_rhs(int et, const orc_ns_params* prm, uint32_t n_elems, const double* coords,
+                        const uint32_t* conn, const uint32_t* p2m, const double* uadv, const double* nu,
+                        const double* a_lss, const double* u_lss, const double* dps_lss, const double* p_lss,
+                        double* rhs)
+{
+  const int k = enodes(et), dim = edim(et);
+  const TauCfg cfg = tau_cfg(prm);
+  const Real dt = prm->dt, th = prm->theta;
+  Real X[24], ua[24], nn[8], av[24], uv[24], dps[8], pv[8], dtamu[24], ppdp[8], res[24];
+  for(uint32_t e = 0; e < n_elems; ++e) {
+    const uint32_t* c = conn + (size_t)e*k;
+    gather(k, dim, c, nullptr, coords, X);
+    gather(k, dim, c, nullptr, uadv, ua);
+    gather(k, 1, c, nullptr, nu, nn);
+    gather(k, dim, c, p2m, a_lss, av);
+    gather(k, dim, c, p2m, u_lss, uv);
+    gather(k, 1, c, p2m, dps_lss, dps);
+    gather(k, 1, c, p2m, p_lss, pv);
+    // lit(dt)*(1 - lit(theta))*lit(a) - lit(u_vec), (1 - lit(theta))*lit(delta_p_sum) - lit(p_vec)
+    for(int r = 0; r < k*dim; ++r) dtamu[r] = dt*(1. - th)*av[r] - uv[r];
+    for(int r = 0; r < k; ++r) ppdp[r] = (1. - th)*dps[r] - pv[r];
+    if(si_velocity_rhs_element(et, cfg, dt, X, ua, nn, av, dtamu, ppdp, res)) return -2;
+    for(int a = 0; a < k; ++a)
+      for(int i = 0; i < dim; ++i) rhs[(size_t)p2m[c[a]]*dim + i] += res[i*k+a];
+  }
+  return 0;
+}
+
+int orc_si_pressure_rhs(int et, const orc_ns_params* prm, uint32_t n_elems, const double* coords,
+                        const uint32_t* conn, const uint32_t* p2m, const double* u, const double* uadv,
+                        const double* nu, const double* u_lss, const double* a_lss, const double* da_lss,
+                        const double* p_lss, double* rhs)
+{
+  const int k = enodes(et), dim = edim(et);
+  const TauCfg cfg = tau_cfg(prm);
+  Real X[24], uu[24], ua[24], nn[8], uv[24], av[24], dav[24], pv[8], res[8];
+  for(uint32_t e = 0; e < n_elems; ++e) {
+    const uint32_t* c = conn + (size_t)e*k;
+    gather(k, dim, c, nullptr, coords, X);
+    gather(k, dim, c, nullptr, u, uu);
+    gather(k, dim, c, nullptr, uadv, ua);
+    gather(k, 1, c, nullptr, nu, nn);
+    gather(k, dim, c, p2m, u_lss, uv);
+    gather(k, dim, c, p2m, a_lss, av);
+    gather(k, dim, c, p2m, da_lss, dav);
+    gather(k, 1, c, p2m, p_lss, pv);
+    if(si_pressure_rhs_element(et, cfg, prm->dt, X, uu, ua, nn, uv, av, dav, pv, res)) return -2;
+    for(int a = 0; a < k; ++a) rhs[p2m[c[a]]] += res[a];
+  }
+  return 0;
+}
+
+int orc_si_apply_aup(int et, const orc_ns_params* prm, uint32_t n_elems, const double* coords,
+                     const uint32_t* conn, const uint32_t* p2m, const double* uadv, const double* nu,
+                     const double* dp_lss, double* rhs)
+{
+  const int k = enodes(et), dim = edim(et);
+  const TauCfg cfg = tau_cfg(prm);
+  Real X[24], ua[24], nn[8], dp[8], res[24];
+  for(uint32_t e = 0; e < n_elems; ++e) {
+    const uint32_t* c = conn + (size_t)e*k;
+    gather(k, dim, c, nullptr, coords, X);
+    gather(k, dim, c, nullptr, uadv, ua);
+    gather(k, 1, c, nullptr, nu, nn);
+    gather(k, 1, c, p2m, dp_lss, dp);
+    if(si_apply_aup_element(et, cfg, prm->dt, prm->theta, X, ua, nn, dp, res)) return -2;
+    for(int a = 0; a < k; ++a)
+      for(int i = 0; i < dim; ++i) rhs[(size_t)p2m[c[a]]*dim + i] += res[i*k+a];
+  }
+  return 0;
+}
+
+} // extern "C"

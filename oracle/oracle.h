@@ -136,6 +136,39 @@ void orc_linearize_u(uint32_t n, int dim, const double* u, const double* u1, con
 /* UpdateActions (NavierStokes.cpp:172-180): u3=u2; u2=u1; u1=u; u+=du; p+=dp, solution node-major (u..,p) */
 void orc_update(uint32_t n, int dim, double* u, double* p, double* u1, double* u2, double* u3, const double* sol);
 
+/* ---------------- NavierStokesSemiImplicit (plugins/UFEM/src/UFEM/ns_semi_implicit) ----------------
+ * Element actions over all elements in element order, accumulated (+=) into the caller's arrays.
+ * Mesh-node fields: coords, u, uadv (N x dim), nu (N).  LSS vectors by row (p2m[node]): velocity-system
+ * vectors rows x dim, pressure-system vectors rows.  Velocity matrix: neq = dim blocks (row-major) at
+ * the block slots of the p2m sparsity; pressure matrix: scalar values at those slots.  Uses the
+ * METRIC/TEZDUYAR tau of prm (dt, theta, supg_type, c1, c2, alphas; rho is not used). */
+/* MassMatrixAssembly: lumped (row-sum) int N^T N into every velocity component (diag: rows x dim) */
+int orc_si_lumped_mass(int etype, uint32_t n_elems, const double* coords, const uint32_t* conn,
+                       const uint32_t* p2m, double* diag);
+/* velocity matrix T + theta dt M (MatrixAssembly.hpp:49-146, :458) */
+int orc_si_velocity_matrix(int etype, const orc_ns_params* prm, uint32_t n_elems, const double* coords,
+                           const uint32_t* conn, const double* uadv, const double* nu, const uint32_t* slots,
+                           double* values);
+/* simplified pressure matrix theta (tau_ps + dt) int G^T G, tau_ps from u (PressureSystem.cpp:166-176) */
+int orc_si_pressure_matrix(int etype, const orc_ns_params* prm, uint32_t n_elems, const double* coords,
+                           const uint32_t* conn, const double* u, const double* nu, const uint32_t* slots,
+                           double* values);
+/* URHSAssembly: velocity_rhs(u_adv, nu, a, dt (1 - theta) a - u_vec, (1 - theta) delta_p_sum - p_vec)
+ * (MatrixAssembly.hpp:206-330, :466-470) */
+int orc_si_velocity_rhs(int etype, const orc_ns_params* prm, uint32_t n_elems, const double* coords,
+                        const uint32_t* conn, const uint32_t* p2m, const double* uadv, const double* nu,
+                        const double* a_lss, const double* u_lss, const double* dps_lss, const double* p_lss,
+                        double* rhs);
+/* PRHSAssembly: pressure_rhs(u_adv, u_vec, a, delta_a, p_vec) with tau_ps from u (:148-204, :487-494) */
+int orc_si_pressure_rhs(int etype, const orc_ns_params* prm, uint32_t n_elems, const double* coords,
+                        const uint32_t* conn, const uint32_t* p2m, const double* u, const double* uadv,
+                        const double* nu, const double* u_lss, const double* a_lss, const double* da_lss,
+                        const double* p_lss, double* rhs);
+/* ApplyAup: apply_aup(u_adv, delta_p) (:332-376, :497-503) */
+int orc_si_apply_aup(int etype, const orc_ns_params* prm, uint32_t n_elems, const double* coords,
+                     const uint32_t* conn, const uint32_t* p2m, const double* uadv, const double* nu,
+                     const double* dp_lss, double* rhs);
+
 #ifdef __cplusplus
 }
 #endif

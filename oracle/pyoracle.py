@@ -98,6 +98,12 @@ class Oracle:
         L.orc_element_volume.restype = C.c_double
         L.orc_element_volume.argtypes = [C.c_int, _d]
         L.orc_element_jacobian.argtypes = [C.c_int, _d, _d, _d]
+        L.orc_si_lumped_mass.argtypes = [C.c_int, C.c_uint32, _d, _u32, _u32, _d]
+        for name in ("orc_si_velocity_matrix", "orc_si_pressure_matrix"):
+            getattr(L, name).argtypes = [C.c_int, C.POINTER(NSParams), C.c_uint32, _d, _u32, _d, _d, _u32, _d]
+        L.orc_si_velocity_rhs.argtypes = [C.c_int, C.POINTER(NSParams), C.c_uint32, _d, _u32, _u32] + [_d] * 7
+        L.orc_si_pressure_rhs.argtypes = [C.c_int, C.POINTER(NSParams), C.c_uint32, _d, _u32, _u32] + [_d] * 8
+        L.orc_si_apply_aup.argtypes = [C.c_int, C.POINTER(NSParams), C.c_uint32, _d, _u32, _u32] + [_d] * 4
 
     # ---------------- PRNG / meshes ----------------
     def uniform_stream(self, seed, n, a=0.0, b=1.0):

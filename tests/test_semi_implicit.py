@@ -1,0 +1,206 @@
+"""UFEM NavierStokesSemiImplicit (plugins/UFEM/src/UFEM/ns_semi_implicit): the oracle restatement pinned by
+closed forms and by the reference's Taylor-Green acceptance bounds (the script's second half,
+atest-ufem-navier-stokes-taylor-green.py:197-225, 340-342), and the device path against the oracle:
+pressure matrix, lumped mass, velocity matrix and the three inner-loop element actions within 1e-12, and
+five Taylor-Green steps within 1e-9 (the device solves iteratively to 1e-13, the oracle directly, like
+the script's Amesos solves)."""
+import numpy as np
+import pytest
+
+from coolfluid3_amd import cabi, cases
+from oracle import pyoracle
+from oracle.semi_implicit import SemiImplicit
+
+U_BOUND, P_BOUND = 1.4e-3, 3.3e-3
+
+
+def _setup(element, segments=32):
+    case = cases.taylor_green_case(segments, element=element)
+    p2m = cases.periodic_p2m(*case.periodic)
+    center = case.bc_regions[0][1]
+    return case, p2m, center
+
+
+def _oracle(case, p2m, center):
+    return SemiImplicit(pyoracle.get(), case.etype, case.coords, case.conn, p2m, case.nu, case.dt, case.theta, 2,
+                        case.u0, case.p0, pbc_nodes=center)
+
+
+def _square(etype):
+    """The unit square as one quadrilateral or as two triangles."""
+    c = np.array([[0., 0.], [1., 0.], [1., 1.], [0., 1.]])
+    conn = np.array([[0, 1, 2, 3]]) if etype == pyoracle.QUAD2D else np.array([[0, 1, 2], [0, 2, 3]])
+    return c, conn.astype(np.uint32)
+
+
+@pytest.mark.parametrize("etype", [pyoracle.QUAD2D, pyoracle.TRIAG2D])
+def test_oracle_closed_forms_on_the_unit_square(etype):
+    """Lumped mass and simplified pressure matrix against the textbook integrals of bilinear and linear
+    elements on the unit square; the velocity matrix with zero viscosity and velocity is the consistent
+    mass matrix plus the bulk term, symmetric."""
+    o = pyoracle.get()
+    c, conn = _square(etype)
+    u = np.zeros((4, 2))
+    si = SemiImplicit(o, etype, c, conn, np.arange(4, dtype=np.uint32), 0.0, 0.01, 1.0, 2, u, np.zeros(4))
+    if etype == pyoracle.QUAD2D:
+        assert np.allclose(si.lumped, 0.25, rtol=0, atol=1e-15)
+    else:   # node masses: one third of the area of the triangles around the node
+        assert np.allclose(si.lumped.reshape(4, 2)[:, 0], [1 / 3, 1 / 6, 1 / 3, 1 / 6], rtol=0, atol=1e-15)
+    # pressure matrix = theta (tau_ps + dt) K with K the P1/Q1 Laplacian of the square
+    K = np.zeros((4, 4))
+    vals = si.pvals
+    for r in range(4):
+        for q in range(si.start[r], si.start[r + 1]):
+            K[r, si.cols[q]] = vals[q]
+    K /= K[0, 0]
+    if etype == pyoracle.QUAD2D:
+        ref = np.array([[4, -1, -2, -1], [-1, 4, -1, -2], [-2, -1, 4, -1], [-1, -2, -1, 4]]) / 4.0
+    else:
+        ref = np.array([[1.0, -0.5, 0.0, -0.5], [-0.5, 1.0, -0.5, 0.0], [0.0, -0.5, 1.0, -0.5], [-0.5, 0.0, -0.5, 1.0]])
+    assert np.allclose(K, ref, atol=1e-14)
+    assert np.allclose(K.sum(axis=1), 0, atol=1e-14)
+    # velocity matrix: T + theta dt M; at nu = 0, u = 0 the bulk coefficient is tau_bulk = (2/dt)/tr(g)
+    V = np.zeros((8, 8))
+    vv = si.velocity_matrix().reshape(-1, 2, 2)
+    for r in range(4):
+        for q in range(si.start[r], si.start[r + 1]):
+            V[2 * r:2 * r + 2, 2 * si.cols[q]:2 * si.cols[q] + 2] = vv[q]
+    assert np.allclose(V, V.T, atol=1e-15)
+    # the mass part integrates 1 over the square once per component: sum of the u-u block without bulk
+    uu = V[0::2, 0::2]
+    assert abs(uu.sum() - 1.0) < 1e-14                          # bulk G^T G rows sum to zero
+    # Aup of a constant pressure increment is the boundary term; on the closed square it sums to zero
+    aup = si.apply_aup(np.ones(4))
+    assert np.allclose(aup.reshape(4, 2).sum(axis=0), 0, atol=1e-15)
+
+
+def test_oracle_periodic_invariants():
+    """On the periodic Taylor-Green mesh: the lumped masses sum to the area for each component, the
+    pressure matrix annihilates constants, Aup of a constant increment vanishes row by row (no boundary)."""
+    case, p2m, center = _setup("quad")
+    si = _oracle(case, p2m, center)
+    assert abs(si.lumped.reshape(-1, 2)[:, 0].sum() - 1.0) < 1e-13
+    Kx = pyoracle.get().spmv(1, si.start, si.cols, si.pvals, np.ones(si.rows))
+    assert np.abs(Kx).max() < 1e-13 * np.abs(si.pvals).max()
+    si.uadv = si.u.copy()
+    assert np.abs(si.apply_aup(np.ones(si.rows))).max() < 1e-14
+
+
+@pytest.mark.parametrize("element", ["quad", "triag"])
+def test_oracle_taylor_green_semi_implicit(element):
+    """Five semi-implicit steps of the script (dt 0.004, theta 0.5, two inner passes, 32 segments) against
+    the exact vortex.  Quadrilaterals are the reference's acceptance configuration
+    (plugins/UFEM/test/CMakeLists.txt:94-97) and meet its bounds; the triangle run is not part of the
+    reference's tests and ends 0.06% above the pressure bound (3.302e-3), so it is held to 3.4e-3."""
+    case, p2m, center = _setup(element)
+    si = _oracle(case, p2m, center)
+    t = 0.0
+    for step in range(5):
+        si.step(cases.taylor_green_exact(case.coords[center], t + case.dt)[1])
+        t += case.dt
+    ue, pe = cases.taylor_green_exact(case.coords, t)
+    eu = np.abs(si.u - ue).max(axis=0)
+    ep = np.abs(si.p - pe).max()
+    print(element, eu, ep)
+    assert eu.max() < U_BOUND
+    assert ep < (P_BOUND if element == "quad" else 3.4e-3)
+
+
+# ------------------------------------------------------------------------------------------------
+# device
+# ------------------------------------------------------------------------------------------------
+class DeviceSI:
+    def __init__(self, case):
+        self.ctx = cabi.Context(0)
+        self.mesh = cabi.Mesh(self.ctx, case.etype, case.coords, case.conn)
+        self.us = cabi.System(self.ctx, self.mesh, neq=2, periodic=case.periodic)
+        self.ps = cabi.System(self.ctx, self.mesh, neq=1, periodic=case.periodic)
+        self.f = cabi.Fields(self.mesh)
+        self.f.set(cabi.FIELD_U, case.u0)
+        self.f.set(cabi.FIELD_P, case.p0)
+        self.f.initialize(case.nu)
+        self.prm = cabi.ns_params(dt=case.dt, theta=case.theta)
+        self.si = cabi.SemiImplicit(self.us, self.ps)
+
+    def close(self):
+        self.si.close()
+        for x in (self.f, self.ps, self.us, self.mesh, self.ctx):
+            x.close()
+
+
+def _close(a, b, tol):
+    return np.abs(a - b).max() <= tol * max(np.abs(b).max(), 1e-300)
+
+
+@pytest.mark.gpu
+@pytest.mark.parametrize("element", ["quad", "triag"])
+def test_device_matrices_and_actions_vs_oracle(element):
+    """Pressure matrix and lumped mass (initial conditions), the velocity matrix and the URHS, PRHS and
+    ApplyAup actions on seeded inner-loop vectors, against the oracle within 1e-12."""
+    case, p2m, center = _setup(element)
+    si = _oracle(case, p2m, center)
+    d = DeviceSI(case)
+    try:
+        assert np.array_equal(d.us.p2m(), p2m)
+        start, cols = d.ps.graph()
+        assert np.array_equal(start, si.start) and np.array_equal(cols, si.cols)
+        assert np.array_equal(d.us.slots(), si.slots)
+        d.si.initial(d.f, d.prm)
+        assert _close(d.ps.values(), si.pvals, 1e-12)
+        assert _close(d.si.get(cabi.SI_VEC_MINV), si.minv, 1e-12)
+        # velocity matrix from u_adv = u (initial linearisation)
+        d.si.velocity_assemble(d.f, d.prm)
+        si.uadv = si.u.copy()
+        assert _close(d.us.values(), si.velocity_matrix(), 1e-12)
+        # actions on seeded vectors
+        rng = np.random.default_rng(7)
+        rows = si.rows
+        a, U, P, dps = rng.normal(size=2 * rows), rng.normal(size=2 * rows), rng.normal(size=rows), rng.normal(size=rows)
+        da, dp = rng.normal(size=2 * rows), rng.normal(size=rows)
+        for w, v in ((cabi.SI_VEC_A, a), (cabi.SI_VEC_U, U), (cabi.SI_VEC_P, P), (cabi.SI_VEC_DPS, dps)):
+            d.si.set(w, v)
+        d.us.set_vector(cabi.SOLUTION, da)
+        d.ps.set_vector(cabi.SOLUTION, dp)
+        d.us.reset_vector(cabi.RHS)
+        d.si.action(d.f, d.prm, cabi.SI_URHS)
+        assert _close(d.us.vector(cabi.RHS), si.velocity_rhs(a, U, dps, P), 1e-12)
+        d.ps.reset_vector(cabi.RHS)
+        d.si.action(d.f, d.prm, cabi.SI_PRHS)
+        assert _close(d.ps.vector(cabi.RHS), si.pressure_rhs(U, a, da, P), 1e-12)
+        d.us.reset_vector(cabi.RHS)
+        d.si.action(d.f, d.prm, cabi.SI_AUP)
+        assert _close(d.us.vector(cabi.RHS), si.apply_aup(dp), 1e-12)
+    finally:
+        d.close()
+
+
+@pytest.mark.gpu
+@pytest.mark.parametrize("element", ["quad", "triag"])
+def test_device_taylor_green_semi_implicit_follows_oracle(element):
+    """Five device steps (cf3hip_si_step: BiCGStab + Jacobi to 1e-13 for both systems) follow the oracle's
+    direct-solve steps within 1e-9 of the field maxima, and the quadrilateral run meets the reference's
+    bounds."""
+    case, p2m, center = _setup(element)
+    si = _oracle(case, p2m, center)
+    d = DeviceSI(case)
+    try:
+        d.si.initial(d.f, d.prm)
+        sp = cabi.solver_params(cabi.BICGSTAB_JACOBI, tol=1e-13, max_iter=5000)
+        t = 0.0
+        for step in range(5):
+            pv = cases.taylor_green_exact(case.coords[center], t + case.dt)[1]
+            stats = d.si.step(d.f, d.prm, sp, sp, pbc=(center, pv))
+            si.step(pv)
+            t += case.dt
+            u, p = d.f.get(cabi.FIELD_U), d.f.get(cabi.FIELD_P)
+            du = np.abs(u - si.u).max() / np.abs(si.u).max()
+            dpp = np.abs(p - si.p).max() / np.abs(si.p).max()
+            print(element, step + 1, f"du {du:.2e} dp {dpp:.2e}", [s.iterations for s in stats])
+            assert du < 1e-9 and dpp < 1e-9
+        ue, pe = cases.taylor_green_exact(case.coords, t)
+        eu, ep = np.abs(u - ue).max(), np.abs(p - pe).max()
+        print(element, "errors", eu, ep)
+        if element == "quad":
+            assert eu < U_BOUND and ep < P_BOUND
+    finally:
+        d.close()
