@@ -208,6 +208,84 @@ def taylor_green_workload(ctx, element):
             "steps": steps, "within_reference_bounds": bool(max(last["u"], last["v"]) < 1.4e-3 and last["p"] < 3.3e-3)}
 
 
+def semi_implicit_workload(ctx, scale_segments=512, scale_steps=2):
+    """The semi-implicit half of the Taylor-Green script (NavierStokesSemiImplicit, two inner passes,
+    BiCGStab + Jacobi to 1e-13 for the velocity and pressure systems): errors after each step at 32
+    segments, then the same step on scale_segments^2 quadrilaterals with the time of the velocity-matrix
+    assembly (element kernel + deterministic gather) and of one step."""
+    from coolfluid3_amd import cabi, cases
+    sp = cabi.solver_params(cabi.BICGSTAB_JACOBI, tol=1e-13, max_iter=20000)
+
+    def setup(segments, element):
+        case = cases.taylor_green_case(segments, element=element)
+        mesh = cabi.Mesh(ctx, case.etype, case.coords, case.conn)
+        us = cabi.System(ctx, mesh, neq=2, periodic=case.periodic)
+        ps = cabi.System(ctx, mesh, neq=1, periodic=case.periodic)
+        f = cabi.Fields(mesh)
+        f.set(cabi.FIELD_U, case.u0)
+        f.set(cabi.FIELD_P, case.p0)
+        f.initialize(case.nu)
+        prm = cabi.ns_params(dt=case.dt, theta=case.theta)
+        si = cabi.SemiImplicit(us, ps)
+        si.initial(f, prm)
+        return case, (mesh, us, ps, f, si), prm
+
+    def close(objs):
+        mesh, us, ps, f, si = objs
+        si.close()
+        for x in (f, ps, us, mesh):
+            x.close()
+
+    def step(case, objs, prm):
+        f, si = objs[3], objs[4]
+        center = case.bc_regions[0][1]
+        pv = cases.taylor_green_exact(case.coords[center], case.t + case.dt)[1]
+        ctx.synchronize()
+        t0 = time.perf_counter()
+        stats = si.step(f, prm, sp, sp, pbc=(center, pv))
+        ms = (time.perf_counter() - t0) * 1e3
+        case.t += case.dt
+        return ms, [int(s.iterations) for s in stats]
+
+    out = {}
+    for element in ("quad", "triag"):
+        case, objs, prm = setup(32, element)
+        steps = []
+        for _ in range(5):
+            ms, its = step(case, objs, prm)
+            ue, pe = cases.taylor_green_exact(case.coords, case.t)
+            u, p = objs[3].get(cabi.FIELD_U), objs[3].get(cabi.FIELD_P)
+            steps.append({"ms": ms, "iterations_u_p_per_pass": its,
+                          "max_error": {"u": float(np.abs(u[:, 0] - ue[:, 0]).max()),
+                                        "v": float(np.abs(u[:, 1] - ue[:, 1]).max()),
+                                        "p": float(np.abs(p - pe).max())}})
+        close(objs)
+        last = steps[-1]["max_error"]
+        out[element] = {"steps": steps, "within_reference_bounds":
+                        bool(max(last["u"], last["v"]) < 1.4e-3 and last["p"] < 3.3e-3)}
+    case, objs, prm = setup(scale_segments, "quad")
+    mesh, us, ps, f, si = objs
+    steps = [step(case, objs, prm) for _ in range(scale_steps)]
+    reps = 10
+    si.velocity_assemble(f, prm)
+    ctx.synchronize()
+    t0 = time.perf_counter()
+    for _ in range(reps):
+        si.velocity_assemble(f, prm)
+    asm_ms = (time.perf_counter() - t0) * 1e3 / reps
+    E, N = case.conn.shape[0], case.n_nodes
+    # compulsory bytes: coordinates, advection velocity, viscosity, connectivity, 2x2 blocks written
+    alg = N * (2 * 8 + 2 * 8 + 8) + E * 4 * 4 + us.n_blocks * 4 * 8
+    close(objs)
+    out["scale"] = {"workload": "Taylor-Green semi-implicit step, %d^2 periodic Q1 quadrilaterals (%d elements, "
+                                "%d velocity rows)" % (scale_segments, E, us.n_nodes),
+                    "step_ms": [s[0] for s in steps], "iterations_u_p_per_pass": [s[1] for s in steps],
+                    "velocity_assembly_ms": asm_ms, "velocity_assembly_melem_s": E / asm_ms / 1e3,
+                    "velocity_assembly_algorithmic_gb_s": alg / asm_ms / 1e6,
+                    "velocity_assembly_hbm_frac": alg / asm_ms / 1e6 / 8000.0}
+    return out
+
+
 def serving_workload(count, max_active, mode="both"):
     """Continuous batching against one-request-at-a-time serving on the same request stream
     (mode: both, sequential or batched)."""
@@ -235,7 +313,7 @@ def main():
     ap.add_argument("--no-cpu-baseline", action="store_true")
     ap.add_argument("--no-solve", action="store_true")
     ap.add_argument("--no-workloads", action="store_true", help="skip the 128^3 / 160^3 / 2D / serving workloads")
-    ap.add_argument("--workloads", default="cavity_2d,taylor_green,precision_tets,hexa_cavity,serving",
+    ap.add_argument("--workloads", default="cavity_2d,taylor_green,semi_implicit,precision_tets,hexa_cavity,serving",
                     help="comma-separated subset of the extra workloads")
     ap.add_argument("--precision-cells", type=int, default=128)
     ap.add_argument("--hexa-cells", type=int, default=160)
@@ -341,6 +419,7 @@ def main():
     if not args.no_workloads:
         for key, fn in (("cavity_2d", lambda: cavity2d_workload(ctx)),
                         ("taylor_green", lambda: {e: taylor_green_workload(ctx, e) for e in ("quad", "triag")}),
+                        ("semi_implicit", lambda: semi_implicit_workload(ctx)),
                         ("precision_tets", lambda: precision_workload(ctx, args.precision_cells)),
                         ("hexa_cavity", lambda: hexa_workload(ctx, args.hexa_cells, args.picard)),
                         ("serving", lambda: serving_workload(args.serving_requests, args.serving_slots, args.serving_mode))):

@@ -32,6 +32,8 @@ def test_errors_without_device():
     assert "null" in lib.cf3hip_last_error().decode() or lib.cf3hip_last_error()
     assert lib.cf3hip_context_create(0, None) == 1
     assert lib.cf3hip_ns_assemble(None, None, None) == 1
+    assert lib.cf3hip_si_create(None, None, None) == 1
+    assert lib.cf3hip_si_step(None, None, None, 2, None, None, 0, None, None, 0, None, None, None) == 1
 
 
 def test_meshgen_matches_oracle(oracle):
