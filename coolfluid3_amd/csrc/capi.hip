@@ -1198,6 +1198,7 @@ int cf3hip_si_action(cf3hip_si si, cf3hip_fields fields, const cf3hip_ns_params*
   API_END
 }
 
+extern "C++" {
 namespace {
 
 DevBuf& si_vector(SemiImplicit& S, int which, size_t& n)
@@ -1234,6 +1235,7 @@ void si_conditions(SemiImplicit& S, const std::vector<const double*>& field, uin
 }
 
 }  // namespace
+}  // extern "C++"
 
 int cf3hip_si_get(cf3hip_si si, int which, double* host)
 {
@@ -1278,11 +1280,20 @@ int cf3hip_si_step(cf3hip_si si, cf3hip_fields fields, const cf3hip_ns_params* p
   // conditions against the fields before the step (the fields change only in Update)
   std::vector<uint32_t> urows, ueqs, prows, peqs;
   std::vector<double> uinc, pinc;
-  std::vector<const double*> uf = {F.u[0].as<double>(), F.u[1].as<double>()};
+  std::vector<const double*> uf;
+  for(int d = 0; d < S.mesh->dim; ++d) uf.push_back(F.u[d].as<double>());
   si_conditions(S, uf, n_ubc, ubc_nodes, ubc_values, urows, ueqs, uinc);
   si_conditions(S, {F.p.as<double>()}, n_pbc, pbc_nodes, pbc_values, prows, peqs, pinc);
   std::vector<uint32_t> udofs(urows.size()), pdofs(prows);
   for(size_t j = 0; j < urows.size(); ++j) udofs[j] = urows[j] * U.neq + ueqs[j];
+  // one condition per dof: a second symmetric_dirichlet on a dof of a TrilinosCrsMatrix subtracts the
+  // cached column once more (TrilinosCrsMatrix.cpp:389-396), which this path does not reproduce
+  for(const std::vector<uint32_t>* d : {&udofs, &pdofs}) {
+    std::vector<uint32_t> sorted(*d);
+    std::sort(sorted.begin(), sorted.end());
+    REQUIRE(std::adjacent_find(sorted.begin(), sorted.end()) == sorted.end(), CF3HIP_ERR_INVALID_ARG,
+            "a degree of freedom appears in more than one Dirichlet condition");
+  }
   ns_linearize(F);                                     // LinearizeU
   si_velocity_assemble(S, F, *prm);                    // VelocityLSS: ZeroLSS + assembly
   si_set_solution(S, F);                               // SetSolution

@@ -2,7 +2,7 @@
 // plugins/UFEM/src/UFEM/ns_semi_implicit/MatrixAssembly.hpp and PressureSystem.cpp, and the time step of
 // ns_semi_implicit/implementation/NavierStokesSemiImplicit.cpp (LinearizeU, velocity assembly,
 // SetSolution, InnerLoop, Update) on a velocity system (neq = dim) and a pressure system (neq = 1) that
-// share one mesh and its periodic links.  Triangles and quadrilaterals.
+// share one mesh and its periodic links.  Triangles, quadrilaterals, tetrahedra and hexahedra.
 //
 // Each element action runs in two passes.  k_si_elem computes one element per thread and writes its
 // element vector or matrix to a staging buffer; k_si_gather then gives every matrix block slot (or
@@ -21,17 +21,18 @@ namespace {
 #pragma clang fp contract(off)
 
 // ------------------------------------------------------------------------------------------------
-// element types: shape functions, Gauss rules (cf3/mesh/LagrangeP1/Triag2D.cpp, Quad2D.cpp;
-// cf3/mesh/Integrators/GaussImplementation.hpp: triangle order 1 centroid w 1/2, order 2 edge
-// midpoints w 1/6; quadrilateral order 1 centre w 4, order 2 the points (x,x),(-x,x),(x,-x),(-x,-x))
+// element types: shape functions, Gauss rules (cf3/mesh/LagrangeP1/Triag2D.cpp, Quad2D.cpp, Tetra3D.cpp,
+// Hexa3D.cpp; cf3/mesh/Integrators/GaussImplementation.hpp), IntegralOrder (MatrixAssembly.hpp:28-47:
+// 1 for triangles and tetrahedra, 2 otherwise)
 // ------------------------------------------------------------------------------------------------
 struct Tri {
-  static constexpr int K = 3, ORDER = 1;   // IntegralOrder (MatrixAssembly.hpp:28-47)
+  static constexpr int K = 3, D = 2, ORDER = 1;
   __device__ static int npts(int order) { return order == 1 ? 1 : 3; }
   __device__ static void point(int order, int q, double xi[2], double& w)
   {
+    // order 1: centroid, w 1/2; order 2: the edge midpoints (1/2, 0), (1/2, 1/2), (0, 1/2), w 1/6
     if(order == 1) { xi[0] = xi[1] = 0.3333333333333333333333333; w = 0.5; return; }
-    xi[0] = q == 1 ? 0.5 : (q == 0 ? 0.5 : 0.0);
+    xi[0] = q == 2 ? 0.0 : 0.5;
     xi[1] = q == 0 ? 0.0 : 0.5;
     w = 1.0 / 6.0;
   }
@@ -44,10 +45,11 @@ struct Tri {
 };
 
 struct Quad {
-  static constexpr int K = 4, ORDER = 2;
+  static constexpr int K = 4, D = 2, ORDER = 2;
   __device__ static int npts(int order) { return order == 1 ? 1 : 4; }
   __device__ static void point(int order, int q, double xi[2], double& w)
   {
+    // order 1: centre, w 4; order 2: (x,x), (-x,x), (x,-x), (-x,-x), w 1
     if(order == 1) { xi[0] = xi[1] = 0.; w = 4.; return; }
     const double x = 0.5773502691896257645091488;
     xi[0] = (q & 1) ? -x : x;
@@ -64,41 +66,104 @@ struct Quad {
   }
 };
 
-// N, physical gradients G = J^-1 dN (J(l, d) = dx_d / dxi_l) and det J at xi
+struct Tet {
+  static constexpr int K = 4, D = 3, ORDER = 1;
+  __device__ static int npts(int order) { return order == 1 ? 1 : 4; }
+  __device__ static void point(int order, int q, double xi[3], double& w)
+  {
+    // order 1: centroid, w 1/6; order 2: (mu,mu,mu) and its three permutations with 1 - 3 mu, w 1/24
+    if(order == 1) { xi[0] = xi[1] = xi[2] = 0.25; w = 0.1666666666666666666666667; return; }
+    const double mu = 0.13819660112501051517954131656;
+    xi[0] = q == 1 ? 1. - 3. * mu : mu;
+    xi[1] = q == 2 ? 1. - 3. * mu : mu;
+    xi[2] = q == 3 ? 1. - 3. * mu : mu;
+    w = 0.04166666666666666666666666666666666667;
+  }
+  __device__ static void shape(const double xi[3], double N[4], double dN[3][4])
+  {
+    N[0] = 1. - xi[0] - xi[1] - xi[2]; N[1] = xi[0]; N[2] = xi[1]; N[3] = xi[2];
+    for(int l = 0; l < 3; ++l)
+      for(int m = 0; m < 4; ++m) dN[l][m] = m == 0 ? -1. : (m == l + 1 ? 1. : 0.);
+  }
+};
+
+struct Hex {
+  static constexpr int K = 8, D = 3, ORDER = 2;
+  __device__ static int npts(int order) { return order == 1 ? 1 : 8; }
+  __device__ static void point(int order, int q, double xi[3], double& w)
+  {
+    // order 1: centre, w 8; order 2: the eight points in the order of GaussImplementation.hpp:615-676
+    if(order == 1) { xi[0] = xi[1] = xi[2] = 0.; w = 8.; return; }
+    const double x = 0.5773502691896257645091488;
+    const signed char s[8][3] = {{1,1,1},{-1,1,1},{1,-1,1},{1,1,-1},{1,-1,-1},{-1,1,-1},{-1,-1,1},{-1,-1,-1}};
+    for(int d = 0; d < 3; ++d) xi[d] = s[q][d] * x;
+    w = 1.;
+  }
+  __device__ static void shape(const double xi[3], double N[8], double dN[3][8])
+  {
+    // node m at (sx, sy, sz): (-,-,-), (+,-,-), (+,+,-), (-,+,-), then the same with z = +1
+    for(int m = 0; m < 8; ++m) {
+      const double sx = ((m ^ (m >> 1)) & 1) ? 1. : -1., sy = (m & 2) ? 1. : -1., sz = (m & 4) ? 1. : -1.;
+      const double a = 1. + sx * xi[0], b = 1. + sy * xi[1], c = 1. + sz * xi[2];
+      N[m] = 0.125 * a * b * c;
+      dN[0][m] = 0.125 * sx * b * c;
+      dN[1][m] = 0.125 * sy * a * c;
+      dN[2][m] = 0.125 * sz * a * b;
+    }
+  }
+};
+
+// inverse Jacobian (J(l, d) = dx_d / dxi_l) and det J at xi
 template<class E>
-__device__ __forceinline__ void point_data(const double X[E::K][2], const double xi[2], double N[E::K],
-                                           double G[2][E::K], double& det)
+__device__ __forceinline__ void inverse_jacobian(const double X[E::K][E::D], const double xi[E::D], double N[E::K],
+                                                 double dN[E::D][E::K], double Ji[E::D][E::D], double& det)
 {
-  double dN[2][E::K];
+  constexpr int D = E::D;
   E::shape(xi, N, dN);
-  double J[2][2];
-  for(int l = 0; l < 2; ++l)
-    for(int d = 0; d < 2; ++d) {
+  double J[D][D];
+  for(int l = 0; l < D; ++l)
+    for(int d = 0; d < D; ++d) {
       double s = 0.;
       for(int m = 0; m < E::K; ++m) s += dN[l][m] * X[m][d];
       J[l][d] = s;
     }
-  det = J[0][0] * J[1][1] - J[0][1] * J[1][0];
-  const double inv = 1. / det;
-  const double Ji[2][2] = {{J[1][1] * inv, -J[0][1] * inv}, {-J[1][0] * inv, J[0][0] * inv}};
-  for(int d = 0; d < 2; ++d)
-    for(int m = 0; m < E::K; ++m) G[d][m] = Ji[d][0] * dN[0][m] + Ji[d][1] * dN[1][m];
+  if constexpr(D == 2) {
+    det = J[0][0] * J[1][1] - J[0][1] * J[1][0];
+    const double inv = 1. / det;
+    Ji[0][0] = J[1][1] * inv; Ji[0][1] = -J[0][1] * inv;
+    Ji[1][0] = -J[1][0] * inv; Ji[1][1] = J[0][0] * inv;
+  } else {
+    const double c00 = J[1][1] * J[2][2] - J[1][2] * J[2][1];
+    const double c01 = J[1][2] * J[2][0] - J[1][0] * J[2][2];
+    const double c02 = J[1][0] * J[2][1] - J[1][1] * J[2][0];
+    det = J[0][0] * c00 + J[0][1] * c01 + J[0][2] * c02;
+    const double inv = 1. / det;
+    Ji[0][0] = c00 * inv;
+    Ji[1][0] = c01 * inv;
+    Ji[2][0] = c02 * inv;
+    Ji[0][1] = (J[0][2] * J[2][1] - J[0][1] * J[2][2]) * inv;
+    Ji[1][1] = (J[0][0] * J[2][2] - J[0][2] * J[2][0]) * inv;
+    Ji[2][1] = (J[0][1] * J[2][0] - J[0][0] * J[2][1]) * inv;
+    Ji[0][2] = (J[0][1] * J[1][2] - J[0][2] * J[1][1]) * inv;
+    Ji[1][2] = (J[0][2] * J[1][0] - J[0][0] * J[1][2]) * inv;
+    Ji[2][2] = (J[0][0] * J[1][1] - J[0][1] * J[1][0]) * inv;
+  }
 }
 
+// N, physical gradients G = J^-1 dN and det J at xi
 template<class E>
-__device__ __forceinline__ void inverse_jacobian(const double X[E::K][2], const double xi[2], double Ji[2][2])
+__device__ __forceinline__ void point_data(const double X[E::K][E::D], const double xi[E::D], double N[E::K],
+                                           double G[E::D][E::K], double& det)
 {
-  double N[E::K], dN[2][E::K], J[2][2];
-  E::shape(xi, N, dN);
-  for(int l = 0; l < 2; ++l)
-    for(int d = 0; d < 2; ++d) {
+  constexpr int D = E::D;
+  double dN[D][E::K], Ji[D][D];
+  inverse_jacobian<E>(X, xi, N, dN, Ji, det);
+  for(int d = 0; d < D; ++d)
+    for(int m = 0; m < E::K; ++m) {
       double s = 0.;
-      for(int m = 0; m < E::K; ++m) s += dN[l][m] * X[m][d];
-      J[l][d] = s;
+      for(int l = 0; l < D; ++l) s += Ji[d][l] * dN[l][m];
+      G[d][m] = s;
     }
-  const double inv = 1. / (J[0][0] * J[1][1] - J[0][1] * J[1][0]);
-  Ji[0][0] = J[1][1] * inv; Ji[0][1] = -J[0][1] * inv;
-  Ji[1][0] = -J[1][0] * inv; Ji[1][1] = J[0][0] * inv;
 }
 
 struct TauP { int type; double dt, c1, c2, a_ps, a_su, a_bu; };
@@ -106,45 +171,55 @@ struct TauP { int type; double dt, c1, c2, a_ps, a_su, a_bu; };
 // ComputeTauImpl (SUPG.hpp:107-219), METRIC or TEZDUYAR, at the element centre with nodal velocity u
 // and element viscosity |mean(nu)|
 template<class E>
-__device__ void tau(const TauP& t, const double X[E::K][2], const double u[E::K][2], const double nu[E::K],
+__device__ void tau(const TauP& t, const double X[E::K][E::D], const double u[E::K][E::D], const double nu[E::K],
                     double& tau_ps, double& tau_su, double& tau_bulk)
 {
+  constexpr int D = E::D, K = E::K;
   double s = 0.;
-  for(int m = 0; m < E::K; ++m) s += nu[m];
-  const double enu = fabs(s / E::K);
-  double xi[2], w, N[E::K], G[2][E::K], det;
+  for(int m = 0; m < K; ++m) s += nu[m];
+  const double enu = fabs(s / K);
+  double xi[D], w, N[K], dN[D][K], Ji[D][D], det;
   E::point(1, 0, xi, w);
-  point_data<E>(X, xi, N, G, det);
-  double uc[2];
-  for(int d = 0; d < 2; ++d) {
+  inverse_jacobian<E>(X, xi, N, dN, Ji, det);
+  double uc[D];
+  for(int d = 0; d < D; ++d) {
     double v = 0.;
-    for(int m = 0; m < E::K; ++m) v += N[m] * u[m][d];
+    for(int m = 0; m < K; ++m) v += N[m] * u[m][d];
     uc[d] = v;
   }
   if(t.type == CF3HIP_SUPG_TEZDUYAR) {
     double h = 1e10;
-    for(int i = 0; i < E::K; ++i)
-      for(int j = 0; j < E::K; ++j)
+    for(int i = 0; i < K; ++i)
+      for(int j = 0; j < K; ++j)
         if(i != j) {
-          const double dx = X[i][0] - X[j][0], dy = X[i][1] - X[j][1];
-          h = fmin(h, dx * dx + dy * dy);
+          double h2 = 0.;
+          for(int d = 0; d < D; ++d) { const double df = X[i][d] - X[j][d]; h2 += df * df; }
+          h = fmin(h, h2);
         }
     h = sqrt(h);
-    const double umag = sqrt(uc[0] * uc[0] + uc[1] * uc[1]);
+    double um2 = 0.;
+    for(int d = 0; d < D; ++d) um2 += uc[d] * uc[d];
+    const double umag = sqrt(um2);
     tau_su = 1. / ((2. * umag) / h + 2. / t.dt + (4. * enu) / (h * h));
     tau_ps = tau_su;
     tau_bulk = tau_su * umag * umag;
   } else {
-    double Ji[2][2];
-    inverse_jacobian<E>(X, xi, Ji);
-    double g[2][2];
-    for(int i = 0; i < 2; ++i)
-      for(int j = 0; j < 2; ++j) g[i][j] = Ji[0][i] * Ji[0][j] + Ji[1][i] * Ji[1][j];
-    double ugu = 0.;
-    for(int j = 0; j < 2; ++j) ugu += (uc[0] * g[0][j] + uc[1] * g[1][j]) * uc[j];
+    double g[D][D];
+    for(int i = 0; i < D; ++i)
+      for(int j = 0; j < D; ++j) {
+        double v = 0.;
+        for(int l = 0; l < D; ++l) v += Ji[l][i] * Ji[l][j];
+        g[i][j] = v;
+      }
+    double ugu = 0., gsq = 0., tr = 0.;
+    for(int j = 0; j < D; ++j) {
+      double v = 0.;
+      for(int i = 0; i < D; ++i) v += uc[i] * g[i][j];
+      ugu += v * uc[j];
+      tr += g[j][j];
+      for(int i = 0; i < D; ++i) gsq += g[i][j] * g[i][j];
+    }
     const double adv2 = fabs(ugu);
-    const double gsq = g[0][0] * g[0][0] + g[0][1] * g[0][1] + g[1][0] * g[1][0] + g[1][1] * g[1][1];
-    const double tr = g[0][0] + g[1][1];
     const double diff = enu * enu * gsq;
     tau_su = 1. / sqrt((4. * t.c1 * t.c1 / (t.dt * t.dt)) + adv2 + t.c2 * diff);
     tau_ps = tau_su;
@@ -158,17 +233,17 @@ __device__ void tau(const TauP& t, const double X[E::K][2], const double u[E::K]
 
 enum { OP_MASS = 0, OP_PMAT = 1, OP_VMAT = 2, OP_URHS = 3, OP_PRHS = 4, OP_AUP = 5 };
 
-// element outputs per element: MASS K, PMAT K*K, VMAT K*K*4 (block (a,b) row-major 2x2),
-// URHS/AUP K*2 (node-major), PRHS K
+// element outputs per element: MASS K, PMAT K*K, VMAT K*K*D*D (block (a,b) row-major D x D),
+// URHS/AUP K*D (node-major), PRHS K
 struct SiArgs {
   uint32_t n_elems;
   const uint32_t* conn;
   const uint32_t* p2m;
-  const double* x[2];
-  const double* u[2];     // velocity field (tau of PRHS and of the pressure matrix)
-  const double* ua[2];    // advection velocity
+  const double* x[3];
+  const double* u[3];     // velocity field (tau of PRHS and of the pressure matrix)
+  const double* ua[3];    // advection velocity
   const double* nu;
-  const double* a;        // LSS vectors (rows x 2 / rows)
+  const double* a;        // LSS vectors (rows x D / rows)
   const double* U;
   const double* da;
   const double* P;
@@ -179,27 +254,39 @@ struct SiArgs {
   double* stage;
 };
 
+template<class E>
+__device__ __forceinline__ void interp(const double N[E::K], const double f[E::K][E::D], double out[E::D])
+{
+  for(int d = 0; d < E::D; ++d) {
+    double v = 0.;
+    for(int m = 0; m < E::K; ++m) v += N[m] * f[m][d];
+    out[d] = v;
+  }
+}
+
 template<class E, int OP>
 __global__ void __launch_bounds__(256) k_si_elem(SiArgs g)
 {
-  constexpr int K = E::K;
+  constexpr int K = E::K, D = E::D;
   const uint32_t e = blockIdx.x * blockDim.x + threadIdx.x;
   if(e >= g.n_elems) return;
   uint32_t cn[K], rw[K];
-  double X[K][2], ua[K][2], uf[K][2], nu[K];
+  double X[K][D], ua[K][D], uf[K][D], nu[K];
   for(int m = 0; m < K; ++m) {
     cn[m] = g.conn[(uint64_t)e * K + m];
     rw[m] = g.p2m[cn[m]];
-    X[m][0] = g.x[0][cn[m]]; X[m][1] = g.x[1][cn[m]];
-    ua[m][0] = g.ua[0][cn[m]]; ua[m][1] = g.ua[1][cn[m]];
-    uf[m][0] = g.u[0][cn[m]]; uf[m][1] = g.u[1][cn[m]];
+    for(int d = 0; d < D; ++d) {
+      X[m][d] = g.x[d][cn[m]];
+      ua[m][d] = g.ua[d][cn[m]];
+      uf[m][d] = g.u[d][cn[m]];
+    }
     nu[m] = g.nu[cn[m]];
   }
   const double dt = g.t.dt;
   double tau_ps = 0., tau_su = 0., tau_bulk = 0.;
   if(OP == OP_PMAT || OP == OP_PRHS) tau<E>(g.t, X, uf, nu, tau_ps, tau_su, tau_bulk);
   else if(OP != OP_MASS) tau<E>(g.t, X, ua, nu, tau_ps, tau_su, tau_bulk);
-  double xi[2], w, N[K], G[2][K], det;
+  double xi[D], w, N[K], G[D][K], det;
 
   if constexpr(OP == OP_MASS || OP == OP_PMAT) {
     // MassMatrixAssembly: row sums of int N^T N; assemble_pp: theta (tau_ps + dt) int G^T G (order 2)
@@ -211,8 +298,15 @@ __global__ void __launch_bounds__(256) k_si_elem(SiArgs g)
       point_data<E>(X, xi, N, G, det);
       const double wq = w * det;
       for(int a = 0; a < K; ++a)
-        for(int b = 0; b < K; ++b)
-          A[a][b] += OP == OP_MASS ? (wq * N[a]) * N[b] : wq * (G[0][a] * G[0][b] + G[1][a] * G[1][b]);
+        for(int b = 0; b < K; ++b) {
+          if(OP == OP_MASS) {
+            A[a][b] += (wq * N[a]) * N[b];
+          } else {
+            double gg = 0.;
+            for(int d = 0; d < D; ++d) gg += G[d][a] * G[d][b];
+            A[a][b] += wq * gg;
+          }
+        }
     }
     double* out = g.stage + (uint64_t)e * (OP == OP_MASS ? K : K * K);
     if(OP == OP_MASS) {
@@ -228,9 +322,12 @@ __global__ void __launch_bounds__(256) k_si_elem(SiArgs g)
     }
   } else if constexpr(OP == OP_VMAT) {
     // VelocityAssembly (MatrixAssembly.hpp:49-146): T + theta dt M
-    double T[K][K], L[K][K], B[2][K][K];
+    double T[K][K], L[K][K], B[D][K][K];
     for(int a = 0; a < K; ++a)
-      for(int b = 0; b < K; ++b) { T[a][b] = 0.; L[a][b] = 0.; B[0][a][b] = 0.; B[1][a][b] = 0.; }
+      for(int b = 0; b < K; ++b) {
+        T[a][b] = 0.; L[a][b] = 0.;
+        for(int d = 0; d < D; ++d) B[d][a][b] = 0.;
+      }
     for(int q = 0; q < E::npts(2); ++q) {   // mass with the second-order rule
       E::point(2, q, xi, w);
       point_data<E>(X, xi, N, G, det);
@@ -246,39 +343,42 @@ __global__ void __launch_bounds__(256) k_si_elem(SiArgs g)
       const double wq = w * det, bulk = wq * tau_bulk, wn = wq * nuq;
       for(int a = 0; a < K; ++a)
         for(int b = 0; b < K; ++b) {
-          L[a][b] += wn * (G[0][a] * G[0][b] + G[1][a] * G[1][b]);
-          B[0][a][b] += (bulk * G[0][a]) * G[0][b];
-          B[1][a][b] += (bulk * G[1][a]) * G[1][b];
+          double gg = 0.;
+          for(int d = 0; d < D; ++d) {
+            gg += G[d][a] * G[d][b];
+            B[d][a][b] += (bulk * G[d][a]) * G[d][b];
+          }
+          L[a][b] += wn * gg;
         }
     }
-    E::point(1, 0, xi, w);   // off-diagonal bulk block with the one-point rule
+    E::point(1, 0, xi, w);   // off-diagonal bulk blocks with the one-point rule
     point_data<E>(X, xi, N, G, det);
     const double bulk = w * det * tau_bulk, tdt = g.theta * dt;
-    double* out = g.stage + (uint64_t)e * K * K * 4;
+    double* out = g.stage + (uint64_t)e * K * K * D * D;
     for(int a = 0; a < K; ++a)
       for(int b = 0; b < K; ++b) {
-        double* o = out + (a * K + b) * 4;
-        o[0] = T[a][b] + tdt * (L[a][b] + B[0][a][b]);
-        o[1] = tdt * ((bulk * G[0][a]) * G[1][b]);
-        o[2] = tdt * ((bulk * G[0][b]) * G[1][a]);
-        o[3] = T[a][b] + tdt * (L[a][b] + B[1][a][b]);
+        double* o = out + (a * K + b) * D * D;
+        for(int r = 0; r < D; ++r)
+          for(int c = 0; c < D; ++c)
+            o[r * D + c] = r == c ? T[a][b] + tdt * (L[a][b] + B[r][a][b]) : tdt * ((bulk * G[r][a]) * G[c][b]);
       }
   } else if constexpr(OP == OP_URHS) {
     // VelocityRHS (MatrixAssembly.hpp:206-330) with a, dt (1 - theta) a - u_vec, (1 - theta) dps - p_vec
-    double av[K][2], dtamu[K][2], ppdp[K];
+    double av[K][D], dtamu[K][D], ppdp[K];
     for(int m = 0; m < K; ++m) {
       const uint64_t r = rw[m];
-      for(int d = 0; d < 2; ++d) {
-        av[m][d] = g.a[r * 2 + d];
-        dtamu[m][d] = dt * (1. - g.theta) * av[m][d] - g.U[r * 2 + d];
+      for(int d = 0; d < D; ++d) {
+        av[m][d] = g.a[r * D + d];
+        dtamu[m][d] = dt * (1. - g.theta) * av[m][d] - g.U[r * D + d];
       }
       ppdp[m] = (1. - g.theta) * g.dps[r] - g.P[r];
     }
-    double res[2][K];
-    for(int a = 0; a < K; ++a) res[0][a] = res[1][a] = 0.;
+    double res[D][K];
+    for(int d = 0; d < D; ++d)
+      for(int a = 0; a < K; ++a) res[d][a] = 0.;
+    // simplices: pass 0 = mass-type terms with the second-order rule, pass 1 = the rest (order 1);
+    // Q1 elements: all terms in one second-order loop
     for(int pass = 0; pass < (E::ORDER == 2 ? 1 : 2); ++pass) {
-      // simplices: pass 0 = mass-type terms with the second-order rule, pass 1 = the rest (order 1);
-      // quadrilaterals: all terms in one second-order loop
       const bool mass_terms = E::ORDER == 2 || pass == 0;
       const bool other_terms = E::ORDER == 2 || pass == 1;
       const int order = (E::ORDER == 2 || pass == 0) ? 2 : 1;
@@ -286,50 +386,56 @@ __global__ void __launch_bounds__(256) k_si_elem(SiArgs g)
         E::point(order, q, xi, w);
         point_data<E>(X, xi, N, G, det);
         const double wq = w * det;
-        double uq[2];
-        for(int d = 0; d < 2; ++d) {
-          double v = 0.;
-          for(int m = 0; m < K; ++m) v += N[m] * ua[m][d];
-          uq[d] = v;
-        }
+        double uq[D];
+        interp<E>(N, ua, uq);
         double adv[K], npa[K];
-        for(int m = 0; m < K; ++m) { adv[m] = uq[0] * G[0][m] + uq[1] * G[1][m]; npa[m] = tau_su * adv[m] + N[m]; }
+        for(int m = 0; m < K; ++m) {
+          double v = 0.;
+          for(int d = 0; d < D; ++d) v += uq[d] * G[d][m];
+          adv[m] = v;
+          npa[m] = tau_su * v + N[m];
+        }
         double f = 0.;
-        for(int i = 0; i < 2; ++i)
+        for(int i = 0; i < D; ++i)
           for(int m = 0; m < K; ++m) f += G[i][m] * dtamu[m][i];
         f *= wq;
         double nuq = 0., np = 0.;
         for(int m = 0; m < K; ++m) { nuq += N[m] * nu[m]; np += N[m] * ppdp[m]; }
         const double b = -wq * np, f_bulk = tau_bulk * f, f_skew = 0.5 * f;
-        for(int i = 0; i < 2; ++i) {
-          double na = 0., ad = 0., gp = 0., dv[2] = {0., 0.};
+        for(int i = 0; i < D; ++i) {
+          double na = 0., ad = 0., gp = 0., dv[D];
+          for(int l = 0; l < D; ++l) dv[l] = 0.;
           for(int m = 0; m < K; ++m) {
             na += N[m] * av[m][i];
             ad += adv[m] * dtamu[m][i];
             gp += G[i][m] * ppdp[m];
-            dv[0] += G[0][m] * dtamu[m][i];
-            dv[1] += G[1][m] * dtamu[m][i];
+            for(int l = 0; l < D; ++l) dv[l] += G[l][m] * dtamu[m][i];
           }
           const double c = -wq * na, ee = wq * ad, a_ = (wq * tau_su) * gp, wv = wq * nuq;
           for(int a = 0; a < K; ++a) {
             double v = 0.;
-            if(other_terms) v += adv[a] * a_ + G[i][a] * (b + f_bulk) + (G[0][a] * (wv * dv[0]) + G[1][a] * (wv * dv[1]));
+            if(other_terms) {
+              double gd = 0.;
+              for(int l = 0; l < D; ++l) gd += G[l][a] * (wv * dv[l]);
+              v += adv[a] * a_ + G[i][a] * (b + f_bulk) + gd;
+            }
             if(mass_terms) v += npa[a] * (c + ee + f_skew * uq[i]);
             res[i][a] += v;
           }
         }
       }
     }
-    double* out = g.stage + (uint64_t)e * K * 2;
-    for(int a = 0; a < K; ++a) { out[a * 2] = res[0][a]; out[a * 2 + 1] = res[1][a]; }
+    double* out = g.stage + (uint64_t)e * K * D;
+    for(int a = 0; a < K; ++a)
+      for(int d = 0; d < D; ++d) out[a * D + d] = res[d][a];
   } else if constexpr(OP == OP_PRHS) {
     // PressureRHS (MatrixAssembly.hpp:148-204)
-    double upda[K][2], apda[K][2], pv[K];
+    double upda[K][D], apda[K][D], pv[K];
     for(int m = 0; m < K; ++m) {
       const uint64_t r = rw[m];
-      for(int d = 0; d < 2; ++d) {
-        upda[m][d] = g.U[r * 2 + d] + dt * g.da[r * 2 + d];
-        apda[m][d] = g.a[r * 2 + d] + g.da[r * 2 + d];
+      for(int d = 0; d < D; ++d) {
+        upda[m][d] = g.U[r * D + d] + dt * g.da[r * D + d];
+        apda[m][d] = g.a[r * D + d] + g.da[r * D + d];
       }
       pv[m] = g.P[r];
     }
@@ -342,17 +448,21 @@ __global__ void __launch_bounds__(256) k_si_elem(SiArgs g)
       E::point(E::ORDER, q, xi, w);
       point_data<E>(X, xi, N, G, det);
       const double wq = w * det, tw = wq * tau_ps;
-      double uq[2];
-      for(int d = 0; d < 2; ++d) {
-        double v = 0.;
-        for(int m = 0; m < K; ++m) v += N[m] * ua[m][d];
-        uq[d] = v;
-      }
+      double uq[D];
+      interp<E>(N, ua, uq);
       double adv[K];
-      for(int m = 0; m < K; ++m) adv[m] = tau_ps * (uq[0] * G[0][m] + uq[1] * G[1][m]);
+      for(int m = 0; m < K; ++m) {
+        double v = 0.;
+        for(int d = 0; d < D; ++d) v += uq[d] * G[d][m];
+        adv[m] = tau_ps * v;
+      }
       for(int a = 0; a < K; ++a)
-        for(int b = 0; b < K; ++b) App[a][b] += tw * (G[0][a] * G[0][b] + G[1][a] * G[1][b]);
-      for(int i = 0; i < 2; ++i) {
+        for(int b = 0; b < K; ++b) {
+          double gg = 0.;
+          for(int d = 0; d < D; ++d) gg += G[d][a] * G[d][b];
+          App[a][b] += tw * gg;
+        }
+      for(int i = 0; i < D; ++i) {
         double ga = 0., av = 0., na = 0.;
         for(int m = 0; m < K; ++m) { ga += G[i][m] * upda[m][i]; av += adv[m] * upda[m][i]; na += N[m] * apda[m][i]; }
         const double a_ = wq * ga, b = wq * av, c = tw * na;
@@ -369,29 +479,31 @@ __global__ void __launch_bounds__(256) k_si_elem(SiArgs g)
     // ApplyAup (MatrixAssembly.hpp:332-376)
     double dp[K];
     for(int m = 0; m < K; ++m) dp[m] = g.dp[rw[m]];
-    double res[2][K];
-    for(int a = 0; a < K; ++a) res[0][a] = res[1][a] = 0.;
+    double res[D][K];
+    for(int d = 0; d < D; ++d)
+      for(int a = 0; a < K; ++a) res[d][a] = 0.;
     for(int q = 0; q < E::npts(E::ORDER); ++q) {
       E::point(E::ORDER, q, xi, w);
       point_data<E>(X, xi, N, G, det);
       const double wq = g.theta * w * det, ws = wq * tau_su;
-      double uq[2];
-      for(int d = 0; d < 2; ++d) {
-        double v = 0.;
-        for(int m = 0; m < K; ++m) v += N[m] * ua[m][d];
-        uq[d] = v;
-      }
+      double uq[D];
+      interp<E>(N, ua, uq);
       double nd = 0.;
       for(int m = 0; m < K; ++m) nd += N[m] * dp[m];
       const double b = -wq * nd;
-      for(int i = 0; i < 2; ++i) {
+      for(int i = 0; i < D; ++i) {
         double a_ = 0.;
         for(int m = 0; m < K; ++m) a_ += G[i][m] * dp[m];
-        for(int m = 0; m < K; ++m) res[i][m] += a_ * (ws * (uq[0] * G[0][m] + uq[1] * G[1][m])) + b * G[i][m];
+        for(int m = 0; m < K; ++m) {
+          double v = 0.;
+          for(int d = 0; d < D; ++d) v += uq[d] * G[d][m];
+          res[i][m] += a_ * (ws * v) + b * G[i][m];
+        }
       }
     }
-    double* out = g.stage + (uint64_t)e * K * 2;
-    for(int a = 0; a < K; ++a) { out[a * 2] = res[0][a]; out[a * 2 + 1] = res[1][a]; }
+    double* out = g.stage + (uint64_t)e * K * D;
+    for(int a = 0; a < K; ++a)
+      for(int d = 0; d < D; ++d) out[a * D + d] = res[d][a];
   }
 }
 
@@ -412,13 +524,14 @@ __global__ void __launch_bounds__(256) k_si_gather(uint32_t n, int width, const 
 }
 
 // SetSolution: row r takes the value of the last mesh node of that row (a node loop in node order)
-__global__ void k_si_set_solution(uint32_t rows, int dim, const uint32_t* __restrict__ src, const double* f0,
-                                  const double* f1, double* __restrict__ out)
+struct Comp3 { const double* c[3]; };
+
+__global__ void k_si_set_solution(uint32_t rows, int dim, const uint32_t* __restrict__ src, Comp3 f,
+                                  double* __restrict__ out)
 {
   const uint32_t r = blockIdx.x * blockDim.x + threadIdx.x;
   if(r >= rows) return;
-  out[(uint64_t)r * dim] = f0[src[r]];
-  if(dim == 2) out[(uint64_t)r * dim + 1] = f1[src[r]];
+  for(int d = 0; d < dim; ++d) out[(uint64_t)r * dim + d] = f.c[d][src[r]];
 }
 
 // delta_a = delta_a - Ml^-1 (Aup delta_p); a += delta_a; U += dt delta_a  (InnerLoop::execute :162-175)
@@ -457,16 +570,17 @@ __global__ void k_si_zero_dofs(uint32_t n, const uint32_t* __restrict__ dofs, do
 }
 
 // Update: u3 = u2, u2 = u1, u1 = u, u = U(p2m), p = P(p2m)
-__global__ void k_si_update(uint32_t n, const uint32_t* __restrict__ p2m, double* u0, double* u1_0, double* u2_0,
-                            double* u3_0, double* u0b, double* u1_1, double* u2_1, double* u3_1,
-                            double* __restrict__ p, const double* __restrict__ U, const double* __restrict__ P)
+// one velocity component d (u3 = u2, u2 = u1, u1 = u, u = U(p2m)); the pressure with comp = -1 (p = P(p2m))
+__global__ void k_si_update(uint32_t n, int dim, int d, const uint32_t* __restrict__ p2m, double* __restrict__ u,
+                            double* __restrict__ u1, double* __restrict__ u2, double* __restrict__ u3,
+                            const double* __restrict__ src)
 {
   const uint32_t i = blockIdx.x * blockDim.x + threadIdx.x;
   if(i >= n) return;
   const uint64_t r = p2m[i];
-  u3_0[i] = u2_0[i]; u2_0[i] = u1_0[i]; u1_0[i] = u0[i]; u0[i] = U[r * 2];
-  u3_1[i] = u2_1[i]; u2_1[i] = u1_1[i]; u1_1[i] = u0b[i]; u0b[i] = U[r * 2 + 1];
-  p[i] = P[r];
+  if(d < 0) { u[i] = src[r]; return; }
+  u3[i] = u2[i]; u2[i] = u1[i]; u1[i] = u[i];
+  u[i] = src[r * dim + d];
 }
 
 void stable_lists(const std::vector<uint32_t>& keys, uint32_t n_keys, std::vector<uint32_t>& ptr,
@@ -488,8 +602,6 @@ void si_create(SemiImplicit& S, Lss& U, Lss& P)
   if(!U.mesh || U.mesh != P.mesh)
     CF3_THROW(CF3HIP_ERR_INVALID_ARG, "the velocity and pressure systems must be created from one mesh");
   Mesh& M = *U.mesh;
-  if(M.etype != CF3HIP_TRIAG2D && M.etype != CF3HIP_QUAD2D)
-    CF3_THROW(CF3HIP_ERR_NOT_IMPLEMENTED, "the semi-implicit element actions run on triangles and quadrilaterals");
   if(U.neq != M.dim || P.neq != 1)
     CF3_THROW(CF3HIP_ERR_INVALID_ARG, "velocity system needs neq = dim, pressure system neq = 1");
   if(U.periodic != P.periodic || U.n_nodes != P.n_nodes || U.n_blocks != P.n_blocks)
@@ -521,7 +633,7 @@ void si_create(SemiImplicit& S, Lss& U, Lss& P)
   for(uint32_t i = 0; i < M.n_nodes; ++i) src[p2m[i]] = i;   // node order: the last node of a row wins
   up(S.src, src);
   up(S.p2m, p2m);
-  S.stage.alloc(&c, sizeof(double) * (size_t)E * K * K * 4);
+  S.stage.alloc(&c, sizeof(double) * (size_t)E * K * K * M.dim * M.dim);
   const size_t vd = (size_t)rows * M.dim;
   for(DevBuf* b : {&S.A, &S.Uv, &S.minv, &S.aup}) { b->alloc(&c, sizeof(double) * vd); lss_fill(c, b->as<double>(), vd, 0.); }
   for(DevBuf* b : {&S.Pv, &S.dps}) { b->alloc(&c, sizeof(double) * rows); lss_fill(c, b->as<double>(), rows, 0.); }
@@ -538,7 +650,7 @@ SiArgs si_args(SemiImplicit& S, Fields& F, const cf3hip_ns_params& prm)
   g.n_elems = M.n_elems;
   g.conn = M.conn.as<uint32_t>();
   g.p2m = S.p2m.as<uint32_t>();
-  for(int d = 0; d < 2; ++d) {
+  for(int d = 0; d < M.dim; ++d) {
     g.x[d] = M.x[d].as<double>();
     g.u[d] = F.u[d].as<double>();
     g.ua[d] = F.uadv[d].as<double>();
@@ -560,8 +672,13 @@ template<int OP>
 void launch_elem(SemiImplicit& S, const SiArgs& g)
 {
   const dim3 grid(grid_for(g.n_elems, 256));
-  if(S.mesh->etype == CF3HIP_TRIAG2D) hipLaunchKernelGGL((k_si_elem<Tri, OP>), grid, dim3(256), 0, S.ctx->stream, g);
-  else hipLaunchKernelGGL((k_si_elem<Quad, OP>), grid, dim3(256), 0, S.ctx->stream, g);
+  hipStream_t st = S.ctx->stream;
+  switch(S.mesh->etype) {
+    case CF3HIP_TRIAG2D: hipLaunchKernelGGL((k_si_elem<Tri, OP>), grid, dim3(256), 0, st, g); break;
+    case CF3HIP_QUAD2D: hipLaunchKernelGGL((k_si_elem<Quad, OP>), grid, dim3(256), 0, st, g); break;
+    case CF3HIP_TETRA3D: hipLaunchKernelGGL((k_si_elem<Tet, OP>), grid, dim3(256), 0, st, g); break;
+    default: hipLaunchKernelGGL((k_si_elem<Hex, OP>), grid, dim3(256), 0, st, g); break;
+  }
   HIP_OK(hipGetLastError());
 }
 
@@ -624,7 +741,7 @@ void si_velocity_assemble(SemiImplicit& S, Fields& F, const cf3hip_ns_params& pr
   lss_fill(c, U.rhs.as<double>(), U.dofs, 0.);
   lss_fill(c, U.sol.as<double>(), U.dofs, 0.);
   launch_elem<OP_VMAT>(S, si_args(S, F, prm));
-  gather(S, true, 4, (uint32_t)U.n_blocks, U.values.as<double>(), false);
+  gather(S, true, U.neq * U.neq, (uint32_t)U.n_blocks, U.values.as<double>(), false);
 }
 
 void si_action(SemiImplicit& S, Fields& F, const cf3hip_ns_params& prm, int which)
@@ -633,13 +750,13 @@ void si_action(SemiImplicit& S, Fields& F, const cf3hip_ns_params& prm, int whic
   const SiArgs g = si_args(S, F, prm);
   if(which == CF3HIP_SI_URHS) {
     launch_elem<OP_URHS>(S, g);
-    gather(S, false, 2, S.u->n_nodes, S.u->rhs.as<double>(), true);
+    gather(S, false, S.u->neq, S.u->n_nodes, S.u->rhs.as<double>(), true);
   } else if(which == CF3HIP_SI_PRHS) {
     launch_elem<OP_PRHS>(S, g);
     gather(S, false, 1, S.p->n_nodes, S.p->rhs.as<double>(), true);
   } else if(which == CF3HIP_SI_AUP) {
     launch_elem<OP_AUP>(S, g);
-    gather(S, false, 2, S.u->n_nodes, S.u->rhs.as<double>(), true);
+    gather(S, false, S.u->neq, S.u->n_nodes, S.u->rhs.as<double>(), true);
   } else {
     CF3_THROW(CF3HIP_ERR_INVALID_ARG, "unknown semi-implicit action");
   }
@@ -650,10 +767,13 @@ void si_set_solution(SemiImplicit& S, Fields& F)
   check_fields(S, F);
   Context& c = *S.ctx;
   const uint32_t rows = S.u->n_nodes;
-  hipLaunchKernelGGL(k_si_set_solution, dim3(grid_for(rows, 256)), dim3(256), 0, c.stream, rows, 2,
-                     S.src.as<uint32_t>(), F.u[0].as<double>(), F.u[1].as<double>(), S.u->sol.as<double>());
+  const int dim = S.mesh->dim;
+  Comp3 fu{{F.u[0].as<double>(), F.u[1].as<double>(), dim == 3 ? F.u[2].as<double>() : nullptr}};
+  Comp3 fp{{F.p.as<double>(), nullptr, nullptr}};
+  hipLaunchKernelGGL(k_si_set_solution, dim3(grid_for(rows, 256)), dim3(256), 0, c.stream, rows, dim,
+                     S.src.as<uint32_t>(), fu, S.u->sol.as<double>());
   hipLaunchKernelGGL(k_si_set_solution, dim3(grid_for(rows, 256)), dim3(256), 0, c.stream, rows, 1,
-                     S.src.as<uint32_t>(), F.p.as<double>(), F.p.as<double>(), S.p->sol.as<double>());
+                     S.src.as<uint32_t>(), fp, S.p->sol.as<double>());
   HIP_OK(hipGetLastError());
 }
 
@@ -673,10 +793,13 @@ void si_update(SemiImplicit& S, Fields& F)
 {
   check_fields(S, F);
   Mesh& M = *S.mesh;
-  hipLaunchKernelGGL(k_si_update, dim3(grid_for(M.n_nodes, 256)), dim3(256), 0, S.ctx->stream, M.n_nodes,
-                     S.p2m.as<uint32_t>(), F.u[0].as<double>(), F.u1[0].as<double>(), F.u2[0].as<double>(),
-                     F.u3[0].as<double>(), F.u[1].as<double>(), F.u1[1].as<double>(), F.u2[1].as<double>(),
-                     F.u3[1].as<double>(), F.p.as<double>(), S.Uv.as<double>(), S.Pv.as<double>());
+  const dim3 grid(grid_for(M.n_nodes, 256));
+  for(int d = 0; d < M.dim; ++d)
+    hipLaunchKernelGGL(k_si_update, grid, dim3(256), 0, S.ctx->stream, M.n_nodes, M.dim, d, S.p2m.as<uint32_t>(),
+                       F.u[d].as<double>(), F.u1[d].as<double>(), F.u2[d].as<double>(), F.u3[d].as<double>(),
+                       S.Uv.as<double>());
+  hipLaunchKernelGGL(k_si_update, grid, dim3(256), 0, S.ctx->stream, M.n_nodes, M.dim, -1, S.p2m.as<uint32_t>(),
+                     F.p.as<double>(), nullptr, nullptr, nullptr, S.Pv.as<double>());
   HIP_OK(hipGetLastError());
 }
 

@@ -283,8 +283,8 @@ int cf3hip_batch_assemble(cf3hip_batch batch, uint32_t n, const cf3hip_lss* lss,
 int cf3hip_ns_residual(cf3hip_lss lss, cf3hip_fields fields, const cf3hip_ns_params* ns_prm, const double* x_dev,
                        double* r_dev);
 
-/* ---- UFEM NavierStokesSemiImplicit (plugins/UFEM/src/UFEM/ns_semi_implicit) on triangles and
- *      quadrilaterals.  u_lss: the VelocityLSS system (neq = dim), p_lss: the PressureLSS system
+/* ---- UFEM NavierStokesSemiImplicit (plugins/UFEM/src/UFEM/ns_semi_implicit) on triangles,
+ *      quadrilaterals, tetrahedra and hexahedra.  u_lss: the VelocityLSS system (neq = dim), p_lss: the PressureLSS system
  *      (neq = 1), created from the same mesh with the same periodic links.  The "solution" vectors of
  *      the two systems hold delta_a and delta_p inside the inner loop and u, p outside it
  *      (implementation/NavierStokesSemiImplicit.cpp:84-190, :344, :354). ---- */
@@ -317,7 +317,8 @@ int cf3hip_si_set(cf3hip_si si, int which, const double* host);
  * passes (URHS, velocity BC in acceleration form at the first pass, velocity solve, PRHS, pressure BC
  * at the first pass, pressure solve, ApplyAup, delta_a -= Ml^-1 Aup delta_p, a/u/p/delta_p_sum
  * updates) and Update.  Dirichlet conditions: the new values at mesh nodes (velocity: n x dim);
- * the increments new - old are taken against the fields.  stats: 2 * nb_iterations entries
+ * the increments new - old are taken against the fields.  Each degree of freedom may carry one
+ * condition (CF3HIP_ERR_INVALID_ARG otherwise): resolve overlapping regions before the call.  stats: 2 * nb_iterations entries
  * (velocity and pressure solve of each pass), may be null. */
 int cf3hip_si_step(cf3hip_si si, cf3hip_fields fields, const cf3hip_ns_params* prm, int nb_iterations,
                    const cf3hip_solver_params* u_solver, const cf3hip_solver_params* p_solver, uint32_t n_ubc,

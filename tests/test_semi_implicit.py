@@ -204,3 +204,139 @@ def test_device_taylor_green_semi_implicit_follows_oracle(element):
             assert eu < U_BOUND and ep < P_BOUND
     finally:
         d.close()
+
+
+def _cavity(n=12):
+    """Lid-driven cavity on n x n Q1 quadrilaterals (no periodic links): no-slip on three walls, the lid
+    u = (1, 0) on top (it takes the top corners), p = 0 at node 0; seeded initial velocity."""
+    from coolfluid3_amd import meshgen
+    coords, conn = meshgen.rect_quads(n, n)
+    N = coords.shape[0]
+    rng = np.random.default_rng(11)
+    u0 = rng.uniform(-0.2, 0.2, size=(N, 2))
+    x, y = coords[:, 0], coords[:, 1]
+    lid = np.flatnonzero(np.isclose(y, 1.0))
+    walls = np.flatnonzero((np.isclose(x, 0.0) | np.isclose(x, 1.0) | np.isclose(y, 0.0)) & ~np.isclose(y, 1.0))
+    nodes = np.concatenate([walls, lid]).astype(np.uint32)
+    vals = np.zeros((len(nodes), 2))
+    vals[len(walls):, 0] = 1.0
+    return coords, conn, u0, nodes, vals
+
+
+@pytest.mark.gpu
+def test_device_cavity_with_velocity_conditions_follows_oracle():
+    """Three semi-implicit steps of a lid-driven cavity (velocity conditions in acceleration form at the
+    first inner pass, zero acceleration rows after it, a pinned pressure node) follow the oracle within
+    1e-9; the boundary velocities are the imposed values after the step."""
+    coords, conn, u0, unodes, uvals = _cavity()
+    N = coords.shape[0]
+    nu, dt, theta = 0.01, 0.01, 1.0
+    si = SemiImplicit(pyoracle.get(), pyoracle.QUAD2D, coords, conn, np.arange(N, dtype=np.uint32), nu, dt, theta,
+                      2, u0, np.zeros(N), pbc_nodes=[0], ubc_nodes=unodes)
+    ctx = cabi.Context(0)
+    mesh = cabi.Mesh(ctx, cabi.QUAD2D, coords, conn)
+    us, ps = cabi.System(ctx, mesh, neq=2), cabi.System(ctx, mesh, neq=1)
+    f = cabi.Fields(mesh)
+    f.set(cabi.FIELD_U, u0)
+    f.set(cabi.FIELD_P, np.zeros(N))
+    f.initialize(nu)
+    prm = cabi.ns_params(dt=dt, theta=theta)
+    dsi = cabi.SemiImplicit(us, ps)
+    try:
+        dsi.initial(f, prm)
+        sp = cabi.solver_params(cabi.BICGSTAB_JACOBI, tol=1e-13, max_iter=5000)
+        for step in range(3):
+            dsi.step(f, prm, sp, sp, ubc=(unodes, uvals), pbc=(np.array([0], np.uint32), np.zeros(1)))
+            si.step(np.zeros(1), uvals)
+            u, p = f.get(cabi.FIELD_U), f.get(cabi.FIELD_P)
+            assert np.allclose(u[unodes], uvals, rtol=0, atol=1e-12)
+            assert np.allclose(si.u[unodes], uvals, rtol=0, atol=1e-12)
+            du = np.abs(u - si.u).max() / np.abs(si.u).max()
+            dpp = np.abs(p - si.p).max() / np.abs(si.p).max()
+            print("cavity", step + 1, f"du {du:.2e} dp {dpp:.2e}")
+            assert du < 1e-9 and dpp < 1e-9
+        # a dof under two conditions is refused
+        twice = np.concatenate([unodes, unodes[:1]])
+        with pytest.raises(cabi.Cf3HipError):
+            dsi.step(f, prm, sp, sp, ubc=(twice, np.vstack([uvals, uvals[:1]])))
+    finally:
+        dsi.close()
+        for x in (f, ps, us, mesh, ctx):
+            x.close()
+
+
+def _unique_velocity_conditions(case):
+    """Velocity conditions of the case's regions with the later region winning on shared nodes (each
+    degree of freedom carries one condition), and its pressure nodes."""
+    vel, pres = {}, {}
+    for _, nodes, spec in case.bc_regions:
+        c = case.coords[nodes]
+        vals = {eq: (v(c) if callable(v) else np.full(len(nodes), float(v))) for eq, v in spec.items()}
+        for j, n in enumerate(nodes):
+            if case.dim in vals:
+                pres[int(n)] = vals[case.dim][j]
+            if 0 in vals:
+                vel[int(n)] = np.array([vals[d][j] for d in range(case.dim)])
+    un = np.array(sorted(vel), dtype=np.uint32)
+    pn = np.array(sorted(pres), dtype=np.uint32)
+    return un, np.array([vel[int(n)] for n in un]), pn, np.array([pres[int(n)] for n in pn])
+
+
+@pytest.mark.gpu
+@pytest.mark.parametrize("name", ["tets", "hexas"])
+def test_device_3d_follows_oracle(name):
+    """Tetrahedra (order-1 rule and the order-2 mass terms) and hexahedra (order 2): initial matrices,
+    velocity matrix and the three actions within 1e-12, then two steps with the case's velocity
+    conditions (later regions win) and pinned pressure within 1e-9."""
+    case = (cases.inflow_cube_tets(4, nu=1e-2, dt=1e-2) if name == "tets"
+            else cases.lid_cavity_cube_hexas(4, nu=1e-2, dt=1e-2, random_velocity=True))
+    N, d = case.n_nodes, case.dim
+    un, uv, pn, pv = _unique_velocity_conditions(case)
+    si = SemiImplicit(pyoracle.get(), case.etype, case.coords, case.conn, np.arange(N, dtype=np.uint32), case.nu,
+                      case.dt, 1.0, 2, case.u0, case.p0, pbc_nodes=pn, ubc_nodes=un)
+    ctx = cabi.Context(0)
+    mesh = cabi.Mesh(ctx, case.etype, case.coords, case.conn)
+    us, ps = cabi.System(ctx, mesh, neq=d), cabi.System(ctx, mesh, neq=1)
+    f = cabi.Fields(mesh)
+    f.set(cabi.FIELD_U, case.u0)
+    f.set(cabi.FIELD_P, case.p0)
+    f.initialize(case.nu)
+    prm = cabi.ns_params(dt=case.dt, theta=1.0)
+    dsi = cabi.SemiImplicit(us, ps)
+    try:
+        dsi.initial(f, prm)
+        assert _close(ps.values(), si.pvals, 1e-12)
+        assert _close(dsi.get(cabi.SI_VEC_MINV), si.minv, 1e-12)
+        dsi.velocity_assemble(f, prm)
+        si.uadv = si.u.copy()
+        assert _close(us.values(), si.velocity_matrix(), 1e-12)
+        rng = np.random.default_rng(3)
+        a, U, P, dps = rng.normal(size=d * N), rng.normal(size=d * N), rng.normal(size=N), rng.normal(size=N)
+        da, dp = rng.normal(size=d * N), rng.normal(size=N)
+        for w, v in ((cabi.SI_VEC_A, a), (cabi.SI_VEC_U, U), (cabi.SI_VEC_P, P), (cabi.SI_VEC_DPS, dps)):
+            dsi.set(w, v)
+        us.set_vector(cabi.SOLUTION, da)
+        ps.set_vector(cabi.SOLUTION, dp)
+        us.reset_vector(cabi.RHS)
+        dsi.action(f, prm, cabi.SI_URHS)
+        assert _close(us.vector(cabi.RHS), si.velocity_rhs(a, U, dps, P), 1e-12)
+        ps.reset_vector(cabi.RHS)
+        dsi.action(f, prm, cabi.SI_PRHS)
+        assert _close(ps.vector(cabi.RHS), si.pressure_rhs(U, a, da, P), 1e-12)
+        us.reset_vector(cabi.RHS)
+        dsi.action(f, prm, cabi.SI_AUP)
+        assert _close(us.vector(cabi.RHS), si.apply_aup(dp), 1e-12)
+        sp = cabi.solver_params(cabi.BICGSTAB_JACOBI, tol=1e-13, max_iter=5000)
+        for step in range(2):
+            dsi.step(f, prm, sp, sp, ubc=(un, uv), pbc=(pn, pv))
+            si.step(pv, uv)
+            u, p = f.get(cabi.FIELD_U), f.get(cabi.FIELD_P)
+            du = np.abs(u - si.u).max() / np.abs(si.u).max()
+            dpp = np.abs(p - si.p).max() / max(np.abs(si.p).max(), 1e-300)
+            print(name, step + 1, f"du {du:.2e} dp {dpp:.2e}")
+            assert np.allclose(u[un], uv, rtol=0, atol=1e-12)
+            assert du < 1e-9 and dpp < 1e-9
+    finally:
+        dsi.close()
+        for x in (f, ps, us, mesh, ctx):
+            x.close()
