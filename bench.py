@@ -208,7 +208,7 @@ def taylor_green_workload(ctx, element):
             "steps": steps, "within_reference_bounds": bool(max(last["u"], last["v"]) < 1.4e-3 and last["p"] < 3.3e-3)}
 
 
-def semi_implicit_workload(ctx, scale_segments=512, scale_steps=2):
+def semi_implicit_workload(ctx, scale_segments=512, scale_steps=2, with_cpu=True):
     """The semi-implicit half of the Taylor-Green script (NavierStokesSemiImplicit, two inner passes,
     BiCGStab + Jacobi to 1e-13 for the velocity and pressure systems): errors after each step at 32
     segments, then the same step on scale_segments^2 quadrilaterals with the time of the velocity-matrix
@@ -276,13 +276,27 @@ def semi_implicit_workload(ctx, scale_segments=512, scale_steps=2):
     E, N = case.conn.shape[0], case.n_nodes
     # compulsory bytes: coordinates, advection velocity, viscosity, connectivity, 2x2 blocks written
     alg = N * (2 * 8 + 2 * 8 + 8) + E * 4 * 4 + us.n_blocks * 4 * 8
+    p2m = us.p2m()
     close(objs)
+    cpu = None
+    if with_cpu:
+        # the oracle's velocity-matrix assembly on the same mesh (one thread, element order)
+        from oracle import pyoracle
+        from oracle.semi_implicit import SemiImplicit as OracleSI
+        osi = OracleSI(pyoracle.get(), case.etype, case.coords, case.conn, p2m, case.nu, case.dt, case.theta, 2,
+                       case.u0, case.p0)
+        t0 = time.perf_counter()
+        osi.velocity_matrix()
+        cpu_ms = (time.perf_counter() - t0) * 1e3
+        cpu = {"value": E / cpu_ms / 1e3, "unit": "Melem/s", "cores": 1, "kind": "port",
+               "sample": "one velocity-matrix assembly of the same mesh (oracle restatement, one thread)"}
     out["scale"] = {"workload": "Taylor-Green semi-implicit step, %d^2 periodic Q1 quadrilaterals (%d elements, "
                                 "%d velocity rows)" % (scale_segments, E, us.n_nodes),
                     "step_ms": [s[0] for s in steps], "iterations_u_p_per_pass": [s[1] for s in steps],
                     "velocity_assembly_ms": asm_ms, "velocity_assembly_melem_s": E / asm_ms / 1e3,
                     "velocity_assembly_algorithmic_gb_s": alg / asm_ms / 1e6,
-                    "velocity_assembly_hbm_frac": alg / asm_ms / 1e6 / 8000.0}
+                    "velocity_assembly_hbm_frac": alg / asm_ms / 1e6 / 8000.0,
+                    "velocity_assembly_cpu_baseline": cpu}
     return out
 
 
@@ -419,7 +433,7 @@ def main():
     if not args.no_workloads:
         for key, fn in (("cavity_2d", lambda: cavity2d_workload(ctx)),
                         ("taylor_green", lambda: {e: taylor_green_workload(ctx, e) for e in ("quad", "triag")}),
-                        ("semi_implicit", lambda: semi_implicit_workload(ctx)),
+                        ("semi_implicit", lambda: semi_implicit_workload(ctx, with_cpu=not args.no_cpu_baseline)),
                         ("precision_tets", lambda: precision_workload(ctx, args.precision_cells)),
                         ("hexa_cavity", lambda: hexa_workload(ctx, args.hexa_cells, args.picard)),
                         ("serving", lambda: serving_workload(args.serving_requests, args.serving_slots, args.serving_mode))):
