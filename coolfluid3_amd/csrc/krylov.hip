@@ -102,7 +102,7 @@ __global__ void k_finalize(const double* __restrict__ partials, unsigned G, doub
   }
 }
 
-// ---- fused iteration (FP64 vectors, neq = 4): five launches, no one-workgroup finalisations ----
+// ---- fused iteration (FP64 vectors, neq = 1 to 4): five launches, no one-workgroup finalisations ----
 // Each kernel that needs a scalar reduces the partial sums of the kernel that produced them itself, in
 // every workgroup, in the fixed order of k_finalize (so all workgroups obtain the same bits); workgroup
 // 0 records the scalars in S for the host and for later kernels.  A kernel never reads a slot of S
@@ -195,6 +195,46 @@ __global__ __launch_bounds__(kRT) void k_spmv4_dot(uint32_t N, const uint32_t* _
       acc[0] += d0[o] * yv;
       if constexpr(NDOT > 1) acc[NDOT - 1] += yv * yv;
     }
+  }
+  block_reduce_store<NDOT>(acc, partials, gridDim.x);
+}
+
+// The same for neq = 1, 2 or 3: 16 lanes per node row stride over its blocks, each lane keeps the NEQ
+// row sums of its blocks, and the 16 partial sums are combined in a fixed shuffle order.
+template<typename VR, int NDOT, int NEQ>
+__global__ __launch_bounds__(kRT) void k_spmvn_dot(uint32_t N, const uint32_t* __restrict__ row_ptr,
+                                                   const uint32_t* __restrict__ cols, const VR* __restrict__ vals,
+                                                   const double* __restrict__ x, double* __restrict__ y,
+                                                   const double* __restrict__ d0, const double* __restrict__ S,
+                                                   double* __restrict__ partials)
+{
+  if(S[S_STOP] != 0.) return;
+  double acc[NDOT];
+  for(int d = 0; d < NDOT; ++d) acc[d] = 0.;
+  const int l = threadIdx.x & 15;
+  const uint32_t step = gridDim.x * (kRT / 16);
+  for(uint32_t row = blockIdx.x * (kRT / 16) + (threadIdx.x >> 4); row < N; row += step) {
+    double ys[NEQ];
+    for(int i = 0; i < NEQ; ++i) ys[i] = 0.;
+    for(uint32_t q = row_ptr[row] + l; q < row_ptr[row + 1]; q += 16) {
+      const VR* b = vals + (uint64_t)q * NEQ * NEQ;
+      const double* xc = x + (uint64_t)cols[q] * NEQ;
+      for(int i = 0; i < NEQ; ++i)
+        for(int j = 0; j < NEQ; ++j) ys[i] += (double)b[i * NEQ + j] * xc[j];
+    }
+    for(int i = 0; i < NEQ; ++i) {
+      ys[i] += __shfl_xor(ys[i], 8, 16);
+      ys[i] += __shfl_xor(ys[i], 4, 16);
+      ys[i] += __shfl_xor(ys[i], 2, 16);
+      ys[i] += __shfl_xor(ys[i], 1, 16);
+    }
+    if(l == 0)
+      for(int i = 0; i < NEQ; ++i) {
+        const uint64_t o = (uint64_t)row * NEQ + i;
+        y[o] = ys[i];
+        acc[0] += d0[o] * ys[i];
+        if constexpr(NDOT > 1) acc[NDOT - 1] += ys[i] * ys[i];
+      }
   }
   block_reduce_store<NDOT>(acc, partials, gridDim.x);
 }
@@ -406,7 +446,7 @@ void read_scalars(Context& c, Scalars& s)
 
 // Device state of one BiCGStab recurrence (Jacobi right preconditioning, dinv = D^-1) and the launches
 // that advance it; the scalars stay on the device and the host reads them once per batch.  With FP64
-// vectors and neq = 4 an iteration is five launches (dot products fused into the products, each
+// vectors and neq = 1 to 4 an iteration is five launches (dot products fused into the products, each
 // reduction completed by the kernel that consumes it); otherwise eleven (separate dot products and
 // one-workgroup finalisations).  Both sum in a fixed order: the results are deterministic.  rel selects
 // the residual-replacement stop test of the reliable phase.
@@ -429,7 +469,7 @@ struct Recurrence {
   {
     r = work; rh = r + n; p = rh + n; v = p + n; ph = v + n; s = ph + n; sh = s + n; t = sh + n;
     dinv = t + n;
-    fused = sizeof(R) == sizeof(double) && L.neq == 4;
+    fused = sizeof(R) == sizeof(double) && L.neq >= 1 && L.neq <= 4;
     Gv = red_grid(n);
     Gs = std::min<unsigned>(kFusedSpmvBlocks, grid_for(L.n_nodes, kRT / 16));
     part = c.red_partials.as<double>();
@@ -471,6 +511,23 @@ struct Recurrence {
     next = it0;
   }
 
+  // fused product y = A x with its partial dot products (see k_spmv4_dot / k_spmvn_dot)
+  template<int NDOT>
+  void spmv_dot(const uint32_t* rp, const uint32_t* cl, const VR* vl, const double* xx, double* yy, const double* d0,
+                double* partials)
+  {
+    switch(L.neq) {
+      case 4: hipLaunchKernelGGL((k_spmv4_dot<VR, NDOT>), dim3(Gs), dim3(kRT), 0, c.stream, L.n_nodes, rp, cl, vl, xx, yy,
+                                 d0, S, partials); break;
+      case 3: hipLaunchKernelGGL((k_spmvn_dot<VR, NDOT, 3>), dim3(Gs), dim3(kRT), 0, c.stream, L.n_nodes, rp, cl, vl, xx,
+                                 yy, d0, S, partials); break;
+      case 2: hipLaunchKernelGGL((k_spmvn_dot<VR, NDOT, 2>), dim3(Gs), dim3(kRT), 0, c.stream, L.n_nodes, rp, cl, vl, xx,
+                                 yy, d0, S, partials); break;
+      default: hipLaunchKernelGGL((k_spmvn_dot<VR, NDOT, 1>), dim3(Gs), dim3(kRT), 0, c.stream, L.n_nodes, rp, cl, vl,
+                                  xx, yy, d0, S, partials); break;
+    }
+  }
+
   // nb iterations; afterwards S holds the iteration count, the stop flag and the last residual norm
   void enqueue(int nb)
   {
@@ -480,11 +537,9 @@ struct Recurrence {
       const uint32_t* cl = L.cols.as<uint32_t>();
       for(int k = 0; k < nb; ++k, ++next) {
         hipLaunchKernelGGL(k_fb_p<R>, dim3(Gv), dim3(kRT), 0, c.stream, n, S, next, rel, 0, Gv, px, r, p, v, ph, dinv);
-        hipLaunchKernelGGL((k_spmv4_dot<VR, 1>), dim3(Gs), dim3(kRT), 0, c.stream, L.n_nodes, rp, cl, vl, ph, v, rh, S,
-                           pv);
+        spmv_dot<1>(rp, cl, vl, ph, v, rh, pv);
         hipLaunchKernelGGL(k_fb_s<R>, dim3(Gv), dim3(kRT), 0, c.stream, n, S, next, Gs, pv, r, v, s, sh, dinv, ps);
-        hipLaunchKernelGGL((k_spmv4_dot<VR, 2>), dim3(Gs), dim3(kRT), 0, c.stream, L.n_nodes, rp, cl, vl, sh, t, s, S,
-                           pt);
+        spmv_dot<2>(rp, cl, vl, sh, t, s, pt);
         hipLaunchKernelGGL(k_fb_x<R>, dim3(Gv), dim3(kRT), 0, c.stream, n, S, Gv, ps, Gs, pt, x, ph, sh, r, s, t, rh, px);
       }
       // close the batch: finalise its last iteration
