@@ -548,5 +548,62 @@ public:
   void update(lss::System& s, Fields& f) { check(cf3hip_ns_update(s.handle(), f.handle())); }
 };
 
+// NavierStokesSemiImplicit (ns_semi_implicit/implementation/NavierStokesSemiImplicit.cpp): a velocity
+// system (neq = dim) and a pressure system (neq = 1) of one mesh, with the options of the reference
+// ("theta" default 1, "nb_iterations" default 2, the ComputeTau options) and its actions.  The
+// solution strategies of the two systems set the solver of each solve.
+class NavierStokesSemiImplicit {
+public:
+  double theta = 1.;                 // NavierStokesSemiImplicit "theta"
+  int nb_iterations = 2;             // "nb_iterations" (InnerLoop passes)
+  std::string supg_type = "metric";
+  double dt = 0.;
+  double c1 = 1., c2 = 16., alpha_ps = 1., alpha_su = 1., alpha_bu = 1.;
+
+  NavierStokesSemiImplicit(lss::System& velocity, lss::System& pressure) : u_(velocity), p_(pressure)
+  {
+    check(cf3hip_si_create(velocity.handle(), pressure.handle(), &h_));
+  }
+  ~NavierStokesSemiImplicit() { if(h_) cf3hip_si_destroy(h_); }
+  NavierStokesSemiImplicit(const NavierStokesSemiImplicit&) = delete;
+  NavierStokesSemiImplicit& operator=(const NavierStokesSemiImplicit&) = delete;
+
+  cf3hip_ns_params params() const
+  {
+    NavierStokesAssembly a;
+    a.theta = theta; a.dt = dt; a.supg_type = supg_type; a.density = 1.;
+    a.c1 = c1; a.c2 = c2; a.alpha_ps = alpha_ps; a.alpha_su = alpha_su; a.alpha_bu = alpha_bu;
+    return a.params();
+  }
+  // the initial-condition actions: PressureAssembly, ZeroVelocitySystem, MassMatrixAssembly,
+  // SetupInnerLoopData
+  void initial(Fields& f)
+  {
+    const cf3hip_ns_params p = params();
+    check(cf3hip_si_initial(h_, f.handle(), &p));
+  }
+  // one time step; conditions are the new values at mesh nodes (velocity: nodes x dim, row-major).
+  // Returns whether every solve reached its tolerance.
+  bool step(Fields& f, const std::vector<uint32_t>& u_nodes, const std::vector<double>& u_values,
+            const std::vector<uint32_t>& p_nodes, const std::vector<double>& p_values)
+  {
+    const cf3hip_ns_params p = params();
+    stats.assign(2 * nb_iterations, cf3hip_solve_stats{});
+    const int rc = cf3hip_si_step(h_, f.handle(), &p, nb_iterations, &u_.solution_strategy().params,
+                                  &p_.solution_strategy().params, (uint32_t)u_nodes.size(), u_nodes.data(),
+                                  u_values.data(), (uint32_t)p_nodes.size(), p_nodes.data(), p_values.data(),
+                                  stats.data());
+    if(rc == CF3HIP_ERR_NOT_CONVERGED) return false;
+    check(rc);
+    return true;
+  }
+  std::vector<cf3hip_solve_stats> stats;   // velocity and pressure solve of each inner pass
+  cf3hip_si handle() const { return h_; }
+private:
+  lss::System& u_;
+  lss::System& p_;
+  cf3hip_si h_ = nullptr;
+};
+
 }  // namespace ufem
 }  // namespace cf3hip

@@ -341,6 +341,78 @@ static void periodic_system(Context& ctx)
   CHECK(throws<BadValue>([&] { lss::System t(ctx, mesh, 3, bad, all); }));
 }
 
+// NavierStokesSemiImplicit through the host layer: a 6 x 6 lid-driven cavity of quadrilaterals; the
+// pressure matrix and, after a step, the velocity matrix against the oracle's element actions; the
+// boundary velocities and the pinned pressure take their imposed values
+static void semi_implicit(Context& ctx)
+{
+  const uint32_t c = 6, N = (c + 1) * (c + 1), E = c * c;
+  std::vector<double> X((size_t)N * 2);
+  std::vector<uint32_t> conn((size_t)E * 4);
+  for(uint32_t j = 0; j <= c; ++j)
+    for(uint32_t i = 0; i <= c; ++i) { X[2 * (j * (c + 1) + i)] = double(i) / c; X[2 * (j * (c + 1) + i) + 1] = double(j) / c; }
+  for(uint32_t j = 0; j < c; ++j)
+    for(uint32_t i = 0; i < c; ++i) {
+      const uint32_t n0 = j * (c + 1) + i, e = j * c + i;
+      conn[4 * e] = n0; conn[4 * e + 1] = n0 + 1; conn[4 * e + 2] = n0 + c + 2; conn[4 * e + 3] = n0 + c + 1;
+    }
+  uint64_t st = 777;
+  std::vector<double> u((size_t)N * 2), p(N, 0.), nu(N, 1e-2);
+  for(double& x : u) x = orc_uniform(&st, -0.2, 0.2);
+  Mesh mesh(ctx, CF3HIP_QUAD2D, X, conn);
+  ufem::Fields f(mesh);
+  f.set(CF3HIP_FIELD_U, u);
+  f.set(CF3HIP_FIELD_P, p);
+  f.initialize(1e-2);
+  lss::System us(ctx, mesh, 2), ps(ctx, mesh, 1);
+  us.solution_strategy().params.tol = 1e-13;
+  ps.solution_strategy().params.tol = 1e-13;
+  ufem::NavierStokesSemiImplicit si(us, ps);
+  si.dt = 0.01;
+  si.initial(f);
+  std::vector<uint32_t> start(N + 1), cols(16 * N), slots((size_t)E * 16);
+  const int64_t nb = orc_build_sparsity(4, N, E, conn.data(), start.data(), cols.data(), (int64_t)cols.size(), slots.data());
+  CHECK(nb > 0);
+  orc_ns_params prm = {1., 0.01, 1., 0, ORC_SUPG_METRIC, 1., 16., 1., 1., 1.};
+  std::vector<double> pref(nb, 0.), pdev(nb);
+  CHECK(orc_si_pressure_matrix(ORC_QUAD2D, &prm, E, X.data(), conn.data(), u.data(), nu.data(), slots.data(),
+                               pref.data()) == 0);
+  check(cf3hip_matrix_get_values(ps.handle(), pdev.data()));
+  double vmax = 0., dmax = 0.;
+  for(int64_t q = 0; q < nb; ++q) { vmax = std::fmax(vmax, std::fabs(pref[q])); dmax = std::fmax(dmax, std::fabs(pdev[q] - pref[q])); }
+  CHECK(dmax <= 1e-12 * vmax);
+  // walls (no slip) and the lid (u = 1, taking the top corners); p = 0 at node 0
+  std::vector<uint32_t> un, pn = {0};
+  std::vector<double> uv, pv = {0.};
+  for(uint32_t n = 0; n < N; ++n) {
+    const uint32_t i = n % (c + 1), j = n / (c + 1);
+    if(j == c) { un.push_back(n); uv.push_back(1.); uv.push_back(0.); }
+    else if(i == 0 || i == c || j == 0) { un.push_back(n); uv.push_back(0.); uv.push_back(0.); }
+  }
+  CHECK(si.step(f, un, uv, pn, pv));
+  CHECK(si.stats.size() == 4 && si.stats[1].iterations > 0);
+  std::vector<double> unew, pnew, uadv;
+  f.get(CF3HIP_FIELD_U, unew, N, 2);
+  f.get(CF3HIP_FIELD_P, pnew, N, 1);
+  f.get(CF3HIP_FIELD_UADV, uadv, N, 2);
+  double bmax = 0.;
+  for(size_t b = 0; b < un.size(); ++b)
+    for(int d = 0; d < 2; ++d) bmax = std::fmax(bmax, std::fabs(unew[2 * un[b] + d] - uv[2 * b + d]));
+  CHECK(bmax <= 1e-12 && std::fabs(pnew[0]) <= 1e-12);
+  // the velocity matrix of the step, before its conditions, from the same advection velocity
+  const cf3hip_ns_params sip = si.params();
+  check(cf3hip_si_velocity_assemble(si.handle(), f.handle(), &sip));
+  std::vector<double> vref((size_t)nb * 4, 0.), vdev((size_t)nb * 4);
+  CHECK(orc_si_velocity_matrix(ORC_QUAD2D, &prm, E, X.data(), conn.data(), uadv.data(), nu.data(), slots.data(),
+                               vref.data()) == 0);
+  check(cf3hip_matrix_get_values(us.handle(), vdev.data()));
+  vmax = dmax = 0.;
+  for(size_t q = 0; q < vref.size(); ++q) { vmax = std::fmax(vmax, std::fabs(vref[q])); dmax = std::fmax(dmax, std::fabs(vdev[q] - vref[q])); }
+  CHECK(dmax <= 1e-12 * vmax);
+  si.theta = 2.;
+  CHECK(throws<SetupError>([&] { si.initial(f); }));
+}
+
 int main(int argc, char** argv)
 {
   if(argc > 1 && std::strcmp(argv[1], "--no-device") == 0) {
@@ -373,6 +445,9 @@ int main(int argc, char** argv)
     std::printf("periodic_system\n");
     std::fflush(stdout);
     periodic_system(ctx);
+    std::printf("semi_implicit\n");
+    std::fflush(stdout);
+    semi_implicit(ctx);
   }
   std::printf("%s: %d failed checks\n", g_fail ? "FAIL" : "OK", g_fail);
   return g_fail ? 1 : 0;
