@@ -17,6 +17,11 @@
 //   symmetric Dirichlet / set_row / apply ...... utest-lss-symmetric-dirichlet.cpp, utest-lss-vector.cpp
 //   BiCGStab / GMRES+ILU0 ...................... Trilinos arithmetic is third-party: "parity by
 //                                                specification", pinned by dense direct solves
+//   semi-implicit actions (orc_si_*) ........... lumped mass and pressure matrix: closed forms on the
+//                                                unit square; URHS, PRHS, ApplyAup and the velocity
+//                                                diagonal blocks: the implicit generic operator's blocks
+//                                                on periodic meshes (tests/test_semi_implicit.py); the
+//                                                off-diagonal one-point bulk blocks: symmetry only
 #include "oracle.h"
 
 #include <algorithm>

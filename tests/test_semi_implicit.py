@@ -340,3 +340,58 @@ def test_device_3d_follows_oracle(name):
         dsi.close()
         for x in (f, ps, us, mesh, ctx):
             x.close()
+
+
+def _implicit_blocks(case, p2m, uadv, nu, dt):
+    """Global blocks of the implicit generic NS element matrices (NavierStokesAssembly.hpp:44-66, pinned by
+    the implicit tests) on LSS rows: A(u,u), A(u,p), A(p,u), A(p,p), T(u,u), T(p,u), dense."""
+    o = pyoracle.get()
+    prm = pyoracle.ns_params(rho=1.0, dt=dt, theta=1.0, use_specializations=False)
+    rows = int(p2m.max()) + 1
+    d, k = case.dim, case.conn.shape[1]
+    A = np.zeros((rows * (d + 1), rows * (d + 1)))
+    T = np.zeros_like(A)
+    for e in range(case.conn.shape[0]):
+        c = case.conn[e]
+        Ae, Te, _, _ = o.ns_element(case.etype, prm, case.coords[c], uadv[c], uadv[c], nu[c])
+        # element var-major (component, node) -> global (component, row) blocks
+        gi = np.concatenate([comp * rows + p2m[c] for comp in range(d + 1)])
+        A[np.ix_(gi, gi)] += Ae
+        T[np.ix_(gi, gi)] += Te
+    U, P = slice(0, d * rows), slice(d * rows, (d + 1) * rows)
+    return A[U, U], A[U, P], A[P, U], A[P, P], T[U, U], T[P, U], rows
+
+
+@pytest.mark.parametrize("element", ["quad", "triag"])
+def test_oracle_actions_against_the_implicit_operator(element):
+    """On the periodic Q1 mesh (no boundary; every rule of order 2 in both expressions) the semi-implicit
+    element actions are the implicit operator's blocks: ApplyAup dp = theta A(u,p) dp, PRHS =
+    -(A(p,u)(U + dt da) + T(p,u)(a + da) + A(p,p) P), URHS = A(u,u) m + A(u,p) q - T(u,u) a with
+    m = dt (1 - theta) a - U and q = (1 - theta) dps - P, after the integration by parts of the pressure
+    gradient.  The implicit element matrices are pinned by their own tests, so a dropped, mis-signed or
+    mis-weighted term of an action shows here (the Taylor-Green bounds alone miss most such mistakes).
+    On P1 triangles the semi-implicit terms outside the mass part use the one-point rule; with a
+    uniform advection velocity their integrands are at most linear, so the identities hold there too."""
+    case, p2m, center = _setup(element, segments=8)
+    rng = np.random.default_rng(5)
+    n = case.n_nodes
+    uadv = rng.uniform(-1, 1, size=(n, 2)) if element == "quad" else np.tile(rng.uniform(-1, 1, size=2), (n, 1))
+    nu = np.full(n, 0.01)
+    dt, theta = 0.01, 0.5
+    Auu, Aup, Apu, App, Tuu, Tpu, rows = _implicit_blocks(case, p2m, uadv, nu, dt)
+    si = SemiImplicit(pyoracle.get(), case.etype, case.coords, case.conn, p2m, 0.01, dt, theta, 2, uadv,
+                      np.zeros(n))
+    si.uadv, si.u, si.nu = uadv.copy(), uadv.copy(), nu
+    # LSS vectors are node-major (row, component); the implicit blocks are component-major
+    cm = lambda v: v.reshape(rows, 2).T.ravel()
+    a, U, da = (rng.normal(size=2 * rows) for _ in range(3))
+    P, dps, dp = (rng.normal(size=rows) for _ in range(3))
+    got = cm(si.apply_aup(dp))
+    ref = theta * (Aup @ dp)
+    assert np.abs(got - ref).max() <= 1e-10 * np.abs(ref).max()
+    got = si.pressure_rhs(U, a, da, P)
+    ref = -(Apu @ cm(U + dt * da) + Tpu @ cm(a + da) + App @ P)
+    assert np.abs(got - ref).max() <= 1e-10 * np.abs(ref).max()
+    got = cm(si.velocity_rhs(a, U, dps, P))
+    ref = Auu @ cm(dt * (1 - theta) * a - U) + Aup @ ((1 - theta) * dps - P) - Tuu @ cm(a)
+    assert np.abs(got - ref).max() <= 1e-10 * np.abs(ref).max()
