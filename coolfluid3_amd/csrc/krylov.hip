@@ -199,8 +199,9 @@ __global__ __launch_bounds__(kRT) void k_spmv4_dot(uint32_t N, const uint32_t* _
   block_reduce_store<NDOT>(acc, partials, gridDim.x);
 }
 
-// The same for neq = 1, 2 or 3: 16 lanes per node row stride over its blocks, each lane keeps the NEQ
-// row sums of its blocks, and the 16 partial sums are combined in a fixed shuffle order.
+// The same for neq = 1, 2 or 3: LANES lanes per node row (8 for scalar rows, 16 otherwise) stride over
+// its blocks, each lane keeps the NEQ row sums of its blocks, and the partial sums are combined in a
+// fixed shuffle order.
 template<typename VR, int NDOT, int NEQ>
 __global__ __launch_bounds__(kRT) void k_spmvn_dot(uint32_t N, const uint32_t* __restrict__ row_ptr,
                                                    const uint32_t* __restrict__ cols, const VR* __restrict__ vals,
@@ -208,26 +209,23 @@ __global__ __launch_bounds__(kRT) void k_spmvn_dot(uint32_t N, const uint32_t* _
                                                    const double* __restrict__ d0, const double* __restrict__ S,
                                                    double* __restrict__ partials)
 {
+  constexpr int LANES = NEQ == 1 ? 8 : 16;
   if(S[S_STOP] != 0.) return;
   double acc[NDOT];
   for(int d = 0; d < NDOT; ++d) acc[d] = 0.;
-  const int l = threadIdx.x & 15;
-  const uint32_t step = gridDim.x * (kRT / 16);
-  for(uint32_t row = blockIdx.x * (kRT / 16) + (threadIdx.x >> 4); row < N; row += step) {
+  const int l = threadIdx.x & (LANES - 1);
+  const uint32_t step = gridDim.x * (kRT / LANES);
+  for(uint32_t row = blockIdx.x * (kRT / LANES) + (threadIdx.x / LANES); row < N; row += step) {
     double ys[NEQ];
     for(int i = 0; i < NEQ; ++i) ys[i] = 0.;
-    for(uint32_t q = row_ptr[row] + l; q < row_ptr[row + 1]; q += 16) {
+    for(uint32_t q = row_ptr[row] + l; q < row_ptr[row + 1]; q += LANES) {
       const VR* b = vals + (uint64_t)q * NEQ * NEQ;
       const double* xc = x + (uint64_t)cols[q] * NEQ;
       for(int i = 0; i < NEQ; ++i)
         for(int j = 0; j < NEQ; ++j) ys[i] += (double)b[i * NEQ + j] * xc[j];
     }
-    for(int i = 0; i < NEQ; ++i) {
-      ys[i] += __shfl_xor(ys[i], 8, 16);
-      ys[i] += __shfl_xor(ys[i], 4, 16);
-      ys[i] += __shfl_xor(ys[i], 2, 16);
-      ys[i] += __shfl_xor(ys[i], 1, 16);
-    }
+    for(int i = 0; i < NEQ; ++i)
+      for(int off = LANES / 2; off > 0; off >>= 1) ys[i] += __shfl_xor(ys[i], off, LANES);
     if(l == 0)
       for(int i = 0; i < NEQ; ++i) {
         const uint64_t o = (uint64_t)row * NEQ + i;
