@@ -300,8 +300,17 @@ def test_hex_body_force_and_tezduyar(oracle):
 
 
 @pytest.mark.parametrize("etype", [TRIAG2D, TETRA3D, HEXA3D])
-def test_combine_theta_and_rhs(oracle, etype):
-    """rhs = -A x + a with the undivided A; matrix = invdt T + theta A' (A' = A with p rows / theta)."""
+def test_combine_theta_invariants(oracle, etype):
+    """Invariants of the combine step (NavierStokesAssembly.hpp:69-71) over theta, with no restated formula:
+      * the rhs -A x + a is formed before the p rows are divided by theta: it does not depend on theta;
+      * the p rows invdt T_p + theta (A_p / theta) do not depend on theta;
+      * the velocity rows invdt T_u + theta A_u are affine in theta, and their slope S = A_u gives the
+        velocity rhs exactly: rhs_u = -S x (no body force), with x the element (u, p) in node-major order;
+      * the intercept invdt T_u is block diagonal in the velocity components (T[u_i, u_j] = 0 for i != j,
+        T[u_i, p] = 0), the same for every component, and sums to V / dt over one component's block
+        (sum_a N_a = 1, so the SUPG part of sum_ab (N_a + tau L_a) N_b integrates to 0);
+      * the slope is the var-major A of the element kernel moved to node-major order by the block
+        assignment row -> (row % k) neq + row / k (BlockAccumulator.hpp:134-184)."""
     rng = np.random.default_rng(etype + 100)
     k, dim = {TRIAG2D: (3, 2), TETRA3D: (4, 3), HEXA3D: (8, 3)}[etype]
     if etype == TRIAG2D:
@@ -312,17 +321,105 @@ def test_combine_theta_and_rhs(oracle, etype):
         X = (_LOC + 1) / 2 * 0.1 + rng.uniform(-0.005, 0.005, (8, 3))
     u = rng.uniform(-1, 1, (k, dim)); ua = rng.uniform(-1, 1, (k, dim))
     p = rng.uniform(-1, 1, k); nu = np.full(k, 0.01)
-    theta, dt = 0.6, 0.01
-    prm = ns_params(dt=dt, theta=theta)
-    A, T, a, _ = oracle.ns_element(etype, prm, X, u, ua, nu)
-    M, rhs = oracle.ns_element_system(etype, prm, X, u, p, ua, nu)
+    dt = 0.01
     neq = dim + 1
-    x = np.concatenate([u.T.ravel(), p])
+    thetas = (0.3, 0.6, 1.0)
+    sys_ = [oracle.ns_element_system(etype, ns_params(dt=dt, theta=t, use_specializations=etype != HEXA3D),
+                                     X, u, p, ua, nu) for t in thetas]
+    Ms = [m for m, _ in sys_]
+    rs = [r for _, r in sys_]
+    scale = np.abs(Ms[2]).max()
+    assert np.array_equal(rs[0], rs[1]) and np.array_equal(rs[0], rs[2])
+    prow = np.array([a * neq + dim for a in range(k)])
+    urow = np.array([a * neq + i for a in range(k) for i in range(dim)])
+    for m in Ms[:2]:
+        assert np.abs(m[prow] - Ms[2][prow]).max() <= 1e-14 * scale
+    S = (Ms[2][urow] - Ms[0][urow]) / (thetas[2] - thetas[0])
+    assert np.abs((Ms[1][urow] - Ms[0][urow]) - (thetas[1] - thetas[0]) * S).max() <= 1e-13 * scale
+    x = np.concatenate([np.concatenate([u[a], [p[a]]]) for a in range(k)])
+    assert np.abs(rs[0][urow] + S @ x).max() <= 1e-11 * (np.abs(S) @ np.abs(x)).max()
+    T = (Ms[0][urow] - thetas[0] * S) * dt          # T_u, rows urow, all columns (node-major)
+    if etype == HEXA3D:
+        # the Gauss-rule volume sum_q det J_q (exact for a trilinear map), from this file's own Q1 shapes
+        g1 = 1 / np.sqrt(3)
+        V = sum(np.linalg.det(q1(np.array([x, y, z]))[1] @ X) for x in (-g1, g1) for y in (-g1, g1) for z in (-g1, g1))
+    else:
+        V = oracle.element_volume(etype, X)
+    for i in range(dim):
+        ri = np.array([a * dim + i for a in range(k)])
+        for j in range(neq):
+            cj = np.array([b * neq + j for b in range(k)])
+            blk = T[np.ix_(ri, cj)]
+            if j == i:
+                assert blk.sum() == pytest.approx(V, rel=1e-10)
+                assert np.abs(blk - T[np.ix_(np.array([a * dim for a in range(k)]),
+                                              np.array([b * neq for b in range(k)]))]).max() <= 1e-10 * V
+            else:
+                assert np.abs(blk).max() <= 1e-10 * V
+    A, _, _, _ = oracle.ns_element(etype, ns_params(dt=dt, use_specializations=etype != HEXA3D), X, u, ua, nu)
     perm = np.array([(r % k) * neq + r // k for r in range(k * neq)])   # var-major -> node-major
-    rhs_vm = -A @ x + a
-    Ap = A.copy(); Ap[dim * k:] /= theta
-    M_vm = T / dt + theta * Ap
-    M_ref = np.zeros_like(M_vm); M_ref[np.ix_(perm, perm)] = M_vm
-    r_ref = np.zeros_like(rhs_vm); r_ref[perm] = rhs_vm
-    assert np.abs(M - M_ref).max() <= 1e-13 * np.abs(M_ref).max()
-    assert np.abs(rhs - r_ref).max() <= 1e-12 * (np.abs(A) @ np.abs(x)).max()
+    A_nm = np.zeros_like(A); A_nm[np.ix_(perm, perm)] = A
+    assert np.abs(S - A_nm[urow]).max() <= 1e-11 * np.abs(A).max()
+
+
+def _metric_tau(g, uc, nu, dt):
+    """tau_su and tau_bulk from an explicit metric g (SUPG.hpp:176-190, c1 = 1, c2 = 16)."""
+    adv = abs(uc @ g @ uc)
+    diff = nu ** 2 * (g ** 2).sum()
+    s = np.sqrt(4 / dt ** 2 + adv + 16 * diff)
+    return 1 / s, s / np.trace(g)
+
+
+@pytest.mark.parametrize("spec", [True, False])
+def test_tau_metric_sheared_tet(oracle, spec):
+    """METRIC tau on a sheared tetrahedron, nodes 0, (a,0,0), (s,b,0), (0,0,c): J (rows d x / d xi_i) is
+    [[a,0,0],[s,b,0],[0,0,c]], whose inverse is written out by hand below, and gij = J^-T J^-1
+    (SUPG.hpp:178).  The transposed product J^-1 J^-T has the same trace and Frobenius norm but gives
+    another u g u^T for a velocity in the sheared plane, so an operand swap fails here."""
+    a, b, c, s = 0.2, 0.1, 0.15, 0.12
+    X = np.array([[0, 0, 0], [a, 0, 0], [s, b, 0], [0, 0, c]])
+    Jinv = np.array([[1 / a, 0, 0], [-s / (a * b), 1 / b, 0], [0, 0, 1 / c]])
+    g = np.array([[1 / a ** 2 + s ** 2 / (a * b) ** 2, -s / (a * b ** 2), 0],
+                  [-s / (a * b ** 2), 1 / b ** 2, 0],
+                  [0, 0, 1 / c ** 2]])
+    assert np.allclose(Jinv.T @ Jinv, g, rtol=1e-14)   # the hand-written forms agree with each other
+    uc = np.array([0.8, -0.5, 0.3])
+    nu, dt = 0.01, 0.02
+    u = np.tile(uc, (4, 1))
+    _, _, _, tau = oracle.ns_element(TETRA3D, ns_params(dt=dt, use_specializations=spec), X, u, u, np.full(4, nu))
+    ts, tb = _metric_tau(g, uc, nu, dt)
+    assert tau[1] == pytest.approx(ts, rel=1e-13) and tau[0] == tau[1]
+    assert tau[2] == pytest.approx(tb, rel=1e-13)
+    gt = Jinv @ Jinv.T
+    assert abs(_metric_tau(gt, uc, nu, dt)[0] - ts) > 1e-6 * ts   # the transposed metric is told apart
+
+
+def test_tau_metric_sheared_tri(oracle):
+    """Sheared triangle 0, (a,0), (s,b): J = [[a,0],[s,b]], gij = J^-T J^-1 written out by hand."""
+    a, b, s = 0.2, 0.1, 0.13
+    X = np.array([[0, 0], [a, 0], [s, b]])
+    g = np.array([[1 / a ** 2 + s ** 2 / (a * b) ** 2, -s / (a * b ** 2)], [-s / (a * b ** 2), 1 / b ** 2]])
+    uc = np.array([0.7, -0.6])
+    nu, dt = 0.02, 0.01
+    u = np.tile(uc, (3, 1))
+    for spec in (True, False):
+        _, _, _, tau = oracle.ns_element(TRIAG2D, ns_params(dt=dt, use_specializations=spec), X, u, u, np.full(3, nu))
+        ts, tb = _metric_tau(g, uc, nu, dt)
+        assert tau[1] == pytest.approx(ts, rel=1e-13) and tau[2] == pytest.approx(tb, rel=1e-13)
+
+
+def test_tau_metric_sheared_hex(oracle):
+    """Parallelepiped x = c + B xi / 2 (xi in [-1,1]^3) with B = [[h,s,0],[0,h,0],[0,0,h]]: J = B^T / 2 at
+    the centre, so gij = J^-T J^-1 = 4 B^-1 B^-T, with B^-1 = [[1/h, -s/h^2, 0],[0,1/h,0],[0,0,1/h]]."""
+    h, s = 0.1, 0.04
+    B = np.array([[h, s, 0], [0, h, 0], [0, 0, h]])
+    X = 0.5 * _LOC @ B.T + np.array([0.3, 0.2, 0.1])
+    Binv = np.array([[1 / h, -s / h ** 2, 0], [0, 1 / h, 0], [0, 0, 1 / h]])
+    g = 4 * Binv @ Binv.T
+    uc = np.array([0.4, 0.9, -0.2])
+    nu, dt = 0.005, 0.01
+    u = np.tile(uc, (8, 1))
+    _, _, _, tau = oracle.ns_element(HEXA3D, ns_params(dt=dt, use_specializations=False), X, u, u, np.full(8, nu))
+    ts, tb = _metric_tau(g, uc, nu, dt)
+    assert tau[1] == pytest.approx(ts, rel=1e-13) and tau[2] == pytest.approx(tb, rel=1e-13)
+    assert abs(_metric_tau(4 * Binv.T @ Binv, uc, nu, dt)[0] - ts) > 1e-6 * ts

@@ -232,8 +232,17 @@ def test_ilu0_exact_on_tridiagonal(oracle):
 def test_node_updates(oracle):
     rng = np.random.default_rng(8)
     u, u1, u2, u3 = (rng.uniform(-1, 1, (7, 3)) for _ in range(4))
-    ua = oracle.linearize_u(u, u1, u2, u3)
-    assert np.allclose(ua, 2.1875 * u - 2.1875 * u1 + 1.3125 * u2 - 0.3125 * u3, rtol=1e-15)
+    # LinearizeU (NavierStokes.cpp:105) extrapolates the last four states, a unit time apart, to half a
+    # step ahead: exact for every cubic in time and for no quartic.  Four weights at the abscissae
+    # 0, -1, -2, -3 are fixed by exactness on 1, t, t^2, t^3, so this pins them and the argument order.
+    t = np.array([0.0, -1.0, -2.0, -3.0])
+    c = rng.uniform(-1, 1, (4, 7, 3))            # per entry: f(t) = c0 + c1 t + c2 t^2 + c3 t^3
+    f = [c[0] + c[1] * tk + c[2] * tk ** 2 + c[3] * tk ** 3 for tk in t]
+    ua = oracle.linearize_u(*f)
+    assert np.allclose(ua, c[0] + c[1] * 0.5 + c[2] * 0.25 + c[3] * 0.125, rtol=1e-13, atol=1e-13)
+    # t^4: the extrapolation error is -(1/2)(3/2)(5/2)(7/2) = -105/16
+    q = [np.full((7, 3), tk ** 4) for tk in t]
+    assert np.allclose(oracle.linearize_u(*q), 0.5 ** 4 - 105.0 / 16.0, rtol=1e-14)
     # first step: u1 = u2 = u3 = u gives u_adv = u
     assert np.allclose(oracle.linearize_u(u, u, u, u), u, rtol=1e-15, atol=1e-15)
     p = rng.uniform(-1, 1, 7); sol = rng.uniform(-1, 1, 28)
