@@ -1,19 +1,19 @@
 """Benchmark of the UFEM Navier-Stokes hot path on one MI355X.
 
-A step is one Assembly action of the reference (ZeroLSS + element assembly + scatter into the
-block-CSR system, the quantity timed by plugins/UFEM/test/ptest-navier-stokes-assembly.py) over the
-synthetic 64^3-hexahedra cube split in 6 P1 tetrahedra (1,572,864 elements, FP64 drop-in), with all
-inputs resident in HBM.  value = elements assembled per second (whole job).
+A step is one Assembly action of the reference (ZeroLSS + element assembly + scatter into the block-CSR
+system, the quantity timed by plugins/UFEM/test/ptest-navier-stokes-assembly.py) over the 128^3-hexahedra
+unit cube split in 6 P1 tetrahedra with 1% skewed elements (12,582,912 elements, 8,586,756 dofs), FP64
+values, all inputs resident in HBM.  value = elements assembled per second (whole job).
 
-Also reported (outside the timed steps): the dominant kernel's roofline (HIP-event duration of the
-assembly kernel against its algorithmic bytes), the FP64 SpMV bandwidth, the full implicit step of
-this configuration (FP64 assembly, BCs, BiCGStab/Jacobi to a 1e-10 relative residual), the CPU
-baseline (the oracle restatement, OpenMP over the host cores, bounded sample), and the other
-workloads: the precision form on the 128^3 tetrahedral channel with 1% skewed elements (FP32 element
-matrices against FP64, both to 1e-8), the 160^3 Q1 hexahedral lid-driven cavity (assembly and Picard
-iterations in both forms), the 32x32 triangle cavity step (GMRES + ILU0), the reference's
-Taylor-Green acceptance run (periodic, quadrilaterals and triangles, errors against the exact vortex),
-and the serving loop (continuous batching against one-at-a-time requests, p50/p99 latency).
+Also reported: the dominant kernel's roofline (HIP-event duration of the assembly kernel against its
+algorithmic bytes, and the HBM traffic per launch from the committed rocprofv3 counter summary), the FP32
+assembly of the same mesh, the SpMV bandwidth, the time to a 1e-8 relative residual per implicit step of
+the FP64 path and of the precision form (FP32 element matrices, FP64 refinement), the CPU baseline (the
+oracle restatement on the host cores: the same assembly, and the implicit steps of the 32x32 triangle
+cavity and of the 64^3 tetrahedral cube), and the other workloads: the FP64 drop-in on the 64^3 cube
+(assembly and BiCGStab/Jacobi step to 1e-10), the 160^3 Q1 hexahedral lid-driven cavity (assembly and
+Picard iterations in both forms), the 32x32 triangle cavity step (GMRES + ILU0), the reference's
+Taylor-Green acceptance run, the semi-implicit solver, and the serving loop (p50/p99 latency).
 """
 import argparse
 import json
@@ -41,30 +41,113 @@ def algorithmic_spmv_bytes(n_nodes, dim, n_blocks, value_bytes):
     return n_blocks * neq * neq * value_bytes + n_blocks * 4 + (n_nodes + 1) * 4 + 2 * n_nodes * neq * value_bytes
 
 
-def cpu_baseline(case, budget_s=20.0):
-    """Oracle restatement on the host cores (the test oracle library, bounded sample)."""
+def host_cpu():
+    """Threads for the host baselines: the CPU share of this process (its affinity mask, capped by
+    OMP_NUM_THREADS when that is set, as on a shared GPU node), the processor model and the machine's
+    logical CPU count."""
+    try:
+        n = len(os.sched_getaffinity(0))
+    except AttributeError:
+        n = os.cpu_count() or 1
+    env = os.environ.get("OMP_NUM_THREADS", "")
+    if env.isdigit() and int(env) > 0:
+        n = min(n, int(env))
+    model = ""
+    try:
+        with open("/proc/cpuinfo") as f:
+            for line in f:
+                if line.startswith("model name"):
+                    model = line.split(":", 1)[1].strip()
+                    break
+    except OSError:
+        pass
+    return n, model, os.cpu_count()
+
+
+def cpu_baseline(case, threads, model, budget_s=20.0):
+    """Oracle restatement of the same FP64 assembly on the host cores (element colouring, OpenMP), plus
+    one thread (what one reference MPI rank does) on the first eighth of the elements."""
     from oracle import pyoracle
     o = pyoracle.get()
     start, cols, slots = o.build_sparsity(case.conn, case.n_nodes)
     prm = pyoracle.ns_params(dt=case.dt)
     nu = np.full(case.n_nodes, case.nu)
-    cores = min(os.cpu_count() or 1, 16)
     reps, t_used = 0, 0.0
-    while t_used < budget_s and reps < 5:
+    while t_used < budget_s / 2 and reps < 5:
         t0 = time.perf_counter()
         o.ns_assemble(pyoracle.TETRA3D, prm, case.coords, case.conn, case.u0, case.p0, case.u0, nu, start, cols,
-                      slots, nthreads=cores)
+                      slots, nthreads=threads)
         t_used += time.perf_counter() - t0
         reps += 1
-        if t_used > budget_s / 2:
-            break
+    e1 = case.n_elems // 8
     t1 = time.perf_counter()
-    o.ns_assemble(pyoracle.TETRA3D, prm, case.coords, case.conn, case.u0, case.p0, case.u0, nu, start, cols, slots,
-                  nthreads=1)
-    single = case.n_elems / (time.perf_counter() - t1) / 1e6
-    return {"value": case.n_elems * reps / t_used / 1e6, "unit": "Melem/s", "cores": cores, "kind": "port",
-            "sample": "%d full assemblies of the same mesh (oracle restatement, element colouring, OpenMP)" % reps,
+    o.ns_assemble(pyoracle.TETRA3D, prm, case.coords, case.conn[:e1], case.u0, case.p0, case.u0, nu, start, cols,
+                  slots[:e1 * 16], nthreads=1)
+    single = e1 / (time.perf_counter() - t1) / 1e6
+    return {"value": case.n_elems * reps / t_used / 1e6, "unit": "Melem/s", "cores": threads, "kind": "port",
+            "cpu_model": model,
+            "sample": "%d full FP64 assemblies of the same %d-element mesh (oracle restatement, element colouring, "
+                      "OpenMP on %d threads); one thread: the first %d elements" % (reps, case.n_elems, threads, e1),
             "single_thread_value": single}
+
+
+def cpu_steps(threads, gpu_iterations=None):
+    """The oracle's implicit step (assembly, Dirichlet conditions, Krylov solve, all FP64) of the 32x32
+    triangle cavity (GMRES(30) + ILU0 to 1e-10, one thread: the ILU0 solves are sequential) and of the
+    64^3 tetrahedral cube (BiCGStab + Jacobi to 1e-10, assembly and Krylov kernels on `threads`)."""
+    from coolfluid3_amd import cases
+    from oracle import pyoracle
+    o = pyoracle.get()
+    out = {}
+    for key, case, nthr in (("cavity_2d_32", cases.lid_cavity_2d(32), 1), ("cube_tets_64", cases.inflow_cube_tets(64),
+                                                                              threads)):
+        et = pyoracle.TRIAG2D if case.dim == 2 else pyoracle.TETRA3D
+        neq = case.dim + 1
+        start, cols, slots = o.build_sparsity(case.conn, case.n_nodes)
+        prm = pyoracle.ns_params(dt=case.dt)
+        nu = np.full(case.n_nodes, case.nu)
+        o.set_krylov_threads(nthr)
+        t0 = time.perf_counter()
+        vals, rhs = o.ns_assemble(et, prm, case.coords, case.conn, case.u0, case.p0, case.u0, nu, start, cols, slots,
+                                  nthreads=nthr)
+        t1 = time.perf_counter()
+        n, e, v = cases.bc_lists(case, case.u0, case.p0)
+        sol = np.zeros_like(rhs)
+        o.dirichlet(neq, start, cols, vals, rhs, sol, n, e, v)
+        t2 = time.perf_counter()
+        if case.dim == 2:
+            x, st = o.gmres_ilu0(neq, start, cols, vals, rhs, sol, restart=30, tol=1e-10)
+            method = "GMRES(30) + ILU0"
+        else:
+            x, st = o.bicgstab(neq, start, cols, vals, rhs, sol, tol=1e-10, maxit=3000)
+            method = "BiCGStab + Jacobi"
+        t3 = time.perf_counter()
+        o.set_krylov_threads(1)
+        out[key] = {"step_ms": (t3 - t0) * 1e3, "assembly_ms": (t1 - t0) * 1e3, "dirichlet_ms": (t2 - t1) * 1e3,
+                    "solve_ms": (t3 - t2) * 1e3, "method": method, "iterations": int(st.iterations),
+                    "rel_residual": float(st.rel_residual), "converged": bool(st.converged), "threads": nthr,
+                    "elements": case.n_elems, "dofs": neq * case.n_nodes}
+        print("bench: cpu %s step: %.1f ms (%s %d it, %d threads)" % (key, out[key]["step_ms"], method,
+                                                                     st.iterations, nthr), file=sys.stderr, flush=True)
+    return out
+
+
+TRAFFIC_RE = r"traffic_per_launch_bytes\s+(\S+)\s*=\s*([0-9]+)"
+
+
+def profiled_traffic(kernel_key, profile="assembly_tet128.md"):
+    """HBM bytes per launch of a kernel from the committed rocprofv3 counter summary (FETCH_SIZE and
+    WRITE_SIZE scaled by a copy kernel of known size), or None."""
+    import re
+    path = os.path.join(ROOT, "profiles", profile)
+    try:
+        with open(path) as f:
+            for m in re.finditer(TRAFFIC_RE, f.read()):
+                if m.group(1) == kernel_key:
+                    return int(m.group(2))
+    except OSError:
+        pass
+    return None
 
 
 def timed_step(ctx, gc, sp, picard=False, label=""):
@@ -318,17 +401,57 @@ def serving_workload(count, max_active, mode="both"):
     return out
 
 
+def dropin_workload(ctx, cells=64, reps=10, solve=True):
+    """The FP64 drop-in on the inflow cube (64^3 x 6 tetrahedra): assembly kernel rate, SpMV bandwidth
+    and the implicit step (FP64 assembly, conditions, BiCGStab/Jacobi to 1e-10)."""
+    from coolfluid3_amd import cabi, cases
+    case = cases.inflow_cube_tets(cells)
+    g = cases.GpuCase(ctx, case)
+    g.fields.linearize()
+    s = g.system
+    res = {"workload": "unit cube %d^3 hexes x 6 P1 tets (%d elements, %d dofs), FP64 drop-in, nu=1e-3 dt=5e-3, "
+                       "no-slip walls and parabolic inflow" % (cells, case.n_elems, s.dofs)}
+    g.assemble(cabi.FP64)
+    ctx.synchronize()
+    ctx.enable_timing(True)
+    for _ in range(reps):
+        g.assemble(cabi.FP64)
+    ctx.synchronize()
+    ms, n = ctx.kernel_time(cabi.KERNEL_ASSEMBLY)
+    kms = ms / max(n, 1)
+    alg = algorithmic_assembly_bytes(case.n_nodes, case.n_elems, 4, 3, s.n_blocks, 8)
+    res["assembly_fp64"] = {"kernel_ms": kms, "Melem_s": case.n_elems / kms / 1e3, "GBps": alg / kms / 1e6,
+                            "hbm_frac": alg / kms / 1e6 / HBM_PEAK_GBS, "algorithmic_bytes": alg}
+    ctx.enable_timing(True)
+    for _ in range(reps):
+        s.apply_device(1.0, s.vector_ptr(cabi.SOLUTION), 0.0, s.vector_ptr(cabi.RHS))
+    ctx.synchronize()
+    ms, n = ctx.kernel_time(cabi.KERNEL_SPMV)
+    sms = ms / max(n, 1)
+    sb = algorithmic_spmv_bytes(case.n_nodes, 3, s.n_blocks, 8)
+    res["spmv_fp64"] = {"kernel_ms": sms, "GBps": sb / sms / 1e6, "hbm_frac": sb / sms / 1e6 / HBM_PEAK_GBS}
+    ctx.enable_timing(False)
+    g.close()
+    if solve:
+        gc = cases.GpuCase(ctx, case)
+        res["step_fp64"] = timed_step(ctx, gc, cabi.solver_params(tol=1e-10), label="drop-in 64^3 FP64 step")
+        gc.close()
+    return res
+
+
 def main():
     ap = argparse.ArgumentParser()
     ap.add_argument("--gpus", type=int, default=1)
     ap.add_argument("--steps", type=int, default=20)
     ap.add_argument("--warmup", type=int, default=3)
-    ap.add_argument("--cells", type=int, default=64, help="cube cells per side (64 = the drop-in-at-scale case)")
+    ap.add_argument("--cells", type=int, default=128, help="cube cells per side of the headline workload "
+                                                           "(128 = the precision-form tetrahedral cube)")
     ap.add_argument("--no-cpu-baseline", action="store_true")
     ap.add_argument("--no-solve", action="store_true")
-    ap.add_argument("--no-workloads", action="store_true", help="skip the 128^3 / 160^3 / 2D / serving workloads")
-    ap.add_argument("--workloads", default="cavity_2d,taylor_green,semi_implicit,precision_tets,hexa_cavity,serving",
-                    help="comma-separated subset of the extra workloads")
+    ap.add_argument("--no-workloads", action="store_true", help="skip the 64^3 / 128^3-steps / 160^3 / 2D / serving "
+                                                               "workloads")
+    ap.add_argument("--workloads", default="dropin_64,cavity_2d,taylor_green,semi_implicit,precision_tets,hexa_cavity,"
+                                           "serving", help="comma-separated subset of the extra workloads")
     ap.add_argument("--precision-cells", type=int, default=128)
     ap.add_argument("--hexa-cells", type=int, default=160)
     ap.add_argument("--picard", type=int, default=2)
@@ -343,12 +466,15 @@ def main():
 
     from coolfluid3_amd import cabi, cases
 
-    case = cases.inflow_cube_tets(args.cells)
+    # the precision-form tetrahedral cube: 1% skewed elements, channel inflow x = 1 / outlet x = 0
+    case = cases.inflow_cube_tets(args.cells, nu=5e-4, dt=5e-3, skew=True, outlet=True)
     ctx = cabi.Context(0)
     g = cases.GpuCase(ctx, case, precision=cabi.FP32)
     g.fields.linearize()
     s = g.system
     E, N, nb = case.n_elems, case.n_nodes, s.n_blocks
+    print("bench: headline workload %d elements, %d dofs, %d node blocks" % (E, s.dofs, nb), file=sys.stderr,
+          flush=True)
     ctx.set_async(True)
     for _ in range(args.warmup):
         g.assemble(cabi.FP64)
@@ -374,10 +500,7 @@ def main():
     a32_kern = a32_ms / max(a32_n, 1)
     alg32 = algorithmic_assembly_bytes(N, E, 4, 3, nb, 4)
 
-    # SpMV bandwidth (FP64 and FP32 values)
-    g.assemble(cabi.FP64)
-    g.assemble(cabi.FP32)
-    ctx.synchronize()
+    # SpMV bandwidth (FP64 values) on the same system
     ctx.enable_timing(True)
     x = s.vector_ptr(cabi.SOLUTION)
     y = s.vector_ptr(cabi.RHS)
@@ -394,59 +517,71 @@ def main():
     ctx.synchronize()
     ctx.set_async(False)
     ctx.enable_timing(False)
+    g.close()
 
     extra = {
         "elements": E, "nodes": N, "dofs": s.dofs, "node_blocks": nb,
         "assembly_kernel_ms": kern_ms, "assembly_fp32_kernel_ms": a32_kern,
         "assembly_fp32_GBps": alg32 / (a32_kern * 1e-3) / 1e9,
-        "assembly_fp32_Melem_s": E / a32_kern / 1e3,
+        "assembly_fp32_hbm_frac": alg32 / (a32_kern * 1e-3) / 1e9 / HBM_PEAK_GBS,
         "spmv_fp64_ms": sp_kern, "spmv_fp64_GBps": sp_bytes / (sp_kern * 1e-3) / 1e9,
         "dofs_per_s_assembly": s.dofs / (ms_per_step * 1e-3),
     }
-
-    if not args.no_solve:
-        # the full implicit step of this configuration: FP64 assembly + BCs + BiCGStab/Jacobi to 1e-10
-        gc = cases.GpuCase(ctx, case)
-        ctx.synchronize()
-        t0 = time.perf_counter()
-        st = gc.step(cabi.solver_params(tol=1e-10), raise_on_failure=False)
-        ctx.synchronize()
-        extra["step_fp64"] = {"ms": (time.perf_counter() - t0) * 1e3, "iterations": st.iterations,
-                              "spmvs": st.spmvs, "rel_residual": st.rel_residual, "tol": 1e-10,
-                              "converged": bool(st.converged)}
-        gc.close()
-
+    traffic = profiled_traffic("k_assemble_fp64_tet%d" % args.cells)
     out = {
         "metric": METRIC, "value": E / (ms_per_step * 1e-3) / 1e6, "unit": "Melem/s", "n_gpus": 1,
         "steps": args.steps, "warmup": args.warmup, "ms_per_step": ms_per_step, "higher_is_better": True,
         "scaling": "weak", "vs_baseline": None, "dtype": "f64", "data": "synthetic",
-        "config": {"workload": "UFEM NS assembly+scatter, unit cube %d^3 hexes x 6 P1 tets (%d elements, %d dofs), "
-                               "FP64 drop-in, nu=1e-3 dt=5e-3, seed 20261202" % (args.cells, E, s.dofs),
+        "config": {"workload": "UFEM NS assembly+scatter, unit cube %d^3 hexes x 6 P1 tets, 1%% skewed (aspect "
+                               "ratio ~20) (%d elements, %d dofs), FP64 values, channel inflow x=1 / outlet x=0, "
+                               "nu=5e-4 dt=5e-3, seed 20261202" % (args.cells, E, s.dofs),
                    "elements": E, "element_type": "P1 tetrahedron (specialised kernel)"},
+        "fp32_assembly_Melem_s": E / a32_kern / 1e3,
         "roofline": {"bound": "hbm", "achieved": achieved, "peak": HBM_PEAK_GBS, "unit": "GB/s",
-                     "frac": achieved / HBM_PEAK_GBS, "traffic": None,
+                     "frac": achieved / HBM_PEAK_GBS, "traffic": traffic,
                      "kernel": "k_assemble<SpecP1<3,double>>", "algorithmic_bytes": alg,
                      "kernel_ms": kern_ms},
         "extra": extra,
     }
-    g.close()
+    threads, model, ncpu = host_cpu()
     if not args.no_workloads:
-        for key, fn in (("cavity_2d", lambda: cavity2d_workload(ctx)),
+        wl = args.workloads.split(",")
+        for key, fn in (("dropin_64", lambda: dropin_workload(ctx, solve=not args.no_solve)),
+                        ("precision_tets", lambda: precision_workload(ctx, args.precision_cells)),
+                        ("cavity_2d", lambda: cavity2d_workload(ctx)),
                         ("taylor_green", lambda: {e: taylor_green_workload(ctx, e) for e in ("quad", "triag")}),
                         ("semi_implicit", lambda: semi_implicit_workload(ctx, with_cpu=not args.no_cpu_baseline)),
-                        ("precision_tets", lambda: precision_workload(ctx, args.precision_cells)),
                         ("hexa_cavity", lambda: hexa_workload(ctx, args.hexa_cells, args.picard)),
-                        ("serving", lambda: serving_workload(args.serving_requests, args.serving_slots, args.serving_mode))):
-            if key not in args.workloads.split(","):
+                        ("serving", lambda: serving_workload(args.serving_requests, args.serving_slots,
+                                                             args.serving_mode))):
+            if key not in wl:
                 continue
             t0 = time.perf_counter()
             extra[key] = fn()
             extra[key]["wall_s"] = time.perf_counter() - t0
             print("bench: %s done in %.1f s: %s" % (key, extra[key]["wall_s"], json.dumps(extra[key])),
                   file=sys.stderr, flush=True)
+        pt = extra.get("precision_tets")
+        if pt:
+            f64, f32 = pt["fp64_steps"], pt["fp32_refined_steps"]
+            out["time_to_1e-8"] = {
+                "workload": pt["workload"],
+                "fp64_ms_per_step": [st["ms"] for st in f64], "fp32_refined_ms_per_step": [st["ms"] for st in f32],
+                "fp64_iterations": [st["iterations"] for st in f64],
+                "fp32_refined_iterations": [st["iterations"] for st in f32],
+                "speedup_steps_2_on": pt["speedup_steps_2_to_%d" % len(f64)],
+                "fp32_vs_fp64_velocity_rel_error": pt["velocity_rel_error_vs_fp64"]}
+        sv = extra.get("serving")
+        if sv and "continuous_batching" in sv:
+            out["serving"] = {k: sv["continuous_batching"][k] for k in ("p50_latency_s", "p99_latency_s",
+                                                                          "throughput_req_s")}
+            if "throughput_ratio" in sv:
+                out["serving"]["throughput_ratio_vs_sequential"] = sv["throughput_ratio"]
     ctx.close()
     if not args.no_cpu_baseline:
-        out["cpu_baseline"] = cpu_baseline(case)
+        out["cpu_baseline"] = cpu_baseline(case, threads, model)
+        out["cpu_baseline"]["logical_cpus_of_the_machine"] = ncpu
+        extra["cpu_steps"] = cpu_steps(threads)
     print(json.dumps(out))
 
 

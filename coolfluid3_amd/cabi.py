@@ -112,6 +112,8 @@ def lib():
         "cf3hip_vector_device_ptr": [_vp, C.c_int, C.POINTER(_vp)],
         "cf3hip_vector_assign": [_vp, C.c_int, C.c_int],
         "cf3hip_lss_dirichlet": [_vp, C.c_uint32, _u32, _u32, _d],
+        "cf3hip_matrix_tie_blockrow_pairs": [_vp, C.c_uint32, _u32, _u32],
+        "cf3hip_lss_periodicity": [_vp, C.c_uint32, _u32, _u32],
         "cf3hip_lss_solve": [_vp, C.POINTER(SolverParams), C.POINTER(SolveStats)],
         "cf3hip_lss_residual": [_vp, _d],
         "cf3hip_fields_create": [_vp, C.POINTER(_vp)],
@@ -404,6 +406,21 @@ class System:
         n, e, v = _u32a(nodes), _u32a(eqs), _f64(values)
         check(lib().cf3hip_lss_dirichlet(self.h, len(n), _p(n, _u32), _p(e, _u32), _p(v)))
 
+    def tie_blockrow_pairs(self, to, frm):
+        """Matrix::tie_blockrow_pairs for the (to[i], frm[i]) block-row pairs in order."""
+        t, f = _u32a(to), _u32a(frm)
+        assert len(t) == len(f)
+        check(lib().cf3hip_matrix_tie_blockrow_pairs(self.h, len(t), _p(t, _u32), _p(f, _u32)))
+
+    def periodicity(self, to, frm):
+        """System::periodicity for the (to[i], frm[i]) block-row pairs in order (matrix, rhs, solution)."""
+        t, f = _u32a(to), _u32a(frm)
+        assert len(t) == len(f)
+        check(lib().cf3hip_lss_periodicity(self.h, len(t), _p(t, _u32), _p(f, _u32)))
+
+    def set_vector_value(self, which, blockrow, ieq, v):
+        check(lib().cf3hip_vector_set_value(self.h, which, blockrow, ieq, v))
+
     def solve(self, params, raise_on_failure=True):
         st = SolveStats()
         code = lib().cf3hip_lss_solve(self.h, C.byref(params), C.byref(st))
@@ -415,6 +432,7 @@ class System:
         r = C.c_double()
         check(lib().cf3hip_lss_residual(self.h, C.byref(r)))
         return r.value
+
 
     def assemble(self, fields, prm):
         check(lib().cf3hip_ns_assemble(self.h, fields.h, C.byref(prm)))

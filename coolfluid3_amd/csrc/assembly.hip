@@ -78,6 +78,25 @@ struct SpecP1 {
     }
   }
 
+  // Tetra3D::volume (cf3/mesh/LagrangeP1/Tetra3D.cpp:104-128, 205-208), the volume the specialised
+  // element terms use (NavierStokesSpecializations.hpp:156): the 24-term expansion of det J in absolute
+  // coordinates, with the reference's products, sums and final division in their source order and no
+  // contraction into FMAs.  Its rounding error grows like |x|^3 / V (about 1e-11 relative at 64^3 cells
+  // of the unit cube): computed any other way, the matrix would differ from the reference's by that much.
+  __device__ __forceinline__ static double tet_volume_ref(const double X[4][3])
+  {
+#pragma clang fp contract(off)
+    const double x0 = X[0][0], y0 = X[0][1], z0 = X[0][2];
+    const double x1 = X[1][0], y1 = X[1][1], z1 = X[1][2];
+    const double x2 = X[2][0], y2 = X[2][1], z2 = X[2][2];
+    const double x3 = X[3][0], y3 = X[3][1], z3 = X[3][2];
+    return (x2*y1*z0 - x3*y1*z0 - x1*y2*z0 + x3*y2*z0 + x1*y3*z0 -
+            x2*y3*z0 - x2*y0*z1 + x3*y0*z1 + x0*y2*z1 - x3*y2*z1 -
+            x0*y3*z1 + x2*y3*z1 + x1*y0*z2 - x3*y0*z2 - x0*y1*z2 +
+            x3*y1*z2 + x0*y3*z2 - x1*y3*z2 - x1*y0*z3 + x2*y0*z3 +
+            x0*y1*z3 - x2*y1*z3 - x0*y2*z3 + x1*y2*z3) / 6.;
+  }
+
   // METRIC tau from the velocity at the centroid (SUPG.hpp:176-190); gij = n_i.n_l / (D^2 V^2).
   // s = 1 / (D V).  c1 = 1, c2 = 16, alpha = 1 on the specialised path.
   template<typename T>
@@ -173,6 +192,13 @@ struct SpecP1 {
         for(int d = 0; d < D; ++d) n[m][d] = (R)n64[m][d];
       V = (R)V64; invV = (R)iv; tsu = (R)ts; tbu = (R)tb;
       if(!(V64 > 0.)) g.status[0] = 1;   // every writer stores the same flag value
+      if constexpr(D == 3 && sizeof(R) == sizeof(double)) {
+        // the element terms take the reference's volume; tau above takes det J of the Jacobian, as the
+        // reference's ComputeTau does (SUPG.hpp:178, ElementData.hpp:302-314)
+        const double Vr = tet_volume_ref(X);
+        V = Vr;
+        invV = 1. / Vr;
+      }
     } else {
       R ucr[D];
       for(int d = 0; d < D; ++d) ucr[d] = (R)uc[d];

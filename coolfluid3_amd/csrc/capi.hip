@@ -888,6 +888,24 @@ int cf3hip_lss_dirichlet(cf3hip_lss lss, uint32_t nbc, const uint32_t* nodes, co
   API_END
 }
 
+int cf3hip_matrix_tie_blockrow_pairs(cf3hip_lss lss, uint32_t npairs, const uint32_t* to, const uint32_t* from)
+{
+  API_BEGIN
+  REQUIRE(lss, CF3HIP_ERR_NOT_CREATED, "system not created");
+  REQUIRE(npairs == 0 || (to && from), CF3HIP_ERR_INVALID_ARG, "null argument");
+  lss_tie(lss->l, npairs, to, from, false);
+  API_END
+}
+
+int cf3hip_lss_periodicity(cf3hip_lss lss, uint32_t npairs, const uint32_t* to, const uint32_t* from)
+{
+  API_BEGIN
+  REQUIRE(lss, CF3HIP_ERR_NOT_CREATED, "system not created");
+  REQUIRE(npairs == 0 || (to && from), CF3HIP_ERR_INVALID_ARG, "null argument");
+  lss_tie(lss->l, npairs, to, from, true);
+  API_END
+}
+
 int cf3hip_lss_solve(cf3hip_lss lss, const cf3hip_solver_params* prm, cf3hip_solve_stats* stats)
 {
   API_BEGIN
@@ -1220,18 +1238,23 @@ void si_conditions(SemiImplicit& S, const std::vector<const double*>& field, uin
                    std::vector<double>& inc)
 {
   const int nc = (int)field.size();
-  std::vector<uint32_t> p2m(S.mesh->n_nodes);
-  HIP_OK(hipMemcpy(p2m.data(), S.p2m.ptr, sizeof(uint32_t) * p2m.size(), hipMemcpyDeviceToHost));
-  for(uint32_t b = 0; b < n; ++b) {
+  if(n == 0) return;
+  const std::vector<uint32_t>& p2m = S.p2m_host;
+  for(uint32_t b = 0; b < n; ++b)
     REQUIRE(nodes[b] < S.mesh->n_nodes, CF3HIP_ERR_INVALID_ARG, "boundary node out of range");
+  // one bulk copy per field component, then the old values are read on the host (conditions in
+  // node order, the components of a node consecutive)
+  const size_t nn = S.mesh->n_nodes;
+  std::vector<double> old(nn * nc);
+  HIP_OK(hipStreamSynchronize(S.ctx->stream));
+  for(int c = 0; c < nc; ++c)
+    HIP_OK(hipMemcpy(old.data() + c * nn, field[c], sizeof(double) * nn, hipMemcpyDeviceToHost));
+  for(uint32_t b = 0; b < n; ++b)
     for(int c = 0; c < nc; ++c) {
-      double old;
-      HIP_OK(hipMemcpy(&old, field[c] + nodes[b], sizeof(double), hipMemcpyDeviceToHost));
       rows.push_back(p2m[nodes[b]]);
       eqs.push_back((uint32_t)c);
-      inc.push_back(values[(size_t)b * nc + c] - old);
+      inc.push_back(values[(size_t)b * nc + c] - old[c * nn + nodes[b]]);
     }
-  }
 }
 
 }  // namespace

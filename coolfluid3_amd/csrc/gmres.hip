@@ -183,7 +183,8 @@ void gmres_ilu0(Lss& L, const cf3hip_solver_params& prm, cf3hip_solve_stats& st)
   double bnorm = std::sqrt(bn2);
   if(bnorm == 0.) bnorm = 1.;
   int it = 0, spmvs = 0;
-  bool conv = false;
+  // a cycle ends when the recurrence residual meets the tolerance (or at the restart length); the
+  // true residual b - A x computed at the start of the next cycle decides convergence
   while(it < prm.max_iter) {
     // r = b - A x ; V_0 = r / ||r||
     lss_spmv(L, 1., x, 0., w.as<double>());
@@ -194,7 +195,7 @@ void gmres_ilu0(Lss& L, const cf3hip_solver_params& prm, cf3hip_solve_stats& st)
     double beta = 0.;
     HIP_OK(hipMemcpyAsync(&beta, base, sizeof(double), hipMemcpyDeviceToHost, c.stream));
     HIP_OK(hipStreamSynchronize(c.stream));
-    if(beta <= prm.tol * bnorm) { conv = true; break; }
+    if(beta <= prm.tol * bnorm) break;
     hipLaunchKernelGGL(k_scale, dim3(gb), dim3(256), 0, c.stream, n, w.as<double>(), base, Vd);
     int j = 0;
     for(; j < m && it < prm.max_iter; ++j) {
@@ -215,20 +216,19 @@ void gmres_ilu0(Lss& L, const cf3hip_solver_params& prm, cf3hip_solve_stats& st)
       double res = 0.;
       HIP_OK(hipMemcpyAsync(&res, base, sizeof(double), hipMemcpyDeviceToHost, c.stream));
       HIP_OK(hipStreamSynchronize(c.stream));
-      if(res <= prm.tol * bnorm) { ++j; conv = true; break; }
+      if(res <= prm.tol * bnorm) { ++j; break; }
     }
     hipLaunchKernelGGL(k_back_subst, dim3(1), dim3(1), 0, c.stream, m, base, j, y);
     hipLaunchKernelGGL(k_lincomb, dim3(gb), dim3(256), 0, c.stream, n, j, Vd, y, w.as<double>());
     precond(w.as<double>(), z.as<double>());
     hipLaunchKernelGGL(k_add, dim3(gb), dim3(256), 0, c.stream, n, z.as<double>(), x);
     HIP_OK(hipGetLastError());
-    if(conv) break;
   }
   st.iterations = it;
   st.outer = 1;
   st.spmvs = spmvs;
   st.rel_residual = lss_residual(L);
-  st.converged = st.rel_residual <= prm.tol * 10. ? 1 : 0;
+  st.converged = st.rel_residual <= prm.tol ? 1 : 0;
 }
 
 }  // namespace cf3hip

@@ -143,6 +143,8 @@ struct Lss {
   DevBuf bc_mask;       // uint8 n_nodes: bit e set when dof (node, e) carries a condition
   DevBuf bc_newmask;    // uint8 n_nodes: conditions added by the current call
   DevBuf xtilde;        // matrix-free residual input (x with eliminated columns at their values)
+  bool tied = false;    // block rows tied (periodicity) since the last reset: the matrix-free
+                        // residual of the element operator no longer describes the system
   // Krylov workspace (allocated lazily)
   DevBuf work;          // generic vector workspace
   size_t work_vectors = 0;
@@ -172,6 +174,7 @@ struct SemiImplicit {
   DevBuf vec_ptr, vec_idx;   // per row: its e*k + a incidences in element order
   DevBuf src;                // per row: the last mesh node of the row (SetSolution)
   DevBuf p2m;                // per mesh node: its row (the identity without periodic links)
+  std::vector<uint32_t> p2m_host;   // host copy of p2m (boundary conditions are given by mesh node)
   DevBuf stage;              // element outputs, n_elems * k * k * 4 doubles
   DevBuf A, Uv, minv, aup;   // rows x dim: a, u, 1 / lumped mass, scratch
   DevBuf Pv, dps;            // rows: p, delta_p_sum
@@ -217,6 +220,7 @@ void ns_solve(Lss& L, Fields& F, const cf3hip_ns_params& nsp, const cf3hip_solve
 void lss_spmv(Lss& L, double alpha, const double* x, double beta, double* y, const double* stop = nullptr);
 void lss_spmv32(Lss& L, float alpha, const float* x, float beta, float* y, const double* stop = nullptr);
 void lss_dirichlet(Lss& L, uint32_t nbc, const uint32_t* nodes, const uint32_t* eqs, const double* values);
+void lss_tie(Lss& L, uint32_t npairs, const uint32_t* to, const uint32_t* from, bool system);
 void lss_point_csr(Lss& L, uint32_t* row_ptr_dev, uint32_t* col_dev, double* val_dev);
 void lss_diagonal(Lss& L, const double* d_dev, int mode);            // mode 0 set, 1 add
 void lss_column_zero(Lss& L, uint32_t col, int ieq, double* out_dev);

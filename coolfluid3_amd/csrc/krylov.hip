@@ -46,7 +46,7 @@ __device__ inline void block_reduce_store(double (&v)[NDOT], double* partials, u
 
 // S layout (double scalars)
 enum { S_RHO = 0, S_RHO_PREV, S_ALPHA, S_OMEGA, S_RHV, S_SN2, S_TS, S_TT, S_RN2, S_BN2, S_SCONV, S_THR2,
-       S_TMP0, S_TMP1, S_STOP, S_ITER, S_MAXR, S_DELTA, S_COUNT };
+       S_TMP0, S_TMP1, S_STOP, S_ITER, S_MAXR, S_DELTA, S_MAXR_ODD, S_COUNT };
 
 // finalize: sum partials of NDOT dots into S[dst + d], then apply a mode-specific scalar update.
 // FIN_RNORM ends an iteration: it counts it and raises S_STOP on convergence or breakdown;
@@ -107,8 +107,9 @@ __global__ void k_finalize(const double* __restrict__ partials, unsigned G, doub
 // every workgroup, in the fixed order of k_finalize (so all workgroups obtain the same bits); workgroup
 // 0 records the scalars in S for the host and for later kernels.  A kernel never reads a slot of S
 // that a workgroup of the same launch writes, except the stop flag, whose value every workgroup also
-// derives itself.  rho alternates between two slots by iteration parity.
-enum { S_RHO_EVEN = S_RHO, S_RHO_ODD = S_RHO_PREV };
+// derives itself.  rho and the running maximum of the residual norm alternate between two slots by
+// iteration parity.
+enum { S_RHO_EVEN = S_RHO, S_RHO_ODD = S_RHO_PREV, S_MAXR_EVEN = S_MAXR };
 
 // sum of G partials per dot product (partials[d * G + i]), result in every thread of the workgroup
 template<int NDOT>
@@ -150,13 +151,14 @@ __global__ void k_fb_p(uint64_t n, double* __restrict__ S, int it, int rel, int 
   const double rn2 = red[0], rho = red[1];
   const bool conv = S[S_SCONV] != 0. || rn2 <= S[S_THR2];
   bool stop = conv || !isfinite(rn2) || rho == 0. || S[S_OMEGA] == 0.;
-  if(rel) stop = stop || sqrt(rn2) < S[S_DELTA] * S[S_MAXR];
+  const double maxr = S[(it & 1) ? S_MAXR_EVEN : S_MAXR_ODD];   // after iteration it - 1
+  if(rel) stop = stop || sqrt(rn2) < S[S_DELTA] * maxr;
   if(leader()) {
     S[S_RN2] = rn2;
     S[(it & 1) ? S_RHO_ODD : S_RHO_EVEN] = rho;
     S[S_ITER] = (double)it;
     if(stop) S[S_STOP] = 1.;
-    else if(rel) S[S_MAXR] = fmax(S[S_MAXR], sqrt(rn2));
+    S[(it & 1) ? S_MAXR_ODD : S_MAXR_EVEN] = (rel && !stop) ? fmax(maxr, sqrt(rn2)) : maxr;
   }
   if(stop || fin_only) return;
   const double rho_old = S[(it & 1) ? S_RHO_EVEN : S_RHO_ODD];
@@ -469,7 +471,8 @@ struct Recurrence {
     dinv = t + n;
     fused = sizeof(R) == sizeof(double) && L.neq >= 1 && L.neq <= 4;
     Gv = red_grid(n);
-    Gs = std::min<unsigned>(kFusedSpmvBlocks, grid_for(L.n_nodes, kRT / 16));
+    // rows per workgroup of the fused products: 16 lanes per node row, 8 for scalar rows (k_spmvn_dot)
+    Gs = std::min<unsigned>(kFusedSpmvBlocks, grid_for(L.n_nodes, kRT / (L.neq == 1 ? 8 : 16)));
     part = c.red_partials.as<double>();
     if(fused) {
       if(!c.fused_partials.ptr) c.fused_partials.alloc(&c, sizeof(double) * kFusedPartials);
@@ -837,6 +840,7 @@ void reliable_bicgstab(Lss& L, Resid resid, const cf3hip_solver_params& prm, cf3
   int stall = 0;
   bool switched = false;
   hipLaunchKernelGGL(k_set_scalar, dim3(1), dim3(1), 0, c.stream, S, (int)S_MAXR, rtrue);
+  hipLaunchKernelGGL(k_set_scalar, dim3(1), dim3(1), 0, c.stream, S, (int)S_MAXR_ODD, rtrue);
   hipLaunchKernelGGL(k_set_scalar, dim3(1), dim3(1), 0, c.stream, S, (int)S_DELTA, delta);
   int it = 0, batch = 1;
   Scalars sc;
@@ -867,6 +871,7 @@ void reliable_bicgstab(Lss& L, Resid resid, const cf3hip_solver_params& prm, cf3
     // breakdown: restart the recurrences from the true residual; otherwise keep the directions
     rc.seed(thr2, breakdown, it);
     hipLaunchKernelGGL(k_set_scalar, dim3(1), dim3(1), 0, c.stream, S, (int)S_MAXR, rtrue);
+    hipLaunchKernelGGL(k_set_scalar, dim3(1), dim3(1), 0, c.stream, S, (int)S_MAXR_ODD, rtrue);
     HIP_OK(hipGetLastError());
     batch = 1;
   }
@@ -905,6 +910,9 @@ void ns_solve(Lss& L, Fields& F, const cf3hip_ns_params& nsp, const cf3hip_solve
     lss_solve(L, prm, st);
     return;
   }
+  if(L.tied)
+    CF3_THROW(CF3HIP_ERR_NOT_IMPLEMENTED, "tied block rows: the matrix-free residual of the element operator does not "
+                                          "describe the system; solve the stored system instead");
   // the true residual is evaluated matrix-free from the FP64 element matrices of the fields
   reliable_bicgstab(L, [&](const double* x, double* r) { ns_residual(L, F, nsp, x, r); }, prm, st);
 }

@@ -424,11 +424,116 @@ void lss_dirichlet(Lss& L, uint32_t nbc, const uint32_t* nodes, const uint32_t* 
   HIP_OK(hipStreamSynchronize(c.stream));
 }
 
+// Matrix::tie_blockrow_pairs / System::periodicity for an ordered list of (to, from) block-row pairs
+// (TrilinosFEVbrMatrix.cpp:558-590, System.cpp:259-282).  One workgroup walks the pairs in order, a
+// barrier between pairs, so a row tied by one pair is seen tied by the next.  Per pair: the two rows
+// must hold the same block columns (err 1: block counts differ, err 2: columns differ); row from is
+// added into row to and becomes x_from - x_to = 0 (identity at column from, minus identity at column
+// to); in row to, the block of column from is moved onto column to.  With rhs / sol (System): rhs_to
+// += rhs_from, rhs_from = 0, and both solution entries take their mean.
+template<typename T>
+__global__ __launch_bounds__(256) void k_tie_pairs(uint32_t npairs, const uint32_t* __restrict__ pairs, int neq,
+                                                   const uint32_t* __restrict__ row_ptr,
+                                                   const uint32_t* __restrict__ cols, T* __restrict__ vals,
+                                                   double* __restrict__ rhs, double* __restrict__ sol,
+                                                   int* __restrict__ err)
+{
+  __shared__ int s_err, s_diag, s_pair;
+  const int t = threadIdx.x;
+  const int nn = neq * neq;
+  for(uint32_t p = 0; p < npairs; ++p) {
+    const uint32_t to = pairs[2 * p], from = pairs[2 * p + 1];
+    const uint32_t bt = row_ptr[to], bf = row_ptr[from];
+    const uint32_t nb = row_ptr[to + 1] - bt;
+    if(t == 0) { s_err = 0; s_diag = -1; s_pair = -1; }
+    __syncthreads();
+    if(row_ptr[from + 1] - bf != nb) {
+      if(t == 0) *err = 1;
+      return;   // uniform: every thread takes this branch
+    }
+    for(uint32_t i = t; i < nb; i += blockDim.x) {
+      const uint32_t c = cols[bt + i];
+      if(c != cols[bf + i]) s_err = 2;
+      if(c == from) s_diag = (int)i;
+      if(c == to) s_pair = (int)i;
+    }
+    __syncthreads();
+    if(s_err || s_diag < 0 || s_pair < 0) {
+      if(t == 0) *err = s_err ? s_err : 2;
+      return;
+    }
+    const int diag = s_diag, pr = s_pair;
+    for(uint32_t o = t; o < nb * nn; o += blockDim.x) {
+      T* vt = vals + (uint64_t)bt * nn + o;
+      T* vf = vals + (uint64_t)bf * nn + o;
+      *vt = *vt + *vf;
+      *vf = T(0);
+    }
+    __syncthreads();
+    if(t < neq) {
+      vals[((uint64_t)bf + diag) * nn + t * neq + t] = T(1);
+      vals[((uint64_t)bf + pr) * nn + t * neq + t] = T(-1);
+    }
+    if(t < nn) {
+      T* vp = vals + ((uint64_t)bt + pr) * nn + t;
+      T* vd = vals + ((uint64_t)bt + diag) * nn + t;
+      *vp = *vp + *vd;
+      *vd = T(0);
+    }
+    if(t < neq && rhs) {
+      const uint64_t dt = (uint64_t)to * neq + t, df = (uint64_t)from * neq + t;
+      rhs[dt] = rhs[dt] + rhs[df];
+      rhs[df] = 0.;
+      const double m = 0.5 * (sol[dt] + sol[df]);
+      sol[df] = m;
+      sol[dt] = m;
+    }
+    __syncthreads();
+  }
+}
+
+void lss_tie(Lss& L, uint32_t npairs, const uint32_t* to, const uint32_t* from, bool system)
+{
+  if(!npairs) return;
+  Context& c = *L.ctx;
+  std::vector<uint32_t> pr(2 * (size_t)npairs);
+  for(uint32_t i = 0; i < npairs; ++i) {
+    if(to[i] >= L.n_nodes || from[i] >= L.n_nodes)
+      CF3_THROW(CF3HIP_ERR_INVALID_ARG, "periodic pair outside the system");
+    if(to[i] == from[i]) CF3_THROW(CF3HIP_ERR_INVALID_ARG, "a block row cannot be tied to itself");
+    pr[2 * i] = to[i];
+    pr[2 * i + 1] = from[i];
+  }
+  DevBuf d_pairs, d_err;
+  d_pairs.alloc(&c, sizeof(uint32_t) * pr.size());
+  d_err.alloc(&c, sizeof(int) * 2);
+  HIP_OK(hipMemcpyAsync(d_pairs.ptr, pr.data(), sizeof(uint32_t) * pr.size(), hipMemcpyHostToDevice, c.stream));
+  HIP_OK(hipMemsetAsync(d_err.ptr, 0, sizeof(int) * 2, c.stream));
+  hipLaunchKernelGGL(k_tie_pairs<double>, dim3(1), dim3(256), 0, c.stream, npairs, d_pairs.as<uint32_t>(), L.neq,
+                     L.row_ptr.as<uint32_t>(), L.cols.as<uint32_t>(), L.values.as<double>(),
+                     system ? L.rhs.as<double>() : (double*)nullptr, system ? L.sol.as<double>() : (double*)nullptr,
+                     d_err.as<int>());
+  if(L.has_fp32)
+    hipLaunchKernelGGL(k_tie_pairs<float>, dim3(1), dim3(256), 0, c.stream, npairs, d_pairs.as<uint32_t>(), L.neq,
+                       L.row_ptr.as<uint32_t>(), L.cols.as<uint32_t>(), L.values32.as<float>(), (double*)nullptr,
+                       (double*)nullptr, d_err.as<int>() + 1);
+  HIP_OK(hipGetLastError());
+  int err[2] = {0, 0};
+  HIP_OK(hipMemcpyAsync(err, d_err.ptr, sizeof(err), hipMemcpyDeviceToHost, c.stream));
+  HIP_OK(hipStreamSynchronize(c.stream));
+  L.tied = true;
+  if(err[0] == 1 || err[1] == 1)
+    CF3_THROW(CF3HIP_ERR_INVALID_ARG, "Number of blocks do not match for the two block rows to be tied together.");
+  if(err[0] || err[1])
+    CF3_THROW(CF3HIP_ERR_INVALID_ARG, "Indices of the entries do not match for the two block rows to be tied together.");
+}
+
 void lss_clear_conditions(Lss& L)
 {
   L.bc_host.clear();
   L.bc_order.clear();
   L.bc_n = 0;
+  L.tied = false;
   if(L.bc_mask.ptr) lss_fill8(*L.ctx, L.bc_mask.as<uint8_t>(), L.n_nodes, 0);
 }
 

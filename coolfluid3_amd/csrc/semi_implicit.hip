@@ -633,6 +633,7 @@ void si_create(SemiImplicit& S, Lss& U, Lss& P)
   for(uint32_t i = 0; i < M.n_nodes; ++i) src[p2m[i]] = i;   // node order: the last node of a row wins
   up(S.src, src);
   up(S.p2m, p2m);
+  S.p2m_host = p2m;
   S.stage.alloc(&c, sizeof(double) * (size_t)E * K * K * M.dim * M.dim);
   const size_t vd = (size_t)rows * M.dim;
   for(DevBuf* b : {&S.A, &S.Uv, &S.minv, &S.aup}) { b->alloc(&c, sizeof(double) * vd); lss_fill(c, b->as<double>(), vd, 0.); }

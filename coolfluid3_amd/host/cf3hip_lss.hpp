@@ -163,10 +163,9 @@ public:
   void set_row(uint32_t iblockrow, uint32_t ieq, double diagval, double offdiagval);
   void get_column_and_replace_to_zero(uint32_t iblockcol, uint32_t ieq, std::vector<double>& values);
   void symmetric_dirichlet(uint32_t blockrow, uint32_t ieq, double value, Vector& rhs);
-  void tie_blockrow_pairs(uint32_t, uint32_t)
-  {
-    throw NotImplemented(CF3HIP_ERR_NOT_IMPLEMENTED, "periodic block-row ties are not implemented by cf3hip");
-  }
+  // TrilinosFEVbrMatrix::tie_blockrow_pairs (TrilinosFEVbrMatrix.cpp:558-590): BadValue when the two
+  // rows hold different block columns
+  void tie_blockrow_pairs(uint32_t iblockrow_to, uint32_t iblockrow_from);
   void set_diagonal(const std::vector<double>& diag);
   void add_diagonal(const std::vector<double>& diag);
   void get_diagonal(std::vector<double>& diag);
@@ -285,6 +284,13 @@ public:
     const std::vector<uint32_t> r = rows(nodes);
     check(cf3hip_lss_dirichlet(h_, (uint32_t)r.size(), r.data(), eqs.data(), values.data()));
   }
+  // System::periodicity (System.cpp:259-282): tie the two block rows, move the rhs of row from into row
+  // to, and give both solution entries their mean
+  void periodicity(uint32_t iblockrow_to, uint32_t iblockrow_from)
+  {
+    const uint32_t t = row(iblockrow_to), f = row(iblockrow_from);
+    check(cf3hip_lss_periodicity(h_, 1, &t, &f));
+  }
   bool solve() { return strategy_.solve(); }
   void reset(double v = 0.)
   {
@@ -402,6 +408,11 @@ inline void Matrix::get_values(BlockAccumulator& b)
 inline void Matrix::set_row(uint32_t iblockrow, uint32_t ieq, double diagval, double offdiagval)
 {
   check(cf3hip_matrix_set_row(sys_.handle(), sys_.row(iblockrow), ieq, diagval, offdiagval));
+}
+inline void Matrix::tie_blockrow_pairs(uint32_t iblockrow_to, uint32_t iblockrow_from)
+{
+  const uint32_t t = sys_.row(iblockrow_to), f = sys_.row(iblockrow_from);
+  check(cf3hip_matrix_tie_blockrow_pairs(sys_.handle(), 1, &t, &f));
 }
 inline void Matrix::get_column_and_replace_to_zero(uint32_t iblockcol, uint32_t ieq, std::vector<double>& values)
 {

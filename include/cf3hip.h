@@ -186,6 +186,18 @@ int cf3hip_vector_scale(cf3hip_lss lss, int which, double alpha);
 int cf3hip_lss_dirichlet(cf3hip_lss lss, uint32_t nbc, const uint32_t* nodes, const uint32_t* eqs,
                          const double* values);
 
+/* ---- Matrix::tie_blockrow_pairs (cf3/math/LSS/Matrix.hpp; TrilinosFEVbrMatrix.cpp:558-590,
+ *      TrilinosCrsMatrix.cpp:413-460) for (to[i], from[i]) block-row pairs applied in order: row from is
+ *      added into row to and becomes x_from - x_to = 0, and in row to the block of column from moves
+ *      onto column to.  Both rows must hold the same block columns (BadValue otherwise).  Also applied
+ *      to the FP32 mirror when enabled.  Host arrays. ---- */
+int cf3hip_matrix_tie_blockrow_pairs(cf3hip_lss lss, uint32_t npairs, const uint32_t* to, const uint32_t* from);
+/* ---- System::periodicity (cf3/math/LSS/System.cpp:259-282) for each pair in order: the tie above,
+ *      rhs_to += rhs_from, rhs_from = 0, and both solution entries set to their mean.  A system with
+ *      tied rows is solved from its stored values (cf3hip_lss_solve); the matrix-free precision form
+ *      of cf3hip_ns_solve refuses it (NotImplemented) until the next assembly or reset. ---- */
+int cf3hip_lss_periodicity(cf3hip_lss lss, uint32_t npairs, const uint32_t* to, const uint32_t* from);
+
 /* ---- SolutionStrategy (cf3/math/LSS/SolutionStrategy.hpp:35-59; Trilinos strategy
  *      TrilinosStratimikosStrategy.cpp:135-207) ---- */
 #define CF3HIP_BICGSTAB_JACOBI 0

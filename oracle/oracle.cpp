@@ -700,6 +700,10 @@ void gather_element(int et, const Uint* cn, const Real* coords, const Real* u, c
 
 // p2m (optional): matrix block row of each mesh node (periodic links; TrilinosDetail.cpp:147-227).  The
 // element is gathered by its mesh nodes; its rhs rows are those of p2m (the slots already are).
+// When set, the scatter accumulates |M| and |b| of every element contribution (orc_ns_assemble_abs):
+// the scale of each entry's rounding error for parity bars.
+bool g_abs_assembly = false;
+
 int assemble_range(int et, const orc_ns_params* prm, const Real* coords, const Uint* conn,
                    const Real* u, const Real* p, const Real* u_adv, const Real* nu, const Real* g,
                    const Uint* slots, Real* values, Real* rhs, const Uint* elems, size_t ne,
@@ -720,12 +724,15 @@ int assemble_range(int et, const orc_ns_params* prm, const Real* coords, const U
       for(int cb = 0; cb < k; ++cb) {
         Real* blk = values + (size_t)slots[(e*k + ra)*k + cb] * nn;
         for(int al = 0; al < neq; ++al)
-          for(int be = 0; be < neq; ++be)
-            blk[al*neq+be] += M[(ra*neq+al)*n + cb*neq+be];
+          for(int be = 0; be < neq; ++be) {
+            const Real m = M[(ra*neq+al)*n + cb*neq+be];
+            blk[al*neq+be] += g_abs_assembly ? std::fabs(m) : m;
+          }
       }
       // TrilinosVector::add_rhs_values (TrilinosVector.cpp:189-203)
       const Uint row = p2m ? p2m[cn[ra]] : cn[ra];
-      for(int al = 0; al < neq; ++al) rhs[(size_t)row*neq+al] += b[ra*neq+al];
+      for(int al = 0; al < neq; ++al)
+        rhs[(size_t)row*neq+al] += g_abs_assembly ? std::fabs(b[ra*neq+al]) : b[ra*neq+al];
     }
   }
   return 0;
@@ -761,9 +768,18 @@ inline int64_t find_block(const Uint* start, const Uint* cols, Uint row, Uint co
   return -1;
 }
 
+// Threads of the Krylov kernels (SpMV rows, dot products, vector updates) of the oracle, 1 by default.
+// Only the CPU-baseline timing raises it; with several threads a dot product sums fixed chunks in
+// order, so results do not depend on the thread count but differ from the one-thread sums in the
+// last bits.
+int g_krylov_threads = 1;
+
 void spmv_plain(int neq, Uint N, const Uint* start, const Uint* cols, const Real* values, const Real* x, Real* y)
 {
   const int nn = neq*neq;
+#ifdef _OPENMP
+#pragma omp parallel for num_threads(g_krylov_threads) if(g_krylov_threads > 1) schedule(static)
+#endif
   for(Uint i = 0; i < N; ++i) {
     for(int al = 0; al < neq; ++al) {
       Real s = 0.;
@@ -779,8 +795,23 @@ void spmv_plain(int neq, Uint N, const Uint* start, const Uint* cols, const Real
 
 Real dot(size_t n, const Real* a, const Real* b)
 {
+  if(g_krylov_threads <= 1) {
+    Real s = 0.;
+    for(size_t i = 0; i < n; ++i) s += a[i]*b[i];
+    return s;
+  }
+  constexpr int kChunks = 256;
+  Real part[kChunks];
+#ifdef _OPENMP
+#pragma omp parallel for num_threads(g_krylov_threads) schedule(static)
+#endif
+  for(int c = 0; c < kChunks; ++c) {
+    Real s = 0.;
+    for(size_t i = n * c / kChunks; i < n * (c + 1) / kChunks; ++i) s += a[i]*b[i];
+    part[c] = s;
+  }
   Real s = 0.;
-  for(size_t i = 0; i < n; ++i) s += a[i]*b[i];
+  for(int c = 0; c < kChunks; ++c) s += part[c];
   return s;
 }
 
@@ -1044,6 +1075,19 @@ int orc_ns_assemble(int et, const orc_ns_params* prm, uint32_t n_nodes, uint32_t
   return err;
 }
 
+int orc_ns_assemble_abs(int et, const orc_ns_params* prm, uint32_t n_nodes, uint32_t n_elems,
+                        const double* coords, const uint32_t* conn, const double* u, const double* p,
+                        const double* u_adv, const double* nu, const double* g,
+                        const uint32_t* start, const uint32_t* cols, const uint32_t* slots,
+                        double* values, double* rhs, int nthreads)
+{
+  g_abs_assembly = true;
+  const int err = orc_ns_assemble(et, prm, n_nodes, n_elems, coords, conn, u, p, u_adv, nu, g, start, cols, slots,
+                                  values, rhs, nthreads);
+  g_abs_assembly = false;
+  return err;
+}
+
 int orc_ns_assemble_p2m(int et, const orc_ns_params* prm, uint32_t n_rows, uint32_t n_elems,
                         const double* coords, const uint32_t* conn, const uint32_t* p2m, const double* u,
                         const double* p, const double* u_adv, const double* nu, const double* g,
@@ -1064,6 +1108,47 @@ void orc_set_row(int neq, const uint32_t* start, const uint32_t* cols, double* v
     for(int j = 0; j < neq; ++j) values[(size_t)q*nn + ieq*neq + j] = offdiag;
     if(cols[q] == br) values[(size_t)q*nn + ieq*neq + ieq] = diag;
   }
+}
+
+// System::periodicity (System.cpp:259-282) with TrilinosFEVbrMatrix::tie_blockrow_pairs
+// (TrilinosFEVbrMatrix.cpp:558-590) for each (to, from) pair in order; serial, so every row is owned.
+// rhs / sol may be null (the matrix part alone).  Returns -1 when the two rows hold different block
+// counts (BadValue of the reference) and -2 when their block columns differ (its cf3_assert).
+int orc_tie_blockrow_pairs(int neq, const uint32_t* start, const uint32_t* cols, double* values, double* rhs,
+                           double* sol, uint32_t npairs, const uint32_t* to, const uint32_t* from)
+{
+  const int nn = neq*neq;
+  for(Uint p = 0; p < npairs; ++p) {
+    const Uint bt = to[p], bf = from[p];
+    const Uint nt = start[bt+1] - start[bt], nf = start[bf+1] - start[bf];
+    if(nt != nf) return -1;
+    int diag = -1, pair = -1;
+    for(Uint i = 0; i < nt; ++i) {
+      const Uint ct = cols[start[bt]+i], cf = cols[start[bf]+i];
+      if(ct != cf) return -2;
+      if(cf == bf) diag = (int)i;
+      if(ct == bt) pair = (int)i;
+      Real* vt = values + (size_t)(start[bt]+i)*nn;
+      Real* vf = values + (size_t)(start[bf]+i)*nn;
+      for(int k = 0; k < nn; ++k) { vt[k] += vf[k]; vf[k] = 0.; }
+    }
+    if(diag < 0 || pair < 0) return -2;
+    Real* vfd = values + (size_t)(start[bf]+diag)*nn;
+    Real* vfp = values + (size_t)(start[bf]+pair)*nn;
+    for(int e = 0; e < neq; ++e) { vfd[e*neq+e] = 1.; vfp[e*neq+e] = -1.; }
+    Real* vtp = values + (size_t)(start[bt]+pair)*nn;
+    Real* vtd = values + (size_t)(start[bt]+diag)*nn;
+    for(int k = 0; k < nn; ++k) { vtp[k] += vtd[k]; vtd[k] = 0.; }
+    if(rhs)
+      for(int e = 0; e < neq; ++e) { rhs[(size_t)bt*neq+e] += rhs[(size_t)bf*neq+e]; rhs[(size_t)bf*neq+e] = 0.; }
+    if(sol)
+      for(int e = 0; e < neq; ++e) {
+        const Real m = 0.5*(sol[(size_t)bt*neq+e] + sol[(size_t)bf*neq+e]);
+        sol[(size_t)bf*neq+e] = m;
+        sol[(size_t)bt*neq+e] = m;
+      }
+  }
+  return 0;
 }
 
 void orc_dirichlet(int neq, uint32_t n_nodes, const uint32_t* start, const uint32_t* cols,
@@ -1128,6 +1213,8 @@ void orc_point_csr(int neq, uint32_t N, const uint32_t* start, const uint32_t* c
       row_ptr[(size_t)i*neq + al + 1] = (Uint)pos;
     }
 }
+
+void orc_set_krylov_threads(int n) { g_krylov_threads = n < 1 ? 1 : n; }
 
 int orc_bicgstab_jacobi(int neq, uint32_t N, const uint32_t* start, const uint32_t* cols, const double* values,
                         const double* b, double* x, double tol, int maxit, orc_solve_stats* st)

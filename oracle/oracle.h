@@ -96,6 +96,13 @@ int orc_ns_assemble(int etype, const orc_ns_params* prm, uint32_t n_nodes, uint3
  * node (a linked node shares its master's row; System::create with periodic_links_nodes/active,
  * cf3/math/LSS/Trilinos/TrilinosDetail.cpp:147-227); start/slots come from the sparsity of p2m[conn].
  * Elements and fields are gathered by mesh node.  Serial. */
+/* the same scatter of |M_e| and |b_e| for every element contribution: per entry the sum of the magnitudes of
+   its contributions (the scale of its rounding error) */
+int orc_ns_assemble_abs(int etype, const orc_ns_params* prm, uint32_t n_nodes, uint32_t n_elems,
+                        const double* coords, const uint32_t* conn, const double* u, const double* p,
+                        const double* u_adv, const double* nu, const double* g,
+                        const uint32_t* start, const uint32_t* cols, const uint32_t* slots,
+                        double* values, double* rhs, int nthreads);
 int orc_ns_assemble_p2m(int etype, const orc_ns_params* prm, uint32_t n_rows, uint32_t n_elems,
                         const double* coords, const uint32_t* conn, const uint32_t* p2m, const double* u,
                         const double* p, const double* u_adv, const double* nu, const double* g,
@@ -107,6 +114,10 @@ void orc_dirichlet(int neq, uint32_t n_nodes, const uint32_t* start, const uint3
                    double* values, double* rhs, double* sol, uint32_t nbc,
                    const uint32_t* bc_node, const uint32_t* bc_eq, const double* bc_val);
 /* Matrix::set_row (Matrix.hpp; TrilinosFEVbrMatrix::set_row): row (blockrow, ieq) -> diag/offdiag */
+/* System::periodicity / Matrix::tie_blockrow_pairs for (to[i], from[i]) pairs in order; rhs, sol may be null.
+   0 ok, -1 block counts differ, -2 block columns differ */
+int orc_tie_blockrow_pairs(int neq, const uint32_t* start, const uint32_t* cols, double* values, double* rhs,
+                           double* sol, uint32_t npairs, const uint32_t* to, const uint32_t* from);
 void orc_set_row(int neq, const uint32_t* start, const uint32_t* cols, double* values,
                  uint32_t blockrow, uint32_t ieq, double diag, double offdiag);
 /* y = alpha*A*x + beta*y  (Matrix::apply, TrilinosDetail.cpp:380-411) */
@@ -124,6 +135,8 @@ typedef struct orc_solve_stats {
   int spmvs;
 } orc_solve_stats;
 /* BiCGStab, right Jacobi preconditioning, stop on ||r|| <= tol*||b|| */
+/* threads of the Krylov kernels (SpMV, dot products); 1 by default, raised only for CPU-baseline timing */
+void orc_set_krylov_threads(int n);
 int orc_bicgstab_jacobi(int neq, uint32_t n_nodes, const uint32_t* start, const uint32_t* cols, const double* values,
                         const double* b, double* x, double tol, int maxit, orc_solve_stats* st);
 /* restarted GMRES(m), right ILU(0) preconditioning on the point matrix, CGS2 orthogonalisation */

@@ -73,10 +73,13 @@ class Oracle:
         L.orc_ns_element_system.argtypes = [C.c_int, C.POINTER(NSParams)] + [_d] * 8
         L.orc_ns_assemble.argtypes = ([C.c_int, C.POINTER(NSParams), C.c_uint32, C.c_uint32, _d, _u32]
                                       + [_d] * 5 + [_u32] * 3 + [_d, _d, C.c_int])
+        L.orc_ns_assemble_abs.argtypes = L.orc_ns_assemble.argtypes
         L.orc_ns_assemble_p2m.argtypes = ([C.c_int, C.POINTER(NSParams), C.c_uint32, C.c_uint32, _d, _u32, _u32]
                                           + [_d] * 5 + [_u32] * 2 + [_d, _d])
         L.orc_dirichlet.argtypes = [C.c_int, C.c_uint32, _u32, _u32, _d, _d, _d, C.c_uint32, _u32, _u32, _d]
         L.orc_set_row.argtypes = [C.c_int, _u32, _u32, _d, C.c_uint32, C.c_uint32, C.c_double, C.c_double]
+        L.orc_tie_blockrow_pairs.restype = C.c_int
+        L.orc_tie_blockrow_pairs.argtypes = [C.c_int, _u32, _u32, _d, _d, _d, C.c_uint32, _u32, _u32]
         L.orc_spmv.argtypes = [C.c_int, C.c_uint32, _u32, _u32, _d, C.c_double, _d, C.c_double, _d]
         L.orc_point_csr.argtypes = [C.c_int, C.c_uint32, _u32, _u32, _d, _u32, _u32, _d]
         L.orc_bicgstab_jacobi.argtypes = [C.c_int, C.c_uint32, _u32, _u32, _d, _d, _d, C.c_double, C.c_int,
@@ -170,7 +173,10 @@ class Oracle:
         return M, rhs
 
     # ---------------- assembly / LSS ----------------
-    def ns_assemble(self, etype, prm, coords, conn, u, p, u_adv, nu, start, cols, slots, g=None, nthreads=1):
+    def ns_assemble(self, etype, prm, coords, conn, u, p, u_adv, nu, start, cols, slots, g=None, nthreads=1,
+                    magnitudes=False):
+        """Values and rhs of the assembled system; with magnitudes, per entry the sum of the magnitudes of its
+        element contributions instead (the scale of the entry's rounding error)."""
         dim = DIM[etype]
         neq = dim + 1
         n_nodes = coords.shape[0]
@@ -178,7 +184,8 @@ class Oracle:
         values = np.zeros(int(start[-1]) * neq * neq)
         rhs = np.zeros(n_nodes * neq)
         gg = _f64(g) if g is not None else np.zeros((n_nodes, dim))
-        err = self.lib.orc_ns_assemble(etype, C.byref(prm), n_nodes, conn.shape[0], _p(_f64(coords)),
+        fn = self.lib.orc_ns_assemble_abs if magnitudes else self.lib.orc_ns_assemble
+        err = fn(etype, C.byref(prm), n_nodes, conn.shape[0], _p(_f64(coords)),
                                        _p(conn, _u32), _p(_f64(u)), _p(_f64(p)), _p(_f64(u_adv)), _p(_f64(nu)),
                                        _p(gg), _p(_u32a(start), _u32), _p(_u32a(cols), _u32),
                                        _p(_u32a(slots), _u32), _p(values), _p(rhs), nthreads)
@@ -213,6 +220,14 @@ class Oracle:
         self.lib.orc_set_row(neq, _p(_u32a(start), _u32), _p(_u32a(cols), _u32), _p(values), blockrow, ieq,
                              diag, offdiag)
 
+    def periodicity(self, neq, start, cols, values, rhs, sol, to, frm):
+        """In place: ties of (to[i], frm[i]) block rows in order (rhs / sol may be None: the matrix only).
+        Returns 0, -1 (block counts differ) or -2 (block columns differ)."""
+        t, f = _u32a(to), _u32a(frm)
+        return self.lib.orc_tie_blockrow_pairs(neq, _p(_u32a(start), _u32), _p(_u32a(cols), _u32), _p(values),
+                                               _p(rhs) if rhs is not None else None,
+                                               _p(sol) if sol is not None else None, len(t), _p(t, _u32), _p(f, _u32))
+
     def spmv(self, neq, start, cols, values, x, alpha=1.0, beta=0.0, y=None):
         n_nodes = len(start) - 1
         y = np.zeros(n_nodes * neq) if y is None else _f64(y).copy()
@@ -229,6 +244,10 @@ class Oracle:
         self.lib.orc_point_csr(neq, n_nodes, _p(_u32a(start), _u32), _p(_u32a(cols), _u32), _p(_f64(values)),
                                _p(rp, _u32), _p(ci, _u32), _p(v))
         return rp, ci, v
+
+    def set_krylov_threads(self, n):
+        """Threads of the oracle's SpMV and dot products (1 by default; CPU-baseline timing only)."""
+        self.lib.orc_set_krylov_threads(int(n))
 
     def bicgstab(self, neq, start, cols, values, b, x0, tol=1e-10, maxit=2000):
         x = _f64(x0).copy()

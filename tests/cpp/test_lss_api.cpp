@@ -60,7 +60,36 @@ static void symmetric_dirichlet(Context& ctx)
   CHECK(sv == 10.);
   // an entry outside the graph is illegal (TrilinosFEVbrMatrix.cpp:175)
   CHECK(throws<BadValue>([&] { double v; s.matrix().get_value(3, 0, v); }));
-  CHECK(throws<NotImplemented>([&] { s.matrix().tie_blockrow_pairs(0, 3); }));
+  // rows 0 and 3 hold different block columns: they cannot be tied (TrilinosFEVbrMatrix.cpp:575-577)
+  CHECK(throws<BadValue>([&] { s.matrix().tie_blockrow_pairs(0, 3); }));
+}
+
+// System::periodicity on a dense 3-node graph (System.cpp:259-282, TrilinosFEVbrMatrix.cpp:558-590):
+// A = [1 2 3; 4 5 6; 7 8 9], rhs = (1, 2, 3), sol = (10, 20, 30); tying row 0 to row 2 gives
+// A = [1 0 -1; 4 5 6; 0 10 20], rhs = (0, 2, 4), sol = (20, 20, 20)
+static void periodicity(Context& ctx)
+{
+  const std::vector<uint32_t> start = {0, 3, 6, 9};
+  const std::vector<uint32_t> conn = {0, 1, 2, 0, 1, 2, 0, 1, 2};
+  lss::System s(ctx, 1, conn, start);
+  for(uint32_t r = 0; r < 3; ++r) {
+    for(uint32_t c = 0; c < 3; ++c) s.matrix().set_value(c, r, 3. * r + c + 1.);
+    s.rhs().set_value(r, 0, r + 1.);
+    s.solution().set_value(r, 0, 10. * (r + 1.));
+  }
+  s.periodicity(2, 0);
+  const double expect[3][3] = {{1, 0, -1}, {4, 5, 6}, {0, 10, 20}};
+  for(uint32_t r = 0; r < 3; ++r)
+    for(uint32_t c = 0; c < 3; ++c) {
+      double v = -1.;
+      s.matrix().get_value(c, r, v);
+      CHECK(v == expect[r][c]);
+    }
+  std::vector<double> rhs, sol;
+  s.rhs().get(rhs);
+  s.solution().get(sol);
+  CHECK(rhs[0] == 0. && rhs[1] == 2. && rhs[2] == 4.);
+  CHECK(sol[0] == 20. && sol[1] == 20. && sol[2] == 20.);
 }
 
 // 1D chain of n nodes with neq = 2 (utest-lss-atomic.cpp style)
@@ -433,6 +462,9 @@ int main(int argc, char** argv)
     std::printf("symmetric_dirichlet\n");
     std::fflush(stdout);
     symmetric_dirichlet(ctx);
+    std::printf("periodicity\n");
+    std::fflush(stdout);
+    periodicity(ctx);
     std::printf("atomic_operations\n");
     std::fflush(stdout);
     atomic_operations(ctx);

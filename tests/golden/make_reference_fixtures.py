@@ -9,7 +9,9 @@ Sources:
   test/mesh/utest-mesh-lagrangep1-tetra3d.cpp:36-175  (nodes, volume, Jacobian, adjoint)
   test/mesh/utest-mesh-lagrangep1-hexa3d.cpp:36-270   (nodes, shape values/gradients, Jacobian, adjoint, det)
   test/mesh/utest-mesh-lagrangep1-triag2d.cpp:36-178  (nodes, volumes, Jacobian, adjoint)
-  test/math/utest-lss-test-matrix.hpp:32-50,669-709   (per-rank element lists, node-graph CSR, periodic pairs)
+  test/math/utest-lss-test-matrix.hpp:32-719          (4-rank Stokes system: per-rank element lists, node-graph
+                                                       CSR, owned-row values before / after the conditions, rhs,
+                                                       solution, Dirichlet and periodic conditions, result)
   test/math/utest-lss-symmetric-dirichlet.cpp:72-208  (2-rank 4-node system, merged to serial)
   test/math/utest-lss-vector.cpp:144-187               (apply y = 2*A*x + y with A == 1)
 Also copies mesh files of the reference (data files, unchanged) into meshes/: the Gambit neutral files
@@ -82,14 +84,26 @@ def main(ref):
     }
     # ---- LSS test matrix (4-rank Stokes system): per-rank element lists and node-graph CSR ----
     src = open(os.path.join(ref, "test/math/utest-lss-test-matrix.hpp")).read()
+    # the bc_value arrays use two macros of the file: P_ (fixed pressure) and U_ (fixed velocity)
+    macros = {"P_": 0.0, "U_": 1.0}
     ranks = []
     for r in range(4):
-        ranks.append({
+        rk = {
             "elem_nodes": [int(x) for x in cpp_array(src, "elem_nodes%d" % r)],
             "column_indices": [int(x) for x in cpp_array(src, "column_indices%d" % r)],
             "rowstart_positions": [int(x) for x in cpp_array(src, "rowstart_positions%d" % r)],
             "periodic_pairs": [int(x) for x in cpp_array(src, "periodic_pairs%d" % r)],
-        })
+        }
+        # the assembled Stokes system of utest-lss-distributed-matrix.cpp:102-166: values of the owned
+        # rows before and after the Dirichlet and periodic conditions, rhs / solution of all local
+        # nodes, the conditions (local node indices) and the reference solution
+        for name in ("global_numbering", "irank_updatable", "bc_node", "bc_eqn"):
+            rk[name] = [int(x) for x in cpp_array(src, "%s%d" % (name, r))]
+        for name in ("mat_prebc", "rhs_prebc", "sol_prebc", "mat_presolve", "rhs_presolve", "sol_presolve",
+                     "result"):
+            rk[name] = [float(x) for x in cpp_array(src, "%s%d" % (name, r))]
+        rk["bc_value"] = [macros[x] if x in macros else float(x) for x in cpp_array(src, "bc_value%d" % r)]
+        ranks.append(rk)
     out["lss_test_matrix"] = {"nbeqs": 3, "element_nodes_per_elem": 3, "ranks": ranks}
     # ---- symmetric Dirichlet: gids 0..3 chain, neq 1, set_row(i,0,2,1), BC on gid 1 value 10 ----
     out["symmetric_dirichlet"] = {
