@@ -308,7 +308,7 @@ int gauss(int et, int order, Real* xi, Real* w)
 // ---------------------------------------------------------------------------------------------
 // Tau (SUPG.hpp:137-219).  u: nodal values (k x dim) of the velocity used for tau.
 // ---------------------------------------------------------------------------------------------
-struct TauCfg { int type; Real c1, c2, a_ps, a_su, a_bu; };
+struct TauCfg { int type; Real c1, c2, a_ps, a_su, a_bu, u_ref; };
 
 int compute_tau(int et, const TauCfg& cfg, const Real* X, const Real* u, Real element_nu, Real dt,
                 Real& tau_ps, Real& tau_su, Real& tau_bulk)
@@ -372,8 +372,43 @@ int compute_tau(int et, const TauCfg& cfg, const Real* X, const Real* u, Real el
     tau_ps = tau_su;
     const Real tau_su_std = 1. / std::sqrt((4./(dt*dt)) + tau_adv_sq + 16.*tau_diff);
     tau_bulk = (1./tau_su_std) / tr;
+  } else if(cfg.type == ORC_SUPG_CF2) {
+    // SUPG.hpp:191-210.  he from the element volume with pi as written there (3.141592654)
+    const Real vol = volume(et, X);
+    const Real he = dim == 2 ? std::sqrt(4./3.141592654*vol) : std::pow(3./4./3.141592654*vol, 1./3.);
+    const Real ree = cfg.u_ref*he/(2.*element_nu);
+    if(!(ree > 0.)) return -3;   // cf3_assert(ree > 0.)
+    const Real xi_ = ree < 3. ? ree/3. : 1.;
+    tau_ps = he*xi_/(2.*cfg.u_ref);
+    tau_bulk = he*cfg.u_ref/xi_;
+    tau_su = 0.;
+    Real umag2 = 0.;
+    for(int d = 0; d < dim; ++d) umag2 += uc[d]*uc[d];
+    const Real umag = std::sqrt(umag2);
+    if(umag > 1e-10) {
+      // (u.eval() * u.nabla()).sum(): u at the centroid times the shape-function gradients there,
+      // summed over the nodes (zero up to rounding: the gradients sum to zero)
+      Real J[9], Ji[9], det, dN[24];
+      jacobian(et, xi, X, J);
+      if(!inverse_det(dim, J, Ji, det)) return -2;
+      shape_gradients(et, xi, dN);
+      Real s = 0.;
+      for(int n = 0; n < k; ++n) {
+        Real un = 0.;
+        for(int d = 0; d < dim; ++d) {
+          Real gdn = 0.;
+          for(int l = 0; l < dim; ++l) gdn += Ji[d*dim+l] * dN[l*k+n];
+          un += uc[d] * gdn;
+        }
+        s += un;
+      }
+      const Real h = 2.*umag / s;
+      const Real tau_adv = h/(2.*umag);
+      const Real tau_time = 0.5*dt;
+      const Real tau_diff = h*h/(4.*element_nu);
+      tau_su = 1./(1./tau_adv + 1./tau_time + 1./tau_diff);
+    }
   } else {
-    // CF2 divides by sum_n(u . grad N_n), identically zero by the partition of unity
     return -3;
   }
   tau_ps *= cfg.a_ps;
@@ -405,7 +440,7 @@ int supg_specialized_tri(const Real* nodes, const Real* u, const Real* u_adv, co
   nrm[2][1] = ND(1,0) - ND(0,0);
 #undef ND
   Real tau_ps, tau_su, tau_bulk;
-  const TauCfg cfg = {ORC_SUPG_METRIC, 1., 16., 1., 1., 1.};
+  const TauCfg cfg = {ORC_SUPG_METRIC, 1., 16., 1., 1., 1., 1.};
   const int err = compute_tau(ORC_TRIAG2D, cfg, nodes, u, nu, dt, tau_ps, tau_su, tau_bulk);
   if(err) return err;
   tau3[0] = tau_ps; tau3[1] = tau_su; tau3[2] = tau_bulk;
@@ -486,7 +521,7 @@ int supg_specialized_tet(const Real* nodes, const Real* u, const Real* u_adv, co
   n[3][2] = ((ND(1,0)-ND(0,0))*(ND(2,1)-ND(0,1)) - (ND(2,0)-ND(0,0))*(ND(1,1)-ND(0,1))) / 2.0;
 #undef ND
   Real tau_ps, tau_su, tau_bulk;
-  const TauCfg cfg = {ORC_SUPG_METRIC, 1., 16., 1., 1., 1.};
+  const TauCfg cfg = {ORC_SUPG_METRIC, 1., 16., 1., 1., 1., 1.};
   const int err = compute_tau(ORC_TETRA3D, cfg, nodes, u, nu, dt, tau_ps, tau_su, tau_bulk);
   if(err) return err;
   tau3[0] = tau_ps; tau3[1] = tau_su; tau3[2] = tau_bulk;
@@ -633,7 +668,7 @@ int generic_element(int et, const TauCfg& cfg, Real dt, Real rho, const Real* X,
 
 TauCfg tau_cfg(const orc_ns_params* p)
 {
-  TauCfg c = {p->supg_type, p->c1, p->c2, p->alpha_ps, p->alpha_su, p->alpha_bu};
+  TauCfg c = {p->supg_type, p->c1, p->c2, p->alpha_ps, p->alpha_su, p->alpha_bu, p->u_ref};
   return c;
 }
 

@@ -33,6 +33,7 @@ typedef struct orc_ns_params {
   int supg_type;       /* ORC_SUPG_*, default METRIC */
   double c1, c2;       /* METRIC constants (1, 16) */
   double alpha_ps, alpha_su, alpha_bu; /* tau scale factors (1,1,1) */
+  double u_ref;        /* CF2 reference velocity (ComputeTau "u_ref", default 1) */
 } orc_ns_params;
 
 /* ---------------- PRNG and synthetic meshes ---------------- */

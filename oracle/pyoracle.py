@@ -26,11 +26,12 @@ class NSParams(C.Structure):
     _fields_ = [("rho", C.c_double), ("dt", C.c_double), ("theta", C.c_double),
                 ("use_specializations", C.c_int), ("supg_type", C.c_int),
                 ("c1", C.c_double), ("c2", C.c_double),
-                ("alpha_ps", C.c_double), ("alpha_su", C.c_double), ("alpha_bu", C.c_double)]
+                ("alpha_ps", C.c_double), ("alpha_su", C.c_double), ("alpha_bu", C.c_double),
+                ("u_ref", C.c_double)]
 
 
-def ns_params(rho=1.0, dt=0.01, theta=1.0, use_specializations=True, supg_type=SUPG_METRIC):
-    return NSParams(rho, dt, theta, int(use_specializations), supg_type, 1.0, 16.0, 1.0, 1.0, 1.0)
+def ns_params(rho=1.0, dt=0.01, theta=1.0, use_specializations=True, supg_type=SUPG_METRIC, u_ref=1.0):
+    return NSParams(rho, dt, theta, int(use_specializations), supg_type, 1.0, 16.0, 1.0, 1.0, 1.0, u_ref)
 
 
 class SolveStats(C.Structure):

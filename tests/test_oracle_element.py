@@ -15,7 +15,7 @@ The reference's only comparison test of these kernels is commented out
 import numpy as np
 import pytest
 
-from oracle.pyoracle import HEXA3D, QUAD2D, SUPG_METRIC, SUPG_TEZDUYAR, TETRA3D, TRIAG2D, ns_params
+from oracle.pyoracle import HEXA3D, QUAD2D, SUPG_CF2, SUPG_METRIC, SUPG_TEZDUYAR, TETRA3D, TRIAG2D, ns_params
 
 
 def p1_grads(nodes):
@@ -423,3 +423,39 @@ def test_tau_metric_sheared_hex(oracle):
     ts, tb = _metric_tau(g, uc, nu, dt)
     assert tau[1] == pytest.approx(ts, rel=1e-13) and tau[2] == pytest.approx(tb, rel=1e-13)
     assert abs(_metric_tau(4 * Binv.T @ Binv, uc, nu, dt)[0] - ts) > 1e-6 * ts
+
+
+@pytest.mark.parametrize("etype", [TETRA3D, TRIAG2D, HEXA3D, QUAD2D])
+@pytest.mark.parametrize("nu", [0.02, 0.1])
+def test_tau_cf2_closed_forms(oracle, etype, nu):
+    """CF2 tau (SUPG.hpp:191-210) on a reference-shaped element: he from the volume (pi written as
+    3.141592654 there), Re_e = u_ref he / (2 nu), xi = min(Re_e / 3, 1), tau_ps = he xi / (2 u_ref),
+    tau_bulk = he u_ref / xi; tau_su = 1 / (1/tau_adv + 2/dt + 1/tau_diff) with h = 2|u| / sum_n u.grad N_n,
+    where the gradients sum to zero, so tau_su is dt / 2 up to rounding; and 0 for |u| <= 1e-10."""
+    pi = 3.141592654
+    k, dim = {TETRA3D: (4, 3), TRIAG2D: (3, 2), HEXA3D: (8, 3), QUAD2D: (4, 2)}[etype]
+    h = 0.1
+    if etype == TETRA3D:
+        X = np.array([[0, 0, 0], [1, 0, 0], [0, 1, 0], [0, 0, 1.0]]) * h
+        V = h ** 3 / 6
+    elif etype == TRIAG2D:
+        X = np.array([[0, 0], [1, 0], [0, 1.0]]) * h
+        V = h ** 2 / 2
+    elif etype == HEXA3D:
+        X = (_LOC + 1) / 2 * h
+        V = h ** 3
+    else:
+        X = (_QLOC + 1) / 2 * h
+        V = h ** 2
+    he = np.sqrt(4 / pi * V) if dim == 2 else (3 / 4 / pi * V) ** (1 / 3)
+    ree = he / (2 * nu)
+    xi = min(ree / 3, 1.0)
+    dt = 0.01
+    prm = ns_params(dt=dt, use_specializations=False, supg_type=SUPG_CF2)
+    u = np.tile([0.7, -0.4, 0.2][:dim], (k, 1))
+    _, _, _, tau = oracle.ns_element(etype, prm, X, u, u, np.full(k, nu))
+    assert tau[0] == pytest.approx(he * xi / 2, rel=1e-14)
+    assert tau[2] == pytest.approx(he / xi, rel=1e-14)
+    assert tau[1] == pytest.approx(dt / 2, rel=1e-13)
+    _, _, _, tau0 = oracle.ns_element(etype, prm, X, 0 * u, 0 * u, np.full(k, nu))
+    assert tau0[1] == 0.0 and tau0[0] == tau[0] and tau0[2] == tau[2]
