@@ -116,6 +116,7 @@ def lib():
         "cf3hip_lss_periodicity": [_vp, C.c_uint32, _u32, _u32],
         "cf3hip_lss_solve": [_vp, C.POINTER(SolverParams), C.POINTER(SolveStats)],
         "cf3hip_lss_residual": [_vp, _d],
+        "cf3hip_lss_residual_noalloc": [_vp, _d],
         "cf3hip_fields_create": [_vp, C.POINTER(_vp)],
         "cf3hip_fields_destroy": [_vp],
         "cf3hip_fields_set": [_vp, C.c_int, _d],
@@ -433,6 +434,11 @@ class System:
         check(lib().cf3hip_lss_residual(self.h, C.byref(r)))
         return r.value
 
+    def residual_noalloc(self):
+        """The same residual computed without device allocation (fixed per-GPU scratch)."""
+        r = C.c_double()
+        check(lib().cf3hip_lss_residual_noalloc(self.h, C.byref(r)))
+        return r.value
 
     def assemble(self, fields, prm):
         check(lib().cf3hip_ns_assemble(self.h, fields.h, C.byref(prm)))

@@ -166,6 +166,7 @@ int cf3hip_context_create(int device, cf3hip_context* out)
   HIP_OK(hipGetDeviceCount(&n));
   REQUIRE(device >= 0 && device < n, CF3HIP_ERR_INVALID_ARG, "no such device");
   HIP_OK(hipSetDevice(device));
+  noalloc_init(device);   // device address of this GPU's scratch of the no-allocation entry points
   auto* h = new cf3hip_context_s();
   h->c.device = device;
   h->c.stream = nullptr;
@@ -926,6 +927,14 @@ int cf3hip_lss_residual(cf3hip_lss lss, double* rel)
   API_BEGIN
   REQUIRE(lss && rel, CF3HIP_ERR_INVALID_ARG, "null argument");
   *rel = lss_residual(lss->l);
+  API_END
+}
+
+int cf3hip_lss_residual_noalloc(cf3hip_lss lss, double* rel)
+{
+  API_BEGIN
+  REQUIRE(lss && rel, CF3HIP_ERR_INVALID_ARG, "null argument");
+  *rel = lss_residual_noalloc(lss->l);
   API_END
 }
 

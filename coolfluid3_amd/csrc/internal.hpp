@@ -221,6 +221,9 @@ void lss_spmv(Lss& L, double alpha, const double* x, double beta, double* y, con
 void lss_spmv32(Lss& L, float alpha, const float* x, float beta, float* y, const double* stop = nullptr);
 void lss_dirichlet(Lss& L, uint32_t nbc, const uint32_t* nodes, const uint32_t* eqs, const double* values);
 void lss_tie(Lss& L, uint32_t npairs, const uint32_t* to, const uint32_t* from, bool system);
+// noalloc.hip: the residual without device allocation (scratch address per device set at context creation)
+void noalloc_init(int device);
+double lss_residual_noalloc(Lss& L);
 void lss_point_csr(Lss& L, uint32_t* row_ptr_dev, uint32_t* col_dev, double* val_dev);
 void lss_diagonal(Lss& L, const double* d_dev, int mode);            // mode 0 set, 1 add
 void lss_column_zero(Lss& L, uint32_t col, int ieq, double* out_dev);

@@ -224,6 +224,12 @@ typedef struct cf3hip_solve_stats {
 int cf3hip_lss_solve(cf3hip_lss lss, const cf3hip_solver_params* prm, cf3hip_solve_stats* stats);
 /* ||rhs - A sol||_2 / ||rhs||_2 (TrilinosStratimikosStrategy::compute_residual) */
 int cf3hip_lss_residual(cf3hip_lss lss, double* rel);
+/* The same residual without any device allocation.  No counterpart in the reference: for callers that
+ * cannot allocate (code under HIP graph capture, tight serving loops, harnesses that forbid hipMalloc).
+ * Partial sums live in a fixed per-GPU device array (2 x 2048 + 8 doubles, held until the process
+ * ends), whatever the system size; systems of 2^32 dofs or more and neq above 4 are refused
+ * (CF3HIP_ERR_LIMIT).  Calls that overlap on one GPU share that array and race. */
+int cf3hip_lss_residual_noalloc(cf3hip_lss lss, double* rel);
 
 /* ---- UFEM NavierStokes fields (tags navier_stokes_solution, linearized_velocity,
  *      navier_stokes_viscosity, body_force; ns_implementation/NavierStokes.cpp:49-68) ---- */
