@@ -42,11 +42,13 @@ class SolveStats(C.Structure):
 class NSParams(C.Structure):
     _fields_ = [("rho", C.c_double), ("dt", C.c_double), ("theta", C.c_double),
                 ("use_specializations", C.c_int), ("supg_type", C.c_int), ("c1", C.c_double), ("c2", C.c_double),
-                ("alpha_ps", C.c_double), ("alpha_su", C.c_double), ("alpha_bu", C.c_double), ("precision", C.c_int)]
+                ("alpha_ps", C.c_double), ("alpha_su", C.c_double), ("alpha_bu", C.c_double), ("precision", C.c_int),
+                ("u_ref", C.c_double)]
 
 
-def ns_params(rho=1.0, dt=0.01, theta=1.0, use_specializations=True, supg_type=SUPG_METRIC, precision=FP64):
-    return NSParams(rho, dt, theta, int(use_specializations), supg_type, 1.0, 16.0, 1.0, 1.0, 1.0, precision)
+def ns_params(rho=1.0, dt=0.01, theta=1.0, use_specializations=True, supg_type=SUPG_METRIC, precision=FP64,
+              u_ref=1.0):
+    return NSParams(rho, dt, theta, int(use_specializations), supg_type, 1.0, 16.0, 1.0, 1.0, 1.0, precision, u_ref)
 
 
 def solver_params(method=BICGSTAB_JACOBI, tol=1e-10, max_iter=5000, restart=30, precision=FP64, inner_tol=1e-2,

@@ -78,25 +78,6 @@ struct SpecP1 {
     }
   }
 
-  // Tetra3D::volume (cf3/mesh/LagrangeP1/Tetra3D.cpp:104-128, 205-208), the volume the specialised
-  // element terms use (NavierStokesSpecializations.hpp:156): the 24-term expansion of det J in absolute
-  // coordinates, with the reference's products, sums and final division in their source order and no
-  // contraction into FMAs.  Its rounding error grows like |x|^3 / V (about 1e-11 relative at 64^3 cells
-  // of the unit cube): computed any other way, the matrix would differ from the reference's by that much.
-  __device__ __forceinline__ static double tet_volume_ref(const double X[4][3])
-  {
-#pragma clang fp contract(off)
-    const double x0 = X[0][0], y0 = X[0][1], z0 = X[0][2];
-    const double x1 = X[1][0], y1 = X[1][1], z1 = X[1][2];
-    const double x2 = X[2][0], y2 = X[2][1], z2 = X[2][2];
-    const double x3 = X[3][0], y3 = X[3][1], z3 = X[3][2];
-    return (x2*y1*z0 - x3*y1*z0 - x1*y2*z0 + x3*y2*z0 + x1*y3*z0 -
-            x2*y3*z0 - x2*y0*z1 + x3*y0*z1 + x0*y2*z1 - x3*y2*z1 -
-            x0*y3*z1 + x2*y3*z1 + x1*y0*z2 - x3*y0*z2 - x0*y1*z2 +
-            x3*y1*z2 + x0*y3*z2 - x1*y3*z2 - x1*y0*z3 + x2*y0*z3 +
-            x0*y1*z3 - x2*y1*z3 - x0*y2*z3 + x1*y2*z3) / 6.;
-  }
-
   // METRIC tau from the velocity at the centroid (SUPG.hpp:176-190); gij = n_i.n_l / (D^2 V^2).
   // s = 1 / (D V).  c1 = 1, c2 = 16, alpha = 1 on the specialised path.
   template<typename T>
@@ -193,8 +174,8 @@ struct SpecP1 {
       V = (R)V64; invV = (R)iv; tsu = (R)ts; tbu = (R)tb;
       if(!(V64 > 0.)) g.status[0] = 1;   // every writer stores the same flag value
       if constexpr(D == 3 && sizeof(R) == sizeof(double)) {
-        // the element terms take the reference's volume; tau above takes det J of the Jacobian, as the
-        // reference's ComputeTau does (SUPG.hpp:178, ElementData.hpp:302-314)
+        // the element terms take the reference's volume (NavierStokesSpecializations.hpp:156), tau above
+        // det J of the Jacobian, as the reference's ComputeTau does (SUPG.hpp:178, ElementData.hpp:302-314)
         const double Vr = tet_volume_ref(X);
         V = Vr;
         invV = 1. / Vr;
@@ -413,7 +394,31 @@ struct GenericP1 {
       for(int m = 0; m < K; ++m) s += ua_n[m][d];
       uc[d] = s * (1.0 / K);
     }
-    if(g.supg_type == CF3HIP_SUPG_METRIC) {
+    if(g.supg_type == CF3HIP_SUPG_CF2) {
+      // volume as the reference's etype::volume: Tetra3D's 24-term expansion, Triag2D's 0.5 det J
+      double vol;
+      if constexpr(D == 3) {
+        double Xv[4][3];
+#pragma unroll
+        for(int m = 0; m < K; ++m)
+#pragma unroll
+          for(int d = 0; d < D; ++d) Xv[m][d] = X[m][d];
+        vol = tet_volume_ref(Xv);
+      } else {
+        vol = 0.5 * det;
+      }
+      R um2 = 0., s = 0.;
+#pragma unroll
+      for(int d = 0; d < D; ++d) um2 += uc[d] * uc[d];
+#pragma unroll
+      for(int m = 0; m < K; ++m) {
+        R un = 0.;
+#pragma unroll
+        for(int d = 0; d < D; ++d) un += uc[d] * G[d][m];
+        s += un;
+      }
+      if(!tau_cf2(D, vol, nu_e, sqrt(um2), s, g.dt, g.u_ref, tps, tsu, tbu)) g.status[0] = 2;
+    } else if(g.supg_type == CF3HIP_SUPG_METRIC) {
       R gij[D][D];
 #pragma unroll
       for(int i = 0; i < D; ++i)
@@ -939,9 +944,8 @@ AsmArgs make_args(Lss& L, Fields& F, const cf3hip_ns_params& prm, bool pack = tr
   if(F.mesh != L.mesh) CF3_THROW(CF3HIP_ERR_INVALID_ARG, "fields belong to another mesh");
   if(L.neq != M.dim + 1) CF3_THROW(CF3HIP_ERR_INVALID_ARG, "the NS system needs neq = dim + 1");
   const bool spec = prm.use_specializations && !is_q1(M.etype);
-  if(!spec && prm.supg_type == CF3HIP_SUPG_CF2)
-    CF3_THROW(CF3HIP_ERR_NOT_IMPLEMENTED,
-              "supg_type cf2 divides by sum_n u.grad(N_n), identically zero by the partition of unity");
+  if(!spec && prm.supg_type == CF3HIP_SUPG_CF2 && !(prm.u_ref > 0.))
+    CF3_THROW(CF3HIP_ERR_INVALID_ARG, "supg_type cf2 needs a positive u_ref (SUPG.hpp:196)");
   AsmArgs g = {};
   g.body_force = F.has_g ? 1 : 0;
   g.xp = M.xp.as<double4>();
@@ -966,6 +970,7 @@ AsmArgs make_args(Lss& L, Fields& F, const cf3hip_ns_params& prm, bool pack = tr
   g.invdt = 1. / prm.dt; g.f4dt2 = 4. / (prm.dt * prm.dt);
   g.supg_type = prm.supg_type;
   g.c1 = prm.c1; g.c2 = prm.c2; g.a_ps = prm.alpha_ps; g.a_su = prm.alpha_su; g.a_bu = prm.alpha_bu;
+  g.u_ref = prm.u_ref;
   g.status = L.ctx->status_flag();
   if(pack) pack_fields(F);
   return g;
@@ -977,6 +982,7 @@ void check_status(Context& c)
   int st = 0;
   HIP_OK(hipMemcpyAsync(&st, c.status_flag(), sizeof(int), hipMemcpyDeviceToHost, c.stream));
   HIP_OK(hipStreamSynchronize(c.stream));
+  if(st == 2) CF3_THROW(CF3HIP_ERR_INVALID_ARG, "ComputeTau cf2: element Reynolds number u_ref he / (2 nu) not positive");
   if(st) CF3_THROW(CF3HIP_ERR_SINGULAR, "non-positive element Jacobian determinant");
 }
 

@@ -34,8 +34,51 @@ struct AsmArgs {
   int supg_type;
   int body_force;      // nonzero when the body force field holds a nonzero value
   double c1, c2, a_ps, a_su, a_bu;
-  int* status;
+  double u_ref;        // CF2 reference velocity
+  int* status;         // 1: non-positive Jacobian determinant, 2: CF2 element Reynolds number not positive
 };
+
+// ComputeTau CF2 (plugins/UFEM/src/UFEM/SUPG.hpp:191-210).  vol: the element volume the reference takes
+// (etype::volume); um: |u| at the centroid; s: sum over the nodes of u . grad N_n at the centroid (zero
+// up to rounding, since the gradients sum to zero: tau_su then comes out as dt / 2 to within rounding).
+// pi is the reference's 3.141592654.  Returns false when Re_e = u_ref he / (2 nu) is not positive (the
+// reference's cf3_assert).
+__device__ __forceinline__ bool tau_cf2(int dim, double vol, double nu_e, double um, double s, double dt,
+                                        double u_ref, double& tps, double& tsu, double& tbu)
+{
+  const double he = dim == 2 ? sqrt(4. / 3.141592654 * vol) : pow(3. / 4. / 3.141592654 * vol, 1. / 3.);
+  const double ree = u_ref * he / (2. * nu_e);
+  const double xi = ree < 3. ? ree / 3. : 1.;
+  tps = he * xi / (2. * u_ref);
+  tbu = he * u_ref / xi;
+  tsu = 0.;
+  if(um > 1e-10) {
+    const double h = 2. * um / s;
+    const double tau_adv = h / (2. * um);
+    const double tau_time = 0.5 * dt;
+    const double tau_diff = h * h / (4. * nu_e);
+    tsu = 1. / (1. / tau_adv + 1. / tau_time + 1. / tau_diff);
+  }
+  return ree > 0.;
+}
+
+// Tetra3D::volume (cf3/mesh/LagrangeP1/Tetra3D.cpp:104-128, 205-208): the 24-term expansion of det J
+// in absolute coordinates, with the reference's products, sums and final division in their source order
+// and no contraction into FMAs.  Its rounding error grows like |x|^3 / V (about 1e-11 relative for
+// 64^3 cells of the unit cube): computed any other way, results would differ from the reference's by that.
+__device__ __forceinline__ double tet_volume_ref(const double X[4][3])
+{
+#pragma clang fp contract(off)
+  const double x0 = X[0][0], y0 = X[0][1], z0 = X[0][2];
+  const double x1 = X[1][0], y1 = X[1][1], z1 = X[1][2];
+  const double x2 = X[2][0], y2 = X[2][1], z2 = X[2][2];
+  const double x3 = X[3][0], y3 = X[3][1], z3 = X[3][2];
+  return (x2*y1*z0 - x3*y1*z0 - x1*y2*z0 + x3*y2*z0 + x1*y3*z0 -
+          x2*y3*z0 - x2*y0*z1 + x3*y0*z1 + x0*y2*z1 - x3*y2*z1 -
+          x0*y3*z1 + x2*y3*z1 + x1*y0*z2 - x3*y0*z2 - x0*y1*z2 +
+          x3*y1*z2 + x0*y3*z2 - x1*y3*z2 - x1*y0*z3 + x2*y0*z3 +
+          x0*y1*z3 - x2*y1*z3 - x0*y2*z3 + x1*y2*z3) / 6.;
+}
 
 template<int B, int K, typename F>
 __device__ __forceinline__ void static_for(F&& f)

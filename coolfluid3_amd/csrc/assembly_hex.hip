@@ -169,7 +169,7 @@ __global__ __launch_bounds__(kThreads) void k_q1(AsmArgs g)
 #pragma unroll
     for(int d = 0; d < D; ++d) { uq[d] = 0.; uc[d] = 0.; gq[d] = 0.; }
     double hmin2 = 1e20;
-    double Xall[SUPG == CF3HIP_SUPG_TEZDUYAR ? K : 1][D];
+    double Xall[SUPG == CF3HIP_SUPG_METRIC ? 1 : K][D];
 #pragma unroll
     for(int m = 0; m < K; ++m) {
       double xp[4], ad[4];
@@ -197,7 +197,7 @@ __global__ __launch_bounds__(kThreads) void k_q1(AsmArgs g)
 #pragma unroll
         for(int d = 0; d < D; ++d) gq[d] += Nm * g.g[d][cn[m]];
       }
-      if constexpr(SUPG == CF3HIP_SUPG_TEZDUYAR) {
+      if constexpr(SUPG != CF3HIP_SUPG_METRIC) {
 #pragma unroll
         for(int d = 0; d < D; ++d) Xall[m][d] = xp[d];
       }
@@ -232,6 +232,34 @@ __global__ __launch_bounds__(kThreads) void k_q1(AsmArgs g)
       tsu = 1. / sqrt(g.c1 * g.c1 * g.f4dt2 + fabs(ugu) + g.c2 * diff);
       tps = tsu;
       tbu = sqrt(g.f4dt2 + fabs(ugu) + 16. * diff) / tr;
+    } else if constexpr(SUPG == CF3HIP_SUPG_CF2) {
+      // volume as the reference's etype::volume: Hexa3D 8 det J at the centre (Hexa3D.cpp:317-320),
+      // Quad2D half the cross product of the diagonals
+      double Jci[D * D];
+      const double detc = inverse<D>(Jc, Jci);
+      double vol;
+      if constexpr(D == 3) vol = 8. * detc;
+      else vol = 0.5 * ((Xall[2][0] - Xall[0][0]) * (Xall[3][1] - Xall[1][1]) -
+                        (Xall[2][1] - Xall[0][1]) * (Xall[3][0] - Xall[1][0]));
+      // sum over the nodes of u_c . grad N_n at the centre, grad N_n = J^-1 dN_n/dxi (dN_n/dxi_k = s_k / 2^D)
+      double um2 = 0., sum = 0.;
+#pragma unroll
+      for(int d = 0; d < D; ++d) um2 += uc[d] * uc[d];
+#pragma unroll
+      for(int m = 0; m < K; ++m) {
+        double sg[3];
+        node_signs(m, sg);
+        double un = 0.;
+#pragma unroll
+        for(int d = 0; d < D; ++d) {
+          double gdn = 0.;
+#pragma unroll
+          for(int l = 0; l < D; ++l) gdn += Jci[d * D + l] * (sg[l] * Q::kInvK);
+          un += uc[d] * gdn;
+        }
+        sum += un;
+      }
+      if(!tau_cf2(D, vol, nu_e, sqrt(um2), sum, g.dt, g.u_ref, tps, tsu, tbu)) g.status[0] = 2;
     } else {
 #pragma unroll
       for(int i = 0; i < K; ++i)
@@ -467,6 +495,8 @@ void launch_mode(const AsmArgs& g, const Lss& L)
   timer_begin(c, timer);
   if(g.supg_type == CF3HIP_SUPG_TEZDUYAR)
     hipLaunchKernelGGL((k_q1<D, MODE, CF3HIP_SUPG_TEZDUYAR>), dim3(L.n_wg), dim3(kThreads), 0, c.stream, g);
+  else if(g.supg_type == CF3HIP_SUPG_CF2)
+    hipLaunchKernelGGL((k_q1<D, MODE, CF3HIP_SUPG_CF2>), dim3(L.n_wg), dim3(kThreads), 0, c.stream, g);
   else
     hipLaunchKernelGGL((k_q1<D, MODE, CF3HIP_SUPG_METRIC>), dim3(L.n_wg), dim3(kThreads), 0, c.stream, g);
   HIP_OK(hipGetLastError());

@@ -664,6 +664,8 @@ SiArgs si_args(SemiImplicit& S, Fields& F, const cf3hip_ns_params& prm)
   g.dps = S.dps.as<double>();
   g.dp = S.p->sol.as<double>();
   g.theta = prm.theta;
+  if(prm.supg_type != CF3HIP_SUPG_METRIC && prm.supg_type != CF3HIP_SUPG_TEZDUYAR)
+    CF3_THROW(CF3HIP_ERR_NOT_IMPLEMENTED, "the semi-implicit actions take supg_type metric or tezduyar");
   g.t = TauP{prm.supg_type, prm.dt, prm.c1, prm.c2, prm.alpha_ps, prm.alpha_su, prm.alpha_bu};
   g.stage = S.stage.as<double>();
   return g;

@@ -517,6 +517,7 @@ public:
   double density = 1.2;              // NavierStokesPhysics "density"
   double dt = 0.;                    // LSSActionUnsteady "dt"
   double c1 = 1., c2 = 16., alpha_ps = 1., alpha_su = 1., alpha_bu = 1.;
+  double u_ref = 1.;                 // ComputeTau "u_ref" (SUPG.hpp:103), used by supg_type cf2
   int precision = CF3HIP_FP64;       // FP32: element matrices of the precision form
 
   cf3hip_ns_params params() const
@@ -535,6 +536,7 @@ public:
     p.c1 = c1; p.c2 = c2;
     p.alpha_ps = alpha_ps; p.alpha_su = alpha_su; p.alpha_bu = alpha_bu;
     p.precision = precision;
+    p.u_ref = u_ref;
     return p;
   }
   // LinearizeU

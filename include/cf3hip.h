@@ -45,7 +45,8 @@ extern "C" {
 /* SUPG tau variants (plugins/UFEM/src/UFEM/SUPG.hpp:62-74) */
 #define CF3HIP_SUPG_TEZDUYAR 0
 #define CF3HIP_SUPG_METRIC   1
-#define CF3HIP_SUPG_CF2      2
+#define CF3HIP_SUPG_CF2      2   /* generic expression only; the specialised tri/tet kernels always use METRIC,
+                                     as the reference's specialisations do (NavierStokesSpecializations.hpp:57-58) */
 
 /* value precision of an assembled matrix / Krylov solve */
 #define CF3HIP_FP64 0
@@ -263,6 +264,8 @@ typedef struct cf3hip_ns_params {
   double c1, c2;            /* METRIC constants (1, 16) */
   double alpha_ps, alpha_su, alpha_bu;
   int precision;            /* CF3HIP_FP64 (drop-in) or CF3HIP_FP32 (element arithmetic and values) */
+  double u_ref;             /* ComputeTau "u_ref" (SUPG.hpp:103), reference velocity of supg_type cf2;
+                               must be positive when cf2 is selected */
 } cf3hip_ns_params;
 /* LinearizeU node loop (NavierStokes.cpp:105) */
 int cf3hip_ns_linearize(cf3hip_fields fields);

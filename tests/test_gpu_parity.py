@@ -162,7 +162,9 @@ def test_sparsity_and_slot_map_bit_exact(ctx, oracle, name):
     ("tet5", False, cabi.SUPG_TEZDUYAR, 1.0), ("tet12", True, cabi.SUPG_METRIC, 1.0),
     ("tet12s", True, cabi.SUPG_METRIC, 1.0),
     ("hex3", False, cabi.SUPG_METRIC, 1.0), ("hex5", False, cabi.SUPG_METRIC, 0.5),
-    ("hex3", False, cabi.SUPG_TEZDUYAR, 0.8)])
+    ("hex3", False, cabi.SUPG_TEZDUYAR, 0.8),
+    ("tri8", False, cabi.SUPG_CF2, 1.0), ("tet5", False, cabi.SUPG_CF2, 0.7), ("tet12s", False, cabi.SUPG_CF2, 1.0),
+    ("quad9", False, cabi.SUPG_CF2, 0.5), ("hex3", False, cabi.SUPG_CF2, 1.0), ("duct", False, cabi.SUPG_CF2, 1.0)])
 def test_assembly_fp64(ctx, oracle, name, spec, supg, theta):
     case = CASES[name]()
     print("case", name, spec, supg, theta, flush=True)
