@@ -36,6 +36,8 @@ struct AsmArgs {
   double c1, c2, a_ps, a_su, a_bu;
   double u_ref;        // CF2 reference velocity
   int* status;         // 1: non-positive Jacobian determinant, 2: CF2 element Reynolds number not positive
+  double* q1rec;       // Q1 cells: per element and Gauss point J^-1, det J, u_adv, nu (Q1Rec doubles)
+  double4* q1tau;      // Q1 cells: per element (tau_ps, tau_su, tau_bulk, 0)
 };
 
 // ComputeTau CF2 (plugins/UFEM/src/UFEM/SUPG.hpp:191-210).  vol: the element volume the reference takes
@@ -162,7 +164,7 @@ __device__ __forceinline__ void lds_load_row(const VR* src, VR v[NN])
 // Q1 cells: hexahedra and quadrilaterals, always the generic expression (assembly_hex.hip)
 inline bool is_q1(int etype) { return etype == CF3HIP_HEXA3D || etype == CF3HIP_QUAD2D; }
 // mode 0 = FP64 matrix + rhs, 1 = FP32 matrix, 2 = matrix-free residual
-void launch_hex(const AsmArgs& g, const Lss& L, int mode);
+void launch_hex(AsmArgs g, Lss& L, int mode);
 
 }  // namespace asmk
 }  // namespace cf3hip

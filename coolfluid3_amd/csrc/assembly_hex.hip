@@ -8,12 +8,14 @@
 // compute_gradient); Gauss rule: GaussImplementation.hpp:29-35 (x = 1/sqrt(3), w = 1), :566-600 (quad
 // point order) and :615-676 (hexahedron point order); combine: NavierStokesAssembly.hpp:69-71.
 //
-// Work decomposition (row owner, one staging pass): a workgroup owns a range of node rows; each of its
-// incidences (element e, local node a) gets kLanes = 2^D threads.  Lane q first computes the quantities
-// of Gauss point q that do not depend on the column node (J^-1, u_adv, nu, the row-node terms) and
-// stores them in LDS; lane b then integrates the element block M(a, b) over the 2^D points.  Blocks are
-// staged in LDS at rows sorted by target CSR block and summed by the block's accumulator threads in a
-// fixed order: deterministic, no floating-point atomics.
+// Work decomposition: a pre-pass (k_q1_rec, one thread per element and Gauss point; k_q1_tau, one thread
+// per element) computes each element's Gauss-point records (J^-1, det J, u_adv, nu at each of the 2^D
+// points) and its tau once, into device buffers of the system.  The row-owner kernel (k_q1, one staging pass) then gives each incidence (element e, local
+// node a) of its rows kLanes = 2^D threads: lane q reads the record of Gauss point q, forms the row-node
+// terms and stores them in LDS; lane b then integrates the element block M(a, b) over the 2^D points.
+// Blocks are staged in LDS at rows sorted by target CSR block and summed by the block's accumulator
+// threads in a fixed order: deterministic, no floating-point atomics.  (The element geometry is thus
+// computed once per element rather than once per incidence, i.e. 2^D times.)
 //
 // Per Gauss point the element block terms are (w = w_q det J_q, G = grad N, L = u_adv . G,
 // SU = N + tau_su L):
@@ -41,6 +43,8 @@ struct Q1 {
   static constexpr int kSlots = kThreads / kLanes;    // incidences per workgroup
   static constexpr int kRec = D * D + 2 * D + 3;      // doubles per (incidence, Gauss point) record
   static constexpr int kRecSlot = kLanes * kRec + 2;  // doubles per incidence (padded: slots on distinct LDS banks)
+  static constexpr int kQRec = D == 3 ? 16 : 8;       // doubles per (element, Gauss point) in the record buffer:
+                                                      // J^-1 (D x D), det J, u_adv (D), nu, padding
   static constexpr double kInvK = 1.0 / K;
 };
 
@@ -111,8 +115,187 @@ __device__ __forceinline__ double inverse(const double J[D * D], double Ji[D * D
   }
 }
 
-template<int D, int MODE, int SUPG>
-__global__ __launch_bounds__(kThreads) void k_q1(AsmArgs g)
+// Element pre-pass, records: one thread per (element e, Gauss point q), the 2^D threads of an element
+// adjacent; each streams the element's nodes once and writes the record of its point (J^-1, det J,
+// u_adv, nu).
+template<int D>
+__global__ __launch_bounds__(256) void k_q1_rec(uint32_t n_elems, AsmArgs g)
+{
+  typedef Q1<D> Q;
+  constexpr int K = Q::K, kQRec = Q::kQRec;
+  const uint64_t t = (uint64_t)blockIdx.x * 256u + threadIdx.x;
+  const uint64_t e = t / K;
+  const int q = (int)(t % K);
+  if(e >= n_elems) return;
+  uint32_t cn[K];
+#pragma unroll
+  for(int h = 0; h < K / 4; ++h) {
+    const uint4 c = reinterpret_cast<const uint4*>(g.conn)[(K / 4) * e + h];
+    cn[4 * h] = c.x; cn[4 * h + 1] = c.y; cn[4 * h + 2] = c.z; cn[4 * h + 3] = c.w;
+  }
+  double xq[3];
+  gauss_point<D>(q, xq);
+  double J[D * D], uq[D], nuq = 0.;
+#pragma unroll
+  for(int v = 0; v < D * D; ++v) J[v] = 0.;
+#pragma unroll
+  for(int d = 0; d < D; ++d) uq[d] = 0.;
+#pragma unroll
+  for(int m = 0; m < K; ++m) {
+    double xp[4], ad[4];
+    unpack(g.xp[cn[m]], xp);
+    unpack(g.ad[cn[m]], ad);
+    double sg[3];
+    node_signs(m, sg);
+    double Nm, dN[3];
+    shape<D>(sg, xq, Nm, dN);
+#pragma unroll
+    for(int k = 0; k < D; ++k)
+#pragma unroll
+      for(int d = 0; d < D; ++d) J[k * D + d] += dN[k] * xp[d];
+#pragma unroll
+    for(int d = 0; d < D; ++d) uq[d] += Nm * ad[d];
+    nuq += Nm * ad[3];
+  }
+  double Ji[D * D];
+  const double det = inverse<D>(J, Ji);
+  if(!(det > 0.)) g.status[0] = 1;   // every writer stores the same flag value
+  double r[kQRec];
+#pragma unroll
+  for(int v = 0; v < D * D; ++v) r[v] = Ji[v];
+  r[D * D] = det;
+#pragma unroll
+  for(int d = 0; d < D; ++d) r[D * D + 1 + d] = uq[d];
+  r[D * D + 1 + D] = nuq;
+#pragma unroll
+  for(int v = D * D + 2 + D; v < kQRec; ++v) r[v] = 0.;
+  double2* dst = reinterpret_cast<double2*>(g.q1rec + (e * K + q) * kQRec);
+#pragma unroll
+  for(int v = 0; v < kQRec / 2; ++v) dst[v] = make_double2(r[2 * v], r[2 * v + 1]);
+}
+
+// Element pre-pass, tau: one thread per element, tau at the element centre (SUPG.hpp:141-219) from J,
+// u_adv and nu there (the shape-function gradients at the centre are s_k / 2^D).
+template<int D, int SUPG>
+__global__ __launch_bounds__(256) void k_q1_tau(uint32_t n_elems, AsmArgs g)
+{
+  typedef Q1<D> Q;
+  constexpr int K = Q::K;
+  const uint64_t e = (uint64_t)blockIdx.x * 256u + threadIdx.x;
+  if(e >= n_elems) return;
+  uint32_t cn[K];
+#pragma unroll
+  for(int h = 0; h < K / 4; ++h) {
+    const uint4 c = reinterpret_cast<const uint4*>(g.conn)[(K / 4) * e + h];
+    cn[4 * h] = c.x; cn[4 * h + 1] = c.y; cn[4 * h + 2] = c.z; cn[4 * h + 3] = c.w;
+  }
+  double Jc[D * D], uc[D], nus = 0.;
+#pragma unroll
+  for(int v = 0; v < D * D; ++v) Jc[v] = 0.;
+#pragma unroll
+  for(int d = 0; d < D; ++d) uc[d] = 0.;
+  double X[SUPG == CF3HIP_SUPG_METRIC ? 1 : K][D];
+#pragma unroll
+  for(int m = 0; m < K; ++m) {
+    double xp[4], ad[4];
+    unpack(g.xp[cn[m]], xp);
+    unpack(g.ad[cn[m]], ad);
+    double sg[3];
+    node_signs(m, sg);
+#pragma unroll
+    for(int k = 0; k < D; ++k)
+#pragma unroll
+      for(int d = 0; d < D; ++d) Jc[k * D + d] += (sg[k] * Q::kInvK) * xp[d];
+#pragma unroll
+    for(int d = 0; d < D; ++d) uc[d] += Q::kInvK * ad[d];
+    nus += ad[3];
+    if constexpr(SUPG != CF3HIP_SUPG_METRIC) {
+#pragma unroll
+      for(int d = 0; d < D; ++d) X[m][d] = xp[d];
+    }
+  }
+  const double nu_e = fabs(nus * Q::kInvK);
+  double tps, tsu, tbu;
+  if constexpr(SUPG == CF3HIP_SUPG_METRIC) {
+    double Jci[D * D];
+    inverse<D>(Jc, Jci);
+    double gij[D * D];
+#pragma unroll
+    for(int i = 0; i < D; ++i)
+#pragma unroll
+      for(int j = 0; j < D; ++j) {
+        double s = 0.;
+#pragma unroll
+        for(int k = 0; k < D; ++k) s += Jci[k * D + i] * Jci[k * D + j];
+        gij[i * D + j] = s;
+      }
+    double ugu = 0., gsq = 0., tr = 0.;
+#pragma unroll
+    for(int j = 0; j < D; ++j) {
+      double s = 0.;
+#pragma unroll
+      for(int i = 0; i < D; ++i) s += uc[i] * gij[i * D + j];
+      ugu += s * uc[j];
+      tr += gij[j * (D + 1)];
+    }
+#pragma unroll
+    for(int v = 0; v < D * D; ++v) gsq += gij[v] * gij[v];
+    const double diff = nu_e * nu_e * gsq;
+    tsu = 1. / sqrt(g.c1 * g.c1 * g.f4dt2 + fabs(ugu) + g.c2 * diff);
+    tps = tsu;
+    tbu = sqrt(g.f4dt2 + fabs(ugu) + 16. * diff) / tr;
+  } else if constexpr(SUPG == CF3HIP_SUPG_CF2) {
+    // volume as the reference's etype::volume: Hexa3D 8 det J at the centre (Hexa3D.cpp:317-320),
+    // Quad2D half the cross product of the diagonals
+    double Jci[D * D];
+    const double detc = inverse<D>(Jc, Jci);
+    double vol;
+    if constexpr(D == 3) vol = 8. * detc;
+    else vol = 0.5 * ((X[2][0] - X[0][0]) * (X[3][1] - X[1][1]) - (X[2][1] - X[0][1]) * (X[3][0] - X[1][0]));
+    // sum over the nodes of u_c . grad N_n at the centre, grad N_n = J^-1 dN_n/dxi (dN_n/dxi_k = s_k / 2^D)
+    double um2 = 0., sum = 0.;
+#pragma unroll
+    for(int d = 0; d < D; ++d) um2 += uc[d] * uc[d];
+#pragma unroll
+    for(int m = 0; m < K; ++m) {
+      double sg[3];
+      node_signs(m, sg);
+      double un = 0.;
+#pragma unroll
+      for(int d = 0; d < D; ++d) {
+        double gdn = 0.;
+#pragma unroll
+        for(int l = 0; l < D; ++l) gdn += Jci[d * D + l] * (sg[l] * Q::kInvK);
+        un += uc[d] * gdn;
+      }
+      sum += un;
+    }
+    if(!tau_cf2(D, vol, nu_e, sqrt(um2), sum, g.dt, g.u_ref, tps, tsu, tbu)) g.status[0] = 2;
+  } else {
+    double hmin2 = 1e20;
+#pragma unroll
+    for(int i = 0; i < K; ++i)
+#pragma unroll
+      for(int j = i + 1; j < K; ++j) {
+        double s = 0.;
+#pragma unroll
+        for(int d = 0; d < D; ++d) { const double df = X[i][d] - X[j][d]; s += df * df; }
+        hmin2 = fmin(hmin2, s);
+      }
+    const double h = sqrt(hmin2);
+    double um2 = 0.;
+#pragma unroll
+    for(int d = 0; d < D; ++d) um2 += uc[d] * uc[d];
+    const double um = sqrt(um2);
+    tsu = 1. / ((2. * um) / h + 2. / g.dt + (4. * nu_e) / (h * h));
+    tps = tsu;
+    tbu = tsu * um * um;
+  }
+  g.q1tau[e] = make_double4(tps * g.a_ps, tsu * g.a_su, tbu * g.a_bu, 0.);
+}
+
+template<int D, int MODE>
+__global__ __launch_bounds__(kThreads, 4) void k_q1(AsmArgs g)
 {
   typedef Q1<D> Q;
   constexpr int K = Q::K, NEQ = Q::NEQ, NN = Q::NN, kLanes = Q::kLanes, kRec = Q::kRec, kRecSlot = Q::kRecSlot;
@@ -143,9 +326,9 @@ __global__ __launch_bounds__(kThreads) void k_q1(AsmArgs g)
   if(MODE != MODE_RESIDUAL && (uint32_t)t < nblk) s_acc[t] = g.blk_acc[blk0 + t];
 
   const bool active = (uint32_t)slot < ninc;
-  uint32_t cn[K];
   uint64_t e = 0;
   int a = 0;
+  uint32_t nb = 0, pos = 0;   // column node b = lane of the incidence's element, staging row of its block
   double tps = 0., tsu = 0., tbu = 0.;
   double rhs_part[NEQ];
 #pragma unroll
@@ -154,136 +337,27 @@ __global__ __launch_bounds__(kThreads) void k_q1(AsmArgs g)
     const uint32_t ea = g.inc[i0 + slot];
     e = ea / K;
     a = (int)(ea % K);
-#pragma unroll
-    for(int h = 0; h < K / 4; ++h) {
-      const uint4 c = reinterpret_cast<const uint4*>(g.conn)[(K / 4) * e + h];
-      cn[4 * h] = c.x; cn[4 * h + 1] = c.y; cn[4 * h + 2] = c.z; cn[4 * h + 3] = c.w;
-    }
-    // ---- stream over the nodes: J at Gauss point q = lane and at the centre, u_adv and nu ----
+    nb = g.conn[e * K + lane];
+    if(MODE != MODE_RESIDUAL) pos = g.inc_pos[(uint64_t)(i0 + slot) * kLanes + lane];
+    // ---- the pre-pass record of Gauss point q = lane, and the element's tau ----
     double xq[3];
     gauss_point<D>(lane, xq);
-    double J[D * D], Jc[D * D], uq[D], uc[D], nuq = 0., nus = 0.;
-    double gq[D];
+    double rq[Q::kQRec];
+    {
+      const double2* src = reinterpret_cast<const double2*>(g.q1rec + (e * K + lane) * Q::kQRec);
 #pragma unroll
-    for(int v = 0; v < D * D; ++v) { J[v] = 0.; Jc[v] = 0.; }
-#pragma unroll
-    for(int d = 0; d < D; ++d) { uq[d] = 0.; uc[d] = 0.; gq[d] = 0.; }
-    double hmin2 = 1e20;
-    double Xall[SUPG == CF3HIP_SUPG_METRIC ? 1 : K][D];
-#pragma unroll
-    for(int m = 0; m < K; ++m) {
-      double xp[4], ad[4];
-      unpack(g.xp[cn[m]], xp);
-      unpack(g.ad[cn[m]], ad);
-      double sg[3];
-      node_signs(m, sg);
-      double Nm, dN[3];
-      shape<D>(sg, xq, Nm, dN);
-#pragma unroll
-      for(int k = 0; k < D; ++k)
-#pragma unroll
-        for(int d = 0; d < D; ++d) {
-          J[k * D + d] += dN[k] * xp[d];
-          Jc[k * D + d] += (sg[k] * Q::kInvK) * xp[d];   // gradients at the centre: s_k / 2^D
-        }
-#pragma unroll
-      for(int d = 0; d < D; ++d) {
-        uq[d] += Nm * ad[d];
-        uc[d] += Q::kInvK * ad[d];
-      }
-      nuq += Nm * ad[3];
-      nus += ad[3];
-      if(MODE != MODE_FP32 && g.body_force) {
-#pragma unroll
-        for(int d = 0; d < D; ++d) gq[d] += Nm * g.g[d][cn[m]];
-      }
-      if constexpr(SUPG != CF3HIP_SUPG_METRIC) {
-#pragma unroll
-        for(int d = 0; d < D; ++d) Xall[m][d] = xp[d];
-      }
+      for(int v = 0; v < Q::kQRec / 2; ++v) { const double2 w2 = src[v]; rq[2 * v] = w2.x; rq[2 * v + 1] = w2.y; }
     }
-    // ---- tau at the centre (SUPG.hpp:141-219), identical on the 2^D lanes ----
-    const double nu_e = fabs(nus * Q::kInvK);
-    if constexpr(SUPG == CF3HIP_SUPG_METRIC) {
-      double Jci[D * D];
-      inverse<D>(Jc, Jci);
-      double gij[D * D];
+    const double4 tt = g.q1tau[e];
+    tps = tt.x; tsu = tt.y; tbu = tt.z;
+    double Ji[D * D], uq[D];
 #pragma unroll
-      for(int i = 0; i < D; ++i)
+    for(int v = 0; v < D * D; ++v) Ji[v] = rq[v];
+    const double det = rq[D * D];
 #pragma unroll
-        for(int j = 0; j < D; ++j) {
-          double s = 0.;
-#pragma unroll
-          for(int k = 0; k < D; ++k) s += Jci[k * D + i] * Jci[k * D + j];
-          gij[i * D + j] = s;
-        }
-      double ugu = 0., gsq = 0., tr = 0.;
-#pragma unroll
-      for(int j = 0; j < D; ++j) {
-        double s = 0.;
-#pragma unroll
-        for(int i = 0; i < D; ++i) s += uc[i] * gij[i * D + j];
-        ugu += s * uc[j];
-        tr += gij[j * (D + 1)];
-      }
-#pragma unroll
-      for(int v = 0; v < D * D; ++v) gsq += gij[v] * gij[v];
-      const double diff = nu_e * nu_e * gsq;
-      tsu = 1. / sqrt(g.c1 * g.c1 * g.f4dt2 + fabs(ugu) + g.c2 * diff);
-      tps = tsu;
-      tbu = sqrt(g.f4dt2 + fabs(ugu) + 16. * diff) / tr;
-    } else if constexpr(SUPG == CF3HIP_SUPG_CF2) {
-      // volume as the reference's etype::volume: Hexa3D 8 det J at the centre (Hexa3D.cpp:317-320),
-      // Quad2D half the cross product of the diagonals
-      double Jci[D * D];
-      const double detc = inverse<D>(Jc, Jci);
-      double vol;
-      if constexpr(D == 3) vol = 8. * detc;
-      else vol = 0.5 * ((Xall[2][0] - Xall[0][0]) * (Xall[3][1] - Xall[1][1]) -
-                        (Xall[2][1] - Xall[0][1]) * (Xall[3][0] - Xall[1][0]));
-      // sum over the nodes of u_c . grad N_n at the centre, grad N_n = J^-1 dN_n/dxi (dN_n/dxi_k = s_k / 2^D)
-      double um2 = 0., sum = 0.;
-#pragma unroll
-      for(int d = 0; d < D; ++d) um2 += uc[d] * uc[d];
-#pragma unroll
-      for(int m = 0; m < K; ++m) {
-        double sg[3];
-        node_signs(m, sg);
-        double un = 0.;
-#pragma unroll
-        for(int d = 0; d < D; ++d) {
-          double gdn = 0.;
-#pragma unroll
-          for(int l = 0; l < D; ++l) gdn += Jci[d * D + l] * (sg[l] * Q::kInvK);
-          un += uc[d] * gdn;
-        }
-        sum += un;
-      }
-      if(!tau_cf2(D, vol, nu_e, sqrt(um2), sum, g.dt, g.u_ref, tps, tsu, tbu)) g.status[0] = 2;
-    } else {
-#pragma unroll
-      for(int i = 0; i < K; ++i)
-#pragma unroll
-        for(int j = i + 1; j < K; ++j) {
-          double s = 0.;
-#pragma unroll
-          for(int d = 0; d < D; ++d) { const double df = Xall[i][d] - Xall[j][d]; s += df * df; }
-          hmin2 = fmin(hmin2, s);
-        }
-      const double h = sqrt(hmin2);
-      double um2 = 0.;
-#pragma unroll
-      for(int d = 0; d < D; ++d) um2 += uc[d] * uc[d];
-      const double um = sqrt(um2);
-      tsu = 1. / ((2. * um) / h + 2. / g.dt + (4. * nu_e) / (h * h));
-      tps = tsu;
-      tbu = tsu * um * um;
-    }
-    tps *= g.a_ps; tsu *= g.a_su; tbu *= g.a_bu;
+    for(int d = 0; d < D; ++d) uq[d] = rq[D * D + 1 + d];
+    const double nuq = rq[D * D + 1 + D];
     // ---- Gauss point record of lane q ----
-    double Ji[D * D];
-    const double det = inverse<D>(J, Ji);
-    if(!(det > 0.)) g.status[0] = 1;   // every writer stores the same flag value
     const double w = det;               // Gauss weight 1
     double sa[3];
     node_signs(a, sa);
@@ -308,13 +382,28 @@ __global__ __launch_bounds__(kThreads) void k_q1(AsmArgs g)
     rec[D * D + 2 * D] = nuq;
     rec[D * D + 2 * D + 1] = delta;
     rec[D * D + 2 * D + 2] = w * (Na + 0.5 * tps * La);
-    // body force source of point q (NavierStokesAssembly.hpp:65)
-    if(MODE != MODE_FP32 && g.body_force)
-#pragma unroll
-      for(int d = 0; d < D; ++d) rhs_part[d] += delta * gq[d];
   }
   __syncthreads();
 
+  // the run of staged rows each accumulator thread sums (its global loads overlap the integration below)
+  uint32_t jb = 0, je = 0, h = 0, c = 1;
+  bool owner = false;
+  if constexpr(MODE != MODE_RESIDUAL) {
+    const uint32_t alast = s_acc[nblk - 1];
+    const uint32_t n_acc = (alast & 255u) + (alast >> 8);
+    owner = (uint32_t)t < n_acc;
+    if(owner) {
+      uint32_t lo = 0, hi = nblk;
+      while(hi - lo > 1) {
+        const uint32_t mid = (lo + hi) >> 1;
+        if((s_acc[mid] & 255u) <= (uint32_t)t) lo = mid; else hi = mid;
+      }
+      h = (uint32_t)t - (s_acc[lo] & 255u);
+      c = s_acc[lo] >> 8;
+      jb = g.blk_beg[(uint64_t)(blk0 + lo) * 4];
+      je = lo + 1 < nblk ? g.blk_beg[(uint64_t)(blk0 + lo + 1) * 4] : ninc * kLanes;
+    }
+  }
   // ---- block (a, b = lane): integrate over the 2^D Gauss points ----
   constexpr int RU = D * D, RW = D * D + D, RNU = D * D + 2 * D;   // record offsets: u_adv, w G_a, nu
   double M[NN];
@@ -330,7 +419,7 @@ __global__ __launch_bounds__(kThreads) void k_q1(AsmArgs g)
     for(int v = 0; v < D * D; ++v) Auu[v] = 0.;
     double App = 0., Add = 0., Tuu = 0.;
     const double* recs = s_rec + (size_t)slot * kRecSlot;
-#pragma unroll
+#pragma unroll 2   // full unrolling makes the scheduler hoist the LDS reads and spill
     for(int q = 0; q < K; ++q) {
       double r[kRec];
       lds_load_row<double, kRec>(recs + q * kRec, r);
@@ -374,7 +463,6 @@ __global__ __launch_bounds__(kThreads) void k_q1(AsmArgs g)
       A[D * NEQ + i] = Ap[i] + tps * Xp[i];
     }
     A[D * NEQ + D] = tps * App * g.inv_rho;
-    const uint32_t nb = g.conn[e * K + b];   // column node (lane-dependent: not taken from cn[])
     load_state<NEQ, double>(g, nb, xb);
     if constexpr(MODE != MODE_FP32) {
 #pragma unroll
@@ -384,6 +472,11 @@ __global__ __launch_bounds__(kThreads) void k_q1(AsmArgs g)
         for(int c = 0; c < NEQ; ++c) s += A[r * NEQ + c] * xb[c];
         rhs_part[r] -= s;
       }
+      // body force source (NavierStokesAssembly.hpp:65), sum_q w SU_a(q) g(q) with g(q) = sum_b N_b(q) g_b:
+      // lane b adds T_uu(a, b) g_b
+      if(g.body_force)
+#pragma unroll
+        for(int d = 0; d < D; ++d) rhs_part[d] += Tuu * g.g[d][nb];
     }
     const double invdt = g.invdt, th = g.theta, thq = g.thq;
 #pragma unroll
@@ -399,7 +492,6 @@ __global__ __launch_bounds__(kThreads) void k_q1(AsmArgs g)
     // r_a = sum_b (b_e(a) - M(a, b) x~_b): lane part with x~ of node b
     if(active) {
       double xt[NEQ];
-      const uint32_t nb = g.conn[e * K + lane];
 #pragma unroll
       for(int c = 0; c < NEQ; ++c) xt[c] = g.xt[(uint64_t)nb * NEQ + c];
 #pragma unroll
@@ -415,25 +507,9 @@ __global__ __launch_bounds__(kThreads) void k_q1(AsmArgs g)
     // ---- stage the blocks at their sorted rows; accumulator threads sum their runs ----
     __syncthreads();   // records are no longer read
     if(active) {
-      const uint32_t pos = g.inc_pos[(uint64_t)(i0 + slot) * kLanes + lane];
       VR* d = s_blk + pos * LDW;
 #pragma unroll
       for(int v = 0; v < NN; ++v) d[v] = (VR)M[v];
-    }
-    const uint32_t alast = s_acc[nblk - 1];
-    const uint32_t n_acc = (alast & 255u) + (alast >> 8);
-    const bool owner = (uint32_t)t < n_acc;
-    uint32_t jb = 0, je = 0, h = 0, c = 1;
-    if(owner) {
-      uint32_t lo = 0, hi = nblk;
-      while(hi - lo > 1) {
-        const uint32_t mid = (lo + hi) >> 1;
-        if((s_acc[mid] & 255u) <= (uint32_t)t) lo = mid; else hi = mid;
-      }
-      h = (uint32_t)t - (s_acc[lo] & 255u);
-      c = s_acc[lo] >> 8;
-      jb = g.blk_beg[(uint64_t)(blk0 + lo) * 4];
-      je = lo + 1 < nblk ? g.blk_beg[(uint64_t)(blk0 + lo + 1) * 4] : ninc * kLanes;
     }
     __syncthreads();
     VR acc[NN];
@@ -492,22 +568,32 @@ void launch_mode(const AsmArgs& g, const Lss& L)
 {
   Context& c = *L.ctx;
   const int timer = MODE == MODE_RESIDUAL ? CF3HIP_KERNEL_RESIDUAL : CF3HIP_KERNEL_ASSEMBLY;
+  const unsigned gp = (unsigned)(((uint64_t)L.mesh->n_elems * Q1<D>::K + 255) / 256);
+  const unsigned ge = (unsigned)((L.mesh->n_elems + 255) / 256);
   timer_begin(c, timer);
+  hipLaunchKernelGGL((k_q1_rec<D>), dim3(gp), dim3(256), 0, c.stream, L.mesh->n_elems, g);
   if(g.supg_type == CF3HIP_SUPG_TEZDUYAR)
-    hipLaunchKernelGGL((k_q1<D, MODE, CF3HIP_SUPG_TEZDUYAR>), dim3(L.n_wg), dim3(kThreads), 0, c.stream, g);
+    hipLaunchKernelGGL((k_q1_tau<D, CF3HIP_SUPG_TEZDUYAR>), dim3(ge), dim3(256), 0, c.stream, L.mesh->n_elems, g);
   else if(g.supg_type == CF3HIP_SUPG_CF2)
-    hipLaunchKernelGGL((k_q1<D, MODE, CF3HIP_SUPG_CF2>), dim3(L.n_wg), dim3(kThreads), 0, c.stream, g);
+    hipLaunchKernelGGL((k_q1_tau<D, CF3HIP_SUPG_CF2>), dim3(ge), dim3(256), 0, c.stream, L.mesh->n_elems, g);
   else
-    hipLaunchKernelGGL((k_q1<D, MODE, CF3HIP_SUPG_METRIC>), dim3(L.n_wg), dim3(kThreads), 0, c.stream, g);
+    hipLaunchKernelGGL((k_q1_tau<D, CF3HIP_SUPG_METRIC>), dim3(ge), dim3(256), 0, c.stream, L.mesh->n_elems, g);
+  hipLaunchKernelGGL((k_q1<D, MODE>), dim3(L.n_wg), dim3(kThreads), 0, c.stream, g);
   HIP_OK(hipGetLastError());
   timer_end(c, timer);
 }
 
 template<int D>
-void launch_dim(const AsmArgs& g, const Lss& L, int mode)
+void launch_dim(AsmArgs g, Lss& L, int mode)
 {
   if(L.passes != 1 || L.pos_stride != Q1<D>::kLanes)
     CF3_THROW(CF3HIP_ERR_INVALID_ARG, "system has no Q1 staging maps");
+  // record buffers of the pre-pass (allocated at the first Q1 launch of the system)
+  const size_t nrec = (size_t)L.mesh->n_elems * Q1<D>::K * Q1<D>::kQRec;
+  if(L.q1rec.bytes < sizeof(double) * nrec) L.q1rec.alloc(L.ctx, sizeof(double) * nrec);
+  if(L.q1tau.bytes < sizeof(double4) * L.mesh->n_elems) L.q1tau.alloc(L.ctx, sizeof(double4) * L.mesh->n_elems);
+  g.q1rec = L.q1rec.as<double>();
+  g.q1tau = L.q1tau.as<double4>();
   if(mode == MODE_FP64) launch_mode<D, MODE_FP64>(g, L);
   else if(mode == MODE_FP32) launch_mode<D, MODE_FP32>(g, L);
   else launch_mode<D, MODE_RESIDUAL>(g, L);
@@ -515,7 +601,7 @@ void launch_dim(const AsmArgs& g, const Lss& L, int mode)
 
 }  // namespace
 
-void launch_hex(const AsmArgs& g, const Lss& L, int mode)
+void launch_hex(AsmArgs g, Lss& L, int mode)
 {
   if(L.mesh->etype == CF3HIP_QUAD2D) launch_dim<2>(g, L, mode);
   else launch_dim<3>(g, L, mode);

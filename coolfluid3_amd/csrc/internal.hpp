@@ -143,6 +143,7 @@ struct Lss {
   DevBuf bc_mask;       // uint8 n_nodes: bit e set when dof (node, e) carries a condition
   DevBuf bc_newmask;    // uint8 n_nodes: conditions added by the current call
   DevBuf xtilde;        // matrix-free residual input (x with eliminated columns at their values)
+  DevBuf q1rec, q1tau;  // Q1 cells: per-element Gauss-point records and tau of the last assembly launch
   bool tied = false;    // block rows tied (periodicity) since the last reset: the matrix-free
                         // residual of the element operator no longer describes the system
   // Krylov workspace (allocated lazily)
