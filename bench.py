@@ -188,10 +188,10 @@ def assembly_rates(ctx, gc, reps, k, dim):
 def precision_workload(ctx, cells, steps=4):
     """Precision form on the tetrahedral cube channel (inflow x = 1, outlet p = 0 on x = 0) with ~1%
     elements of aspect ratio ~20 (nu = 5e-4), `steps` implicit steps: FP32 element matrices, BiCGStab on
-    the FP32 values with FP64 matrix-free residual replacement (FP64 flexible GMRES around FP32 inner
-    solves where that stalls), against the FP64 assembly and BiCGStab, both to a 1e-8 relative residual
-    of the FP64 system.  Step 1 includes one-time allocations of both paths and is left out of the
-    summed speedup."""
+    the block-column scaled FP32 operator fl32(A32 D^-1) (D the 4x4 diagonal blocks, x = D^-1 y) with FP64
+    matrix-free residual replacement, against the FP64 assembly and BiCGStab + point Jacobi (the drop-in
+    solve), both to a 1e-8 relative residual of the FP64 system.  Step 1 includes one-time allocations
+    of both paths and is left out of the summed speedup."""
     from coolfluid3_amd import cabi, cases
     case = cases.inflow_cube_tets(cells, nu=5e-4, dt=5e-3, skew=True, outlet=True)
     res = {"workload": "unit cube %d^3 hexes x 6 P1 tets (%d elements, %d dofs), 1%% skewed (aspect ratio ~20), "

@@ -31,7 +31,8 @@ class Cf3HipError(RuntimeError):
 
 class SolverParams(C.Structure):
     _fields_ = [("method", C.c_int), ("restart", C.c_int), ("max_iter", C.c_int), ("tol", C.c_double),
-                ("precision", C.c_int), ("inner_tol", C.c_double), ("max_outer", C.c_int)]
+                ("precision", C.c_int), ("inner_tol", C.c_double), ("max_outer", C.c_int),
+                ("block_scale", C.c_int), ("fp32_vectors", C.c_int)]
 
 
 class SolveStats(C.Structure):
@@ -52,8 +53,8 @@ def ns_params(rho=1.0, dt=0.01, theta=1.0, use_specializations=True, supg_type=S
 
 
 def solver_params(method=BICGSTAB_JACOBI, tol=1e-10, max_iter=5000, restart=30, precision=FP64, inner_tol=1e-2,
-                  max_outer=0):
-    return SolverParams(method, restart, max_iter, tol, precision, inner_tol, max_outer)
+                  max_outer=0, block_scale=1, fp32_vectors=0):
+    return SolverParams(method, restart, max_iter, tol, precision, inner_tol, max_outer, block_scale, fp32_vectors)
 
 
 _lib = None
@@ -94,6 +95,7 @@ def lib():
         "cf3hip_lss_get_graph": [_vp, _u32, _u32],
         "cf3hip_lss_get_slots": [_vp, _u32],
         "cf3hip_lss_enable_fp32": [_vp],
+        "cf3hip_lss_round_fp32": [_vp],
         "cf3hip_matrix_reset": [_vp, C.c_double],
         "cf3hip_matrix_set_value": [_vp, C.c_uint32, C.c_uint32, C.c_double],
         "cf3hip_matrix_add_value": [_vp, C.c_uint32, C.c_uint32, C.c_double],
@@ -332,6 +334,10 @@ class System:
 
     def enable_fp32(self):
         check(lib().cf3hip_lss_enable_fp32(self.h))
+
+    def round_fp32(self):
+        """FP32 mirror := the FP64 values rounded to nearest."""
+        check(lib().cf3hip_lss_round_fp32(self.h))
 
     def values(self):
         v = np.zeros(self.n_blocks * self.neq * self.neq)

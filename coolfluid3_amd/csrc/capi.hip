@@ -523,6 +523,15 @@ int cf3hip_lss_enable_fp32(cf3hip_lss lss)
   API_END
 }
 
+int cf3hip_lss_round_fp32(cf3hip_lss lss)
+{
+  API_BEGIN
+  REQUIRE(lss, CF3HIP_ERR_NOT_CREATED, "system not created");
+  lss_round_fp32(lss->l);
+  HIP_OK(hipStreamSynchronize(lss->l.ctx->stream));
+  API_END
+}
+
 // ---------------------------------------------------------------------------------------------
 int cf3hip_matrix_reset(cf3hip_lss lss, double value)
 {

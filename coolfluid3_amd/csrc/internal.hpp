@@ -110,6 +110,8 @@ struct Lss {
   DevBuf diag_blk;      // uint32 n_nodes: block index of the diagonal
   DevBuf values;        // double n_blocks*neq*neq
   DevBuf values32;      // float mirror (precision form)
+  DevBuf values32b;     // block-column scaled FP32 operator fl32(A32 D^-1) of a block-scaled precision solve
+  DevBuf bdinv;         // double n_nodes*neq*neq: D^-1, the inverted diagonal blocks of that solve
   bool has_fp32 = false;
   DevBuf rhs, sol;      // double dofs
   // element scatter structures (lss_create from a mesh)
@@ -208,6 +210,7 @@ void batch_assemble(Batch& B, uint32_t n, Lss* const* L, Fields* const* F, const
 // lss.hip
 void lss_fill(Context& c, double* p, size_t n, double v);
 void lss_fill32(Context& c, float* p, size_t n, float v);
+void lss_round_fp32(Lss& L);   // values32 = fl32(values)
 void lss_copy(Context& c, double* dst, const double* src, size_t n);
 void lss_fill8(Context& c, uint8_t* p, size_t n, uint8_t v);
 void lss_clear_conditions(Lss& L);

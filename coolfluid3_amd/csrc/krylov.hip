@@ -172,14 +172,14 @@ __global__ void k_fb_p(uint64_t n, double* __restrict__ S, int it, int rel, int 
   }
 }
 
-// y = A x (neq = 4, values VR, FP64 vectors) with the partial dot products of the BiCGStab step:
-// NDOT = 1: d0 . y (d0 = rh); NDOT = 2: d0 . y and y . y (d0 = s).  Fixed grid, rows in a
-// grid-stride loop (16 lanes per node row).
-template<typename VR, int NDOT>
+// y = A x (neq = 4, values VR, vectors XR: FP64, or FP32 with FP32 values) with the partial dot
+// products of the BiCGStab step, summed in FP64: NDOT = 1: d0 . y (d0 = rh); NDOT = 2: d0 . y and
+// y . y (d0 = s).  Fixed grid, rows in a grid-stride loop (16 lanes per node row).
+template<typename VR, int NDOT, typename XR = double>
 __global__ __launch_bounds__(kRT) void k_spmv4_dot(uint32_t N, const uint32_t* __restrict__ row_ptr,
                                                    const uint32_t* __restrict__ cols, const VR* __restrict__ vals,
-                                                   const double* __restrict__ x, double* __restrict__ y,
-                                                   const double* __restrict__ d0, const double* __restrict__ S,
+                                                   const XR* __restrict__ x, XR* __restrict__ y,
+                                                   const XR* __restrict__ d0, const double* __restrict__ S,
                                                    double* __restrict__ partials)
 {
   if(S[S_STOP] != 0.) return;
@@ -190,12 +190,14 @@ __global__ __launch_bounds__(kRT) void k_spmv4_dot(uint32_t N, const uint32_t* _
   for(uint32_t row = blockIdx.x * (kRT / 16) + (threadIdx.x >> 4); row < N; row += step) {
     int a;
     bool out;
-    const double yv = spmv4::row<VR>(row_ptr, cols, vals, x, row, l, a, out);
+    XR yv;
+    if constexpr(sizeof(XR) == sizeof(float)) yv = spmv4::row_f32(row_ptr, cols, vals, x, row, l, a, out);
+    else yv = spmv4::row<VR>(row_ptr, cols, vals, x, row, l, a, out);
     if(out) {
       const uint64_t o = (uint64_t)row * 4 + a;
       y[o] = yv;
-      acc[0] += d0[o] * yv;
-      if constexpr(NDOT > 1) acc[NDOT - 1] += yv * yv;
+      acc[0] += (double)d0[o] * (double)yv;
+      if constexpr(NDOT > 1) acc[NDOT - 1] += (double)yv * (double)yv;
     }
   }
   block_reduce_store<NDOT>(acc, partials, gridDim.x);
@@ -469,7 +471,8 @@ struct Recurrence {
   {
     r = work; rh = r + n; p = rh + n; v = p + n; ph = v + n; s = ph + n; sh = s + n; t = sh + n;
     dinv = t + n;
-    fused = sizeof(R) == sizeof(double) && L.neq >= 1 && L.neq <= 4;
+    // FP64 vectors with neq = 1 to 4; FP32 vectors (with FP32 values) with neq = 4
+    fused = sizeof(R) == sizeof(double) ? (L.neq >= 1 && L.neq <= 4) : (sizeof(VR) == sizeof(float) && L.neq == 4);
     Gv = red_grid(n);
     // rows per workgroup of the fused products: 16 lanes per node row, 8 for scalar rows (k_spmvn_dot)
     Gs = std::min<unsigned>(kFusedSpmvBlocks, grid_for(L.n_nodes, kRT / (L.neq == 1 ? 8 : 16)));
@@ -514,9 +517,14 @@ struct Recurrence {
 
   // fused product y = A x with its partial dot products (see k_spmv4_dot / k_spmvn_dot)
   template<int NDOT>
-  void spmv_dot(const uint32_t* rp, const uint32_t* cl, const VR* vl, const double* xx, double* yy, const double* d0,
+  void spmv_dot(const uint32_t* rp, const uint32_t* cl, const VR* vl, const R* xx, R* yy, const R* d0,
                 double* partials)
   {
+    if constexpr(sizeof(R) == sizeof(float)) {
+      hipLaunchKernelGGL((k_spmv4_dot<VR, NDOT, float>), dim3(Gs), dim3(kRT), 0, c.stream, L.n_nodes, rp, cl, vl, xx, yy,
+                         d0, S, partials);
+      return;
+    } else
     switch(L.neq) {
       case 4: hipLaunchKernelGGL((k_spmv4_dot<VR, NDOT>), dim3(Gs), dim3(kRT), 0, c.stream, L.n_nodes, rp, cl, vl, xx, yy,
                                  d0, S, partials); break;
@@ -685,13 +693,183 @@ void lss_solve(Lss& L, const cf3hip_solver_params& prm, cf3hip_solve_stats& st)
 
 namespace {
 
-__global__ void k_flush_increment(uint64_t n, double* __restrict__ xhi, double* __restrict__ xlo)
+template<typename XL>
+__global__ void k_flush_increment(uint64_t n, double* __restrict__ xhi, XL* __restrict__ xlo)
 {
   const uint64_t stride = (uint64_t)gridDim.x * blockDim.x;
   for(uint64_t i = (uint64_t)blockIdx.x * blockDim.x + threadIdx.x; i < n; i += stride) {
-    xhi[i] += xlo[i];
-    xlo[i] = 0.;
+    xhi[i] += (double)xlo[i];
+    xlo[i] = XL(0);
   }
+}
+
+// ---- block-column scaling of the precision form: B = fl32(A32 D^-1), x = D^-1 y ----
+// D^-1 of each node's diagonal block, by Gauss-Jordan elimination with partial pivoting in FP64 on the
+// FP32 values; a block with a zero or non-finite pivot keeps D^-1 = I (its column block is not scaled)
+template<int NEQ>
+__global__ void k_block_inverse(uint32_t N, const uint32_t* __restrict__ diag_blk, const float* __restrict__ vals,
+                                double* __restrict__ dinv)
+{
+  constexpr int NN = NEQ * NEQ;
+  const uint32_t i = blockIdx.x * blockDim.x + threadIdx.x;
+  if(i >= N) return;
+  double a[NEQ][NEQ], b[NEQ][NEQ];
+  const float* d = vals + (uint64_t)diag_blk[i] * NN;
+#pragma unroll
+  for(int r = 0; r < NEQ; ++r)
+#pragma unroll
+    for(int c = 0; c < NEQ; ++c) { a[r][c] = (double)d[r * NEQ + c]; b[r][c] = r == c ? 1. : 0.; }
+  bool ok = true;
+#pragma unroll
+  for(int k = 0; k < NEQ; ++k) {
+    int piv = k;
+    double pv = fabs(a[k][k]);
+#pragma unroll
+    for(int r = k + 1; r < NEQ; ++r)
+      if(fabs(a[r][k]) > pv) { pv = fabs(a[r][k]); piv = r; }
+#pragma unroll
+    for(int r = k + 1; r < NEQ; ++r)
+      if(r == piv)
+#pragma unroll
+        for(int c = 0; c < NEQ; ++c) {
+          const double ta = a[k][c]; a[k][c] = a[r][c]; a[r][c] = ta;
+          const double tb = b[k][c]; b[k][c] = b[r][c]; b[r][c] = tb;
+        }
+    ok = ok && pv > 0. && isfinite(pv);
+    const double inv = ok ? 1. / a[k][k] : 0.;
+#pragma unroll
+    for(int c = 0; c < NEQ; ++c) { a[k][c] *= inv; b[k][c] *= inv; }
+#pragma unroll
+    for(int r = 0; r < NEQ; ++r) {
+      if(r == k) continue;
+      const double f = a[r][k];
+#pragma unroll
+      for(int c = 0; c < NEQ; ++c) { a[r][c] -= f * a[k][c]; b[r][c] -= f * b[k][c]; }
+    }
+  }
+  double* o = dinv + (uint64_t)i * NN;
+#pragma unroll
+  for(int r = 0; r < NEQ; ++r)
+#pragma unroll
+    for(int c = 0; c < NEQ; ++c) o[r * NEQ + c] = ok ? b[r][c] : (r == c ? 1. : 0.);
+}
+
+// out(block, row r) = fl32(vals(block, row r) D^-1 of the block's column node), sums in FP64
+template<int NEQ>
+__global__ void k_scale_cols(uint64_t n_rows, const uint32_t* __restrict__ cols, const float* __restrict__ vals,
+                             const double* __restrict__ dinv, float* __restrict__ out)
+{
+  constexpr int NN = NEQ * NEQ;
+  const uint64_t t = (uint64_t)blockIdx.x * blockDim.x + threadIdx.x;
+  if(t >= n_rows) return;
+  const uint64_t blk = t / NEQ;
+  const int r = (int)(t % NEQ);
+  const double* D = dinv + (uint64_t)cols[blk] * NN;
+  double a[NEQ];
+#pragma unroll
+  for(int k = 0; k < NEQ; ++k) a[k] = (double)vals[blk * NN + r * NEQ + k];
+#pragma unroll
+  for(int c = 0; c < NEQ; ++c) {
+    double s = 0.;
+#pragma unroll
+    for(int k = 0; k < NEQ; ++k) s += a[k] * D[k * NEQ + c];
+    out[blk * NN + r * NEQ + c] = (float)s;
+  }
+}
+
+// xhi += D^-1 xlo per node (xlo holds the increment in the scaled variable y), xlo = 0; with xhi ==
+// nullptr the product replaces xlo in place
+template<int NEQ, typename XL>
+__global__ void k_apply_bdiag(uint32_t N, const double* __restrict__ dinv, double* xhi, XL* xlo)
+{
+  constexpr int NN = NEQ * NEQ;
+  const uint32_t i = blockIdx.x * blockDim.x + threadIdx.x;
+  if(i >= N) return;
+  const double* D = dinv + (uint64_t)i * NN;
+  double y[NEQ];
+#pragma unroll
+  for(int k = 0; k < NEQ; ++k) y[k] = (double)xlo[(uint64_t)i * NEQ + k];
+#pragma unroll
+  for(int r = 0; r < NEQ; ++r) {
+    double s = 0.;
+#pragma unroll
+    for(int k = 0; k < NEQ; ++k) s += D[r * NEQ + k] * y[k];
+    if(xhi) {
+      xhi[(uint64_t)i * NEQ + r] += s;
+      xlo[(uint64_t)i * NEQ + r] = XL(0);
+    } else {
+      xlo[(uint64_t)i * NEQ + r] = (XL)s;
+    }
+  }
+}
+
+template<int NEQ>
+void block_scale_n(Lss& L)
+{
+  Context& c = *L.ctx;
+  const size_t nv = (size_t)L.n_blocks * NEQ * NEQ;
+  if(L.values32b.bytes < sizeof(float) * nv) L.values32b.alloc(&c, sizeof(float) * nv);
+  if(L.bdinv.bytes < sizeof(double) * L.n_nodes * NEQ * NEQ) L.bdinv.alloc(&c, sizeof(double) * L.n_nodes * NEQ * NEQ);
+  hipLaunchKernelGGL(k_block_inverse<NEQ>, dim3(grid_for(L.n_nodes, 256)), dim3(256), 0, c.stream, L.n_nodes,
+                     L.diag_blk.as<uint32_t>(), L.values32.as<float>(), L.bdinv.as<double>());
+  const uint64_t nr = (uint64_t)L.n_blocks * NEQ;
+  hipLaunchKernelGGL(k_scale_cols<NEQ>, dim3(grid_for(nr, 256)), dim3(256), 0, c.stream, nr, L.cols.as<uint32_t>(),
+                     L.values32.as<float>(), L.bdinv.as<double>(), L.values32b.as<float>());
+  HIP_OK(hipGetLastError());
+}
+
+// L.values32b = fl32(A32 D^-1) and L.bdinv = D^-1 from the current FP32 values
+void block_scale_fp32(Lss& L)
+{
+  switch(L.neq) {
+    case 1: block_scale_n<1>(L); break;
+    case 2: block_scale_n<2>(L); break;
+    case 3: block_scale_n<3>(L); break;
+    case 4: block_scale_n<4>(L); break;
+    default: CF3_THROW(CF3HIP_ERR_NOT_IMPLEMENTED, "block scaling of the precision form supports 1 to 4 equations per node");
+  }
+}
+
+// xhi += D^-1 xlo, xlo = 0 (xhi != nullptr), or xlo := D^-1 xlo
+template<typename XL>
+void apply_bdiag(Lss& L, double* xhi, XL* xlo)
+{
+  Context& c = *L.ctx;
+  const dim3 g(grid_for(L.n_nodes, 256)), b(256);
+  const double* D = L.bdinv.as<double>();
+  switch(L.neq) {
+    case 1: hipLaunchKernelGGL((k_apply_bdiag<1, XL>), g, b, 0, c.stream, L.n_nodes, D, xhi, xlo); break;
+    case 2: hipLaunchKernelGGL((k_apply_bdiag<2, XL>), g, b, 0, c.stream, L.n_nodes, D, xhi, xlo); break;
+    case 3: hipLaunchKernelGGL((k_apply_bdiag<3, XL>), g, b, 0, c.stream, L.n_nodes, D, xhi, xlo); break;
+    default: hipLaunchKernelGGL((k_apply_bdiag<4, XL>), g, b, 0, c.stream, L.n_nodes, D, xhi, xlo); break;
+  }
+  HIP_OK(hipGetLastError());
+}
+
+// During a block-scaled solve the FP32 kernels read the scaled operator: the two buffers trade places
+// and are restored when the solve ends (also on an exception).
+struct ScaledFp32 {
+  Lss& L;
+  bool on;
+  ScaledFp32(Lss& L_, bool on_) : L(L_), on(on_) { if(on) swap(); }
+  ~ScaledFp32() { if(on) swap(); }
+  void swap()
+  {
+    std::swap(L.values32.ptr, L.values32b.ptr);
+    std::swap(L.values32.bytes, L.values32b.bytes);
+    std::swap(L.values32.ctx, L.values32b.ctx);
+  }
+};
+
+template<typename XL>
+void flush_increment(Lss& L, bool bs, double* xhi, XL* xlo)
+{
+  const uint64_t n = L.dofs;
+  if(bs) apply_bdiag(L, xhi, xlo);
+  else
+    hipLaunchKernelGGL(k_flush_increment<XL>, dim3(std::min<unsigned>(grid_for(n, 256), 8192)), dim3(256), 0,
+                       L.ctx->stream, n, xhi, xlo);
+  HIP_OK(hipGetLastError());
 }
 
 constexpr int kMaxOuterBasis = 16;   // FGMRES restart length cap of the precision form
@@ -705,7 +883,7 @@ constexpr int kMaxOuterBasis = 16;   // FGMRES restart length cap of the precisi
 // inside one BiCGStab) cannot when the solution is sensitive to the matrix.  Convergence is declared
 // on the FP64 true residual ||b' - A' x|| <= tol ||b'||.
 template<class Resid>
-void fgmres_ir(Lss& L, Resid resid, const cf3hip_solver_params& prm, cf3hip_solve_stats& st)
+void fgmres_ir(Lss& L, Resid resid, const cf3hip_solver_params& prm, cf3hip_solve_stats& st, bool bs = false)
 {
   using namespace arnoldi;
   if(prm.method != CF3HIP_BICGSTAB_JACOBI)
@@ -765,6 +943,7 @@ void fgmres_ir(Lss& L, Resid resid, const cf3hip_solver_params& prm, cf3hip_solv
       int sp = 0;
       its += bicgstab<double, float>(L, vj, zj, L.work.as<double>(), itol, std::max(1, prm.max_iter - its), cv, sp);
       spmvs += sp;
+      if(bs) apply_bdiag<double>(L, nullptr, zj);   // z_j = D^-1 y_j
       ++outer;
       resid(zj, w);   // A' z_j = b' - r(z_j)
       hipLaunchKernelGGL(k_sub, dim3(gb), dim3(256), 0, c.stream, n, b0, w);
@@ -794,49 +973,68 @@ void fgmres_ir(Lss& L, Resid resid, const cf3hip_solver_params& prm, cf3hip_solv
 
 }  // namespace
 
-// Precision form, first phase: BiCGStab with Jacobi preconditioning on the FP32 values (FP32 stored,
-// FP64 vectors and FP64 accumulation in the SpMV), with reliable updates: the FP64 solution accumulates
-// the increments and, whenever the recursive residual has dropped by the factor delta (or meets the
-// tolerance), the residual is replaced by the true FP64 residual b - A x given by `resid`.  The search
-// directions are kept across replacements (no restart), so on most systems the iteration count stays
-// close to that of the FP64 method.  When the FP32 operator is too far from the FP64 one for that
-// (a replacement raises the true residual, or kSwitchStall replacements in a row fail to halve it), the
-// solve continues from the current FP64 iterate with fgmres_ir, which converges in that regime.
-// Convergence is declared on the FP64 true residual only.
-template<class Resid>
-void reliable_bicgstab(Lss& L, Resid resid, const cf3hip_solver_params& prm, cf3hip_solve_stats& st)
+// Precision form, first phase: BiCGStab with Jacobi preconditioning on the FP32 values (FP32 stored;
+// vectors R = FP64 with FP64 accumulation in the products, or R = FP32 with FP32 accumulation), with
+// reliable updates: the FP64 solution accumulates the increments and, whenever the recursive residual has
+// dropped by the factor delta (or meets the tolerance), the residual is replaced by the true FP64 residual
+// b - A x given by `resid`.  The search directions are kept across replacements (no restart), so on most
+// systems the iteration count stays close to that of the FP64 method.  With block scaling the recurrence
+// runs on B = fl32(A32 D^-1) and its increment is y, x = D^-1 y.  When the FP32 operator is too far from
+// the FP64 one for that (the true residual grows beyond kDiverge times its best value, or kSwitchStall
+// replacements in a row fail to halve it: single rises are ordinary for BiCGStab), the solve continues
+// from the current FP64 iterate with fgmres_ir.  Convergence is declared on the FP64 true residual only.
+template<typename R, class Resid>
+void reliable_core(Lss& L, Resid resid, const cf3hip_solver_params& prm, cf3hip_solve_stats& st)
 {
-  if(prm.method != CF3HIP_BICGSTAB_JACOBI) CF3_THROW(CF3HIP_ERR_NOT_IMPLEMENTED, "precision form uses BiCGStab");
-  if(!L.has_fp32) CF3_THROW(CF3HIP_ERR_NOT_CREATED, "FP32 values not enabled");
   Context& c = *L.ctx;
   const uint64_t n = L.dofs;
+  // the recurrence (9 vectors) and the increment xlo in type R; with R = FP32 also an FP64 true residual
+  // and an FP64 zero vector behind them (7 n doubles in all)
   ensure_work(L, 10, sizeof(double));
-  double* work = L.work.as<double>();
-  double* xlo = work + 9 * n;
+  R* work = L.work.as<R>();
+  R* xlo = work + 9 * n;
   double* xhi = L.sol.as<double>();
-  Recurrence<double, float> rc(L, xlo, work, 1);
-  double* r = rc.r;
+  Recurrence<R, float> rc(L, xlo, work, 1);
+  R* r = rc.r;
   double* S = rc.S;
+  constexpr bool f32 = sizeof(R) == sizeof(float);
+  double* rt = f32 ? reinterpret_cast<double*>(work + 10 * n) : reinterpret_cast<double*>(r);   // FP64 true residual
+  double* zero64 = f32 ? rt + n : reinterpret_cast<double*>(xlo);
   // replacement after every 10-fold drop of the recursive residual: on the 128^3 channel and the 160^3
-  // cavity this gave fewer iterations than replacing after 100-fold drops, and never more fallbacks
+  // cavity this gave fewer iterations than replacing after 100-fold drops
   const double delta = 0.1;
   st = cf3hip_solve_stats{0, 0, 0, 0, 0.};
+  const bool bs = prm.block_scale != 0;
+  if(bs) block_scale_fp32(L);
+  ScaledFp32 scaled(L, bs);   // from here the FP32 values are B = fl32(A32 D^-1); xlo holds y
+  auto to_r = [&]() {
+    if constexpr(f32) convert_f64_to_f32(c, rt, reinterpret_cast<float*>(r), n);
+  };
 
   // ||b'|| = ||r(0)||
-  lss_fill(c, xlo, n, 0.);
-  resid(xlo, r);
-  double bn = norm2(c, r, n);
+  lss_fill(c, zero64, n, 0.);
+  if constexpr(f32) lss_fill32(c, reinterpret_cast<float*>(xlo), n, 0.f);
+  resid(zero64, rt);
+  double bn = norm2(c, rt, n);
   if(bn == 0.) bn = 1.;
   const double thr2 = prm.tol * prm.tol * bn * bn;
-  resid(xhi, r);
+  resid(xhi, rt);
   int updates = 1;
-  double rtrue = norm2(c, r, n);
-  hipLaunchKernelGGL((k_jacobi<double, float>), dim3(grid_for(n, 256)), dim3(256), 0, c.stream, L.n_nodes, L.neq,
-                     L.diag_blk.as<uint32_t>(), L.values32.as<float>(), const_cast<double*>(rc.dinv));
+  double rtrue = norm2(c, rt, n);
+  to_r();
+  hipLaunchKernelGGL((k_jacobi<R, float>), dim3(grid_for(n, 256)), dim3(256), 0, c.stream, L.n_nodes, L.neq,
+                     L.diag_blk.as<uint32_t>(), L.values32.as<float>(), const_cast<R*>(rc.dinv));
   rc.seed(thr2, true, 0);
   bool conv = rtrue * rtrue <= thr2;
-  constexpr int kSwitchStall = 2;
-  double rprog = rtrue, rprev = rtrue;
+  // the fallback is for operators the FP32 recurrence cannot follow: on the 128^3 channel (block-scaled
+  // FP32 operator) segments that fail to halve the true residual come in long runs and still converge
+  // sooner than the FGMRES phase would, so only a long stall or a true divergence switches
+  constexpr int kSwitchStall = 40;
+  constexpr double kDiverge = 1e3;
+  // iterations enqueued after a replacement before the first host read (a segment is ~10-40 iterations;
+  // launches past a stop return at once)
+  constexpr int kBatchAfterReplace = 4;
+  double rprog = rtrue, rbest = rtrue;
   int stall = 0;
   bool switched = false;
   hipLaunchKernelGGL(k_set_scalar, dim3(1), dim3(1), 0, c.stream, S, (int)S_MAXR, rtrue);
@@ -854,11 +1052,10 @@ void reliable_bicgstab(Lss& L, Resid resid, const cf3hip_solver_params& prm, cf3
     }
     // reliable update: move the increment into the FP64 solution, replace the residual
     const bool breakdown = !std::isfinite(sc.v[S_RN2]) || rc.rho(sc, it) == 0.;
-    hipLaunchKernelGGL(k_flush_increment, dim3(std::min<unsigned>(grid_for(n, 256), 8192)), dim3(256), 0, c.stream,
-                       n, xhi, xlo);
-    resid(xhi, r);
+    flush_increment(L, bs, xhi, xlo);
+    resid(xhi, rt);
     ++updates;
-    rtrue = norm2(c, r, n);
+    rtrue = norm2(c, rt, n);
     if(rtrue * rtrue <= thr2) { conv = true; break; }
     if(rtrue < 0.5 * rprog) {
       rprog = rtrue;
@@ -866,14 +1063,15 @@ void reliable_bicgstab(Lss& L, Resid resid, const cf3hip_solver_params& prm, cf3
     } else {
       ++stall;
     }
-    if(!std::isfinite(rtrue) || rtrue > rprev || stall >= kSwitchStall) { switched = true; break; }
-    rprev = rtrue;
+    rbest = std::min(rbest, rtrue);
+    if(!std::isfinite(rtrue) || rtrue > kDiverge * rbest || stall >= kSwitchStall) { switched = true; break; }
+    to_r();
     // breakdown: restart the recurrences from the true residual; otherwise keep the directions
     rc.seed(thr2, breakdown, it);
     hipLaunchKernelGGL(k_set_scalar, dim3(1), dim3(1), 0, c.stream, S, (int)S_MAXR, rtrue);
     hipLaunchKernelGGL(k_set_scalar, dim3(1), dim3(1), 0, c.stream, S, (int)S_MAXR_ODD, rtrue);
     HIP_OK(hipGetLastError());
-    batch = 1;
+    batch = kBatchAfterReplace;
   }
   const int spmvs = 2 * it;
   if(switched) {
@@ -881,7 +1079,7 @@ void reliable_bicgstab(Lss& L, Resid resid, const cf3hip_solver_params& prm, cf3
     cf3hip_solver_params p2 = prm;
     p2.max_iter = std::max(1, prm.max_iter - it);
     cf3hip_solve_stats s2;
-    fgmres_ir(L, resid, p2, s2);
+    fgmres_ir(L, resid, p2, s2, bs);
     st.iterations = it + s2.iterations;
     st.outer = updates + s2.outer;
     st.spmvs = spmvs + s2.spmvs;
@@ -890,10 +1088,9 @@ void reliable_bicgstab(Lss& L, Resid resid, const cf3hip_solver_params& prm, cf3
     return;
   }
   if(!conv) {
-    hipLaunchKernelGGL(k_flush_increment, dim3(std::min<unsigned>(grid_for(n, 256), 8192)), dim3(256), 0, c.stream, n,
-                       xhi, xlo);
-    resid(xhi, r);
-    rtrue = norm2(c, r, n);
+    flush_increment(L, bs, xhi, xlo);
+    resid(xhi, rt);
+    rtrue = norm2(c, rt, n);
     conv = rtrue * rtrue <= thr2;
   }
   st.iterations = it;
@@ -901,6 +1098,15 @@ void reliable_bicgstab(Lss& L, Resid resid, const cf3hip_solver_params& prm, cf3
   st.spmvs = spmvs;
   st.rel_residual = rtrue / bn;
   st.converged = conv ? 1 : 0;
+}
+
+template<class Resid>
+void reliable_bicgstab(Lss& L, Resid resid, const cf3hip_solver_params& prm, cf3hip_solve_stats& st)
+{
+  if(prm.method != CF3HIP_BICGSTAB_JACOBI) CF3_THROW(CF3HIP_ERR_NOT_IMPLEMENTED, "precision form uses BiCGStab");
+  if(!L.has_fp32) CF3_THROW(CF3HIP_ERR_NOT_CREATED, "FP32 values not enabled");
+  if(prm.fp32_vectors && L.neq == 4) reliable_core<float>(L, resid, prm, st);
+  else reliable_core<double>(L, resid, prm, st);
 }
 
 void ns_solve(Lss& L, Fields& F, const cf3hip_ns_params& nsp, const cf3hip_solver_params& prm,

@@ -23,6 +23,14 @@ __global__ void k_fill(T* __restrict__ p, uint64_t n, T v)
   for(uint64_t j = i; j < n; j += stride) p[j] = v;
 }
 
+// FP32 mirror of FP64 values, each rounded to nearest
+__global__ void k_round_fp32(float* __restrict__ dst, const double* __restrict__ src, uint64_t n)
+{
+  const uint64_t i = (uint64_t)blockIdx.x * blockDim.x + threadIdx.x;
+  const uint64_t stride = (uint64_t)gridDim.x * blockDim.x;
+  for(uint64_t j = i; j < n; j += stride) dst[j] = (float)src[j];
+}
+
 // FP64, neq = 4: 16 lanes per node row (lane layout of spmv4::row_f64)
 __global__ __launch_bounds__(256) void k_spmv4_f64(uint32_t N, const uint32_t* __restrict__ row_ptr,
                                                    const uint32_t* __restrict__ cols, const double* __restrict__ vals,
@@ -241,6 +249,16 @@ void lss_copy(Context& c, double* dst, const double* src, size_t n)
   if(!n || dst == src) return;
   hipLaunchKernelGGL(k_copy, dim3(std::min<unsigned>(grid_for(n, 256), 4096)), dim3(256), 0, c.stream, dst, src,
                      (uint64_t)n);
+  HIP_OK(hipGetLastError());
+}
+
+void lss_round_fp32(Lss& L)
+{
+  if(!L.has_fp32) CF3_THROW(CF3HIP_ERR_NOT_CREATED, "FP32 values not enabled");
+  const uint64_t n = (uint64_t)L.n_blocks * L.neq * L.neq;
+  if(!n) return;
+  hipLaunchKernelGGL(k_round_fp32, dim3(std::min<unsigned>(grid_for(n, 256), 8192)), dim3(256), 0, L.ctx->stream,
+                     L.values32.as<float>(), L.values.as<double>(), n);
   HIP_OK(hipGetLastError());
 }
 

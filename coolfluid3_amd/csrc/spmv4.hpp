@@ -53,6 +53,27 @@ __device__ __forceinline__ double row_mixed(const uint32_t* __restrict__ row_ptr
   return acc;
 }
 
+// FP32 values and vectors, FP32 accumulation (lane layout of row_mixed): the inner product of the
+// precision form with FP32 vectors
+__device__ __forceinline__ float row_f32(const uint32_t* __restrict__ row_ptr, const uint32_t* __restrict__ cols,
+                                         const float* __restrict__ vals, const float* __restrict__ x, uint32_t row,
+                                         int l, int& a, bool& out)
+{
+  const uint32_t q0 = row_ptr[row], q1 = row_ptr[row + 1];
+  const int sub = l >> 2;
+  a = l & 3;
+  float acc = 0.f;
+  for(uint32_t q = q0 + sub; q < q1; q += 4) {
+    const float4 v = *reinterpret_cast<const float4*>(vals + (uint64_t)q * 16 + 4 * a);
+    const float4 xv = *reinterpret_cast<const float4*>(x + (uint64_t)cols[q] * 4);
+    acc += v.x * xv.x + v.y * xv.y + v.z * xv.z + v.w * xv.w;
+  }
+  acc += __shfl_xor(acc, 4, 16);
+  acc += __shfl_xor(acc, 8, 16);
+  out = l < 4;
+  return acc;
+}
+
 template<typename VR>
 __device__ __forceinline__ double row(const uint32_t* __restrict__ row_ptr, const uint32_t* __restrict__ cols,
                                       const VR* __restrict__ vals, const double* __restrict__ x, uint32_t r, int l,

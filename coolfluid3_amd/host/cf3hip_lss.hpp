@@ -194,6 +194,8 @@ public:
     params.precision = CF3HIP_FP64;
     params.inner_tol = 1e-2;
     params.max_outer = 0;
+    params.block_scale = 1;
+    params.fp32_vectors = 0;
   }
   cf3hip_solver_params params;
   cf3hip_solve_stats stats = {};

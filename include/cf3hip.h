@@ -128,6 +128,10 @@ int cf3hip_lss_get_graph(cf3hip_lss lss, uint32_t* starting_indices, uint32_t* n
 int cf3hip_lss_get_slots(cf3hip_lss lss, uint32_t* slots);
 /* enable an FP32 mirror of the values (precision form); the FP32 assembly writes it */
 int cf3hip_lss_enable_fp32(cf3hip_lss lss);
+/* round the FP64 values into the FP32 mirror (each value to nearest): the FP32 operator of a precision
+ * solve on a stored system whose values were set or assembled in FP64.  No counterpart in the reference
+ * (its Trilinos matrices hold one precision); callers: cf3hip_lss_solve with precision CF3HIP_FP32. */
+int cf3hip_lss_round_fp32(cf3hip_lss lss);
 
 /* ---- Matrix interface (cf3/math/LSS/Matrix.hpp:35-180) ---- */
 int cf3hip_matrix_reset(cf3hip_lss lss, double value);                          /* Matrix::reset */
@@ -213,6 +217,13 @@ typedef struct cf3hip_solver_params {
                         (restart length min(restart, 16)) */
   double inner_tol;  /* relative tolerance of each FP32 inner solve of the GMRES phase (0: 1e-2) */
   int max_outer;     /* cap on the FP32 inner solves of the GMRES phase (0: no cap) */
+  int block_scale;   /* precision FP32 only.  0: the FP32 Krylov phase runs on the FP32 values with point
+                        Jacobi; 1 (the default of the host layers): on B = fl32(A32 D^-1), D the diagonal
+                        blocks of the FP32 values inverted in FP64, with x = D^-1 y (block-Jacobi right
+                        preconditioning, applied before the FP32 rounding).  The FP32 mirror itself is
+                        left unchanged. */
+  int fp32_vectors;  /* precision FP32, 4 equations per node: 1 = the Krylov vectors of the FP32 phase in
+                        FP32 and its products accumulated in FP32 (dot products in FP64); 0 = FP64 */
 } cf3hip_solver_params;
 typedef struct cf3hip_solve_stats {
   int iterations;    /* BiCGStab iterations (FP32 form: both phases) */
