@@ -7,10 +7,15 @@
 // stop on ||b - A x|| <= tol ||b|| -- restated identically in the oracle.
 //
 // ILU(0) factorisation and the triangular solves use level schedules (rows of one level only read rows
-// of earlier levels); one workgroup walks the levels with barriers.  These kernels serve the small
-// two-dimensional system of the lid-driven cavity case; the large systems use BiCGStab.
+// of earlier levels); one workgroup walks the levels with barriers.  The schedules depend on the pattern
+// only and are built once per system on the host.  Systems of up to kSmallMaxN dofs (the two-dimensional
+// cavity and Taylor-Green cases) run the whole restarted solve in one launch of one workgroup
+// (k_gmres_small): the preconditioned vector lives in LDS, so the triangular solves and the product
+// read their operand from LDS, every reduction is a workgroup reduction in a fixed order, and the host
+// waits once per solve.  Larger systems take the multi-launch loop (device-wide multi-dots).
 #include <algorithm>
 #include <cmath>
+#include <vector>
 
 #include "arnoldi.hpp"
 #include "internal.hpp"
@@ -30,40 +35,24 @@ __device__ inline uint32_t find_col(const uint32_t* col, uint32_t lo, uint32_t h
   return lo;
 }
 
-// levels (single thread: a dynamic programme in row order) and rows sorted by level
-__global__ void k_levels(uint32_t n, const uint32_t* __restrict__ rp, const uint32_t* __restrict__ col,
-                         uint32_t* __restrict__ diag, uint32_t* __restrict__ lev, uint32_t* __restrict__ ulev,
-                         uint32_t* __restrict__ lorder, uint32_t* __restrict__ lptr, uint32_t* __restrict__ uorder,
-                         uint32_t* __restrict__ uptr, uint32_t* __restrict__ counts)
+// Rows are taken by groups of kGL lanes (kWG / kGL rows of a level at a time).  A row of at most kSeg
+// entries is staged in LDS; its lower entries are eliminated in column order, the lanes of the group
+// sharing the row k of each step: l_ik = a_ik / u_kk, then a_ij -= l_ik u_kj over the upper entries
+// u_kj of row k (lane-strided; each j appears once, so the updates never collide), then the row is
+// written back.  Longer rows are eliminated by the group's first lane in global memory.
+constexpr int kGL = 16;                 // lanes per row
+constexpr int kGroups = kWG / kGL;
+constexpr uint32_t kSeg = 48;           // staged entries per row (48 doubles + 48 columns per group)
+
+__device__ inline void group_sync() { __builtin_amdgcn_fence(__ATOMIC_SEQ_CST, "wavefront"); __builtin_amdgcn_wave_barrier(); }
+
+__device__ inline uint32_t find_lds(const uint32_t* c, uint32_t lo, uint32_t hi, uint32_t j)
 {
-  if(threadIdx.x != 0 || blockIdx.x != 0) return;
-  uint32_t lmax = 0, umax = 0;
-  for(uint32_t i = 0; i < n; ++i) {
-    const uint32_t d = find_col(col, rp[i], rp[i + 1], i);
-    diag[i] = d;
-    uint32_t l = 0;
-    for(uint32_t q = rp[i]; q < d; ++q) l = max(l, lev[col[q]] + 1);
-    lev[i] = l;
-    lmax = max(lmax, l);
+  while(lo < hi) {
+    const uint32_t mid = (lo + hi) >> 1;
+    if(c[mid] < j) lo = mid + 1; else hi = mid;
   }
-  for(uint32_t ii = n; ii-- > 0;) {
-    uint32_t l = 0;
-    for(uint32_t q = diag[ii] + 1; q < rp[ii + 1]; ++q) l = max(l, ulev[col[q]] + 1);
-    ulev[ii] = l;
-    umax = max(umax, l);
-  }
-  for(uint32_t l = 0; l <= lmax + 1; ++l) lptr[l] = 0;
-  for(uint32_t i = 0; i < n; ++i) ++lptr[lev[i] + 1];
-  for(uint32_t l = 0; l <= lmax; ++l) lptr[l + 1] += lptr[l];
-  for(uint32_t l = 0; l <= lmax; ++l) counts[l] = lptr[l];
-  for(uint32_t i = 0; i < n; ++i) lorder[counts[lev[i]]++] = i;
-  for(uint32_t l = 0; l <= umax + 1; ++l) uptr[l] = 0;
-  for(uint32_t i = 0; i < n; ++i) ++uptr[ulev[i] + 1];
-  for(uint32_t l = 0; l <= umax; ++l) uptr[l + 1] += uptr[l];
-  for(uint32_t l = 0; l <= umax; ++l) counts[l] = uptr[l];
-  for(uint32_t i = 0; i < n; ++i) uorder[counts[ulev[i]]++] = i;
-  counts[0] = lmax + 1;
-  counts[1] = umax + 1;
+  return lo;
 }
 
 __global__ __launch_bounds__(kWG) void k_ilu0_factor(const uint32_t* __restrict__ rp, const uint32_t* __restrict__ col,
@@ -71,17 +60,42 @@ __global__ __launch_bounds__(kWG) void k_ilu0_factor(const uint32_t* __restrict_
                                                     const uint32_t* __restrict__ lorder, const uint32_t* __restrict__ lptr,
                                                     uint32_t nlev)
 {
+  __shared__ double sv[kGroups][kSeg];
+  __shared__ uint32_t sc[kGroups][kSeg];
+  const uint32_t g = threadIdx.x / kGL, lane = threadIdx.x % kGL;
+  double* rv = sv[g];
+  uint32_t* rc = sc[g];
   for(uint32_t l = 0; l < nlev; ++l) {
-    for(uint32_t t = lptr[l] + threadIdx.x; t < lptr[l + 1]; t += kWG) {
+    for(uint32_t t = lptr[l] + g; t < lptr[l + 1]; t += kGroups) {
       const uint32_t i = lorder[t];
-      for(uint32_t q = rp[i]; q < diag[i]; ++q) {
-        const uint32_t k = col[q];
-        const double lik = val[q] / val[diag[k]];
-        val[q] = lik;
-        for(uint32_t r = diag[k] + 1; r < rp[k + 1]; ++r) {
-          const uint32_t j = col[r];
-          const uint32_t pos = find_col(col, q + 1, rp[i + 1], j);
-          if(pos < rp[i + 1] && col[pos] == j) val[pos] -= lik * val[r];
+      const uint32_t q0 = rp[i], q1 = rp[i + 1], d = diag[i];
+      const uint32_t len = q1 - q0, nlow = d - q0;
+      if(len <= kSeg) {
+        for(uint32_t p = lane; p < len; p += kGL) { rv[p] = val[q0 + p]; rc[p] = col[q0 + p]; }
+        group_sync();
+        for(uint32_t qq = 0; qq < nlow; ++qq) {
+          const uint32_t k = rc[qq];
+          const double lik = rv[qq] / val[diag[k]];
+          group_sync();
+          if(lane == 0) rv[qq] = lik;
+          for(uint32_t r = diag[k] + 1 + lane; r < rp[k + 1]; r += kGL) {
+            const uint32_t j = col[r];
+            const uint32_t pos = find_lds(rc, qq + 1, len, j);
+            if(pos < len && rc[pos] == j) rv[pos] -= lik * val[r];
+          }
+          group_sync();
+        }
+        for(uint32_t p = lane; p < len; p += kGL) val[q0 + p] = rv[p];
+      } else if(lane == 0) {
+        for(uint32_t q = q0; q < d; ++q) {
+          const uint32_t k = col[q];
+          const double lik = val[q] / val[diag[k]];
+          val[q] = lik;
+          for(uint32_t r = diag[k] + 1; r < rp[k + 1]; ++r) {
+            const uint32_t j = col[r];
+            const uint32_t pos = find_col(col, q + 1, q1, j);
+            if(pos < q1 && col[pos] == j) val[pos] -= lik * val[r];
+          }
         }
       }
     }
@@ -89,7 +103,14 @@ __global__ __launch_bounds__(kWG) void k_ilu0_factor(const uint32_t* __restrict_
   }
 }
 
-// x = U^-1 L^-1 b
+// sum of v over the kGL lanes of a row group, the same in every lane (fixed butterfly order)
+__device__ inline double group_sum(double v)
+{
+  for(int off = 1; off < kGL; off <<= 1) v += __shfl_xor(v, off, kGL);
+  return v;
+}
+
+// x = U^-1 L^-1 b, a row group per row of a level
 __global__ __launch_bounds__(kWG) void k_ilu0_solve(const uint32_t* __restrict__ rp, const uint32_t* __restrict__ col,
                                                    const double* __restrict__ val, const uint32_t* __restrict__ diag,
                                                    const uint32_t* __restrict__ lorder, const uint32_t* __restrict__ lptr,
@@ -97,24 +118,340 @@ __global__ __launch_bounds__(kWG) void k_ilu0_solve(const uint32_t* __restrict__
                                                    const uint32_t* __restrict__ uptr, uint32_t nulev,
                                                    const double* __restrict__ b, double* __restrict__ x)
 {
+  const uint32_t g = threadIdx.x / kGL, lane = threadIdx.x % kGL;
   for(uint32_t l = 0; l < nlev; ++l) {
-    for(uint32_t t = lptr[l] + threadIdx.x; t < lptr[l + 1]; t += kWG) {
+    for(uint32_t t = lptr[l] + g; t < lptr[l + 1]; t += kGroups) {
       const uint32_t i = lorder[t];
-      double s = b[i];
-      for(uint32_t q = rp[i]; q < diag[i]; ++q) s -= val[q] * x[col[q]];
-      x[i] = s;
+      double s = 0.;
+      for(uint32_t q = rp[i] + lane; q < diag[i]; q += kGL) s += val[q] * x[col[q]];
+      s = group_sum(s);
+      if(lane == 0) x[i] = b[i] - s;
     }
     __syncthreads();
   }
   for(uint32_t l = 0; l < nulev; ++l) {
-    for(uint32_t t = uptr[l] + threadIdx.x; t < uptr[l + 1]; t += kWG) {
+    for(uint32_t t = uptr[l] + g; t < uptr[l + 1]; t += kGroups) {
       const uint32_t i = uorder[t];
-      double s = x[i];
-      for(uint32_t q = diag[i] + 1; q < rp[i + 1]; ++q) s -= val[q] * x[col[q]];
-      x[i] = s / val[diag[i]];
+      const uint32_t d = diag[i];
+      double s = 0.;
+      for(uint32_t q = d + 1 + lane; q < rp[i + 1]; q += kGL) s += val[q] * x[col[q]];
+      s = group_sum(s);
+      if(lane == 0) x[i] = (x[i] - s) / val[d];
     }
     __syncthreads();
   }
+}
+
+// ---- the whole solve in one workgroup (systems of at most kSmallMaxN dofs) ----
+constexpr int kSW = 1024;                       // threads of the single workgroup
+constexpr uint32_t kSmallMaxN = 6144;           // dofs of one LDS vector (48 KB)
+constexpr int kSR = (int)(kSmallMaxN / kSW);    // rows per thread (row i = t + kSW r)
+constexpr int kSWaves = kSW / 64;
+using arnoldi::kMaxV;                           // basis vectors (restart length <= 32)
+
+struct SmallArgs {
+  uint32_t n, nlev, nulev;
+  int m, max_iter;
+  double tol;
+  const uint32_t *prow, *pcol, *diag, *lorder, *lptr, *uorder, *uptr;
+  const double *aval, *fval, *b;
+  double *x, *V;
+  double* out;   // [0] iterations, [1] relative true residual at exit
+};
+
+// sum of v over the workgroup, the same value in every thread (waves in a fixed order)
+__device__ inline double wg_sum(double v, double* red)
+{
+  for(int off = 32; off > 0; off >>= 1) v += __shfl_down(v, off, 64);
+  __syncthreads();   // earlier readers of red are done
+  if((threadIdx.x & 63) == 0) red[threadIdx.x >> 6] = v;
+  __syncthreads();
+  double s = 0.;
+  for(int k = 0; k < kSWaves; ++k) s += red[k];
+  return s;
+}
+
+// c[k] = V_k . w for k < nv, w in registers (rows t + kSW r), into LDS c[] for every thread
+__device__ inline void wg_mdot(int nv, uint32_t n, const double* V, const double wr[kSR], double* red, double* c)
+{
+  const int t = threadIdx.x;
+  for(int k = 0; k < nv; ++k) {
+    double p = 0.;
+#pragma unroll
+    for(int r = 0; r < kSR; ++r) {
+      const uint32_t i = t + kSW * r;
+      if(i < n) p += V[(uint64_t)k * n + i] * wr[r];
+    }
+    for(int off = 32; off > 0; off >>= 1) p += __shfl_down(p, off, 64);
+    if((t & 63) == 0) red[(t >> 6) * kMaxV + k] = p;
+  }
+  __syncthreads();
+  if(t < nv) {
+    double s = 0.;
+    for(int w = 0; w < kSWaves; ++w) s += red[w * kMaxV + t];
+    c[t] = s;
+  }
+  __syncthreads();
+}
+
+// zs := U^-1 L^-1 zs in place (the factors of A in fval), level by level, a row group per row
+__device__ inline void ilu_solve_lds(const SmallArgs& a, double* zs)
+{
+  const uint32_t g = threadIdx.x / kGL, lane = threadIdx.x % kGL;
+  constexpr uint32_t kG = kSW / kGL;
+  for(uint32_t l = 0; l < a.nlev; ++l) {
+    for(uint32_t k = a.lptr[l] + g; k < a.lptr[l + 1]; k += kG) {
+      const uint32_t i = a.lorder[k];
+      double s = 0.;
+      for(uint32_t q = a.prow[i] + lane; q < a.diag[i]; q += kGL) s += a.fval[q] * zs[a.pcol[q]];
+      s = group_sum(s);
+      if(lane == 0) zs[i] -= s;
+    }
+    __syncthreads();
+  }
+  for(uint32_t l = 0; l < a.nulev; ++l) {
+    for(uint32_t k = a.uptr[l] + g; k < a.uptr[l + 1]; k += kG) {
+      const uint32_t i = a.uorder[k];
+      const uint32_t d = a.diag[i];
+      double s = 0.;
+      for(uint32_t q = d + 1 + lane; q < a.prow[i + 1]; q += kGL) s += a.fval[q] * zs[a.pcol[q]];
+      s = group_sum(s);
+      if(lane == 0) zs[i] = (zs[i] - s) / a.fval[d];
+    }
+    __syncthreads();
+  }
+}
+
+// wr = A zs for the thread's rows
+__device__ inline void spmv_lds(const SmallArgs& a, const double* zs, double wr[kSR])
+{
+#pragma unroll
+  for(int r = 0; r < kSR; ++r) {
+    const uint32_t i = threadIdx.x + kSW * r;
+    double s = 0.;
+    if(i < a.n)
+      for(uint32_t q = a.prow[i]; q < a.prow[i + 1]; ++q) s += a.aval[q] * zs[a.pcol[q]];
+    wr[r] = s;
+  }
+}
+
+// Restarted GMRES(m) with ILU(0) right preconditioning, classical Gram-Schmidt twice and Givens
+// rotations (the method of the multi-launch loop below and of the oracle's orc_gmres_ilu0).  A cycle
+// ends when the recurrence residual meets the tolerance or at the restart length; the true residual
+// b - A x at the start of the next cycle decides convergence.
+__global__ __launch_bounds__(kSW) void k_gmres_small(SmallArgs a)
+{
+  __shared__ double zs[kSmallMaxN];
+  __shared__ double red[kSWaves * kMaxV];
+  __shared__ double H[kMaxV * (kMaxV - 1)];
+  __shared__ double cs[kMaxV], sn[kMaxV], g[kMaxV], hc[kMaxV], y[kMaxV], cc[kMaxV];
+  __shared__ double s_res;
+  const uint32_t t = threadIdx.x;
+  const uint32_t n = a.n;
+  const int m = a.m;
+  double wr[kSR];
+  double p = 0.;
+#pragma unroll
+  for(int r = 0; r < kSR; ++r) {
+    const uint32_t i = t + kSW * r;
+    if(i < n) p += a.b[i] * a.b[i];
+  }
+  double bn = sqrt(wg_sum(p, red));
+  if(bn == 0.) bn = 1.;
+  const double thr = a.tol * bn;
+  int it = 0;
+  double rel = 0.;
+  for(;;) {
+    // w = b - A x (x staged in LDS; each thread stages and updates its own rows)
+#pragma unroll
+    for(int r = 0; r < kSR; ++r) {
+      const uint32_t i = t + kSW * r;
+      if(i < n) zs[i] = a.x[i];
+    }
+    __syncthreads();
+    spmv_lds(a, zs, wr);
+    p = 0.;
+#pragma unroll
+    for(int r = 0; r < kSR; ++r) {
+      const uint32_t i = t + kSW * r;
+      wr[r] = i < n ? a.b[i] - wr[r] : 0.;
+      p += wr[r] * wr[r];
+    }
+    const double beta = sqrt(wg_sum(p, red));
+    rel = beta / bn;
+    if(beta <= thr || it >= a.max_iter) break;
+#pragma unroll
+    for(int r = 0; r < kSR; ++r) {
+      const uint32_t i = t + kSW * r;
+      if(i < n) a.V[i] = wr[r] / beta;
+    }
+    if(t == 0) {
+      g[0] = beta;
+      for(int k = 1; k <= m; ++k) g[k] = 0.;
+    }
+    int j = 0;
+    for(; j < m && it < a.max_iter; ++j) {
+      ++it;
+      // z = M^-1 V_j (V_j rows were written by this same thread)
+#pragma unroll
+      for(int r = 0; r < kSR; ++r) {
+        const uint32_t i = t + kSW * r;
+        if(i < n) zs[i] = a.V[(uint64_t)j * n + i];
+      }
+      __syncthreads();
+      ilu_solve_lds(a, zs);
+      spmv_lds(a, zs, wr);
+      // two passes of classical Gram-Schmidt against V_0 .. V_j
+      for(int pass = 0; pass < 2; ++pass) {
+        wg_mdot(j + 1, n, a.V, wr, red, cc);
+#pragma unroll
+        for(int r = 0; r < kSR; ++r) {
+          const uint32_t i = t + kSW * r;
+          if(i < n) {
+            double s = wr[r];
+            for(int k = 0; k <= j; ++k) s -= cc[k] * a.V[(uint64_t)k * n + i];
+            wr[r] = s;
+          }
+        }
+        if((int)t <= j) hc[t] = pass == 0 ? cc[t] : hc[t] + cc[t];
+      }
+      p = 0.;
+#pragma unroll
+      for(int r = 0; r < kSR; ++r) p += wr[r] * wr[r];
+      const double hn = sqrt(wg_sum(p, red));
+#pragma unroll
+      for(int r = 0; r < kSR; ++r) {
+        const uint32_t i = t + kSW * r;
+        if(i < n) a.V[(uint64_t)(j + 1) * n + i] = wr[r] / hn;
+      }
+      if(t == 0) {
+        hc[j + 1] = hn;
+        for(int k = 0; k < j; ++k) {
+          const double t0 = cs[k] * hc[k] + sn[k] * hc[k + 1];
+          hc[k + 1] = -sn[k] * hc[k] + cs[k] * hc[k + 1];
+          hc[k] = t0;
+        }
+        const double den = sqrt(hc[j] * hc[j] + hc[j + 1] * hc[j + 1]);
+        cs[j] = hc[j] / den;
+        sn[j] = hc[j + 1] / den;
+        hc[j] = den;
+        hc[j + 1] = 0.;
+        g[j + 1] = -sn[j] * g[j];
+        g[j] = cs[j] * g[j];
+        for(int k = 0; k <= j; ++k) H[k * m + j] = hc[k];
+        s_res = fabs(g[j + 1]);
+      }
+      __syncthreads();
+      if(!(s_res > thr)) { ++j; break; }
+    }
+    // x += M^-1 V y, H y = g (upper triangular, j x j)
+    if(t == 0)
+      for(int i = j - 1; i >= 0; --i) {
+        double s = g[i];
+        for(int l = i + 1; l < j; ++l) s -= H[i * m + l] * y[l];
+        y[i] = s / H[i * m + i];
+      }
+    __syncthreads();
+#pragma unroll
+    for(int r = 0; r < kSR; ++r) {
+      const uint32_t i = t + kSW * r;
+      if(i < n) {
+        double s = 0.;
+        for(int k = 0; k < j; ++k) s += y[k] * a.V[(uint64_t)k * n + i];
+        zs[i] = s;
+      }
+    }
+    __syncthreads();
+    ilu_solve_lds(a, zs);
+#pragma unroll
+    for(int r = 0; r < kSR; ++r) {
+      const uint32_t i = t + kSW * r;
+      if(i < n) a.x[i] += zs[i];
+    }
+    __syncthreads();   // zs is restaged from x next
+  }
+  if(t == 0) {
+    a.out[0] = (double)it;
+    a.out[1] = rel;
+  }
+}
+
+// level schedules of the point pattern (natural order), on the host: a row's lower level is one more
+// than the largest of its lower columns', its upper level likewise over its upper columns from the end
+void build_ilu_plan(Lss& L)
+{
+  Context& c = *L.ctx;
+  const uint32_t N = L.n_nodes;
+  const int neq = L.neq;
+  std::vector<uint32_t> rp(N + 1), cl(L.n_blocks);
+  HIP_OK(hipMemcpyAsync(rp.data(), L.row_ptr.ptr, sizeof(uint32_t) * (N + 1), hipMemcpyDeviceToHost, c.stream));
+  HIP_OK(hipMemcpyAsync(cl.data(), L.cols.ptr, sizeof(uint32_t) * L.n_blocks, hipMemcpyDeviceToHost, c.stream));
+  HIP_OK(hipStreamSynchronize(c.stream));
+  const uint64_t n = (uint64_t)N * neq;
+  // point row r = i neq + a holds, for each block column s of row i, columns cl[s] neq + b (ascending)
+  std::vector<uint32_t> prow(n + 1), diag(n), lev(n), ulev(n);
+  const uint64_t nn = (uint64_t)neq * neq;
+  for(uint32_t i = 0; i < N; ++i) {
+    const uint32_t deg = rp[i + 1] - rp[i];
+    uint32_t si = 0;
+    for(uint32_t s2 = 0; s2 < deg; ++s2)
+      if(cl[rp[i] + s2] == i) si = s2;
+    for(int a2 = 0; a2 < neq; ++a2) {
+      const uint64_t r = (uint64_t)i * neq + a2;
+      prow[r] = (uint32_t)(nn * rp[i] + (uint64_t)a2 * neq * deg);
+      diag[r] = prow[r] + si * neq + a2;
+    }
+  }
+  prow[n] = (uint32_t)(nn * rp[N]);
+  auto col_of = [&](uint32_t i, uint32_t k) {   // k-th entry of point row (i, .): column
+    const uint32_t deg = rp[i + 1] - rp[i];
+    (void)deg;
+    return cl[rp[i] + k / neq] * neq + k % neq;
+  };
+  uint32_t lmax = 0, umax = 0;
+  for(uint64_t r = 0; r < n; ++r) {
+    const uint32_t i = (uint32_t)(r / neq);
+    uint32_t l = 0;
+    for(uint32_t q = prow[r]; q < diag[r]; ++q) l = std::max(l, lev[col_of(i, q - prow[r])] + 1);
+    lev[r] = l;
+    lmax = std::max(lmax, l);
+  }
+  for(uint64_t r = n; r-- > 0;) {
+    const uint32_t i = (uint32_t)(r / neq);
+    uint32_t l = 0;
+    for(uint32_t q = diag[r] + 1; q < prow[r + 1]; ++q) l = std::max(l, ulev[col_of(i, q - prow[r])] + 1);
+    ulev[r] = l;
+    umax = std::max(umax, l);
+  }
+  auto order = [&](const std::vector<uint32_t>& lv, uint32_t nl, std::vector<uint32_t>& ord, std::vector<uint32_t>& ptr) {
+    ptr.assign(nl + 1, 0);
+    for(uint64_t r = 0; r < n; ++r) ++ptr[lv[r] + 1];
+    for(uint32_t l = 0; l < nl; ++l) ptr[l + 1] += ptr[l];
+    std::vector<uint32_t> fill(ptr.begin(), ptr.end() - 1);
+    ord.resize(n);
+    for(uint64_t r = 0; r < n; ++r) ord[fill[lv[r]]++] = (uint32_t)r;
+  };
+  std::vector<uint32_t> lorder, lptr, uorder, uptr;
+  order(lev, lmax + 1, lorder, lptr);
+  order(ulev, umax + 1, uorder, uptr);
+  auto up = [&](DevBuf& d, const std::vector<uint32_t>& h) {
+    d.alloc(&c, sizeof(uint32_t) * h.size());
+    HIP_OK(hipMemcpyAsync(d.ptr, h.data(), sizeof(uint32_t) * h.size(), hipMemcpyHostToDevice, c.stream));
+  };
+  up(L.ilu.diag, diag);
+  up(L.ilu.lorder, lorder);
+  up(L.ilu.lptr, lptr);
+  up(L.ilu.uorder, uorder);
+  up(L.ilu.uptr, uptr);
+  L.ilu.prow.alloc(&c, sizeof(uint32_t) * (n + 1));
+  L.ilu.pcol.alloc(&c, sizeof(uint32_t) * prow[n]);
+  L.ilu.aval.alloc(&c, sizeof(double) * prow[n]);
+  L.ilu.fval.alloc(&c, sizeof(double) * prow[n]);
+  L.ilu.out.alloc(&c, sizeof(double) * 2);
+  HIP_OK(hipStreamSynchronize(c.stream));   // the host vectors are released on return
+  L.ilu.nlev = lmax + 1;
+  L.ilu.nulev = umax + 1;
+  L.ilu.ready = true;
 }
 
 }  // namespace
@@ -129,43 +466,46 @@ void gmres_ilu0(Lss& L, const cf3hip_solver_params& prm, cf3hip_solve_stats& st)
   const uint64_t n = L.dofs;
   const uint64_t nnz = L.n_blocks * L.neq * L.neq;
   if(n >= 0xffffffffull || nnz >= 0xffffffffull) CF3_THROW(CF3HIP_ERR_LIMIT, "system too large for GMRES+ILU0");
-  // point CSR copy of the matrix, factorised in place (rebuilt at every solve like the reference)
-  DevBuf prow, pcol, pval, diag, lev, ulev, lorder, lptr, uorder, uptr, counts;
-  prow.alloc(&c, sizeof(uint32_t) * (n + 1));
-  pcol.alloc(&c, sizeof(uint32_t) * nnz);
-  pval.alloc(&c, sizeof(double) * nnz);
-  diag.alloc(&c, sizeof(uint32_t) * n);
-  lev.alloc(&c, sizeof(uint32_t) * n);
-  ulev.alloc(&c, sizeof(uint32_t) * n);
-  lorder.alloc(&c, sizeof(uint32_t) * n);
-  uorder.alloc(&c, sizeof(uint32_t) * n);
-  lptr.alloc(&c, sizeof(uint32_t) * (n + 2));
-  uptr.alloc(&c, sizeof(uint32_t) * (n + 2));
-  counts.alloc(&c, sizeof(uint32_t) * (n + 2));
-  lss_point_csr(L, prow.as<uint32_t>(), pcol.as<uint32_t>(), pval.as<double>());
-  hipLaunchKernelGGL(k_levels, dim3(1), dim3(1), 0, c.stream, (uint32_t)n, prow.as<uint32_t>(), pcol.as<uint32_t>(),
-                     diag.as<uint32_t>(), lev.as<uint32_t>(), ulev.as<uint32_t>(), lorder.as<uint32_t>(),
-                     lptr.as<uint32_t>(), uorder.as<uint32_t>(), uptr.as<uint32_t>(), counts.as<uint32_t>());
+  // point CSR copy of the matrix, factorised in place at every solve like the reference; the level
+  // schedules of its pattern are kept with the system
+  if(!L.ilu.ready) build_ilu_plan(L);
+  auto& I = L.ilu;
+  lss_point_csr(L, I.prow.as<uint32_t>(), I.pcol.as<uint32_t>(), I.aval.as<double>());
+  HIP_OK(hipMemcpyAsync(I.fval.ptr, I.aval.ptr, I.aval.bytes, hipMemcpyDeviceToDevice, c.stream));
+  hipLaunchKernelGGL(k_ilu0_factor, dim3(1), dim3(kWG), 0, c.stream, I.prow.as<uint32_t>(), I.pcol.as<uint32_t>(),
+                     I.fval.as<double>(), I.diag.as<uint32_t>(), I.lorder.as<uint32_t>(), I.lptr.as<uint32_t>(), I.nlev);
   HIP_OK(hipGetLastError());
-  uint32_t nl[2];
-  HIP_OK(hipMemcpyAsync(nl, counts.ptr, sizeof(nl), hipMemcpyDeviceToHost, c.stream));
-  HIP_OK(hipStreamSynchronize(c.stream));
-  hipLaunchKernelGGL(k_ilu0_factor, dim3(1), dim3(kWG), 0, c.stream, prow.as<uint32_t>(), pcol.as<uint32_t>(),
-                     pval.as<double>(), diag.as<uint32_t>(), lorder.as<uint32_t>(), lptr.as<uint32_t>(), nl[0]);
-  HIP_OK(hipGetLastError());
+  if(I.V.bytes < sizeof(double) * (m + 1) * n) I.V.alloc(&c, sizeof(double) * (m + 1) * n);
+  if(n <= kSmallMaxN) {
+    SmallArgs a{(uint32_t)n, I.nlev, I.nulev, m, prm.max_iter, prm.tol,
+                I.prow.as<uint32_t>(), I.pcol.as<uint32_t>(), I.diag.as<uint32_t>(), I.lorder.as<uint32_t>(),
+                I.lptr.as<uint32_t>(), I.uorder.as<uint32_t>(), I.uptr.as<uint32_t>(), I.aval.as<double>(),
+                I.fval.as<double>(), L.rhs.as<double>(), L.sol.as<double>(), I.V.as<double>(), I.out.as<double>()};
+    hipLaunchKernelGGL(k_gmres_small, dim3(1), dim3(kSW), 0, c.stream, a);
+    HIP_OK(hipGetLastError());
+    double o[2];
+    HIP_OK(hipMemcpyAsync(o, I.out.ptr, sizeof(o), hipMemcpyDeviceToHost, c.stream));
+    HIP_OK(hipStreamSynchronize(c.stream));
+    st.iterations = (int)o[0];
+    st.outer = 1;
+    st.spmvs = 0;
+    st.rel_residual = o[1];
+    st.converged = o[1] <= prm.tol ? 1 : 0;
+    return;
+  }
   auto precond = [&](const double* b, double* x) {
-    hipLaunchKernelGGL(k_ilu0_solve, dim3(1), dim3(kWG), 0, c.stream, prow.as<uint32_t>(), pcol.as<uint32_t>(),
-                       pval.as<double>(), diag.as<uint32_t>(), lorder.as<uint32_t>(), lptr.as<uint32_t>(), nl[0],
-                       uorder.as<uint32_t>(), uptr.as<uint32_t>(), nl[1], b, x);
+    hipLaunchKernelGGL(k_ilu0_solve, dim3(1), dim3(kWG), 0, c.stream, I.prow.as<uint32_t>(), I.pcol.as<uint32_t>(),
+                       I.fval.as<double>(), I.diag.as<uint32_t>(), I.lorder.as<uint32_t>(), I.lptr.as<uint32_t>(),
+                       I.nlev, I.uorder.as<uint32_t>(), I.uptr.as<uint32_t>(), I.nulev, b, x);
   };
 
   // workspace: V (m+1) n, w, z, u ; small: bookkeeping, h1, h2, y, scalars
-  DevBuf V, w, z, small;
-  V.alloc(&c, sizeof(double) * (m + 1) * n);
-  w.alloc(&c, sizeof(double) * n);
-  z.alloc(&c, sizeof(double) * n);
-  small.alloc(&c, sizeof(double) * GmresDev::size(m));
-  double* base = small.as<double>();
+  if(I.w.bytes < sizeof(double) * n) I.w.alloc(&c, sizeof(double) * n);
+  if(I.z.bytes < sizeof(double) * n) I.z.alloc(&c, sizeof(double) * n);
+  if(I.small.bytes < sizeof(double) * GmresDev::size(m)) I.small.alloc(&c, sizeof(double) * GmresDev::size(m));
+  DevBuf& w = I.w;
+  DevBuf& z = I.z;
+  double* base = I.small.as<double>();
   const GmresDev gd{m, base};
   double* h1 = gd.h1();
   double* h2 = gd.h2();
@@ -174,7 +514,7 @@ void gmres_ilu0(Lss& L, const cf3hip_solver_params& prm, cf3hip_solve_stats& st)
   const unsigned gb = grid_for(n, 256);
   double* x = L.sol.as<double>();
   const double* b = L.rhs.as<double>();
-  double* Vd = V.as<double>();
+  double* Vd = I.V.as<double>();
 
   multidot(c, n, 1, b, b, sc + 1);
   double bn2 = 0.;

@@ -153,8 +153,16 @@ struct Lss {
   size_t work_vectors = 0;
   DevBuf ir;            // FP64 FGMRES workspace of the precision form (basis, directions, bookkeeping)
   int work_real_bytes = 0;
-  DevBuf ilu;           // ILU0 data (point CSR) for GMRES
-  bool ilu_ready = false;
+  // GMRES + ILU(0): the point pattern's level schedules (built once per system, on the host) and the
+  // per-solve point values, factors and Krylov workspace (kept between solves)
+  struct {
+    bool ready = false;
+    uint32_t nlev = 0, nulev = 0;
+    DevBuf prow, pcol, diag, lorder, lptr, uorder, uptr;
+    DevBuf aval, fval;
+    DevBuf V, w, z, small, out;
+    int m_ws = 0;
+  } ilu;
 };
 
 struct Fields {
