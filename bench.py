@@ -455,7 +455,7 @@ def main():
     ap.add_argument("--precision-cells", type=int, default=128)
     ap.add_argument("--hexa-cells", type=int, default=160)
     ap.add_argument("--picard", type=int, default=2)
-    ap.add_argument("--serving-requests", type=int, default=128)
+    ap.add_argument("--serving-requests", type=int, default=512)
     ap.add_argument("--serving-slots", type=int, default=16)
     ap.add_argument("--serving-mode", default="both", choices=("both", "sequential", "batched"))
     args = ap.parse_args()

@@ -89,3 +89,47 @@ def test_mix_of_requests_is_deterministic():
         assert np.abs(m.u - s.u).max() <= 1e-12 * np.abs(s.u).max()
         assert np.abs(m.p - s.p).max() <= 1e-12 * max(np.abs(s.p).max(), 1.0)
     assert summary["all_converged"]
+
+
+@pytest.mark.gpu
+def test_continuous_progress_and_admission():
+    """No tick barrier: a small request admitted together with a large one finishes long before it; a
+    request whose memory estimate exceeds the whole HBM budget is rejected instead of blocking the queue;
+    the estimate charged on admission covers the device memory each request ends with."""
+    big, small = serving.Request(0, 0.0, 48, 200.0, 0.05, steps=3), serving.Request(1, 0.0, 5, 200.0, 0.05, steps=3)
+    huge = serving.Request(2, 0.0, 300, 200.0, 0.05, steps=3)
+    budget = 4 * serving.device_bytes_estimate(48)
+    assert serving.device_bytes_estimate(300) > budget
+    srv = serving.Server(max_active=4, hbm_budget=budget)
+    recs, summary = srv.run([big, huge, small])
+    by = {r.rid: r for r in recs}
+    print(f"big done {by[0].done:.3f} s, small done {by[1].done:.3f} s; {summary}")
+    assert by[2].rejected and not by[0].rejected and not by[1].rejected
+    assert summary["rejected"] == 1 and summary["all_converged"]
+    assert by[1].done < 0.5 * by[0].done
+    for r in (by[0], by[1]):
+        assert 0 < r.device_bytes <= r.bytes_estimate, (r.rid, r.device_bytes, r.bytes_estimate)
+
+
+@pytest.mark.gpu
+@pytest.mark.timeout(900)
+def test_serving_stream_512():
+    """The 512-request stream (Poisson arrivals at 200 req/s, cells log-uniform in [12, 64], Re in
+    [10, 1000], 10 steps each) through the continuous server: every request converges at every step, and
+    a sample of requests gives the same fields served alone as in the mix."""
+    reqs = serving.make_requests(512)
+    sample = [r.rid for r in sorted(reqs, key=lambda r: r.cells)[::128]]
+    srv = serving.Server(max_active=16, keep_fields=set(sample), log=lambda m: print("serving: " + m, flush=True))
+    recs, summary = srv.run(reqs)
+    print("stream:", summary, flush=True)
+    assert summary["requests"] == 512 and summary["rejected"] == 0 and summary["all_converged"]
+    by = {r.rid: r for r in recs}
+    alone_srv = serving.Server(max_active=1, keep_fields=True)
+    for rid in sample:
+        r = [q for q in reqs if q.rid == rid][0]
+        a = alone_srv.run([serving.Request(r.rid, 0.0, r.cells, r.re, r.amp, r.steps)])[0][0]
+        m = by[rid]
+        du = np.abs(a.u - m.u).max() / np.abs(a.u).max()
+        print(f"request {rid} ({r.cells}^3): alone vs mix velocity difference {du:.1e}", flush=True)
+        assert a.iterations == m.iterations
+        assert du <= 1e-12 and np.abs(a.p - m.p).max() <= 1e-12 * max(np.abs(a.p).max(), 1.0)
