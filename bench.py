@@ -295,7 +295,7 @@ def semi_implicit_workload(ctx, scale_segments=512, scale_steps=2, with_cpu=True
     """The semi-implicit half of the Taylor-Green script (NavierStokesSemiImplicit, two inner passes,
     BiCGStab + Jacobi to 1e-13 for the velocity and pressure systems): errors after each step at 32
     segments, then the same step on scale_segments^2 quadrilaterals with the time of the velocity-matrix
-    assembly (element kernel + deterministic gather) and of one step."""
+    assembly (wall time of the call, and the kernel time of the fused row-owner kernel) and of one step."""
     from coolfluid3_amd import cabi, cases
     sp = cabi.solver_params(cabi.BICGSTAB_JACOBI, tol=1e-13, max_iter=20000)
 
@@ -356,6 +356,14 @@ def semi_implicit_workload(ctx, scale_segments=512, scale_steps=2, with_cpu=True
     for _ in range(reps):
         si.velocity_assemble(f, prm)
     asm_ms = (time.perf_counter() - t0) * 1e3 / reps
+    # kernel time of the fused row-owner velocity assembly (HIP events inside the library)
+    ctx.enable_timing(True)
+    for _ in range(reps):
+        si.velocity_assemble(f, prm)
+    ctx.synchronize()
+    kms, kn = ctx.kernel_time(cabi.KERNEL_ASSEMBLY)
+    ctx.enable_timing(False)
+    kms = kms / kn if kn else None
     E, N = case.conn.shape[0], case.n_nodes
     # compulsory bytes: coordinates, advection velocity, viscosity, connectivity, 2x2 blocks written
     alg = N * (2 * 8 + 2 * 8 + 8) + E * 4 * 4 + us.n_blocks * 4 * 8
@@ -379,6 +387,8 @@ def semi_implicit_workload(ctx, scale_segments=512, scale_steps=2, with_cpu=True
                     "velocity_assembly_ms": asm_ms, "velocity_assembly_melem_s": E / asm_ms / 1e3,
                     "velocity_assembly_algorithmic_gb_s": alg / asm_ms / 1e6,
                     "velocity_assembly_hbm_frac": alg / asm_ms / 1e6 / 8000.0,
+                    "velocity_assembly_kernel_ms": kms,
+                    "velocity_assembly_kernel_hbm_frac": alg / kms / 1e6 / 8000.0 if kms else None,
                     "velocity_assembly_cpu_baseline": cpu}
     return out
 

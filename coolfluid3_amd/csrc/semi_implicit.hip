@@ -9,7 +9,7 @@
 // vector row) one thread that adds its contributions in element order, so the sums are deterministic
 // and need no atomics.  The contribution lists are the element-to-slot map of the velocity system (the
 // two systems have the same node graph) and the rows p2m[conn], sorted stably once at set-up.
-#include "internal.hpp"
+#include "assembly_common.hpp"
 
 #include <algorithm>
 #include <vector>
@@ -507,6 +507,178 @@ __global__ void __launch_bounds__(256) k_si_elem(SiArgs g)
   }
 }
 
+// VelocityAssembly on Q1 cells without the element staging buffer: the row-owner form of the velocity
+// system's own scatter maps (one staging pass, K lanes per incidence, as in the Q1 Navier-Stokes kernel).
+// A pre-pass (k_si_ctr_q1, one thread per element) writes each element's tau_bulk, w det J tau_bulk at
+// the centre and the gradients there (the one-point rule of the off-diagonal bulk blocks).  In the
+// row-owner kernel lane q of an incidence (element e, local node a) computes the data of Gauss point q of
+// the second-order rule (w det J, w det J nu_q, N and physical gradients at q) into LDS; lane b then forms
+// the block (a, b) from those records with the arithmetic of k_si_elem's OP_VMAT, point by point in the
+// same order.  The blocks are summed in LDS by the block's accumulator threads in a fixed order and every
+// block of the workgroup's rows is written once: no atomics.
+struct SiMaps {
+  const uint32_t* wg_rows;
+  const uint32_t* row_ptr;
+  const uint32_t* inc_ptr;
+  const uint32_t* inc;
+  const uint8_t* inc_pos;
+  const uint8_t* blk_beg;
+  const uint16_t* blk_acc;
+  double* values;
+};
+
+template<class E>
+__global__ __launch_bounds__(256) void k_si_ctr_q1(SiArgs g, double* __restrict__ ctr)
+{
+  constexpr int K = E::K, D = E::D, kC = 2 + D * K;
+  const uint32_t e = blockIdx.x * blockDim.x + threadIdx.x;
+  if(e >= g.n_elems) return;
+  double X[K][D], ua[K][D], nu[K];
+  for(int n = 0; n < K; ++n) {
+    const uint32_t c = g.conn[(uint64_t)e * K + n];
+    for(int d = 0; d < D; ++d) { X[n][d] = g.x[d][c]; ua[n][d] = g.ua[d][c]; }
+    nu[n] = g.nu[c];
+  }
+  double tau_ps, tau_su, tau_bulk;
+  tau<E>(g.t, X, ua, nu, tau_ps, tau_su, tau_bulk);
+  double xi[D], w, N[K], G[D][K], det;
+  E::point(1, 0, xi, w);
+  point_data<E>(X, xi, N, G, det);
+  double* o = ctr + (uint64_t)e * kC;
+  o[0] = w * det * tau_bulk;
+  o[1] = tau_bulk;
+  for(int d = 0; d < D; ++d)
+    for(int n = 0; n < K; ++n) o[2 + d * K + n] = G[d][n];
+}
+
+template<class E>
+__global__ __launch_bounds__(asmk::kThreads) void k_si_vmat_q1(SiArgs g, SiMaps m, const double* __restrict__ ctrs)
+{
+  using namespace asmk;
+  constexpr int K = E::K, D = E::D, NN = D * D, kLanes = K, kSlots = kThreads / K;
+  constexpr int kPt = 2 + K + D * K;        // w det, w det nu, N[K], G[D][K]
+  constexpr int kC = 2 + D * K;             // pre-pass: w det tau_bulk at the centre, tau_bulk, Gc[D][K]
+  constexpr int kSlotW = K * kPt + 1;
+  constexpr int LDW = lds_row<double, NN>();
+  constexpr size_t kRecBytes = sizeof(double) * kSlots * kSlotW;
+  constexpr size_t kBlkBytes = sizeof(double) * kThreads * LDW;
+  __shared__ __align__(16) unsigned char s_raw[kRecBytes > kBlkBytes ? kRecBytes : kBlkBytes];
+  double* s_rec = reinterpret_cast<double*>(s_raw);   // point records, then
+  double* s_blk = reinterpret_cast<double*>(s_raw);   // staged blocks
+  __shared__ uint16_t s_acc[kThreads];
+
+  const uint32_t wg = xcd_order(blockIdx.x, gridDim.x);
+  const uint32_t r0 = m.wg_rows[wg], r1 = m.wg_rows[wg + 1];
+  const uint32_t i0 = m.inc_ptr[r0];
+  const uint32_t ninc = m.inc_ptr[r1] - i0;
+  const uint32_t blk0 = m.row_ptr[r0];
+  const uint32_t nblk = m.row_ptr[r1] - blk0;
+  const int t = threadIdx.x;
+  const int slot = t / kLanes, lane = t % kLanes;
+  if((uint32_t)t < nblk) s_acc[t] = m.blk_acc[blk0 + t];
+  const bool active = (uint32_t)slot < ninc;
+  int a = 0;
+  uint32_t pos = 0, e = 0;
+  if(active) {
+    const uint32_t ea = m.inc[i0 + slot];
+    e = ea / K;
+    a = (int)(ea % K);
+    pos = m.inc_pos[(uint64_t)(i0 + slot) * kLanes + lane];
+    double X[K][D], nu[K];
+    for(int n = 0; n < K; ++n) {
+      const uint32_t c = g.conn[(uint64_t)e * K + n];
+      for(int d = 0; d < D; ++d) X[n][d] = g.x[d][c];
+      nu[n] = g.nu[c];
+    }
+    double xi[D], w, N[K], G[D][K], det;
+    E::point(2, lane, xi, w);
+    point_data<E>(X, xi, N, G, det);
+    double nuq = 0.;
+    for(int n = 0; n < K; ++n) nuq += N[n] * nu[n];
+    const double wq = w * det;
+    double* rec = s_rec + (size_t)slot * kSlotW + lane * kPt;
+    rec[0] = wq;
+    rec[1] = wq * nuq;
+    for(int n = 0; n < K; ++n) rec[2 + n] = N[n];
+    for(int d = 0; d < D; ++d)
+      for(int n = 0; n < K; ++n) rec[2 + K + d * K + n] = G[d][n];
+  }
+  __syncthreads();
+  // the run of staged rows each accumulator thread sums
+  const uint32_t alast = s_acc[nblk - 1];
+  const uint32_t n_acc = (alast & 255u) + (alast >> 8);
+  const bool owner = (uint32_t)t < n_acc;
+  uint32_t jb = 0, je = 0, h = 0, c = 1;
+  if(owner) {
+    uint32_t lo = 0, hi = nblk;
+    while(hi - lo > 1) {
+      const uint32_t mid = (lo + hi) >> 1;
+      if((s_acc[mid] & 255u) <= (uint32_t)t) lo = mid; else hi = mid;
+    }
+    h = (uint32_t)t - (s_acc[lo] & 255u);
+    c = s_acc[lo] >> 8;
+    jb = m.blk_beg[(uint64_t)(blk0 + lo) * 4];
+    je = lo + 1 < nblk ? m.blk_beg[(uint64_t)(blk0 + lo + 1) * 4] : ninc * kLanes;
+  }
+  // block (a, b = lane): T + theta dt (L + B) on the diagonal, the one-point bulk term off it
+  double M[NN];
+  if(active) {
+    const int b = lane;
+    const double* recs = s_rec + (size_t)slot * kSlotW;
+    const double* ctr = ctrs + (uint64_t)e * kC;
+    const double tau_bulk = ctr[1];
+    double T = 0., L = 0., B[D];
+    for(int d = 0; d < D; ++d) B[d] = 0.;
+    for(int q = 0; q < K; ++q) {   // Q1: the second-order rule has K points, the order of k_si_elem
+      const double* r = recs + q * kPt;
+      const double wq = r[0], wn = r[1];
+      const double* N = r + 2;
+      const double* G = r + 2 + K;
+      T += (wq * N[a]) * N[b];
+      const double bulk = wq * tau_bulk;
+      double gg = 0.;
+      for(int d = 0; d < D; ++d) {
+        gg += G[d * K + a] * G[d * K + b];
+        B[d] += (bulk * G[d * K + a]) * G[d * K + b];
+      }
+      L += wn * gg;
+    }
+    const double bulk1 = ctr[0], tdt = g.theta * g.t.dt;
+    const double* Gc = ctr + 2;
+    for(int r = 0; r < D; ++r)
+      for(int cc = 0; cc < D; ++cc)
+        M[r * D + cc] = r == cc ? T + tdt * (L + B[r]) : tdt * ((bulk1 * Gc[r * K + a]) * Gc[cc * K + b]);
+  }
+  __syncthreads();   // records are no longer read
+  if(active) {
+    double* d = s_blk + pos * LDW;
+    for(int v = 0; v < NN; ++v) d[v] = M[v];
+  }
+  __syncthreads();
+  double acc[NN];
+  for(int v = 0; v < NN; ++v) acc[v] = 0.;
+  if(owner)
+    for(uint32_t j = jb + h; j < je; j += c) {
+      double v[NN];
+      lds_load_row<double, NN>(s_blk + j * LDW, v);
+      for(int q = 0; q < NN; ++q) acc[q] += v[q];
+    }
+  __syncthreads();
+  if(owner) {
+    double* d = s_blk + t * LDW;
+    for(int q = 0; q < NN; ++q) d[q] = acc[q];
+  }
+  __syncthreads();
+  double* dst = m.values + (uint64_t)blk0 * NN;
+  for(uint32_t o = t; o < nblk * NN; o += kThreads) {
+    const uint32_t sb = o / NN, v = o % NN;
+    const uint32_t ac = s_acc[sb], f = ac & 255u, na = ac >> 8;
+    double sum = s_blk[f * LDW + v];
+    for(uint32_t r = 1; r < na; ++r) sum += s_blk[(f + r) * LDW + v];
+    dst[o] = sum;
+  }
+}
+
 // out[r * width + c] = (add ? out : 0) + sum over j in [ptr[r], ptr[r+1]) of stage[idx[j] * width + c],
 // in list (element) order
 __global__ void __launch_bounds__(256) k_si_gather(uint32_t n, int width, const uint32_t* __restrict__ ptr,
@@ -743,7 +915,28 @@ void si_velocity_assemble(SemiImplicit& S, Fields& F, const cf3hip_ns_params& pr
   lss_clear_conditions(U);
   lss_fill(c, U.rhs.as<double>(), U.dofs, 0.);
   lss_fill(c, U.sol.as<double>(), U.dofs, 0.);
-  launch_elem<OP_VMAT>(S, si_args(S, F, prm));
+  const SiArgs g = si_args(S, F, prm);
+  const int et = S.mesh->etype;
+  if((et == CF3HIP_QUAD2D || et == CF3HIP_HEXA3D) && U.passes == 1 && U.pos_stride == (int)S.mesh->k && U.n_wg) {
+    const SiMaps mp{U.wg_rows.as<uint32_t>(), U.row_ptr.as<uint32_t>(), U.inc_ptr.as<uint32_t>(), U.inc.as<uint32_t>(),
+                    U.inc_pos.as<uint8_t>(), U.blk_beg.as<uint8_t>(), U.blk_acc.as<uint16_t>(), U.values.as<double>()};
+    // the per-element centre records go to the element staging buffer of the two-pass path (unused here;
+    // 2 + D K doubles per element against its K^2 D^2)
+    double* ctr = S.stage.as<double>();
+    const dim3 ge(grid_for(S.mesh->n_elems, 256));
+    timer_begin(c, CF3HIP_KERNEL_ASSEMBLY);
+    if(et == CF3HIP_QUAD2D) {
+      hipLaunchKernelGGL(k_si_ctr_q1<Quad>, ge, dim3(256), 0, c.stream, g, ctr);
+      hipLaunchKernelGGL(k_si_vmat_q1<Quad>, dim3(U.n_wg), dim3(asmk::kThreads), 0, c.stream, g, mp, ctr);
+    } else {
+      hipLaunchKernelGGL(k_si_ctr_q1<Hex>, ge, dim3(256), 0, c.stream, g, ctr);
+      hipLaunchKernelGGL(k_si_vmat_q1<Hex>, dim3(U.n_wg), dim3(asmk::kThreads), 0, c.stream, g, mp, ctr);
+    }
+    HIP_OK(hipGetLastError());
+    timer_end(c, CF3HIP_KERNEL_ASSEMBLY);
+    return;
+  }
+  launch_elem<OP_VMAT>(S, g);
   gather(S, true, U.neq * U.neq, (uint32_t)U.n_blocks, U.values.as<double>(), false);
 }
 
