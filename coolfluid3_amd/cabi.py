@@ -19,6 +19,7 @@ SUPG_TEZDUYAR, SUPG_METRIC, SUPG_CF2 = 0, 1, 2
 FP64, FP32 = 0, 1
 RHS, SOLUTION = 0, 1
 BICGSTAB_JACOBI, GMRES_ILU0 = 0, 1
+BUILDER_VBR, BUILDER_CRS = 0, 1
 FIELD_U, FIELD_P, FIELD_UADV, FIELD_U1, FIELD_U2, FIELD_U3, FIELD_NU, FIELD_G = range(8)
 KERNEL_ASSEMBLY, KERNEL_SPMV, KERNEL_SPMV_FP32, KERNEL_RESIDUAL = range(4)
 
@@ -108,6 +109,11 @@ def lib():
         "cf3hip_matrix_get_values": [_vp, _d],
         "cf3hip_matrix_get_values_fp32": [_vp, _f],
         "cf3hip_matrix_debug_data": [_vp, _u32, _u32, _d],
+        "cf3hip_lss_set_builder": [_vp, C.c_int],
+        "cf3hip_matrix_get_column_and_replace_to_zero": [_vp, C.c_uint32, C.c_uint32, _d],
+        "cf3hip_lss_get_builder": [_vp, C.POINTER(C.c_int)],
+        "cf3hip_matrix_clone_to": [_vp, _vp],
+        "cf3hip_matrix_read_native": [_vp, C.c_char_p],
         "cf3hip_vector_reset": [_vp, C.c_int, C.c_double],
         "cf3hip_vector_get": [_vp, C.c_int, _d],
         "cf3hip_vector_set": [_vp, C.c_int, _d],
@@ -356,6 +362,27 @@ class System:
         v = np.zeros(nnz)
         check(lib().cf3hip_matrix_debug_data(self.h, _p(rp, _u32), _p(ci, _u32), _p(v)))
         return rp, ci, v
+
+    def set_builder(self, builder):
+        """Matrix builder semantics: BUILDER_VBR (TrilinosFEVbrMatrix) or BUILDER_CRS (TrilinosCrsMatrix)."""
+        check(lib().cf3hip_lss_set_builder(self.h, builder))
+
+    def builder(self):
+        b = C.c_int()
+        check(lib().cf3hip_lss_get_builder(self.h, C.byref(b)))
+        return b.value
+
+    def column_zero(self, iblockcol, ieq):
+        """Matrix::get_column_and_replace_to_zero: the column (scalar rows), then zeroed in the matrix."""
+        out = np.zeros(self.dofs)
+        check(lib().cf3hip_matrix_get_column_and_replace_to_zero(self.h, iblockcol, ieq, _p(out)))
+        return out
+
+    def clone_to(self, other):
+        check(lib().cf3hip_matrix_clone_to(self.h, other.h))
+
+    def read_native(self, path):
+        check(lib().cf3hip_matrix_read_native(self.h, os.fsencode(path)))
 
     def reset(self, value=0.0):
         check(lib().cf3hip_matrix_reset(self.h, value))

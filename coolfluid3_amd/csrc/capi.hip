@@ -684,11 +684,50 @@ static int matrix_diagonal(cf3hip_lss lss, const double* diag, int mode)
 int cf3hip_matrix_set_diagonal(cf3hip_lss lss, const double* diag) { return matrix_diagonal(lss, diag, 0); }
 int cf3hip_matrix_add_diagonal(cf3hip_lss lss, const double* diag) { return matrix_diagonal(lss, diag, 1); }
 
+int cf3hip_lss_set_builder(cf3hip_lss lss, int builder)
+{
+  API_BEGIN
+  REQUIRE(lss, CF3HIP_ERR_NOT_CREATED, "system not created");
+  REQUIRE(builder == CF3HIP_BUILDER_VBR || builder == CF3HIP_BUILDER_CRS, CF3HIP_ERR_INVALID_ARG, "unknown matrix builder");
+  Lss& L = lss->l;
+  REQUIRE(L.bc_order.empty() && L.crs_col.empty() && !L.tied, CF3HIP_ERR_INVALID_ARG,
+          "choose the matrix builder before the first condition or tie");
+  L.builder = builder;
+  API_END
+}
+
+int cf3hip_lss_get_builder(cf3hip_lss lss, int* builder)
+{
+  API_BEGIN
+  REQUIRE(lss && builder, CF3HIP_ERR_INVALID_ARG, "null argument");
+  *builder = lss->l.builder;
+  API_END
+}
+
+int cf3hip_matrix_clone_to(cf3hip_lss src, cf3hip_lss dst)
+{
+  API_BEGIN
+  REQUIRE(src && dst, CF3HIP_ERR_NOT_CREATED, "system not created");
+  REQUIRE(src != dst, CF3HIP_ERR_INVALID_ARG, "clone_to needs another system");
+  lss_crs_clone(src->l, dst->l);
+  API_END
+}
+
+int cf3hip_matrix_read_native(cf3hip_lss lss, const char* path)
+{
+  API_BEGIN
+  REQUIRE(lss && path, CF3HIP_ERR_INVALID_ARG, "null argument");
+  lss_read_native(lss->l, path);
+  API_END
+}
+
 int cf3hip_matrix_get_column_and_replace_to_zero(cf3hip_lss lss, uint32_t iblockcol, uint32_t ieq, double* values)
 {
   API_BEGIN
   REQUIRE(lss && values, CF3HIP_ERR_INVALID_ARG, "null argument");
   Lss& L = lss->l;
+  REQUIRE(L.builder != CF3HIP_BUILDER_CRS, CF3HIP_ERR_NOT_IMPLEMENTED,
+          "get_column_and_replace_to_zero is not implemented for the CRS matrix builder");
   REQUIRE(iblockcol < L.n_nodes && ieq < (uint32_t)L.neq, CF3HIP_ERR_ILLEGAL_ENTRY, "Trying to access an illegal entry.");
   DevBuf d;
   d.alloc(L.ctx, sizeof(double) * L.dofs);

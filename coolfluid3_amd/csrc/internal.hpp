@@ -146,6 +146,12 @@ struct Lss {
   DevBuf bc_newmask;    // uint8 n_nodes: conditions added by the current call
   DevBuf xtilde;        // matrix-free residual input (x with eliminated columns at their values)
   DevBuf q1rec, q1tau;  // Q1 cells: per-element Gauss-point records and tau of the last assembly launch
+  // matrix builder semantics: CF3HIP_BUILDER_VBR (TrilinosFEVbrMatrix) or CF3HIP_BUILDER_CRS
+  // (TrilinosCrsMatrix: symmetric Dirichlet with columns cached until the next reset)
+  int builder = 0;
+  std::unordered_map<uint64_t, uint64_t> crs_col;   // CRS: dof -> offset of its cached column in crs_cache
+  DevBuf crs_cache;                                 // CRS: cached column values (deg(node) * neq per dof)
+  uint64_t crs_used = 0;
   bool tied = false;    // block rows tied (periodicity) since the last reset: the matrix-free
                         // residual of the element operator no longer describes the system
   // Krylov workspace (allocated lazily)
@@ -232,6 +238,8 @@ void ns_solve(Lss& L, Fields& F, const cf3hip_ns_params& nsp, const cf3hip_solve
 void lss_spmv(Lss& L, double alpha, const double* x, double beta, double* y, const double* stop = nullptr);
 void lss_spmv32(Lss& L, float alpha, const float* x, float beta, float* y, const double* stop = nullptr);
 void lss_dirichlet(Lss& L, uint32_t nbc, const uint32_t* nodes, const uint32_t* eqs, const double* values);
+void lss_crs_clone(Lss& src, Lss& dst);                 // TrilinosCrsMatrix::clone_to
+void lss_read_native(Lss& L, const char* path);          // TrilinosCrsMatrix::read_native
 void lss_tie(Lss& L, uint32_t npairs, const uint32_t* to, const uint32_t* from, bool system);
 // noalloc.hip: the residual without device allocation (scratch address per device set at context creation)
 void noalloc_init(int device);

@@ -1119,6 +1119,9 @@ void ns_solve(Lss& L, Fields& F, const cf3hip_ns_params& nsp, const cf3hip_solve
   if(L.tied)
     CF3_THROW(CF3HIP_ERR_NOT_IMPLEMENTED, "tied block rows: the matrix-free residual of the element operator does not "
                                           "describe the system; solve the stored system instead");
+  if(L.builder == CF3HIP_BUILDER_CRS)
+    CF3_THROW(CF3HIP_ERR_NOT_IMPLEMENTED, "the matrix-free residual models the conditions of the block (VBR) builder; "
+                                          "solve the stored CRS system instead");
   // the true residual is evaluated matrix-free from the FP64 element matrices of the fields
   reliable_bicgstab(L, [&](const double* x, double* r) { ns_residual(L, F, nsp, x, r); }, prm, st);
 }

@@ -8,6 +8,10 @@
 #include <algorithm>
 #include <unordered_map>
 
+#include <cctype>
+#include <cstdio>
+#include <string>
+
 #include "internal.hpp"
 #include "spmv4.hpp"
 
@@ -319,9 +323,253 @@ void lss_spmv_mixed(Lss& L, const double* x, double* y, const double* stop)
   timer_end(c, CF3HIP_KERNEL_SPMV_FP32);
 }
 
+namespace {
+
+// TrilinosCrsMatrix::symmetric_dirichlet (TrilinosCrsMatrix.cpp:325-402) followed by the solution value of
+// System::dirichlet (System.cpp:240-255), the conditions one after the other in one workgroup.  The first
+// application of a dof since the reset caches the column entries of the rows of its node's neighbours
+// (the dof's own row excluded) and eliminates them into the rhs, then makes the dof's row the identity;
+// a later application of the same dof only subtracts the cached column times its value from those rows'
+// rhs again.  Both end with rhs = sol = value at the dof.  fresh[b] marks first applications; off[b] is the
+// dof's cache offset (deg(node) * neq entries, row order of the node's block columns, then equation).
+template<typename V32>
+__global__ __launch_bounds__(256) void k_crs_dirichlet(uint32_t n, int neq, const uint32_t* __restrict__ node,
+                                                       const uint32_t* __restrict__ eq, const double* __restrict__ val,
+                                                       const uint8_t* __restrict__ fresh, const uint64_t* __restrict__ off,
+                                                       const uint32_t* __restrict__ row_ptr,
+                                                       const uint32_t* __restrict__ cols, double* values,
+                                                       V32* values32, double* rhs, double* sol, double* cache)
+{
+  const uint32_t nn = (uint32_t)(neq * neq);
+  for(uint32_t b = 0; b < n; ++b) {
+    const uint32_t i = node[b];
+    const uint32_t e = eq[b];
+    const double v = val[b];
+    const uint32_t q0 = row_ptr[i], deg = row_ptr[i + 1] - q0;
+    const uint64_t bc = (uint64_t)i * neq + e;
+    double* cb = cache + off[b];
+    const uint32_t nent = deg * (uint32_t)neq;
+    if(fresh[b]) {
+      for(uint32_t t = threadIdx.x; t < nent; t += blockDim.x) {
+        const uint32_t c = cols[q0 + t / neq], j = t % neq;
+        const uint64_t r = (uint64_t)c * neq + j;
+        double cv = 0.;
+        if(r != bc) {
+          // block (c, i) of row c (the graph is structurally symmetric)
+          uint32_t lo = row_ptr[c], hi = row_ptr[c + 1];
+          while(lo < hi) {
+            const uint32_t mid = (lo + hi) >> 1;
+            if(cols[mid] < i) lo = mid + 1; else hi = mid;
+          }
+          if(lo < row_ptr[c + 1] && cols[lo] == i) {
+            const uint64_t k = (uint64_t)lo * nn + (uint64_t)j * neq + e;
+            cv = values[k];
+            rhs[r] -= cv * v;
+            values[k] = 0.;
+            if(values32) values32[k] = V32(0);
+          }
+        }
+        cb[t] = cv;
+      }
+      __syncthreads();
+      // the dof's own row: 1 on the diagonal, 0 elsewhere
+      for(uint32_t t = threadIdx.x; t < nent; t += blockDim.x) {
+        const uint32_t q = q0 + t / neq, l = t % neq;
+        const uint64_t k = (uint64_t)q * nn + (uint64_t)e * neq + l;
+        const double d = (cols[q] == i && l == e) ? 1. : 0.;
+        values[k] = d;
+        if(values32) values32[k] = (V32)d;
+      }
+    } else {
+      for(uint32_t t = threadIdx.x; t < nent; t += blockDim.x) {
+        const uint32_t c = cols[q0 + t / neq], j = t % neq;
+        const uint64_t r = (uint64_t)c * neq + j;
+        if(r != bc) rhs[r] -= cb[t] * v;
+      }
+    }
+    __syncthreads();
+    if(threadIdx.x == 0) {
+      rhs[bc] = v;
+      sol[bc] = v;
+    }
+    __syncthreads();
+  }
+}
+
+// the conditions of one call on a CRS-builder system (host bookkeeping of the column cache)
+void crs_dirichlet(Lss& L, uint32_t nbc, const uint32_t* nodes, const uint32_t* eqs, const double* values)
+{
+  Context& c = *L.ctx;
+  const int neq = L.neq;
+  std::vector<uint32_t> rp(L.n_nodes + 1);
+  HIP_OK(hipMemcpyAsync(rp.data(), L.row_ptr.ptr, sizeof(uint32_t) * (L.n_nodes + 1), hipMemcpyDeviceToHost, c.stream));
+  HIP_OK(hipStreamSynchronize(c.stream));
+  std::vector<uint8_t> fresh(nbc);
+  std::vector<uint64_t> off(nbc);
+  uint64_t need = L.crs_used;
+  for(uint32_t b = 0; b < nbc; ++b) {
+    if(nodes[b] >= L.n_nodes || eqs[b] >= (uint32_t)neq)
+      CF3_THROW(CF3HIP_ERR_INVALID_ARG, "Dirichlet condition on a dof outside the system");
+    const uint64_t d = (uint64_t)nodes[b] * neq + eqs[b];
+    auto it = L.crs_col.find(d);
+    if(it == L.crs_col.end()) {
+      fresh[b] = 1;
+      off[b] = need;
+      L.crs_col.emplace(d, need);
+      need += (uint64_t)(rp[nodes[b] + 1] - rp[nodes[b]]) * neq;
+    } else {
+      fresh[b] = 0;
+      off[b] = it->second;
+    }
+  }
+  if(L.crs_cache.bytes < sizeof(double) * std::max<uint64_t>(need, 1)) {   // grow, keeping the cached columns
+    DevBuf nb;
+    nb.alloc(&c, sizeof(double) * std::max<uint64_t>(2 * need, 64));
+    if(L.crs_used) HIP_OK(hipMemcpyAsync(nb.ptr, L.crs_cache.ptr, sizeof(double) * L.crs_used, hipMemcpyDeviceToDevice, c.stream));
+    HIP_OK(hipStreamSynchronize(c.stream));
+    std::swap(L.crs_cache.ptr, nb.ptr);
+    std::swap(L.crs_cache.bytes, nb.bytes);
+    std::swap(L.crs_cache.ctx, nb.ctx);
+  }
+  L.crs_used = need;
+  DevBuf d_node, d_eq, d_val, d_fresh, d_off;
+  d_node.alloc(&c, sizeof(uint32_t) * nbc);
+  d_eq.alloc(&c, sizeof(uint32_t) * nbc);
+  d_val.alloc(&c, sizeof(double) * nbc);
+  d_fresh.alloc(&c, nbc);
+  d_off.alloc(&c, sizeof(uint64_t) * nbc);
+  HIP_OK(hipMemcpyAsync(d_node.ptr, nodes, sizeof(uint32_t) * nbc, hipMemcpyHostToDevice, c.stream));
+  HIP_OK(hipMemcpyAsync(d_eq.ptr, eqs, sizeof(uint32_t) * nbc, hipMemcpyHostToDevice, c.stream));
+  HIP_OK(hipMemcpyAsync(d_val.ptr, values, sizeof(double) * nbc, hipMemcpyHostToDevice, c.stream));
+  HIP_OK(hipMemcpyAsync(d_fresh.ptr, fresh.data(), nbc, hipMemcpyHostToDevice, c.stream));
+  HIP_OK(hipMemcpyAsync(d_off.ptr, off.data(), sizeof(uint64_t) * nbc, hipMemcpyHostToDevice, c.stream));
+  hipLaunchKernelGGL(k_crs_dirichlet<float>, dim3(1), dim3(256), 0, c.stream, nbc, neq, d_node.as<uint32_t>(),
+                     d_eq.as<uint32_t>(), d_val.as<double>(), d_fresh.as<uint8_t>(), d_off.as<uint64_t>(),
+                     L.row_ptr.as<uint32_t>(), L.cols.as<uint32_t>(), L.values.as<double>(),
+                     L.has_fp32 ? L.values32.as<float>() : (float*)nullptr, L.rhs.as<double>(), L.sol.as<double>(),
+                     L.crs_cache.as<double>());
+  HIP_OK(hipGetLastError());
+  HIP_OK(hipStreamSynchronize(c.stream));
+}
+
+}  // namespace
+
+// TrilinosCrsMatrix::clone_to (TrilinosCrsMatrix.cpp:518-538): the values and the cached Dirichlet columns
+// into another CRS-builder system of the same graph
+void lss_crs_clone(Lss& src, Lss& dst)
+{
+  Context& c = *src.ctx;
+  if(src.builder != CF3HIP_BUILDER_CRS)
+    CF3_THROW(CF3HIP_ERR_NOT_IMPLEMENTED, "clone_to is not implemented for the block (VBR) matrix builder");
+  if(dst.builder != CF3HIP_BUILDER_CRS)
+    CF3_THROW(CF3HIP_ERR_NOT_CREATED, "clone_to needs another system with the CRS matrix builder");
+  if(dst.neq != src.neq || dst.n_nodes != src.n_nodes || dst.n_blocks != src.n_blocks)
+    CF3_THROW(CF3HIP_ERR_NOT_CREATED, "clone_to needs a system of the same graph");
+  std::vector<uint32_t> a(src.n_nodes + 1), b(src.n_nodes + 1), ca(src.n_blocks), cb(src.n_blocks);
+  HIP_OK(hipMemcpyAsync(a.data(), src.row_ptr.ptr, sizeof(uint32_t) * a.size(), hipMemcpyDeviceToHost, c.stream));
+  HIP_OK(hipMemcpyAsync(ca.data(), src.cols.ptr, sizeof(uint32_t) * ca.size(), hipMemcpyDeviceToHost, c.stream));
+  HIP_OK(hipStreamSynchronize(c.stream));
+  Context& cd = *dst.ctx;
+  HIP_OK(hipMemcpyAsync(b.data(), dst.row_ptr.ptr, sizeof(uint32_t) * b.size(), hipMemcpyDeviceToHost, cd.stream));
+  HIP_OK(hipMemcpyAsync(cb.data(), dst.cols.ptr, sizeof(uint32_t) * cb.size(), hipMemcpyDeviceToHost, cd.stream));
+  HIP_OK(hipStreamSynchronize(cd.stream));
+  if(a != b || ca != cb) CF3_THROW(CF3HIP_ERR_NOT_CREATED, "clone_to needs a system of the same graph");
+  const size_t nv = (size_t)src.n_blocks * src.neq * src.neq;
+  HIP_OK(hipMemcpyAsync(dst.values.ptr, src.values.ptr, sizeof(double) * nv, hipMemcpyDeviceToDevice, c.stream));
+  if(src.has_fp32 && dst.has_fp32)
+    HIP_OK(hipMemcpyAsync(dst.values32.ptr, src.values32.ptr, sizeof(float) * nv, hipMemcpyDeviceToDevice, c.stream));
+  else if(dst.has_fp32)
+    lss_round_fp32(dst);
+  dst.crs_col = src.crs_col;
+  dst.crs_used = src.crs_used;
+  if(src.crs_used) {
+    if(dst.crs_cache.bytes < sizeof(double) * src.crs_used) dst.crs_cache.alloc(&cd, sizeof(double) * src.crs_used);
+    HIP_OK(hipMemcpyAsync(dst.crs_cache.ptr, src.crs_cache.ptr, sizeof(double) * src.crs_used, hipMemcpyDeviceToDevice,
+                          c.stream));
+  }
+  HIP_OK(hipStreamSynchronize(c.stream));
+}
+
+// TrilinosCrsMatrix::read_native (TrilinosCrsMatrix.cpp:540-546, EpetraExt::readEpetraLinearSystem): a
+// Matrix Market coordinate file (real, general or symmetric, 1-based indices) of dofs x dofs whose entries
+// lie in the system's pattern; the matrix is reset to zero first (entries absent from the file are zero)
+void lss_read_native(Lss& L, const char* path)
+{
+  if(L.builder != CF3HIP_BUILDER_CRS)
+    CF3_THROW(CF3HIP_ERR_NOT_IMPLEMENTED, "read_native is not implemented for the block (VBR) matrix builder");
+  FILE* f = std::fopen(path, "r");
+  if(!f) CF3_THROW(CF3HIP_ERR_INVALID_ARG, std::string("cannot open ") + path);
+  char line[512];
+  bool header = false, symmetric = false, sized = false;
+  uint64_t M = 0, Ncol = 0, nnz = 0, got = 0;
+  const int neq = L.neq;
+  Context& c = *L.ctx;
+  std::vector<uint32_t> rp(L.n_nodes + 1), cl(L.n_blocks);
+  HIP_OK(hipMemcpyAsync(rp.data(), L.row_ptr.ptr, sizeof(uint32_t) * rp.size(), hipMemcpyDeviceToHost, c.stream));
+  HIP_OK(hipMemcpyAsync(cl.data(), L.cols.ptr, sizeof(uint32_t) * cl.size(), hipMemcpyDeviceToHost, c.stream));
+  HIP_OK(hipStreamSynchronize(c.stream));
+  const size_t nn = (size_t)neq * neq;
+  std::vector<double> vals((size_t)L.n_blocks * nn, 0.);
+  std::string err;
+  auto put = [&](uint64_t r, uint64_t col, double v) {
+    const uint32_t i = (uint32_t)(r / neq), j = (uint32_t)(col / neq);
+    const uint32_t* lo = cl.data() + rp[i];
+    const uint32_t* hi = cl.data() + rp[i + 1];
+    const uint32_t* p = std::lower_bound(lo, hi, j);
+    if(p == hi || *p != j) return false;
+    vals[(size_t)(p - cl.data()) * nn + (r % neq) * neq + col % neq] = v;
+    return true;
+  };
+  while(std::fgets(line, sizeof(line), f)) {
+    if(!header) {
+      std::string h(line);
+      for(auto& ch : h) ch = (char)std::tolower(ch);
+      if(h.rfind("%%matrixmarket", 0) != 0 || h.find("coordinate") == std::string::npos ||
+         (h.find("real") == std::string::npos && h.find("integer") == std::string::npos)) {
+        err = "not a Matrix Market real coordinate file";
+        break;
+      }
+      symmetric = h.find("symmetric") != std::string::npos;
+      header = true;
+      continue;
+    }
+    if(line[0] == '%' || line[0] == '\n') continue;
+    if(!sized) {
+      if(std::sscanf(line, "%llu %llu %llu", (unsigned long long*)&M, (unsigned long long*)&Ncol,
+                     (unsigned long long*)&nnz) != 3) { err = "bad size line"; break; }
+      if(M != L.dofs || Ncol != L.dofs) { err = "matrix size differs from the system's"; break; }
+      sized = true;
+      continue;
+    }
+    unsigned long long r, col;
+    double v;
+    if(std::sscanf(line, "%llu %llu %lf", &r, &col, &v) != 3 || r < 1 || col < 1 || r > M || col > Ncol) {
+      err = "bad entry line";
+      break;
+    }
+    if(!put(r - 1, col - 1, v) || (symmetric && r != col && !put(col - 1, r - 1, v))) {
+      std::fclose(f);
+      CF3_THROW(CF3HIP_ERR_ILLEGAL_ENTRY, "Trying to access an illegal entry");
+    }
+    ++got;
+  }
+  std::fclose(f);
+  if(err.empty() && (!sized || got != nnz)) err = "entry count differs from the size line";
+  if(!err.empty()) CF3_THROW(CF3HIP_ERR_INVALID_ARG, err + " (" + path + ")");
+  HIP_OK(hipMemcpyAsync(L.values.ptr, vals.data(), sizeof(double) * vals.size(), hipMemcpyHostToDevice, c.stream));
+  HIP_OK(hipStreamSynchronize(c.stream));
+  lss_clear_conditions(L);
+  if(L.has_fp32) lss_round_fp32(L);
+  HIP_OK(hipStreamSynchronize(c.stream));
+}
+
 void lss_dirichlet(Lss& L, uint32_t nbc, const uint32_t* nodes, const uint32_t* eqs, const double* values)
 {
   if(!nbc) return;
+  if(L.builder == CF3HIP_BUILDER_CRS) {
+    crs_dirichlet(L, nbc, nodes, eqs, values);
+    return;
+  }
   Context& c = *L.ctx;
   const int neq = L.neq;
   // Host bookkeeping of the ordered list.  A dof constrained for the first time (in this call or an
@@ -552,6 +800,8 @@ void lss_clear_conditions(Lss& L)
   L.bc_order.clear();
   L.bc_n = 0;
   L.tied = false;
+  L.crs_col.clear();
+  L.crs_used = 0;
   if(L.bc_mask.ptr) lss_fill8(*L.ctx, L.bc_mask.as<uint8_t>(), L.n_nodes, 0);
 }
 

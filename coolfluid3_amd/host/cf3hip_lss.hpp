@@ -16,6 +16,8 @@
 #pragma once
 
 #include <cstdint>
+#include <fstream>
+#include <ostream>
 #include <stdexcept>
 #include <string>
 #include <vector>
@@ -173,6 +175,20 @@ public:
   // y = alpha A x + beta y for vectors of the same system (TrilinosDetail.cpp:380-411)
   void apply(Vector& y, const Vector& x, double alpha = 1., double beta = 0.);
   void debug_data(std::vector<uint32_t>& row_indices, std::vector<uint32_t>& col_indices, std::vector<double>& values);
+  // the two Trilinos builders' type names (LSSAction matrix_builder)
+  std::string type_name() const;
+  std::string name() const { return "Matrix"; }
+  // Matrix::print (TrilinosFEVbrMatrix.cpp:704-733, TrilinosCrsMatrix.cpp:589-625): one line
+  // "column -row value" per stored scalar entry, point rows in order, then the summary lines; with a file
+  // name (:736-743, :629-636) the Tecplot header lines first
+  void print(std::ostream& stream);
+  void print(const std::string& filename, std::ios_base::openmode mode = std::ios_base::out);
+  // Epetra's own Print format is not reproduced: the same lines as print
+  void print_native(std::ostream& stream) { print(stream); }
+  // TrilinosCrsMatrix::clone_to / read_native (TrilinosCrsMatrix.cpp:518-546); NotImplemented for the
+  // block builder (TrilinosFEVbrMatrix.cpp:798-808)
+  void clone_to(Matrix& other);
+  void read_native(const std::string& path);
   bool is_created() const;
   uint32_t neq() const;
   uint32_t blockrow_size() const;
@@ -294,6 +310,18 @@ public:
     check(cf3hip_lss_periodicity(h_, 1, &t, &f));
   }
   bool solve() { return strategy_.solve(); }
+  // LSSAction matrix_builder: "cf3.math.LSS.TrilinosFEVbrMatrix" (default) or
+  // "cf3.math.LSS.TrilinosCrsMatrix" semantics (before the first condition)
+  void set_matrix_builder(const std::string& builder)
+  {
+    int b;
+    if(builder == "cf3.math.LSS.TrilinosFEVbrMatrix" || builder == "HipFEVbrMatrix") b = CF3HIP_BUILDER_VBR;
+    else if(builder == "cf3.math.LSS.TrilinosCrsMatrix" || builder == "HipCrsMatrix") b = CF3HIP_BUILDER_CRS;
+    else throw BadValue(CF3HIP_ERR_INVALID_ARG, "unknown matrix builder " + builder);
+    check(cf3hip_lss_set_builder(h_, b));
+  }
+  // mesh nodes indexed by the matrix calls (block rows without periodic links)
+  uint32_t n_mesh_nodes() const { return p2m_.empty() ? n_nodes() : (uint32_t)p2m_.size(); }
   void reset(double v = 0.)
   {
     matrix_.reset(v);
@@ -466,6 +494,48 @@ inline void Matrix::debug_data(std::vector<uint32_t>& row_indices, std::vector<u
   values.resize((size_t)nb * neq * neq);
   check(cf3hip_matrix_debug_data(sys_.handle(), row_indices.data(), col_indices.data(), values.data()));
 }
+inline std::string Matrix::type_name() const
+{
+  int b = 0;
+  check(cf3hip_lss_get_builder(sys_.handle(), &b));
+  return b == CF3HIP_BUILDER_CRS ? "HipCrsMatrix" : "HipFEVbrMatrix";
+}
+inline void Matrix::print(std::ostream& stream)
+{
+  if(!is_created()) {
+    stream << name() << " of type " << type_name() << "::is_created() is false, nothing is printed.";
+    return;
+  }
+  std::vector<uint32_t> rp, ci;
+  std::vector<double> v;
+  debug_data(rp, ci, v);
+  const uint32_t q = neq(), nrow = (uint32_t)rp.size() - 1;
+  // scalar index of a point row or column: the last mesh node of its block row (the reference's m2p)
+  std::vector<uint32_t> m2p(nrow / q);
+  for(uint32_t r = 0; r < nrow / q; ++r) m2p[r] = r;
+  for(uint32_t i = 0; i < sys_.n_mesh_nodes(); ++i) m2p[sys_.row(i)] = i;
+  for(uint32_t r = 0; r < nrow; ++r)
+    for(uint32_t k = rp[r]; k < rp[r + 1]; ++k)
+      stream << m2p[ci[k] / q] * q + ci[k] % q << " " << -(int64_t)(m2p[r / q] * q + r % q) << " " << v[k] << "\n";
+  stream << "# name:                 " << name() << "\n";
+  stream << "# type_name:            " << type_name() << "\n";
+  stream << "# process:              " << 0 << "\n";
+  stream << "# number of equations:  " << q << "\n";
+  stream << "# number of rows:       " << nrow << "\n";
+  stream << "# number of cols:       " << nrow << "\n";
+  stream << "# number of block rows: " << nrow / q << "\n";
+  stream << "# number of block cols: " << nrow / q << "\n";
+  stream << "# number of entries:    " << rp[nrow] << "\n" << std::flush;
+}
+inline void Matrix::print(const std::string& filename, std::ios_base::openmode mode)
+{
+  std::ofstream stream(filename.c_str(), mode);
+  stream << "VARIABLES=COL,ROW,VAL\n" << std::flush;
+  stream << "ZONE T=\"" << type_name() << "::" << name() << "\"\n" << std::flush;
+  print(stream);
+}
+inline void Matrix::clone_to(Matrix& other) { check(cf3hip_matrix_clone_to(sys_.handle(), other.sys_.handle())); }
+inline void Matrix::read_native(const std::string& path) { check(cf3hip_matrix_read_native(sys_.handle(), path.c_str())); }
 inline bool Matrix::is_created() const { return sys_.is_created(); }
 inline uint32_t Matrix::neq() const { return sys_.neq(); }
 inline uint32_t Matrix::blockrow_size() const { return sys_.n_nodes(); }

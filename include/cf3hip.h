@@ -148,6 +148,28 @@ int cf3hip_matrix_get_block_values(cf3hip_lss lss, uint32_t nb, const uint32_t* 
 /* Matrix::set_diagonal / add_diagonal (Matrix.hpp:114-117): host, dofs */
 int cf3hip_matrix_set_diagonal(cf3hip_lss lss, const double* diag);
 int cf3hip_matrix_add_diagonal(cf3hip_lss lss, const double* diag);
+/* ---- matrix builder (LSSAction option matrix_builder, LSSAction.cpp:105-109).  The node-block storage
+ *      is the same for both; the builder selects the semantics of the reference's two matrices:
+ *      CF3HIP_BUILDER_VBR (default, TrilinosFEVbrMatrix) or CF3HIP_BUILDER_CRS (TrilinosCrsMatrix), whose
+ *      symmetric Dirichlet (TrilinosCrsMatrix.cpp:325-402) caches each eliminated column until the next
+ *      reset and subtracts it from the rhs again when the same condition is applied again, whose
+ *      get_column_and_replace_to_zero is not implemented (:318-321), and which implements clone_to
+ *      (:518-538) and read_native (:540-546).  Choose before the first condition or tie
+ *      (CF3HIP_ERR_INVALID_ARG otherwise).  The matrix-free residual of the precision form refuses a CRS
+ *      system. ---- */
+#define CF3HIP_BUILDER_VBR 0
+#define CF3HIP_BUILDER_CRS 1
+int cf3hip_lss_set_builder(cf3hip_lss lss, int builder);
+int cf3hip_lss_get_builder(cf3hip_lss lss, int* builder);
+/* Matrix::clone_to (TrilinosCrsMatrix.cpp:518-538): values and cached Dirichlet columns into another
+ * CRS-builder system of the same graph (CF3HIP_ERR_NOT_CREATED for another builder or graph, as the
+ * reference's SetupError); the VBR builder returns CF3HIP_ERR_NOT_IMPLEMENTED (TrilinosFEVbrMatrix.cpp:798) */
+int cf3hip_matrix_clone_to(cf3hip_lss src, cf3hip_lss dst);
+/* Matrix::read_native (TrilinosCrsMatrix.cpp:540-546): a Matrix Market coordinate file (real, general or
+ * symmetric) of dofs x dofs into the values, which are zeroed first; an entry outside the pattern gives
+ * CF3HIP_ERR_ILLEGAL_ENTRY; the conditions are cleared.  VBR: CF3HIP_ERR_NOT_IMPLEMENTED
+ * (TrilinosFEVbrMatrix.cpp:805) */
+int cf3hip_matrix_read_native(cf3hip_lss lss, const char* path);
 /* Matrix::get_column_and_replace_to_zero (Matrix.hpp:103, TrilinosFEVbrMatrix.cpp:452-478):
  * values (host, dofs) receive column (iblockcol, ieq), which is then zeroed */
 int cf3hip_matrix_get_column_and_replace_to_zero(cf3hip_lss lss, uint32_t iblockcol, uint32_t ieq, double* values);

@@ -329,3 +329,57 @@ def get():
     if _ORACLE is None:
         _ORACLE = Oracle()
     return _ORACLE
+
+
+class CrsConditions:
+    """TrilinosCrsMatrix::symmetric_dirichlet (cf3/math/LSS/Trilinos/TrilinosCrsMatrix.cpp:325-402) followed by
+    the solution value of System::dirichlet (cf3/math/LSS/System.cpp:240-255), on node-block CSR arrays
+    (block (c, i) of row c holds A[(c, j), (i, e)] at j * neq + e), in place.  The first application of a dof
+    since the reset (`reset`, Matrix::reset clearing m_symmetric_dirichlet_values) visits the point rows of
+    the dof's node's neighbours in the order of :345-353: every row but the dof's own has its entry in the
+    dof's column cached, eliminated into the rhs and zeroed (:360-376); the dof's own row becomes 1 at its
+    column and 0 elsewhere (:377-389).  A later application only subtracts the cached column times the new
+    value from those rows' rhs (:392-398).  Both set rhs and solution at the dof to the value.  Pure
+    Python loops: for small systems (test oracle only)."""
+
+    def __init__(self):
+        self.cache = {}
+
+    def reset(self):
+        self.cache.clear()
+
+    @staticmethod
+    def _block(start, cols, row, col):
+        for q in range(int(start[row]), int(start[row + 1])):
+            if int(cols[q]) == col:
+                return q
+        return -1
+
+    def apply(self, neq, start, cols, values, rhs, sol, node, eq, value):
+        nn = neq * neq
+        bc = node * neq + eq
+        cached = self.cache.get(bc)
+        if cached is None:
+            cached = {}
+            for q in range(int(start[node]), int(start[node + 1])):
+                c = int(cols[q])
+                for j in range(neq):
+                    r = c * neq + j
+                    if r != bc:
+                        qb = self._block(start, cols, c, node)
+                        if qb < 0:
+                            continue
+                        k = qb * nn + j * neq + eq
+                        cached[r] = values[k]
+                        rhs[r] -= values[k] * value
+                        values[k] = 0.0
+                    else:
+                        for q2 in range(int(start[node]), int(start[node + 1])):
+                            for l in range(neq):
+                                values[q2 * nn + eq * neq + l] = 1.0 if (int(cols[q2]) == node and l == eq) else 0.0
+            self.cache[bc] = cached
+        else:
+            for r, v in cached.items():
+                rhs[r] -= v * value
+        rhs[bc] = value
+        sol[bc] = value

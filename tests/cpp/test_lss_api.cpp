@@ -7,6 +7,8 @@
 #include <cstdio>
 #include <cstdlib>
 #include <cstring>
+#include <sstream>
+#include <string>
 #include <vector>
 
 #include "cf3hip_lss.hpp"
@@ -37,11 +39,12 @@ static bool throws(F&& f)
 }
 
 // 4-node chain 0-1-2-3, one equation per node (utest-lss-symmetric-dirichlet.cpp:146-210)
-static void symmetric_dirichlet(Context& ctx)
+static void symmetric_dirichlet(Context& ctx, const char* builder)
 {
   const std::vector<uint32_t> start = {0, 2, 5, 8, 10};
   const std::vector<uint32_t> conn = {0, 1, 0, 1, 2, 1, 2, 3, 2, 3};
   lss::System s(ctx, 1, conn, start);
+  s.set_matrix_builder(builder);   // the reference runs this test once per matrix_builder
   for(uint32_t i = 0; i < 4; ++i) s.matrix().set_row(i, 0, 2., 1.);
   s.dirichlet(1, 0, 10.);
   const double expect[4][4] = {{2, 0, 0, 0}, {0, 1, 0, 0}, {0, 0, 2, 1}, {0, 0, 1, 2}};
@@ -62,6 +65,57 @@ static void symmetric_dirichlet(Context& ctx)
   CHECK(throws<BadValue>([&] { double v; s.matrix().get_value(3, 0, v); }));
   // rows 0 and 3 hold different block columns: they cannot be tied (TrilinosFEVbrMatrix.cpp:575-577)
   CHECK(throws<BadValue>([&] { s.matrix().tie_blockrow_pairs(0, 3); }));
+}
+
+// The two builders differ when a condition is applied again before a reset: TrilinosCrsMatrix subtracts
+// its cached column again (TrilinosCrsMatrix.cpp:392-398), TrilinosFEVbrMatrix finds the column already
+// zero.  Also: print format (TrilinosFEVbrMatrix.cpp:704-743), clone_to and read_native (CRS only) and
+// get_column_and_replace_to_zero (VBR only).
+static void matrix_builders(Context& ctx)
+{
+  const std::vector<uint32_t> start = {0, 2, 5, 8, 10};
+  const std::vector<uint32_t> conn = {0, 1, 0, 1, 2, 1, 2, 3, 2, 3};
+  for(int crs = 0; crs < 2; ++crs) {
+    lss::System s(ctx, 1, conn, start);
+    s.set_matrix_builder(crs ? "cf3.math.LSS.TrilinosCrsMatrix" : "cf3.math.LSS.TrilinosFEVbrMatrix");
+    CHECK(s.matrix().type_name() == (crs ? "HipCrsMatrix" : "HipFEVbrMatrix"));
+    for(uint32_t i = 0; i < 4; ++i) s.matrix().set_row(i, 0, 2., 1.);
+    s.dirichlet(1, 0, 10.);
+    s.dirichlet(1, 0, 10.);
+    std::vector<double> rhs;
+    s.rhs().get(rhs);
+    if(crs) CHECK(rhs[0] == -20. && rhs[1] == 10. && rhs[2] == -20. && rhs[3] == 0.);
+    else CHECK(rhs[0] == -10. && rhs[1] == 10. && rhs[2] == -10. && rhs[3] == 0.);
+    // print: one "column -row value" line per stored entry (10), then nine summary lines
+    std::ostringstream out;
+    s.matrix().print(out);
+    std::istringstream in(out.str());
+    std::string line;
+    std::vector<std::string> lines;
+    while(std::getline(in, line)) lines.push_back(line);
+    CHECK(lines.size() == 19);
+    CHECK(lines[0] == "0 0 2");              // row 0, column 0
+    CHECK(lines[1] == "1 0 0");              // row 0, column 1: eliminated
+    CHECK(lines[2] == "0 -1 0" && lines[3] == "1 -1 1" && lines[4] == "2 -1 0");
+    CHECK(lines[10] == "# name:                 Matrix");
+    CHECK(lines[11] == std::string("# type_name:            ") + (crs ? "HipCrsMatrix" : "HipFEVbrMatrix"));
+    CHECK(lines[18] == "# number of entries:    10");
+    lss::System t(ctx, 1, conn, start);
+    t.set_matrix_builder(crs ? "cf3.math.LSS.TrilinosCrsMatrix" : "cf3.math.LSS.TrilinosFEVbrMatrix");
+    std::vector<double> col;
+    if(crs) {
+      s.matrix().clone_to(t.matrix());
+      double a = 0., b = 0.;
+      s.matrix().get_value(2, 3, a);
+      t.matrix().get_value(2, 3, b);
+      CHECK(a == 1. && b == 1.);
+      CHECK(throws<NotImplemented>([&] { s.matrix().get_column_and_replace_to_zero(2, 0, col); }));
+    } else {
+      CHECK(throws<NotImplemented>([&] { s.matrix().clone_to(t.matrix()); }));
+      CHECK(throws<NotImplemented>([&] { s.matrix().read_native("/nonexistent.mtx"); }));
+    }
+    CHECK(throws<BadValue>([&] { s.set_matrix_builder("cf3.math.LSS.TrilinosCrsMatrix"); }));   // after a condition
+  }
 }
 
 // System::periodicity on a dense 3-node graph (System.cpp:259-282, TrilinosFEVbrMatrix.cpp:558-590):
@@ -461,7 +515,11 @@ int main(int argc, char** argv)
     Context ctx(0);
     std::printf("symmetric_dirichlet\n");
     std::fflush(stdout);
-    symmetric_dirichlet(ctx);
+    symmetric_dirichlet(ctx, "cf3.math.LSS.TrilinosFEVbrMatrix");
+    symmetric_dirichlet(ctx, "cf3.math.LSS.TrilinosCrsMatrix");
+    std::printf("matrix_builders\n");
+    std::fflush(stdout);
+    matrix_builders(ctx);
     std::printf("periodicity\n");
     std::fflush(stdout);
     periodicity(ctx);
