@@ -18,7 +18,7 @@ TRIAG2D, TETRA3D, HEXA3D, QUAD2D = 0, 1, 2, 3
 SUPG_TEZDUYAR, SUPG_METRIC, SUPG_CF2 = 0, 1, 2
 FP64, FP32 = 0, 1
 RHS, SOLUTION = 0, 1
-BICGSTAB_JACOBI, GMRES_ILU0 = 0, 1
+BICGSTAB_JACOBI, GMRES_ILU0, RCG_JACOBI = 0, 1, 2
 BUILDER_VBR, BUILDER_CRS = 0, 1
 FIELD_U, FIELD_P, FIELD_UADV, FIELD_U1, FIELD_U2, FIELD_U3, FIELD_NU, FIELD_G = range(8)
 KERNEL_ASSEMBLY, KERNEL_SPMV, KERNEL_SPMV_FP32, KERNEL_RESIDUAL = range(4)
@@ -33,7 +33,11 @@ class Cf3HipError(RuntimeError):
 class SolverParams(C.Structure):
     _fields_ = [("method", C.c_int), ("restart", C.c_int), ("max_iter", C.c_int), ("tol", C.c_double),
                 ("precision", C.c_int), ("inner_tol", C.c_double), ("max_outer", C.c_int),
-                ("block_scale", C.c_int), ("fp32_vectors", C.c_int)]
+                ("block_scale", C.c_int), ("fp32_vectors", C.c_int), ("recycle", C.c_int), ("rcg_space", C.c_int)]
+
+
+class SiOptions(C.Structure):
+    _fields_ = [("enable_body_force", C.c_int), ("pressure_rcg_solve", C.c_int)]
 
 
 class SolveStats(C.Structure):
@@ -54,8 +58,9 @@ def ns_params(rho=1.0, dt=0.01, theta=1.0, use_specializations=True, supg_type=S
 
 
 def solver_params(method=BICGSTAB_JACOBI, tol=1e-10, max_iter=5000, restart=30, precision=FP64, inner_tol=1e-2,
-                  max_outer=0, block_scale=1, fp32_vectors=0):
-    return SolverParams(method, restart, max_iter, tol, precision, inner_tol, max_outer, block_scale, fp32_vectors)
+                  max_outer=0, block_scale=1, fp32_vectors=0, recycle=0, rcg_space=0):
+    return SolverParams(method, restart, max_iter, tol, precision, inner_tol, max_outer, block_scale, fp32_vectors,
+                        recycle, rcg_space)
 
 
 _lib = None
@@ -148,6 +153,8 @@ def lib():
         "cf3hip_si_velocity_assemble": [_vp, _vp, C.POINTER(NSParams)],
         "cf3hip_si_set_solution": [_vp, _vp],
         "cf3hip_si_action": [_vp, _vp, C.POINTER(NSParams), C.c_int],
+        "cf3hip_si_set_options": [_vp, C.POINTER(SiOptions)],
+        "cf3hip_si_get_options": [_vp, C.POINTER(SiOptions)],
         "cf3hip_si_get": [_vp, C.c_int, _d],
         "cf3hip_si_set": [_vp, C.c_int, _d],
         "cf3hip_si_step": [_vp, _vp, C.POINTER(NSParams), C.c_int, C.POINTER(SolverParams), C.POINTER(SolverParams),
@@ -521,6 +528,16 @@ class SemiImplicit:
 
     def action(self, fields, prm, which):
         check(lib().cf3hip_si_action(self.h, fields.h, C.byref(prm), which))
+
+    def set_options(self, enable_body_force=False, pressure_rcg_solve=False):
+        """The enable_body_force and pressure_rcg_solve options of NavierStokesSemiImplicit."""
+        check(lib().cf3hip_si_set_options(self.h, C.byref(SiOptions(int(enable_body_force),
+                                                                      int(pressure_rcg_solve)))))
+
+    def options(self):
+        o = SiOptions()
+        check(lib().cf3hip_si_get_options(self.h, C.byref(o)))
+        return {"enable_body_force": bool(o.enable_body_force), "pressure_rcg_solve": bool(o.pressure_rcg_solve)}
 
     def _n(self, which):
         rows = self.u_system.n_nodes

@@ -1317,6 +1317,24 @@ void si_conditions(SemiImplicit& S, const std::vector<const double*>& field, uin
 }  // namespace
 }  // extern "C++"
 
+int cf3hip_si_set_options(cf3hip_si si, const cf3hip_si_options* options)
+{
+  API_BEGIN
+  REQUIRE(si && options, CF3HIP_ERR_INVALID_ARG, "null argument");
+  si->s.body_force = options->enable_body_force != 0;
+  si->s.rcg = options->pressure_rcg_solve != 0;
+  API_END
+}
+
+int cf3hip_si_get_options(cf3hip_si si, cf3hip_si_options* options)
+{
+  API_BEGIN
+  REQUIRE(si && options, CF3HIP_ERR_INVALID_ARG, "null argument");
+  options->enable_body_force = si->s.body_force ? 1 : 0;
+  options->pressure_rcg_solve = si->s.rcg ? 1 : 0;
+  API_END
+}
+
 int cf3hip_si_get(cf3hip_si si, int which, double* host)
 {
   API_BEGIN
@@ -1411,7 +1429,18 @@ int cf3hip_si_step(cf3hip_si si, cf3hip_fields fields, const cf3hip_ns_params* p
     } else {
       si_zero_dofs(S, pdofs, P.rhs.as<double>());
     }
-    lss_solve(P, *p_solver, st);
+    if(S.rcg) {   // FirstPressureStrategy / SecondPressureStrategy
+      cf3hip_solver_params rp = *p_solver;
+      if(rp.method != CF3HIP_RCG_JACOBI) {
+        rp.method = CF3HIP_RCG_JACOBI;
+        rp.restart = 40;
+        rp.recycle = 16;
+      }
+      rp.rcg_space = i == 0 ? 0 : 1;
+      lss_solve(P, rp, st);
+    } else {
+      lss_solve(P, *p_solver, st);
+    }
     if(stats) stats[2 * i + 1] = st;
     REQUIRE(st.converged, CF3HIP_ERR_NOT_CONVERGED, "pressure solve did not reach the tolerance");
     lss_fill(c, U.rhs.as<double>(), U.dofs, 0.);

@@ -4,6 +4,7 @@
 #include <hip/hip_runtime.h>
 
 #include <cstdint>
+#include <memory>
 #include <string>
 #include <unordered_map>
 #include <vector>
@@ -93,6 +94,19 @@ struct Mesh {
   DevBuf conn;        // uint32 n_elems*k, element-major
 };
 
+// recycle space of the RCG solver (rcg.hip): W in the first k columns of V and AW = A W in those of
+// AV, the next columns the search directions of the current cycle; E^-1 = (W^T A W)^-1 on the device
+constexpr int kRcgSpaces = 2;      // per system (the first and the second pressure strategies)
+constexpr int kMaxRecycle = 32;    // recycled vectors
+constexpr int kMaxCycle = 96;      // search directions per cycle
+constexpr int kDefaultCycle = 40;
+struct RcgSpace {
+  uint64_t n = 0;
+  int k = 0;
+  double print[2] = {0., 0.};      // fingerprint of the matrix values W was built on
+  DevBuf V, AV, tmp, vec, sc, small, einv, gram;
+};
+
 struct Lss {
   Context* ctx = nullptr;
   Mesh* mesh = nullptr;   // null when created from a caller graph
@@ -169,6 +183,7 @@ struct Lss {
     DevBuf V, w, z, small, out;
     int m_ws = 0;
   } ilu;
+  std::unique_ptr<RcgSpace> rcg[kRcgSpaces];   // RCG recycle spaces (kept between solves)
 };
 
 struct Fields {
@@ -197,6 +212,9 @@ struct SemiImplicit {
   DevBuf Pv, dps;            // rows: p, delta_p_sum
   DevBuf pvals0;             // the pressure matrix before its Dirichlet conditions
   bool initialized = false;
+  // options of NavierStokesSemiImplicit (NavierStokesSemiImplicit.cpp:309-317)
+  bool body_force = false;   // enable_body_force: the body_force field enters the velocity rhs
+  bool rcg = false;          // pressure_rcg_solve: first and later pressure solves by two RCG spaces
 };
 
 // batched FP32 assembly of many systems (serving): a device table of per-system arguments and one
@@ -253,6 +271,7 @@ void lss_axpby(Context& c, size_t n, double alpha, const double* x, double beta,
 void lss_solve(Lss& L, const cf3hip_solver_params& prm, cf3hip_solve_stats& st);
 void lss_solve_precision(Lss& L, const cf3hip_solver_params& prm, cf3hip_solve_stats& st);
 double lss_residual(Lss& L);
+void rcg_solve(Lss& L, const cf3hip_solver_params& prm, cf3hip_solve_stats& st);   // rcg.hip
 
 // nodes.hip
 void ns_linearize(Fields& F);

@@ -673,6 +673,10 @@ void lss_solve(Lss& L, const cf3hip_solver_params& prm, cf3hip_solve_stats& st)
     gmres_ilu0(L, prm, st);
     return;
   }
+  if(prm.method == CF3HIP_RCG_JACOBI) {
+    rcg_solve(L, prm, st);
+    return;
+  }
   if(prm.method != CF3HIP_BICGSTAB_JACOBI) CF3_THROW(CF3HIP_ERR_INVALID_ARG, "unknown Krylov method");
   if(prm.precision == CF3HIP_FP64) {
     ensure_work(L, 9, sizeof(double));

@@ -249,6 +249,7 @@ struct SiArgs {
   const double* P;
   const double* dps;
   const double* dp;
+  const double* gf[3];     // body force (enable_body_force), null otherwise
   double theta;
   TauP t;
   double* stage;
@@ -423,6 +424,22 @@ __global__ void __launch_bounds__(256) k_si_elem(SiArgs g)
             res[i][a] += v;
           }
         }
+      }
+    }
+    if(g.gf[0]) {
+      // the body-force overload (MatrixAssembly.hpp:339-363): N^T g_i(x_q) w_q over the second-order rule,
+      // added after the terms above
+      double gn[K][D];
+      for(int m = 0; m < K; ++m)
+        for(int d = 0; d < D; ++d) gn[m][d] = g.gf[d][cn[m]];
+      for(int q = 0; q < E::npts(2); ++q) {
+        E::point(2, q, xi, w);
+        point_data<E>(X, xi, N, G, det);
+        const double wq = w * det;
+        double gq[D];
+        interp<E>(N, gn, gq);
+        for(int i = 0; i < D; ++i)
+          for(int a = 0; a < K; ++a) res[i][a] += N[a] * gq[i] * wq;
       }
     }
     double* out = g.stage + (uint64_t)e * K * D;
@@ -835,6 +852,7 @@ SiArgs si_args(SemiImplicit& S, Fields& F, const cf3hip_ns_params& prm)
   g.P = S.Pv.as<double>();
   g.dps = S.dps.as<double>();
   g.dp = S.p->sol.as<double>();
+  for(int d = 0; d < M.dim; ++d) g.gf[d] = S.body_force ? F.g[d].as<double>() : nullptr;
   g.theta = prm.theta;
   if(prm.supg_type != CF3HIP_SUPG_METRIC && prm.supg_type != CF3HIP_SUPG_TEZDUYAR)
     CF3_THROW(CF3HIP_ERR_NOT_IMPLEMENTED, "the semi-implicit actions take supg_type metric or tezduyar");

@@ -212,6 +212,17 @@ public:
     params.max_outer = 0;
     params.block_scale = 1;
     params.fp32_vectors = 0;
+    params.recycle = 0;
+    params.rcg_space = 0;
+  }
+  // RCGStrategy (RCGStrategy.cpp:59-171): recycling CG with `recycle` vectors kept between solves,
+  // cycles of `restart` directions, in recycle space `space` of the system
+  void use_rcg(int recycle, int restart, int space = 0)
+  {
+    params.method = CF3HIP_RCG_JACOBI;
+    params.recycle = recycle;
+    params.restart = restart;
+    params.rcg_space = space;
   }
   cf3hip_solver_params params;
   cf3hip_solve_stats stats = {};
@@ -644,6 +655,8 @@ public:
   std::string supg_type = "metric";
   double dt = 0.;
   double c1 = 1., c2 = 16., alpha_ps = 1., alpha_su = 1., alpha_bu = 1.;
+  bool enable_body_force = false;    // "enable_body_force" (NavierStokesSemiImplicit.cpp:313-317)
+  bool pressure_rcg_solve = false;   // "pressure_rcg_solve" (:309-311, :532-537)
 
   NavierStokesSemiImplicit(lss::System& velocity, lss::System& pressure) : u_(velocity), p_(pressure)
   {
@@ -664,6 +677,7 @@ public:
   // SetupInnerLoopData
   void initial(Fields& f)
   {
+    apply_options();
     const cf3hip_ns_params p = params();
     check(cf3hip_si_initial(h_, f.handle(), &p));
   }
@@ -672,6 +686,7 @@ public:
   bool step(Fields& f, const std::vector<uint32_t>& u_nodes, const std::vector<double>& u_values,
             const std::vector<uint32_t>& p_nodes, const std::vector<double>& p_values)
   {
+    apply_options();
     const cf3hip_ns_params p = params();
     stats.assign(2 * nb_iterations, cf3hip_solve_stats{});
     const int rc = cf3hip_si_step(h_, f.handle(), &p, nb_iterations, &u_.solution_strategy().params,
@@ -685,6 +700,11 @@ public:
   std::vector<cf3hip_solve_stats> stats;   // velocity and pressure solve of each inner pass
   cf3hip_si handle() const { return h_; }
 private:
+  void apply_options()
+  {
+    const cf3hip_si_options o = {enable_body_force ? 1 : 0, pressure_rcg_solve ? 1 : 0};
+    check(cf3hip_si_set_options(h_, &o));
+  }
   lss::System& u_;
   lss::System& p_;
   cf3hip_si h_ = nullptr;

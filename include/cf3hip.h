@@ -229,9 +229,14 @@ int cf3hip_lss_periodicity(cf3hip_lss lss, uint32_t npairs, const uint32_t* to, 
  *      TrilinosStratimikosStrategy.cpp:135-207) ---- */
 #define CF3HIP_BICGSTAB_JACOBI 0
 #define CF3HIP_GMRES_ILU0      1
+/* recycling conjugate gradients for symmetric positive definite systems (RCGStrategy,
+ * cf3/math/LSS/Trilinos/RCGStrategy.cpp:59-171, Belos::RCGSolMgr): Jacobi-preconditioned CG deflated by
+ * a recycle space of the system that is kept between solves and refreshed from harmonic Ritz vectors
+ * after every cycle of `restart` directions; FP64 only */
+#define CF3HIP_RCG_JACOBI      2
 typedef struct cf3hip_solver_params {
-  int method;        /* CF3HIP_BICGSTAB_JACOBI or CF3HIP_GMRES_ILU0 */
-  int restart;       /* GMRES restart length */
+  int method;        /* CF3HIP_BICGSTAB_JACOBI, CF3HIP_GMRES_ILU0 or CF3HIP_RCG_JACOBI */
+  int restart;       /* GMRES restart length; RCG: directions per cycle ("Num Blocks", <= 96, 0: 40) */
   int max_iter;
   double tol;        /* stop on ||b - A x||_2 <= tol ||b||_2 */
   int precision;     /* CF3HIP_FP64, or CF3HIP_FP32 = FP32-matrix BiCGStab with FP64 residual
@@ -246,6 +251,8 @@ typedef struct cf3hip_solver_params {
                         left unchanged. */
   int fp32_vectors;  /* precision FP32, 4 equations per node: 1 = the Krylov vectors of the FP32 phase in
                         FP32 and its products accumulated in FP32 (dot products in FP64); 0 = FP64 */
+  int recycle;       /* RCG: recycled vectors ("Num Recycled Blocks", 0 to 32; 0 = plain Jacobi CG) */
+  int rcg_space;     /* RCG: which of the system's two recycle spaces the solve uses and updates (0, 1) */
 } cf3hip_solver_params;
 typedef struct cf3hip_solve_stats {
   int iterations;    /* BiCGStab iterations (FP32 form: both phases) */
@@ -353,6 +360,20 @@ typedef struct cf3hip_si_s* cf3hip_si;
 #define CF3HIP_SI_VEC_MINV 4   /* reciprocal of the lumped velocity mass (SetupInnerLoopData, :236-262) */
 int cf3hip_si_create(cf3hip_lss u_lss, cf3hip_lss p_lss, cf3hip_si* si);
 int cf3hip_si_destroy(cf3hip_si si);
+/* options of NavierStokesSemiImplicit (implementation/NavierStokesSemiImplicit.cpp:309-317), both off
+ * by default as there */
+typedef struct cf3hip_si_options {
+  int enable_body_force;   /* "enable_body_force": URHSAssembly takes the overload of velocity_rhs with the
+                              body_force field (CF3HIP_FIELD_G), + int N^T g_i over the second-order rule
+                              (MatrixAssembly.hpp:339-363, :460-477) */
+  int pressure_rcg_solve;  /* "pressure_rcg_solve": the first pressure solve of each step and the later ones
+                              use two RCG strategies, each with its own recycle space kept between steps
+                              (:158-166, :532-537): CF3HIP_RCG_JACOBI in recycle spaces 0 and 1 of p_lss,
+                              with p_solver's tol and max_iter, and its restart and recycle when p_solver
+                              selects CF3HIP_RCG_JACOBI (otherwise 40 and 16) */
+} cf3hip_si_options;
+int cf3hip_si_set_options(cf3hip_si si, const cf3hip_si_options* options);
+int cf3hip_si_get_options(cf3hip_si si, cf3hip_si_options* options);
 /* initial conditions: PressureSystemAssembly (PressureSystem.cpp:120-176: p_lss reset, the simplified
  * pressure matrix theta (tau_ps + dt) int G^T G with tau_ps from u), ZeroVelocitySystem +
  * MassMatrixAssembly (lumped mass into the velocity rhs) + SetupInnerLoopData (its reciprocal) */

@@ -1778,15 +1778,32 @@ int orc_si_pressure_matrix(int et, const orc_ns_params* prm, uint32_t n_elems, c
   return 0;
 }
 
+// the body-force overload of VelocityRHS (MatrixAssembly.hpp:339-363): after the terms above, every
+// point of the second-order rule adds N^T g_i(x_q) w_q to component i, g interpolated from the nodes
+static int si_body_force_element(int et, const Real* X, const Real* g, Real* res)
+{
+  const int k = enodes(et), dim = edim(et);
+  Real gxi[24], gw[8], N[8], G[24], det, gq[3];
+  const int ngp = gauss(et, 2, gxi, gw);
+  for(int q = 0; q < ngp; ++q) {
+    if(!point_data(et, gxi + q*dim, X, N, G, det)) return -2;
+    interp(k, dim, N, g, gq);
+    const Real w = gw[q]*det;
+    for(int i = 0; i < dim; ++i)
+      for(int a = 0; a < k; ++a) res[i*k+a] += N[a]*gq[i]*w;
+  }
+  return 0;
+}
+
 int orc_si_velocity_rhs(int et, const orc_ns_params* prm, uint32_t n_elems, const double* coords,
                         const uint32_t* conn, const uint32_t* p2m, const double* uadv, const double* nu,
                         const double* a_lss, const double* u_lss, const double* dps_lss, const double* p_lss,
-                        double* rhs)
+                        double* rhs, const double* body_force)
 {
   const int k = enodes(et), dim = edim(et);
   const TauCfg cfg = tau_cfg(prm);
   const Real dt = prm->dt, th = prm->theta;
-  Real X[24], ua[24], nn[8], av[24], uv[24], dps[8], pv[8], dtamu[24], ppdp[8], res[24];
+  Real X[24], ua[24], nn[8], av[24], uv[24], dps[8], pv[8], dtamu[24], ppdp[8], res[24], gn[24];
   for(uint32_t e = 0; e < n_elems; ++e) {
     const uint32_t* c = conn + (size_t)e*k;
     gather(k, dim, c, nullptr, coords, X);
@@ -1800,6 +1817,10 @@ int orc_si_velocity_rhs(int et, const orc_ns_params* prm, uint32_t n_elems, cons
     for(int r = 0; r < k*dim; ++r) dtamu[r] = dt*(1. - th)*av[r] - uv[r];
     for(int r = 0; r < k; ++r) ppdp[r] = (1. - th)*dps[r] - pv[r];
     if(si_velocity_rhs_element(et, cfg, dt, X, ua, nn, av, dtamu, ppdp, res)) return -2;
+    if(body_force) {
+      gather(k, dim, c, nullptr, body_force, gn);
+      if(si_body_force_element(et, X, gn, res)) return -2;
+    }
     for(int a = 0; a < k; ++a)
       for(int i = 0; i < dim; ++i) rhs[(size_t)p2m[c[a]]*dim + i] += res[i*k+a];
   }

@@ -168,11 +168,12 @@ int orc_si_pressure_matrix(int etype, const orc_ns_params* prm, uint32_t n_elems
                            const uint32_t* conn, const double* u, const double* nu, const uint32_t* slots,
                            double* values);
 /* URHSAssembly: velocity_rhs(u_adv, nu, a, dt (1 - theta) a - u_vec, (1 - theta) delta_p_sum - p_vec)
- * (MatrixAssembly.hpp:206-330, :466-470) */
+ * (MatrixAssembly.hpp:206-330, :466-470); body_force (n_nodes x dim, or null) selects the overload of
+ * enable_body_force (:339-363, :472-477) */
 int orc_si_velocity_rhs(int etype, const orc_ns_params* prm, uint32_t n_elems, const double* coords,
                         const uint32_t* conn, const uint32_t* p2m, const double* uadv, const double* nu,
                         const double* a_lss, const double* u_lss, const double* dps_lss, const double* p_lss,
-                        double* rhs);
+                        double* rhs, const double* body_force);
 /* PRHSAssembly: pressure_rhs(u_adv, u_vec, a, delta_a, p_vec) with tau_ps from u (:148-204, :487-494) */
 int orc_si_pressure_rhs(int etype, const orc_ns_params* prm, uint32_t n_elems, const double* coords,
                         const uint32_t* conn, const uint32_t* p2m, const double* u, const double* uadv,

@@ -105,7 +105,7 @@ class Oracle:
         L.orc_si_lumped_mass.argtypes = [C.c_int, C.c_uint32, _d, _u32, _u32, _d]
         for name in ("orc_si_velocity_matrix", "orc_si_pressure_matrix"):
             getattr(L, name).argtypes = [C.c_int, C.POINTER(NSParams), C.c_uint32, _d, _u32, _d, _d, _u32, _d]
-        L.orc_si_velocity_rhs.argtypes = [C.c_int, C.POINTER(NSParams), C.c_uint32, _d, _u32, _u32] + [_d] * 7
+        L.orc_si_velocity_rhs.argtypes = [C.c_int, C.POINTER(NSParams), C.c_uint32, _d, _u32, _u32] + [_d] * 8
         L.orc_si_pressure_rhs.argtypes = [C.c_int, C.POINTER(NSParams), C.c_uint32, _d, _u32, _u32] + [_d] * 8
         L.orc_si_apply_aup.argtypes = [C.c_int, C.POINTER(NSParams), C.c_uint32, _d, _u32, _u32] + [_d] * 4
 

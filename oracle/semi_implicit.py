@@ -29,7 +29,7 @@ _u32a = po._u32a
 
 class SemiImplicit:
     def __init__(self, o, etype, coords, conn, p2m, nu, dt, theta=1.0, nb_iterations=2, u0=None, p0=None,
-                 pbc_nodes=(), ubc_nodes=()):
+                 pbc_nodes=(), ubc_nodes=(), body_force=None):
         self.o, self.et = o, etype
         self.dim = po.DIM[etype]
         self.coords = _f64(coords)
@@ -46,6 +46,8 @@ class SemiImplicit:
         self.p = _f64(p0).copy()
         self.u1, self.u2, self.u3, self.uadv = (self.u.copy() for _ in range(4))
         self.nu = np.full(self.n, float(nu))                      # nu_eff = nu (:288)
+        # enable_body_force (:313-317): the nodal body_force field (n x dim) enters the velocity rhs
+        self.g = None if body_force is None else _f64(body_force).reshape(self.n, self.dim).copy()
         self.pbc_nodes = np.asarray(pbc_nodes, dtype=np.int64)
         self.ubc_nodes = np.asarray(ubc_nodes, dtype=np.int64)
         L = o.lib
@@ -88,7 +90,8 @@ class SemiImplicit:
         rhs = np.zeros(self.rows * self.dim)
         self._check(self.o.lib.orc_si_velocity_rhs(self.et, C.byref(self.prm), self.ne, _p(self.coords),
                                                    _p(self.conn, _u32), _p(self.p2m, _u32), _p(self.uadv),
-                                                   _p(self.nu), _p(a), _p(U), _p(dps), _p(P), _p(rhs)))
+                                                   _p(self.nu), _p(a), _p(U), _p(dps), _p(P), _p(rhs),
+                                                   None if self.g is None else _p(self.g)))
         return rhs
 
     def pressure_rhs(self, U, a, da, P):
