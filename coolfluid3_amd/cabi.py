@@ -14,7 +14,7 @@ HEADER = os.path.join(os.path.dirname(HERE), "include", "cf3hip.h")
 OK = 0
 ERR = {1: "INVALID_ARG", 2: "NOT_CREATED", 3: "HIP", 4: "OUT_OF_MEMORY", 5: "NOT_CONVERGED", 6: "ILLEGAL_ENTRY",
        7: "NOT_IMPLEMENTED", 8: "LIMIT", 9: "SINGULAR"}
-TRIAG2D, TETRA3D, HEXA3D, QUAD2D = 0, 1, 2, 3
+TRIAG2D, TETRA3D, HEXA3D, QUAD2D, PRISM3D = 0, 1, 2, 3, 4
 SUPG_TEZDUYAR, SUPG_METRIC, SUPG_CF2 = 0, 1, 2
 FP64, FP32 = 0, 1
 RHS, SOLUTION = 0, 1

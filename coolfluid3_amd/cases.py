@@ -132,6 +132,26 @@ def lid_cavity_cube_hexas(n, seed=DEFAULT_SEED, nu=2e-3, dt=5e-2, jitter=0.05, r
                 bc_regions=regions)
 
 
+def lid_cavity_cube_prisms(n, seed=DEFAULT_SEED, nu=2e-3, dt=5e-2, jitter=0.05, random_velocity=False):
+    """The cavity of lid_cavity_cube_hexas on prisms: each of the n^3 cells split into two prisms along its
+    xy diagonal (meshgen.box_prisms), the same node jitter, conditions and initial fields."""
+    coords, conn = meshgen.box_prisms(n, n, n)
+    dims = (n, n, n)
+    coords, used = meshgen.jitter(coords, dims, jitter, seed)
+    N = coords.shape[0]
+    if random_velocity:
+        u0 = meshgen.uniform(seed, N * 3, -1.0, 1.0, offset=used).reshape(N, 3)
+    else:
+        u0 = np.zeros((N, 3))
+    zero = {0: 0.0, 1: 0.0, 2: 0.0}
+    regions = [("xmin", _face_nodes(dims, 0, 0), zero), ("xmax", _face_nodes(dims, 0, 1), zero),
+               ("ymin", _face_nodes(dims, 1, 0), zero), ("ymax", _face_nodes(dims, 1, 1), zero),
+               ("zmin", _face_nodes(dims, 2, 0), zero), ("lid", _face_nodes(dims, 2, 1), {0: 1.0, 1: 0.0, 2: 0.0}),
+               ("pressure", np.array([0], np.uint32), {3: 0.0})]
+    return Case("lid_cavity_cube_prisms_%d" % n, cabi.PRISM3D, coords, conn, dims, nu, dt, u0=u0, p0=np.zeros(N),
+                bc_regions=regions)
+
+
 def bc_lists(case, u, p):
     """Ordered (nodes, eqs, values) of the Dirichlet increments new - old for the current fields."""
     nodes, eqs, vals = [], [], []

@@ -943,7 +943,7 @@ AsmArgs make_args(Lss& L, Fields& F, const cf3hip_ns_params& prm, bool pack = tr
   const Mesh& M = *L.mesh;
   if(F.mesh != L.mesh) CF3_THROW(CF3HIP_ERR_INVALID_ARG, "fields belong to another mesh");
   if(L.neq != M.dim + 1) CF3_THROW(CF3HIP_ERR_INVALID_ARG, "the NS system needs neq = dim + 1");
-  const bool spec = prm.use_specializations && !is_q1(M.etype);
+  const bool spec = prm.use_specializations && !is_cell(M.etype);
   if(!spec && prm.supg_type == CF3HIP_SUPG_CF2 && !(prm.u_ref > 0.))
     CF3_THROW(CF3HIP_ERR_INVALID_ARG, "supg_type cf2 needs a positive u_ref (SUPG.hpp:196)");
   AsmArgs g = {};
@@ -1014,7 +1014,7 @@ void assemble(Lss& L, Fields& F, const cf3hip_ns_params& prm)
   const Mesh& M = *L.mesh;
   if(prm.precision == CF3HIP_FP32 && !L.has_fp32)
     CF3_THROW(CF3HIP_ERR_NOT_CREATED, "FP32 assembly needs cf3hip_lss_enable_fp32");
-  const bool spec = prm.use_specializations && !is_q1(M.etype);
+  const bool spec = prm.use_specializations && !is_cell(M.etype);
   Context& c = *L.ctx;
   const bool fp32 = prm.precision == CF3HIP_FP32;
   // ZeroLSS: every value and rhs entry is overwritten by the kernel; the conditions of the previous
@@ -1027,7 +1027,7 @@ void assemble(Lss& L, Fields& F, const cf3hip_ns_params& prm)
   if(!c.async_mode) hipLaunchKernelGGL(k_clear_flag, dim3(1), dim3(1), 0, c.stream, g.status);
   g.values = fp32 ? L.values32.ptr : L.values.ptr;
 
-  if(is_q1(M.etype)) {
+  if(is_cell(M.etype)) {
     launch_hex(g, L, fp32 ? 1 : 0);
   } else if(spec && M.etype == CF3HIP_TETRA3D) {
     if(fp32) launch_spec<SpecP1<3, float>, float, float, false>(g, L);
@@ -1058,8 +1058,8 @@ void ns_residual(Lss& L, Fields& F, const cf3hip_ns_params& prm, const double* x
                      L.bc_n ? L.bc_vfirst.as<double>() : (const double*)nullptr, x, L.xtilde.as<double>());
   g.xt = L.xtilde.as<double>();
   g.res = r;
-  const bool spec = prm.use_specializations && !is_q1(M.etype);
-  if(is_q1(M.etype)) launch_hex(g, L, 2);
+  const bool spec = prm.use_specializations && !is_cell(M.etype);
+  if(is_cell(M.etype)) launch_hex(g, L, 2);
   else if(spec && M.etype == CF3HIP_TETRA3D) launch_residual<SpecP1<3, double> >(g, L);
   else if(spec && M.etype == CF3HIP_TRIAG2D) launch_residual<SpecP1<2, double> >(g, L);
   else if(M.etype == CF3HIP_TETRA3D) launch_residual<GenericP1<3> >(g, L);

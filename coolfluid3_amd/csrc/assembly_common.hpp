@@ -161,8 +161,6 @@ __device__ __forceinline__ void lds_load_row(const VR* src, VR v[NN])
   }
 }
 
-// Q1 cells: hexahedra and quadrilaterals, always the generic expression (assembly_hex.hip)
-inline bool is_q1(int etype) { return etype == CF3HIP_HEXA3D || etype == CF3HIP_QUAD2D; }
 // mode 0 = FP64 matrix + rhs, 1 = FP32 matrix, 2 = matrix-free residual
 void launch_hex(AsmArgs g, Lss& L, int mode);
 

@@ -88,7 +88,7 @@ using namespace cf3hip;
 
 namespace {
 
-int etype_nodes(int et) { return et == CF3HIP_TRIAG2D ? 3 : (et == CF3HIP_HEXA3D ? 8 : 4); }
+int etype_nodes(int et) { return et == CF3HIP_TRIAG2D ? 3 : (et == CF3HIP_HEXA3D ? 8 : (et == CF3HIP_PRISM3D ? 6 : 4)); }
 int etype_dim(int et) { return et == CF3HIP_TRIAG2D || et == CF3HIP_QUAD2D ? 2 : 3; }
 
 void h2d(Context& c, void* dst, const void* src, size_t bytes)
@@ -293,7 +293,8 @@ int cf3hip_mesh_create(cf3hip_context ctx, int etype, uint32_t n_nodes, uint32_t
 {
   API_BEGIN
   REQUIRE(ctx && coords && conn && out, CF3HIP_ERR_INVALID_ARG, "null argument");
-  REQUIRE(etype == CF3HIP_TRIAG2D || etype == CF3HIP_TETRA3D || etype == CF3HIP_HEXA3D || etype == CF3HIP_QUAD2D,
+  REQUIRE(etype == CF3HIP_TRIAG2D || etype == CF3HIP_TETRA3D || etype == CF3HIP_HEXA3D || etype == CF3HIP_QUAD2D ||
+              etype == CF3HIP_PRISM3D,
           CF3HIP_ERR_INVALID_ARG, "unknown element type");
   REQUIRE(n_nodes > 0 && n_elems > 0, CF3HIP_ERR_INVALID_ARG, "empty mesh");
   const int k = etype_nodes(etype), dim = etype_dim(etype);

@@ -285,6 +285,12 @@ void si_update(SemiImplicit& S, Fields& F);
 void si_zero_dofs(SemiImplicit& S, const std::vector<uint32_t>& dofs, double* v);
 void ns_update(Lss& L, Fields& F);
 
+// cells of the Gauss-point lane kernel (assembly_hex.hip): Q1 hexahedra and quadrilaterals and P1
+// prisms, which always take the generic expression
+inline bool is_cell(int etype) { return etype == CF3HIP_HEXA3D || etype == CF3HIP_QUAD2D || etype == CF3HIP_PRISM3D; }
+// lanes per incidence of that kernel: one per Gauss point / column node, 8 on prisms (6 used)
+inline int cell_lanes(int etype, int k) { return etype == CF3HIP_PRISM3D ? 8 : k; }
+
 inline unsigned grid_for(size_t n, unsigned block)
 {
   size_t g = (n + block - 1) / block;

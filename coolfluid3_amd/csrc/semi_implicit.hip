@@ -1,8 +1,9 @@
 // UFEM NavierStokesSemiImplicit on the device (gfx950): the element actions of
 // plugins/UFEM/src/UFEM/ns_semi_implicit/MatrixAssembly.hpp and PressureSystem.cpp, and the time step of
 // ns_semi_implicit/implementation/NavierStokesSemiImplicit.cpp (LinearizeU, velocity assembly,
-// SetSolution, InnerLoop, Update) on a velocity system (neq = dim) and a pressure system (neq = 1) that
-// share one mesh and its periodic links.  Triangles, quadrilaterals, tetrahedra and hexahedra.
+// SetSolution, InnerLoop, Update) on a velocity system (neq = dim) and a pressure system (neq = 1)
+// that share one mesh and its periodic links.  Triangles, quadrilaterals, tetrahedra, hexahedra and
+// prisms (ns_semi_implicit/implementation/{Triags,Quads,Tetras,Hexas,Prisms}MatrixAssembly.cpp).
 //
 // Each element action runs in two passes.  k_si_elem computes one element per thread and writes its
 // element vector or matrix to a staging buffer; k_si_gather then gives every matrix block slot (or
@@ -22,8 +23,8 @@ namespace {
 
 // ------------------------------------------------------------------------------------------------
 // element types: shape functions, Gauss rules (cf3/mesh/LagrangeP1/Triag2D.cpp, Quad2D.cpp, Tetra3D.cpp,
-// Hexa3D.cpp; cf3/mesh/Integrators/GaussImplementation.hpp), IntegralOrder (MatrixAssembly.hpp:28-47:
-// 1 for triangles and tetrahedra, 2 otherwise)
+// Hexa3D.cpp, Prism.cpp; cf3/mesh/Integrators/GaussImplementation.hpp), IntegralOrder
+// (MatrixAssembly.hpp:28-47: 1 for triangles and tetrahedra, 2 otherwise)
 // ------------------------------------------------------------------------------------------------
 struct Tri {
   static constexpr int K = 3, D = 2, ORDER = 1;
@@ -109,6 +110,35 @@ struct Hex {
       dN[0][m] = 0.125 * sx * b * c;
       dN[1][m] = 0.125 * sy * a * c;
       dN[2][m] = 0.125 * sz * a * b;
+    }
+  }
+};
+
+struct Prism {
+  static constexpr int K = 6, D = 3, ORDER = 2;
+  __device__ static int npts(int order) { return order == 1 ? 1 : 6; }
+  __device__ static void point(int order, int q, double xi[3], double& w)
+  {
+    // order 1: (1/3, 1/3, 0), w 1; order 2: the triangle's edge midpoints (1/2, 0), (1/2, 1/2), (0, 1/2)
+    // at zeta = +1/sqrt(3), then at zeta = -1/sqrt(3), w 1/6 (GaussImplementation.hpp:430-494)
+    if(order == 1) { xi[0] = xi[1] = 0.3333333333333333333333333; xi[2] = 0.; w = 1.; return; }
+    const int t = q < 3 ? q : q - 3;
+    xi[0] = t == 2 ? 0. : 0.5;
+    xi[1] = t == 0 ? 0. : 0.5;
+    xi[2] = q < 3 ? 0.5773502691896257645091488 : -0.5773502691896257645091488;
+    w = 1. / 6.;
+  }
+  __device__ static void shape(const double xi[3], double N[6], double dN[3][6])
+  {
+    // the triangle (0,0), (1,0), (0,1) at zeta = -1 (nodes 0-2), then at zeta = +1 (Prism.cpp:24-58)
+    const double lo = 0.5 * (1. - xi[2]), hi = 0.5 * (1. + xi[2]);
+    const double t[3] = {1. - xi[0] - xi[1], xi[0], xi[1]};
+    const double tx[3] = {-1., 1., 0.}, ty[3] = {-1., 0., 1.};
+    for(int m = 0; m < 3; ++m) {
+      N[m] = lo * t[m]; N[m + 3] = hi * t[m];
+      dN[0][m] = lo * tx[m]; dN[0][m + 3] = hi * tx[m];
+      dN[1][m] = lo * ty[m]; dN[1][m + 3] = hi * ty[m];
+      dN[2][m] = -0.5 * t[m]; dN[2][m + 3] = 0.5 * t[m];
     }
   }
 };
@@ -870,6 +900,7 @@ void launch_elem(SemiImplicit& S, const SiArgs& g)
     case CF3HIP_TRIAG2D: hipLaunchKernelGGL((k_si_elem<Tri, OP>), grid, dim3(256), 0, st, g); break;
     case CF3HIP_QUAD2D: hipLaunchKernelGGL((k_si_elem<Quad, OP>), grid, dim3(256), 0, st, g); break;
     case CF3HIP_TETRA3D: hipLaunchKernelGGL((k_si_elem<Tet, OP>), grid, dim3(256), 0, st, g); break;
+    case CF3HIP_PRISM3D: hipLaunchKernelGGL((k_si_elem<Prism, OP>), grid, dim3(256), 0, st, g); break;
     default: hipLaunchKernelGGL((k_si_elem<Hex, OP>), grid, dim3(256), 0, st, g); break;
   }
   HIP_OK(hipGetLastError());

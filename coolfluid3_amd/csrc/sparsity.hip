@@ -422,9 +422,10 @@ void build_element_structures(Lss& L)
     HIP_OK(hipGetLastError());
   }
 
-  const bool q1 = M.etype == CF3HIP_HEXA3D || M.etype == CF3HIP_QUAD2D;   // one lane per column node
+  const bool q1 = is_cell(M.etype);   // one lane per column node (and Gauss point)
+  const int lanes = cell_lanes(M.etype, (int)k);
   L.passes = q1 ? 1 : (int)k;
-  L.pos_stride = q1 ? (int)k : 4;
+  L.pos_stride = q1 ? lanes : 4;
 
   // ---- per-slot contribution lists sorted by (slot, b, q) ----
   DevBuf contrib_ptr, contrib, acc_cnt, row_acc;
@@ -460,7 +461,7 @@ void build_element_structures(Lss& L)
   HIP_OK(hipStreamSynchronize(c.stream));
   // a workgroup holds at most inc_limit incidences (simplices: one incidence per thread; Q1 cells:
   // one (incidence, column node) pair per thread) and kAsmThreads accumulator threads
-  const uint64_t inc_limit = q1 ? kAsmThreads / k : kAsmThreads;
+  const uint64_t inc_limit = q1 ? kAsmThreads / lanes : kAsmThreads;
   std::vector<uint32_t> wg;
   wg.push_back(0);
   uint32_t r = 0;

@@ -79,6 +79,16 @@ def box_hexas(nx, ny, nz, lx=1.0, ly=1.0, lz=1.0):
     return _box_nodes(nx, ny, nz, lx, ly, lz), _hex_conn(nx, ny, nz)
 
 
+def box_prisms(nx, ny, nz, lx=1.0, ly=1.0, lz=1.0):
+    """Each hexahedron of the box split into two prisms along its xy diagonal from (i, j) to (i+1, j+1):
+    bottom triangles (i,j), (i+1,j), (i+1,j+1) and (i,j), (i+1,j+1), (i,j+1), counter-clockwise seen from
+    +z, each extruded to the layer above (the Prism3D node order: bottom triangle, then top)."""
+    h = _hex_conn(nx, ny, nz)
+    a = h[:, [0, 1, 2, 4, 5, 6]]
+    b = h[:, [0, 2, 3, 4, 6, 7]]
+    return _box_nodes(nx, ny, nz, lx, ly, lz), np.ascontiguousarray(np.stack([a, b], axis=1).reshape(-1, 6))
+
+
 def box_tetras(nx, ny, nz, lx=1.0, ly=1.0, lz=1.0):
     h = _hex_conn(nx, ny, nz)
     conn = h[:, KUHN].reshape(-1, 4)

@@ -166,8 +166,10 @@ def _block(lines, title):
 def read_msh(path):
     """Read a Gmsh 2.2 ASCII file.  Every element's first tag names a physical group of $PhysicalNames
     (required, as in the reference); elements of the mesh dimension must be triangles, quadrilaterals,
-    tetrahedra or hexahedra (the reference also reads prisms, pyramids and higher orders, for which this
-    path has no kernel: ValueError)."""
+    tetrahedra or hexahedra (ValueError otherwise).  The reference also reads pyramids and higher orders,
+    which have no kernel here, and prisms, through the node order 0, 2, 1, 3, 5, 4
+    (gmsh/Shared.cpp:375-390), which turns a prism of positive gmsh orientation into one of negative
+    Jacobian; prisms for this path come from meshgen.box_prisms."""
     with open(path) as f:
         lines = [l.strip() for l in f.read().splitlines()]
     fmt = _block(lines, "MeshFormat")

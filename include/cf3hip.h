@@ -36,11 +36,12 @@ extern "C" {
 #define CF3HIP_ERR_LIMIT          8  /* mesh exceeds a documented kernel limit              */
 #define CF3HIP_ERR_SINGULAR       9  /* non-invertible element Jacobian (ElementData.hpp:312-313) */
 
-/* element types (cf3/mesh/LagrangeP1: Triag2D, Tetra3D, Hexa3D) */
+/* element types (cf3/mesh/LagrangeP1: Triag2D, Tetra3D, Hexa3D, Quad2D, Prism3D) */
 #define CF3HIP_TRIAG2D 0
 #define CF3HIP_TETRA3D 1
 #define CF3HIP_HEXA3D  2
 #define CF3HIP_QUAD2D  3   /* LagrangeP1 Quad2D: generic expression, 2 x 2 Gauss points */
+#define CF3HIP_PRISM3D 4   /* LagrangeP1 Prism3D (NavierStokesPrisms.cpp): generic expression, 6 Gauss points */
 
 /* SUPG tau variants (plugins/UFEM/src/UFEM/SUPG.hpp:62-74) */
 #define CF3HIP_SUPG_TEZDUYAR 0

@@ -38,15 +38,22 @@ namespace {
 typedef double Real;
 typedef uint32_t Uint;
 
-inline int enodes(int et) { return et == ORC_TRIAG2D ? 3 : (et == ORC_HEXA3D ? 8 : 4); }
+inline int enodes(int et) { return et == ORC_TRIAG2D ? 3 : (et == ORC_HEXA3D ? 8 : (et == ORC_PRISM3D ? 6 : 4)); }
 inline int edim(int et) { return et == ORC_TRIAG2D || et == ORC_QUAD2D ? 2 : 3; }
 
 // ---------------------------------------------------------------------------------------------
 // Shape functions: Triag.cpp:24-41, Tetra.cpp:24-48, Hexa.cpp:24-89, Quad.cpp (compute_value,
-// compute_gradient: node order (-1,-1), (1,-1), (1,1), (-1,1))
+// compute_gradient: node order (-1,-1), (1,-1), (1,1), (-1,1)), Prism.cpp:24-58 (the triangle
+// (0,0), (1,0), (0,1) at zeta = -1, then at zeta = +1)
 // ---------------------------------------------------------------------------------------------
 void shape_values(int et, const Real* xi, Real* N)
 {
+  if(et == ORC_PRISM3D) {
+    const Real lo = 0.5 * (1. - xi[2]), hi = 0.5 * (1. + xi[2]), l0 = 1. - xi[0] - xi[1];
+    N[0] = lo * l0; N[1] = lo * xi[0]; N[2] = lo * xi[1];
+    N[3] = hi * l0; N[4] = hi * xi[0]; N[5] = hi * xi[1];
+    return;
+  }
   if(et == ORC_QUAD2D) {
     const Real ksi = xi[0], eta = xi[1];
     N[0] = 0.25 * (1.0 - ksi) * (1.0 - eta);
@@ -75,6 +82,17 @@ void shape_values(int et, const Real* xi, Real* N)
 // dN(xi_dir, node) row-major dim x k
 void shape_gradients(int et, const Real* xi, Real* dN)
 {
+  if(et == ORC_PRISM3D) {
+    const Real ksi = xi[0], eta = xi[1], zta = xi[2];
+    Real* K = dN; Real* E = dN + 6; Real* Z = dN + 12;
+    K[0] = 0.5 * (zta - 1.); E[0] = 0.5 * (zta - 1.); Z[0] = 0.5 * (ksi + eta - 1.);
+    K[1] = 0.5 * (1. - zta); E[1] = 0.;              Z[1] = -0.5 * ksi;
+    K[2] = 0.;               E[2] = 0.5 * (1. - zta); Z[2] = -0.5 * eta;
+    K[3] = -0.5 * (zta + 1.); E[3] = -0.5 * (zta + 1.); Z[3] = 0.5 * (1. - ksi - eta);
+    K[4] = 0.5 * (1. + zta); E[4] = 0.;              Z[4] = 0.5 * ksi;
+    K[5] = 0.;               E[5] = 0.5 * (1. + zta); Z[5] = 0.5 * eta;
+    return;
+  }
   if(et == ORC_QUAD2D) {
     const Real ksi = xi[0], eta = xi[1];
     Real* K = dN; Real* E = dN + 4;
@@ -109,6 +127,19 @@ void shape_gradients(int et, const Real* xi, Real* dN)
 // ---------------------------------------------------------------------------------------------
 void jacobian(int et, const Real* xi, const Real* X, Real* J)
 {
+  if(et == ORC_PRISM3D) {
+    // Prism3D::compute_jacobian (Prism3D.cpp:93-127): J(k, d) = sum_m dN_m/dxi_k x_m,d, grouped as the
+    // bottom / top node differences (which cancel exactly on right prisms)
+    const Real ksi = xi[0], eta = xi[1], zta = xi[2];
+    const Real lo = -0.5*zta + 0.5, hi = 0.5*zta + 0.5, l0 = -eta - ksi + 1;
+    for(int d = 0; d < 3; ++d) {
+      const Real x0 = X[0*3+d], x1 = X[1*3+d], x2 = X[2*3+d], x3 = X[3*3+d], x4 = X[4*3+d], x5 = X[5*3+d];
+      J[0*3+d] = -x0*lo + x1*lo - x3*hi + x4*hi;
+      J[1*3+d] = -x0*lo + x2*lo - x3*hi + x5*hi;
+      J[2*3+d] = -0.5*eta*x2 + 0.5*eta*x5 - 0.5*ksi*x1 + 0.5*ksi*x4 - 0.5*x0*l0 + 0.5*x3*l0;
+    }
+    return;
+  }
   if(et == ORC_QUAD2D) {
     // Quad2D::compute_jacobian with the JacobianCoefficients of Quad2D.hpp:71-96
     const Real bx = 0.25*(-X[0] + X[2] + X[4] - X[6]), cx = 0.25*(-X[0] - X[2] + X[4] + X[6]);
@@ -266,6 +297,13 @@ Real volume(int et, const Real* X)
   if(et == ORC_QUAD2D)   // Quad2D::volume: half the cross product of the diagonals
     return 0.5 * ((X[4] - X[0]) * (X[7] - X[3]) - (X[5] - X[1]) * (X[6] - X[2]));
   if(et == ORC_TETRA3D) return tetra_det(X) / 6.;
+  if(et == ORC_PRISM3D) {   // Prism3D::volume (Prism3D.cpp:163-170): det J at (1/3, 1/3, 0)
+    const Real c[3] = {0.3333333333333333333333333, 0.3333333333333333333333333, 0.};
+    Real J[9], Ji[9], det;
+    jacobian(et, c, X, J);
+    inverse_det(3, J, Ji, det);
+    return det;
+  }
   const Real zero[3] = {0., 0., 0.};
   return 8 * hexa_det(zero, X);
 }
@@ -297,6 +335,17 @@ int gauss(int et, int order, Real* xi, Real* w)
     std::memcpy(xi, c, sizeof(c));
     for(int i = 0; i < 4; ++i) w[i] = 0.04166666666666666666666666666666666667;
     return 4;
+  }
+  if(et == ORC_PRISM3D) {   // GaussImplementation.hpp:430-494: (1/3, 1/3, 0) w 1; the triangle's edge
+                            // midpoints at zeta = +x, then at zeta = -x, w 1/6
+    if(order == 1) { xi[0] = xi[1] = 0.3333333333333333333333333; xi[2] = 0.; w[0] = 1.; return 1; }
+    const Real x = 0.5773502691896257645091488;
+    const Real t[3][2] = {{0.5, 0.0}, {0.5, 0.5}, {0.0, 0.5}};
+    for(int q = 0; q < 6; ++q) {
+      xi[3*q] = t[q % 3][0]; xi[3*q+1] = t[q % 3][1]; xi[3*q+2] = q < 3 ? x : -x;
+      w[q] = 1. / 6.;
+    }
+    return 6;
   }
   if(order == 1) { xi[0] = xi[1] = xi[2] = 0.; w[0] = 8.; return 1; }
   const Real x = 0.5773502691896257645091488;

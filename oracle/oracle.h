@@ -20,7 +20,7 @@ extern "C" {
 #endif
 
 /* element types (cf3/mesh/LagrangeP1) */
-enum { ORC_TRIAG2D = 0, ORC_TETRA3D = 1, ORC_HEXA3D = 2, ORC_QUAD2D = 3 };
+enum { ORC_TRIAG2D = 0, ORC_TETRA3D = 1, ORC_HEXA3D = 2, ORC_QUAD2D = 3, ORC_PRISM3D = 4 };
 
 /* SUPG tau variants (plugins/UFEM/src/UFEM/SUPG.hpp:62-74) */
 enum { ORC_SUPG_TEZDUYAR = 0, ORC_SUPG_METRIC = 1, ORC_SUPG_CF2 = 2 };

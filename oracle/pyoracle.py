@@ -12,10 +12,10 @@ import numpy as np
 _HERE = os.path.dirname(os.path.abspath(__file__))
 LIB_PATH = os.path.join(_HERE, "build", "liboracle.so")
 
-TRIAG2D, TETRA3D, HEXA3D, QUAD2D = 0, 1, 2, 3
+TRIAG2D, TETRA3D, HEXA3D, QUAD2D, PRISM3D = 0, 1, 2, 3, 4
 SUPG_TEZDUYAR, SUPG_METRIC, SUPG_CF2 = 0, 1, 2
-NODES = {TRIAG2D: 3, TETRA3D: 4, HEXA3D: 8, QUAD2D: 4}
-DIM = {TRIAG2D: 2, TETRA3D: 3, HEXA3D: 3, QUAD2D: 2}
+NODES = {TRIAG2D: 3, TETRA3D: 4, HEXA3D: 8, QUAD2D: 4, PRISM3D: 6}
+DIM = {TRIAG2D: 2, TETRA3D: 3, HEXA3D: 3, QUAD2D: 2, PRISM3D: 3}
 
 _d = C.POINTER(C.c_double)
 _u32 = C.POINTER(C.c_uint32)

@@ -9,6 +9,8 @@ Sources:
   test/mesh/utest-mesh-lagrangep1-tetra3d.cpp:36-175  (nodes, volume, Jacobian, adjoint)
   test/mesh/utest-mesh-lagrangep1-hexa3d.cpp:36-270   (nodes, shape values/gradients, Jacobian, adjoint, det)
   test/mesh/utest-mesh-lagrangep1-triag2d.cpp:36-178  (nodes, volumes, Jacobian, adjoint)
+  test/mesh/utest-mesh-lagrangep1-prism3d.cpp:36-178  (nodes, volume, shape values/gradients, integrals,
+                                                       det, Jacobian, adjoint)
   test/math/utest-lss-test-matrix.hpp:32-719          (4-rank Stokes system: per-rank element lists, node-graph
                                                        CSR, owned-row values before / after the conditions, rhs,
                                                        solution, Dirichlet and periodic conditions, result)
@@ -75,6 +77,19 @@ def main(ref):
         "jacobian_skewed": [[0.5625, 0.06250, 0.06250], [0.07500, 0.5750, 0.07500], [0.1125, 0.1125, 0.6125]],
         "adjoint_skewed": [[0.34375, -0.03125, -0.03125], [-0.03750, 0.3375, -0.03750],
                            [-0.05625, -0.05625, 0.31875]],
+    }
+    out["prism3d"] = {
+        "mapped_coords": [0.5, 0.4, 0.1],
+        "unit_nodes": [[0, 0, 0], [1, 0, 0], [0, 1, 0], [0, 0, 1], [1, 0, 1], [0, 1, 1]],
+        "skewed_nodes": [[0.2, 0.3, 0], [1.2, 0.3, 0], [0.2, 1.3, 0], [0.2, 0.3, 1], [1.2, 0.3, 1], [0.2, 1.3, 1]],
+        "volume_unit": 0.5, "volume_skewed": 0.5,
+        "shape_values": [0.045, 0.225, 0.18, 0.055, 0.275, 0.22],
+        "shape_gradients": [[-0.45, 0.45, 0, -0.55, 0.55, 0], [-0.45, 0, 0.45, -0.55, 0, 0.55],
+                            [-0.05, -0.25, -0.2, 0.05, 0.25, 0.2]],
+        "integral_det_order1": 0.5, "integral_det_order2": 0.5,
+        "det_unit": 0.5, "det_skewed": 0.5,
+        "jacobian_skewed": [[1, 0, 0], [0, 1, 0], [0, 0, 0.5]],
+        "adjoint_skewed": [[0.5, 0, 0], [0, 0.5, 0], [0, 0, 1.0]],
     }
     out["triag2d"] = {
         "nodes": [[0.5, 0.3], [1.1, 1.2], [0.8, 2.1]],

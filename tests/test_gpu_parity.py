@@ -47,7 +47,7 @@ def oracle_system(o, case, spec=True, supg=pyoracle.SUPG_METRIC, theta=1.0, g=No
     prm = pyoracle.ns_params(rho=case.rho, dt=case.dt, theta=theta, use_specializations=spec, supg_type=supg)
     ua = o.linearize_u(case.u0, case.u0, case.u0, case.u0)
     et = {cabi.TRIAG2D: pyoracle.TRIAG2D, cabi.TETRA3D: pyoracle.TETRA3D, cabi.HEXA3D: pyoracle.HEXA3D,
-          cabi.QUAD2D: pyoracle.QUAD2D}[case.etype]
+          cabi.QUAD2D: pyoracle.QUAD2D, cabi.PRISM3D: pyoracle.PRISM3D}[case.etype]
     vals, rhs = o.ns_assemble(et, prm, case.coords, case.conn, case.u0, case.p0, ua,
                               np.full(case.n_nodes, case.nu), start, cols, slots, g=g)
     return start, cols, slots, vals, rhs
@@ -136,10 +136,13 @@ CASES = {
     "tet12s": lambda: cases.inflow_cube_tets(12, skew=True),
     "hex3": lambda: cases.lid_cavity_cube_hexas(3, random_velocity=True),
     "hex5": lambda: cases.lid_cavity_cube_hexas(5, random_velocity=True),
+    "prism3": lambda: cases.lid_cavity_cube_prisms(3, random_velocity=True),
+    "prism4": lambda: cases.lid_cavity_cube_prisms(4, random_velocity=True),
 }
 
 
-@pytest.mark.parametrize("name", ["tri8", "ring", "rect", "tet5", "tet12", "hex5", "duct", "scattered", "quad9"])
+@pytest.mark.parametrize("name", ["tri8", "ring", "rect", "tet5", "tet12", "hex5", "duct", "scattered", "quad9",
+                                  "prism4"])
 def test_sparsity_and_slot_map_bit_exact(ctx, oracle, name):
     case = CASES[name]()
     print("case", name, flush=True)
@@ -164,7 +167,9 @@ def test_sparsity_and_slot_map_bit_exact(ctx, oracle, name):
     ("hex3", False, cabi.SUPG_METRIC, 1.0), ("hex5", False, cabi.SUPG_METRIC, 0.5),
     ("hex3", False, cabi.SUPG_TEZDUYAR, 0.8),
     ("tri8", False, cabi.SUPG_CF2, 1.0), ("tet5", False, cabi.SUPG_CF2, 0.7), ("tet12s", False, cabi.SUPG_CF2, 1.0),
-    ("quad9", False, cabi.SUPG_CF2, 0.5), ("hex3", False, cabi.SUPG_CF2, 1.0), ("duct", False, cabi.SUPG_CF2, 1.0)])
+    ("quad9", False, cabi.SUPG_CF2, 0.5), ("hex3", False, cabi.SUPG_CF2, 1.0), ("duct", False, cabi.SUPG_CF2, 1.0),
+    ("prism3", False, cabi.SUPG_METRIC, 1.0), ("prism4", False, cabi.SUPG_METRIC, 0.5),
+    ("prism3", False, cabi.SUPG_TEZDUYAR, 0.8), ("prism3", False, cabi.SUPG_CF2, 1.0)])
 def test_assembly_fp64(ctx, oracle, name, spec, supg, theta):
     case = CASES[name]()
     print("case", name, spec, supg, theta, flush=True)
@@ -182,7 +187,7 @@ def test_assembly_fp64(ctx, oracle, name, spec, supg, theta):
     g.close()
 
 
-@pytest.mark.parametrize("name", ["tri8", "ring", "rect", "tet12", "tet12s", "hex5", "duct", "quad9"])
+@pytest.mark.parametrize("name", ["tri8", "ring", "rect", "tet12", "tet12s", "hex5", "duct", "quad9", "prism4"])
 def test_assembly_fp32_error_stated(ctx, oracle, name):
     case = CASES[name]()
     start, cols, slots, vref, rref = oracle_system(oracle, case)
@@ -195,7 +200,7 @@ def test_assembly_fp32_error_stated(ctx, oracle, name):
     g.close()
 
 
-@pytest.mark.parametrize("name,spec", [("tet5", False), ("hex3", False)])
+@pytest.mark.parametrize("name,spec", [("tet5", False), ("hex3", False), ("prism3", False)])
 def test_assembly_body_force(ctx, oracle, name, spec):
     """The body-force source term a[u_i] += (N + tau_su u_adv.grad N)^T g of the generic expression."""
     case = CASES[name]()
@@ -332,7 +337,7 @@ def test_fp32_refinement_reaches_fp64_residual(ctx, oracle):
     g.close()
 
 
-@pytest.mark.parametrize("name", ["tet5", "hex3"])
+@pytest.mark.parametrize("name", ["tet5", "hex3", "prism3"])
 def test_matrix_free_residual_matches_stored_system(ctx, oracle, name):
     """r = b' - A' x from the FP64 element operator equals rhs - A x of the assembled, constrained system."""
     case = CASES[name]()
@@ -347,7 +352,7 @@ def test_matrix_free_residual_matches_stored_system(ctx, oracle, name):
     A = s.values()
     b = s.vector(cabi.RHS)
     r_ref = b - oracle.spmv(4, start, cols, A, x)
-    if case.etype == cabi.HEXA3D:
+    if case.etype in (cabi.HEXA3D, cabi.PRISM3D):
         g.fields.copy(cabi.FIELD_UADV, cabi.FIELD_U)   # same linearisation as the assembly
     s.set_vector(cabi.SOLUTION, x)
     s.ns_residual_device(g.fields, g.prm, s.vector_ptr(cabi.SOLUTION), s.vector_ptr(cabi.RHS))
@@ -491,8 +496,10 @@ def test_hexa_picard_precision_form(ctx, oracle):
     g64.close()
 
 
-def test_time_step_vs_oracle(ctx, oracle):
-    case = CASES["tet5"]()
+@pytest.mark.parametrize("name", ["tet5", "prism3"])
+def test_time_step_vs_oracle(ctx, oracle, name):
+    case = CASES[name]()
+    print("case", name, flush=True)
     g = gpu_case(ctx, case)
     st = g.step(cabi.solver_params(tol=1e-12))
     # oracle step

@@ -10,12 +10,15 @@ The reference's only comparison test of these kernels is commented out
   * generic hexahedron terms against an independent 4x4x4 tensor Gauss integration on affine hexes,
     and generic quadrilateral (Quad2D) terms against a 4x4 rule on parallelograms, with its shape
     functions, Gauss rule, volume and METRIC tau checked against closed forms;
+  * generic prism (Prism3D) terms against a collapsed-Gauss triangle rule times a 4-point rule in zeta
+    on affine prisms, and METRIC tau at the prism centre (1/3, 1/3, 0) against its closed form;
   * the combine step (p rows / theta, rhs formed before the division, node-major reorder).
 """
 import numpy as np
 import pytest
 
-from oracle.pyoracle import HEXA3D, QUAD2D, SUPG_CF2, SUPG_METRIC, SUPG_TEZDUYAR, TETRA3D, TRIAG2D, ns_params
+from oracle.pyoracle import (HEXA3D, PRISM3D, QUAD2D, SUPG_CF2, SUPG_METRIC, SUPG_TEZDUYAR, TETRA3D, TRIAG2D,
+                             ns_params)
 
 
 def p1_grads(nodes):
@@ -187,6 +190,86 @@ def test_generic_hex_vs_tensor_gauss(oracle, case):
     # the mass matrix of every velocity block sums to the volume (partition of unity)
     if case == 0:
         assert T[:8, :8].sum() == pytest.approx(abs(np.linalg.det(B)), rel=1e-13)
+
+
+# ---- generic prism (Prism3D) vs a collapsed-Gauss triangle rule x Gauss in zeta on affine prisms ----
+def prism_sf(xi):
+    """Wedge shape functions written from their definition: the P1 triangle (0,0), (1,0), (0,1) times
+    the linear interval (1 - zeta)/2, (1 + zeta)/2, bottom nodes first."""
+    t = np.array([1 - xi[0] - xi[1], xi[0], xi[1]])
+    dt = np.array([[-1.0, 1.0, 0.0], [-1.0, 0.0, 1.0]])
+    z = np.array([(1 - xi[2]) / 2, (1 + xi[2]) / 2])
+    dz = np.array([-0.5, 0.5])
+    N = np.concatenate([t * z[0], t * z[1]])
+    dN = np.zeros((3, 6))
+    dN[0] = np.concatenate([dt[0] * z[0], dt[0] * z[1]])
+    dN[1] = np.concatenate([dt[1] * z[0], dt[1] * z[1]])
+    dN[2] = np.concatenate([t * dz[0], t * dz[1]])
+    return N, dN
+
+
+def prism_reference_terms(X, ua, nu, tau_ps, tau_su, tau_bulk, rho):
+    x, w = np.polynomial.legendre.leggauss(4)
+    u01, w01 = (x + 1) / 2, w / 2                # Gauss-Legendre on [0, 1]
+    k, n = 6, 24
+    A = np.zeros((n, n)); T = np.zeros((n, n))
+    for i, a in enumerate(u01):
+        for j, b in enumerate(u01):
+            for l, c in enumerate(x):
+                xi = np.array([a, (1 - a) * b, c])   # collapsed map of the unit square onto the triangle
+                N, dN = prism_sf(xi)
+                J = dN @ X
+                G = np.linalg.solve(J, dN)
+                wq = w01[i] * w01[j] * (1 - a) * w[l] * np.linalg.det(J)
+                L = ua @ G
+                SU = N + tau_su * L
+                P = 3 * k
+                for R in range(3):
+                    A[P:, R * k:(R + 1) * k] += wq * (np.outer(N + 0.5 * tau_ps * L, G[R]) + tau_ps * np.outer(G[R], L))
+                    A[R * k:(R + 1) * k, R * k:(R + 1) * k] += wq * (nu * G.T @ G + np.outer(SU, L))
+                    A[R * k:(R + 1) * k, P:] += wq * np.outer(SU, G[R]) / rho
+                    for C in range(3):
+                        A[R * k:(R + 1) * k, C * k:(C + 1) * k] += wq * np.outer(tau_bulk * G[R] + 0.5 * ua[R] * SU, G[C])
+                    T[P:, R * k:(R + 1) * k] += wq * tau_ps * np.outer(G[R], N)
+                    T[R * k:(R + 1) * k, R * k:(R + 1) * k] += wq * np.outer(SU, N)
+                A[P:, P:] += wq * tau_ps * G.T @ G / rho
+    return A, T
+
+
+_PLOC = np.array([[0, 0, -1], [1, 0, -1], [0, 1, -1], [0, 0, 1], [1, 0, 1], [0, 1, 1.]])
+
+
+@pytest.mark.parametrize("case", [0, 1])
+def test_generic_prism_vs_collapsed_gauss(oracle, case):
+    rng = np.random.default_rng(60 + case)
+    B = np.eye(3) * 0.2 + rng.uniform(-0.03, 0.03, (3, 3))   # affine map: the 6-point rule is exact
+    X = _PLOC @ B.T + rng.uniform(0, 1, 3)
+    ua = np.array([0.4, -0.7, 0.2]) * case
+    nu, rho = 0.01, 1.1
+    u = np.tile(ua, (6, 1))
+    A, T, a, tau = oracle.ns_element(PRISM3D, ns_params(rho=rho, dt=0.02, use_specializations=False), X, u, u,
+                                     np.full(6, nu))
+    Ar, Tr = prism_reference_terms(X, ua, nu, tau[0], tau[1], tau[2], rho)
+    assert np.abs(A - Ar).max() <= 1e-12 * np.abs(Ar).max()
+    assert np.abs(T - Tr).max() <= 1e-12 * np.abs(Tr).max()
+    if case == 0:   # the mass matrix of a velocity block sums to the volume
+        assert T[:6, :6].sum() == pytest.approx(abs(np.linalg.det(B)), rel=1e-13)
+
+
+def test_tau_metric_sheared_prism(oracle):
+    """Affine prism x = c + B xi: J = B^T everywhere, so gij = J^-T J^-1 = B^-1 B^-T at the centre."""
+    h, s = 0.1, 0.04
+    B = np.array([[h, s, 0], [0, h, 0], [0, 0, h / 2]])
+    X = _PLOC @ B.T + np.array([0.3, 0.2, 0.1])
+    Binv = np.array([[1 / h, -s / h ** 2, 0], [0, 1 / h, 0], [0, 0, 2 / h]])
+    g = Binv @ Binv.T
+    uc = np.array([0.4, 0.9, -0.2])
+    nu, dt = 0.005, 0.01
+    u = np.tile(uc, (6, 1))
+    _, _, _, tau = oracle.ns_element(PRISM3D, ns_params(dt=dt, use_specializations=False), X, u, u, np.full(6, nu))
+    ts, tb = _metric_tau(g, uc, nu, dt)
+    assert tau[1] == pytest.approx(ts, rel=1e-13) and tau[2] == pytest.approx(tb, rel=1e-13)
+    assert abs(_metric_tau(Binv.T @ Binv, uc, nu, dt)[0] - ts) > 1e-6 * ts
 
 
 # ---- generic quadrilateral (Quad2D) ----

@@ -1,13 +1,13 @@
 """Pins the oracle's element geometry, shape functions and Gauss rules.
 
 Known answers come from the reference's own unit tests (tests/golden/reference_fixtures.json):
-test/mesh/utest-mesh-lagrangep1-tetra3d.cpp, -hexa3d.cpp, -triag2d.cpp.  Gauss rules are pinned by
+test/mesh/utest-mesh-lagrangep1-tetra3d.cpp, -hexa3d.cpp, -triag2d.cpp, -prism3d.cpp.  Gauss rules are pinned by
 exactness (a plausible wrong abscissa or weight fails the polynomial integrals).
 """
 import numpy as np
 import pytest
 
-from oracle.pyoracle import HEXA3D, TETRA3D, TRIAG2D
+from oracle.pyoracle import HEXA3D, PRISM3D, TETRA3D, TRIAG2D
 
 
 def ulps(a, b):
@@ -45,6 +45,33 @@ def test_hexa_fixture(oracle, fixtures):
             np.linalg.det(oracle.hexa_jacobian(mm, skew)), rel=1e-12)
 
 
+def test_prism_fixture(oracle, fixtures):
+    f = fixtures["prism3d"]
+    m = f["mapped_coords"]
+    unit, skew = np.array(f["unit_nodes"], float), np.array(f["skewed_nodes"], float)
+    assert ulps(oracle.shape_values(PRISM3D, m), f["shape_values"]) < 10
+    assert ulps(oracle.shape_gradients(PRISM3D, m), f["shape_gradients"]) < 3
+    J = oracle.element_jacobian(PRISM3D, m, skew)
+    assert ulps(J, f["jacobian_skewed"]) < 2
+    # compute_jacobian_adjoint = inverse * det (Prism3D.cpp:131-140)
+    assert ulps(np.linalg.inv(J) * np.linalg.det(J), f["adjoint_skewed"]) < 2
+    assert np.linalg.det(oracle.element_jacobian(PRISM3D, m, unit)) == pytest.approx(f["det_unit"], rel=1e-7)
+    assert np.linalg.det(J) == pytest.approx(f["det_skewed"], rel=1e-7)
+    assert oracle.element_volume(PRISM3D, unit) == pytest.approx(f["volume_unit"], rel=1e-6)
+    assert oracle.element_volume(PRISM3D, skew) == pytest.approx(f["volume_skewed"], rel=1e-6)
+    # gauss_integrate of det J at the fixture's mapped point (a constant) with both rules: 0.5 times the
+    # sum of the weights, which is the reference prism's volume 1
+    for order, key in ((1, "integral_det_order1"), (2, "integral_det_order2")):
+        _, w = oracle.gauss_points(PRISM3D, order)
+        assert w.sum() * np.linalg.det(J) == pytest.approx(f[key], rel=1e-8)
+    # the Jacobian is the shape-function gradients applied to the nodes, at random points
+    rng = np.random.default_rng(5)
+    for _ in range(4):
+        mm = np.array([rng.uniform(0, 0.5), rng.uniform(0, 0.5), rng.uniform(-1, 1)])
+        assert np.allclose(oracle.element_jacobian(PRISM3D, mm, skew), oracle.shape_gradients(PRISM3D, mm) @ skew,
+                           rtol=0, atol=1e-15)
+
+
 def test_triag_fixture(oracle, fixtures):
     f = fixtures["triag2d"]
     nodes = np.array(f["nodes"])
@@ -53,7 +80,7 @@ def test_triag_fixture(oracle, fixtures):
         assert oracle.element_volume(TRIAG2D, np.array(nd, float)) == vol
 
 
-@pytest.mark.parametrize("etype", [TRIAG2D, TETRA3D, HEXA3D])
+@pytest.mark.parametrize("etype", [TRIAG2D, TETRA3D, HEXA3D, PRISM3D])
 def test_shape_partition_of_unity(oracle, etype):
     rng = np.random.default_rng(etype)
     dim = 2 if etype == TRIAG2D else 3
@@ -79,6 +106,14 @@ def test_gauss_exactness(oracle):
     assert w.sum() == 8.0
     assert (w * c[:, 0] ** 2 * c[:, 1] ** 2).sum() == pytest.approx(8 / 9, abs=1e-14)
     assert abs((w * c[:, 0] ** 3 * c[:, 2]).sum()) < 1e-15
+    # prism 6 points (triangle edge midpoints x 2-point Gauss in zeta, w 1/6): exact for degree 2 in
+    # (xi, eta) times degree 3 in zeta over the reference prism (volume 1)
+    c, w = oracle.gauss_points(PRISM3D, 2)
+    assert w.sum() == pytest.approx(1.0, abs=1e-15)
+    assert (w * c[:, 0] ** 2 * c[:, 2] ** 2).sum() == pytest.approx(1 / 12 * 2 / 3, abs=1e-15)
+    assert (w * c[:, 0] * c[:, 1]).sum() == pytest.approx(2 / 24, abs=1e-15)
+    assert abs((w * c[:, 1] * c[:, 2] ** 3).sum()) < 1e-15
+    assert np.allclose(oracle.gauss_points(PRISM3D, 1)[0], [[1 / 3, 1 / 3, 0.0]])
     # one-point rules are the centroids (tau evaluation point, SUPG.hpp:145)
     assert np.allclose(oracle.gauss_points(TETRA3D, 1)[0], 0.25)
     assert np.allclose(oracle.gauss_points(HEXA3D, 1)[0], 0.0)

@@ -283,13 +283,14 @@ def _unique_velocity_conditions(case):
 
 
 @pytest.mark.gpu
-@pytest.mark.parametrize("name", ["tets", "hexas"])
+@pytest.mark.parametrize("name", ["tets", "hexas", "prisms"])
 def test_device_3d_follows_oracle(name):
-    """Tetrahedra (order-1 rule and the order-2 mass terms) and hexahedra (order 2): initial matrices,
+    """Tetrahedra (order-1 rule and the order-2 mass terms), hexahedra and prisms (order 2): initial matrices,
     velocity matrix and the three actions within 1e-12, then two steps with the case's velocity
     conditions (later regions win) and pinned pressure within 1e-9."""
     case = (cases.inflow_cube_tets(4, nu=1e-2, dt=1e-2) if name == "tets"
-            else cases.lid_cavity_cube_hexas(4, nu=1e-2, dt=1e-2, random_velocity=True))
+            else cases.lid_cavity_cube_hexas(4, nu=1e-2, dt=1e-2, random_velocity=True) if name == "hexas"
+            else cases.lid_cavity_cube_prisms(4, nu=1e-2, dt=1e-2, random_velocity=True))
     N, d = case.n_nodes, case.dim
     un, uv, pn, pv = _unique_velocity_conditions(case)
     si = SemiImplicit(pyoracle.get(), case.etype, case.coords, case.conn, np.arange(N, dtype=np.uint32), case.nu,
