@@ -17,7 +17,10 @@ Sources:
   test/math/utest-lss-symmetric-dirichlet.cpp:72-208  (2-rank 4-node system, merged to serial)
   test/math/utest-lss-vector.cpp:144-187               (apply y = 2*A*x + y with A == 1)
 Also copies mesh files of the reference (data files, unchanged) into meshes/: the Gambit neutral files
-plugins/UFEM/test/meshes/ring2d-tris.neu (669 nodes, 1204 triangles, boundary sets inner and outer) and
+plugins/UFEM/test/meshes/ring2d-tris.neu (669 nodes, 1204 triangles, boundary sets inner and outer),
+plugins/UFEM/test/meshes/ring2d-quads.neu (560 nodes, 490 quadrilaterals; atest-ufem-heat2d-disk),
+plugins/UFEM/test/meshes/kvs15.neu (16953 nodes, 33538 triangles of the channel around a cylinder; sets
+in, out, symm and wall; atest-ufem-navier-stokes-cylinder2d) and
 resources/hextet.neu (hexahedra and tetrahedra; test/mesh/utest-mesh-neu.cpp:123-125), and the Gmsh
 files resources/rectangle-tg-p1.msh (507 nodes, 929 triangles; test/mesh/utest-mesh-gmsh.cpp:96),
 resources/rectangle-qd-p1.msh (56 nodes, 42 quadrilaterals) and resources/sqduct_3000e.msh (3751 nodes,
@@ -29,7 +32,8 @@ import re
 import shutil
 import sys
 
-MESHES = ["plugins/UFEM/test/meshes/ring2d-tris.neu", "resources/hextet.neu", "resources/rectangle-tg-p1.msh",
+MESHES = ["plugins/UFEM/test/meshes/ring2d-tris.neu", "plugins/UFEM/test/meshes/ring2d-quads.neu",
+          "plugins/UFEM/test/meshes/kvs15.neu", "resources/hextet.neu", "resources/rectangle-tg-p1.msh",
           "resources/rectangle-qd-p1.msh", "resources/sqduct_3000e.msh"]
 
 

@@ -71,6 +71,26 @@ def ring_case():
                            bcs=[("inner", {0: 0.0, 1: 0.0}), ("outer", {0: 1.0, 1: 0.0})], pin=("outer", 2))
 
 
+def ring_quads_case():
+    """The reference's annulus of quadrilaterals (atest-ufem-heat2d-disk's mesh): no-slip on the inner
+    circle, u = (1, 0) on the outer one, p pinned at one outer node."""
+    return cases.mesh_case(os.path.join(MESHES, "ring2d-quads.neu"),
+                           bcs=[("inner", {0: 0.0, 1: 0.0}), ("outer", {0: 1.0, 1: 0.0})], pin=("outer", 2))
+
+
+def cylinder_case():
+    """atest-ufem-navier-stokes-cylinder2d.py on its mesh kvs15.neu: density 1000, dynamic viscosity 10
+    (nu = 0.01), u = u_in = (2, 0) initially and on the regions in and symm, no-slip on the wall,
+    p = 0 on the outlet, in that order; time step 0.1; the generic expression (use_specializations
+    False)."""
+    case = cases.mesh_case(os.path.join(MESHES, "kvs15.neu"), nu=10.0 / 1000.0, dt=0.1,
+                           bcs=[("in", {0: 2.0, 1: 0.0}), ("symm", {0: 2.0, 1: 0.0}), ("wall", {0: 0.0, 1: 0.0}),
+                                ("out", {2: 0.0})])
+    case.rho = 1000.0
+    case.u0 = np.tile([2.0, 0.0], (case.n_nodes, 1))
+    return case
+
+
 def rectangle_case():
     """The reference's rectangle of triangles (Gmsh file): no-slip on the wall, u = (0, 1) on the inlet,
     p = 0 on the outlet."""
@@ -129,6 +149,8 @@ CASES = {
     "rect": rectangle_case,
     "duct": duct_case,
     "rectq": rectangle_quads_case,
+    "ringq": ring_quads_case,
+    "cyl": cylinder_case,
     "scattered": scattered_tets_case,
     "quad9": quads_case,
     "tet5": lambda: cases.inflow_cube_tets(5),
@@ -293,7 +315,7 @@ def test_gmres_ilu0_lid_cavity(ctx, oracle):
     g.close()
 
 
-@pytest.mark.parametrize("name", ["ring", "rect", "rectq", "duct"])
+@pytest.mark.parametrize("name", ["ring", "rect", "rectq", "duct", "ringq"])
 def test_file_mesh_step_vs_oracle(ctx, oracle, name):
     """Meshes read from the reference's Gambit neutral and Gmsh files: one implicit step (FP64 assembly,
     BCs, GMRES(30)+ILU0 to 1e-10) on the device matches the oracle's system and solution."""
@@ -314,6 +336,38 @@ def test_file_mesh_step_vs_oracle(ctx, oracle, name):
     xo, _ = oracle.gmres_ilu0(neq, start, cols, vref, rref, sol, restart=30, tol=1e-12)
     diff = np.abs(x - xo).max() / np.abs(xo).max()
     print(f"{name}: {case.n_elems} cells, GMRES {st.iterations} it, solution difference {diff:.2e}")
+    g.close()
+
+
+def test_cylinder_steps_vs_oracle(ctx, oracle):
+    """The reference's cylinder run (atest-ufem-navier-stokes-cylinder2d.py, 33538 triangles, generic
+    expression): two implicit steps on the device against the oracle's steps (the same assembly,
+    conditions and updates; GMRES(30)+ILU0 to 1e-12 on both sides: 841 and 1748 iterations in the
+    oracle)."""
+    case = CASES["cyl"]()
+    print("case cylinder", case.n_elems, "cells", flush=True)
+    neq = case.dim + 1
+    g = gpu_case(ctx, case, spec=False)
+    start, cols, slots = oracle.build_sparsity(case.conn, case.n_nodes)
+    prm = pyoracle.ns_params(rho=case.rho, dt=case.dt, theta=1.0, use_specializations=False)
+    u, p = case.u0.copy(), case.p0.copy()
+    u1, u2, u3 = u.copy(), u.copy(), u.copy()
+    nu = np.full(case.n_nodes, case.nu)
+    for step in range(2):
+        st = g.step(cabi.solver_params(cabi.GMRES_ILU0, tol=1e-12, restart=30, max_iter=5000))
+        assert st.converged
+        ua = oracle.linearize_u(u, u1, u2, u3)
+        vals, rhs = oracle.ns_assemble(pyoracle.TRIAG2D, prm, case.coords, case.conn, u, p, ua, nu, start, cols, slots)
+        n, e, v = cases.bc_lists(case, u, p)
+        sol = np.zeros_like(rhs)
+        oracle.dirichlet(neq, start, cols, vals, rhs, sol, n, e, v)
+        x, sto = oracle.gmres_ilu0(neq, start, cols, vals, rhs, sol, restart=30, tol=1e-12)
+        oracle.update(u, p, u1, u2, u3, x)
+        du = np.abs(g.u - u).max() / np.abs(u).max()
+        dp = np.abs(g.p - p).max() / max(np.abs(p).max(), 1e-300)
+        print(f"step {step + 1}: GMRES {st.iterations} it (oracle {sto.iterations}), du {du:.2e} dp {dp:.2e}",
+              flush=True)
+        assert du < 1e-8 and dp < 1e-8
     g.close()
 
 

@@ -32,6 +32,46 @@ def test_ring2d_triangles():
         assert np.allclose(np.hypot(c[nodes, 0], c[nodes, 1]), r, atol=1e-9)
 
 
+def test_ring2d_quadrilaterals():
+    """plugins/UFEM/test/meshes/ring2d-quads.neu (atest-ufem-heat2d-disk): the annulus 1 <= r <= 2 in 490
+    quadrilaterals between two 70-gons."""
+    m = meshio.read_neu(os.path.join(MESHES, "ring2d-quads.neu"))
+    assert (m.name, m.dim, m.n_nodes) == ("ring2d-quads", 2, 560)
+    q = m.blocks["Quad2D"]
+    assert q.shape == (490, 4) and list(m.blocks) == ["Quad2D"]
+    c = m.coords
+    area = 0.5 * ((c[q[:, 2], 0] - c[q[:, 0], 0]) * (c[q[:, 3], 1] - c[q[:, 1], 1]) -
+                  (c[q[:, 2], 1] - c[q[:, 0], 1]) * (c[q[:, 3], 0] - c[q[:, 1], 0]))
+    assert area.min() > 0
+    exact = 0.5 * 70 * (4 - 1) * np.sin(2 * np.pi / 70)
+    assert abs(area.sum() - exact) < 1e-6 * exact             # the file's coordinates carry ~8 digits
+    for name, r in (("inner", 1.0), ("outer", 2.0)):
+        assert m.boundaries[name]["Line2D"].shape == (70, 2)
+        nodes = m.boundary_nodes(name)
+        assert np.allclose(np.hypot(c[nodes, 0], c[nodes, 1]), r, atol=1e-9)
+
+
+def test_kvs15_cylinder_channel():
+    """plugins/UFEM/test/meshes/kvs15.neu (atest-ufem-navier-stokes-cylinder2d): the box [-25.5, 34.5] x
+    [-30, 30] around a cylinder of radius 0.5 (a 200-gon) in 33538 triangles."""
+    m = meshio.read_neu(os.path.join(MESHES, "kvs15.neu"))
+    assert (m.dim, m.n_nodes) == (2, 16953)
+    t = m.blocks["Triag2D"]
+    assert t.shape == (33538, 3)
+    c = m.coords
+    area = 0.5 * ((c[t[:, 1], 0] - c[t[:, 0], 0]) * (c[t[:, 2], 1] - c[t[:, 0], 1]) -
+                  (c[t[:, 2], 0] - c[t[:, 0], 0]) * (c[t[:, 1], 1] - c[t[:, 0], 1]))
+    assert area.min() > 0
+    exact = 60.0 * 60.0 - 0.5 * 200 * 0.25 * np.sin(2 * np.pi / 200)
+    assert abs(area.sum() - exact) < 1e-9 * exact
+    assert {k: m.boundaries[k]["Line2D"].shape[0] for k in m.boundaries} == {"in": 30, "out": 74, "symm": 64,
+                                                                             "wall": 200}
+    assert np.all(c[m.boundary_nodes("in"), 0] == -25.5) and np.all(c[m.boundary_nodes("out"), 0] == 34.5)
+    assert np.all(np.abs(c[m.boundary_nodes("symm"), 1]) == 30.0)
+    w = m.boundary_nodes("wall")
+    assert np.allclose(np.hypot(c[w, 0], c[w, 1]), 0.5, atol=1e-9)
+
+
 def test_hextet_mixed_cells():
     """resources/hextet.neu: hexahedra (node order mapped to cf3's) and tetrahedra, three boundary sets
     with quadrilateral and triangular sides."""
