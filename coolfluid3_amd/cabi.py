@@ -53,8 +53,9 @@ class NSParams(C.Structure):
 
 
 def ns_params(rho=1.0, dt=0.01, theta=1.0, use_specializations=True, supg_type=SUPG_METRIC, precision=FP64,
-              u_ref=1.0):
-    return NSParams(rho, dt, theta, int(use_specializations), supg_type, 1.0, 16.0, 1.0, 1.0, 1.0, precision, u_ref)
+              u_ref=1.0, alpha_ps=1.0, alpha_su=1.0, alpha_bu=1.0):
+    return NSParams(rho, dt, theta, int(use_specializations), supg_type, 1.0, 16.0, alpha_ps, alpha_su, alpha_bu,
+                    precision, u_ref)
 
 
 def solver_params(method=BICGSTAB_JACOBI, tol=1e-10, max_iter=5000, restart=30, precision=FP64, inner_tol=1e-2,

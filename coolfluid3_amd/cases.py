@@ -284,6 +284,47 @@ def taylor_green_case(segments=32, dt=0.004, theta=0.5, element="quad"):
     return case
 
 
+def si_laminar_channel_case(refinement=1):
+    """The channel of plugins/UFEM/test/atest-ufem-navier-stokes-semi-implicit-laminar-channel-2d.py: two
+    BlockArrays blocks [0, 10] x [0, 1] and [0, 10] x [1, 2] of (20 r) x (16 r) quadrilaterals (:35-58),
+    linked periodically right -> left (:77-82), density and dynamic viscosity 1 (:84-85), time step 0.5
+    (:87), velocity 0 and body force (2, 0) initially (:107-117), no-slip on the bottom and top and the
+    pressure 10 at the point region (5, 1) (:119-123).  The flow it settles to is the Poiseuille profile
+    u = y (2 - y), v = 0, which the script checks at every node (:150-156).  The velocity conditions are
+    given once per LSS row: a periodic partner of a wall node carries the same zero value."""
+    from . import blockmesh
+    blocks = blockmesh.BlockArrays()
+    blocks.create_points(2, 6)[:] = [[0.0, 0.0], [10.0, 0.0], [0.0, 1.0], [10.0, 1.0], [0.0, 2.0], [10.0, 2.0]]
+    bl = blocks.create_blocks(2)
+    bl[0], bl[1] = [0, 1, 3, 2], [2, 3, 5, 4]
+    sd = blocks.create_block_subdivisions()
+    sd[0], sd[1] = [20 * refinement, 16 * refinement], [20 * refinement, 16 * refinement]
+    blocks.create_block_gradings()
+    lp = blocks.create_patch_nb_faces("left", 2)
+    lp[0], lp[1] = [2, 0], [4, 2]
+    blocks.create_patch_nb_faces("bottom", 1)[0] = [0, 1]
+    blocks.create_patch_nb_faces("top", 1)[0] = [5, 4]
+    rp = blocks.create_patch_nb_faces("right", 2)
+    rp[0], rp[1] = [1, 3], [3, 5]
+    m = blocks.create_mesh()
+    coords, conn = m.coords, m.blocks["Quad2D"]
+    x, y = coords[:, 0], coords[:, 1]
+    tol = 1e-12
+    right, left = np.flatnonzero(np.abs(x - 10.0) < tol), np.flatnonzero(np.abs(x) < tol)
+    links = periodic_links(coords, [(right, left, (-10.0, 0.0))])
+    p2m = periodic_p2m(*links)
+    walls = np.flatnonzero((np.abs(y) < tol) | (np.abs(y - 2.0) < tol))
+    _, first = np.unique(p2m[walls], return_index=True)
+    walls = np.sort(walls[first]).astype(np.uint32)
+    center = np.flatnonzero((np.abs(x - 5.0) < tol) & (np.abs(y - 1.0) < tol)).astype(np.uint32)
+    N = coords.shape[0]
+    case = Case("si_laminar_channel_%d" % refinement, cabi.QUAD2D, coords, conn, None, 1.0, 0.5, theta=0.5,
+                u0=np.zeros((N, 2)), p0=np.zeros(N), periodic=links)
+    case.bc_regions = [("walls", walls, {0: 0.0, 1: 0.0}), ("center", center, {2: 10.0})]
+    case.body_force = np.tile([2.0, 0.0], (N, 1))
+    return case
+
+
 def mesh_case(path, etype_name=None, nu=1e-2, dt=1e-2, seed=DEFAULT_SEED, bcs=(), pin=None):
     """A case on a mesh file (meshio.read_mesh: .neu or .msh) with one element type: velocity U[-1,1] per
     node from the seed, p = 0.  bcs: [(boundary set, {eq: value or callable(coords)})], applied in that

@@ -144,6 +144,47 @@ def test_device_body_force_urhs_vs_oracle(name):
 
 
 @pytest.mark.gpu
+def test_semi_implicit_laminar_channel_acceptance():
+    """atest-ufem-navier-stokes-semi-implicit-laminar-channel-2d.py: both options on (a body force drives
+    the periodic channel, the pressure solves by the two RCG strategies), theta 0.5, two inner passes,
+    alpha_su 0, 50 steps of 0.5; the script's check: |u - y (2 - y)| <= 1e-2 and |v| <= 1e-3 at every
+    node.  Velocity solves to 1e-6 (the script's Block CG tolerance), pressure solves to 1e-8."""
+    case = cases.si_laminar_channel_case()
+    ctx = cabi.Context(0)
+    mesh = cabi.Mesh(ctx, case.etype, case.coords, case.conn)
+    us = cabi.System(ctx, mesh, neq=2, periodic=case.periodic)
+    ps = cabi.System(ctx, mesh, neq=1, periodic=case.periodic)
+    f = cabi.Fields(mesh)
+    f.set(cabi.FIELD_U, case.u0)
+    f.set(cabi.FIELD_P, case.p0)
+    f.set(cabi.FIELD_G, case.body_force)
+    f.initialize(case.nu)
+    prm = cabi.ns_params(dt=case.dt, theta=case.theta, alpha_su=0.0)
+    dsi = cabi.SemiImplicit(us, ps)
+    try:
+        dsi.set_options(enable_body_force=True, pressure_rcg_solve=True)
+        dsi.initial(f, prm)
+        walls, center = case.bc_regions[0][1], case.bc_regions[1][1]
+        usp = cabi.solver_params(cabi.BICGSTAB_JACOBI, tol=1e-6, max_iter=3000)
+        psp = cabi.solver_params(cabi.RCG_JACOBI, tol=1e-8, max_iter=3000, restart=40, recycle=16)
+        its = []
+        for step in range(50):
+            stats = dsi.step(f, prm, usp, psp, ubc=(walls, np.zeros((len(walls), 2))), pbc=(center, [10.0]))
+            its.append([s.iterations for s in stats])
+        u = f.get(cabi.FIELD_U)
+        y = case.coords[:, 1]
+        eu = np.abs(u[:, 0] - y * (2.0 - y)).max()
+        ev = np.abs(u[:, 1]).max()
+        print("pressure iterations (first, second pass): steps 1-3", its[:3], "last", its[-1], flush=True)
+        print(f"max |u - y(2-y)| {eu:.3e}, max |v| {ev:.3e}", flush=True)
+        assert eu <= 1e-2 and ev <= 1e-3
+    finally:
+        dsi.close()
+        for x in (f, ps, us, mesh, ctx):
+            x.close()
+
+
+@pytest.mark.gpu
 def test_device_steps_with_body_force_and_rcg_pressure_follow_oracle():
     """Taylor-Green quadrilaterals with a body force, both options on: four device steps (velocity by
     BiCGStab to 1e-13, pressure by the two RCG strategies to 1e-13) follow the oracle's direct solves
