@@ -82,15 +82,6 @@ __device__ __forceinline__ double tet_volume_ref(const double X[4][3])
           x0*y1*z3 - x2*y1*z3 - x0*y2*z3 + x1*y2*z3) / 6.;
 }
 
-template<int B, int K, typename F>
-__device__ __forceinline__ void static_for(F&& f)
-{
-  if constexpr(B < K) {
-    f(std::integral_constant<int, B>{});
-    static_for<B + 1, K>(f);
-  }
-}
-
 // one LDS row of a staged element block: 16-byte aligned, odd number of 16-byte chunks (bank spread)
 template<typename VR, int NN>
 constexpr int lds_row()

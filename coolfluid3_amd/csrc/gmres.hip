@@ -60,6 +60,7 @@ __global__ __launch_bounds__(kWG) void k_ilu0_factor(const uint32_t* __restrict_
                                                     const uint32_t* __restrict__ lorder, const uint32_t* __restrict__ lptr,
                                                     uint32_t nlev)
 {
+#pragma clang fp contract(off)   // the oracle's arithmetic: l_ik = a_ik / u_kk, then a_ij - (l_ik u_kj)
   __shared__ double sv[kGroups][kSeg];
   __shared__ uint32_t sc[kGroups][kSeg];
   const uint32_t g = threadIdx.x / kGL, lane = threadIdx.x % kGL;
@@ -143,8 +144,8 @@ __global__ __launch_bounds__(kWG) void k_ilu0_solve(const uint32_t* __restrict__
 }
 
 // ---- the whole solve in one workgroup (systems of at most kSmallMaxN dofs) ----
-constexpr int kSW = 1024;                       // threads of the single workgroup
-constexpr uint32_t kSmallMaxN = 6144;           // dofs of one LDS vector (48 KB)
+constexpr int kSW = 512;                        // threads of the single workgroup (256 VGPRs per lane)
+constexpr uint32_t kSmallMaxN = 4096;           // dofs of one LDS vector (32 KB)
 constexpr int kSR = (int)(kSmallMaxN / kSW);    // rows per thread (row i = t + kSW r)
 constexpr int kSWaves = kSW / 64;
 using arnoldi::kMaxV;                           // basis vectors (restart length <= 32)
@@ -157,6 +158,13 @@ struct SmallArgs {
   const double *aval, *fval, *b;
   double *x, *V;
   double* out;   // [0] iterations, [1] relative true residual at exit
+  uint32_t nsteps, nlow;   // node-level sweep steps (the first nlow are the lower sweep)
+  const uint4* smeta;
+  const uint32_t* sint;
+  const double* sval;
+  const uint2* emeta;      // ELL slices of the point matrix
+  const uint32_t* ecol;
+  const double* eval;
 };
 
 // sum of v over the workgroup, the same value in every thread (waves in a fixed order)
@@ -194,44 +202,206 @@ __device__ inline void wg_mdot(int nv, uint32_t n, const double* V, const double
   __syncthreads();
 }
 
-// zs := U^-1 L^-1 zs in place (the factors of A in fval), level by level, a row group per row
-__device__ inline void ilu_solve_lds(const SmallArgs& a, double* zs)
+// ---- triangular sweeps by node levels ----
+// A node's level is one more than the largest level of the nodes it couples to below it (lower sweep)
+// or above it (upper sweep); the point rows of one node are solved together, in row order.  Step s of
+// the sweeps (the lower levels, then the upper ones) holds R nodes and the record (R, W, ib, vb); slot k
+// (k < R) holds a node and the rows a of that node, each in one contiguous record:
+//   sint[ib + k (2 + W) + ...]   the node, its number c_k of off-node blocks (W = max c_k), then the node
+//                                columns of those blocks, ascending (the node itself past c_k)
+//   sval[vb + (k neq + a)(W + 1) neq + ...]   row a: entry (a, b) of the j-th off-node block at j neq + b
+//                                (zeros past c_k), then entry (a, b) of the node's own diagonal block at
+//                                W neq + b
+// so that a lane finds every entry of its row at fixed offsets from one address, and every address of
+// a step is known before the step before it ends: each lane loads its next rows while waiting at the
+// barriers.  A level of more than kSlots nodes takes several steps.
+// Four lanes per node (lane a takes point row a; kSlots nodes per step).  Every row is reduced in the
+// oracle's order and without contraction: s = b (lower) or x (upper); s -= l_q x_q (u_q x_q) over the
+// columns of the other nodes in ascending order, then over those of the node's own rows; upper:
+// x = s / u_ii.  The sweeps give the oracle's results bit for bit (up to the sign of a zero).
+constexpr uint32_t kSlots = kSW / 4;
+constexpr int kAhead = 2;                // steps whose rows are in flight
+constexpr uint32_t kMaxSteps = 4096;     // sweep steps of the single-workgroup solver (records in LDS)
+
+template<int NEQ>
+struct SlotRow {
+  static constexpr int KB = NEQ == 1 ? 8 : NEQ == 2 ? 6 : 4;   // off-node blocks held in registers
+  uint32_t node, cnt;
+  uint32_t col[KB];
+  double v[KB][NEQ];
+  double d[NEQ];
+};
+
+// row a of slot k of step M (nothing when the slot or lane is idle)
+template<int NEQ>
+__device__ inline void slot_load(const SmallArgs& g, const uint4 M, uint32_t k, uint32_t a, SlotRow<NEQ>& P)
 {
-  const uint32_t g = threadIdx.x / kGL, lane = threadIdx.x % kGL;
-  constexpr uint32_t kG = kSW / kGL;
-  for(uint32_t l = 0; l < a.nlev; ++l) {
-    for(uint32_t k = a.lptr[l] + g; k < a.lptr[l + 1]; k += kG) {
-      const uint32_t i = a.lorder[k];
-      double s = 0.;
-      for(uint32_t q = a.prow[i] + lane; q < a.diag[i]; q += kGL) s += a.fval[q] * zs[a.pcol[q]];
-      s = group_sum(s);
-      if(lane == 0) zs[i] -= s;
+  constexpr int KB = SlotRow<NEQ>::KB;
+  const uint32_t R = M.x, W = M.y;
+  if(k >= R || a >= (uint32_t)NEQ) return;
+  const uint32_t* ri = g.sint + M.z + k * (2 + W);
+  const double* rv = g.sval + M.w + (k * NEQ + a) * ((W + 1) * NEQ);
+  P.node = ri[0];
+  P.cnt = ri[1];
+#pragma unroll
+  for(int j = 0; j < KB; ++j)
+    if((uint32_t)j < W) {
+      P.col[j] = ri[2 + j];
+#pragma unroll
+      for(int b = 0; b < NEQ; ++b) P.v[j][b] = rv[j * NEQ + b];
+    } else {   // past the step's width: a valid column and a zero entry
+      P.col[j] = P.node;
+#pragma unroll
+      for(int b = 0; b < NEQ; ++b) P.v[j][b] = 0.;
     }
-    __syncthreads();
-  }
-  for(uint32_t l = 0; l < a.nulev; ++l) {
-    for(uint32_t k = a.uptr[l] + g; k < a.uptr[l + 1]; k += kG) {
-      const uint32_t i = a.uorder[k];
-      const uint32_t d = a.diag[i];
-      double s = 0.;
-      for(uint32_t q = d + 1 + lane; q < a.prow[i + 1]; q += kGL) s += a.fval[q] * zs[a.pcol[q]];
-      s = group_sum(s);
-      if(lane == 0) zs[i] = (zs[i] - s) / a.fval[d];
-    }
-    __syncthreads();
+#pragma unroll
+  for(int b = 0; b < NEQ; ++b) P.d[b] = rv[W * NEQ + b];
+}
+
+// the off-node products l_q x_q of row a, in place of the entries (the mul of each term of the oracle's
+// s = s - l_q x_q, which does not depend on s)
+// every register block, without branches (the entries past cnt are zeros at valid columns), so that
+// the LDS reads issue together
+template<int NEQ>
+__device__ inline void off_products(SlotRow<NEQ>& P, const double* zs)
+{
+#pragma clang fp contract(off)
+  constexpr int KB = SlotRow<NEQ>::KB;
+#pragma unroll
+  for(int j = 0; j < KB; ++j) {
+#pragma unroll
+    for(int b = 0; b < NEQ; ++b) P.v[j][b] = P.v[j][b] * zs[P.col[j] * NEQ + b];
   }
 }
 
-// wr = A zs for the thread's rows
-__device__ inline void spmv_lds(const SmallArgs& a, const double* zs, double wr[kSR])
+// s -= (off-node terms of row a), ascending: the products in P, then the blocks past the registers.
+// The register blocks past c_k hold zero products: s - 0 is s (up to the sign of a zero s).
+template<int NEQ>
+__device__ inline double off_subtract(const SmallArgs& g, const uint4 M, uint32_t k, uint32_t a,
+                                      const SlotRow<NEQ>& P, const double* zs, double s)
 {
+#pragma clang fp contract(off)
+  constexpr int KB = SlotRow<NEQ>::KB;
 #pragma unroll
-  for(int r = 0; r < kSR; ++r) {
-    const uint32_t i = threadIdx.x + kSW * r;
-    double s = 0.;
-    if(i < a.n)
-      for(uint32_t q = a.prow[i]; q < a.prow[i + 1]; ++q) s += a.aval[q] * zs[a.pcol[q]];
-    wr[r] = s;
+  for(int j = 0; j < KB; ++j) {
+#pragma unroll
+    for(int b = 0; b < NEQ; ++b) s = s - P.v[j][b];
+  }
+  if(P.cnt > (uint32_t)KB) {
+    const uint32_t W = M.y;
+    const uint32_t* ri = g.sint + M.z + k * (2 + W);
+    const double* rv = g.sval + M.w + (k * NEQ + a) * ((W + 1) * NEQ);
+    for(uint32_t j = KB; j < P.cnt; ++j) {
+      const uint32_t c = ri[2 + j];
+#pragma unroll
+      for(int b = 0; b < NEQ; ++b) s = s - rv[j * NEQ + b] * zs[c * NEQ + b];
+    }
+  }
+  return s;
+}
+
+// v of lane P of each group of four lanes (the lanes of one node), in all four (DPP quad_perm)
+template<int P>
+__device__ inline double quad_bcast(double v)
+{
+  constexpr int ctrl = P * 0x55;   // quad_perm [P, P, P, P]
+  const uint64_t u = __builtin_bit_cast(uint64_t, v);
+  const uint32_t lo = (uint32_t)__builtin_amdgcn_update_dpp(0, (int)(uint32_t)u, ctrl, 0xf, 0xf, false);
+  const uint32_t hi = (uint32_t)__builtin_amdgcn_update_dpp(0, (int)(uint32_t)(u >> 32), ctrl, 0xf, 0xf, false);
+  return __builtin_bit_cast(double, ((uint64_t)hi << 32) | lo);
+}
+
+// the rows of slot k of step M (row a in this lane); every lane of the wave calls it (the broadcasts
+// between the rows of a node run in all lanes)
+template<int NEQ>
+__device__ inline void slot_rows(const SmallArgs& g, const uint4 M, bool upper, uint32_t k, uint32_t a,
+                                 SlotRow<NEQ>& P, double* zs)
+{
+#pragma clang fp contract(off)
+  const bool act = k < M.x && a < (uint32_t)NEQ;
+  const uint32_t r0 = act ? P.node * NEQ : 0;
+  double s = 0.;
+  if(act) {
+    off_products<NEQ>(P, zs);
+    s = zs[r0 + a];
+  }
+  if(!upper) {
+    // s = b - off-node terms, then the node's earlier rows in row order: after round p, row p is final
+    if(act) s = off_subtract<NEQ>(g, M, k, a, P, zs, s);
+    static_for<0, NEQ - 1>([&](auto pc) {
+      constexpr int p = decltype(pc)::value;
+      const double xp = quad_bcast<p>(s);
+      if(a > (uint32_t)p) s = s - P.d[p] * xp;
+    });
+    if(act) zs[r0 + a] = s;
+  } else {
+    // s = y - off-node terms, then from the last row: s -= the node's later rows (ascending), x = s / u_aa
+    if(act) s = off_subtract<NEQ>(g, M, k, a, P, zs, s);
+    double xr[NEQ];
+    double x = 0.;
+    static_for<0, NEQ>([&](auto pc) {
+      constexpr int p = NEQ - 1 - decltype(pc)::value;
+      if(a == (uint32_t)p) {
+#pragma unroll
+        for(int b = p + 1; b < NEQ; ++b) s = s - P.d[b] * xr[b];
+        x = s / P.d[p];
+      }
+      xr[p] = quad_bcast<p>(x);
+    });
+    if(act) zs[r0 + a] = x;
+  }
+}
+
+// zs := U^-1 L^-1 zs in place (the factors of A), one barrier per node level.  ml: the step records in
+// LDS, followed by kAhead zero records.  The rows of the next kAhead steps are in flight in a ring of
+// registers.
+template<int NEQ>
+__device__ inline void ilu_solve_lds(const SmallArgs& g, const uint4* ml, double* zs)
+{
+  const uint32_t k0 = threadIdx.x >> 2, a = threadIdx.x & 3;
+  SlotRow<NEQ> P[kAhead];
+#pragma unroll
+  for(int i = 0; i < kAhead; ++i) slot_load<NEQ>(g, ml[i], k0, a, P[i]);
+  for(uint32_t st = 0; st < g.nsteps; st += kAhead) {
+#pragma unroll
+    for(int i = 0; i < kAhead; ++i)
+      if(st + i < g.nsteps) {
+        const uint4 M = ml[st + i], Mf = ml[st + i + kAhead];
+        slot_rows<NEQ>(g, M, st + i >= g.nlow, k0, a, P[i], zs);
+        slot_load<NEQ>(g, Mf, k0, a, P[i]);
+        __syncthreads();
+      }
+  }
+}
+
+// wr = A zs for the thread's rows: 64-row ELL slices (per slice (base, width); entry q of row i at
+// base + q 64 + i % 64), each row summed in ascending column order as in the point matrix; the rows of
+// a thread advance kSpmvRows at a time, so each entry index has their loads in flight together
+constexpr int kSpmvRows = 4;
+__device__ inline void spmv_lds(const SmallArgs& g, const double* zs, double wr[kSR])
+{
+  static_assert(kSR % kSpmvRows == 0, "rows per thread");
+  const uint32_t lane = threadIdx.x & 63;
+#pragma unroll
+  for(int r0 = 0; r0 < kSR; r0 += kSpmvRows) {
+    uint32_t base[kSpmvRows], len[kSpmvRows], wmax = 0;
+#pragma unroll
+    for(int r = 0; r < kSpmvRows; ++r) {
+      const uint32_t i = threadIdx.x + kSW * (r0 + r);
+      const uint2 E = i < g.n ? g.emeta[i >> 6] : make_uint2(0, 0);
+      base[r] = E.x + lane;
+      len[r] = i < g.n ? g.prow[i + 1] - g.prow[i] : 0;
+      wmax = wmax > E.y ? wmax : E.y;
+      wr[r0 + r] = 0.;
+    }
+    for(uint32_t q = 0; q < wmax; ++q) {
+#pragma unroll
+      for(int r = 0; r < kSpmvRows; ++r)
+        if(q < len[r]) {
+          const uint32_t e = base[r] + q * 64;
+          wr[r0 + r] += g.eval[e] * zs[g.ecol[e]];
+        }
+    }
   }
 }
 
@@ -239,9 +409,11 @@ __device__ inline void spmv_lds(const SmallArgs& a, const double* zs, double wr[
 // rotations (the method of the multi-launch loop below and of the oracle's orc_gmres_ilu0).  A cycle
 // ends when the recurrence residual meets the tolerance or at the restart length; the true residual
 // b - A x at the start of the next cycle decides convergence.
+template<int NEQ>
 __global__ __launch_bounds__(kSW) void k_gmres_small(SmallArgs a)
 {
   __shared__ double zs[kSmallMaxN];
+  __shared__ uint4 ml[kMaxSteps + kAhead];
   __shared__ double red[kSWaves * kMaxV];
   __shared__ double H[kMaxV * (kMaxV - 1)];
   __shared__ double cs[kMaxV], sn[kMaxV], g[kMaxV], hc[kMaxV], y[kMaxV], cc[kMaxV];
@@ -250,6 +422,7 @@ __global__ __launch_bounds__(kSW) void k_gmres_small(SmallArgs a)
   const uint32_t n = a.n;
   const int m = a.m;
   double wr[kSR];
+  for(uint32_t i = t; i < a.nsteps + kAhead; i += kSW) ml[i] = a.smeta[i];   // visible after wg_sum
   double p = 0.;
 #pragma unroll
   for(int r = 0; r < kSR; ++r) {
@@ -299,7 +472,7 @@ __global__ __launch_bounds__(kSW) void k_gmres_small(SmallArgs a)
         if(i < n) zs[i] = a.V[(uint64_t)j * n + i];
       }
       __syncthreads();
-      ilu_solve_lds(a, zs);
+      ilu_solve_lds<NEQ>(a, ml, zs);
       spmv_lds(a, zs, wr);
       // two passes of classical Gram-Schmidt against V_0 .. V_j
       for(int pass = 0; pass < 2; ++pass) {
@@ -362,7 +535,7 @@ __global__ __launch_bounds__(kSW) void k_gmres_small(SmallArgs a)
       }
     }
     __syncthreads();
-    ilu_solve_lds(a, zs);
+    ilu_solve_lds<NEQ>(a, ml, zs);
 #pragma unroll
     for(int r = 0; r < kSR; ++r) {
       const uint32_t i = t + kSW * r;
@@ -374,6 +547,128 @@ __global__ __launch_bounds__(kSW) void k_gmres_small(SmallArgs a)
     a.out[0] = (double)it;
     a.out[1] = rel;
   }
+}
+
+constexpr uint32_t kNoSrc = 0xffffffffu;
+
+// dst[e] = from[src[e]] (0 where src[e] is kNoSrc) over two tables in one launch
+__global__ void k_gather2(uint64_t n1, const uint32_t* __restrict__ src1, const double* __restrict__ from1,
+                          double* __restrict__ dst1, uint64_t n2, const uint32_t* __restrict__ src2,
+                          const double* __restrict__ from2, double* __restrict__ dst2)
+{
+  const uint64_t e = (uint64_t)blockIdx.x * blockDim.x + threadIdx.x;
+  if(e < n1) {
+    const uint32_t q = src1[e];
+    dst1[e] = q == kNoSrc ? 0. : from1[q];
+  } else if(e < n1 + n2) {
+    const uint32_t q = src2[e - n1];
+    dst2[e - n1] = q == kNoSrc ? 0. : from2[q];
+  }
+}
+
+// The single-workgroup solver's tables (layouts at slot_load and spmv_lds), on the host.  rp/cl: the
+// block pattern (columns ascending in each row); prow: point row offsets (point row (i, a) holds, for
+// each block column position s of node row i, the columns cl[s] neq + b at prow + s neq + b).
+void build_slot_plan(Lss& L, const std::vector<uint32_t>& rp, const std::vector<uint32_t>& cl,
+                     const std::vector<uint32_t>& prow)
+{
+  Context& c = *L.ctx;
+  const uint32_t N = L.n_nodes;
+  const uint32_t neq = (uint32_t)L.neq;
+  const uint64_t n = (uint64_t)N * neq;
+  std::vector<uint32_t> dpos(N);   // position of the diagonal block in each node row
+  for(uint32_t i = 0; i < N; ++i)
+    for(uint32_t s = rp[i]; s < rp[i + 1]; ++s)
+      if(cl[s] == i) dpos[i] = s - rp[i];
+  std::vector<uint32_t> blev(N, 0), bulev(N, 0);
+  uint32_t nl = 0, nu = 0;
+  for(uint32_t i = 0; i < N; ++i) {
+    for(uint32_t s = rp[i]; s < rp[i + 1]; ++s)
+      if(cl[s] < i) blev[i] = std::max(blev[i], blev[cl[s]] + 1);
+    nl = std::max(nl, blev[i] + 1);
+  }
+  for(uint32_t i = N; i-- > 0;) {
+    for(uint32_t s = rp[i]; s < rp[i + 1]; ++s)
+      if(cl[s] > i) bulev[i] = std::max(bulev[i], bulev[cl[s]] + 1);
+    nu = std::max(nu, bulev[i] + 1);
+  }
+  std::vector<std::vector<uint32_t>> levels(nl + nu);
+  for(uint32_t i = 0; i < N; ++i) {
+    levels[blev[i]].push_back(i);
+    levels[nl + bulev[i]].push_back(i);
+  }
+  std::vector<std::vector<uint32_t>> steps;   // levels cut into steps of at most kSlots nodes
+  uint32_t nlow = 0;
+  for(uint32_t l = 0; l < nl + nu; ++l) {
+    for(size_t k = 0; k < levels[l].size(); k += kSlots)
+      steps.emplace_back(levels[l].begin() + k, levels[l].begin() + std::min<size_t>(levels[l].size(), k + kSlots));
+    if(l + 1 == nl) nlow = (uint32_t)steps.size();
+  }
+  std::vector<uint32_t> meta, sint, ssrc;
+  for(uint32_t st = 0; st < (uint32_t)steps.size(); ++st) {
+    const bool upper = st >= nlow;
+    const std::vector<uint32_t>& nodes = steps[st];
+    const uint32_t R = (uint32_t)nodes.size();
+    std::vector<std::vector<uint32_t>> pos(R);   // off-node block positions of each slot, ascending
+    uint32_t W = 0;
+    for(uint32_t k = 0; k < R; ++k) {
+      const uint32_t i = nodes[k];
+      for(uint32_t s = 0; s < rp[i + 1] - rp[i]; ++s)
+        if(upper ? cl[rp[i] + s] > i : cl[rp[i] + s] < i) pos[k].push_back(s);
+      W = std::max(W, (uint32_t)pos[k].size());
+    }
+    if(sint.size() > 0xffffffffull - (2ull + W) * R || ssrc.size() > 0xffffffffull - (W + 1ull) * neq * neq * R)
+      CF3_THROW(CF3HIP_ERR_LIMIT, "ILU sweep tables too large");
+    meta.insert(meta.end(), {R, W, (uint32_t)sint.size(), (uint32_t)ssrc.size()});
+    for(uint32_t k = 0; k < R; ++k) {
+      const uint32_t i = nodes[k];
+      sint.push_back(i);
+      sint.push_back((uint32_t)pos[k].size());
+      for(uint32_t j = 0; j < W; ++j) sint.push_back(j < pos[k].size() ? cl[rp[i] + pos[k][j]] : i);
+      for(uint32_t a = 0; a < neq; ++a) {
+        const uint32_t r0 = prow[(uint64_t)i * neq + a];
+        for(uint32_t j = 0; j < W; ++j)
+          for(uint32_t b = 0; b < neq; ++b) ssrc.push_back(j < pos[k].size() ? r0 + pos[k][j] * neq + b : kNoSrc);
+        for(uint32_t b = 0; b < neq; ++b) ssrc.push_back(r0 + dpos[i] * neq + b);
+      }
+    }
+  }
+  meta.insert(meta.end(), 4 * kAhead, 0u);   // zero records read ahead past the last step
+  // 64-row ELL slices of the point matrix
+  const uint32_t nsl = (uint32_t)((n + 63) / 64);
+  std::vector<uint32_t> emeta, ecol, esrc;
+  for(uint32_t sl = 0; sl < nsl; ++sl) {
+    uint32_t W = 0;
+    for(uint64_t r = (uint64_t)sl * 64; r < std::min<uint64_t>(n, (uint64_t)sl * 64 + 64); ++r)
+      W = std::max(W, prow[r + 1] - prow[r]);
+    emeta.insert(emeta.end(), {(uint32_t)ecol.size(), W});
+    for(uint32_t q = 0; q < W; ++q)
+      for(uint32_t l = 0; l < 64; ++l) {
+        const uint64_t r = (uint64_t)sl * 64 + l;
+        const bool in = r < n && q < prow[r + 1] - prow[r];
+        const uint32_t i = (uint32_t)(r / neq);
+        ecol.push_back(in ? cl[rp[i] + q / neq] * neq + q % neq : 0);
+        esrc.push_back(in ? prow[r] + q : kNoSrc);
+      }
+  }
+  auto up = [&](DevBuf& d, const std::vector<uint32_t>& h) {   // at least one uint4 (smeta[0] is read)
+    d.alloc(&c, sizeof(uint32_t) * std::max<size_t>(h.size(), 4));
+    HIP_OK(hipMemcpyAsync(d.ptr, h.data(), sizeof(uint32_t) * h.size(), hipMemcpyHostToDevice, c.stream));
+  };
+  auto& I = L.ilu;
+  I.nsv = ssrc.size();
+  I.nev = esrc.size();
+  up(I.smeta, meta);
+  up(I.sint, sint);
+  up(I.ssrc, ssrc);
+  up(I.emeta, emeta);
+  up(I.ecol, ecol);
+  up(I.esrc, esrc);
+  I.sval.alloc(&c, sizeof(double) * std::max<size_t>(ssrc.size(), 1));
+  I.eval.alloc(&c, sizeof(double) * std::max<size_t>(esrc.size(), 1));
+  HIP_OK(hipStreamSynchronize(c.stream));   // the host vectors are released on return
+  I.nsteps = (uint32_t)steps.size();
+  I.nlow = nlow;
 }
 
 // level schedules of the point pattern (natural order), on the host: a row's lower level is one more
@@ -451,6 +746,7 @@ void build_ilu_plan(Lss& L)
   HIP_OK(hipStreamSynchronize(c.stream));   // the host vectors are released on return
   L.ilu.nlev = lmax + 1;
   L.ilu.nulev = umax + 1;
+  if(n <= kSmallMaxN) build_slot_plan(L, rp, cl, prow);
   L.ilu.ready = true;
 }
 
@@ -476,12 +772,22 @@ void gmres_ilu0(Lss& L, const cf3hip_solver_params& prm, cf3hip_solve_stats& st)
                      I.fval.as<double>(), I.diag.as<uint32_t>(), I.lorder.as<uint32_t>(), I.lptr.as<uint32_t>(), I.nlev);
   HIP_OK(hipGetLastError());
   if(I.V.bytes < sizeof(double) * (m + 1) * n) I.V.alloc(&c, sizeof(double) * (m + 1) * n);
-  if(n <= kSmallMaxN) {
+  if(n <= kSmallMaxN && I.nsteps <= kMaxSteps) {
     SmallArgs a{(uint32_t)n, I.nlev, I.nulev, m, prm.max_iter, prm.tol,
                 I.prow.as<uint32_t>(), I.pcol.as<uint32_t>(), I.diag.as<uint32_t>(), I.lorder.as<uint32_t>(),
                 I.lptr.as<uint32_t>(), I.uorder.as<uint32_t>(), I.uptr.as<uint32_t>(), I.aval.as<double>(),
-                I.fval.as<double>(), L.rhs.as<double>(), L.sol.as<double>(), I.V.as<double>(), I.out.as<double>()};
-    hipLaunchKernelGGL(k_gmres_small, dim3(1), dim3(kSW), 0, c.stream, a);
+                I.fval.as<double>(), L.rhs.as<double>(), L.sol.as<double>(), I.V.as<double>(), I.out.as<double>(),
+                I.nsteps, I.nlow, I.smeta.as<uint4>(), I.sint.as<uint32_t>(), I.sval.as<double>(),
+                I.emeta.as<uint2>(), I.ecol.as<uint32_t>(), I.eval.as<double>()};
+    hipLaunchKernelGGL(k_gather2, dim3(grid_for(I.nsv + I.nev, 256)), dim3(256), 0, c.stream, I.nsv,
+                       I.ssrc.as<uint32_t>(), I.fval.as<double>(), I.sval.as<double>(), I.nev, I.esrc.as<uint32_t>(),
+                       I.aval.as<double>(), I.eval.as<double>());
+    switch(L.neq) {
+      case 1: hipLaunchKernelGGL(k_gmres_small<1>, dim3(1), dim3(kSW), 0, c.stream, a); break;
+      case 2: hipLaunchKernelGGL(k_gmres_small<2>, dim3(1), dim3(kSW), 0, c.stream, a); break;
+      case 3: hipLaunchKernelGGL(k_gmres_small<3>, dim3(1), dim3(kSW), 0, c.stream, a); break;
+      default: hipLaunchKernelGGL(k_gmres_small<4>, dim3(1), dim3(kSW), 0, c.stream, a); break;
+    }
     HIP_OK(hipGetLastError());
     double o[2];
     HIP_OK(hipMemcpyAsync(o, I.out.ptr, sizeof(o), hipMemcpyDeviceToHost, c.stream));

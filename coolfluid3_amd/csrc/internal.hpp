@@ -6,6 +6,7 @@
 #include <cstdint>
 #include <memory>
 #include <string>
+#include <type_traits>
 #include <unordered_map>
 #include <vector>
 
@@ -75,6 +76,16 @@ struct Context {
 
 void timer_begin(Context& c, int kernel);
 void timer_end(Context& c, int kernel);
+
+// f(integral_constant<int, i>) for i = B .. K-1, unrolled at compile time
+template<int B, int K, typename F>
+__device__ __forceinline__ void static_for(F&& f)
+{
+  if constexpr(B < K) {
+    f(std::integral_constant<int, B>{});
+    static_for<B + 1, K>(f);
+  }
+}
 
 constexpr int kMaxRedBlocks = 1024;
 // assembly workgroup: at most kAsmThreads incidences (one per thread) and kAsmThreads CSR blocks
@@ -182,6 +193,14 @@ struct Lss {
     DevBuf aval, fval;
     DevBuf V, w, z, small, out;
     int m_ws = 0;
+    // single-workgroup solver: node-level triangular sweeps in slot-major order (per step the
+    // (rows, width, int base, value base) record, the int and value-source tables) and a 64-row
+    // sliced-ELL copy of the point pattern (per slice (base, width), columns and value sources);
+    // the value tables are gathered from fval / aval at every solve
+    uint32_t nsteps = 0, nlow = 0;
+    uint64_t nsv = 0, nev = 0;   // entries of sval and eval
+    DevBuf smeta, sint, ssrc, sval;
+    DevBuf emeta, ecol, esrc, eval;
   } ilu;
   std::unique_ptr<RcgSpace> rcg[kRcgSpaces];   // RCG recycle spaces (kept between solves)
 };

@@ -901,11 +901,12 @@ Real dot(size_t n, const Real* a, const Real* b)
 
 // ILU(0) on the point matrix (IKJ variant) -- Ifpack ILU level 0 is third-party; restated by
 // specification.  Point CSR with sorted columns.
-struct PointCSR { std::vector<Uint> rp, ci; std::vector<Real> v; std::vector<Uint> diag; };
+struct PointCSR { std::vector<Uint> rp, ci; std::vector<Real> v; std::vector<Uint> diag; int neq = 1; };
 
 void to_point(int neq, Uint N, const Uint* start, const Uint* cols, const Real* values, PointCSR& P)
 {
   const size_t n = (size_t)N*neq;
+  P.neq = neq;
   P.rp.assign(n+1, 0);
   const size_t nnz = (size_t)start[N]*neq*neq;
   P.ci.resize(nnz); P.v.resize(nnz);
@@ -943,9 +944,16 @@ void ilu0_solve(const PointCSR& P, const Real* b, Real* x)
     for(Uint q = P.rp[i]; q != P.diag[i]; ++q) s -= P.v[q] * x[P.ci[q]];
     x[i] = s;
   }
+  // upper rows: the entries of the other nodes' columns first, then those of the row's own node (the
+  // block rows of a node are solved together), each in ascending column order
+  const Uint neq = (Uint)P.neq;
   for(size_t ii = n; ii-- > 0;) {
+    const Uint own = (Uint)(ii / neq) * neq;
     Real s = x[ii];
-    for(Uint q = P.diag[ii]+1; q != P.rp[ii+1]; ++q) s -= P.v[q] * x[P.ci[q]];
+    for(Uint q = P.diag[ii]+1; q != P.rp[ii+1]; ++q)
+      if(P.ci[q] >= own + neq) s -= P.v[q] * x[P.ci[q]];
+    for(Uint q = P.diag[ii]+1; q != P.rp[ii+1]; ++q)
+      if(P.ci[q] < own + neq) s -= P.v[q] * x[P.ci[q]];
     x[ii] = s / P.v[P.diag[ii]];
   }
 }

@@ -82,6 +82,28 @@ def test_bicgstab_fp64_small_neq(ctx, oracle, neq):
     mesh.close()
 
 
+@pytest.mark.parametrize("neq", [1, 2, 3, 4])
+def test_gmres_ilu0_small_neq(ctx, oracle, neq):
+    """GMRES(30)+ILU0 on 1 to 4 equations per node (the single-workgroup solver, one kernel per block
+    size): the solution matches the oracle's GMRES+ILU0 solution to 1e-8, its residual on the oracle's
+    matrix is <= 1e-10, and the iteration counts agree within 10%."""
+    mesh, s, start, cols, vals, b = build_system(ctx, oracle, neq)
+    print("neq", neq, "dofs", s.dofs, flush=True)
+    s.set_vector(cabi.RHS, b)
+    s.reset_vector(cabi.SOLUTION, 0.0)
+    st = s.solve(cabi.solver_params(cabi.GMRES_ILU0, tol=1e-10, restart=30, max_iter=2000))
+    x = s.vector(cabi.SOLUTION)
+    xo, sto = oracle.gmres_ilu0(neq, start, cols, vals, b, np.zeros_like(b), restart=30, tol=1e-10)
+    res = np.linalg.norm(b - oracle.spmv(neq, start, cols, vals, x)) / np.linalg.norm(b)
+    diff = np.abs(x - xo).max() / np.abs(xo).max()
+    print(f"neq {neq}: gpu {st.iterations} it, oracle {sto.iterations} it, residual {res:.2e}, solution diff {diff:.2e}")
+    assert st.converged and res <= 1e-10
+    assert diff <= 1e-8
+    assert abs(st.iterations - sto.iterations) <= max(2, sto.iterations // 10)
+    s.close()
+    mesh.close()
+
+
 def oracle_system(o, case):
     start, cols, slots = o.build_sparsity(case.conn, case.n_nodes)
     prm = pyoracle.ns_params(rho=case.rho, dt=case.dt, theta=case.theta,

@@ -229,6 +229,25 @@ def test_ilu0_exact_on_tridiagonal(oracle):
     assert st.iterations == 1 and st.rel_residual < 1e-13
 
 
+def test_ilu0_exact_on_block_tridiagonal(oracle):
+    """With 3 equations per node and full blocks coupling each node to its neighbours only, ILU(0) drops
+    no fill: it is the exact LU factorisation and GMRES converges in one iteration (the node-block order
+    of the triangular sweeps included)."""
+    N, neq = 12, 3
+    cols = []; start = [0]
+    for i in range(N):
+        cols += [j for j in (i - 1, i, i + 1) if 0 <= j < N]; start.append(len(cols))
+    start = np.array(start, np.uint32); cols = np.array(cols, np.uint32)
+    rng = np.random.default_rng(5)
+    values = rng.uniform(-1, 1, len(cols) * neq * neq)
+    for i in range(N):
+        s = start[i] + (1 if i > 0 else 0)
+        values[s * neq * neq:(s + 1) * neq * neq] += 6.0 * np.eye(neq).ravel()
+    b = rng.uniform(-1, 1, N * neq)
+    x, st = oracle.gmres_ilu0(neq, start, cols, values, b, np.zeros(N * neq), restart=30, tol=1e-12)
+    assert st.iterations == 1 and st.rel_residual < 1e-13
+
+
 def test_node_updates(oracle):
     rng = np.random.default_rng(8)
     u, u1, u2, u3 = (rng.uniform(-1, 1, (7, 3)) for _ in range(4))
