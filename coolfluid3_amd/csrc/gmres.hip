@@ -55,14 +55,27 @@ __device__ inline uint32_t find_lds(const uint32_t* c, uint32_t lo, uint32_t hi,
   return lo;
 }
 
+// The update lists of the elimination (pattern only, built with the level schedules): lower entry q of
+// row i (column k) takes the pivot u_kk at fpiv[q] and updates a_ij -= l_ik u_kj for the pairs p in
+// [fptr[q], fptr[q+1]): u_kj at fsrc[p], a_ij at offset fdst[p] of row i.  A row whose lists fit
+// kPairs is staged whole (its entries, pivots and the u_kj) by one batch of loads, then eliminated in
+// LDS; other rows search row k's columns.
+constexpr uint32_t kPairs = 128;
+struct FactorPlan {
+  const uint32_t *fptr, *fpiv, *fsrc;
+  const uint16_t* fdst;
+};
+
 __global__ __launch_bounds__(kWG) void k_ilu0_factor(const uint32_t* __restrict__ rp, const uint32_t* __restrict__ col,
                                                     double* __restrict__ val, const uint32_t* __restrict__ diag,
                                                     const uint32_t* __restrict__ lorder, const uint32_t* __restrict__ lptr,
-                                                    uint32_t nlev)
+                                                    uint32_t nlev, FactorPlan fp)
 {
 #pragma clang fp contract(off)   // the oracle's arithmetic: l_ik = a_ik / u_kk, then a_ij - (l_ik u_kj)
   __shared__ double sv[kGroups][kSeg];
   __shared__ uint32_t sc[kGroups][kSeg];
+  __shared__ double spv[kGroups][kPairs];   // staged rows: pivots (first nlow), then the u_kj
+  __shared__ uint16_t spd[kGroups][kPairs];
   const uint32_t g = threadIdx.x / kGL, lane = threadIdx.x % kGL;
   double* rv = sv[g];
   uint32_t* rc = sc[g];
@@ -71,7 +84,29 @@ __global__ __launch_bounds__(kWG) void k_ilu0_factor(const uint32_t* __restrict_
       const uint32_t i = lorder[t];
       const uint32_t q0 = rp[i], q1 = rp[i + 1], d = diag[i];
       const uint32_t len = q1 - q0, nlow = d - q0;
-      if(len <= kSeg) {
+      const uint32_t pb = fp.fptr[q0], np = fp.fptr[d] - pb;
+      if(len <= kSeg && nlow + np <= kPairs) {
+        double* pv = spv[g];
+        uint16_t* pd = spd[g];
+        uint32_t* pp = rc;   // pair offsets of each lower entry (nlow + 1 <= kSeg)
+        for(uint32_t p = lane; p < len; p += kGL) rv[p] = val[q0 + p];
+        for(uint32_t p = lane; p < nlow; p += kGL) pv[p] = val[fp.fpiv[q0 + p]];
+        for(uint32_t p = lane; p <= nlow; p += kGL) pp[p] = fp.fptr[q0 + p] - pb;
+        for(uint32_t p = lane; p < np; p += kGL) {
+          pv[nlow + p] = val[fp.fsrc[pb + p]];
+          pd[p] = fp.fdst[pb + p];
+        }
+        group_sync();
+        for(uint32_t qq = 0; qq < nlow; ++qq) {
+          const double lik = rv[qq] / pv[qq];
+          group_sync();
+          if(lane == 0) rv[qq] = lik;
+          for(uint32_t p = pp[qq] + lane; p < pp[qq + 1]; p += kGL) rv[pd[p]] -= lik * pv[nlow + p];
+          group_sync();
+        }
+        for(uint32_t p = lane; p < len; p += kGL) val[q0 + p] = rv[p];
+        group_sync();   // rc (pp) and the staging are reused by the group's next row
+      } else if(len <= kSeg) {
         for(uint32_t p = lane; p < len; p += kGL) { rv[p] = val[q0 + p]; rc[p] = col[q0 + p]; }
         group_sync();
         for(uint32_t qq = 0; qq < nlow; ++qq) {
@@ -87,6 +122,7 @@ __global__ __launch_bounds__(kWG) void k_ilu0_factor(const uint32_t* __restrict_
           group_sync();
         }
         for(uint32_t p = lane; p < len; p += kGL) val[q0 + p] = rv[p];
+        group_sync();
       } else if(lane == 0) {
         for(uint32_t q = q0; q < d; ++q) {
           const uint32_t k = col[q];
@@ -394,6 +430,7 @@ __device__ inline void spmv_lds(const SmallArgs& g, const double* zs, double wr[
       wmax = wmax > E.y ? wmax : E.y;
       wr[r0 + r] = 0.;
     }
+#pragma unroll 4
     for(uint32_t q = 0; q < wmax; ++q) {
 #pragma unroll
       for(int r = 0; r < kSpmvRows; ++r)
@@ -729,11 +766,45 @@ void build_ilu_plan(Lss& L)
   std::vector<uint32_t> lorder, lptr, uorder, uptr;
   order(lev, lmax + 1, lorder, lptr);
   order(ulev, umax + 1, uorder, uptr);
+  // update lists of the elimination (at k_ilu0_factor)
+  const uint64_t nnz = prow[n];
+  std::vector<uint32_t> fptr(nnz + 1, 0), fpiv(nnz, 0), fsrc;
+  std::vector<uint16_t> fdst;
+  std::vector<int64_t> at(n, -1);   // column -> offset in the current row
+  for(uint64_t r = 0; r < n; ++r) {
+    const uint32_t i = (uint32_t)(r / neq);
+    const uint32_t len = prow[r + 1] - prow[r];
+    if(len > 0xffffu) CF3_THROW(CF3HIP_ERR_LIMIT, "ILU row too long");
+    for(uint32_t o = 0; o < len; ++o) at[col_of(i, o)] = o;
+    for(uint32_t q = prow[r]; q < prow[r + 1]; ++q) {
+      fptr[q] = (uint32_t)fsrc.size();
+      if(q < diag[r]) {
+        const uint32_t k = col_of(i, q - prow[r]);
+        fpiv[q] = diag[k];
+        const uint32_t ki = k / neq;
+        for(uint32_t rr = diag[k] + 1; rr < prow[k + 1]; ++rr) {
+          const int64_t o = at[col_of(ki, rr - prow[k])];
+          if(o >= 0) {
+            fsrc.push_back(rr);
+            fdst.push_back((uint16_t)o);
+          }
+        }
+        if(fsrc.size() >= 0xffffffffull) CF3_THROW(CF3HIP_ERR_LIMIT, "ILU update lists too large");
+      }
+    }
+    for(uint32_t o = 0; o < len; ++o) at[col_of(i, o)] = -1;
+  }
+  fptr[nnz] = (uint32_t)fsrc.size();
   auto up = [&](DevBuf& d, const std::vector<uint32_t>& h) {
     d.alloc(&c, sizeof(uint32_t) * h.size());
     HIP_OK(hipMemcpyAsync(d.ptr, h.data(), sizeof(uint32_t) * h.size(), hipMemcpyHostToDevice, c.stream));
   };
   up(L.ilu.diag, diag);
+  up(L.ilu.fptr, fptr);
+  up(L.ilu.fpiv, fpiv);
+  up(L.ilu.fsrc, fsrc);
+  L.ilu.fdst.alloc(&c, sizeof(uint16_t) * std::max<size_t>(fdst.size(), 1));
+  HIP_OK(hipMemcpyAsync(L.ilu.fdst.ptr, fdst.data(), sizeof(uint16_t) * fdst.size(), hipMemcpyHostToDevice, c.stream));
   up(L.ilu.lorder, lorder);
   up(L.ilu.lptr, lptr);
   up(L.ilu.uorder, uorder);
@@ -769,7 +840,8 @@ void gmres_ilu0(Lss& L, const cf3hip_solver_params& prm, cf3hip_solve_stats& st)
   lss_point_csr(L, I.prow.as<uint32_t>(), I.pcol.as<uint32_t>(), I.aval.as<double>());
   HIP_OK(hipMemcpyAsync(I.fval.ptr, I.aval.ptr, I.aval.bytes, hipMemcpyDeviceToDevice, c.stream));
   hipLaunchKernelGGL(k_ilu0_factor, dim3(1), dim3(kWG), 0, c.stream, I.prow.as<uint32_t>(), I.pcol.as<uint32_t>(),
-                     I.fval.as<double>(), I.diag.as<uint32_t>(), I.lorder.as<uint32_t>(), I.lptr.as<uint32_t>(), I.nlev);
+                     I.fval.as<double>(), I.diag.as<uint32_t>(), I.lorder.as<uint32_t>(), I.lptr.as<uint32_t>(), I.nlev,
+                     FactorPlan{I.fptr.as<uint32_t>(), I.fpiv.as<uint32_t>(), I.fsrc.as<uint32_t>(), I.fdst.as<uint16_t>()});
   HIP_OK(hipGetLastError());
   if(I.V.bytes < sizeof(double) * (m + 1) * n) I.V.alloc(&c, sizeof(double) * (m + 1) * n);
   if(n <= kSmallMaxN && I.nsteps <= kMaxSteps) {

@@ -190,6 +190,7 @@ struct Lss {
     bool ready = false;
     uint32_t nlev = 0, nulev = 0;
     DevBuf prow, pcol, diag, lorder, lptr, uorder, uptr;
+    DevBuf fptr, fpiv, fsrc, fdst;   // update lists of the factorisation
     DevBuf aval, fval;
     DevBuf V, w, z, small, out;
     int m_ws = 0;
