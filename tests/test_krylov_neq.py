@@ -1,5 +1,6 @@
 """BiCGStab/Jacobi on systems with 1, 2 and 3 equations per node (the fused small-neq products), in FP64
-and in the precision form on stored FP32 values, against the oracle.
+and in the precision form on stored FP32 values, and GMRES(30)+ILU0 on 1 to 4 equations per node (the
+single-workgroup solver), against the oracle.
 
 Bars: the FP64 solution of the device solve matches the oracle's BiCGStab solution to 1e-8 (relative,
 inf-norm) and its residual on the oracle's matrix is <= 1e-10; iteration counts agree within 10%
