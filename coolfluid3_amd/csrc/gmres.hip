@@ -493,7 +493,11 @@ __global__ __launch_bounds__(kSW) void k_gmres_small(SmallArgs a)
 #pragma unroll
     for(int r = 0; r < kSR; ++r) {
       const uint32_t i = t + kSW * r;
-      if(i < n) a.V[i] = wr[r] / beta;
+      if(i < n) {   // V_0, and the first input of the preconditioner (no reader of zs is left)
+        const double v = wr[r] / beta;
+        a.V[i] = v;
+        zs[i] = v;
+      }
     }
     if(t == 0) {
       g[0] = beta;
@@ -502,12 +506,7 @@ __global__ __launch_bounds__(kSW) void k_gmres_small(SmallArgs a)
     int j = 0;
     for(; j < m && it < a.max_iter; ++j) {
       ++it;
-      // z = M^-1 V_j (V_j rows were written by this same thread)
-#pragma unroll
-      for(int r = 0; r < kSR; ++r) {
-        const uint32_t i = t + kSW * r;
-        if(i < n) zs[i] = a.V[(uint64_t)j * n + i];
-      }
+      // z = M^-1 V_j (zs holds V_j: each thread wrote its rows with V_j)
       __syncthreads();
       ilu_solve_lds<NEQ>(a, ml, zs);
       spmv_lds(a, zs, wr);
@@ -532,7 +531,11 @@ __global__ __launch_bounds__(kSW) void k_gmres_small(SmallArgs a)
 #pragma unroll
       for(int r = 0; r < kSR; ++r) {
         const uint32_t i = t + kSW * r;
-        if(i < n) a.V[(uint64_t)(j + 1) * n + i] = wr[r] / hn;
+        if(i < n) {   // V_{j+1}, and the next input of the preconditioner
+          const double v = wr[r] / hn;
+          a.V[(uint64_t)(j + 1) * n + i] = v;
+          zs[i] = v;
+        }
       }
       if(t == 0) {
         hc[j + 1] = hn;
