@@ -12,7 +12,8 @@
 // cavity and Taylor-Green cases) run the whole restarted solve in one launch of one workgroup
 // (k_gmres_small): the preconditioned vector lives in LDS, so the triangular solves and the product
 // read their operand from LDS, every reduction is a workgroup reduction in a fixed order, and the host
-// waits once per solve.  Larger systems take the multi-launch loop (device-wide multi-dots).
+// waits once per solve; its triangular solves walk node levels (the point rows of a node together).
+// Larger systems take the multi-launch loop (device-wide multi-dots) and the point-level solves.
 #include <algorithm>
 #include <cmath>
 #include <vector>
