@@ -141,44 +141,7 @@ __global__ __launch_bounds__(kWG) void k_ilu0_factor(const uint32_t* __restrict_
   }
 }
 
-// sum of v over the kGL lanes of a row group, the same in every lane (fixed butterfly order)
-__device__ inline double group_sum(double v)
-{
-  for(int off = 1; off < kGL; off <<= 1) v += __shfl_xor(v, off, kGL);
-  return v;
-}
 
-// x = U^-1 L^-1 b, a row group per row of a level
-__global__ __launch_bounds__(kWG) void k_ilu0_solve(const uint32_t* __restrict__ rp, const uint32_t* __restrict__ col,
-                                                   const double* __restrict__ val, const uint32_t* __restrict__ diag,
-                                                   const uint32_t* __restrict__ lorder, const uint32_t* __restrict__ lptr,
-                                                   uint32_t nlev, const uint32_t* __restrict__ uorder,
-                                                   const uint32_t* __restrict__ uptr, uint32_t nulev,
-                                                   const double* __restrict__ b, double* __restrict__ x)
-{
-  const uint32_t g = threadIdx.x / kGL, lane = threadIdx.x % kGL;
-  for(uint32_t l = 0; l < nlev; ++l) {
-    for(uint32_t t = lptr[l] + g; t < lptr[l + 1]; t += kGroups) {
-      const uint32_t i = lorder[t];
-      double s = 0.;
-      for(uint32_t q = rp[i] + lane; q < diag[i]; q += kGL) s += val[q] * x[col[q]];
-      s = group_sum(s);
-      if(lane == 0) x[i] = b[i] - s;
-    }
-    __syncthreads();
-  }
-  for(uint32_t l = 0; l < nulev; ++l) {
-    for(uint32_t t = uptr[l] + g; t < uptr[l + 1]; t += kGroups) {
-      const uint32_t i = uorder[t];
-      const uint32_t d = diag[i];
-      double s = 0.;
-      for(uint32_t q = d + 1 + lane; q < rp[i + 1]; q += kGL) s += val[q] * x[col[q]];
-      s = group_sum(s);
-      if(lane == 0) x[i] = (x[i] - s) / val[d];
-    }
-    __syncthreads();
-  }
-}
 
 // ---- the whole solve in one workgroup (systems of at most kSmallMaxN dofs) ----
 constexpr int kSW = 512;                        // threads of the single workgroup (256 VGPRs per lane)
@@ -389,26 +352,50 @@ __device__ inline void slot_rows(const SmallArgs& g, const uint4 M, bool upper, 
   }
 }
 
-// zs := U^-1 L^-1 zs in place (the factors of A), one barrier per node level.  ml: the step records in
-// LDS, followed by kAhead zero records.  The rows of the next kAhead steps are in flight in a ring of
-// registers.
-template<int NEQ>
-__device__ inline void ilu_solve_lds(const SmallArgs& g, const uint4* ml, double* zs)
+// zs := U^-1 L^-1 zs in place (the factors of A), one barrier per node level.  hdr(st): step record
+// st (zero records follow the last step, kAhead of them).  zs: the LDS copy of the vector in the
+// single-workgroup solver, the vector itself in global memory otherwise.  The rows of the next kAhead
+// steps are in flight in a ring of registers.
+template<int NEQ, typename Hdr>
+__device__ inline void ilu_sweeps(const SmallArgs& g, Hdr hdr, double* zs)
 {
   const uint32_t k0 = threadIdx.x >> 2, a = threadIdx.x & 3;
   SlotRow<NEQ> P[kAhead];
 #pragma unroll
-  for(int i = 0; i < kAhead; ++i) slot_load<NEQ>(g, ml[i], k0, a, P[i]);
+  for(int i = 0; i < kAhead; ++i) slot_load<NEQ>(g, hdr(i), k0, a, P[i]);
   for(uint32_t st = 0; st < g.nsteps; st += kAhead) {
 #pragma unroll
     for(int i = 0; i < kAhead; ++i)
       if(st + i < g.nsteps) {
-        const uint4 M = ml[st + i], Mf = ml[st + i + kAhead];
+        const uint4 M = hdr(st + i), Mf = hdr(st + i + kAhead);
         slot_rows<NEQ>(g, M, st + i >= g.nlow, k0, a, P[i], zs);
         slot_load<NEQ>(g, Mf, k0, a, P[i]);
         __syncthreads();
       }
   }
+}
+
+// ml: the step records copied to LDS
+template<int NEQ>
+__device__ inline void ilu_solve_lds(const SmallArgs& g, const uint4* ml, double* zs)
+{
+  ilu_sweeps<NEQ>(g, [ml](uint32_t st) { return ml[st]; }, zs);
+}
+
+// x = U^-1 L^-1 b in one workgroup, the vector in global memory (the multi-launch solver).  The step
+// records are read through the constant address space: the solver never writes them, so every lane
+// gets them by scalar loads.
+typedef const __attribute__((address_space(4))) uint32_t* ConstWords;
+template<int NEQ>
+__global__ __launch_bounds__(kSW) void k_ilu0_sweep(SmallArgs g, const double* __restrict__ b, double* x)
+{
+  for(uint32_t i = threadIdx.x; i < g.n; i += kSW) x[i] = b[i];
+  __syncthreads();
+  const uint4* m = g.smeta;
+  ilu_sweeps<NEQ>(g, [m](uint32_t st) {
+    const ConstWords p = (ConstWords)(uintptr_t)m + 4ull * st;
+    return make_uint4(p[0], p[1], p[2], p[3]);
+  }, x);
 }
 
 // wr = A zs for the thread's rows: 64-row ELL slices (per slice (base, width); entry q of row i at
@@ -821,7 +808,7 @@ void build_ilu_plan(Lss& L)
   HIP_OK(hipStreamSynchronize(c.stream));   // the host vectors are released on return
   L.ilu.nlev = lmax + 1;
   L.ilu.nulev = umax + 1;
-  if(n <= kSmallMaxN) build_slot_plan(L, rp, cl, prow);
+  build_slot_plan(L, rp, cl, prow);
   L.ilu.ready = true;
 }
 
@@ -875,10 +862,24 @@ void gmres_ilu0(Lss& L, const cf3hip_solver_params& prm, cf3hip_solve_stats& st)
     st.converged = o[1] <= prm.tol ? 1 : 0;
     return;
   }
+  // node-level sweeps of the factors, as in the single-workgroup solver, on vectors in global memory
+  hipLaunchKernelGGL(k_gather2, dim3(grid_for(I.nsv, 256)), dim3(256), 0, c.stream, I.nsv, I.ssrc.as<uint32_t>(),
+                     I.fval.as<double>(), I.sval.as<double>(), (uint64_t)0, I.esrc.as<uint32_t>(), I.aval.as<double>(),
+                     I.eval.as<double>());
+  SmallArgs sw{};
+  sw.n = (uint32_t)n;
+  sw.nsteps = I.nsteps;
+  sw.nlow = I.nlow;
+  sw.smeta = I.smeta.as<uint4>();
+  sw.sint = I.sint.as<uint32_t>();
+  sw.sval = I.sval.as<double>();
   auto precond = [&](const double* b, double* x) {
-    hipLaunchKernelGGL(k_ilu0_solve, dim3(1), dim3(kWG), 0, c.stream, I.prow.as<uint32_t>(), I.pcol.as<uint32_t>(),
-                       I.fval.as<double>(), I.diag.as<uint32_t>(), I.lorder.as<uint32_t>(), I.lptr.as<uint32_t>(),
-                       I.nlev, I.uorder.as<uint32_t>(), I.uptr.as<uint32_t>(), I.nulev, b, x);
+    switch(L.neq) {
+      case 1: hipLaunchKernelGGL(k_ilu0_sweep<1>, dim3(1), dim3(kSW), 0, c.stream, sw, b, x); break;
+      case 2: hipLaunchKernelGGL(k_ilu0_sweep<2>, dim3(1), dim3(kSW), 0, c.stream, sw, b, x); break;
+      case 3: hipLaunchKernelGGL(k_ilu0_sweep<3>, dim3(1), dim3(kSW), 0, c.stream, sw, b, x); break;
+      default: hipLaunchKernelGGL(k_ilu0_sweep<4>, dim3(1), dim3(kSW), 0, c.stream, sw, b, x); break;
+    }
   };
 
   // workspace: V (m+1) n, w, z, u ; small: bookkeeping, h1, h2, y, scalars

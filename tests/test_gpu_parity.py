@@ -342,7 +342,7 @@ def test_file_mesh_step_vs_oracle(ctx, oracle, name):
 def test_cylinder_steps_vs_oracle(ctx, oracle):
     """The reference's cylinder run (atest-ufem-navier-stokes-cylinder2d.py, 33538 triangles, generic
     expression): two implicit steps on the device against the oracle's steps (the same assembly,
-    conditions and updates; GMRES(30)+ILU0 to 1e-12 on both sides: 841 and 1748 iterations in the
+    conditions and updates; GMRES(30)+ILU0 to 1e-12 on both sides: 841 and 1796 iterations in the
     oracle)."""
     case = CASES["cyl"]()
     print("case cylinder", case.n_elems, "cells", flush=True)
