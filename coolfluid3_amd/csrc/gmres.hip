@@ -151,11 +151,11 @@ constexpr int kSWaves = kSW / 64;
 using arnoldi::kMaxV;                           // basis vectors (restart length <= 32)
 
 struct SmallArgs {
-  uint32_t n, nlev, nulev;
+  uint32_t n;
   int m, max_iter;
   double tol;
-  const uint32_t *prow, *pcol, *diag, *lorder, *lptr, *uorder, *uptr;
-  const double *aval, *fval, *b;
+  const uint32_t* prow;    // point row offsets (row lengths of the ELL product)
+  const double* b;
   double *x, *V;
   double* out;   // [0] iterations, [1] relative true residual at exit
   uint32_t nsteps, nlow;   // node-level sweep steps (the first nlow are the lower sweep)
@@ -699,8 +699,8 @@ void build_slot_plan(Lss& L, const std::vector<uint32_t>& rp, const std::vector<
   I.nlow = nlow;
 }
 
-// level schedules of the point pattern (natural order), on the host: a row's lower level is one more
-// than the largest of its lower columns', its upper level likewise over its upper columns from the end
+// the factorisation's level schedule of the point pattern (natural order), on the host: a row's level is
+// one more than the largest of its lower columns'; its update lists; and the solvers' sweep tables
 void build_ilu_plan(Lss& L)
 {
   Context& c = *L.ctx;
@@ -712,7 +712,7 @@ void build_ilu_plan(Lss& L)
   HIP_OK(hipStreamSynchronize(c.stream));
   const uint64_t n = (uint64_t)N * neq;
   // point row r = i neq + a holds, for each block column s of row i, columns cl[s] neq + b (ascending)
-  std::vector<uint32_t> prow(n + 1), diag(n), lev(n), ulev(n);
+  std::vector<uint32_t> prow(n + 1), diag(n), lev(n);
   const uint64_t nn = (uint64_t)neq * neq;
   for(uint32_t i = 0; i < N; ++i) {
     const uint32_t deg = rp[i + 1] - rp[i];
@@ -731,20 +731,13 @@ void build_ilu_plan(Lss& L)
     (void)deg;
     return cl[rp[i] + k / neq] * neq + k % neq;
   };
-  uint32_t lmax = 0, umax = 0;
+  uint32_t lmax = 0;
   for(uint64_t r = 0; r < n; ++r) {
     const uint32_t i = (uint32_t)(r / neq);
     uint32_t l = 0;
     for(uint32_t q = prow[r]; q < diag[r]; ++q) l = std::max(l, lev[col_of(i, q - prow[r])] + 1);
     lev[r] = l;
     lmax = std::max(lmax, l);
-  }
-  for(uint64_t r = n; r-- > 0;) {
-    const uint32_t i = (uint32_t)(r / neq);
-    uint32_t l = 0;
-    for(uint32_t q = diag[r] + 1; q < prow[r + 1]; ++q) l = std::max(l, ulev[col_of(i, q - prow[r])] + 1);
-    ulev[r] = l;
-    umax = std::max(umax, l);
   }
   auto order = [&](const std::vector<uint32_t>& lv, uint32_t nl, std::vector<uint32_t>& ord, std::vector<uint32_t>& ptr) {
     ptr.assign(nl + 1, 0);
@@ -754,9 +747,8 @@ void build_ilu_plan(Lss& L)
     ord.resize(n);
     for(uint64_t r = 0; r < n; ++r) ord[fill[lv[r]]++] = (uint32_t)r;
   };
-  std::vector<uint32_t> lorder, lptr, uorder, uptr;
+  std::vector<uint32_t> lorder, lptr;
   order(lev, lmax + 1, lorder, lptr);
-  order(ulev, umax + 1, uorder, uptr);
   // update lists of the elimination (at k_ilu0_factor)
   const uint64_t nnz = prow[n];
   std::vector<uint32_t> fptr(nnz + 1, 0), fpiv(nnz, 0), fsrc;
@@ -798,8 +790,6 @@ void build_ilu_plan(Lss& L)
   HIP_OK(hipMemcpyAsync(L.ilu.fdst.ptr, fdst.data(), sizeof(uint16_t) * fdst.size(), hipMemcpyHostToDevice, c.stream));
   up(L.ilu.lorder, lorder);
   up(L.ilu.lptr, lptr);
-  up(L.ilu.uorder, uorder);
-  up(L.ilu.uptr, uptr);
   L.ilu.prow.alloc(&c, sizeof(uint32_t) * (n + 1));
   L.ilu.pcol.alloc(&c, sizeof(uint32_t) * prow[n]);
   L.ilu.aval.alloc(&c, sizeof(double) * prow[n]);
@@ -807,7 +797,6 @@ void build_ilu_plan(Lss& L)
   L.ilu.out.alloc(&c, sizeof(double) * 2);
   HIP_OK(hipStreamSynchronize(c.stream));   // the host vectors are released on return
   L.ilu.nlev = lmax + 1;
-  L.ilu.nulev = umax + 1;
   build_slot_plan(L, rp, cl, prow);
   L.ilu.ready = true;
 }
@@ -836,10 +825,8 @@ void gmres_ilu0(Lss& L, const cf3hip_solver_params& prm, cf3hip_solve_stats& st)
   HIP_OK(hipGetLastError());
   if(I.V.bytes < sizeof(double) * (m + 1) * n) I.V.alloc(&c, sizeof(double) * (m + 1) * n);
   if(n <= kSmallMaxN && I.nsteps <= kMaxSteps) {
-    SmallArgs a{(uint32_t)n, I.nlev, I.nulev, m, prm.max_iter, prm.tol,
-                I.prow.as<uint32_t>(), I.pcol.as<uint32_t>(), I.diag.as<uint32_t>(), I.lorder.as<uint32_t>(),
-                I.lptr.as<uint32_t>(), I.uorder.as<uint32_t>(), I.uptr.as<uint32_t>(), I.aval.as<double>(),
-                I.fval.as<double>(), L.rhs.as<double>(), L.sol.as<double>(), I.V.as<double>(), I.out.as<double>(),
+    SmallArgs a{(uint32_t)n, m, prm.max_iter, prm.tol, I.prow.as<uint32_t>(), L.rhs.as<double>(), L.sol.as<double>(),
+                I.V.as<double>(), I.out.as<double>(),
                 I.nsteps, I.nlow, I.smeta.as<uint4>(), I.sint.as<uint32_t>(), I.sval.as<double>(),
                 I.emeta.as<uint2>(), I.ecol.as<uint32_t>(), I.eval.as<double>()};
     hipLaunchKernelGGL(k_gather2, dim3(grid_for(I.nsv + I.nev, 256)), dim3(256), 0, c.stream, I.nsv,

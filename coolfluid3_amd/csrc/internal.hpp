@@ -188,8 +188,8 @@ struct Lss {
   // per-solve point values, factors and Krylov workspace (kept between solves)
   struct {
     bool ready = false;
-    uint32_t nlev = 0, nulev = 0;
-    DevBuf prow, pcol, diag, lorder, lptr, uorder, uptr;
+    uint32_t nlev = 0;
+    DevBuf prow, pcol, diag, lorder, lptr;   // point pattern and the factorisation's level schedule
     DevBuf fptr, fpiv, fsrc, fdst;   // update lists of the factorisation
     DevBuf aval, fval;
     DevBuf V, w, z, small, out;
