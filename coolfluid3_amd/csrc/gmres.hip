@@ -662,8 +662,8 @@ void build_slot_plan(Lss& L, const std::vector<uint32_t>& rp, const std::vector<
     }
   }
   meta.insert(meta.end(), 4 * kAhead, 0u);   // zero records read ahead past the last step
-  // 64-row ELL slices of the point matrix
-  const uint32_t nsl = (uint32_t)((n + 63) / 64);
+  // 64-row ELL slices of the point matrix (the product of the single-workgroup solver only)
+  const uint32_t nsl = n <= kSmallMaxN ? (uint32_t)((n + 63) / 64) : 0;
   std::vector<uint32_t> emeta, ecol, esrc;
   for(uint32_t sl = 0; sl < nsl; ++sl) {
     uint32_t W = 0;
