@@ -179,6 +179,11 @@ __device__ inline double wg_sum(double v, double* red)
   return s;
 }
 
+// Rows of a thread past n hold zeros in its row registers (wr): their loads read row n - 1 (a valid
+// address) and add 0 * v, so no load sits behind a branch and its wait (p + 0 is p, up to the sign of
+// a zero p).
+__device__ inline uint32_t row_at(uint32_t i, uint32_t n) { return i < n ? i : n - 1; }
+
 // c[k] = V_k . w for k < nv, w in registers (rows t + kSW r), into LDS c[] for every thread
 __device__ inline void wg_mdot(int nv, uint32_t n, const double* V, const double wr[kSR], double* red, double* c)
 {
@@ -186,10 +191,7 @@ __device__ inline void wg_mdot(int nv, uint32_t n, const double* V, const double
   for(int k = 0; k < nv; ++k) {
     double p = 0.;
 #pragma unroll
-    for(int r = 0; r < kSR; ++r) {
-      const uint32_t i = t + kSW * r;
-      if(i < n) p += V[(uint64_t)k * n + i] * wr[r];
-    }
+    for(int r = 0; r < kSR; ++r) p += V[(uint64_t)k * n + row_at(t + kSW * r, n)] * wr[r];
     for(int off = 32; off > 0; off >>= 1) p += __shfl_down(p, off, 64);
     if((t & 63) == 0) red[(t >> 6) * kMaxV + k] = p;
   }
@@ -408,24 +410,26 @@ __device__ inline void spmv_lds(const SmallArgs& g, const double* zs, double wr[
   const uint32_t lane = threadIdx.x & 63;
 #pragma unroll
   for(int r0 = 0; r0 < kSR; r0 += kSpmvRows) {
-    uint32_t base[kSpmvRows], len[kSpmvRows], wmax = 0;
+    uint32_t base[kSpmvRows], len[kSpmvRows], wid[kSpmvRows], wmax = 0;
 #pragma unroll
-    for(int r = 0; r < kSpmvRows; ++r) {
+    for(int r = 0; r < kSpmvRows; ++r) {   // rows past n: the slice of row n - 1, length 0
       const uint32_t i = threadIdx.x + kSW * (r0 + r);
-      const uint2 E = i < g.n ? g.emeta[i >> 6] : make_uint2(0, 0);
+      const uint32_t ic = row_at(i, g.n);
+      const uint2 E = g.emeta[ic >> 6];
       base[r] = E.x + lane;
-      len[r] = i < g.n ? g.prow[i + 1] - g.prow[i] : 0;
+      wid[r] = E.y;
+      len[r] = i < g.n ? g.prow[ic + 1] - g.prow[ic] : 0;
       wmax = wmax > E.y ? wmax : E.y;
       wr[r0 + r] = 0.;
     }
 #pragma unroll 4
     for(uint32_t q = 0; q < wmax; ++q) {
 #pragma unroll
-      for(int r = 0; r < kSpmvRows; ++r)
-        if(q < len[r]) {
-          const uint32_t e = base[r] + q * 64;
-          wr[r0 + r] += g.eval[e] * zs[g.ecol[e]];
-        }
+      for(int r = 0; r < kSpmvRows; ++r) {   // past the row's length: a valid entry of its slice, times 0
+        const uint32_t e = base[r] + (q < wid[r] ? q : wid[r] - 1) * 64;
+        const double v = g.eval[e];
+        wr[r0 + r] += (q < len[r] ? v : 0.) * zs[g.ecol[e]];
+      }
     }
   }
 }
@@ -501,15 +505,13 @@ __global__ __launch_bounds__(kSW) void k_gmres_small(SmallArgs a)
       // two passes of classical Gram-Schmidt against V_0 .. V_j
       for(int pass = 0; pass < 2; ++pass) {
         wg_mdot(j + 1, n, a.V, wr, red, cc);
+        for(int k = 0; k <= j; ++k) {   // every row: w -= c_k V_k in the order of k
+          const double ck = cc[k];
 #pragma unroll
-        for(int r = 0; r < kSR; ++r) {
-          const uint32_t i = t + kSW * r;
-          if(i < n) {
-            double s = wr[r];
-            for(int k = 0; k <= j; ++k) s -= cc[k] * a.V[(uint64_t)k * n + i];
-            wr[r] = s;
-          }
+          for(int r = 0; r < kSR; ++r) wr[r] -= ck * a.V[(uint64_t)k * n + row_at(t + kSW * r, n)];
         }
+#pragma unroll
+        for(int r = 0; r < kSR; ++r) wr[r] = t + kSW * r < n ? wr[r] : 0.;
         if((int)t <= j) hc[t] = pass == 0 ? cc[t] : hc[t] + cc[t];
       }
       p = 0.;
@@ -553,14 +555,18 @@ __global__ __launch_bounds__(kSW) void k_gmres_small(SmallArgs a)
         y[i] = s / H[i * m + i];
       }
     __syncthreads();
+    double sy[kSR];
+#pragma unroll
+    for(int r = 0; r < kSR; ++r) sy[r] = 0.;
+    for(int k = 0; k < j; ++k) {
+      const double yk = y[k];
+#pragma unroll
+      for(int r = 0; r < kSR; ++r) sy[r] += yk * a.V[(uint64_t)k * n + row_at(t + kSW * r, n)];
+    }
 #pragma unroll
     for(int r = 0; r < kSR; ++r) {
       const uint32_t i = t + kSW * r;
-      if(i < n) {
-        double s = 0.;
-        for(int k = 0; k < j; ++k) s += y[k] * a.V[(uint64_t)k * n + i];
-        zs[i] = s;
-      }
+      if(i < n) zs[i] = sy[r];
     }
     __syncthreads();
     ilu_solve_lds<NEQ>(a, ml, zs);
