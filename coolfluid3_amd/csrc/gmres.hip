@@ -90,12 +90,54 @@ __global__ __launch_bounds__(kWG) void k_ilu0_factor(const uint32_t* __restrict_
         double* pv = spv[g];
         uint16_t* pd = spd[g];
         uint32_t* pp = rc;   // pair offsets of each lower entry (nlow + 1 <= kSeg)
-        for(uint32_t p = lane; p < len; p += kGL) rv[p] = val[q0 + p];
-        for(uint32_t p = lane; p < nlow; p += kGL) pv[p] = val[fp.fpiv[q0 + p]];
-        for(uint32_t p = lane; p <= nlow; p += kGL) pp[p] = fp.fptr[q0 + p] - pb;
-        for(uint32_t p = lane; p < np; p += kGL) {
-          pv[nlow + p] = val[fp.fsrc[pb + p]];
-          pd[p] = fp.fdst[pb + p];
+        // every load of the row at once: indices past a count read its last entry (valid), and only
+        // the entries within the counts are stored
+        constexpr int kRowIt = (int)(kSeg + kGL - 1) / kGL, kPairIt = (int)kPairs / kGL;
+        double ra[kRowIt], pa[kRowIt], qa[kPairIt];
+        uint32_t oa[kRowIt + 1], sa[kPairIt];
+        uint16_t da[kPairIt];
+        const uint32_t lowm = nlow ? nlow - 1 : 0;
+#pragma unroll
+        for(int m = 0; m < kRowIt; ++m) {
+          const uint32_t p = lane + m * kGL;
+          ra[m] = val[q0 + (p < len ? p : len - 1)];
+          pa[m] = val[fp.fpiv[q0 + (p < lowm ? p : lowm)]];
+        }
+#pragma unroll
+        for(int m = 0; m <= kRowIt; ++m) {
+          const uint32_t p = lane + m * kGL;
+          oa[m] = fp.fptr[q0 + (p < nlow ? p : nlow)];
+        }
+        if(np) {
+#pragma unroll
+          for(int m = 0; m < kPairIt; ++m) {
+            const uint32_t p = lane + m * kGL, pc = pb + (p < np ? p : np - 1);
+            sa[m] = fp.fsrc[pc];
+            da[m] = fp.fdst[pc];
+          }
+#pragma unroll
+          for(int m = 0; m < kPairIt; ++m) qa[m] = val[sa[m]];
+        }
+#pragma unroll
+        for(int m = 0; m < kRowIt; ++m) {
+          const uint32_t p = lane + m * kGL;
+          if(p < len) rv[p] = ra[m];
+          if(p < nlow) pv[p] = pa[m];
+        }
+#pragma unroll
+        for(int m = 0; m <= kRowIt; ++m) {
+          const uint32_t p = lane + m * kGL;
+          if(p <= nlow) pp[p] = oa[m] - pb;
+        }
+        if(np) {
+#pragma unroll
+          for(int m = 0; m < kPairIt; ++m) {
+            const uint32_t p = lane + m * kGL;
+            if(p < np) {
+              pv[nlow + p] = qa[m];
+              pd[p] = da[m];
+            }
+          }
         }
         group_sync();
         for(uint32_t qq = 0; qq < nlow; ++qq) {
